@@ -1,0 +1,167 @@
+"""ctypes binding of libgpgmp_hip.so (the C ABI declared in include/gpgmp_hip.h).
+
+The library must be built (``python -m inchman_amd.build`` or
+``__graft_entry__.build()``); loading fails loudly otherwise -- there is no
+fallback implementation.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpgmp_hip.so")
+
+MAX_SPECIES = 8
+MAX_REACTIONS = 16
+
+OK = 0
+E_INVALID = -1
+E_HIP = -2
+E_CAPACITY = -3
+E_STATE = -4
+E_UNSUPPORTED = -5
+
+STOCHASTIC_HOMOGENEOUS = 0
+STOCHASTIC_INHOMOGENEOUS = 1
+DETERMINISTIC_HOMOGENEOUS_RK4 = 2
+DETERMINISTIC_INHOMOGENEOUS_RK4 = 3
+
+
+class GpgmpError(RuntimeError):
+    """A negative status from the C ABI (the reference raised gpgmp::Error)."""
+
+    def __init__(self, code, msg):
+        super().__init__("gpgmp error %d: %s" % (code, msg))
+        self.code = code
+
+
+class ReactionDesc(C.Structure):
+    _fields_ = [
+        ("rate", C.c_float),
+        ("reactant_stoich", C.c_int32 * MAX_SPECIES),
+        ("product_stoich", C.c_int32 * MAX_SPECIES),
+    ]
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32),
+        ("physical_length", C.c_float), ("p0", C.c_float),
+        ("num_species", C.c_int32),
+        ("diffusion", C.c_float * MAX_SPECIES),
+        ("num_reactions", C.c_int32),
+        ("reactions", C.POINTER(ReactionDesc)),
+        ("boundary_mask", C.c_int32 * 4),
+        ("source_mask", C.c_int32 * 4),
+        ("max_time", C.c_float),
+        ("solver_type", C.c_int32),
+        ("diffusivity_x", C.POINTER(C.c_float)), ("diffusivity_y", C.POINTER(C.c_float)),
+        ("drift_x", C.POINTER(C.c_float)), ("drift_y", C.POINTER(C.c_float)),
+    ]
+
+
+class RddParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32),
+        ("h", C.c_float), ("dt", C.c_float),
+        ("D0u", C.c_float), ("D0v", C.c_float),
+        ("k", C.c_float * 4),
+    ]
+
+
+class ParticleParams(C.Structure):
+    _fields_ = [
+        ("side", C.c_int32),
+        ("D", C.c_float), ("dt", C.c_float), ("vortex_amp", C.c_float), ("p_annihilate", C.c_float),
+        ("seed", C.c_uint32),
+    ]
+
+
+_P = C.c_void_p
+_I = C.c_int32
+_U = C.c_uint32
+_F = C.c_float
+_pI = C.POINTER(C.c_int32)
+_pU = C.POINTER(C.c_uint32)
+_pF = C.POINTER(C.c_float)
+_pD = C.POINTER(C.c_double)
+
+# name -> argtypes (all return int status except gpgmp_last_error)
+SIGNATURES = {
+    "gpgmp_version": [_pI, _pI],
+    "gpgmp_model_create": [C.POINTER(ModelDesc), C.POINTER(_P)],
+    "gpgmp_model_destroy": [_P],
+    "gpgmp_model_rate_literals": [_P, _pI, _pD],
+    "gpgmp_solver_create": [_P, _P, _U, C.POINTER(_P)],
+    "gpgmp_solver_destroy": [_P],
+    "gpgmp_solver_set_state": [_P, _pI],
+    "gpgmp_solver_get_state": [_P, _pI],
+    "gpgmp_solver_set_state_f32": [_P, _pF],
+    "gpgmp_solver_get_state_f32": [_P, _pF],
+    "gpgmp_solver_state_device": [_P, C.POINTER(_P)],
+    "gpgmp_solver_do_reactions": [_P, _F, _pI],
+    "gpgmp_solver_diffuse": [_P, _I, _F, _pI],
+    "gpgmp_solver_next_diffusion_time": [_P, _pF],
+    "gpgmp_solver_time_steps": [_P, _pF, _pF],
+    "gpgmp_solver_run": [_P, _I, _U, _pF],
+    "gpgmp_solver_run_continue": [_P, _I, _I, _pF],
+    "gpgmp_solver_set_launch": [_P, _I, _I],
+    "gpgmp_solver_synchronize": [_P],
+    "gpgmp_batch_create": [_P, _I, _pI, _pI, _pU, _P, C.POINTER(_P)],
+    "gpgmp_batch_destroy": [_P],
+    "gpgmp_batch_set_state": [_P, _I, _pI],
+    "gpgmp_batch_get_state": [_P, _I, _pI],
+    "gpgmp_batch_set_launch": [_P, _I, _I],
+    "gpgmp_batch_autotune": [_P, _I, _pI, _pI],
+    "gpgmp_batch_run": [_P, _I, _I],
+    "gpgmp_batch_reset": [_P],
+    "gpgmp_batch_total_cells": [_P, C.POINTER(C.c_int64)],
+    "gpgmp_batch_synchronize": [_P],
+    "gpgmp_min_timestep_device": [_P, _P, _I, _F, _pF, _P],
+    "gpgmp_rdd_create": [C.POINTER(RddParams), _I, _I, _P, C.POINTER(_P)],
+    "gpgmp_rdd_destroy": [_P],
+    "gpgmp_rdd_forward": [_P, _P, _P, _P, _I],
+    "gpgmp_rdd_backward": [_P, _P, _P, _P, _P, _P],
+    "gpgmp_det_hom_diffuse_device": [_P, _I, _I, _I, _pF, _F, _F, _P],
+    "gpgmp_det_inhom_sweep_device": [_P, _I, _I, _I, _I, _F, _P, _P, _F, _P],
+    "gpgmp_sort_pairs_device": [_P, _P, _P, _P, _I, _I, _P],
+    "gpgmp_cell_offsets_device": [_P, _I, _I, _P, _P],
+    "gpgmp_cell_left_right_device": [_P, _I, _I, _P, _P, _P],
+    "gpgmp_particles_create": [C.POINTER(ParticleParams), _I, _P, C.POINTER(_P)],
+    "gpgmp_particles_destroy": [_P],
+    "gpgmp_particles_set": [_P, _pF, _pF, _pU, _I],
+    "gpgmp_particles_get": [_P, _pF, _pF, _pU, _pU, _pU, _pI],
+    "gpgmp_particles_count": [_P, _pI],
+    "gpgmp_particles_step": [_P, _I, _I],
+}
+
+_lib = None
+
+
+def lib():
+    """Loads the library once; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libgpgmp_hip.so is not built (%s); run `python -m inchman_amd.build`" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.gpgmp_last_error.argtypes = []
+    L.gpgmp_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def check(code):
+    if code != OK:
+        msg = lib().gpgmp_last_error()
+        raise GpgmpError(code, msg.decode() if msg else "")
+    return code
+
+
+def call(name, *args):
+    return check(getattr(lib(), name)(*args))

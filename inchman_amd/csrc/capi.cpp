@@ -1,0 +1,462 @@
+// C ABI of the GPGMP HIP library: status codes at the boundary, C++ inside.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host.h"
+
+using namespace gpgmp;
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        return GPGMP_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return GPGMP_E_CAPACITY;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return GPGMP_E_INVALID;
+    }
+}
+
+struct StreamHolder {
+    hipStream_t s = nullptr;
+    bool own = false;
+    void init(void *user) {
+        if (user) {
+            s = (hipStream_t)user;
+            own = false;
+        } else {
+            GPGMP_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            own = true;
+        }
+    }
+    ~StreamHolder() {
+        if (own && s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+};
+}  // namespace
+
+struct gpgmp_model {
+    Model m;
+};
+
+// Plain solver: one realization driven through the gpgmp::Solver-shaped calls, or
+// through whole host loops (run).
+struct gpgmp_solver {
+    Model m;
+    StreamHolder stream;
+    std::unique_ptr<GmpEngine> eng;
+    std::vector<float> dtx, dty, nx, ny;
+    DevBuf<LaunchPlan> plan_d;
+    DevBuf<int32_t> seed_d;
+    std::unique_ptr<GmpRunner> runner;
+    uint32_t runner_seed0 = 0;
+};
+
+struct gpgmp_batch {
+    Model m;
+    StreamHolder stream;
+    std::unique_ptr<GmpEngine> eng;
+    std::vector<float> dtx, dty;
+    std::vector<uint32_t> seed0s;
+    std::unique_ptr<GmpRunner> runner;
+};
+
+extern "C" {
+
+const char *gpgmp_last_error(void) { return g_last_error.c_str(); }
+
+int gpgmp_version(int32_t *major, int32_t *minor) {
+    if (major) *major = 1;
+    if (minor) *minor = 0;
+    return GPGMP_OK;
+}
+
+int gpgmp_model_create(const gpgmp_model_desc *desc, gpgmp_model **out) {
+    return guarded([&] {
+        GPGMP_REQUIRE(desc && out, GPGMP_E_INVALID, "null argument");
+        *out = nullptr;
+        std::unique_ptr<gpgmp_model> h(new gpgmp_model{compile_model(*desc)});
+        *out = h.release();
+    });
+}
+
+int gpgmp_model_destroy(gpgmp_model *m) {
+    delete m;
+    return GPGMP_OK;
+}
+
+int gpgmp_model_rate_literals(const gpgmp_model *m, int32_t *is_int, double *value) {
+    return guarded([&] {
+        GPGMP_REQUIRE(m && is_int && value, GPGMP_E_INVALID, "null argument");
+        for (int i = 0; i < m->m.table.R; ++i) {
+            is_int[i] = (m->m.table.lit_is_int >> i) & 1u;
+            value[i] = m->m.table.lit_dbl[i];
+        }
+    });
+}
+
+// ---- plain solver -------------------------------------------------------------------
+int gpgmp_solver_create(const gpgmp_model *model, void *stream, uint32_t flags, gpgmp_solver **out) {
+    (void)flags;
+    return guarded([&] {
+        GPGMP_REQUIRE(model && out, GPGMP_E_INVALID, "null argument");
+        *out = nullptr;
+        GPGMP_REQUIRE(!model->m.deterministic(), GPGMP_E_UNSUPPORTED,
+                      "deterministic solver types are served by the gpgmp_det_* and gpgmp_rdd_* calls");
+        std::unique_ptr<gpgmp_solver> s(new gpgmp_solver());
+        s->m = model->m;
+        s->m.desc.reactions = s->m.reactions.empty() ? nullptr : s->m.reactions.data();
+        s->stream.init(stream);
+        const gpgmp_model_desc &d = s->m.desc;
+        const int S = d.num_species;
+        s->eng.reset(new GmpEngine(s->m, {d.width}, {d.height}, s->stream.s));
+        s->dtx.assign(S, 0.f);
+        s->dty.assign(S, 0.f);
+        if (!s->m.inhomogeneous()) {
+            // HomogeneousSolver.cpp:87-95 (h = physicalLength / gridWidth in float)
+            const float h = d.physical_length / (float)d.width;
+            for (int i = 0; i < S; ++i)
+                s->dtx[i] = (d.diffusion[i] > 0) ? homogeneous_dt(d.p0, h, d.diffusion[i]) : d.max_time;
+            s->dty = s->dtx;
+        } else {
+            // InhomogeneousSolver.cpp:189-212: per species and axis, the minimum admissible
+            // step over the cells (both axes use PhysicalCellWidth, as the reference's
+            // computeIndividualTimestepY does)
+            const int A = d.width * d.height;
+            for (int i = 0; i < S; ++i) {
+                float mx = INFINITY, my = INFINITY;
+                for (int c = 0; c < A; ++c) {
+                    mx = std::min(mx, inhomogeneous_cell_dt(s->m.Dx[(size_t)i * A + c], s->m.rx[(size_t)i * A + c],
+                                                            s->m.cell_width));
+                    my = std::min(my, inhomogeneous_cell_dt(s->m.Dy[(size_t)i * A + c], s->m.ry[(size_t)i * A + c],
+                                                            s->m.cell_width));
+                }
+                s->dtx[i] = mx;
+                s->dty[i] = my;
+                s->eng->set_probs(0, i, 0, s->m.Dx.data() + (size_t)i * A, s->m.rx.data() + (size_t)i * A, mx,
+                                  s->m.cell_width);
+                s->eng->set_probs(0, i, 1, s->m.Dy.data() + (size_t)i * A, s->m.ry.data() + (size_t)i * A, my,
+                                  s->m.cell_height);
+            }
+        }
+        s->nx = s->dtx;
+        s->ny = s->dty;
+        s->plan_d.alloc(4);
+        s->seed_d.alloc(4);
+        *out = s.release();
+    });
+}
+
+int gpgmp_solver_destroy(gpgmp_solver *s) {
+    if (s) {
+        s->runner.reset();
+        if (s->stream.s) (void)hipStreamSynchronize(s->stream.s);
+    }
+    delete s;
+    return GPGMP_OK;
+}
+
+int gpgmp_solver_set_state(gpgmp_solver *s, const int32_t *host) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && host, GPGMP_E_INVALID, "null argument");
+        s->eng->set_state(0, host);
+    });
+}
+
+int gpgmp_solver_get_state(gpgmp_solver *s, int32_t *host) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && host, GPGMP_E_INVALID, "null argument");
+        s->eng->get_state(0, host);
+    });
+}
+
+int gpgmp_solver_set_state_f32(gpgmp_solver *, const float *) {
+    g_last_error = "stochastic solvers hold int32 counts";
+    return GPGMP_E_UNSUPPORTED;
+}
+
+int gpgmp_solver_get_state_f32(gpgmp_solver *, float *) {
+    g_last_error = "stochastic solvers hold int32 counts";
+    return GPGMP_E_UNSUPPORTED;
+}
+
+int gpgmp_solver_state_device(gpgmp_solver *s, void **dev_ptr) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && dev_ptr, GPGMP_E_INVALID, "null argument");
+        *dev_ptr = s->eng->state();
+    });
+}
+
+namespace {
+// one API-level launch with a single plan and the given seeds (current, previous)
+void api_launch(gpgmp_solver *s, int mode, const LaunchPlan &p, int32_t seed_cur, int32_t seed_prev,
+                const int32_t *lin, int32_t *lout) {
+    const int32_t seeds[2] = {seed_cur, seed_prev};
+    GPGMP_HIP_CHECK(hipMemcpyAsync(s->plan_d.get(), &p, sizeof p, hipMemcpyHostToDevice, s->stream.s));
+    GPGMP_HIP_CHECK(hipMemcpyAsync(s->seed_d.get(), seeds, sizeof seeds, hipMemcpyHostToDevice, s->stream.s));
+    s->eng->launch(mode, s->plan_d.get(), s->seed_d.get(), s->seed_d.get() + 1, lin, lout);
+    // the host staging above is stack memory: keep the call synchronous like the
+    // reference's finish() after every launch (Kernel.h:252-253)
+    GPGMP_HIP_CHECK(hipStreamSynchronize(s->stream.s));
+}
+}  // namespace
+
+int gpgmp_solver_do_reactions(gpgmp_solver *s, float dt, int32_t *seed) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && seed, GPGMP_E_INVALID, "null argument");
+        if (s->m.table.R > 0) {
+            LaunchPlan p{};
+            p.deltat = dt;
+            api_launch(s, s->m.inhomogeneous() ? 1 : 0, p, *seed, *seed, s->eng->lost(1), s->eng->lost(0));
+        }
+        *seed = (int32_t)((uint32_t)*seed + 1u);  // Solver.cpp:239
+    });
+}
+
+int gpgmp_solver_diffuse(gpgmp_solver *s, int32_t species, float t, int32_t *seed) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && seed, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(species >= 0 && species < s->m.desc.num_species, GPGMP_E_INVALID, "species out of range");
+        const uint32_t bit = 1u << species;
+        if (!s->m.inhomogeneous()) {
+            // homogeneous_stochastic_diffuse + updateState (+ sources) with the same seed
+            LaunchPlan p{};
+            p.out_mask = bit;
+            api_launch(s, 0, p, *seed, *seed, s->eng->lost(1), s->eng->lost(0));
+            LaunchPlan g{};
+            g.gather_mask = bit;
+            api_launch(s, 0, g, *seed, *seed, s->eng->lost(0), s->eng->lost(1));
+            s->nx[species] += s->dtx[species];
+            s->ny[species] = s->nx[species];
+            return;
+        }
+        if (s->nx[species] <= t) {
+            LaunchPlan p{};
+            p.out_mask = bit;
+            api_launch(s, 1, p, *seed, *seed, s->eng->lost(1), s->eng->lost(0));
+            LaunchPlan g{};
+            g.gather_mask = bit;
+            api_launch(s, 2, g, *seed, *seed, s->eng->lost(0), s->eng->lost(1));
+            *seed = (int32_t)((uint32_t)*seed + 1u);
+            s->nx[species] += s->dtx[species];
+        }
+        if (s->ny[species] <= t) {
+            LaunchPlan p{};
+            p.out_mask = bit;
+            api_launch(s, 2, p, *seed, *seed, s->eng->lost(0), s->eng->lost(1));
+            LaunchPlan g{};
+            g.gather_mask = bit;
+            api_launch(s, 1, g, *seed, *seed, s->eng->lost(1), s->eng->lost(0));
+            *seed = (int32_t)((uint32_t)*seed + 1u);
+            s->ny[species] += s->dty[species];
+        }
+    });
+}
+
+int gpgmp_solver_next_diffusion_time(gpgmp_solver *s, float *out) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && out, GPGMP_E_INVALID, "null argument");
+        for (int i = 0; i < s->m.desc.num_species; ++i)
+            out[i] = (!s->m.inhomogeneous()) ? s->nx[i] : ((s->nx[i] < s->ny[i]) ? s->nx[i] : s->ny[i]);
+    });
+}
+
+int gpgmp_solver_time_steps(gpgmp_solver *s, float *dt_x, float *dt_y) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && dt_x && dt_y, GPGMP_E_INVALID, "null argument");
+        for (int i = 0; i < s->m.desc.num_species; ++i) {
+            dt_x[i] = s->dtx[i];
+            dt_y[i] = s->dty[i];
+        }
+    });
+}
+
+int gpgmp_solver_run(gpgmp_solver *s, int32_t steps, uint32_t seed0, float *sim_time_out) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        s->runner.reset();
+        s->runner.reset(new GmpRunner(*s->eng, s->dtx, s->dty, {seed0}));
+        s->runner_seed0 = seed0;
+        s->runner->run(steps, 0);
+        if (sim_time_out) *sim_time_out = s->runner->sim_time();
+    });
+}
+
+int gpgmp_solver_run_continue(gpgmp_solver *s, int32_t steps, int32_t graph_chunk, float *sim_time_out) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(s->runner != nullptr, GPGMP_E_STATE, "no run to continue");
+        s->runner->run(steps, graph_chunk);
+        if (sim_time_out) *sim_time_out = s->runner->sim_time();
+    });
+}
+
+int gpgmp_solver_set_launch(gpgmp_solver *s, int32_t block, int32_t cpt) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        s->eng->set_launch(block, cpt);
+    });
+}
+
+int gpgmp_solver_synchronize(gpgmp_solver *s) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s->stream.s));
+    });
+}
+
+// ---- batch ------------------------------------------------------------------------------
+int gpgmp_batch_create(const gpgmp_model *tmpl, int32_t n, const int32_t *widths, const int32_t *heights,
+                       const uint32_t *seed0s, void *stream, gpgmp_batch **out) {
+    return guarded([&] {
+        GPGMP_REQUIRE(tmpl && widths && heights && seed0s && out, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(n >= 1, GPGMP_E_INVALID, "n must be positive");
+        *out = nullptr;
+        GPGMP_REQUIRE(tmpl->m.desc.solver_type == GPGMP_STOCHASTIC_HOMOGENEOUS, GPGMP_E_UNSUPPORTED,
+                      "batches run the stochastic homogeneous solver");
+        std::unique_ptr<gpgmp_batch> b(new gpgmp_batch());
+        b->m = tmpl->m;
+        b->m.desc.reactions = b->m.reactions.empty() ? nullptr : b->m.reactions.data();
+        b->stream.init(stream);
+        std::vector<int> W(widths, widths + n), H(heights, heights + n);
+        int64_t total = 0;
+        for (int r = 0; r < n; ++r) total += (int64_t)W[r] * H[r];
+        GPGMP_REQUIRE(total * b->m.desc.num_species <= ((int64_t)1 << 31), GPGMP_E_CAPACITY,
+                      "batch exceeds 2^31 counts");
+        b->eng.reset(new GmpEngine(b->m, W, H, b->stream.s));
+        const gpgmp_model_desc &d = b->m.desc;
+        const float h = d.physical_length / (float)d.width;
+        const int S = d.num_species;
+        b->dtx.assign(S, 0.f);
+        for (int i = 0; i < S; ++i)
+            b->dtx[i] = (d.diffusion[i] > 0) ? homogeneous_dt(d.p0, h, d.diffusion[i]) : d.max_time;
+        b->dty = b->dtx;
+        b->seed0s.assign(seed0s, seed0s + n);
+        *out = b.release();
+    });
+}
+
+int gpgmp_batch_destroy(gpgmp_batch *b) {
+    if (b) b->runner.reset();
+    delete b;
+    return GPGMP_OK;
+}
+
+int gpgmp_batch_set_state(gpgmp_batch *b, int32_t r, const int32_t *host) {
+    return guarded([&] {
+        GPGMP_REQUIRE(b && host, GPGMP_E_INVALID, "null argument");
+        b->eng->set_state(r, host);
+    });
+}
+
+int gpgmp_batch_get_state(gpgmp_batch *b, int32_t r, int32_t *host) {
+    return guarded([&] {
+        GPGMP_REQUIRE(b && host, GPGMP_E_INVALID, "null argument");
+        b->eng->get_state(r, host);
+    });
+}
+
+int gpgmp_batch_set_launch(gpgmp_batch *b, int32_t block, int32_t cpt) {
+    return guarded([&] {
+        GPGMP_REQUIRE(b, GPGMP_E_INVALID, "null argument");
+        b->eng->set_launch(block, cpt);
+    });
+}
+
+int gpgmp_batch_run(gpgmp_batch *b, int32_t steps, int32_t graph_chunk) {
+    return guarded([&] {
+        GPGMP_REQUIRE(b, GPGMP_E_INVALID, "null argument");
+        if (!b->runner) b->runner.reset(new GmpRunner(*b->eng, b->dtx, b->dty, b->seed0s));
+        b->runner->run(steps, graph_chunk);
+    });
+}
+
+int gpgmp_batch_reset(gpgmp_batch *b) {
+    return guarded([&] {
+        GPGMP_REQUIRE(b, GPGMP_E_INVALID, "null argument");
+        b->runner.reset();
+    });
+}
+
+// Project-written autotuner: every candidate (block, cells per thread) runs the same
+// trial steps from the same saved state with a fresh schedule; the fastest is kept and
+// the state and schedule are restored.  Results never depend on the shape (RNG keys
+// carry no tile or batch offset), only the speed does.
+int gpgmp_batch_autotune(gpgmp_batch *b, int32_t trial_steps, int32_t *block_out, int32_t *cpt_out) {
+    return guarded([&] {
+        GPGMP_REQUIRE(b && trial_steps > 0, GPGMP_E_INVALID, "bad argument");
+        GmpEngine &e = *b->eng;
+        hipStream_t st = b->stream.s;
+        const size_t n = e.state_elems();
+        DevBuf<int32_t> saved(n);
+        GPGMP_HIP_CHECK(hipMemcpyAsync(saved.get(), e.state(), n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+        GPGMP_HIP_CHECK(hipStreamSynchronize(st));
+        static const int blocks[] = {64, 128, 256};
+        static const int cpts[] = {1, 2, 4};
+        double best = 1e300;
+        int bb = e.block(), bc = e.cpt();
+        hipEvent_t t0, t1;
+        GPGMP_HIP_CHECK(hipEventCreate(&t0));
+        GPGMP_HIP_CHECK(hipEventCreate(&t1));
+        for (int blk : blocks)
+            for (int c : cpts) {
+                e.set_launch(blk, c);
+                GPGMP_HIP_CHECK(hipMemcpyAsync(e.state(), saved.get(), n * sizeof(int32_t),
+                                               hipMemcpyDeviceToDevice, st));
+                GmpRunner trial(e, b->dtx, b->dty, b->seed0s);
+                GPGMP_HIP_CHECK(hipEventRecord(t0, st));
+                trial.run(trial_steps, 0);
+                GPGMP_HIP_CHECK(hipEventRecord(t1, st));
+                GPGMP_HIP_CHECK(hipEventSynchronize(t1));
+                float ms = 0.f;
+                GPGMP_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+                if (ms < best) {
+                    best = ms;
+                    bb = blk;
+                    bc = c;
+                }
+            }
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        e.set_launch(bb, bc);
+        GPGMP_HIP_CHECK(hipMemcpyAsync(e.state(), saved.get(), n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+        GPGMP_HIP_CHECK(hipStreamSynchronize(st));
+        if (block_out) *block_out = bb;
+        if (cpt_out) *cpt_out = bc;
+    });
+}
+
+int gpgmp_batch_total_cells(gpgmp_batch *b, int64_t *cells) {
+    return guarded([&] {
+        GPGMP_REQUIRE(b && cells, GPGMP_E_INVALID, "null argument");
+        *cells = b->eng->total_cells();
+    });
+}
+
+int gpgmp_batch_synchronize(gpgmp_batch *b) {
+    return guarded([&] {
+        GPGMP_REQUIRE(b, GPGMP_E_INVALID, "null argument");
+        GPGMP_HIP_CHECK(hipStreamSynchronize(b->stream.s));
+    });
+}
+
+}  // extern "C"

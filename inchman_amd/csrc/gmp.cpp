@@ -1,0 +1,514 @@
+// Model compile, host-loop schedule, device engine and runner of the stochastic solvers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "host.h"
+
+namespace gpgmp {
+
+// Header constants are printed "%.6e" (ss << std::scientific) and re-read as float
+// literals by the OpenCL compiler (Solver.cpp:494-505).
+float header_float(float v) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.6e", (double)v);
+    return std::strtof(buf, nullptr);
+}
+
+// p0/((Real)2.*GMP_DIMENSIONALITY)*h*h/D evaluated in float (HomogeneousSolver.cpp:89).
+float homogeneous_dt(float p0, float h, float D) {
+    const float six = (float)2. * 3;
+    float a = p0 / six;
+    a = a * h;
+    a = a * h;
+    return a / D;
+}
+
+// Admissible per-cell step of the stochastic inhomogeneous solver
+// (StochasticInhomogeneousDriftDiffusion.cl:130-151), in double on the host.
+float inhomogeneous_cell_dt(float Df, float rf, float hf) {
+    const double D = Df, r = rf, h = hf;
+    double ts;
+    if (r == 0.0) {
+        ts = h * h / (2.0 * D) * (1.0 - 2.0 / 3.0);
+    } else if (D == 0.0) {
+        ts = h / std::fabs(r);
+    } else {
+        const double eps = h * r / (2.0 * D);
+        double temp = 1.0 + (-1.0 / (std::tanh(eps) * eps) + 1.0 / (std::sinh(eps) * std::sinh(eps)));
+        if (temp <= 0.0) temp = 1.0 / 3.0;
+        ts = h / r * std::tanh(eps) * temp;
+    }
+    return (float)ts;
+}
+
+Model compile_model(const gpgmp_model_desc &d) {
+    Model m;
+    m.desc = d;
+    const int S = d.num_species, R = d.num_reactions;
+    GPGMP_REQUIRE(S >= 1 && S <= kMaxSpecies, GPGMP_E_INVALID, "num_species must be in [1, 8]");
+    GPGMP_REQUIRE(R >= 0 && R <= kMaxReactions, GPGMP_E_INVALID, "num_reactions must be in [0, 16]");
+    GPGMP_REQUIRE(R == 0 || d.reactions != nullptr, GPGMP_E_INVALID, "reactions is NULL");
+    GPGMP_REQUIRE(d.width >= 16 && d.height >= 16 && d.width % 16 == 0 && d.height % 16 == 0,
+                  GPGMP_E_INVALID, "grid sides must be positive multiples of 16");
+    GPGMP_REQUIRE((int64_t)d.width * d.height <= (int64_t)1 << 28, GPGMP_E_CAPACITY, "grid too large");
+    GPGMP_REQUIRE(d.physical_length > 0.f && std::isfinite(d.physical_length), GPGMP_E_INVALID,
+                  "physical_length must be positive");
+    GPGMP_REQUIRE(d.solver_type >= 0 && d.solver_type <= 3, GPGMP_E_INVALID, "unknown solver type");
+    for (int b = 0; b < 4; ++b)
+        GPGMP_REQUIRE(d.boundary_mask[b] == 0 || d.boundary_mask[b] == 1, GPGMP_E_INVALID,
+                      "boundary masks must be 0 or 1");
+    const int A = d.width * d.height;
+    if (R > 0) m.reactions.assign(d.reactions, d.reactions + R);
+    m.desc.reactions = m.reactions.empty() ? nullptr : m.reactions.data();
+
+    RxTable &T = m.table;
+    std::memset(&T, 0, sizeof T);
+    T.S = S;
+    T.R = R;
+    for (int i = 0; i < R; ++i) {
+        const gpgmp_reaction_desc &rd = m.reactions[i];
+        GPGMP_REQUIRE(std::isfinite(rd.rate) && rd.rate >= 0.f, GPGMP_E_INVALID,
+                      "reaction rates must be finite and non-negative");
+        // the law's rate is printed with `ss << rate` (Reaction.cpp:122): %g, 6 digits
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "%g", (double)rd.rate);
+        bool is_int = true;
+        for (const char *c = buf; *c; ++c)
+            if (!((*c >= '0' && *c <= '9') || *c == '-')) is_int = false;
+        if (is_int) {
+            T.lit_is_int |= 1u << i;
+            T.lit_int[i] = (int32_t)std::strtol(buf, nullptr, 10);
+        }
+        T.lit_dbl[i] = std::strtod(buf, nullptr);
+        int nf = 0;
+        for (int j = 0; j < S; ++j) {
+            GPGMP_REQUIRE(rd.reactant_stoich[j] >= 0 && rd.product_stoich[j] >= 0, GPGMP_E_INVALID,
+                          "stoichiometries must be non-negative");
+            for (int q = 0; q < rd.reactant_stoich[j]; ++q) {
+                GPGMP_REQUIRE(nf < kMaxFactors, GPGMP_E_INVALID, "reaction order above 6");
+                T.fs[i][nf] = (uint8_t)j;
+                T.fo[i][nf] = (uint8_t)q;
+                ++nf;
+            }
+            const int net = rd.product_stoich[j] - rd.reactant_stoich[j];
+            GPGMP_REQUIRE(net >= -127 && net <= 127, GPGMP_E_INVALID, "stoichiometry out of range");
+            T.net[i][j] = (int8_t)net;
+        }
+        T.nf[i] = (uint8_t)nf;
+    }
+    // StayProbability = (1.-p0) printed "%.6e" (Solver.cpp:505); the thresholds follow
+    // StochasticHomogeneousDiffusion.cl:58-62.
+    {
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "%.6e", 1.0 - (double)d.p0);
+        const float stay = std::strtof(buf, nullptr);
+        const float pi = (float)((1. - (double)stay) / (2. * 3));
+        m.probs.stay = stay;
+        m.probs.p1 = stay + pi;
+        m.probs.p2 = m.probs.p1 + pi;
+        m.probs.p3 = m.probs.p2 + pi;
+        m.probs.p4 = m.probs.p3 + pi;
+    }
+    m.cell_width = header_float(d.physical_length / (float)d.width);
+    m.cell_height = header_float(d.physical_length / (float)d.height);
+    if (m.inhomogeneous()) {
+        GPGMP_REQUIRE(d.diffusivity_x && d.diffusivity_y && d.drift_x && d.drift_y, GPGMP_E_INVALID,
+                      "inhomogeneous solvers need diffusivity and drift fields");
+        const size_t n = (size_t)S * A;
+        m.Dx.assign(d.diffusivity_x, d.diffusivity_x + n);
+        m.Dy.assign(d.diffusivity_y, d.diffusivity_y + n);
+        m.rx.assign(d.drift_x, d.drift_x + n);
+        m.ry.assign(d.drift_y, d.drift_y + n);
+        for (size_t i = 0; i < n; ++i)
+            GPGMP_REQUIRE(std::isfinite(m.Dx[i]) && std::isfinite(m.Dy[i]) && m.Dx[i] >= 0.f &&
+                              m.Dy[i] >= 0.f && std::isfinite(m.rx[i]) && std::isfinite(m.ry[i]),
+                          GPGMP_E_INVALID, "fields must be finite with non-negative diffusivity");
+    } else {
+        for (int s = 0; s < S; ++s)
+            GPGMP_REQUIRE(std::isfinite(d.diffusion[s]) && d.diffusion[s] >= 0.f, GPGMP_E_INVALID,
+                          "diffusion constants must be finite and non-negative");
+    }
+    m.desc.diffusivity_x = m.Dx.empty() ? nullptr : m.Dx.data();
+    m.desc.diffusivity_y = m.Dy.empty() ? nullptr : m.Dy.data();
+    m.desc.drift_x = m.rx.empty() ? nullptr : m.rx.data();
+    m.desc.drift_y = m.ry.empty() ? nullptr : m.ry.data();
+    return m;
+}
+
+// ------------------------------------------------------------------------------------
+Schedule::Schedule(bool inh, int S, bool has_reactions, const std::vector<float> &dtx,
+                   const std::vector<float> &dty)
+    : inh_(inh), S_(S), rx_(has_reactions), dtx_(dtx), dty_(dty), nx_(dtx), ny_(dty) {}
+
+std::vector<float> Schedule::next_diffusion_time() const {
+    std::vector<float> out(S_);
+    for (int s = 0; s < S_; ++s) out[s] = (!inh_) ? nx_[s] : ((nx_[s] < ny_[s]) ? nx_[s] : ny_[s]);
+    return out;
+}
+
+StepDecision Schedule::next() {
+    StepDecision d;
+    const std::vector<float> next = next_diffusion_time();
+    float mindt = next[0];
+    int minspec = 0;
+    for (int i = 1; i < S_; ++i)
+        if (next[i] < mindt) {
+            mindt = next[i];
+            minspec = i;
+        }
+    d.deltat = mindt - sim_time_;
+    int32_t off = rx_ ? 1 : 0;  // Solver::doReactions increments the seed
+    for (int i = 0; i < S_; ++i) {
+        if (!(i == minspec || next[i] <= mindt)) continue;
+        if (!(next[i] > 0)) continue;
+        if (!inh_) {
+            d.due_x |= 1u << i;
+            d.off_x[i] = off;
+            nx_[i] += dtx_[i];
+        } else {
+            if (nx_[i] <= mindt) {
+                d.due_x |= 1u << i;
+                d.off_x[i] = off++;
+                nx_[i] += dtx_[i];
+            }
+            if (ny_[i] <= mindt) {
+                d.due_y |= 1u << i;
+                d.off_y[i] = off++;
+                ny_[i] += dty_[i];
+            }
+        }
+    }
+    d.off_end = off;
+    sim_time_ = mindt;
+    return d;
+}
+
+// ------------------------------------------------------------------------------------
+GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vector<int> &H,
+                     hipStream_t stream)
+    : table_(m.table), probs_(m.probs), inh_(m.inhomogeneous()), stream_(stream) {
+    bm_ = make_int4(m.desc.boundary_mask[0], m.desc.boundary_mask[1], m.desc.boundary_mask[2],
+                    m.desc.boundary_mask[3]);
+    const int S = table_.S;
+    const int nd = inh_ ? 2 : 4;
+    uint64_t so = 0, lo = 0, po = 0;
+    for (size_t r = 0; r < W.size(); ++r) {
+        GPGMP_REQUIRE(W[r] >= 16 && H[r] >= 16 && W[r] % 16 == 0 && H[r] % 16 == 0, GPGMP_E_INVALID,
+                      "grid sides must be positive multiples of 16");
+        RealInfo ri{};
+        ri.W = W[r];
+        ri.H = H[r];
+        ri.state_off = so;
+        ri.lost_off = lo;
+        ri.prob_off = po;
+        // source counts per boundary as HomogeneousSolver::diffuse passes them
+        // (HomogeneousSolver.cpp:141-156: boundary 0 takes sourceMask[1])
+        const int32_t *sm = m.desc.source_mask;
+        if (!inh_) {
+            ri.src[0] = sm[0] > 0 ? sm[1] : 0;
+            ri.src[1] = sm[1] > 0 ? sm[1] : 0;
+            ri.src[2] = sm[2] > 0 ? sm[2] : 0;
+            ri.src[3] = sm[3] > 0 ? sm[3] : 0;
+        }
+        const uint64_t A = (uint64_t)W[r] * H[r];
+        so += A * S;
+        lo += A * S * nd;
+        po += A * S;
+        total_cells_ += (int64_t)A;
+        reals_.push_back(ri);
+    }
+    state_.alloc(so);
+    lost_[0].alloc(lo);
+    lost_[1].alloc(lo);
+    if (inh_) {
+        prob_[0].alloc(po);
+        prob_[1].alloc(po);
+    }
+    reals_dev_.alloc(reals_.size());
+    GPGMP_HIP_CHECK(hipMemcpyAsync(reals_dev_.get(), reals_.data(), reals_.size() * sizeof(RealInfo),
+                                   hipMemcpyHostToDevice, stream_));
+    // zero the outflow buffers once so that a stale read can never see garbage
+    GPGMP_HIP_CHECK(hipMemsetAsync(lost_[0].get(), 0, lo * sizeof(int32_t), stream_));
+    GPGMP_HIP_CHECK(hipMemsetAsync(lost_[1].get(), 0, lo * sizeof(int32_t), stream_));
+    GPGMP_HIP_CHECK(hipMemsetAsync(state_.get(), 0, so * sizeof(int32_t), stream_));
+    build_tiles();
+}
+
+void GmpEngine::build_tiles() {
+    std::vector<Tile> tiles;
+    const int per = block_ * cpt_;
+    for (size_t r = 0; r < reals_.size(); ++r) {
+        const int A = reals_[r].W * reals_[r].H;
+        for (int c = 0; c < A; c += per) tiles.push_back(Tile{(int32_t)r, (int32_t)c});
+    }
+    ntiles_ = (int)tiles.size();
+    tiles_dev_.alloc(tiles.size());
+    GPGMP_HIP_CHECK(hipMemcpyAsync(tiles_dev_.get(), tiles.data(), tiles.size() * sizeof(Tile),
+                                   hipMemcpyHostToDevice, stream_));
+    GPGMP_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void GmpEngine::set_launch(int block, int cpt) {
+    GPGMP_REQUIRE(block >= 64 && block <= 256 && block % 64 == 0, GPGMP_E_INVALID,
+                  "block must be 64, 128, 192 or 256");
+    GPGMP_REQUIRE(cpt >= 1 && cpt <= 16, GPGMP_E_INVALID, "cells per thread must be in [1, 16]");
+    if (block == block_ && cpt == cpt_) return;
+    GPGMP_HIP_CHECK(hipStreamSynchronize(stream_));
+    block_ = block;
+    cpt_ = cpt;
+    build_tiles();
+}
+
+void GmpEngine::set_state(int r, const int32_t *host) {
+    GPGMP_REQUIRE(r >= 0 && r < num_real(), GPGMP_E_INVALID, "realization index out of range");
+    const RealInfo &ri = reals_[r];
+    const size_t n = (size_t)ri.W * ri.H * S();
+    for (size_t i = 0; i < n; ++i) GPGMP_REQUIRE(host[i] >= 0, GPGMP_E_INVALID, "negative count");
+    GPGMP_HIP_CHECK(hipMemcpyAsync(state_.get() + ri.state_off, host, n * sizeof(int32_t),
+                                   hipMemcpyHostToDevice, stream_));
+    GPGMP_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void GmpEngine::get_state(int r, int32_t *host) {
+    GPGMP_REQUIRE(r >= 0 && r < num_real(), GPGMP_E_INVALID, "realization index out of range");
+    const RealInfo &ri = reals_[r];
+    const size_t n = (size_t)ri.W * ri.H * S();
+    GPGMP_HIP_CHECK(hipMemcpyAsync(host, state_.get() + ri.state_off, n * sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, stream_));
+    GPGMP_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void GmpEngine::set_probs(int r, int s, int axis, const float *D, const float *rr, float dt, float dx) {
+    const RealInfo &ri = reals_[r];
+    const int A = ri.W * ri.H;
+    DevBuf<float> dD(A), dr(A);
+    GPGMP_HIP_CHECK(hipMemcpyAsync(dD.get(), D, A * sizeof(float), hipMemcpyHostToDevice, stream_));
+    GPGMP_HIP_CHECK(hipMemcpyAsync(dr.get(), rr, A * sizeof(float), hipMemcpyHostToDevice, stream_));
+    GPGMP_HIP_CHECK(launch_inh_probs(prob_[axis].get() + ri.prob_off + (size_t)s * A, dD.get(), dr.get(),
+                                     dt, dx, A, stream_));
+    GPGMP_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cur,
+                       const int32_t *seeds_prev, const int32_t *lost_in, int32_t *lost_out) {
+    StepLaunch a{};
+    a.mode = mode;
+    a.state = state_.get();
+    a.lost_in = lost_in;
+    a.lost_out = lost_out;
+    a.prob = inh_ ? prob_[mode == 2 ? 1 : 0].get() : nullptr;
+    a.reals = reals_dev_.get();
+    a.tiles = tiles_dev_.get();
+    a.ntiles = ntiles_;
+    a.plan = plan;
+    a.seeds_cur = seeds_cur;
+    a.seeds_prev = seeds_prev;
+    a.T = &table_;
+    a.P = &probs_;
+    a.bm = bm_;
+    a.block = block_;
+    a.cpt = cpt_;
+    a.stream = stream_;
+    GPGMP_HIP_CHECK(launch_gmp_step(a));
+}
+
+// ------------------------------------------------------------------------------------
+struct GmpRunner::Chunk {
+    int cap = 0, nreal = 0;
+    LaunchPlan *hplan = nullptr;   // pinned [2*cap + 1]
+    int32_t *hseed = nullptr;      // pinned [(cap + 1) * nreal]
+    DevBuf<LaunchPlan> dplan;
+    DevBuf<int32_t> dseed;
+    hipEvent_t copied = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    Chunk(int cap_, int nreal_) : cap(cap_), nreal(nreal_) {
+        GPGMP_HIP_CHECK(hipHostMalloc((void **)&hplan, sizeof(LaunchPlan) * (2 * cap + 1)));
+        GPGMP_HIP_CHECK(hipHostMalloc((void **)&hseed, sizeof(int32_t) * (size_t)(cap + 1) * nreal));
+        dplan.alloc(2 * cap + 1);
+        dseed.alloc((size_t)(cap + 1) * nreal);
+        GPGMP_HIP_CHECK(hipEventCreateWithFlags(&copied, hipEventDisableTiming));
+    }
+    ~Chunk() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (copied) {
+            (void)hipEventSynchronize(copied);
+            (void)hipEventDestroy(copied);
+        }
+        if (hplan) (void)hipHostFree(hplan);
+        if (hseed) (void)hipHostFree(hseed);
+    }
+};
+
+GmpRunner::GmpRunner(GmpEngine &eng, const std::vector<float> &dtx, const std::vector<float> &dty,
+                     const std::vector<uint32_t> &seed0s)
+    : eng_(eng), sched_(eng.inh_, eng.S(), eng.table_.R > 0, dtx, dty) {
+    GPGMP_REQUIRE((int)seed0s.size() == eng.num_real(), GPGMP_E_INVALID, "one seed0 per realization");
+    for (uint32_t s0 : seed0s) seeds_.emplace_back(s0);
+    prev_seed_.assign(eng.num_real(), 0);
+}
+
+GmpRunner::~GmpRunner() {
+    (void)hipStreamSynchronize(eng_.stream());
+    for (Chunk *c : chunks_) delete c;
+}
+
+// Issues the launches of one chunk whose plans/seeds are already in its host buffers.
+void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
+    const int nreal = eng_.num_real();
+    const int lps = eng_.inh_ ? 2 : 1;  // launches per step
+    GPGMP_HIP_CHECK(hipMemcpyAsync(c.dplan.get(), c.hplan, sizeof(LaunchPlan) * lps * nsteps,
+                                   hipMemcpyHostToDevice, eng_.stream()));
+    GPGMP_HIP_CHECK(hipMemcpyAsync(c.dseed.get(), c.hseed, sizeof(int32_t) * (size_t)(nsteps + 1) * nreal,
+                                   hipMemcpyHostToDevice, eng_.stream()));
+    GPGMP_HIP_CHECK(hipEventRecord(c.copied, eng_.stream()));
+    auto body = [&]() {
+        for (int j = 0; j < nsteps; ++j) {
+            const int32_t *scur = c.dseed.get() + (size_t)(j + 1) * nreal;
+            const int32_t *sprev = c.dseed.get() + (size_t)j * nreal;
+            if (!eng_.inh_) {
+                const int p = (parity_ + j) & 1;
+                eng_.launch(0, c.dplan.get() + j, scur, sprev, eng_.lost(p ^ 1), eng_.lost(p));
+            } else {
+                eng_.launch(1, c.dplan.get() + 2 * j, scur, sprev, eng_.lost(1), eng_.lost(0));
+                eng_.launch(2, c.dplan.get() + 2 * j + 1, scur, sprev, eng_.lost(0), eng_.lost(1));
+            }
+        }
+    };
+    if (!use_graph) {
+        body();
+        return;
+    }
+    if (!c.graph || c.graph_steps != nsteps) {
+        if (c.graph) {
+            GPGMP_HIP_CHECK(hipGraphExecDestroy(c.graph));
+            c.graph = nullptr;
+        }
+        // capture on a private stream, then replay on the engine's stream
+        hipStream_t cap;
+        GPGMP_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipStream_t saved = eng_.stream();
+        struct Restore {
+            GmpEngine &e;
+            hipStream_t s;
+            ~Restore() { e.set_stream(s); }
+        } restore{eng_, saved};
+        eng_.set_stream(cap);
+        hipGraph_t g = nullptr;
+        GPGMP_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(cap, &g);
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipStreamDestroy(cap);
+            throw;
+        }
+        GPGMP_HIP_CHECK(hipStreamEndCapture(cap, &g));
+        eng_.set_stream(saved);
+        const hipError_t ie = hipGraphInstantiate(&c.graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        (void)hipStreamDestroy(cap);
+        GPGMP_HIP_CHECK(ie);
+        c.graph_steps = nsteps;
+    }
+    GPGMP_HIP_CHECK(hipGraphLaunch(c.graph, eng_.stream()));
+}
+
+void GmpRunner::run(int steps, int graph_chunk) {
+    GPGMP_REQUIRE(steps >= 0, GPGMP_E_INVALID, "steps must be non-negative");
+    const int nreal = eng_.num_real();
+    const bool use_graph = graph_chunk > 0;
+    int cap = use_graph ? graph_chunk : 256;
+    if (!eng_.inh_ && use_graph && (cap & 1)) ++cap;  // keeps the outflow-buffer parity per chunk
+    if (cap != chunk_cap_) {
+        GPGMP_HIP_CHECK(hipStreamSynchronize(eng_.stream()));
+        for (Chunk *c : chunks_) delete c;
+        chunks_.clear();
+        chunks_.push_back(new Chunk(cap, nreal));
+        chunks_.push_back(new Chunk(cap, nreal));
+        chunk_cap_ = cap;
+    }
+    parity_ = 0;  // nothing is pending at entry
+    int done = 0, ci = 0;
+    while (done < steps) {
+        const int n = std::min(cap, steps - done);
+        Chunk &c = *chunks_[ci];
+        GPGMP_HIP_CHECK(hipEventSynchronize(c.copied));  // host buffers free again
+        for (int r = 0; r < nreal; ++r) c.hseed[r] = prev_seed_[r];
+        for (int j = 0; j < n; ++j) {
+            const StepDecision d = sched_.next();
+            for (int r = 0; r < nreal; ++r) c.hseed[(size_t)(j + 1) * nreal + r] = seeds_[r].next();
+            const float dts = (eng_.table_.R > 0) ? d.deltat : 0.f;
+            if (!eng_.inh_) {
+                LaunchPlan &p = c.hplan[j];
+                std::memset(&p, 0, sizeof p);
+                p.deltat = dts;
+                p.gather_mask = has_prev_ ? prev_.due_x : 0u;
+                p.gather_off = has_prev_ ? (eng_.table_.R > 0 ? 1 : 0) : 0;
+                p.out_mask = d.due_x;
+                for (int s = 0; s < kMaxSpecies; ++s) p.out_off[s] = d.off_x[s];
+            } else {
+                LaunchPlan &px = c.hplan[2 * j];
+                LaunchPlan &py = c.hplan[2 * j + 1];
+                std::memset(&px, 0, sizeof px);
+                std::memset(&py, 0, sizeof py);
+                px.deltat = dts;
+                px.gather_mask = has_prev_ ? prev_.due_y : 0u;
+                px.out_mask = d.due_x;
+                py.deltat = 0.f;
+                py.gather_mask = d.due_x;
+                py.out_mask = d.due_y;
+                for (int s = 0; s < kMaxSpecies; ++s) {
+                    px.out_off[s] = d.off_x[s];
+                    py.out_off[s] = d.off_y[s];
+                }
+            }
+            prev_ = d;
+            has_prev_ = true;
+        }
+        for (int r = 0; r < nreal; ++r) prev_seed_[r] = c.hseed[(size_t)n * nreal + r];
+        issue(c, n, use_graph && n == cap);
+        if (!eng_.inh_) parity_ = (parity_ + n) & 1;
+        done += n;
+        ci ^= 1;
+    }
+    // final gather of the last launch's outflow (and source boundaries)
+    if (has_prev_) {
+        Chunk &c = *chunks_[ci];
+        GPGMP_HIP_CHECK(hipEventSynchronize(c.copied));
+        LaunchPlan &p = c.hplan[0];
+        std::memset(&p, 0, sizeof p);
+        p.deltat = 0.f;
+        p.out_mask = 0;
+        for (int r = 0; r < nreal; ++r) {
+            c.hseed[r] = prev_seed_[r];
+            c.hseed[nreal + r] = prev_seed_[r];
+        }
+        if (!eng_.inh_) {
+            p.gather_mask = prev_.due_x;
+            p.gather_off = eng_.table_.R > 0 ? 1 : 0;
+        } else {
+            p.gather_mask = prev_.due_y;
+        }
+        GPGMP_HIP_CHECK(hipMemcpyAsync(c.dplan.get(), c.hplan, sizeof(LaunchPlan), hipMemcpyHostToDevice,
+                                       eng_.stream()));
+        GPGMP_HIP_CHECK(hipMemcpyAsync(c.dseed.get(), c.hseed, sizeof(int32_t) * 2 * nreal,
+                                       hipMemcpyHostToDevice, eng_.stream()));
+        GPGMP_HIP_CHECK(hipEventRecord(c.copied, eng_.stream()));
+        const int32_t *scur = c.dseed.get() + nreal;
+        const int32_t *sprev = c.dseed.get();
+        if (!eng_.inh_)
+            eng_.launch(0, c.dplan.get(), scur, sprev, eng_.lost(parity_ ^ 1), eng_.lost(parity_));
+        else
+            eng_.launch(1, c.dplan.get(), scur, sprev, eng_.lost(1), eng_.lost(0));
+        has_prev_ = false;
+    }
+}
+
+}  // namespace gpgmp

@@ -1,0 +1,150 @@
+// Numeric contract of the GPGMP HIP path, shared by host and device code.
+//
+// The reference leaves several results implementation-defined (OpenCL built-in
+// log, literal promotion, contraction).  This header fixes them so that integer
+// molecule counts are bit-exact between this library and its CPU oracle:
+//   * Threefry-2x32-20 (Salmon et al., SC'11), the generator the reference calls as
+//     threefry2x32(ctr, key) (Random123/threefry.h:733, Gillespie.cl:69).
+//   * u = (float)((double)(int32)w * 2^-32 + 0.5) (Gillespie.cl:72-73 with
+//     M_RAN_INVM32 = 2^-32, Solver.cpp:469-471).  (int32)w * 2^-32 + 0.5 equals
+//     (w ^ 2^31) / 2^32 exactly, so the float result is the round-to-nearest
+//     conversion of the unsigned integer w ^ 2^31, scaled by 2^-32.
+//   * log: double-precision 2*atanh((m-1)/(m+1)) series by a fixed sequence of
+//     IEEE operations, rounded to float.
+//   * Everything is compiled with -ffp-contract=off and IEEE division.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define GPGMP_HD __host__ __device__ __forceinline__
+#else
+#include <math.h>
+#define GPGMP_HD inline
+#endif
+
+namespace gpgmp {
+
+GPGMP_HD uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+// Threefry-2x32, 20 rounds: rotations {13,15,26,6,17,29,16,24}, key parity
+// 0x1BD11BDA, key injection s after every fourth round adds (ks[s%3], ks[(s+1)%3]+s).
+struct u32x2 {
+    uint32_t x, y;
+};
+
+#define GPGMP_TF_ROUND(R) \
+    x0 += x1;             \
+    x1 = rotl32(x1, R);   \
+    x1 ^= x0;
+
+GPGMP_HD u32x2 threefry2x32_20(uint32_t c0, uint32_t c1, uint32_t k0, uint32_t k1) {
+    const uint32_t k2 = 0x1BD11BDAu ^ k0 ^ k1;
+    uint32_t x0 = c0 + k0;
+    uint32_t x1 = c1 + k1;
+    GPGMP_TF_ROUND(13) GPGMP_TF_ROUND(15) GPGMP_TF_ROUND(26) GPGMP_TF_ROUND(6)
+    x0 += k1; x1 += k2 + 1u;
+    GPGMP_TF_ROUND(17) GPGMP_TF_ROUND(29) GPGMP_TF_ROUND(16) GPGMP_TF_ROUND(24)
+    x0 += k2; x1 += k0 + 2u;
+    GPGMP_TF_ROUND(13) GPGMP_TF_ROUND(15) GPGMP_TF_ROUND(26) GPGMP_TF_ROUND(6)
+    x0 += k0; x1 += k1 + 3u;
+    GPGMP_TF_ROUND(17) GPGMP_TF_ROUND(29) GPGMP_TF_ROUND(16) GPGMP_TF_ROUND(24)
+    x0 += k1; x1 += k2 + 4u;
+    GPGMP_TF_ROUND(13) GPGMP_TF_ROUND(15) GPGMP_TF_ROUND(26) GPGMP_TF_ROUND(6)
+    x0 += k2; x1 += k0 + 5u;
+    return u32x2{x0, x1};
+}
+#undef GPGMP_TF_ROUND
+
+// Reference counter/key convention: key {k0, 0xdecafbad + seed}, counter {n, 0xf00dcafe}.
+constexpr uint32_t kKeyBase = 0xdecafbadu;
+constexpr uint32_t kCtrHi = 0xf00dcafeu;
+
+// uniform in [0,1]: the float nearest (w ^ 2^31) * 2^-32
+GPGMP_HD float u01(uint32_t w) { return (float)(w ^ 0x80000000u) * 2.3283064365386963e-10f; }
+
+// (double)(int32)w * 2^-32 + 0.5, exact in double
+GPGMP_HD double u01_d(uint32_t w) { return (double)(w ^ 0x80000000u) * 2.3283064365386962890625e-10; }
+
+// natural log in double: fixed operation sequence (see file header)
+GPGMP_HD double log_d(double y) {
+    if (!(y > 0.0)) return (y == 0.0) ? -__builtin_inf() : __builtin_nan("");
+    if (y == __builtin_inf()) return y;
+    // y = m * 2^e, m in [1,2): exact bit manipulation of a normal double
+    uint64_t b;
+    __builtin_memcpy(&b, &y, 8);
+    int e = (int)((b >> 52) & 0x7ff);
+    if (e == 0) {  // subnormal: scale into the normal range first
+        y = y * 18014398509481984.0;  // 2^54
+        __builtin_memcpy(&b, &y, 8);
+        e = (int)((b >> 52) & 0x7ff) - 54;
+    }
+    e -= 1023;
+    b = (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+    double m;
+    __builtin_memcpy(&m, &b, 8);
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e = e + 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double p = 1.0 / 15.0;
+    p = 1.0 / 13.0 + z * p;
+    p = 1.0 / 11.0 + z * p;
+    p = 1.0 / 9.0 + z * p;
+    p = 1.0 / 7.0 + z * p;
+    p = 1.0 / 5.0 + z * p;
+    p = 1.0 / 3.0 + z * p;
+    p = z * p;
+    const double two_s = 2.0 * s;
+    return (double)e * 0.6931471805599453094 + (two_s + two_s * p);
+}
+
+GPGMP_HD float log_f(float y) { return (float)log_d((double)y); }
+
+// sin(pi t) in double: t reduced to [0,2), folded to [0,1/2], Taylor series to x^21
+GPGMP_HD double sinpi_d(double t) {
+    double t2 = t - 2.0 * __builtin_floor(t * 0.5);
+    double sign = 1.0;
+    if (t2 >= 1.0) {
+        sign = -1.0;
+        t2 = t2 - 1.0;
+    }
+    const double r = (t2 > 0.5) ? (1.0 - t2) : t2;
+    const double x = 3.141592653589793116 * r;
+    const double z = x * x;
+    double p = 1.0 / 51090942171709440000.0;
+    p = -1.0 / 121645100408832000.0 + z * p;
+    p = 1.0 / 355687428096000.0 + z * p;
+    p = -1.0 / 1307674368000.0 + z * p;
+    p = 1.0 / 6227020800.0 + z * p;
+    p = -1.0 / 39916800.0 + z * p;
+    p = 1.0 / 362880.0 + z * p;
+    p = -1.0 / 5040.0 + z * p;
+    p = 1.0 / 120.0 + z * p;
+    p = -1.0 / 6.0 + z * p;
+    return sign * (x + x * (z * p));
+}
+
+// Gaussian pair by the Marsaglia polar method; key {id, seed}, counter {step, attempt}
+GPGMP_HD void gauss2(uint32_t id, uint32_t seed, uint32_t step, double *gx, double *gy) {
+    for (uint32_t j = 0; j < 64u; ++j) {
+        const u32x2 w = threefry2x32_20(step, j, id, seed);
+        const double a = 2.0 * (((double)w.x + 0.5) * 2.3283064365386962890625e-10) - 1.0;
+        const double b = 2.0 * (((double)w.y + 0.5) * 2.3283064365386962890625e-10) - 1.0;
+        const double s = a * a + b * b;
+        if (s < 1.0 && s > 0.0) {
+            const double f = __builtin_sqrt(-2.0 * log_d(s) / s);
+            *gx = a * f;
+            *gy = b * f;
+            return;
+        }
+    }
+    *gx = 0.0;
+    *gy = 0.0;
+}
+
+}  // namespace gpgmp

@@ -1,0 +1,218 @@
+// Host-side C++ of the GPGMP HIP library: model compile, schedule, solver classes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gpgmp_hip.h"
+#include "stochastic_launch.h"
+#include "stochastic_types.h"
+
+namespace gpgmp {
+
+// Error carrying a C-ABI status code (the reference's gpgmp::Error, DiffusionModel.h:68-105,
+// becomes a return code at the boundary).
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define GPGMP_HIP_CHECK(expr)                                                                  \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::gpgmp::Error(GPGMP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define GPGMP_REQUIRE(cond, code, msg)                  \
+    do {                                                \
+        if (!(cond)) throw ::gpgmp::Error((code), (msg)); \
+    } while (0)
+
+// Owning device allocation.
+template <typename T>
+class DevBuf {
+  public:
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) return;
+        GPGMP_HIP_CHECK(hipMalloc((void **)&p_, n * sizeof(T)));
+        n_ = n;
+    }
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T *get() const { return p_; }
+    size_t size() const { return n_; }
+
+  private:
+    T *p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Compiled model: the reaction table and the constants the reference bakes into the
+// kernel header (Solver.cpp:449-567).
+struct Model {
+    gpgmp_model_desc desc{};
+    std::vector<gpgmp_reaction_desc> reactions;
+    std::vector<float> Dx, Dy, rx, ry;  // inhomogeneous fields [S][H][W]
+    RxTable table{};
+    HomProbs probs{};
+    float cell_width = 0.f;   // PhysicalCellWidth as printed into the header
+    float cell_height = 0.f;  // PhysicalCellHeight
+    bool inhomogeneous() const {
+        return desc.solver_type == GPGMP_STOCHASTIC_INHOMOGENEOUS ||
+               desc.solver_type == GPGMP_DETERMINISTIC_INHOMOGENEOUS_RK4;
+    }
+    bool deterministic() const {
+        return desc.solver_type == GPGMP_DETERMINISTIC_HOMOGENEOUS_RK4 ||
+               desc.solver_type == GPGMP_DETERMINISTIC_INHOMOGENEOUS_RK4;
+    }
+};
+
+Model compile_model(const gpgmp_model_desc &d);
+float header_float(float v);
+float homogeneous_dt(float p0, float h, float D);
+float inhomogeneous_cell_dt(float D, float r, float h);
+
+// boost::mt19937(seed0) + uniform_int<>(0, INT_MAX), seed_n = draw_n + n
+// (DiffusionModel.cpp:1467-1484); the draw is eng() >> 1 for this range.
+class SeedStream {
+  public:
+    explicit SeedStream(uint32_t seed0) : eng_(seed0) {}
+    int32_t next() {
+        const uint32_t draw = (uint32_t)eng_() >> 1;
+        return (int32_t)(draw + (uint32_t)n_++);
+    }
+
+  private:
+    std::mt19937 eng_;
+    int32_t n_ = 0;
+};
+
+// One host-loop iteration as decided by DiffusionModel::oclSolve
+// (DiffusionModel.cpp:1479-1603) for a stochastic solver.  Seeds are offsets from the
+// iteration's seed_n: the SSA uses seed_n and increments it (Solver.cpp:239);
+// homogeneous diffusion reuses the incremented seed for every species
+// (HomogeneousSolver.cpp:136); each inhomogeneous sweep takes the current seed and
+// increments it (InhomogeneousSolver.cpp:278,370).
+struct StepDecision {
+    float deltat = 0.f;
+    uint32_t due_x = 0, due_y = 0;  // homogeneous: due_x holds the diffused species
+    int32_t off_x[kMaxSpecies] = {};
+    int32_t off_y[kMaxSpecies] = {};
+    int32_t off_end = 0;            // seed offset after the iteration
+};
+
+// The multi-rate schedule: which species diffuse when.  It depends only on the time
+// steps, so it is computed on the host ahead of the launches.
+class Schedule {
+  public:
+    Schedule(bool inhomogeneous, int S, bool has_reactions, const std::vector<float> &dtx,
+             const std::vector<float> &dty);
+    StepDecision next();
+    float sim_time() const { return sim_time_; }
+    std::vector<float> next_diffusion_time() const;
+
+  private:
+    bool inh_;
+    int S_;
+    bool rx_;
+    std::vector<float> dtx_, dty_, nx_, ny_;
+    float sim_time_ = 0.f;
+};
+
+// Device engine of the stochastic solvers: a set of realizations of one network.
+class GmpEngine {
+  public:
+    GmpEngine(const Model &m, const std::vector<int> &W, const std::vector<int> &H, hipStream_t stream);
+
+    int num_real() const { return (int)reals_.size(); }
+    int S() const { return table_.S; }
+    const RealInfo &real(int r) const { return reals_[r]; }
+    int64_t total_cells() const { return total_cells_; }
+    int32_t *state() const { return state_.get(); }
+    size_t state_elems() const { return state_.size(); }
+    hipStream_t stream() const { return stream_; }
+    void set_stream(hipStream_t s) { stream_ = s; }
+
+    void set_launch(int block, int cpt);
+    int block() const { return block_; }
+    int cpt() const { return cpt_; }
+
+    void set_state(int r, const int32_t *host);
+    void get_state(int r, int32_t *host);
+
+    // inhomogeneous probability table of realization r, species s, axis; D, r host [A]
+    void set_probs(int r, int s, int axis, const float *D, const float *rr, float dt, float dx);
+
+    // one launch (all device pointers)
+    void launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cur, const int32_t *seeds_prev,
+                const int32_t *lost_in, int32_t *lost_out);
+
+    int32_t *lost(int i) const { return lost_[i].get(); }
+
+    RxTable table_{};
+    HomProbs probs_{};
+    int4 bm_{};
+    bool inh_ = false;
+
+  private:
+    void build_tiles();
+    std::vector<RealInfo> reals_;
+    DevBuf<int32_t> state_;
+    DevBuf<int32_t> lost_[2];
+    DevBuf<float2> prob_[2];
+    DevBuf<RealInfo> reals_dev_;
+    DevBuf<Tile> tiles_dev_;
+    int ntiles_ = 0;
+    int block_ = 256, cpt_ = 1;
+    int64_t total_cells_ = 0;
+    hipStream_t stream_;
+};
+
+// Runs host-loop iterations of every realization of an engine (one shared schedule,
+// one seed stream per realization): uploads chunks of launch plans and seeds, issues
+// the launches (optionally as a replayed HIP graph per chunk) and the final gather.
+class GmpRunner {
+  public:
+    GmpRunner(GmpEngine &eng, const std::vector<float> &dtx, const std::vector<float> &dty,
+              const std::vector<uint32_t> &seed0s);
+    ~GmpRunner();
+    void run(int steps, int graph_chunk);
+    float sim_time() const { return sched_.sim_time(); }
+    std::vector<float> next_diffusion_time() const { return sched_.next_diffusion_time(); }
+
+  private:
+    struct Chunk;
+    void issue(Chunk &c, int nlaunch_steps, bool use_graph);
+    GmpEngine &eng_;
+    Schedule sched_;
+    std::vector<SeedStream> seeds_;
+    StepDecision prev_{};
+    bool has_prev_ = false;
+    std::vector<int32_t> prev_seed_;
+    int parity_ = 0;
+    std::vector<Chunk *> chunks_;
+    int chunk_cap_ = 0;
+    int graph_steps_ = 0;
+};
+
+}  // namespace gpgmp

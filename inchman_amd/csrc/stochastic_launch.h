@@ -1,0 +1,34 @@
+// Host-side launch interface of the stochastic GMP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "stochastic_types.h"
+
+namespace gpgmp {
+
+struct StepLaunch {
+    int mode;               // 0 homogeneous, 1 inhomogeneous X sweeps (+SSA), 2 inhomogeneous Y sweeps
+    int32_t *state;
+    const int32_t *lost_in;
+    int32_t *lost_out;
+    const float2 *prob;     // inhomogeneous probability table of this launch's axis
+    const RealInfo *reals;
+    const Tile *tiles;
+    int ntiles;
+    const LaunchPlan *plan; // device pointer, one plan for all realizations
+    const int32_t *seeds_cur;   // [num realizations] seed_n of this step
+    const int32_t *seeds_prev;  // [num realizations] seed of the previous step (source boundaries)
+    const RxTable *T;
+    const HomProbs *P;
+    int4 bm;                // boundary masks -x, +x, -y, +y
+    int block;              // threads per block (64..256, multiple of 64)
+    int cpt;                // cells per thread
+    hipStream_t stream;
+};
+
+hipError_t launch_gmp_step(const StepLaunch &a);
+hipError_t launch_inh_probs(float2 *out, const float *D, const float *r, float dt, float dx, int n,
+                            hipStream_t stream);
+
+}  // namespace gpgmp

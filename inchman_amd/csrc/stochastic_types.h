@@ -1,0 +1,61 @@
+// Shared host/device types of the stochastic GMP kernels.
+#pragma once
+
+#include <stdint.h>
+
+namespace gpgmp {
+
+constexpr int kMaxSpecies = 8;
+constexpr int kMaxReactions = 16;
+constexpr int kMaxFactors = 6;
+// Upper bound on SSA events per cell per reaction step: a runaway model (hazards so
+// large that t += dt stops advancing in float) ends instead of spinning forever.
+constexpr uint32_t kMaxEventsPerCall = 1u << 24;
+
+// Reaction table compiled from the model (the reference generates the same
+// arithmetic as kernel source, Solver.cpp:843-905 and Reaction.cpp:119-138).
+// Passed to kernels by value so every field is wave-uniform (scalar loads).
+struct RxTable {
+    int32_t S, R;
+    uint32_t lit_is_int;               // bit i: rate literal of reaction i is an int literal
+    int32_t lit_int[kMaxReactions];
+    double lit_dbl[kMaxReactions];
+    uint8_t nf[kMaxReactions];         // number of falling-factorial factors
+    uint8_t fs[kMaxReactions][kMaxFactors];  // factor species
+    uint8_t fo[kMaxReactions][kMaxFactors];  // factor offset m of (X - m)
+    int8_t net[kMaxReactions][kMaxSpecies];  // product - reactant stoichiometry
+};
+
+// One realization (grid) of a launch.  The plain solver is a batch of one.
+struct RealInfo {
+    uint64_t state_off;   // int32 elements: state[state_off + s*A + cell]
+    uint64_t lost_off;    // int32 elements into each lost buffer: + (s*ndir + d)*A + cell
+    uint64_t prob_off;    // float2 elements into the inhomogeneous probability tables
+    int32_t W, H;
+    int32_t src[4];       // source-boundary counts per boundary (0 = none)
+};
+
+// Work of one launch, shared by all realizations of the launch (they follow one
+// schedule).  Seeds are offsets from the realization's host-loop seed of the step
+// (seed_n of DiffusionModel.cpp:1482): the SSA uses seed_n itself, the diffusion
+// sweeps seed_n + offset.  deltat <= 0 disables the SSA.
+struct alignas(16) LaunchPlan {
+    float deltat;
+    uint32_t gather_mask;   // species whose pending outflow of the previous launch is gathered
+    uint32_t out_mask;      // species whose outflow is computed in this launch
+    int32_t gather_off;     // source-boundary seed offset, relative to the previous step's seed
+    int32_t out_off[kMaxSpecies];
+};
+
+// A tile: cells [cell0, cell0 + cells) of realization r.
+struct Tile {
+    int32_t r;
+    int32_t cell0;
+};
+
+// Homogeneous diffusion thresholds (StochasticHomogeneousDiffusion.cl:58-62)
+struct HomProbs {
+    float stay, p1, p2, p3, p4;
+};
+
+}  // namespace gpgmp

@@ -1,0 +1,331 @@
+"""Python mirror of the GPGMP solver interface over the C ABI.
+
+Class and method names follow the reference's C++ solver plugin
+(exec/src/gpgmp/src/Solver.h:15-239 and its subclasses) so that code driving a
+reference solver reads the same here:
+
+    model = DiffusionModel(width=32, height=32, physical_length=32.0, p0=0.15)
+    a = model.addSpecies("A", 1.0) ...
+    model.addReaction(rate=1.0, products={a: 1})
+    solver = model.createSolver()           # HomogeneousSolver / InhomogeneousSolver
+    seed = 7
+    seed = solver.doReactions(dt, seed)     # the reference mutates `int &seed`
+    seed = solver.diffuse(species, t, seed)
+    solver.getNextDiffusionTime()
+
+Seeds are returned instead of mutated in place (Python has no `int&`).
+All compute runs in libgpgmp_hip.so; nothing here falls back to the CPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, lib
+
+STOCHASTIC_HOMOGENEOUS = _lib.STOCHASTIC_HOMOGENEOUS
+STOCHASTIC_INHOMOGENEOUS = _lib.STOCHASTIC_INHOMOGENEOUS
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _uptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+class Species:
+    """Species (Species.h): identifier, index and homogeneous diffusion constant."""
+
+    def __init__(self, id, index, diffusion_constant):
+        self.id = id
+        self.index = index
+        self.diffusionConstant = float(diffusion_constant)
+
+
+class Reaction:
+    """Mass-action reaction (Reaction.h): rate and stoichiometry maps by species."""
+
+    def __init__(self, rate, reactants=None, products=None, id=None):
+        self.rate = float(rate)
+        self.reactants = dict(reactants or {})
+        self.products = dict(products or {})
+        self.id = id
+
+
+class DiffusionModel:
+    """The model description the solvers read (DiffusionModel.h:149-773 setters)."""
+
+    def __init__(self, width, height, physical_length, p0=0.15,
+                 solver=STOCHASTIC_HOMOGENEOUS, boundary_mask=(0, 0, 0, 0),
+                 source_mask=(0, 0, 0, 0), max_time=1e30):
+        self.width = int(width)
+        self.height = int(height)
+        self.physical_length = float(physical_length)
+        self.p0 = float(p0)
+        self.solver = int(solver)
+        self.boundary_mask = tuple(int(b) for b in boundary_mask)
+        self.source_mask = tuple(int(s) for s in source_mask)
+        self.max_time = float(max_time)
+        self.species = []
+        self.reactions = []
+        self.fields = None  # (Dx, Dy, rx, ry) each [S][H][W] float32
+
+    # DiffusionModel::addSpecies
+    def addSpecies(self, id, diffusion_constant=0.0):
+        sp = Species(id, len(self.species), diffusion_constant)
+        self.species.append(sp)
+        return sp
+
+    # DiffusionModel::addReaction (stoichiometry given as {species: count})
+    def addReaction(self, rate, reactants=None, products=None, id=None):
+        r = Reaction(rate, reactants, products, id)
+        self.reactions.append(r)
+        return r
+
+    def setDriftDiffusivityFields(self, Dx, Dy, rx, ry):
+        S, H, W = len(self.species), self.height, self.width
+        arrs = [np.ascontiguousarray(a, dtype=np.float32).reshape(S, H, W) for a in (Dx, Dy, rx, ry)]
+        self.fields = tuple(arrs)
+
+    def numSpecies(self):
+        return len(self.species)
+
+    def gridArea(self):
+        return self.width * self.height
+
+    def _desc(self):
+        S, R = len(self.species), len(self.reactions)
+        if R > _lib.MAX_REACTIONS or S > _lib.MAX_SPECIES:
+            raise ValueError("model exceeds %d species / %d reactions" % (_lib.MAX_SPECIES, _lib.MAX_REACTIONS))
+        rx = (_lib.ReactionDesc * max(R, 1))()
+        for i, r in enumerate(self.reactions):
+            rx[i].rate = r.rate
+            for sp, n in r.reactants.items():
+                rx[i].reactant_stoich[sp.index if isinstance(sp, Species) else int(sp)] += int(n)
+            for sp, n in r.products.items():
+                rx[i].product_stoich[sp.index if isinstance(sp, Species) else int(sp)] += int(n)
+        d = _lib.ModelDesc()
+        d.width, d.height = self.width, self.height
+        d.physical_length, d.p0 = self.physical_length, self.p0
+        d.num_species, d.num_reactions = S, R
+        for i, sp in enumerate(self.species):
+            d.diffusion[i] = sp.diffusionConstant
+        d.reactions = C.cast(rx, C.POINTER(_lib.ReactionDesc))
+        for i in range(4):
+            d.boundary_mask[i] = self.boundary_mask[i]
+            d.source_mask[i] = self.source_mask[i]
+        d.max_time = self.max_time
+        d.solver_type = self.solver
+        keep = [rx]
+        if self.fields is not None:
+            for name, arr in zip(("diffusivity_x", "diffusivity_y", "drift_x", "drift_y"), self.fields):
+                setattr(d, name, _fptr(arr))
+                keep.append(arr)
+        return d, keep
+
+    def compile(self):
+        d, keep = self._desc()
+        h = C.c_void_p()
+        call("gpgmp_model_create", C.byref(d), C.byref(h))
+        return CompiledModel(h, self)
+
+    def createSolver(self, stream=None):
+        """Factory of DiffusionModel::oclSolve (DiffusionModel.cpp:1422-1440)."""
+        cm = self.compile()
+        cls = HomogeneousSolver if self.solver == STOCHASTIC_HOMOGENEOUS else InhomogeneousSolver
+        return cls(cm, stream)
+
+
+class CompiledModel:
+    def __init__(self, handle, model):
+        self.handle = handle
+        self.model = model
+
+    def rateLiterals(self):
+        R = len(self.model.reactions)
+        is_int = np.zeros(max(R, 1), np.int32)
+        val = np.zeros(max(R, 1), np.float64)
+        call("gpgmp_model_rate_literals", self.handle, _iptr(is_int),
+             val.ctypes.data_as(C.POINTER(C.c_double)))
+        return is_int[:R].astype(bool), val[:R]
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().gpgmp_model_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)  # torch.cuda.Stream
+
+
+class Solver:
+    """gpgmp::Solver (Solver.h:15-239) over the HIP library."""
+
+    def __init__(self, compiled, stream=None):
+        self._cm = compiled
+        self.model = compiled.model
+        h = C.c_void_p()
+        call("gpgmp_solver_create", compiled.handle, _stream_ptr(stream), 0, C.byref(h))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().gpgmp_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        m = self.model
+        return (m.numSpecies(), m.height, m.width)
+
+    # Solver::synchronizeStateBufferToDevice (host state -> device)
+    def synchronizeStateBufferToDevice(self, state):
+        a = np.ascontiguousarray(state, dtype=np.int32).reshape(self.shape)
+        call("gpgmp_solver_set_state", self.handle, _iptr(a))
+
+    # Solver::synchronizeStateBufferFromDevice (device -> host state)
+    def synchronizeStateBufferFromDevice(self):
+        a = np.empty(self.shape, np.int32)
+        call("gpgmp_solver_get_state", self.handle, _iptr(a))
+        return a
+
+    def stateDevicePointer(self):
+        p = C.c_void_p()
+        call("gpgmp_solver_state_device", self.handle, C.byref(p))
+        return p.value
+
+    # Solver::doReactions(Real dt, int &seed) -> returns the updated seed
+    def doReactions(self, dt, seed):
+        s = C.c_int32(seed)
+        call("gpgmp_solver_do_reactions", self.handle, C.c_float(dt), C.byref(s))
+        return s.value
+
+    # Solver::diffuse(int species, Real currentTime, int &seed) -> returns the updated seed
+    def diffuse(self, species, current_time, seed):
+        s = C.c_int32(seed)
+        call("gpgmp_solver_diffuse", self.handle, int(species), C.c_float(current_time), C.byref(s))
+        return s.value
+
+    # Solver::getNextDiffusionTime()
+    def getNextDiffusionTime(self):
+        out = np.zeros(self.model.numSpecies(), np.float32)
+        call("gpgmp_solver_next_diffusion_time", self.handle, _fptr(out))
+        return out
+
+    def timeSteps(self):
+        S = self.model.numSpecies()
+        x = np.zeros(S, np.float32)
+        y = np.zeros(S, np.float32)
+        call("gpgmp_solver_time_steps", self.handle, _fptr(x), _fptr(y))
+        return x, y
+
+    def run(self, steps, seed0):
+        """DiffusionModel::oclSolve's loop (DiffusionModel.cpp:1467-1603) for `steps`
+        iterations from seed0, on the device without per-step synchronisation."""
+        t = C.c_float()
+        call("gpgmp_solver_run", self.handle, int(steps), C.c_uint32(seed0), C.byref(t))
+        return t.value
+
+    def runContinue(self, steps, graph_chunk=0):
+        t = C.c_float()
+        call("gpgmp_solver_run_continue", self.handle, int(steps), int(graph_chunk), C.byref(t))
+        return t.value
+
+    def setLaunch(self, block, cells_per_thread):
+        call("gpgmp_solver_set_launch", self.handle, int(block), int(cells_per_thread))
+
+    def synchronize(self):
+        call("gpgmp_solver_synchronize", self.handle)
+
+
+class HomogeneousSolver(Solver):
+    """HomogeneousSolver (HomogeneousSolver.h:15-81)."""
+
+
+class InhomogeneousSolver(Solver):
+    """InhomogeneousSolver (InhomogeneousSolver.h:15-162)."""
+
+
+class Ensemble:
+    """Batched ensemble: n realizations of one homogeneous network on grids of their
+    own size, each with its own seed0, advanced together (no reference counterpart)."""
+
+    def __init__(self, model, widths, heights, seed0s, stream=None):
+        self._cm = model.compile()
+        self.model = model
+        self.widths = np.ascontiguousarray(widths, dtype=np.int32)
+        self.heights = np.ascontiguousarray(heights, dtype=np.int32)
+        self.seed0s = np.ascontiguousarray(seed0s, dtype=np.uint32)
+        n = len(self.widths)
+        h = C.c_void_p()
+        call("gpgmp_batch_create", self._cm.handle, n, _iptr(self.widths), _iptr(self.heights),
+             _uptr(self.seed0s), _stream_ptr(stream), C.byref(h))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().gpgmp_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.widths)
+
+    def setState(self, r, state):
+        S = self.model.numSpecies()
+        a = np.ascontiguousarray(state, dtype=np.int32).reshape(S, self.heights[r], self.widths[r])
+        call("gpgmp_batch_set_state", self.handle, int(r), _iptr(a))
+
+    def getState(self, r):
+        S = self.model.numSpecies()
+        a = np.empty((S, self.heights[r], self.widths[r]), np.int32)
+        call("gpgmp_batch_get_state", self.handle, int(r), _iptr(a))
+        return a
+
+    def setLaunch(self, block, cells_per_thread):
+        call("gpgmp_batch_set_launch", self.handle, int(block), int(cells_per_thread))
+
+    def autotune(self, trial_steps=20):
+        b = C.c_int32()
+        c = C.c_int32()
+        call("gpgmp_batch_autotune", self.handle, int(trial_steps), C.byref(b), C.byref(c))
+        return b.value, c.value
+
+    def run(self, steps, graph_chunk=0):
+        call("gpgmp_batch_run", self.handle, int(steps), int(graph_chunk))
+
+    def reset(self):
+        call("gpgmp_batch_reset", self.handle)
+
+    def totalCells(self):
+        n = C.c_int64()
+        call("gpgmp_batch_total_cells", self.handle, C.byref(n))
+        return n.value
+
+    def synchronize(self):
+        call("gpgmp_batch_synchronize", self.handle)

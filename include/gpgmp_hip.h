@@ -1,0 +1,216 @@
+/*
+ * gpgmp_hip.h -- C ABI of the MI355X-native GPGMP solver (libgpgmp_hip.so).
+ *
+ * This boundary replaces the reference's OpenCL solver plugin interface
+ * (Inchman exec/src/gpgmp/src):
+ *   gpgmp::Solver                 Solver.h:15-239
+ *   gpgmp::HomogeneousSolver      HomogeneousSolver.h:15-81
+ *   gpgmp::InhomogeneousSolver    InhomogeneousSolver.h:15-162
+ *   gpgmp::IndividualSolver       IndividualSolver.h:36-297 (sort / cell indices)
+ *   DiffusionModel::oclSolve      DiffusionModel.cpp:1414-1620 (the host loop)
+ * Each entry point names the reference member it replaces.
+ *
+ * Conventions
+ *   - Every call returns int: GPGMP_OK (0) or a negative GPGMP_E_* code; no C++
+ *     exception crosses the ABI (the reference throws or exit(1)s, DiffusionModel.h:68-105).
+ *     gpgmp_last_error() returns the message of the calling thread's last failure.
+ *   - Host buffers are caller-owned and copied; device buffers are library-owned
+ *     unless a function takes a device pointer ("_device" suffix or documented).
+ *   - Streams are passed as void* (a hipStream_t; NULL = the default stream).
+ *   - A handle is used by one thread at a time; different handles may be used
+ *     concurrently on different streams.
+ *   - Counts are int32 in layout [species][height][width] with x fastest, the
+ *     reference's buffer order (index x + y*W + s*W*H, Solver.cpp:476-482).
+ */
+#ifndef GPGMP_HIP_H
+#define GPGMP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPGMP_OK 0
+#define GPGMP_E_INVALID (-1)     /* bad argument or model */
+#define GPGMP_E_HIP (-2)         /* HIP runtime error */
+#define GPGMP_E_CAPACITY (-3)    /* a size exceeds a capacity */
+#define GPGMP_E_STATE (-4)       /* call not valid in the handle's state */
+#define GPGMP_E_UNSUPPORTED (-5) /* feature outside this build */
+
+#define GPGMP_MAX_SPECIES 8
+#define GPGMP_MAX_REACTIONS 16
+
+/* Solver types (DiffusionModel.cpp:250-266 string map; Solver.h SolverType). */
+#define GPGMP_STOCHASTIC_HOMOGENEOUS 0
+#define GPGMP_STOCHASTIC_INHOMOGENEOUS 1
+#define GPGMP_DETERMINISTIC_HOMOGENEOUS_RK4 2
+#define GPGMP_DETERMINISTIC_INHOMOGENEOUS_RK4 3
+
+/* One reaction: stoichiometry per species and the rate constant
+ * (Reaction(id, Real rate, ...) Reaction.cpp:39-56; law generated at :119-138). */
+typedef struct {
+    float rate;
+    int32_t reactant_stoich[GPGMP_MAX_SPECIES];
+    int32_t product_stoich[GPGMP_MAX_SPECIES];
+} gpgmp_reaction_desc;
+
+/* Model description (the DiffusionModel setters used by the solvers,
+ * DiffusionModel.h:149-773). */
+typedef struct {
+    int32_t width, height;        /* grid cells; multiples of 16 (Globals.h:27) */
+    float physical_length;        /* domain side in um (physicalLength) */
+    float p0;                     /* homogeneous jump probability (setP0) */
+    int32_t num_species;
+    float diffusion[GPGMP_MAX_SPECIES]; /* um^2/s, homogeneous solvers (Species::diffusionConstant) */
+    int32_t num_reactions;
+    const gpgmp_reaction_desc *reactions;
+    int32_t boundary_mask[4];     /* -x,+x,-y,+y: 0 reflective, 1 absorbing (setBoundaryMasks) */
+    int32_t source_mask[4];       /* source-boundary particle counts */
+    float max_time;               /* time step of non-diffusing species (maxTime) */
+    int32_t solver_type;          /* GPGMP_* solver type */
+    /* inhomogeneous solvers: fields [S][H][W] on the host, copied at create
+     * (the generated computeDiffusionConstants_forSpecies_* output, Solver.cpp:659-777) */
+    const float *diffusivity_x, *diffusivity_y, *drift_x, *drift_y;
+} gpgmp_model_desc;
+
+typedef struct gpgmp_model gpgmp_model;
+typedef struct gpgmp_solver gpgmp_solver;
+typedef struct gpgmp_batch gpgmp_batch;
+typedef struct gpgmp_rdd gpgmp_rdd;
+typedef struct gpgmp_particles gpgmp_particles;
+
+const char *gpgmp_last_error(void);
+int gpgmp_version(int32_t *major, int32_t *minor);
+
+/* ---- model ---------------------------------------------------------------------- */
+/* Validates the description and compiles the reaction table; replaces the run-time
+ * kernel source generation and build (Solver.cpp:449-1070). */
+int gpgmp_model_create(const gpgmp_model_desc *desc, gpgmp_model **out);
+int gpgmp_model_destroy(gpgmp_model *m);
+/* Compiled per-reaction rate literal as the reference prints it (Reaction.cpp:122):
+ * is_int[i], value[i]. */
+int gpgmp_model_rate_literals(const gpgmp_model *m, int32_t *is_int, double *value);
+
+/* ---- plain solver (gpgmp::Solver and subclasses) --------------------------------- */
+/* Solver::Solver + HomogeneousSolver/InhomogeneousSolver constructors
+ * (Solver.cpp:18-172, HomogeneousSolver.cpp:8-125, InhomogeneousSolver.cpp:10-213). */
+int gpgmp_solver_create(const gpgmp_model *m, void *stream, uint32_t flags, gpgmp_solver **out);
+int gpgmp_solver_destroy(gpgmp_solver *s);
+/* Solver::synchronizeStateBufferToDevice / FromDevice (Solver.h:85-90). */
+int gpgmp_solver_set_state(gpgmp_solver *s, const int32_t *host_counts);
+int gpgmp_solver_get_state(gpgmp_solver *s, int32_t *host_counts);
+int gpgmp_solver_set_state_f32(gpgmp_solver *s, const float *host_conc);   /* deterministic */
+int gpgmp_solver_get_state_f32(gpgmp_solver *s, float *host_conc);
+/* Device-resident state buffer (int32 or float, [S][H][W]); valid until destroy. */
+int gpgmp_solver_state_device(gpgmp_solver *s, void **dev_ptr);
+/* Solver::doReactions(Real dt, int &seed) (Solver.h:52, Solver.cpp:207-248): seed += 1. */
+int gpgmp_solver_do_reactions(gpgmp_solver *s, float dt, int32_t *seed);
+/* Solver::diffuse(int species, Real t, int &seed) (Solver.h:80; HomogeneousSolver.cpp:132-164,
+ * InhomogeneousSolver.cpp:254-427): inhomogeneous sweeps add 1 to seed each. */
+int gpgmp_solver_diffuse(gpgmp_solver *s, int32_t species, float t, int32_t *seed);
+/* Solver::getNextDiffusionTime() (Solver.h:68; HomogeneousSolver.h:42,
+ * InhomogeneousSolver.cpp:238-252): out[num_species]. */
+int gpgmp_solver_next_diffusion_time(gpgmp_solver *s, float *out);
+/* Per-species time steps: homogeneous dt[s] (both arrays), inhomogeneous dtX/dtY. */
+int gpgmp_solver_time_steps(gpgmp_solver *s, float *dt_x, float *dt_y);
+/* DiffusionModel::oclSolve main loop (DiffusionModel.cpp:1467-1603) for `steps`
+ * iterations from seed0 (boost::mt19937(seed0), uniform_int<>(0,INT_MAX)), with no
+ * output/alert events, as fused launches without host synchronisation. */
+int gpgmp_solver_run(gpgmp_solver *s, int32_t steps, uint32_t seed0, float *sim_time_out);
+/* Continues the loop of the last gpgmp_solver_run for `steps` more iterations;
+ * graph_chunk > 0 replays chunks of that many iterations as a HIP graph. */
+int gpgmp_solver_run_continue(gpgmp_solver *s, int32_t steps, int32_t graph_chunk, float *sim_time_out);
+/* launch shape of the solver's kernels: threads per block (64..256), cells per thread */
+int gpgmp_solver_set_launch(gpgmp_solver *s, int32_t block, int32_t cells_per_thread);
+int gpgmp_solver_synchronize(gpgmp_solver *s);
+
+/* ---- batched ensemble (no reference counterpart) ----------------------------------- */
+/* n realizations of one network on grids of their own size, each with its own seed0;
+ * the cell width h = physical_length/width of the template is kept for every grid.
+ * Counts are bit-exact with a plain solver of the same grid and seed0. */
+int gpgmp_batch_create(const gpgmp_model *tmpl, int32_t n, const int32_t *widths,
+                       const int32_t *heights, const uint32_t *seed0s, void *stream,
+                       gpgmp_batch **out);
+int gpgmp_batch_destroy(gpgmp_batch *b);
+int gpgmp_batch_set_state(gpgmp_batch *b, int32_t r, const int32_t *host_counts);
+int gpgmp_batch_get_state(gpgmp_batch *b, int32_t r, int32_t *host_counts);
+/* launch shape: threads per block (64..256) and cells per thread (1..16) */
+int gpgmp_batch_set_launch(gpgmp_batch *b, int32_t block, int32_t cells_per_thread);
+/* Times the candidate launch shapes on `trial_steps` steps from a copy of the current
+ * state (restored afterwards) and keeps the fastest; reports it. */
+int gpgmp_batch_autotune(gpgmp_batch *b, int32_t trial_steps, int32_t *block_out,
+                         int32_t *cpt_out);
+/* Runs `steps` host-loop iterations of every realization.  graph_chunk > 0 captures
+ * chunks of that many iterations in a HIP graph that is replayed. */
+int gpgmp_batch_run(gpgmp_batch *b, int32_t steps, int32_t graph_chunk);
+/* Restarts the host loops at time 0 with the realizations' seed0 (state is kept). */
+int gpgmp_batch_reset(gpgmp_batch *b);
+int gpgmp_batch_total_cells(gpgmp_batch *b, int64_t *cells);
+int gpgmp_batch_synchronize(gpgmp_batch *b);
+
+/* ---- inhomogeneous time step (computeIndividualTimestepX/Y + reduceBlocks) -------- */
+/* Per-cell admissible time step (StochasticInhomogeneousDriftDiffusion.cl:130-151) and
+ * its minimum over n cells, on the device in double; D, r device pointers. */
+int gpgmp_min_timestep_device(const float *D, const float *r, int32_t n, float h, float *out_host,
+                              void *stream);
+
+/* ---- deterministic reaction-drift-diffusion (Schnakenberg, fp32) ------------------ */
+typedef struct {
+    int32_t width, height;
+    float h, dt;
+    float D0u, D0v;          /* per-species diffusivity scale of the field D */
+    float k[4];              /* k1 (0->U), k2 (0->V), k3 (2U+V->3U), k4 (U->0) */
+} gpgmp_rdd_params;
+/* Workspace for `steps` steps with a checkpoint every ckpt_every steps. */
+int gpgmp_rdd_create(const gpgmp_rdd_params *p, int32_t steps, int32_t ckpt_every, void *stream,
+                     gpgmp_rdd **out);
+int gpgmp_rdd_destroy(gpgmp_rdd *r);
+/* Forward: u,v (device, in/out) advanced `steps` fused Euler steps with field D; stores
+ * checkpoints when save != 0. */
+int gpgmp_rdd_forward(gpgmp_rdd *r, const float *D, float *u, float *v, int32_t save);
+/* Adjoint of the last saved forward: lambda_u/lambda_v (device, in: dL/du_T, dL/dv_T;
+ * clobbered), outputs gD (device, [H][W]) and gk (device, 4 floats). */
+int gpgmp_rdd_backward(gpgmp_rdd *r, const float *D, float *lu, float *lv, float *gD, float *gk);
+
+/* Deterministic stencils of Deterministic.cl on device fp32 buffers (Jacobi). */
+int gpgmp_det_hom_diffuse_device(float *state, int32_t W, int32_t H, int32_t S, const float *D_host,
+                                 float dt, float h, void *stream);
+int gpgmp_det_inhom_sweep_device(float *state, int32_t W, int32_t H, int32_t species, int32_t axis,
+                                 float dt, const float *D_axis, const float *r_axis, float h,
+                                 void *stream);
+
+/* ---- individual particles (IndividualSolver sort / cell indices) ------------------- */
+/* Stable LSD radix sort of n (key, value) pairs by the 32-bit key, device pointers;
+ * keys_tmp/vals_tmp are n-element scratch (IndividualSolver::sortSpecies,
+ * IndividualSolver.cpp:673-808).  key_bits limits the passes (32 = full key). */
+int gpgmp_sort_pairs_device(uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp,
+                            int32_t n, int32_t key_bits, void *stream);
+/* CSR offsets off[c] = #(key < c), c = 0..ncells, of sorted keys. */
+int gpgmp_cell_offsets_device(const uint32_t *sorted_keys, int32_t n, int32_t ncells, uint32_t *off,
+                              void *stream);
+/* computeCellIndices (IndividualSolver.cl:564-619): left/right per cell. */
+int gpgmp_cell_left_right_device(const uint32_t *sorted_keys, int32_t n, int32_t ncells,
+                                 uint32_t *left, uint32_t *right, void *stream);
+
+typedef struct {
+    int32_t side;            /* cells per side, 1 um cells, periodic domain [0, side)^2 */
+    float D, dt, vortex_amp, p_annihilate;
+    uint32_t seed;
+} gpgmp_particle_params;
+int gpgmp_particles_create(const gpgmp_particle_params *p, int32_t capacity, void *stream,
+                           gpgmp_particles **out);
+int gpgmp_particles_destroy(gpgmp_particles *ps);
+int gpgmp_particles_set(gpgmp_particles *ps, const float *x, const float *y, const uint32_t *id,
+                        int32_t n);
+/* n_out holds the live count; buffers sized by the caller to that count */
+int gpgmp_particles_get(gpgmp_particles *ps, float *x, float *y, uint32_t *id, uint32_t *keys,
+                        uint32_t *off, int32_t *n_out);
+int gpgmp_particles_count(gpgmp_particles *ps, int32_t *n_out);
+/* `steps` steps starting at step index `step0`: move, stable sort, offsets, annihilate. */
+int gpgmp_particles_step(gpgmp_particles *ps, int32_t step0, int32_t steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

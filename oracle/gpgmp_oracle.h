@@ -1,0 +1,141 @@
+/*
+ * CPU oracle for the GPGMP hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library restates the reference algorithms
+ * (Inchman exec/src/gpgmp/src/<kernel>.cl and the C++ host that drives them) on the
+ * CPU so that the HIP path can be checked against it.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only
+ * as the checker: the product library never links or calls it.
+ *
+ * Every function cites the reference file:line it follows.  Parity pinning
+ * (what each function is checked against) is listed per function; functions
+ * whose only check is agreement with the HIP path are marked
+ * "parity unpinned".
+ *
+ * Numeric contract shared with the HIP path (the reference leaves these
+ * implementation-defined):
+ *   - Threefry-2x32-20 (Salmon et al., SC'11), KAT-pinned.
+ *   - uniform map u = (float)((double)(int32)w * 2^-32 + 0.5)
+ *     (Gillespie.cl:72-73, constant Solver.cpp:469-471).
+ *   - log: (float) of a double-precision log evaluated by a fixed sequence of
+ *     IEEE double operations (oracle_logf below); pinned against libm log().
+ *   - rate literals: the decimal the reference prints with `ss << rate`
+ *     (%g, 6 significant digits, Reaction.cpp:122) parsed back; a literal
+ *     without '.', 'e' or "inf"/"nan" is an int literal (int arithmetic),
+ *     anything else a double literal (double arithmetic).
+ *   - header constants printed "%.6e" then parsed as float (Solver.cpp:494-505).
+ *   - no floating-point contraction (compiled with -ffp-contract=off).
+ */
+#ifndef GPGMP_ORACLE_H
+#define GPGMP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORACLE_MAX_SPECIES 16
+#define ORACLE_MAX_REACTIONS 32
+#define ORACLE_MAX_FACTORS 8
+
+/* Compiled reaction table: one entry per reaction, species in index order. */
+typedef struct {
+    int num_species;
+    int num_reactions;
+    /* literal of the rate as printed by the reference */
+    int lit_is_int[ORACLE_MAX_REACTIONS];
+    int32_t lit_int[ORACLE_MAX_REACTIONS];
+    double lit_dbl[ORACLE_MAX_REACTIONS];
+    /* falling-factorial factors: species index and offset m of (X - m) */
+    int n_factors[ORACLE_MAX_REACTIONS];
+    int factor_species[ORACLE_MAX_REACTIONS][ORACLE_MAX_FACTORS];
+    int factor_offset[ORACLE_MAX_REACTIONS][ORACLE_MAX_FACTORS];
+    /* net stoichiometric change (product - reactant) */
+    int net[ORACLE_MAX_REACTIONS][ORACLE_MAX_SPECIES];
+} oracle_reaction_table;
+
+/* ---- primitives ---- */
+void oracle_threefry2x32_20(uint32_t c0, uint32_t c1, uint32_t k0, uint32_t k1,
+                            uint32_t *o0, uint32_t *o1);
+float oracle_u01(uint32_t w);
+float oracle_logf(float y);
+double oracle_logd(double y);
+void oracle_mt19937_seeds(uint32_t seed0, int n, int32_t *out);
+
+/* ---- model compile ---- */
+int oracle_compile_reactions(int num_species, int num_reactions, const float *rates,
+                             const int *reactant_stoich, const int *product_stoich,
+                             oracle_reaction_table *out);
+float oracle_stay_probability(float p0);
+float oracle_header_float(float v);
+float oracle_homogeneous_dt(float p0, float h, float D);
+float oracle_inhomogeneous_cell_dt(float D, float r, float h);
+float oracle_inhomogeneous_dt(const float *D, const float *r, int n, float h);
+
+/* ---- stochastic kernels ---- */
+void oracle_ssa(int32_t *state, int W, int H, const oracle_reaction_table *tab,
+                int32_t seed, float deltat, int nthreads);
+void oracle_hom_diffuse(int32_t *state, int W, int H, int species, int32_t seed,
+                        float stay, const int *boundary_mask, const int *source_mask,
+                        int nthreads);
+void oracle_inhom_sweep(int32_t *state, int W, int H, int species, int axis, int32_t seed,
+                        float dt, const float *D_axis, const float *r_axis, float h,
+                        const int *boundary_mask, int nthreads);
+
+/* ---- host schedule (DiffusionModel::oclSolve) ---- */
+typedef struct {
+    int W, H, S;
+    float physical_length;
+    float p0;
+    int inhomogeneous;          /* 0: stochastic_homogeneous, 1: stochastic_inhomogeneous */
+    const float *diffusion;     /* [S] (homogeneous) */
+    const float *Dx, *Dy, *rx, *ry; /* [S][H][W] (inhomogeneous) */
+    int boundary_mask[4];
+    int source_mask[4];
+    float max_time;             /* dt of non-diffusing species; loop bound */
+} oracle_model;
+
+int oracle_run(const oracle_model *m, const oracle_reaction_table *tab, int32_t *state,
+               int steps, uint32_t seed0, int nthreads, float *sim_time_out);
+
+/* ---- deterministic (FP64 Jacobi) ---- */
+void oracle_det_hom_diffuse(double *state, int W, int H, int S, const double *D, double dt,
+                            double h);
+void oracle_det_inhom_sweep(double *state, int W, int H, int species, int axis, double dt,
+                            const double *D_axis, const double *r_axis, double h);
+void oracle_det_rk4(double *state, int W, int H, const oracle_reaction_table *tab, double dt);
+
+/* ---- reaction-drift-diffusion (Schnakenberg, FP64) ---- */
+typedef struct {
+    int W, H;
+    double h, dt;
+    double D0u, D0v;
+    double k[4];
+} oracle_rdd_params;
+void oracle_rdd_forward(const oracle_rdd_params *p, const double *D, double *u, double *v,
+                        int steps, int nthreads);
+double oracle_rdd_loss(int n, const double *u, const double *v, const double *ut,
+                       const double *vt);
+double oracle_rdd_grad(const oracle_rdd_params *p, const double *D, const double *u0,
+                       const double *v0, const double *ut, const double *vt, int steps,
+                       int ckpt_every, double *gD, double *gk, int nthreads);
+
+/* ---- individual particles ---- */
+void oracle_stable_sort_pairs(uint32_t *keys, uint32_t *vals, int n);
+void oracle_cell_offsets(const uint32_t *sorted_keys, int n, int ncells, uint32_t *off);
+void oracle_cell_left_right(const uint32_t *sorted_keys, int n, int ncells, uint32_t *left,
+                            uint32_t *right);
+typedef struct {
+    int side;          /* cells per side (domain [0,side)^2, 1 um cells) */
+    float D, dt, vortex_amp;
+    float p_annihilate;
+    uint32_t seed;
+} oracle_particle_params;
+int oracle_particle_step(const oracle_particle_params *p, int step, int n, float *x, float *y,
+                         uint32_t *id, uint32_t *keys, uint32_t *off, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

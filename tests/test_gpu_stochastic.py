@@ -1,0 +1,161 @@
+"""Parity of the HIP stochastic kernels with the CPU oracle: bit-exact counts on the
+same inputs and seeds (SSA, homogeneous and inhomogeneous diffusion, the host loop,
+batched ensemble vs plain solver)."""
+import numpy as np
+import pytest
+
+from inchman_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(model):
+    return model.createSolver()
+
+
+@pytest.mark.parametrize("seed,dt", [(7, 0.025), (123456789, 0.5), (-5, 3.0)])
+def test_ssa_plain_gmp(gpu, oracle, seed, dt):
+    m = configs.plain_gmp_model(32)
+    st = configs.plain_gmp_state(32, seed=abs(seed) + 1)
+    s = _solver(m)
+    s.synchronizeStateBufferToDevice(st)
+    new_seed = s.doReactions(dt, seed)
+    assert new_seed == seed + 1
+    got = s.synchronizeStateBufferFromDevice()
+    want = oracle.ssa(st, oracle.table_from_model(m), seed, dt)
+    assert np.array_equal(got, want)
+    assert not np.array_equal(got, st)
+
+
+def test_ssa_schnakenberg(gpu, oracle):
+    m = configs.schnakenberg_model(64)
+    st = configs.schnakenberg_state(64, seed=3)
+    s = _solver(m)
+    s.synchronizeStateBufferToDevice(st)
+    s.doReactions(0.11, 99)
+    got = s.synchronizeStateBufferFromDevice()
+    want = oracle.ssa(st, oracle.table_from_model(m), 99, 0.11)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("bm,sm", [((0, 0, 0, 0), (0, 0, 0, 0)), ((1, 0, 1, 0), (0, 0, 0, 0)),
+                                   ((0, 0, 0, 0), (0, 5, 3, 2))])
+def test_hom_diffuse(gpu, oracle, bm, sm):
+    m = configs.plain_gmp_model(48, 32)
+    m.boundary_mask = bm
+    m.source_mask = sm
+    st = configs.plain_gmp_state(48, 32, seed=11)
+    st[2] = np.random.default_rng(5).poisson(3, st[2].shape)
+    s = _solver(m)
+    s.synchronizeStateBufferToDevice(st)
+    seed = 4242
+    want = st.copy()
+    stay = oracle.stay_probability(m.p0)
+    for sp in range(3):
+        seed2 = s.diffuse(sp, 1.0, seed)
+        assert seed2 == seed  # homogeneous diffusion keeps the seed
+        want = oracle.hom_diffuse(want, sp, seed, stay, bm, sm)
+    got = s.synchronizeStateBufferFromDevice()
+    assert np.array_equal(got, want)
+    if bm == (0, 0, 0, 0) and sm == (0, 0, 0, 0):
+        assert np.array_equal(got.sum(axis=(1, 2)), st.sum(axis=(1, 2)))
+
+
+def test_inhom_sweeps(gpu, oracle):
+    m = configs.schnakenberg_model(64, 48)
+    st = configs.schnakenberg_state(64, 48, seed=8)
+    s = _solver(m)
+    s.synchronizeStateBufferToDevice(st)
+    dtx, dty = s.timeSteps()
+    Dx, Dy, rx, ry = m.fields
+    h = oracle.header_float(m.physical_length / m.width)
+    hy = oracle.header_float(m.physical_length / m.height)
+    for sp in range(2):
+        assert dtx[sp] == oracle.inhomogeneous_dt(Dx[sp], rx[sp], h)
+        assert dty[sp] == oracle.inhomogeneous_dt(Dy[sp], ry[sp], h)
+    seed = 17
+    want = st.copy()
+    t = float(max(dtx.max(), dty.max()))
+    for sp in range(2):
+        want = oracle.inhom_sweep(want, sp, 0, seed, dtx[sp], Dx, rx, h)
+        want = oracle.inhom_sweep(want, sp, 1, seed + 1, dty[sp], Dy, ry, hy)
+        seed2 = s.diffuse(sp, t, seed)
+        assert seed2 == seed + 2
+        seed = seed2
+    got = s.synchronizeStateBufferFromDevice()
+    assert np.array_equal(got, want)
+    assert np.array_equal(got.sum(axis=(1, 2)), st.sum(axis=(1, 2)))
+
+
+def test_run_plain_gmp(gpu, oracle):
+    m = configs.plain_gmp_model(32)
+    st = configs.plain_gmp_state(32)
+    s = _solver(m)
+    s.synchronizeStateBufferToDevice(st)
+    t_gpu = s.run(1000, 1)
+    got = s.synchronizeStateBufferFromDevice()
+    want, t_cpu = oracle.run(m, st, 1000, 1)
+    assert t_gpu == t_cpu
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("side,steps", [(64, 300), (256, 40)])
+def test_run_schnakenberg(gpu, oracle, side, steps):
+    m = configs.schnakenberg_model(side)
+    st = configs.schnakenberg_state(side)
+    s = _solver(m)
+    s.synchronizeStateBufferToDevice(st)
+    t_gpu = s.run(steps, 5)
+    got = s.synchronizeStateBufferFromDevice()
+    want, t_cpu = oracle.run(m, st, steps, 5)
+    assert t_gpu == t_cpu
+    assert np.array_equal(got, want)
+
+
+def test_run_continue_graph(gpu):
+    m = configs.schnakenberg_model(64)
+    st = configs.schnakenberg_state(64)
+    a = _solver(m)
+    a.synchronizeStateBufferToDevice(st)
+    a.run(100, 9)
+    ref = a.synchronizeStateBufferFromDevice()
+    b = _solver(m)
+    b.synchronizeStateBufferToDevice(st)
+    b.run(30, 9)
+    b.runContinue(50, graph_chunk=16)
+    b.runContinue(20, graph_chunk=0)
+    assert np.array_equal(b.synchronizeStateBufferFromDevice(), ref)
+
+
+def test_ensemble_matches_plain(gpu):
+    m = configs.plain_gmp_model(32)
+    sides = np.array([16, 32, 48, 64, 80, 128, 96], np.int32)
+    seeds = configs.ensemble_seeds(len(sides))
+    states = [configs.ensemble_state(sd, r) for r, sd in enumerate(sides)]
+    from inchman_amd import Ensemble
+
+    ens = Ensemble(m, sides, sides, seeds)
+    for r, st in enumerate(states):
+        ens.setState(r, st)
+    ens.setLaunch(128, 2)
+    ens.run(40, graph_chunk=0)
+    ens.run(60, graph_chunk=20)
+    for r, sd in enumerate(sides):
+        pm = configs.plain_gmp_model(int(sd))
+        p = pm.createSolver()
+        p.synchronizeStateBufferToDevice(states[r])
+        p.run(100, int(seeds[r]))
+        assert np.array_equal(ens.getState(r), p.synchronizeStateBufferFromDevice()), r
+
+
+def test_launch_shape_invariance(gpu):
+    m = configs.plain_gmp_model(64)
+    st = configs.plain_gmp_state(64)
+    outs = []
+    for block, cpt in [(256, 1), (64, 4), (128, 3)]:
+        s = m.createSolver()
+        s.setLaunch(block, cpt)
+        s.synchronizeStateBufferToDevice(st)
+        s.run(50, 3)
+        outs.append(s.synchronizeStateBufferFromDevice())
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
