@@ -1,0 +1,152 @@
+"""Pins the oracle's SSA and stochastic diffusion against closed forms, invariants and
+the reference's own statistical acceptance criteria (python/gpgmp/test/*.py)."""
+import numpy as np
+
+from inchman_amd import DiffusionModel, configs
+from inchman_amd.gpgmp import STOCHASTIC_HOMOGENEOUS
+
+
+def _bd_table(oracle, k, g):
+    # 0 -> A (k), A -> 0 (g)
+    return oracle.compile_reactions(1, [k, g], [[0], [1]], [[1], [0]])
+
+
+def test_birth_death_is_poisson(oracle):
+    """Stationary law of 0->A (k), A->0 (g) is Poisson(k/g): mean = variance."""
+    t = _bd_table(oracle, 10.0, 1.0)
+    st = np.zeros((1, 64, 64), np.int32)
+    st = oracle.ssa(st, t, 3, 40.0)
+    a = st[0].astype(np.float64).ravel()
+    assert abs(a.mean() - 10.0) < 0.25
+    assert abs(a.var() - 10.0) < 1.0
+
+
+def test_birth_death_transient_mean(oracle):
+    """E[A(t)] = k/g + (A0 - k/g) e^{-g t}."""
+    t = _bd_table(oracle, 4.0, 0.5)
+    st = np.full((1, 64, 64), 30, np.int32)
+    st = oracle.ssa(st, t, 9, 1.0)
+    want = 8.0 + (30 - 8.0) * np.exp(-0.5)
+    assert abs(st.mean() - want) < 0.15
+
+
+def test_ab_reaction_steady_state(oracle):
+    """A+B reaction without diffusion (python/gpgmp/test/abreaction.py:111-118):
+    2A->0 k1=1e-3, A+B->0 k2=1e-2, 0->A k3=1.2, 0->B k4=1.0; means within 1% of
+    9.6 (A) and 12.1 (B)."""
+    t = oracle.compile_reactions(2, [1e-3, 1e-2, 1.2, 1.0], [[2, 0], [1, 1], [0, 0], [0, 0]],
+                                 [[0, 0], [0, 0], [1, 0], [0, 1]])
+    st = np.zeros((2, 128, 128), np.int32)
+    st = oracle.ssa(st, t, 1, 200.0)
+    means = []
+    for i in range(4):
+        st = oracle.ssa(st, t, 100 + i, 50.0)
+        means.append(st.reshape(2, -1).mean(axis=1))
+    a, b = np.mean(means, axis=0)
+    assert abs(a - 9.6) / 9.6 <= 0.01, a
+    assert abs(b - 12.1) / 12.1 <= 0.01, b
+
+
+def test_hom_diffusion_conserves_and_spreads(oracle):
+    """Reflective homogeneous sweep conserves particles; per-axis displacement variance
+    of one sweep is 2 (1-stay)/6 h^2 = 2 D dt (HomogeneousSolver.cpp:89)."""
+    st = np.zeros((1, 64, 64), np.int32)
+    st[0, 32, 32] = 200000
+    p0 = 0.3
+    stay = oracle.stay_probability(p0)
+    out = oracle.hom_diffuse(st, 0, 77, stay)
+    assert out.sum() == st.sum()
+    ys, xs = np.nonzero(out[0])
+    w = out[0][ys, xs].astype(np.float64)
+    mx = (w * (xs - 32)).sum() / w.sum()
+    vx = (w * (xs - 32) ** 2).sum() / w.sum()
+    vy = (w * (ys - 32) ** 2).sum() / w.sum()
+    assert abs(mx) < 0.005
+    assert abs(vx - 2 * p0 / 6) < 0.005 and abs(vy - 2 * p0 / 6) < 0.005
+    # at most one hop per sweep
+    assert out[0].sum() == out[0, 31:34, 31:34].sum()
+
+
+def test_absorbing_boundary_loses_particles(oracle):
+    st = np.zeros((1, 16, 16), np.int32)
+    st[0, :, 0] = 1000
+    out = oracle.hom_diffuse(st, 0, 5, oracle.stay_probability(0.6), bm=(1, 0, 0, 0))
+    lost = st.sum() - out.sum()
+    # each particle at x = 0 leaves through -x with probability p0/6
+    assert abs(lost / 16000 - 0.1) < 0.01
+
+
+def test_source_boundary_adds(oracle):
+    st = np.zeros((1, 16, 16), np.int32)
+    stay = oracle.stay_probability(0.6)
+    out = oracle.hom_diffuse(st, 0, 5, stay, sm=(0, 0, 600, 0))
+    # y = 0 row gains Binomial(600, p0/6) per cell
+    row = out[0, 0].astype(np.float64)
+    assert abs(row.mean() - 60) < 4
+    assert out[0, 1:].sum() == 0
+
+
+def test_heat_kernel(oracle):
+    """python/gpgmp/test/homogeneous_diffusion.py: 1e5 particles at the centre of a
+    128x128 grid of side 100, D = 1; the centre slices of n/(N dx^2) match the 2-D
+    heat kernel with RMSE <= 5e-4."""
+    m = DiffusionModel(128, 128, 100.0, p0=0.5, solver=STOCHASTIC_HOMOGENEOUS)
+    m.addSpecies("A", 1.0)
+    st = np.zeros((1, 128, 128), np.int32)
+    st[0, 64, 64] = 100000
+    out, t = oracle.run(m, st, 400, 3)
+    dx = 100.0 / 128
+    x = np.arange(128) * dx - 50.0
+    heat = 1.0 / (4 * np.pi * t) * np.exp(-(x * x) / (4 * t))
+    n = out[0] / (1e5 * dx * dx)
+    for sl in (n[64, :], n[:, 64]):
+        assert np.sqrt(np.mean((sl - heat) ** 2)) <= 5e-4
+    assert out.sum() == 100000
+
+
+def test_inhom_sweep_mean_and_variance(oracle):
+    """One inhomogeneous sweep moves the centre of mass by r dt and adds 2 D dt of
+    variance along its axis (StochasticInhomogeneousDriftDiffusion.cl:518-520)."""
+    W = H = 64
+    D = np.full((1, H, W), 0.8, np.float32)
+    r = np.full((1, H, W), 0.4, np.float32)
+    h = 1.0
+    dt = oracle.inhomogeneous_dt(D[0], r[0], h)
+    st = np.zeros((1, H, W), np.int32)
+    st[0, 32, 32] = 400000
+    for axis in (0, 1):
+        out = oracle.inhom_sweep(st, 0, axis, 11, dt, D, r, h)
+        assert out.sum() == st.sum()
+        prof = out[0].sum(axis=0) if axis == 0 else out[0].sum(axis=1)
+        pos = np.arange(64) - 32.0
+        w = prof.astype(np.float64)
+        mean = (w * pos).sum() / w.sum()
+        var = (w * pos * pos).sum() / w.sum() - mean ** 2
+        assert abs(mean - 0.4 * dt) < 0.003
+        assert abs(var - 2 * 0.8 * dt) < 0.005
+    # the step tends to the pure-diffusion limit h^2/(6D) for small r (the reference's
+    # expression cancels catastrophically for eps = h r/(2D) below ~1e-5, as there)
+    assert abs(oracle.inhomogeneous_dt(np.array([0.8], np.float32), np.array([1e-3], np.float32), h)
+               - 1.0 / (6 * 0.8)) < 1e-5
+    # and to h/|r| when D = 0
+    assert oracle.inhomogeneous_dt(np.array([0.0], np.float32), np.array([0.5], np.float32), h) == 2.0
+
+
+def test_schedule_seeds_and_times(oracle):
+    """The host loop of the plain GMP configuration: the simulated time is always one of
+    the species' float-accumulated diffusion times (DiffusionModel.cpp:1489-1500), and
+    float rounding of those sums can add short iterations (so 40 iterations end
+    at or before 40 steps of A)."""
+    m = configs.plain_gmp_model(32)
+    st = configs.plain_gmp_state(32)
+    out, t = oracle.run(m, st, 40, 1)
+    times = set()
+    for D in (1.0, 0.5, 0.1):
+        dt = np.float32(oracle.homogeneous_dt(0.15, 1.0, D))
+        acc = np.float32(0)
+        for _ in range(60):
+            acc = np.float32(acc + dt)
+            times.add(float(acc))
+    assert t in times
+    assert t <= float(np.float32(40 * 0.025)) + 1e-6
+    assert (out >= 0).all()
