@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X GPGMP hot path; prints one JSON line.
+
+Headline workload (value): the plain stochastic inhomogeneous drift-diffusion
+solver on the 256x256 Schnakenberg configuration (2 species, 4 reactions,
+D_s(x) = D0_s (1 + 0.5 sin(2 pi x/L)), drift (0.02, 0), initial Poisson(200)/(75)).
+A step is one host-loop iteration of DiffusionModel::oclSolve: the SSA over the
+iteration's dt plus the due X/Y diffusion sweeps; the default K = 10^4 steps is
+the whole configured run.  value = cells * K / wall time (cell-steps/s) with the
+state resident in HBM.
+
+Also reported (extra): the batched ensemble (512 realizations, autotuned launch
+shape, HIP-graph chunks) against the plain per-realization solver, and the CPU
+oracle timed on a bounded sample of the headline workload.
+
+    python bench.py [--gpus 1] [--steps 10000] [--warmup 100]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-steps/s at 1 GPU, batched vs plain; HBM GB/s; adjoint grad/s at 1e-4 rel"
+
+
+def cpu_baseline(steps):
+    """The CPU oracle (a restatement of the reference kernels, OpenMP over cells) on the
+    first `steps` iterations of the headline workload."""
+    from oracle import oracle
+    from inchman_amd import configs
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    m = configs.schnakenberg_model(256)
+    st = configs.schnakenberg_state(256)
+    t0 = time.perf_counter()
+    oracle.run(m, st, steps, 5, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": 65536 * steps / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": "first %d host-loop iterations of the 256x256 Schnakenberg run (%.1f s)" % (steps, dt)}
+
+
+def plain_run(solver, st, seed0, steps, timing):
+    import torch
+
+    solver.synchronizeStateBufferToDevice(st)
+    solver.run(0, seed0)
+    solver.kernelTimes(on=timing)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.runContinue(steps, 0)
+    solver.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms, n, b = solver.kernelTimes(on=False)
+    return wall, ms, n, b
+
+
+def ensemble_extra(steps=200):
+    import numpy as np
+    import torch
+    from inchman_amd import Ensemble, configs
+
+    n = 512
+    sides = configs.ensemble_sides(n)
+    seeds = configs.ensemble_seeds(n)
+    m = configs.plain_gmp_model(32)
+    ens = Ensemble(m, sides, sides, seeds)
+    states = [configs.ensemble_state(sd, r) for r, sd in enumerate(sides)]
+    for r, st in enumerate(states):
+        ens.setState(r, st)
+    block, cpt = ens.autotune(10)
+    for r, st in enumerate(states):
+        ens.setState(r, st)
+    ens.reset()
+    ens.run(20, 0)
+    ens.synchronize()
+    cells = ens.totalCells()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.run(steps, 50)
+    ens.synchronize()
+    wall = time.perf_counter() - t0
+    batched = cells * steps / wall
+    # plain solver, one realization after the other, on a slice spanning the sizes
+    idx = np.argsort(sides)[:: max(1, n // 24)]
+    tot, cs = 0.0, 0
+    for r in idx:
+        pm = configs.plain_gmp_model(int(sides[r]))
+        p = pm.createSolver()
+        p.synchronizeStateBufferToDevice(states[r])
+        p.run(20, int(seeds[r]))
+        p.synchronize()
+        t0 = time.perf_counter()
+        p.runContinue(steps, 50)
+        p.synchronize()
+        tot += time.perf_counter() - t0
+        cs += int(sides[r]) ** 2 * steps
+    plain = cs / tot
+    return {"workload": "512 realizations of the plain GMP network, sides 64..512 (sum %d cells), %d steps"
+                        % (cells, steps),
+            "batched_cell_steps_per_s": batched, "plain_cell_steps_per_s": plain,
+            "batched_over_plain": batched / plain, "launch": {"block": block, "cells_per_thread": cpt},
+            "graph_chunk": 50}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the solver on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from inchman_amd import configs
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    m = configs.schnakenberg_model(256)
+    st = configs.schnakenberg_state(256)
+    seed0 = 5
+    solver = m.createSolver()
+    # warmup: W iterations from the initial state
+    solver.synchronizeStateBufferToDevice(st)
+    solver.run(args.warmup, seed0)
+    solver.synchronize()
+    # timed: K iterations from the initial state, per-launch HIP events on the solver's stream
+    wall, ms, n, b = plain_run(solver, st, seed0, args.steps, timing=True)
+    cells = 256 * 256
+    value = cells * args.steps / wall
+    dom = int(max(range(3), key=lambda i: ms[i]))
+    avg_ms = ms[dom] / max(n[dom], 1)
+    achieved = (b[dom] / max(n[dom], 1)) / (avg_ms * 1e-3) / 1e9
+    names = {0: "k_gmp_step<S,MODE_HOM>", 1: "k_gmp_step<2,MODE_INH_X> (SSA + X sweeps)",
+             2: "k_gmp_step<2,MODE_INH_Y> (Y sweeps)"}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "stochastic inhomogeneous drift-diffusion, Schnakenberg, 256x256, "
+                               "reference-exact Threefry streams",
+                   "grid": [256, 256], "species": 2, "reactions": 4, "iterations": args.steps},
+        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": avg_ms, "launches": int(n[dom]),
+                     "algorithmic_bytes_per_launch": b[dom] / max(n[dom], 1),
+                     "note": "RNG-bound (one Threefry-2x32-20 per event and per particle pair): "
+                             "VALU busy is the binding resource, HBM bytes are small"},
+    }
+    extra = {"kernel_ms": {names[i]: ms[i] for i in range(3) if n[i]},
+             "kernel_launches": {names[i]: int(n[i]) for i in range(3) if n[i]}}
+    # same sample as the CPU baseline, for a like-for-like ratio
+    w2, _, _, _ = plain_run(solver, st, seed0, args.cpu_steps, timing=False)
+    extra["gpu_on_cpu_sample_cell_steps_per_s"] = cells * args.cpu_steps / w2
+    if not args.no_extra:
+        extra["ensemble"] = ensemble_extra()
+    out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    out["extra"] = extra
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -105,6 +105,7 @@ SIGNATURES = {
     "gpgmp_solver_run": [_P, _I, _U, _pF],
     "gpgmp_solver_run_continue": [_P, _I, _I, _pF],
     "gpgmp_solver_set_launch": [_P, _I, _I],
+    "gpgmp_solver_kernel_times": [_P, _I, _pD, C.POINTER(C.c_int64), _pD],
     "gpgmp_solver_synchronize": [_P],
     "gpgmp_batch_create": [_P, _I, _pI, _pI, _pU, _P, C.POINTER(_P)],
     "gpgmp_batch_destroy": [_P],
@@ -114,6 +115,7 @@ SIGNATURES = {
     "gpgmp_batch_autotune": [_P, _I, _pI, _pI],
     "gpgmp_batch_run": [_P, _I, _I],
     "gpgmp_batch_reset": [_P],
+    "gpgmp_batch_kernel_times": [_P, _I, _pD, C.POINTER(C.c_int64), _pD],
     "gpgmp_batch_total_cells": [_P, C.POINTER(C.c_int64)],
     "gpgmp_batch_synchronize": [_P],
     "gpgmp_min_timestep_device": [_P, _P, _I, _F, _pF, _P],

@@ -317,6 +317,15 @@ int gpgmp_solver_set_launch(gpgmp_solver *s, int32_t block, int32_t cpt) {
     });
 }
 
+int gpgmp_solver_kernel_times(gpgmp_solver *s, int32_t on, double *ms, int64_t *n, double *bytes) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(s->runner != nullptr, GPGMP_E_STATE, "no run");
+        if (ms && n) s->runner->kernel_times(ms, n, bytes);
+        s->runner->set_timing(on != 0);
+    });
+}
+
 int gpgmp_solver_synchronize(gpgmp_solver *s) {
     return guarded([&] {
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
@@ -442,6 +451,15 @@ int gpgmp_batch_autotune(gpgmp_batch *b, int32_t trial_steps, int32_t *block_out
         GPGMP_HIP_CHECK(hipStreamSynchronize(st));
         if (block_out) *block_out = bb;
         if (cpt_out) *cpt_out = bc;
+    });
+}
+
+int gpgmp_batch_kernel_times(gpgmp_batch *b, int32_t on, double *ms, int64_t *n, double *bytes) {
+    return guarded([&] {
+        GPGMP_REQUIRE(b, GPGMP_E_INVALID, "null argument");
+        if (!b->runner) b->runner.reset(new GmpRunner(*b->eng, b->dtx, b->dty, b->seed0s));
+        if (ms && n) b->runner->kernel_times(ms, n, bytes);
+        b->runner->set_timing(on != 0);
     });
 }
 

@@ -295,7 +295,8 @@ void GmpEngine::set_probs(int r, int s, int axis, const float *D, const float *r
 }
 
 void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cur,
-                       const int32_t *seeds_prev, const int32_t *lost_in, int32_t *lost_out) {
+                       const int32_t *seeds_prev, const int32_t *lost_in, int32_t *lost_out,
+                       hipEvent_t ev_start, hipEvent_t ev_stop) {
     StepLaunch a{};
     a.mode = mode;
     a.state = state_.get();
@@ -314,6 +315,8 @@ void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cu
     a.block = block_;
     a.cpt = cpt_;
     a.stream = stream_;
+    a.ev_start = ev_start;
+    a.ev_stop = ev_stop;
     GPGMP_HIP_CHECK(launch_gmp_step(a));
 }
 
@@ -356,6 +359,55 @@ GmpRunner::GmpRunner(GmpEngine &eng, const std::vector<float> &dtx, const std::v
 GmpRunner::~GmpRunner() {
     (void)hipStreamSynchronize(eng_.stream());
     for (Chunk *c : chunks_) delete c;
+    for (auto &e : ev_pool_) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+}
+
+// Algorithmic HBM bytes of one launch: counts read and written (all species when the
+// SSA runs, else the touched ones), ND outflow words read per gathered species, ND
+// written per species with outflow, plus the (p1,p2) table of inhomogeneous sweeps.
+static double launch_bytes(const GmpEngine &e, int mode, const LaunchPlan &p) {
+    const int S = e.S();
+    const int nd = (mode == 0) ? 4 : 2;
+    const int ng = __builtin_popcount(p.gather_mask), no = __builtin_popcount(p.out_mask);
+    const bool ssa = (mode != 2) && p.deltat > 0.f && e.table_.R > 0;
+    const int nstate = ssa ? S : __builtin_popcount(p.gather_mask | p.out_mask);
+    const double per_cell = 8.0 * nstate + 4.0 * nd * ng + 4.0 * nd * no + (mode != 0 ? 8.0 * no : 0.0);
+    return per_cell * (double)e.total_cells();
+}
+
+void GmpRunner::timed_launch(int mode, const LaunchPlan *plan, const LaunchPlan &hplan, const int32_t *scur,
+                             const int32_t *sprev, const int32_t *lin, int32_t *lout) {
+    if (!timing_) {
+        eng_.launch(mode, plan, scur, sprev, lin, lout);
+        return;
+    }
+    if (ev_used_ == ev_pool_.size()) {
+        hipEvent_t a, b;
+        GPGMP_HIP_CHECK(hipEventCreate(&a));
+        GPGMP_HIP_CHECK(hipEventCreate(&b));
+        ev_pool_.emplace_back(a, b);
+        ev_mode_.push_back(0);
+    }
+    auto &e = ev_pool_[ev_used_];
+    ev_mode_[ev_used_] = mode;
+    kt_bytes_[mode] += launch_bytes(eng_, mode, hplan);
+    ++ev_used_;
+    eng_.launch(mode, plan, scur, sprev, lin, lout, e.first, e.second);
+}
+
+void GmpRunner::collect_times() {
+    if (ev_used_ == 0) return;
+    GPGMP_HIP_CHECK(hipStreamSynchronize(eng_.stream()));
+    for (size_t i = 0; i < ev_used_; ++i) {
+        float ms = 0.f;
+        GPGMP_HIP_CHECK(hipEventElapsedTime(&ms, ev_pool_[i].first, ev_pool_[i].second));
+        kt_ms_[ev_mode_[i]] += ms;
+        kt_n_[ev_mode_[i]] += 1;
+    }
+    ev_used_ = 0;
 }
 
 // Issues the launches of one chunk whose plans/seeds are already in its host buffers.
@@ -373,10 +425,11 @@ void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
             const int32_t *sprev = c.dseed.get() + (size_t)j * nreal;
             if (!eng_.inh_) {
                 const int p = (parity_ + j) & 1;
-                eng_.launch(0, c.dplan.get() + j, scur, sprev, eng_.lost(p ^ 1), eng_.lost(p));
+                timed_launch(0, c.dplan.get() + j, c.hplan[j], scur, sprev, eng_.lost(p ^ 1), eng_.lost(p));
             } else {
-                eng_.launch(1, c.dplan.get() + 2 * j, scur, sprev, eng_.lost(1), eng_.lost(0));
-                eng_.launch(2, c.dplan.get() + 2 * j + 1, scur, sprev, eng_.lost(0), eng_.lost(1));
+                timed_launch(1, c.dplan.get() + 2 * j, c.hplan[2 * j], scur, sprev, eng_.lost(1), eng_.lost(0));
+                timed_launch(2, c.dplan.get() + 2 * j + 1, c.hplan[2 * j + 1], scur, sprev, eng_.lost(0),
+                             eng_.lost(1));
             }
         }
     };
@@ -384,6 +437,13 @@ void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
         body();
         return;
     }
+    const bool saved_timing = timing_;
+    timing_ = false;  // events are not captured into graphs
+    struct TimingRestore {
+        bool &t;
+        bool v;
+        ~TimingRestore() { t = v; }
+    } timing_restore{timing_, saved_timing};
     if (!c.graph || c.graph_steps != nsteps) {
         if (c.graph) {
             GPGMP_HIP_CHECK(hipGraphExecDestroy(c.graph));
@@ -509,6 +569,7 @@ void GmpRunner::run(int steps, int graph_chunk) {
             eng_.launch(1, c.dplan.get(), scur, sprev, eng_.lost(1), eng_.lost(0));
         has_prev_ = false;
     }
+    if (timing_) collect_times();
 }
 
 }  // namespace gpgmp

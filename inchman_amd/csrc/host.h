@@ -165,7 +165,8 @@ class GmpEngine {
 
     // one launch (all device pointers)
     void launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cur, const int32_t *seeds_prev,
-                const int32_t *lost_in, int32_t *lost_out);
+                const int32_t *lost_in, int32_t *lost_out, hipEvent_t ev_start = nullptr,
+                hipEvent_t ev_stop = nullptr);
 
     int32_t *lost(int i) const { return lost_[i].get(); }
 
@@ -198,6 +199,16 @@ class GmpRunner {
     ~GmpRunner();
     void run(int steps, int graph_chunk);
     float sim_time() const { return sched_.sim_time(); }
+    // Per-launch HIP-event timing of the eager (non-graph) launches, by kernel mode
+    // (0 homogeneous, 1 SSA + X sweeps, 2 Y sweeps); accumulated over run() calls.
+    void set_timing(bool on) { timing_ = on; }
+    void kernel_times(double *ms, int64_t *n, double *bytes) const {
+        for (int i = 0; i < 3; ++i) {
+            ms[i] = kt_ms_[i];
+            n[i] = kt_n_[i];
+            if (bytes) bytes[i] = kt_bytes_[i];
+        }
+    }
     std::vector<float> next_diffusion_time() const { return sched_.next_diffusion_time(); }
 
   private:
@@ -211,6 +222,16 @@ class GmpRunner {
     std::vector<int32_t> prev_seed_;
     int parity_ = 0;
     std::vector<Chunk *> chunks_;
+    void timed_launch(int mode, const LaunchPlan *plan, const LaunchPlan &hplan, const int32_t *scur,
+                      const int32_t *sprev, const int32_t *lin, int32_t *lout);
+    void collect_times();
+    bool timing_ = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
+    std::vector<int> ev_mode_;
+    size_t ev_used_ = 0;
+    double kt_ms_[3] = {0, 0, 0};
+    int64_t kt_n_[3] = {0, 0, 0};
+    double kt_bytes_[3] = {0, 0, 0};  // algorithmic HBM bytes of the timed launches
     int chunk_cap_ = 0;
     int graph_steps_ = 0;
 };

@@ -16,6 +16,7 @@
 // updateState launches: a host-loop iteration is one launch (homogeneous) or two
 // (inhomogeneous: X sweeps, then Y sweeps).  RNG keys never contain a batch or
 // tile offset, so results do not depend on the launch shape or batch layout.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "gpgmp_numeric.h"
@@ -304,9 +305,16 @@ namespace gpgmp {
 template <int S, int MODE>
 static hipError_t launch_s(const StepLaunch &a) {
     const int grid = 8 * ((a.ntiles + 7) / 8);
-    hipLaunchKernelGGL((k_gmp_step<S, MODE>), dim3(grid), dim3(a.block), 0, a.stream, a.state,
-                       a.lost_in, a.lost_out, a.prob, a.reals, a.tiles, a.ntiles, a.plan, a.seeds_cur,
-                       a.seeds_prev, *a.T, *a.P, a.bm, a.cpt);
+    if (a.ev_start) {
+        // start/stop timestamps taken by the dispatch packet itself
+        hipExtLaunchKernelGGL((k_gmp_step<S, MODE>), dim3(grid), dim3(a.block), 0, a.stream, a.ev_start,
+                              a.ev_stop, 0, a.state, a.lost_in, a.lost_out, a.prob, a.reals, a.tiles,
+                              a.ntiles, a.plan, a.seeds_cur, a.seeds_prev, *a.T, *a.P, a.bm, a.cpt);
+    } else {
+        hipLaunchKernelGGL((k_gmp_step<S, MODE>), dim3(grid), dim3(a.block), 0, a.stream, a.state,
+                           a.lost_in, a.lost_out, a.prob, a.reals, a.tiles, a.ntiles, a.plan, a.seeds_cur,
+                           a.seeds_prev, *a.T, *a.P, a.bm, a.cpt);
+    }
     return hipGetLastError();
 }
 

@@ -25,6 +25,8 @@ struct StepLaunch {
     int block;              // threads per block (64..256, multiple of 64)
     int cpt;                // cells per thread
     hipStream_t stream;
+    hipEvent_t ev_start = nullptr;  // optional: timestamps of this dispatch
+    hipEvent_t ev_stop = nullptr;
 };
 
 hipError_t launch_gmp_step(const StepLaunch &a);

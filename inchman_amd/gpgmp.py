@@ -254,6 +254,16 @@ class Solver:
     def setLaunch(self, block, cells_per_thread):
         call("gpgmp_solver_set_launch", self.handle, int(block), int(cells_per_thread))
 
+    def kernelTimes(self, on=True):
+        """Accumulated HIP-event times (ms, launches) per kernel mode of the current
+        run, then switches per-launch timing on/off."""
+        ms = np.zeros(3, np.float64)
+        n = np.zeros(3, np.int64)
+        b = np.zeros(3, np.float64)
+        call("gpgmp_solver_kernel_times", self.handle, int(on), ms.ctypes.data_as(C.POINTER(C.c_double)),
+             n.ctypes.data_as(C.POINTER(C.c_int64)), b.ctypes.data_as(C.POINTER(C.c_double)))
+        return ms, n, b
+
     def synchronize(self):
         call("gpgmp_solver_synchronize", self.handle)
 
@@ -321,6 +331,14 @@ class Ensemble:
 
     def reset(self):
         call("gpgmp_batch_reset", self.handle)
+
+    def kernelTimes(self, on=True):
+        ms = np.zeros(3, np.float64)
+        n = np.zeros(3, np.int64)
+        b = np.zeros(3, np.float64)
+        call("gpgmp_batch_kernel_times", self.handle, int(on), ms.ctypes.data_as(C.POINTER(C.c_double)),
+             n.ctypes.data_as(C.POINTER(C.c_int64)), b.ctypes.data_as(C.POINTER(C.c_double)))
+        return ms, n, b
 
     def totalCells(self):
         n = C.c_int64()

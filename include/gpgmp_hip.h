@@ -123,6 +123,10 @@ int gpgmp_solver_run(gpgmp_solver *s, int32_t steps, uint32_t seed0, float *sim_
 int gpgmp_solver_run_continue(gpgmp_solver *s, int32_t steps, int32_t graph_chunk, float *sim_time_out);
 /* launch shape of the solver's kernels: threads per block (64..256), cells per thread */
 int gpgmp_solver_set_launch(gpgmp_solver *s, int32_t block, int32_t cells_per_thread);
+/* Reads the accumulated per-launch HIP-event times of the current run's eager launches
+ * (ms[3], n[3], algorithmic HBM bytes[3] by kernel: 0 homogeneous, 1 SSA + X sweeps,
+ * 2 Y sweeps; may be NULL), then switches the timing on or off for the following launches. */
+int gpgmp_solver_kernel_times(gpgmp_solver *s, int32_t on, double *ms, int64_t *n, double *bytes);
 int gpgmp_solver_synchronize(gpgmp_solver *s);
 
 /* ---- batched ensemble (no reference counterpart) ----------------------------------- */
@@ -146,6 +150,7 @@ int gpgmp_batch_autotune(gpgmp_batch *b, int32_t trial_steps, int32_t *block_out
 int gpgmp_batch_run(gpgmp_batch *b, int32_t steps, int32_t graph_chunk);
 /* Restarts the host loops at time 0 with the realizations' seed0 (state is kept). */
 int gpgmp_batch_reset(gpgmp_batch *b);
+int gpgmp_batch_kernel_times(gpgmp_batch *b, int32_t on, double *ms, int64_t *n, double *bytes);
 int gpgmp_batch_total_cells(gpgmp_batch *b, int64_t *cells);
 int gpgmp_batch_synchronize(gpgmp_batch *b);
 
