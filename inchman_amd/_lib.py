@@ -121,10 +121,12 @@ SIGNATURES = {
     "gpgmp_min_timestep_device": [_P, _P, _I, _F, _pF, _P],
     "gpgmp_rdd_create": [C.POINTER(RddParams), _I, _I, _P, C.POINTER(_P)],
     "gpgmp_rdd_destroy": [_P],
+    "gpgmp_rdd_set_rates": [_P, _pF],
     "gpgmp_rdd_forward": [_P, _P, _P, _P, _I],
     "gpgmp_rdd_backward": [_P, _P, _P, _P, _P, _P],
-    "gpgmp_det_hom_diffuse_device": [_P, _I, _I, _I, _pF, _F, _F, _P],
-    "gpgmp_det_inhom_sweep_device": [_P, _I, _I, _I, _I, _F, _P, _P, _F, _P],
+    "gpgmp_det_hom_diffuse_device": [_P, _P, _I, _I, _I, _pF, _F, _F, _P],
+    "gpgmp_det_inhom_sweep_device": [_P, _P, _I, _I, _I, _I, _F, _P, _P, _F, _P],
+    "gpgmp_det_rk4_device": [_P, _P, _F, _P],
     "gpgmp_sort_pairs_device": [_P, _P, _P, _P, _I, _I, _P],
     "gpgmp_cell_offsets_device": [_P, _I, _I, _P, _P],
     "gpgmp_cell_left_right_device": [_P, _I, _I, _P, _P, _P],

@@ -96,3 +96,30 @@ def ensemble_seeds(n=512, seed=0xBA7C):
 
 def ensemble_state(side, r, seed=12345):
     return plain_gmp_state(int(side), seed=seed + 7919 * r)
+
+
+# ---- differentiable reaction-diffusion (1024x1024, 2000 steps) -------------------------
+RDD_D0 = (0.05, 1.0)
+
+
+def rdd_problem(width=1024, height=None, steps=2000, seed=7, target_seed=11):
+    """Schnakenberg (k1..k4 of the stochastic configuration) in concentration form on a
+    seeded log-normal diffusivity field: D = exp(0.3 g), g a unit-variance Gaussian
+    random field with correlation length 8 cells (white noise smoothed by a Gaussian
+    of sigma 8, rescaled).  dt = 0.2 h^2 / (4 max_s D0_s max D).  Initial u, v near
+    200 / 75; the target is a seeded field of the magnitude the run reaches."""
+    from scipy.ndimage import gaussian_filter
+
+    height = width if height is None else height
+    rng = np.random.default_rng(seed)
+    g = gaussian_filter(rng.standard_normal((height, width)), 8.0, mode="reflect")
+    g = (g - g.mean()) / g.std()
+    D = np.exp(0.3 * g).astype(np.float32)
+    dt = float(np.float32(0.2 / (4.0 * max(RDD_D0) * float(D.max()))))
+    u0 = (200.0 + np.sqrt(200.0) * rng.standard_normal((height, width))).astype(np.float32)
+    v0 = (75.0 + np.sqrt(75.0) * rng.standard_normal((height, width))).astype(np.float32)
+    trng = np.random.default_rng(target_seed)
+    ut = (1.0 + 0.1 * trng.standard_normal((height, width))).astype(np.float32)
+    vt = (60.0 + 5.0 * trng.standard_normal((height, width))).astype(np.float32)
+    return {"D": D, "u0": u0, "v0": v0, "ut": ut, "vt": vt, "dt": dt, "steps": steps,
+            "k": SCHNAKENBERG_K, "D0": RDD_D0, "h": 1.0}

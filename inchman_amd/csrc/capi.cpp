@@ -54,10 +54,6 @@ struct StreamHolder {
 };
 }  // namespace
 
-struct gpgmp_model {
-    Model m;
-};
-
 // Plain solver: one realization driven through the gpgmp::Solver-shaped calls, or
 // through whole host loops (run).
 struct gpgmp_solver {
@@ -79,6 +75,8 @@ struct gpgmp_batch {
     std::vector<uint32_t> seed0s;
     std::unique_ptr<GmpRunner> runner;
 };
+
+void gpgmp_set_last_error(const char *msg) { g_last_error = msg ? msg : ""; }
 
 extern "C" {
 

@@ -1,0 +1,31 @@
+// Host-side launch interface of the deterministic kernels.
+#pragma once
+
+#include <hip/hip_ext.h>
+#include <hip/hip_runtime.h>
+
+#include "stochastic_types.h"
+
+namespace gpgmp {
+
+struct RddConsts {
+    float dt, cu, cv;  // c_s = D0_s / h^2
+    float k[4];
+};
+
+hipError_t launch_det_hom_diffuse(const float *in, float *out, int W, int H, float coef, hipStream_t s);
+hipError_t launch_det_inhom_sweep(const float *in, float *out, const float *D, const float *r, int W, int H,
+                                  int axis, float dt_h2, float dt_2h, hipStream_t s);
+hipError_t launch_det_rk4(float *state, int A, const RxTable &T, float dt, hipStream_t s);
+hipError_t launch_rdd_step(const float *u, const float *v, const float *D, float *un, float *vn, int W, int H,
+                           const RddConsts &c, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+hipError_t launch_rdd_adjoint(const float *lu, const float *lv, const float *u, const float *v, const float *D,
+                              float *lun, float *lvn, float *gD, double *gk_tiles, int W, int H,
+                              const RddConsts &c, hipStream_t s, hipEvent_t e0 = nullptr,
+                              hipEvent_t e1 = nullptr);
+hipError_t launch_rdd_gk_reduce(const double *gk_tiles, int ntiles, float *gk, hipStream_t s);
+hipError_t launch_fill_f32(float *p, size_t n, float v, hipStream_t s);
+hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t s);
+int rdd_num_tiles(int W, int H);
+
+}  // namespace gpgmp

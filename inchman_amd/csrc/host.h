@@ -237,3 +237,11 @@ class GmpRunner {
 };
 
 }  // namespace gpgmp
+
+// Opaque handles of the C ABI that more than one translation unit needs.
+struct gpgmp_model {
+    gpgmp::Model m;
+};
+
+// Sets the calling thread's gpgmp_last_error() message.
+void gpgmp_set_last_error(const char *msg);

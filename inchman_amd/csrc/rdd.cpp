@@ -1,0 +1,195 @@
+// Deterministic entry points: the reference's stencils/RK4 on device buffers, and the
+// differentiable Schnakenberg reaction-diffusion solver (forward with checkpoints,
+// adjoint with segment recomputation).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+
+#include "deterministic_launch.h"
+#include "host.h"
+
+using namespace gpgmp;
+
+template <typename F>
+static int guarded_det(F &&f) {
+    try {
+        f();
+        return GPGMP_OK;
+    } catch (const Error &e) {
+        gpgmp_set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        gpgmp_set_last_error("host allocation failed");
+        return GPGMP_E_CAPACITY;
+    } catch (const std::exception &e) {
+        gpgmp_set_last_error(e.what());
+        return GPGMP_E_INVALID;
+    }
+}
+
+struct gpgmp_rdd {
+    gpgmp_rdd_params p{};
+    int steps = 0, ckpt = 0, nck = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    size_t A = 0;
+    DevBuf<float> ck;        // nck checkpoints of (u, v)
+    DevBuf<float> seg;       // ckpt recomputed states of one segment
+    DevBuf<float> tmp;       // ping-pong partner of the caller's buffers
+    DevBuf<double> gk_tiles; // per-tile dL/dk accumulators
+    int ntiles = 0;
+    bool saved = false;
+    RddConsts c{};
+    ~gpgmp_rdd() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+static RddConsts rdd_consts(const gpgmp_rdd_params &p) {
+    RddConsts c;
+    c.dt = p.dt;
+    c.cu = p.D0u / (p.h * p.h);
+    c.cv = p.D0v / (p.h * p.h);
+    for (int i = 0; i < 4; ++i) c.k[i] = p.k[i];
+    return c;
+}
+
+extern "C" {
+
+int gpgmp_det_hom_diffuse_device(const float *in, float *out, int32_t W, int32_t H, int32_t S, const float *D_host,
+                                 float dt, float h, void *stream) {
+    return guarded_det([&] {
+        GPGMP_REQUIRE(in && out && D_host && in != out, GPGMP_E_INVALID, "bad buffers (Jacobi needs in != out)");
+        GPGMP_REQUIRE(W >= 2 && H >= 2 && S >= 1, GPGMP_E_INVALID, "bad shape");
+        const size_t A = (size_t)W * H;
+        for (int s = 0; s < S; ++s)
+            GPGMP_HIP_CHECK(launch_det_hom_diffuse(in + s * A, out + s * A, W, H, dt * D_host[s] / (h * h),
+                                                   (hipStream_t)stream));
+    });
+}
+
+int gpgmp_det_inhom_sweep_device(const float *in, float *out, int32_t W, int32_t H, int32_t species, int32_t axis,
+                                 float dt, const float *D_axis, const float *r_axis, float h, void *stream) {
+    return guarded_det([&] {
+        GPGMP_REQUIRE(in && out && D_axis && r_axis && in != out, GPGMP_E_INVALID, "bad buffers");
+        GPGMP_REQUIRE(W >= 2 && H >= 2 && species >= 0 && (axis == 0 || axis == 1), GPGMP_E_INVALID, "bad shape");
+        const size_t o = (size_t)species * W * H;
+        GPGMP_HIP_CHECK(launch_det_inhom_sweep(in + o, out + o, D_axis + o, r_axis + o, W, H, axis, dt / (h * h),
+                                               dt / (2.f * h), (hipStream_t)stream));
+    });
+}
+
+int gpgmp_det_rk4_device(const gpgmp_model *m, float *state, float dt, void *stream) {
+    return guarded_det([&] {
+        GPGMP_REQUIRE(m && state, GPGMP_E_INVALID, "null argument");
+        const int A = m->m.desc.width * m->m.desc.height;
+        if (m->m.table.R == 0) return;
+        GPGMP_HIP_CHECK(launch_det_rk4(state, A, m->m.table, dt, (hipStream_t)stream));
+    });
+}
+
+int gpgmp_rdd_create(const gpgmp_rdd_params *p, int32_t steps, int32_t ckpt_every, void *stream, gpgmp_rdd **out) {
+    return guarded_det([&] {
+        GPGMP_REQUIRE(p && out, GPGMP_E_INVALID, "null argument");
+        *out = nullptr;
+        GPGMP_REQUIRE(p->width >= 2 && p->height >= 2, GPGMP_E_INVALID, "grid sides must be >= 2");
+        GPGMP_REQUIRE(steps >= 0 && ckpt_every >= 1, GPGMP_E_INVALID, "bad steps / checkpoint interval");
+        GPGMP_REQUIRE(p->h > 0.f && p->dt >= 0.f, GPGMP_E_INVALID, "bad h / dt");
+        std::unique_ptr<gpgmp_rdd> r(new gpgmp_rdd());
+        r->p = *p;
+        r->steps = steps;
+        r->ckpt = std::min(ckpt_every, std::max(steps, 1));
+        r->nck = steps > 0 ? (steps + r->ckpt - 1) / r->ckpt : 0;
+        r->A = (size_t)p->width * p->height;
+        if (stream) {
+            r->stream = (hipStream_t)stream;
+        } else {
+            GPGMP_HIP_CHECK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+            r->own_stream = true;
+        }
+        r->ck.alloc((size_t)std::max(r->nck, 1) * 2 * r->A);
+        r->seg.alloc((size_t)r->ckpt * 2 * r->A);
+        r->tmp.alloc(2 * r->A);
+        r->ntiles = rdd_num_tiles(p->width, p->height);
+        r->gk_tiles.alloc((size_t)r->ntiles * 4);
+        r->c = rdd_consts(*p);
+        *out = r.release();
+    });
+}
+
+int gpgmp_rdd_destroy(gpgmp_rdd *r) {
+    delete r;
+    return GPGMP_OK;
+}
+
+int gpgmp_rdd_set_rates(gpgmp_rdd *r, const float *k) {
+    return guarded_det([&] {
+        GPGMP_REQUIRE(r && k, GPGMP_E_INVALID, "null argument");
+        for (int i = 0; i < 4; ++i) r->p.k[i] = k[i];
+        r->c = rdd_consts(r->p);
+        r->saved = false;
+    });
+}
+
+int gpgmp_rdd_forward(gpgmp_rdd *r, const float *D, float *u, float *v, int32_t save) {
+    return guarded_det([&] {
+        GPGMP_REQUIRE(r && D && u && v, GPGMP_E_INVALID, "null argument");
+        const int W = r->p.width, H = r->p.height;
+        const size_t A = r->A;
+        float *bu[2] = {u, r->tmp.get()};
+        float *bv[2] = {v, r->tmp.get() + A};
+        int cur = 0;
+        for (int n = 0; n < r->steps; ++n) {
+            if (save && n % r->ckpt == 0) {
+                float *c = r->ck.get() + (size_t)(n / r->ckpt) * 2 * A;
+                GPGMP_HIP_CHECK(hipMemcpyAsync(c, bu[cur], A * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
+                GPGMP_HIP_CHECK(hipMemcpyAsync(c + A, bv[cur], A * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
+            }
+            GPGMP_HIP_CHECK(launch_rdd_step(bu[cur], bv[cur], D, bu[cur ^ 1], bv[cur ^ 1], W, H, r->c, r->stream));
+            cur ^= 1;
+        }
+        if (cur) {
+            GPGMP_HIP_CHECK(hipMemcpyAsync(u, bu[1], A * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
+            GPGMP_HIP_CHECK(hipMemcpyAsync(v, bv[1], A * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
+        }
+        r->saved = save != 0;
+    });
+}
+
+int gpgmp_rdd_backward(gpgmp_rdd *r, const float *D, float *lu, float *lv, float *gD, float *gk) {
+    return guarded_det([&] {
+        GPGMP_REQUIRE(r && D && lu && lv && gD && gk, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(r->saved, GPGMP_E_STATE, "backward needs a forward run with save != 0");
+        const int W = r->p.width, H = r->p.height;
+        const size_t A = r->A;
+        GPGMP_HIP_CHECK(launch_fill_f32(gD, A, 0.f, r->stream));
+        GPGMP_HIP_CHECK(launch_fill_f64(r->gk_tiles.get(), (size_t)r->ntiles * 4, 0.0, r->stream));
+        float *blu[2] = {lu, r->tmp.get()};
+        float *blv[2] = {lv, r->tmp.get() + A};
+        int cur = 0;
+        for (int c = r->nck - 1; c >= 0; --c) {
+            const int n0 = c * r->ckpt, n1 = std::min(r->steps, n0 + r->ckpt);
+            const float *cku = r->ck.get() + (size_t)c * 2 * A;
+            auto su = [&](int n) -> float * {
+                return (n == n0) ? const_cast<float *>(cku) : r->seg.get() + (size_t)(n - n0) * 2 * A;
+            };
+            for (int n = n0 + 1; n < n1; ++n)
+                GPGMP_HIP_CHECK(launch_rdd_step(su(n - 1), su(n - 1) + A, D, su(n), su(n) + A, W, H, r->c, r->stream));
+            for (int n = n1 - 1; n >= n0; --n) {
+                GPGMP_HIP_CHECK(launch_rdd_adjoint(blu[cur], blv[cur], su(n), su(n) + A, D, blu[cur ^ 1], blv[cur ^ 1],
+                                                   gD, r->gk_tiles.get(), W, H, r->c, r->stream));
+                cur ^= 1;
+            }
+        }
+        if (cur) {
+            GPGMP_HIP_CHECK(hipMemcpyAsync(lu, blu[1], A * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
+            GPGMP_HIP_CHECK(hipMemcpyAsync(lv, blv[1], A * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
+        }
+        GPGMP_HIP_CHECK(launch_rdd_gk_reduce(r->gk_tiles.get(), r->ntiles, gk, r->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+"""Differentiable Schnakenberg reaction-diffusion on the GPU (torch.autograd.Function).
+
+Forward: `steps` fused explicit Euler steps (reaction + mirror-BC diffusion with the
+per-cell diffusivity field D scaled by D0_u, D0_v) in fp32, storing the state every
+`ckpt_every` steps.  Backward: the hand-written adjoint kernel walks the steps in
+reverse, recomputing each checkpoint segment, and returns dL/dD, dL/dk1..k4 and
+dL/du0, dL/dv0.  Everything runs in libgpgmp_hip.so on torch's current stream.
+"""
+import ctypes as C
+
+import torch
+
+from . import _lib
+from ._lib import call, lib
+
+
+def _stream_handle():
+    return int(torch.cuda.current_stream().cuda_stream)
+
+
+class ReactionDiffusion:
+    """Handle of one problem size: grid, step count, checkpoint interval, dt, D0."""
+
+    def __init__(self, width, height, steps, dt, h=1.0, D0=(0.05, 1.0), k=(1.0, 3.0, 1e-4, 1.0),
+                 ckpt_every=100):
+        self.width, self.height, self.steps = int(width), int(height), int(steps)
+        p = _lib.RddParams()
+        p.width, p.height = self.width, self.height
+        p.h, p.dt = float(h), float(dt)
+        p.D0u, p.D0v = float(D0[0]), float(D0[1])
+        for i in range(4):
+            p.k[i] = float(k[i])
+        self.params = p
+        hnd = C.c_void_p()
+        call("gpgmp_rdd_create", C.byref(p), self.steps, int(ckpt_every), _stream_handle(), C.byref(hnd))
+        self.handle = hnd
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().gpgmp_rdd_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, *ts):
+        for t in ts:
+            assert t.is_cuda and t.dtype == torch.float32 and t.is_contiguous(), "fp32 contiguous CUDA tensors"
+            assert t.numel() == self.width * self.height
+
+    def set_rates(self, k):
+        kk = (C.c_float * 4)(*[float(x) for x in k])
+        call("gpgmp_rdd_set_rates", self.handle, kk)
+
+    def forward(self, D, u, v, save=True):
+        """Advances u, v (in place) by `steps` steps."""
+        self._check(D, u, v)
+        call("gpgmp_rdd_forward", self.handle, D.data_ptr(), u.data_ptr(), v.data_ptr(), int(save))
+        return u, v
+
+    def backward(self, D, lu, lv):
+        """Adjoint of the last saved forward; lu, lv (in place): dL/du_T -> dL/du_0."""
+        self._check(D, lu, lv)
+        gD = torch.empty_like(D)
+        gk = torch.empty(4, dtype=torch.float32, device=D.device)
+        call("gpgmp_rdd_backward", self.handle, D.data_ptr(), lu.data_ptr(), lv.data_ptr(), gD.data_ptr(),
+             gk.data_ptr())
+        return gD, gk, lu, lv
+
+
+class RDDFunction(torch.autograd.Function):
+    """(D, k, u0, v0) -> (u_T, v_T); k is a 4-element tensor of rate constants."""
+
+    @staticmethod
+    def forward(ctx, D, k, u0, v0, solver):
+        solver.set_rates(k.detach().cpu().tolist())
+        Dc = D.detach().contiguous()
+        u = u0.detach().clone().contiguous()
+        v = v0.detach().clone().contiguous()
+        solver.forward(Dc, u, v, save=True)
+        ctx.solver = solver
+        ctx.save_for_backward(Dc, k)
+        return u, v
+
+    @staticmethod
+    def backward(ctx, gu, gv):
+        Dc, k = ctx.saved_tensors
+        lu = (gu if gu is not None else torch.zeros_like(Dc)).detach().clone().contiguous()
+        lv = (gv if gv is not None else torch.zeros_like(Dc)).detach().clone().contiguous()
+        gD, gk, gu0, gv0 = ctx.solver.backward(Dc, lu, lv)
+        return gD, gk.to(k.device, k.dtype), gu0, gv0, None
+
+
+def simulate(solver, D, k, u0, v0):
+    return RDDFunction.apply(D, k, u0, v0, solver)

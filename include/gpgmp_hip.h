@@ -171,19 +171,26 @@ typedef struct {
 int gpgmp_rdd_create(const gpgmp_rdd_params *p, int32_t steps, int32_t ckpt_every, void *stream,
                      gpgmp_rdd **out);
 int gpgmp_rdd_destroy(gpgmp_rdd *r);
+/* Replaces the rate constants k1..k4 (host floats) for the following forward runs. */
+int gpgmp_rdd_set_rates(gpgmp_rdd *r, const float *k);
 /* Forward: u,v (device, in/out) advanced `steps` fused Euler steps with field D; stores
  * checkpoints when save != 0. */
 int gpgmp_rdd_forward(gpgmp_rdd *r, const float *D, float *u, float *v, int32_t save);
-/* Adjoint of the last saved forward: lambda_u/lambda_v (device, in: dL/du_T, dL/dv_T;
- * clobbered), outputs gD (device, [H][W]) and gk (device, 4 floats). */
+/* Adjoint of the last saved forward: lu/lv (device) carry dL/du_T, dL/dv_T in and
+ * dL/du_0, dL/dv_0 out; outputs gD = dL/dD (device, [H][W]) and gk = dL/dk1..k4
+ * (device, 4 floats). */
 int gpgmp_rdd_backward(gpgmp_rdd *r, const float *D, float *lu, float *lv, float *gD, float *gk);
 
-/* Deterministic stencils of Deterministic.cl on device fp32 buffers (Jacobi). */
-int gpgmp_det_hom_diffuse_device(float *state, int32_t W, int32_t H, int32_t S, const float *D_host,
-                                 float dt, float h, void *stream);
-int gpgmp_det_inhom_sweep_device(float *state, int32_t W, int32_t H, int32_t species, int32_t axis,
-                                 float dt, const float *D_axis, const float *r_axis, float h,
-                                 void *stream);
+/* Deterministic stencils and RK4 of Deterministic.cl on device fp32 buffers, Jacobi
+ * (in != out): homogeneous_deterministic_diffuse (Deterministic.cl:17-65) for every
+ * species, inhomogeneous_deterministic_diffuseX/Y (:71-136) for one species (only that
+ * species' slice of `out` is written), deterministic_performReaction_rk4 (:179-238). */
+int gpgmp_det_hom_diffuse_device(const float *in, float *out, int32_t W, int32_t H, int32_t S,
+                                 const float *D_host, float dt, float h, void *stream);
+int gpgmp_det_inhom_sweep_device(const float *in, float *out, int32_t W, int32_t H, int32_t species,
+                                 int32_t axis, float dt, const float *D_axis, const float *r_axis,
+                                 float h, void *stream);
+int gpgmp_det_rk4_device(const gpgmp_model *m, float *state, float dt, void *stream);
 
 /* ---- individual particles (IndividualSolver sort / cell indices) ------------------- */
 /* Stable LSD radix sort of n (key, value) pairs by the 32-bit key, device pointers;

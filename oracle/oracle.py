@@ -238,6 +238,11 @@ def rdd_forward(p, D, u, v, steps, nthreads=0):
     return u, v
 
 
+def rdd_loss(u, v, ut, vt):
+    arr = [np.ascontiguousarray(a, np.float64) for a in (u, v, ut, vt)]
+    return lib().oracle_rdd_loss(arr[0].size, *[_p(a) for a in arr])
+
+
 def rdd_grad(p, D, u0, v0, ut, vt, steps, ckpt_every, nthreads=0):
     arr = [np.ascontiguousarray(a, np.float64) for a in (D, u0, v0, ut, vt)]
     gD = np.zeros_like(arr[0])
