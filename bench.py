@@ -108,6 +108,71 @@ def ensemble_extra(steps=200):
             "graph_chunk": 50}
 
 
+def rdd_extra(reps=3):
+    """Differentiable reaction-diffusion, 1024x1024, 2000 steps, checkpoints every 100:
+    forward, adjoint (segment recompute + reverse sweep) and grad/s."""
+    import torch
+    from inchman_amd import configs
+    from inchman_amd.rdd import ReactionDiffusion
+
+    prob = configs.rdd_problem(1024)
+    W = H = 1024
+    steps = prob["steps"]
+    r = ReactionDiffusion(W, H, steps, prob["dt"], prob["h"], prob["D0"], prob["k"], 100)
+    dev = lambda a: torch.from_numpy(a).cuda()  # noqa: E731
+    D, u0, v0, ut, vt = (dev(prob[k]) for k in ("D", "u0", "v0", "ut", "vt"))
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fw, bw = [], []
+    for it in range(reps + 1):
+        u, v = u0.clone(), v0.clone()
+        torch.cuda.synchronize()
+        e[0].record()
+        r.forward(D, u, v, save=True)
+        e[1].record()
+        lu = ((2.0 / u.numel()) * (u - ut)).contiguous()
+        lv = ((2.0 / u.numel()) * (v - vt)).contiguous()
+        torch.cuda.synchronize()
+        e[1].record()
+        r.backward(D, lu, lv)
+        e[2].record()
+        torch.cuda.synchronize()
+        if it:  # the first repetition is the warmup
+            fw.append(e[0].elapsed_time(e[1]))
+            bw.append(e[1].elapsed_time(e[2]))
+    f, b = min(fw), min(bw)
+    return {"workload": "Schnakenberg RDD 1024x1024, 2000 fp32 Euler steps, checkpoints every 100",
+            "forward_ms": f, "backward_ms": b, "adjoint_over_forward": b / f, "grad_per_s": 1e3 / (f + b),
+            "forward_cell_steps_per_s": W * H * steps / (f * 1e-3),
+            "forward_algorithmic_GBps": 20.0 * W * H * steps / (f * 1e-3) / 1e9}
+
+
+def particles_extra(n=10_000_000, side=1024, steps=1000):
+    """10^7 particles on a 1024x1024 periodic domain: move, stable radix sort, offsets,
+    in-cell A+A->0 (p = 0.05 per pair per step)."""
+    import numpy as np
+    import torch
+    from inchman_amd.particles import ParticleSystem
+
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0, side, n).astype(np.float32)
+    y = rng.uniform(0, side, n).astype(np.float32)
+    ids = np.arange(n, dtype=np.uint32)
+    ps = ParticleSystem(side, D=1.0, dt=0.01, vortex_amp=0.1, p_annihilate=0.05, seed=17, capacity=n)
+    out = {"workload": "10^7 particles, 1024x1024 periodic cells, %d steps" % steps}
+    for label, k in (("first_10_steps", 10), ("full_run", steps)):
+        ps.set(x, y, ids)
+        p0 = ps.processed()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ps.step(0, k)
+        n_end = ps.count()
+        wall = time.perf_counter() - t0
+        done = ps.processed() - p0
+        out[label] = {"particle_steps_per_s": done / wall, "particle_steps": done, "wall_s": wall,
+                      "live_at_end": n_end}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +238,8 @@ def main():
     extra["gpu_on_cpu_sample_cell_steps_per_s"] = cells * args.cpu_steps / w2
     if not args.no_extra:
         extra["ensemble"] = ensemble_extra()
+        extra["rdd"] = rdd_extra()
+        extra["particles"] = particles_extra()
     out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     out["extra"] = extra
     print(json.dumps(out), flush=True)

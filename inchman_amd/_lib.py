@@ -136,6 +136,7 @@ SIGNATURES = {
     "gpgmp_particles_get": [_P, _pF, _pF, _pU, _pU, _pU, _pI],
     "gpgmp_particles_count": [_P, _pI],
     "gpgmp_particles_step": [_P, _I, _I],
+    "gpgmp_particles_processed": [_P, C.POINTER(C.c_uint64)],
 }
 
 _lib = None

@@ -33,24 +33,11 @@ int guarded(F &&f) {
     }
 }
 
+// The caller's stream; NULL is the default (null) stream, so work is ordered with
+// every other blocking stream of the process (torch's default stream included).
 struct StreamHolder {
     hipStream_t s = nullptr;
-    bool own = false;
-    void init(void *user) {
-        if (user) {
-            s = (hipStream_t)user;
-            own = false;
-        } else {
-            GPGMP_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-            own = true;
-        }
-    }
-    ~StreamHolder() {
-        if (own && s) {
-            (void)hipStreamSynchronize(s);
-            (void)hipStreamDestroy(s);
-        }
-    }
+    void init(void *user) { s = (hipStream_t)user; }
 };
 }  // namespace
 

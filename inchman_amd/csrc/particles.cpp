@@ -1,0 +1,236 @@
+// C ABI of the individual-particle path: radix sort, cell offsets and the particle system.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <memory>
+
+#include "host.h"
+#include "particles_launch.h"
+
+using namespace gpgmp;
+
+template <typename F>
+static int guarded_p(F &&f) {
+    try {
+        f();
+        return GPGMP_OK;
+    } catch (const Error &e) {
+        gpgmp_set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        gpgmp_set_last_error("host allocation failed");
+        return GPGMP_E_CAPACITY;
+    } catch (const std::exception &e) {
+        gpgmp_set_last_error(e.what());
+        return GPGMP_E_INVALID;
+    }
+}
+
+static int bits_for(uint64_t ncells) {
+    int b = 0;
+    while (((uint64_t)1 << b) < ncells) ++b;
+    return std::max(b, 1);
+}
+
+struct gpgmp_particles {
+    gpgmp_particle_params p{};
+    int capacity = 0;
+    hipStream_t stream = nullptr;
+    DevBuf<float> x, y, x2, y2;
+    DevBuf<uint32_t> id, id2, key, val, key2, val2, key_out, keep, off, tile_sum, hist, digit_total;
+    DevBuf<int> dn;  // [0] live count, [1] next count
+    DevBuf<unsigned long long> processed;  // particle-steps done
+    int *n_host = nullptr;  // pinned
+    int n_ub = 0;
+    bool stepped = false;  // keys/offsets exist once a step has sorted the list
+    ~gpgmp_particles() {
+        (void)hipStreamSynchronize(stream);
+        if (n_host) (void)hipHostFree(n_host);
+    }
+    ParticleBuffers buffers() {
+        ParticleBuffers b;
+        b.x = x.get(); b.y = y.get(); b.x2 = x2.get(); b.y2 = y2.get();
+        b.id = id.get(); b.id2 = id2.get(); b.key = key.get(); b.val = val.get();
+        b.key2 = key2.get(); b.val2 = val2.get(); b.key_out = key_out.get(); b.keep = keep.get();
+        b.off = off.get(); b.tile_sum = tile_sum.get(); b.hist = hist.get(); b.digit_total = digit_total.get();
+        b.d_n = dn.get(); b.d_n2 = dn.get() + 1;
+        b.processed = processed.get();
+        b.n_ub = n_ub;
+        return b;
+    }
+    int read_n() {
+        GPGMP_HIP_CHECK(hipMemcpyAsync(n_host, dn.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
+        GPGMP_HIP_CHECK(hipStreamSynchronize(stream));
+        n_ub = *n_host;
+        return *n_host;
+    }
+};
+
+extern "C" {
+
+int gpgmp_sort_pairs_device(uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp, int32_t n,
+                            int32_t key_bits, void *stream) {
+    return guarded_p([&] {
+        GPGMP_REQUIRE(n >= 0, GPGMP_E_INVALID, "n must be non-negative");
+        if (n == 0) return;
+        GPGMP_REQUIRE(keys && vals && keys_tmp && vals_tmp, GPGMP_E_INVALID, "null buffer");
+        GPGMP_REQUIRE(key_bits >= 1 && key_bits <= 32, GPGMP_E_INVALID, "key_bits must be in [1, 32]");
+        hipStream_t s = (hipStream_t)stream;
+        const int nt = std::max(radix_tiles(n), 1);
+        DevBuf<uint32_t> hist((size_t)256 * nt), tot(256);
+        bool in_tmp = false;
+        GPGMP_HIP_CHECK(radix_sort_pairs(keys, vals, keys_tmp, vals_tmp, nullptr, n, key_bits, hist.get(), tot.get(),
+                                         s, &in_tmp));
+        if (in_tmp) {
+            GPGMP_HIP_CHECK(hipMemcpyAsync(keys, keys_tmp, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+            GPGMP_HIP_CHECK(hipMemcpyAsync(vals, vals_tmp, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+        }
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s));  // scratch is freed on return
+    });
+}
+
+int gpgmp_cell_offsets_device(const uint32_t *sorted_keys, int32_t n, int32_t ncells, uint32_t *off, void *stream) {
+    return guarded_p([&] {
+        GPGMP_REQUIRE(off && ncells >= 1 && n >= 0 && (n == 0 || sorted_keys), GPGMP_E_INVALID, "bad argument");
+        GPGMP_HIP_CHECK(launch_lower_bound(sorted_keys, nullptr, n, ncells, off, (hipStream_t)stream));
+    });
+}
+
+int gpgmp_cell_left_right_device(const uint32_t *sorted_keys, int32_t n, int32_t ncells, uint32_t *left,
+                                 uint32_t *right, void *stream) {
+    return guarded_p([&] {
+        GPGMP_REQUIRE(left && right && ncells >= 1 && n >= 0 && (n == 0 || sorted_keys), GPGMP_E_INVALID,
+                      "bad argument");
+        hipStream_t s = (hipStream_t)stream;
+        DevBuf<uint32_t> off((size_t)ncells + 1);
+        GPGMP_HIP_CHECK(launch_lower_bound(sorted_keys, nullptr, n, ncells, off.get(), s));
+        GPGMP_HIP_CHECK(launch_left_right(off.get(), ncells, left, right, s));
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int gpgmp_particles_create(const gpgmp_particle_params *p, int32_t capacity, void *stream, gpgmp_particles **out) {
+    return guarded_p([&] {
+        GPGMP_REQUIRE(p && out, GPGMP_E_INVALID, "null argument");
+        *out = nullptr;
+        GPGMP_REQUIRE(p->side >= 1 && p->side <= 46340, GPGMP_E_INVALID, "side out of range");
+        GPGMP_REQUIRE(capacity >= 1 && capacity <= (1 << 28), GPGMP_E_CAPACITY, "capacity out of range");
+        GPGMP_REQUIRE(p->D >= 0.f && p->dt >= 0.f && p->p_annihilate >= 0.f && p->p_annihilate <= 1.f,
+                      GPGMP_E_INVALID, "bad parameters");
+        std::unique_ptr<gpgmp_particles> ps(new gpgmp_particles());
+        ps->p = *p;
+        ps->capacity = capacity;
+        ps->stream = (hipStream_t)stream;  // NULL: the default stream
+        const size_t n = capacity;
+        for (DevBuf<float> *b : {&ps->x, &ps->y, &ps->x2, &ps->y2}) b->alloc(n);
+        for (DevBuf<uint32_t> *b : {&ps->id, &ps->id2, &ps->key, &ps->val, &ps->key2, &ps->val2, &ps->key_out,
+                                    &ps->keep})
+            b->alloc(n);
+        const size_t ncells = (size_t)p->side * p->side;
+        ps->off.alloc(ncells + 1);
+        const int nt = std::max(radix_tiles(capacity), 1);
+        ps->tile_sum.alloc(nt);
+        ps->hist.alloc((size_t)256 * nt);
+        ps->digit_total.alloc(256);
+        ps->dn.alloc(2);
+        ps->processed.alloc(1);
+        GPGMP_HIP_CHECK(hipMemsetAsync(ps->processed.get(), 0, sizeof(unsigned long long), ps->stream));
+        GPGMP_HIP_CHECK(hipHostMalloc((void **)&ps->n_host, sizeof(int)));
+        *ps->n_host = 0;
+        GPGMP_HIP_CHECK(hipMemsetAsync(ps->dn.get(), 0, 2 * sizeof(int), ps->stream));
+        GPGMP_HIP_CHECK(hipStreamSynchronize(ps->stream));
+        *out = ps.release();
+    });
+}
+
+int gpgmp_particles_destroy(gpgmp_particles *ps) {
+    delete ps;
+    return GPGMP_OK;
+}
+
+int gpgmp_particles_set(gpgmp_particles *ps, const float *x, const float *y, const uint32_t *id, int32_t n) {
+    return guarded_p([&] {
+        GPGMP_REQUIRE(ps && n >= 0 && (n == 0 || (x && y && id)), GPGMP_E_INVALID, "bad argument");
+        GPGMP_REQUIRE(n <= ps->capacity, GPGMP_E_CAPACITY, "more particles than the capacity");
+        const float L = (float)ps->p.side;
+        for (int i = 0; i < n; ++i)
+            GPGMP_REQUIRE(x[i] >= 0.f && x[i] < L && y[i] >= 0.f && y[i] < L, GPGMP_E_INVALID,
+                          "positions must lie in [0, side)");
+        hipStream_t s = ps->stream;
+        if (n) {
+            GPGMP_HIP_CHECK(hipMemcpyAsync(ps->x.get(), x, (size_t)n * 4, hipMemcpyHostToDevice, s));
+            GPGMP_HIP_CHECK(hipMemcpyAsync(ps->y.get(), y, (size_t)n * 4, hipMemcpyHostToDevice, s));
+            GPGMP_HIP_CHECK(hipMemcpyAsync(ps->id.get(), id, (size_t)n * 4, hipMemcpyHostToDevice, s));
+        }
+        *ps->n_host = n;
+        GPGMP_HIP_CHECK(hipMemcpyAsync(ps->dn.get(), ps->n_host, sizeof(int), hipMemcpyHostToDevice, s));
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s));
+        ps->n_ub = n;
+        ps->stepped = false;
+    });
+}
+
+int gpgmp_particles_count(gpgmp_particles *ps, int32_t *n_out) {
+    return guarded_p([&] {
+        GPGMP_REQUIRE(ps && n_out, GPGMP_E_INVALID, "null argument");
+        *n_out = ps->read_n();
+    });
+}
+
+int gpgmp_particles_get(gpgmp_particles *ps, float *x, float *y, uint32_t *id, uint32_t *keys, uint32_t *off,
+                        int32_t *n_out) {
+    return guarded_p([&] {
+        GPGMP_REQUIRE(ps && n_out, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(ps->stepped || (!keys && !off), GPGMP_E_STATE, "keys and offsets exist after a step");
+        const int n = ps->read_n();
+        *n_out = n;
+        hipStream_t s = ps->stream;
+        const size_t ncells = (size_t)ps->p.side * ps->p.side;
+        if (n) {
+            if (x) GPGMP_HIP_CHECK(hipMemcpyAsync(x, ps->x.get(), (size_t)n * 4, hipMemcpyDeviceToHost, s));
+            if (y) GPGMP_HIP_CHECK(hipMemcpyAsync(y, ps->y.get(), (size_t)n * 4, hipMemcpyDeviceToHost, s));
+            if (id) GPGMP_HIP_CHECK(hipMemcpyAsync(id, ps->id.get(), (size_t)n * 4, hipMemcpyDeviceToHost, s));
+            if (keys) GPGMP_HIP_CHECK(hipMemcpyAsync(keys, ps->key_out.get(), (size_t)n * 4, hipMemcpyDeviceToHost, s));
+        }
+        if (off) {
+            // offsets of the stored (compacted, sorted) keys
+            GPGMP_HIP_CHECK(launch_lower_bound(ps->key_out.get(), ps->dn.get(), n, (int)ncells, ps->off.get(), s));
+            GPGMP_HIP_CHECK(hipMemcpyAsync(off, ps->off.get(), (ncells + 1) * 4, hipMemcpyDeviceToHost, s));
+        }
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int gpgmp_particles_processed(gpgmp_particles *ps, uint64_t *particle_steps) {
+    return guarded_p([&] {
+        GPGMP_REQUIRE(ps && particle_steps, GPGMP_E_INVALID, "null argument");
+        unsigned long long v = 0;
+        GPGMP_HIP_CHECK(hipMemcpyAsync(&v, ps->processed.get(), sizeof v, hipMemcpyDeviceToHost, ps->stream));
+        GPGMP_HIP_CHECK(hipStreamSynchronize(ps->stream));
+        *particle_steps = v;
+    });
+}
+
+int gpgmp_particles_step(gpgmp_particles *ps, int32_t step0, int32_t steps) {
+    return guarded_p([&] {
+        GPGMP_REQUIRE(ps && steps >= 0, GPGMP_E_INVALID, "bad argument");
+        ParticleStepArgs a;
+        a.side = ps->p.side;
+        a.D = ps->p.D;
+        a.dt = ps->p.dt;
+        a.vortex_amp = ps->p.vortex_amp;
+        a.p_annihilate = ps->p.p_annihilate;
+        a.seed = ps->p.seed;
+        a.key_bits = bits_for((uint64_t)ps->p.side * ps->p.side);
+        for (int k = 0; k < steps; ++k) {
+            // the live count only shrinks: grids sized by the last value read back
+            if (k % 8 == 0) ps->read_n();
+            if (ps->n_ub == 0) break;
+            ParticleBuffers b = ps->buffers();
+            GPGMP_HIP_CHECK(particle_step(a, step0 + k, b, ps->stream));
+            ps->stepped = true;
+        }
+    });
+}
+
+}  // extern "C"

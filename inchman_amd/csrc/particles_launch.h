@@ -1,0 +1,36 @@
+// Host-side launch interface of the particle kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+namespace gpgmp {
+
+struct ParticleStepArgs {
+    int side;
+    float D, dt, vortex_amp, p_annihilate;
+    uint32_t seed;
+    int key_bits;  // bits the cell keys occupy (passes = ceil(key_bits / 8))
+};
+
+// Device buffers of a particle system (capacity >= every n it holds).
+struct ParticleBuffers {
+    float *x, *y, *x2, *y2;
+    uint32_t *id, *id2, *key, *val, *key2, *val2, *key_out, *keep, *off, *tile_sum, *hist, *digit_total;
+    int *d_n, *d_n2;  // live count (device) and its next value
+    unsigned long long *processed;  // particle-steps done so far (device)
+    int n_ub;         // host upper bound of the live count (grid sizes)
+};
+
+int radix_tiles(int n);
+hipError_t radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *ktmp, uint32_t *vtmp, const int *d_n,
+                            int n_ub, int key_bits, uint32_t *hist, uint32_t *digit_total, hipStream_t s,
+                            bool *result_in_tmp);
+hipError_t launch_lower_bound(const uint32_t *keys, const int *d_n, int n_host, int ncells, uint32_t *off,
+                              hipStream_t s);
+hipError_t launch_left_right(const uint32_t *off, int ncells, uint32_t *left, uint32_t *right, hipStream_t s);
+hipError_t particle_step(const ParticleStepArgs &a, int step, ParticleBuffers &b, hipStream_t s);
+
+}  // namespace gpgmp

@@ -33,7 +33,6 @@ struct gpgmp_rdd {
     gpgmp_rdd_params p{};
     int steps = 0, ckpt = 0, nck = 0;
     hipStream_t stream = nullptr;
-    bool own_stream = false;
     size_t A = 0;
     DevBuf<float> ck;        // nck checkpoints of (u, v)
     DevBuf<float> seg;       // ckpt recomputed states of one segment
@@ -43,8 +42,7 @@ struct gpgmp_rdd {
     bool saved = false;
     RddConsts c{};
     ~gpgmp_rdd() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        if (own_stream && stream) (void)hipStreamDestroy(stream);
+        (void)hipStreamSynchronize(stream);
     }
 };
 
@@ -104,12 +102,7 @@ int gpgmp_rdd_create(const gpgmp_rdd_params *p, int32_t steps, int32_t ckpt_ever
         r->ckpt = std::min(ckpt_every, std::max(steps, 1));
         r->nck = steps > 0 ? (steps + r->ckpt - 1) / r->ckpt : 0;
         r->A = (size_t)p->width * p->height;
-        if (stream) {
-            r->stream = (hipStream_t)stream;
-        } else {
-            GPGMP_HIP_CHECK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
-            r->own_stream = true;
-        }
+        r->stream = (hipStream_t)stream;  // NULL: the default stream
         r->ck.alloc((size_t)std::max(r->nck, 1) * 2 * r->A);
         r->seg.alloc((size_t)r->ckpt * 2 * r->A);
         r->tmp.alloc(2 * r->A);

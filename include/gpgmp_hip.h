@@ -221,6 +221,8 @@ int gpgmp_particles_get(gpgmp_particles *ps, float *x, float *y, uint32_t *id, u
 int gpgmp_particles_count(gpgmp_particles *ps, int32_t *n_out);
 /* `steps` steps starting at step index `step0`: move, stable sort, offsets, annihilate. */
 int gpgmp_particles_step(gpgmp_particles *ps, int32_t step0, int32_t steps);
+/* Sum over the steps run so far of the particles each step processed. */
+int gpgmp_particles_processed(gpgmp_particles *ps, uint64_t *particle_steps);
 
 #ifdef __cplusplus
 }
