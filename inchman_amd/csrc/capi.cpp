@@ -455,6 +455,17 @@ int gpgmp_batch_total_cells(gpgmp_batch *b, int64_t *cells) {
     });
 }
 
+int gpgmp_min_timestep_device(const float *D, const float *r, int32_t n, float h, float *out_host, void *stream) {
+    return guarded([&] {
+        GPGMP_REQUIRE(D && r && out_host && n >= 1 && h > 0.f, GPGMP_E_INVALID, "bad argument");
+        hipStream_t s = (hipStream_t)stream;
+        DevBuf<float> scratch(1025);
+        GPGMP_HIP_CHECK(launch_min_dt(D, r, n, h, scratch.get(), scratch.get() + 1024, s));
+        GPGMP_HIP_CHECK(hipMemcpyAsync(out_host, scratch.get() + 1024, sizeof(float), hipMemcpyDeviceToHost, s));
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
 int gpgmp_batch_synchronize(gpgmp_batch *b) {
     return guarded([&] {
         GPGMP_REQUIRE(b, GPGMP_E_INVALID, "null argument");

@@ -19,6 +19,8 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "gpgmp_numeric.h"
 #include "stochastic_types.h"
 
@@ -294,6 +296,56 @@ __global__ void k_inh_probs(float2 *__restrict__ out, const float *__restrict__ 
     out[i] = make_float2((p1 <= p2) ? p1 : p2, p2);
 }
 
+// Per-cell admissible time step of the stochastic inhomogeneous solver
+// (computeIndividualTimestepX/Y, StochasticInhomogeneousDriftDiffusion.cl:130-151), in
+// double, and its minimum over the cells: block minima, then one block over the
+// minima (the reference reduces in a single work-group, which caps grids at 512^2,
+// InhomogeneousSolver.cpp:63-85).  Min is order-independent, so the result is exact.
+__device__ __forceinline__ float cell_dt(float Df, float rf, float hf) {
+    const double D = Df, r = rf, h = hf;
+    double ts;
+    if (r == 0.0) {
+        ts = h * h / (2.0 * D) * (1.0 - 2.0 / 3.0);
+    } else if (D == 0.0) {
+        ts = h / fabs(r);
+    } else {
+        const double eps = h * r / (2.0 * D);
+        const double th = tanh(eps), sh = sinh(eps);
+        double temp = 1.0 + (-1.0 / (th * eps) + 1.0 / (sh * sh));
+        if (temp <= 0.0) temp = 1.0 / 3.0;
+        ts = h / r * th * temp;
+    }
+    return (float)ts;
+}
+
+__device__ __forceinline__ float block_min(float v, float *smem) {
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_down(v, off, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) smem[wave] = v;
+    __syncthreads();
+    float m = smem[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fminf(m, smem[w]);
+    return m;
+}
+
+__global__ __launch_bounds__(256) void k_min_dt(const float *__restrict__ D, const float *__restrict__ r, int n,
+                                                float h, float *__restrict__ block_out) {
+    __shared__ float sm[4];
+    float v = __builtin_inff();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        v = fminf(v, cell_dt(D[i], r[i], h));
+    const float m = block_min(v, sm);
+    if (threadIdx.x == 0) block_out[blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(256) void k_min_reduce(const float *__restrict__ in, int n, float *__restrict__ out) {
+    __shared__ float sm[4];
+    float v = __builtin_inff();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v = fminf(v, in[i]);
+    const float m = block_min(v, sm);
+    if (threadIdx.x == 0) *out = m;
+}
+
 }  // namespace gpgmp
 
 // ---------------------------------------------------------------------------------------
@@ -342,6 +394,15 @@ hipError_t launch_gmp_step(const StepLaunch &a) {
         case MODE_INH_Y: return launch_m<MODE_INH_Y>(a);
         default: return hipErrorInvalidValue;
     }
+}
+
+// blocks: scratch of at least 1024 floats; out: one device float
+hipError_t launch_min_dt(const float *D, const float *r, int n, float h, float *blocks, float *out,
+                         hipStream_t stream) {
+    const int nb = n > 0 ? std::min(1024, (n + 255) / 256) : 1;
+    hipLaunchKernelGGL(k_min_dt, dim3(nb), dim3(256), 0, stream, D, r, n, h, blocks);
+    hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(256), 0, stream, blocks, nb, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_inh_probs(float2 *out, const float *D, const float *r, float dt, float dx, int n,

@@ -30,6 +30,8 @@ struct StepLaunch {
 };
 
 hipError_t launch_gmp_step(const StepLaunch &a);
+hipError_t launch_min_dt(const float *D, const float *r, int n, float h, float *blocks, float *out,
+                         hipStream_t stream);
 hipError_t launch_inh_probs(float2 *out, const float *D, const float *r, float dt, float dx, int n,
                             hipStream_t stream);
 

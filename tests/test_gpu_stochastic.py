@@ -159,3 +159,25 @@ def test_launch_shape_invariance(gpu):
         s.run(50, 3)
         outs.append(s.synchronizeStateBufferFromDevice())
     assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
+def test_min_timestep_device(gpu, oracle):
+    """computeIndividualTimestepX/Y + reduceBlocks on the device (double, multi-level
+    min) against the host evaluation the solvers use."""
+    import ctypes as C
+    import torch
+
+    from inchman_amd._lib import call
+
+    for side in (64, 256, 1024):
+        m = configs.schnakenberg_model(side)
+        Dx, Dy, rx, ry = m.fields
+        h = oracle.header_float(m.physical_length / m.width)
+        for s in range(2):
+            for D, r in ((Dx[s], rx[s]), (Dy[s], ry[s])):
+                tD = torch.from_numpy(np.ascontiguousarray(D)).cuda()
+                tr = torch.from_numpy(np.ascontiguousarray(r)).cuda()
+                out = C.c_float()
+                call("gpgmp_min_timestep_device", tD.data_ptr(), tr.data_ptr(), D.size, h, C.byref(out), None)
+                want = np.float32(oracle.inhomogeneous_dt(D, r, h))
+                assert abs(np.float32(out.value) - want) <= np.spacing(want), (side, s)
