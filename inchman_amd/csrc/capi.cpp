@@ -302,6 +302,14 @@ int gpgmp_solver_set_launch(gpgmp_solver *s, int32_t block, int32_t cpt) {
     });
 }
 
+int gpgmp_solver_set_event_budget(gpgmp_solver *s, uint32_t budget) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && budget >= 1, GPGMP_E_INVALID, "bad argument");
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s->stream.s));
+        s->eng->set_hot_budget(budget);
+    });
+}
+
 int gpgmp_solver_kernel_times(gpgmp_solver *s, int32_t on, double *ms, int64_t *n, double *bytes) {
     return guarded([&] {
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
@@ -373,6 +381,14 @@ int gpgmp_batch_set_launch(gpgmp_batch *b, int32_t block, int32_t cpt) {
     return guarded([&] {
         GPGMP_REQUIRE(b, GPGMP_E_INVALID, "null argument");
         b->eng->set_launch(block, cpt);
+    });
+}
+
+int gpgmp_batch_set_event_budget(gpgmp_batch *b, uint32_t budget) {
+    return guarded([&] {
+        GPGMP_REQUIRE(b && budget >= 1, GPGMP_E_INVALID, "bad argument");
+        GPGMP_HIP_CHECK(hipStreamSynchronize(b->stream.s));
+        b->eng->set_hot_budget(budget);
     });
 }
 

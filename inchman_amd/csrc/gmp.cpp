@@ -229,6 +229,14 @@ GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vecto
         prob_[0].alloc(po);
         prob_[1].alloc(po);
     }
+    // deferred busy cells: at most one entry per cell of the launch, capped at 2^20
+    hot_.cap = (uint32_t)std::min<int64_t>(total_cells_, (int64_t)1 << 20);
+    hot_.budget = 32;  // measured best for the 256x256 Schnakenberg run (16..32)
+    hot_e_.alloc(hot_.cap);
+    hot_count_.alloc(2);
+    hot_.e = hot_e_.get();
+    hot_.count = hot_count_.get();
+    GPGMP_HIP_CHECK(hipMemsetAsync(hot_count_.get(), 0, 2 * sizeof(uint32_t), stream_));
     reals_dev_.alloc(reals_.size());
     GPGMP_HIP_CHECK(hipMemcpyAsync(reals_dev_.get(), reals_.data(), reals_.size() * sizeof(RealInfo),
                                    hipMemcpyHostToDevice, stream_));
@@ -296,7 +304,8 @@ void GmpEngine::set_probs(int r, int s, int axis, const float *D, const float *r
 
 void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cur,
                        const int32_t *seeds_prev, const int32_t *lost_in, int32_t *lost_out,
-                       hipEvent_t ev_start, hipEvent_t ev_stop) {
+                       hipEvent_t ev_start, hipEvent_t ev_stop, hipEvent_t ev_hot_start,
+                       hipEvent_t ev_hot_stop) {
     StepLaunch a{};
     a.mode = mode;
     a.state = state_.get();
@@ -317,6 +326,10 @@ void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cu
     a.stream = stream_;
     a.ev_start = ev_start;
     a.ev_stop = ev_stop;
+    a.hot = hot_;
+    a.hot_grid = (int)std::min<uint32_t>(hot_.cap, 1024u);
+    a.ev_hot_start = ev_hot_start;
+    a.ev_hot_stop = ev_hot_stop;
     GPGMP_HIP_CHECK(launch_gmp_step(a));
 }
 
@@ -384,18 +397,26 @@ void GmpRunner::timed_launch(int mode, const LaunchPlan *plan, const LaunchPlan 
         eng_.launch(mode, plan, scur, sprev, lin, lout);
         return;
     }
-    if (ev_used_ == ev_pool_.size()) {
-        hipEvent_t a, b;
-        GPGMP_HIP_CHECK(hipEventCreate(&a));
-        GPGMP_HIP_CHECK(hipEventCreate(&b));
-        ev_pool_.emplace_back(a, b);
-        ev_mode_.push_back(0);
-    }
-    auto &e = ev_pool_[ev_used_];
-    ev_mode_[ev_used_] = mode;
+    // returns a copy: growing the pool would invalidate references
+    auto next_pair = [&](int m) -> std::pair<hipEvent_t, hipEvent_t> {
+        if (ev_used_ == ev_pool_.size()) {
+            hipEvent_t a, b;
+            GPGMP_HIP_CHECK(hipEventCreate(&a));
+            GPGMP_HIP_CHECK(hipEventCreate(&b));
+            ev_pool_.emplace_back(a, b);
+            ev_mode_.push_back(0);
+        }
+        ev_mode_[ev_used_] = m;
+        return ev_pool_[ev_used_++];
+    };
+    const std::pair<hipEvent_t, hipEvent_t> e = next_pair(mode);
     kt_bytes_[mode] += launch_bytes(eng_, mode, hplan);
-    ++ev_used_;
-    eng_.launch(mode, plan, scur, sprev, lin, lout, e.first, e.second);
+    if (mode == 2) {
+        eng_.launch(mode, plan, scur, sprev, lin, lout, e.first, e.second);
+    } else {
+        const std::pair<hipEvent_t, hipEvent_t> h = next_pair(3);
+        eng_.launch(mode, plan, scur, sprev, lin, lout, e.first, e.second, h.first, h.second);
+    }
 }
 
 void GmpRunner::collect_times() {

@@ -154,6 +154,9 @@ class GmpEngine {
     void set_stream(hipStream_t s) { stream_ = s; }
 
     void set_launch(int block, int cpt);
+    // SSA events a lane runs before its cell moves to the hot-cell kernel
+    void set_hot_budget(uint32_t b) { hot_.budget = b; }
+    uint32_t hot_budget() const { return hot_.budget; }
     int block() const { return block_; }
     int cpt() const { return cpt_; }
 
@@ -166,7 +169,8 @@ class GmpEngine {
     // one launch (all device pointers)
     void launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cur, const int32_t *seeds_prev,
                 const int32_t *lost_in, int32_t *lost_out, hipEvent_t ev_start = nullptr,
-                hipEvent_t ev_stop = nullptr);
+                hipEvent_t ev_stop = nullptr, hipEvent_t ev_hot_start = nullptr,
+                hipEvent_t ev_hot_stop = nullptr);
 
     int32_t *lost(int i) const { return lost_[i].get(); }
 
@@ -183,6 +187,9 @@ class GmpEngine {
     DevBuf<float2> prob_[2];
     DevBuf<RealInfo> reals_dev_;
     DevBuf<Tile> tiles_dev_;
+    DevBuf<HotEntry> hot_e_;
+    DevBuf<uint32_t> hot_count_;
+    HotList hot_{};
     int ntiles_ = 0;
     int block_ = 256, cpt_ = 1;
     int64_t total_cells_ = 0;
@@ -200,10 +207,11 @@ class GmpRunner {
     void run(int steps, int graph_chunk);
     float sim_time() const { return sched_.sim_time(); }
     // Per-launch HIP-event timing of the eager (non-graph) launches, by kernel mode
-    // (0 homogeneous, 1 SSA + X sweeps, 2 Y sweeps); accumulated over run() calls.
+    // (0 homogeneous, 1 SSA + X sweeps, 2 Y sweeps, 3 hot cells); accumulated over
+    // run() calls.
     void set_timing(bool on) { timing_ = on; }
     void kernel_times(double *ms, int64_t *n, double *bytes) const {
-        for (int i = 0; i < 3; ++i) {
+        for (int i = 0; i < 4; ++i) {
             ms[i] = kt_ms_[i];
             n[i] = kt_n_[i];
             if (bytes) bytes[i] = kt_bytes_[i];
@@ -229,9 +237,10 @@ class GmpRunner {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
     std::vector<int> ev_mode_;
     size_t ev_used_ = 0;
-    double kt_ms_[3] = {0, 0, 0};
-    int64_t kt_n_[3] = {0, 0, 0};
-    double kt_bytes_[3] = {0, 0, 0};  // algorithmic HBM bytes of the timed launches
+    // index 3: the hot-cell kernel that follows every launch with an SSA
+    double kt_ms_[4] = {0, 0, 0, 0};
+    int64_t kt_n_[4] = {0, 0, 0, 0};
+    double kt_bytes_[4] = {0, 0, 0, 0};  // algorithmic HBM bytes of the timed launches
     int chunk_cap_ = 0;
     int graph_steps_ = 0;
 };

@@ -27,6 +27,10 @@ struct StepLaunch {
     hipStream_t stream;
     hipEvent_t ev_start = nullptr;  // optional: timestamps of this dispatch
     hipEvent_t ev_stop = nullptr;
+    HotList hot{};                  // deferred busy cells (modes with an SSA)
+    int hot_grid = 1024;            // waves of the hot-cell kernel
+    hipEvent_t ev_hot_start = nullptr;  // optional: timestamps of the hot-cell dispatch
+    hipEvent_t ev_hot_stop = nullptr;
 };
 
 hipError_t launch_gmp_step(const StepLaunch &a);

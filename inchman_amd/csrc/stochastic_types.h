@@ -53,6 +53,22 @@ struct Tile {
     int32_t cell0;
 };
 
+// A cell whose SSA used up its per-lane event budget in k_gmp_step: the hot-cell
+// kernel continues it from (time, ctr) and then computes its outflow.
+struct HotEntry {
+    int32_t r;      // realization
+    int32_t c;      // cell in the realization
+    float time;     // SSA clock reached
+    uint32_t ctr;   // Threefry counter reached
+};
+
+struct HotList {
+    HotEntry *e;
+    uint32_t *count;  // [0] entries pushed, [1] blocks of the hot kernel finished
+    uint32_t cap;
+    uint32_t budget;  // SSA events a lane runs before deferring its cell
+};
+
 // Homogeneous diffusion thresholds (StochasticHomogeneousDiffusion.cl:58-62)
 struct HomProbs {
     float stay, p1, p2, p3, p4;

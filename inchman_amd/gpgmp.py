@@ -254,12 +254,16 @@ class Solver:
     def setLaunch(self, block, cells_per_thread):
         call("gpgmp_solver_set_launch", self.handle, int(block), int(cells_per_thread))
 
+    def setEventBudget(self, budget):
+        """SSA events per lane before a busy cell moves to the one-wave-per-cell kernel."""
+        call("gpgmp_solver_set_event_budget", self.handle, int(budget))
+
     def kernelTimes(self, on=True):
         """Accumulated HIP-event times (ms, launches) per kernel mode of the current
         run, then switches per-launch timing on/off."""
-        ms = np.zeros(3, np.float64)
-        n = np.zeros(3, np.int64)
-        b = np.zeros(3, np.float64)
+        ms = np.zeros(4, np.float64)
+        n = np.zeros(4, np.int64)
+        b = np.zeros(4, np.float64)
         call("gpgmp_solver_kernel_times", self.handle, int(on), ms.ctypes.data_as(C.POINTER(C.c_double)),
              n.ctypes.data_as(C.POINTER(C.c_int64)), b.ctypes.data_as(C.POINTER(C.c_double)))
         return ms, n, b
@@ -333,9 +337,9 @@ class Ensemble:
         call("gpgmp_batch_reset", self.handle)
 
     def kernelTimes(self, on=True):
-        ms = np.zeros(3, np.float64)
-        n = np.zeros(3, np.int64)
-        b = np.zeros(3, np.float64)
+        ms = np.zeros(4, np.float64)
+        n = np.zeros(4, np.int64)
+        b = np.zeros(4, np.float64)
         call("gpgmp_batch_kernel_times", self.handle, int(on), ms.ctypes.data_as(C.POINTER(C.c_double)),
              n.ctypes.data_as(C.POINTER(C.c_int64)), b.ctypes.data_as(C.POINTER(C.c_double)))
         return ms, n, b

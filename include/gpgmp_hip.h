@@ -124,9 +124,13 @@ int gpgmp_solver_run_continue(gpgmp_solver *s, int32_t steps, int32_t graph_chun
 /* launch shape of the solver's kernels: threads per block (64..256), cells per thread */
 int gpgmp_solver_set_launch(gpgmp_solver *s, int32_t block, int32_t cells_per_thread);
 /* Reads the accumulated per-launch HIP-event times of the current run's eager launches
- * (ms[3], n[3], algorithmic HBM bytes[3] by kernel: 0 homogeneous, 1 SSA + X sweeps,
- * 2 Y sweeps; may be NULL), then switches the timing on or off for the following launches. */
+ * (ms[4], n[4], algorithmic HBM bytes[4] by kernel: 0 homogeneous, 1 SSA + X sweeps,
+ * 2 Y sweeps, 3 hot-cell continuation; may be NULL), then switches the timing on or off
+ * for the following launches. */
 int gpgmp_solver_kernel_times(gpgmp_solver *s, int32_t on, double *ms, int64_t *n, double *bytes);
+/* SSA events a lane runs before its cell moves to the one-wave-per-cell continuation
+ * kernel (default 32); results do not depend on it. */
+int gpgmp_solver_set_event_budget(gpgmp_solver *s, uint32_t budget);
 int gpgmp_solver_synchronize(gpgmp_solver *s);
 
 /* ---- batched ensemble (no reference counterpart) ----------------------------------- */
@@ -151,6 +155,7 @@ int gpgmp_batch_run(gpgmp_batch *b, int32_t steps, int32_t graph_chunk);
 /* Restarts the host loops at time 0 with the realizations' seed0 (state is kept). */
 int gpgmp_batch_reset(gpgmp_batch *b);
 int gpgmp_batch_kernel_times(gpgmp_batch *b, int32_t on, double *ms, int64_t *n, double *bytes);
+int gpgmp_batch_set_event_budget(gpgmp_batch *b, uint32_t budget);
 int gpgmp_batch_total_cells(gpgmp_batch *b, int64_t *cells);
 int gpgmp_batch_synchronize(gpgmp_batch *b);
 
