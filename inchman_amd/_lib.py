@@ -124,6 +124,7 @@ SIGNATURES = {
     "gpgmp_rdd_create": [C.POINTER(RddParams), _I, _I, _P, C.POINTER(_P)],
     "gpgmp_rdd_destroy": [_P],
     "gpgmp_rdd_set_rates": [_P, _pF],
+    "gpgmp_rdd_set_steps_per_launch": [_P, _I],
     "gpgmp_rdd_forward": [_P, _P, _P, _P, _I],
     "gpgmp_rdd_backward": [_P, _P, _P, _P, _P, _P],
     "gpgmp_det_hom_diffuse_device": [_P, _P, _I, _I, _I, _pF, _F, _F, _P],

@@ -108,32 +108,84 @@ __global__ void k_det_rk4(float *__restrict__ state, int A, RxTable T, float dt)
 // update; the reactions are the concentration-form mass-action laws):
 //   u' = u + dt*( cu*(sum_nbr D_n u_n - 4 D u) + k1 - k4 u + k3 u^2 v )
 //   v' = v + dt*( cv*(sum_nbr D_n v_n - 4 D v) + k2 - k3 u^2 v ),  c_s = D0_s/h^2.
-// A 256-thread block owns a 64x16 tile; D*u and D*v of the tile and its one-cell halo
-// are staged in LDS once, so every product is computed once per cell.
+// A 256-thread block owns a 64x16 tile and stages u, v, D of the tile plus an NS-cell
+// halo in LDS; NS = 2 advances two steps per launch (temporal blocking), recomputing
+// the one-cell ring of the intermediate step, so each cell's inputs are read from HBM
+// once per two steps.  The halo outside the grid is the reference's mirror image
+// (g -> -g, g -> 2(W-1)-g), and the neighbour sum is taken as (l + r) + (t + b), which
+// is invariant under both reflections, so a mirrored halo cell evolves bit-identically
+// to the cell it mirrors and NS = 2 equals two NS = 1 launches bit for bit.
 struct RddK {
     float dt, cu, cv, k1, k2, k3, k4;
 };
 
 constexpr int RT_X = 64, RT_Y = 16;
-constexpr int RL_X = RT_X + 2, RL_Y = RT_Y + 2;
 
+// mirror index for g in [-(n-1), 2(n-1)]; halo cells of a partial tile that lie further
+// out feed no in-grid result and are only clamped to a valid address
+__device__ __forceinline__ int reflect(int g, int n) {
+    g = g < 0 ? -g : g;
+    g = g >= n ? 2 * (n - 1) - g : g;
+    return g < 0 ? 0 : (g >= n ? n - 1 : g);
+}
+
+__device__ __forceinline__ void rdd_cell(float uc, float vc, float dc, float dul, float dur, float dut, float dub,
+                                         float dvl, float dvr, float dvt, float dvb, const RddK &p, float &un,
+                                         float &vn) {
+    const float lu = ((dul + dur) + (dut + dub)) - 4.f * (dc * uc);
+    const float lv = ((dvl + dvr) + (dvt + dvb)) - 4.f * (dc * vc);
+    const float uuv = uc * uc * vc;
+    const float fu = p.k1 - p.k4 * uc + p.k3 * uuv;
+    const float fv = p.k2 - p.k3 * uuv;
+    un = uc + p.dt * (p.cu * lu + fu);
+    vn = vc + p.dt * (p.cv * lv + fv);
+}
+
+template <int NS>
 __global__ __launch_bounds__(256) void k_rdd_step(const float *__restrict__ u, const float *__restrict__ v,
-                                                  const float *__restrict__ D, float *__restrict__ un,
+                                                  const float *__restrict__ D, float *__restrict__ umid,
+                                                  float *__restrict__ vmid, float *__restrict__ un,
                                                   float *__restrict__ vn, int W, int H, RddK p) {
-    __shared__ float su[RL_Y][RL_X];
-    __shared__ float sv[RL_Y][RL_X];
+    constexpr int LX = RT_X + 2 * NS, LY = RT_Y + 2 * NS;
+    __shared__ float sd[LY][LX];
+    __shared__ float su[2][LY][LX];
+    __shared__ float sv[2][LY][LX];
     const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
-    for (int idx = threadIdx.x; idx < RL_X * RL_Y; idx += blockDim.x) {
-        const int ly = idx / RL_X, lx = idx - ly * RL_X;
-        int gx = x0 + lx - 1, gy = y0 + ly - 1;
-        gx = gx < 0 ? 1 : (gx >= W ? W - 2 : gx);
-        gy = gy < 0 ? 1 : (gy >= H ? H - 2 : gy);
+    for (int idx = threadIdx.x; idx < LX * LY; idx += blockDim.x) {
+        const int ly = idx / LX, lx = idx - ly * LX;
+        const int gx = reflect(x0 + lx - NS, W), gy = reflect(y0 + ly - NS, H);
         const size_t g = (size_t)gy * W + gx;
-        const float d = D[g];
-        su[ly][lx] = d * u[g];
-        sv[ly][lx] = d * v[g];
+        sd[ly][lx] = D[g];
+        su[0][ly][lx] = u[g];
+        sv[0][ly][lx] = v[g];
     }
     __syncthreads();
+    // intermediate steps on the shrinking region (ring r = s .. L-1-s)
+#pragma unroll
+    for (int s = 1; s < NS; ++s) {
+        const int a = (s - 1) & 1, b = s & 1;
+        const int nx = LX - 2 * s, ny = LY - 2 * s;
+        for (int idx = threadIdx.x; idx < nx * ny; idx += blockDim.x) {
+            const int ly = idx / nx + s, lx = idx % nx + s;
+            float uo, vo;
+            rdd_cell(su[a][ly][lx], sv[a][ly][lx], sd[ly][lx], sd[ly][lx - 1] * su[a][ly][lx - 1],
+                     sd[ly][lx + 1] * su[a][ly][lx + 1], sd[ly - 1][lx] * su[a][ly - 1][lx],
+                     sd[ly + 1][lx] * su[a][ly + 1][lx], sd[ly][lx - 1] * sv[a][ly][lx - 1],
+                     sd[ly][lx + 1] * sv[a][ly][lx + 1], sd[ly - 1][lx] * sv[a][ly - 1][lx],
+                     sd[ly + 1][lx] * sv[a][ly + 1][lx], p, uo, vo);
+            su[b][ly][lx] = uo;
+            sv[b][ly][lx] = vo;
+            if (umid && lx >= NS && lx < NS + RT_X && ly >= NS && ly < NS + RT_Y) {
+                const int gx = x0 + lx - NS, gy = y0 + ly - NS;
+                if (gx < W && gy < H) {
+                    umid[(size_t)gy * W + gx] = uo;
+                    vmid[(size_t)gy * W + gx] = vo;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    constexpr int a = (NS - 1) & 1;
     const int tx = threadIdx.x & (RT_X - 1);
     const int ty0 = (threadIdx.x >> 6) * 4;
     const int gx = x0 + tx;
@@ -143,16 +195,16 @@ __global__ __launch_bounds__(256) void k_rdd_step(const float *__restrict__ u, c
         const int ty = ty0 + k;
         const int gy = y0 + ty;
         if (gy >= H) break;
+        const int lx = tx + NS, ly = ty + NS;
+        float uo, vo;
+        rdd_cell(su[a][ly][lx], sv[a][ly][lx], sd[ly][lx], sd[ly][lx - 1] * su[a][ly][lx - 1],
+                 sd[ly][lx + 1] * su[a][ly][lx + 1], sd[ly - 1][lx] * su[a][ly - 1][lx],
+                 sd[ly + 1][lx] * su[a][ly + 1][lx], sd[ly][lx - 1] * sv[a][ly][lx - 1],
+                 sd[ly][lx + 1] * sv[a][ly][lx + 1], sd[ly - 1][lx] * sv[a][ly - 1][lx],
+                 sd[ly + 1][lx] * sv[a][ly + 1][lx], p, uo, vo);
         const size_t g = (size_t)gy * W + gx;
-        const float uc = u[g], vc = v[g];
-        const int lx = tx + 1, ly = ty + 1;
-        const float lu = su[ly][lx - 1] + su[ly][lx + 1] + su[ly - 1][lx] + su[ly + 1][lx] - 4.f * su[ly][lx];
-        const float lv = sv[ly][lx - 1] + sv[ly][lx + 1] + sv[ly - 1][lx] + sv[ly + 1][lx] - 4.f * sv[ly][lx];
-        const float uuv = uc * uc * vc;
-        const float fu = p.k1 - p.k4 * uc + p.k3 * uuv;
-        const float fv = p.k2 - p.k3 * uuv;
-        un[g] = uc + p.dt * (p.cu * lu + fu);
-        vn[g] = vc + p.dt * (p.cv * lv + fv);
+        un[g] = uo;
+        vn[g] = vo;
     }
 }
 
@@ -164,19 +216,67 @@ __global__ __launch_bounds__(256) void k_rdd_step(const float *__restrict__ u, c
 //   lv_prev  = lv + dt*cv*D*(T(lv) - 4 lv) + dt*(k3 u^2 lu - k3 u^2 lv)
 //   dL/dD   += dt*( cu*u*(T(lu) - 4 lu) + cv*v*(T(lv) - 4 lv) )
 //   dL/dk   += dt*( sum lu, sum lv, sum u^2 v (lu - lv), -sum u lu )
-// Block sums of dL/dk go to a per-tile double accumulator (same tile every step, so
-// there is no race and the summation order is fixed).
+// NS = 2 walks two steps back per launch: lambda of the intermediate step is formed on
+// the tile plus a one-cell ring (zero outside the grid, where lambda vanishes), from
+// lambda staged with a two-cell halo.  Block sums of dL/dk go, step by step, to a
+// per-tile double accumulator (same tile every launch: no race, fixed order), and
+// dL/dD is accumulated in the same order as by NS = 1 launches.
+__device__ __forceinline__ float tw(int c, int n) { return (c == 0 || c == n - 1) ? 2.f : 1.f; }
+
+__device__ __forceinline__ void adj_cell(float l0, float m0, float ll, float lr, float lt, float lb, float ml,
+                                         float mr, float mt, float mb, float wl, float wr, float wt, float wb,
+                                         float uc, float vc, float dc, const RddK &p, float &lo, float &mo,
+                                         float &gd, float (&gk)[4]) {
+    const float tu = (wl * ll + wr * lr) + (wt * lt + wb * lb);
+    const float tv = (wl * ml + wr * mr) + (wt * mt + wb * mb);
+    const float au = tu - 4.f * l0, av = tv - 4.f * m0;
+    const float uv = uc * vc, uu = uc * uc;
+    lo = l0 + p.dt * p.cu * dc * au + p.dt * ((-p.k4 + 2.f * p.k3 * uv) * l0 - 2.f * p.k3 * uv * m0);
+    mo = m0 + p.dt * p.cv * dc * av + p.dt * (p.k3 * uu * l0 - p.k3 * uu * m0);
+    gd = p.dt * (p.cu * uc * au + p.cv * vc * av);
+    gk[0] += l0;
+    gk[1] += m0;
+    gk[2] += uu * vc * (l0 - m0);
+    gk[3] -= uc * l0;
+}
+
+__device__ __forceinline__ void tile_gk_add(float (&gk)[4], double (*red)[4], double *gk_tiles, float dt) {
+    double d[4] = {gk[0], gk[1], gk[2], gk[3]};
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        for (int off = 32; off > 0; off >>= 1) d[j] += __shfl_down(d[j], off, 64);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) red[wave][j] = d[j];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const double s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+        const size_t tile = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+        gk_tiles[tile * 4 + threadIdx.x] += (double)dt * s;
+    }
+    __syncthreads();
+}
+
+// states: u1/v1 = state before the later step (NS = 1: the only one), u0/v0 = state
+// before the earlier step (NS = 2)
+template <int NS>
 __global__ __launch_bounds__(256) void k_rdd_adjoint(
-    const float *__restrict__ lu, const float *__restrict__ lv, const float *__restrict__ u,
-    const float *__restrict__ v, const float *__restrict__ D, float *__restrict__ lun, float *__restrict__ lvn,
-    float *__restrict__ gD, double *__restrict__ gk_tiles, int W, int H, RddK p) {
-    __shared__ float slu[RL_Y][RL_X];
-    __shared__ float slv[RL_Y][RL_X];
+    const float *__restrict__ lu, const float *__restrict__ lv, const float *__restrict__ u1,
+    const float *__restrict__ v1, const float *__restrict__ u0, const float *__restrict__ v0,
+    const float *__restrict__ D, float *__restrict__ lun, float *__restrict__ lvn, float *__restrict__ gD,
+    double *__restrict__ gk_tiles, int W, int H, RddK p) {
+    constexpr int LX = RT_X + 2 * NS, LY = RT_Y + 2 * NS;
+    constexpr int MX = RT_X + 2, MY = RT_Y + 2;  // ring of the intermediate lambda
+    __shared__ float slu[LY][LX];
+    __shared__ float slv[LY][LX];
+    __shared__ float mlu[NS == 2 ? MY : 1][NS == 2 ? MX : 1];
+    __shared__ float mlv[NS == 2 ? MY : 1][NS == 2 ? MX : 1];
     __shared__ double red[4][4];
     const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
-    for (int idx = threadIdx.x; idx < RL_X * RL_Y; idx += blockDim.x) {
-        const int ly = idx / RL_X, lx = idx - ly * RL_X;
-        const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+    for (int idx = threadIdx.x; idx < LX * LY; idx += blockDim.x) {
+        const int ly = idx / LX, lx = idx - ly * LX;
+        const int gx = x0 + lx - NS, gy = y0 + ly - NS;
         float a = 0.f, b = 0.f;
         if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
             const size_t g = (size_t)gy * W + gx;
@@ -190,55 +290,90 @@ __global__ __launch_bounds__(256) void k_rdd_adjoint(
     const int tx = threadIdx.x & (RT_X - 1);
     const int ty0 = (threadIdx.x >> 6) * 4;
     const int gx = x0 + tx;
-    float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f;
+    float gd_late[4] = {0.f, 0.f, 0.f, 0.f};
+    if (NS == 2) {
+        // later step.  Core cells by their owner threads, exactly as an NS = 1 launch
+        // would (same partial sums of dL/dk, same dL/dD term) ...
+        float gk[4] = {0.f, 0.f, 0.f, 0.f};
+        if (gx < W) {
+            const float wl = tw(gx - 1, W), wr = tw(gx + 1, W);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int ty = ty0 + k;
+                const int gy = y0 + ty;
+                if (gy >= H) break;
+                const size_t g = (size_t)gy * W + gx;
+                const int lx = tx + NS, ly = ty + NS;
+                float lo, mo;
+                adj_cell(slu[ly][lx], slv[ly][lx], slu[ly][lx - 1], slu[ly][lx + 1], slu[ly - 1][lx], slu[ly + 1][lx],
+                         slv[ly][lx - 1], slv[ly][lx + 1], slv[ly - 1][lx], slv[ly + 1][lx], wl, wr, tw(gy - 1, H),
+                         tw(gy + 1, H), u1[g], v1[g], D[g], p, lo, mo, gd_late[k], gk);
+                mlu[ty + 1][tx + 1] = lo;
+                mlv[ty + 1][tx + 1] = mo;
+            }
+        }
+        // ... and the one-cell ring around the tile (no dL/dD or dL/dk: those cells
+        // belong to the neighbouring tiles); zero outside the grid.
+        constexpr int RING = 2 * (RT_X + 2) + 2 * RT_Y;
+        for (int idx = threadIdx.x; idx < RING; idx += blockDim.x) {
+            int mx, my;
+            if (idx < RT_X + 2) { mx = idx; my = 0; }
+            else if (idx < 2 * (RT_X + 2)) { mx = idx - (RT_X + 2); my = RT_Y + 1; }
+            else if (idx < 2 * (RT_X + 2) + RT_Y) { mx = 0; my = idx - 2 * (RT_X + 2) + 1; }
+            else { mx = RT_X + 1; my = idx - 2 * (RT_X + 2) - RT_Y + 1; }
+            const int cx = x0 + mx - 1, cy = y0 + my - 1;
+            float lo = 0.f, mo = 0.f;
+            if (cx >= 0 && cx < W && cy >= 0 && cy < H) {
+                const size_t g = (size_t)cy * W + cx;
+                const int lx = mx + 1, ly = my + 1;
+                float gd;
+                float gtmp[4] = {0.f, 0.f, 0.f, 0.f};
+                adj_cell(slu[ly][lx], slv[ly][lx], slu[ly][lx - 1], slu[ly][lx + 1], slu[ly - 1][lx], slu[ly + 1][lx],
+                         slv[ly][lx - 1], slv[ly][lx + 1], slv[ly - 1][lx], slv[ly + 1][lx], tw(cx - 1, W),
+                         tw(cx + 1, W), tw(cy - 1, H), tw(cy + 1, H), u1[g], v1[g], D[g], p, lo, mo, gd, gtmp);
+            }
+            mlu[my][mx] = lo;
+            mlv[my][mx] = mo;
+        }
+        // core cells beyond the grid edge of a partial tile hold no lambda
+        for (int idx = threadIdx.x; idx < RT_X * RT_Y; idx += blockDim.x) {
+            const int cy = idx / RT_X, cx = idx - cy * RT_X;
+            if (x0 + cx >= W || y0 + cy >= H) {
+                mlu[cy + 1][cx + 1] = 0.f;
+                mlv[cy + 1][cx + 1] = 0.f;
+            }
+        }
+        tile_gk_add(gk, red, gk_tiles, p.dt);  // contains the barrier that publishes mlu/mlv
+    }
+    float gk[4] = {0.f, 0.f, 0.f, 0.f};
     if (gx < W) {
-        const float wl = (gx - 1 == 0 || gx - 1 == W - 1) ? 2.f : 1.f;
-        const float wr = (gx + 1 == 0 || gx + 1 == W - 1) ? 2.f : 1.f;
+        const float wl = tw(gx - 1, W), wr = tw(gx + 1, W);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int ty = ty0 + k;
             const int gy = y0 + ty;
             if (gy >= H) break;
             const size_t g = (size_t)gy * W + gx;
-            const int lx = tx + 1, ly = ty + 1;
-            const float wt = (gy - 1 == 0 || gy - 1 == H - 1) ? 2.f : 1.f;
-            const float wb = (gy + 1 == 0 || gy + 1 == H - 1) ? 2.f : 1.f;
-            const float l0 = slu[ly][lx], m0 = slv[ly][lx];
-            const float tu = wl * slu[ly][lx - 1] + wr * slu[ly][lx + 1] + wt * slu[ly - 1][lx] + wb * slu[ly + 1][lx];
-            const float tv = wl * slv[ly][lx - 1] + wr * slv[ly][lx + 1] + wt * slv[ly - 1][lx] + wb * slv[ly + 1][lx];
-            const float au = tu - 4.f * l0, av = tv - 4.f * m0;
-            const float uc = u[g], vc = v[g], dc = D[g];
-            const float uv = uc * vc, uu = uc * uc;
-            lun[g] = l0 + p.dt * p.cu * dc * au + p.dt * ((-p.k4 + 2.f * p.k3 * uv) * l0 - 2.f * p.k3 * uv * m0);
-            lvn[g] = m0 + p.dt * p.cv * dc * av + p.dt * (p.k3 * uu * l0 - p.k3 * uu * m0);
-            gD[g] += p.dt * (p.cu * uc * au + p.cv * vc * av);
-            g0 += l0;
-            g1 += m0;
-            g2 += uu * vc * (l0 - m0);
-            g3 -= uc * l0;
+            const float wt = tw(gy - 1, H), wb = tw(gy + 1, H);
+            float lo, mo, gd;
+            if (NS == 2) {
+                const int lx = tx + 1, ly = ty + 1;
+                adj_cell(mlu[ly][lx], mlv[ly][lx], mlu[ly][lx - 1], mlu[ly][lx + 1], mlu[ly - 1][lx], mlu[ly + 1][lx],
+                         mlv[ly][lx - 1], mlv[ly][lx + 1], mlv[ly - 1][lx], mlv[ly + 1][lx], wl, wr, wt, wb, u0[g], v0[g],
+                         D[g], p, lo, mo, gd, gk);
+                gD[g] = (gD[g] + gd_late[k]) + gd;
+            } else {
+                const int lx = tx + 1, ly = ty + 1;
+                adj_cell(slu[ly][lx], slv[ly][lx], slu[ly][lx - 1], slu[ly][lx + 1], slu[ly - 1][lx], slu[ly + 1][lx],
+                         slv[ly][lx - 1], slv[ly][lx + 1], slv[ly - 1][lx], slv[ly + 1][lx], wl, wr, wt, wb, u1[g], v1[g],
+                         D[g], p, lo, mo, gd, gk);
+                gD[g] += gd;
+            }
+            lun[g] = lo;
+            lvn[g] = mo;
         }
     }
-    // block reduction of the four dL/dk partial sums (wave64 shuffles, then LDS)
-    double d0 = g0, d1 = g1, d2 = g2, d3 = g3;
-    for (int off = 32; off > 0; off >>= 1) {
-        d0 += __shfl_down(d0, off, 64);
-        d1 += __shfl_down(d1, off, 64);
-        d2 += __shfl_down(d2, off, 64);
-        d3 += __shfl_down(d3, off, 64);
-    }
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) {
-        red[wave][0] = d0;
-        red[wave][1] = d1;
-        red[wave][2] = d2;
-        red[wave][3] = d3;
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        const double s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
-        const size_t tile = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
-        gk_tiles[tile * 4 + threadIdx.x] += (double)p.dt * s;
-    }
+    tile_gk_add(gk, red, gk_tiles, p.dt);
 }
 
 // dL/dk = sum over tiles (fixed order) -> float[4]
@@ -315,27 +450,30 @@ static RddK make_k(const RddConsts &c) {
 
 dim3 rdd_grid(int W, int H) { return dim3((W + RT_X - 1) / RT_X, (H + RT_Y - 1) / RT_Y); }
 
-hipError_t launch_rdd_step(const float *u, const float *v, const float *D, float *un, float *vn, int W, int H,
-                           const RddConsts &c, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    if (e0) {
-        hipExtLaunchKernelGGL(k_rdd_step, rdd_grid(W, H), dim3(256), 0, s, e0, e1, 0, u, v, D, un, vn, W, H,
-                              make_k(c));
+hipError_t launch_rdd_step(const float *u, const float *v, const float *D, float *umid, float *vmid, float *un,
+                           float *vn, int W, int H, const RddConsts &c, int ns, hipStream_t s, hipEvent_t e0,
+                           hipEvent_t e1) {
+    const RddK k = make_k(c);
+    const dim3 g = rdd_grid(W, H), b(256);
+    if (ns == 2) {
+        if (e0) hipExtLaunchKernelGGL(k_rdd_step<2>, g, b, 0, s, e0, e1, 0, u, v, D, umid, vmid, un, vn, W, H, k);
+        else hipLaunchKernelGGL(k_rdd_step<2>, g, b, 0, s, u, v, D, umid, vmid, un, vn, W, H, k);
     } else {
-        hipLaunchKernelGGL(k_rdd_step, rdd_grid(W, H), dim3(256), 0, s, u, v, D, un, vn, W, H, make_k(c));
+        if (e0) hipExtLaunchKernelGGL(k_rdd_step<1>, g, b, 0, s, e0, e1, 0, u, v, D, umid, vmid, un, vn, W, H, k);
+        else hipLaunchKernelGGL(k_rdd_step<1>, g, b, 0, s, u, v, D, umid, vmid, un, vn, W, H, k);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_rdd_adjoint(const float *lu, const float *lv, const float *u, const float *v, const float *D,
-                              float *lun, float *lvn, float *gD, double *gk_tiles, int W, int H,
-                              const RddConsts &c, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    if (e0) {
-        hipExtLaunchKernelGGL(k_rdd_adjoint, rdd_grid(W, H), dim3(256), 0, s, e0, e1, 0, lu, lv, u, v, D, lun,
-                              lvn, gD, gk_tiles, W, H, make_k(c));
-    } else {
-        hipLaunchKernelGGL(k_rdd_adjoint, rdd_grid(W, H), dim3(256), 0, s, lu, lv, u, v, D, lun, lvn, gD,
-                           gk_tiles, W, H, make_k(c));
-    }
+hipError_t launch_rdd_adjoint(const float *lu, const float *lv, const float *u1, const float *v1, const float *u0,
+                              const float *v0, const float *D, float *lun, float *lvn, float *gD, double *gk_tiles,
+                              int W, int H, const RddConsts &c, int ns, hipStream_t s) {
+    const RddK k = make_k(c);
+    const dim3 g = rdd_grid(W, H), b(256);
+    if (ns == 2)
+        hipLaunchKernelGGL(k_rdd_adjoint<2>, g, b, 0, s, lu, lv, u1, v1, u0, v0, D, lun, lvn, gD, gk_tiles, W, H, k);
+    else
+        hipLaunchKernelGGL(k_rdd_adjoint<1>, g, b, 0, s, lu, lv, u1, v1, u0, v0, D, lun, lvn, gD, gk_tiles, W, H, k);
     return hipGetLastError();
 }
 

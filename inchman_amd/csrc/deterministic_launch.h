@@ -17,12 +17,15 @@ hipError_t launch_det_hom_diffuse(const float *in, float *out, int W, int H, flo
 hipError_t launch_det_inhom_sweep(const float *in, float *out, const float *D, const float *r, int W, int H,
                                   int axis, float dt_h2, float dt_2h, hipStream_t s);
 hipError_t launch_det_rk4(float *state, int A, const RxTable &T, float dt, hipStream_t s);
-hipError_t launch_rdd_step(const float *u, const float *v, const float *D, float *un, float *vn, int W, int H,
-                           const RddConsts &c, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-hipError_t launch_rdd_adjoint(const float *lu, const float *lv, const float *u, const float *v, const float *D,
-                              float *lun, float *lvn, float *gD, double *gk_tiles, int W, int H,
-                              const RddConsts &c, hipStream_t s, hipEvent_t e0 = nullptr,
-                              hipEvent_t e1 = nullptr);
+// ns = 1 or 2 steps per launch (2 needs W, H >= 3); umid/vmid (may be NULL) receive
+// the intermediate state of a two-step launch
+hipError_t launch_rdd_step(const float *u, const float *v, const float *D, float *umid, float *vmid, float *un,
+                           float *vn, int W, int H, const RddConsts &c, int ns, hipStream_t s,
+                           hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// adjoint of ns steps: u1/v1 = state before the later step, u0/v0 = before the earlier one
+hipError_t launch_rdd_adjoint(const float *lu, const float *lv, const float *u1, const float *v1, const float *u0,
+                              const float *v0, const float *D, float *lun, float *lvn, float *gD, double *gk_tiles,
+                              int W, int H, const RddConsts &c, int ns, hipStream_t s);
 hipError_t launch_rdd_gk_reduce(const double *gk_tiles, int ntiles, float *gk, hipStream_t s);
 hipError_t launch_fill_f32(float *p, size_t n, float v, hipStream_t s);
 hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t s);

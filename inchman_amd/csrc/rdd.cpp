@@ -40,6 +40,7 @@ struct gpgmp_rdd {
     DevBuf<double> gk_tiles; // per-tile dL/dk accumulators
     int ntiles = 0;
     bool saved = false;
+    int ns = 2;  // steps per launch (temporal blocking)
     RddConsts c{};
     ~gpgmp_rdd() {
         (void)hipStreamSynchronize(stream);
@@ -109,6 +110,7 @@ int gpgmp_rdd_create(const gpgmp_rdd_params *p, int32_t steps, int32_t ckpt_ever
         r->ntiles = rdd_num_tiles(p->width, p->height);
         r->gk_tiles.alloc((size_t)r->ntiles * 4);
         r->c = rdd_consts(*p);
+        r->ns = (p->width >= 3 && p->height >= 3) ? 2 : 1;
         *out = r.release();
     });
 }
@@ -127,6 +129,15 @@ int gpgmp_rdd_set_rates(gpgmp_rdd *r, const float *k) {
     });
 }
 
+int gpgmp_rdd_set_steps_per_launch(gpgmp_rdd *r, int32_t ns) {
+    return guarded_det([&] {
+        GPGMP_REQUIRE(r && (ns == 1 || ns == 2), GPGMP_E_INVALID, "steps per launch must be 1 or 2");
+        GPGMP_REQUIRE(ns == 1 || (r->p.width >= 3 && r->p.height >= 3), GPGMP_E_INVALID,
+                      "two steps per launch need sides >= 3");
+        r->ns = ns;
+    });
+}
+
 int gpgmp_rdd_forward(gpgmp_rdd *r, const float *D, float *u, float *v, int32_t save) {
     return guarded_det([&] {
         GPGMP_REQUIRE(r && D && u && v, GPGMP_E_INVALID, "null argument");
@@ -135,14 +146,20 @@ int gpgmp_rdd_forward(gpgmp_rdd *r, const float *D, float *u, float *v, int32_t 
         float *bu[2] = {u, r->tmp.get()};
         float *bv[2] = {v, r->tmp.get() + A};
         int cur = 0;
-        for (int n = 0; n < r->steps; ++n) {
+        int n = 0;
+        while (n < r->steps) {
             if (save && n % r->ckpt == 0) {
                 float *c = r->ck.get() + (size_t)(n / r->ckpt) * 2 * A;
                 GPGMP_HIP_CHECK(hipMemcpyAsync(c, bu[cur], A * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
                 GPGMP_HIP_CHECK(hipMemcpyAsync(c + A, bv[cur], A * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
             }
-            GPGMP_HIP_CHECK(launch_rdd_step(bu[cur], bv[cur], D, bu[cur ^ 1], bv[cur ^ 1], W, H, r->c, r->stream));
+            // two steps per launch unless a checkpoint or the end falls in between
+            int k = std::min(r->ns, r->steps - n);
+            if (save) k = std::min(k, r->ckpt - n % r->ckpt);
+            GPGMP_HIP_CHECK(launch_rdd_step(bu[cur], bv[cur], D, nullptr, nullptr, bu[cur ^ 1], bv[cur ^ 1], W, H, r->c, k,
+                                            r->stream));
             cur ^= 1;
+            n += k;
         }
         if (cur) {
             GPGMP_HIP_CHECK(hipMemcpyAsync(u, bu[1], A * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
@@ -169,11 +186,33 @@ int gpgmp_rdd_backward(gpgmp_rdd *r, const float *D, float *lu, float *lv, float
             auto su = [&](int n) -> float * {
                 return (n == n0) ? const_cast<float *>(cku) : r->seg.get() + (size_t)(n - n0) * 2 * A;
             };
-            for (int n = n0 + 1; n < n1; ++n)
-                GPGMP_HIP_CHECK(launch_rdd_step(su(n - 1), su(n - 1) + A, D, su(n), su(n) + A, W, H, r->c, r->stream));
-            for (int n = n1 - 1; n >= n0; --n) {
-                GPGMP_HIP_CHECK(launch_rdd_adjoint(blu[cur], blv[cur], su(n), su(n) + A, D, blu[cur ^ 1], blv[cur ^ 1],
-                                                   gD, r->gk_tiles.get(), W, H, r->c, r->stream));
+            // recompute the states n0+1 .. n1-1 of the segment
+            int n = n0;
+            while (n + 1 < n1) {
+                if (r->ns == 2 && n + 2 < n1) {
+                    GPGMP_HIP_CHECK(launch_rdd_step(su(n), su(n) + A, D, su(n + 1), su(n + 1) + A, su(n + 2),
+                                                    su(n + 2) + A, W, H, r->c, 2, r->stream));
+                    n += 2;
+                } else {
+                    GPGMP_HIP_CHECK(launch_rdd_step(su(n), su(n) + A, D, nullptr, nullptr, su(n + 1), su(n + 1) + A, W, H,
+                                                    r->c, 1, r->stream));
+                    n += 1;
+                }
+            }
+            // walk the segment's steps backwards
+            n = n1 - 1;
+            while (n >= n0) {
+                if (r->ns == 2 && n - 1 >= n0) {
+                    GPGMP_HIP_CHECK(launch_rdd_adjoint(blu[cur], blv[cur], su(n), su(n) + A, su(n - 1), su(n - 1) + A, D,
+                                                       blu[cur ^ 1], blv[cur ^ 1], gD, r->gk_tiles.get(), W, H, r->c, 2,
+                                                       r->stream));
+                    n -= 2;
+                } else {
+                    GPGMP_HIP_CHECK(launch_rdd_adjoint(blu[cur], blv[cur], su(n), su(n) + A, nullptr, nullptr, D,
+                                                       blu[cur ^ 1], blv[cur ^ 1], gD, r->gk_tiles.get(), W, H, r->c, 1,
+                                                       r->stream));
+                    n -= 1;
+                }
                 cur ^= 1;
             }
         }

@@ -55,6 +55,10 @@ class ReactionDiffusion:
         kk = (C.c_float * 4)(*[float(x) for x in k])
         call("gpgmp_rdd_set_rates", self.handle, kk)
 
+    def set_steps_per_launch(self, ns):
+        """1 or 2 Euler steps per kernel launch (results are bit-identical)."""
+        call("gpgmp_rdd_set_steps_per_launch", self.handle, int(ns))
+
     def forward(self, D, u, v, save=True):
         """Advances u, v (in place) by `steps` steps."""
         self._check(D, u, v)

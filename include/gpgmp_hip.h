@@ -178,6 +178,11 @@ int gpgmp_rdd_create(const gpgmp_rdd_params *p, int32_t steps, int32_t ckpt_ever
 int gpgmp_rdd_destroy(gpgmp_rdd *r);
 /* Replaces the rate constants k1..k4 (host floats) for the following forward runs. */
 int gpgmp_rdd_set_rates(gpgmp_rdd *r, const float *k);
+/* Euler steps per kernel launch, 1 or 2 (default 2 when both sides are >= 3): two steps
+ * per launch keep the intermediate state in LDS (2-cell halo) and give results
+ * bit-identical to one step per launch.  No counterpart in the reference (a launch
+ * shape of this build's forward/adjoint). */
+int gpgmp_rdd_set_steps_per_launch(gpgmp_rdd *r, int32_t ns);
 /* Forward: u,v (device, in/out) advanced `steps` fused Euler steps with field D; stores
  * checkpoints when save != 0. */
 int gpgmp_rdd_forward(gpgmp_rdd *r, const float *D, float *u, float *v, int32_t save);

@@ -138,3 +138,49 @@ def test_rdd_autograd_function(gpu):
     assert torch.allclose(D.grad, gD, rtol=1e-6, atol=1e-12)
     assert torch.allclose(k.grad, gk.cpu(), rtol=1e-6)
     assert torch.allclose(u0.grad, gu0, rtol=1e-6, atol=1e-12)
+
+
+def _rdd_run(prob, steps, ckpt, ns):
+    r = _rdd(prob, steps=steps, ckpt=ckpt)
+    r.set_steps_per_launch(ns)
+    D, u, v = _dev(prob["D"]), _dev(prob["u0"]), _dev(prob["v0"])
+    r.forward(D, u, v, save=True)
+    n = u.numel()
+    lu = ((2.0 / n) * (u - _dev(prob["ut"]))).contiguous()
+    lv = ((2.0 / n) * (v - _dev(prob["vt"]))).contiguous()
+    gD, gk, gu0, gv0 = r.backward(D, lu, lv)
+    torch.cuda.synchronize()
+    return [t.cpu().numpy() for t in (u, v, gD, gk, gu0, gv0)]
+
+
+@pytest.mark.parametrize("W,H,steps,ckpt", [(100, 37, 37, 10), (3, 5, 9, 4), (64, 16, 20, 7), (130, 70, 41, 100),
+                                            (257, 19, 3, 1)])
+def test_rdd_two_steps_per_launch_bit_identical(gpu, oracle, W, H, steps, ckpt):
+    """Temporal blocking (two Euler steps per launch, 2-cell LDS halo) gives the same bits
+    as one step per launch, forward and adjoint, on ragged tiles, odd step counts and
+    checkpoint intervals that split the launch pairs; both agree with the FP64 oracle."""
+    prob = configs.rdd_problem(W, H, steps=steps)
+    one = _rdd_run(prob, steps, ckpt, 1)
+    two = _rdd_run(prob, steps, ckpt, 2)
+    for a, b in zip(one, two):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    p = _oracle_params(oracle, prob)
+    uw, vw = oracle.rdd_forward(p, prob["D"].astype(np.float64), prob["u0"], prob["v0"], steps)
+    assert _rel(two[0], uw) <= 1e-5 and _rel(two[1], vw) <= 1e-5
+    _, gDw, gkw = oracle.rdd_grad(p, prob["D"], prob["u0"], prob["v0"], prob["ut"], prob["vt"], steps, ckpt)
+    assert np.linalg.norm(two[2] - gDw) / np.linalg.norm(gDw) <= 1e-4
+    assert np.all(np.abs(two[3] - gkw) <= 1e-4 * np.abs(gkw))
+
+
+def test_rdd_two_cell_sides(gpu, oracle):
+    """Sides of 2 cells run one step per launch (the 2-cell halo needs sides >= 3)."""
+    prob = configs.rdd_problem(2, 7, steps=9)
+    r = _rdd(prob, steps=9, ckpt=4)
+    with pytest.raises(Exception):
+        r.set_steps_per_launch(2)
+    u, v, gD, gk, _, _ = _rdd_run(prob, 9, 4, 1)
+    p = _oracle_params(oracle, prob)
+    uw, vw = oracle.rdd_forward(p, prob["D"].astype(np.float64), prob["u0"], prob["v0"], 9)
+    assert _rel(u, uw) <= 1e-5 and _rel(v, vw) <= 1e-5
+    _, gDw, gkw = oracle.rdd_grad(p, prob["D"], prob["u0"], prob["v0"], prob["ut"], prob["vt"], 9, 4)
+    assert np.linalg.norm(gD - gDw) / np.linalg.norm(gDw) <= 1e-4
