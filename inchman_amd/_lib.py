@@ -106,8 +106,12 @@ SIGNATURES = {
     "gpgmp_solver_run_continue": [_P, _I, _I, _pF],
     "gpgmp_solver_set_launch": [_P, _I, _I],
     "gpgmp_solver_kernel_times": [_P, _I, _pD, C.POINTER(C.c_int64), _pD],
-    "gpgmp_solver_set_event_budget": [_P, _U],
-    "gpgmp_batch_set_event_budget": [_P, _U],
+    "gpgmp_solver_set_event_budget": [_P, _U, _I],
+    "gpgmp_solver_set_lanes_per_cell": [_P, _I, _I],
+    "gpgmp_solver_get_launch": [_P, _pI, _pI, _pI, _pI],
+    "gpgmp_batch_set_lanes_per_cell": [_P, _I, _I],
+    "gpgmp_batch_get_launch": [_P, _pI, _pI, _pI, _pI],
+    "gpgmp_batch_set_event_budget": [_P, _U, _I],
     "gpgmp_solver_synchronize": [_P],
     "gpgmp_batch_create": [_P, _I, _pI, _pI, _pU, _P, C.POINTER(_P)],
     "gpgmp_batch_destroy": [_P],

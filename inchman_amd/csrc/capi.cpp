@@ -302,11 +302,29 @@ int gpgmp_solver_set_launch(gpgmp_solver *s, int32_t block, int32_t cpt) {
     });
 }
 
-int gpgmp_solver_set_event_budget(gpgmp_solver *s, uint32_t budget) {
+int gpgmp_solver_set_lanes_per_cell(gpgmp_solver *s, int32_t ssa_lanes, int32_t sweep_lanes) {
     return guarded([&] {
-        GPGMP_REQUIRE(s && budget >= 1, GPGMP_E_INVALID, "bad argument");
+        GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        s->eng->set_launch(s->eng->block(), s->eng->cpt(), ssa_lanes, sweep_lanes);
+    });
+}
+
+int gpgmp_solver_get_launch(gpgmp_solver *s, int32_t *block, int32_t *cpt, int32_t *ssa_lanes, int32_t *sweep_lanes) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && block && cpt && ssa_lanes && sweep_lanes, GPGMP_E_INVALID, "null argument");
+        *block = s->eng->block();
+        *cpt = s->eng->cpt();
+        *ssa_lanes = s->eng->grp();
+        *sweep_lanes = s->eng->grp_sweep();
+    });
+}
+
+int gpgmp_solver_set_event_budget(gpgmp_solver *s, uint32_t budget, int32_t below_active) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && budget >= 1 && below_active >= 0 && below_active <= 65, GPGMP_E_INVALID,
+                      "bad argument");
         GPGMP_HIP_CHECK(hipStreamSynchronize(s->stream.s));
-        s->eng->set_hot_budget(budget);
+        s->eng->set_hot_budget(budget, below_active);
     });
 }
 
@@ -384,11 +402,29 @@ int gpgmp_batch_set_launch(gpgmp_batch *b, int32_t block, int32_t cpt) {
     });
 }
 
-int gpgmp_batch_set_event_budget(gpgmp_batch *b, uint32_t budget) {
+int gpgmp_batch_set_lanes_per_cell(gpgmp_batch *b, int32_t ssa_lanes, int32_t sweep_lanes) {
     return guarded([&] {
-        GPGMP_REQUIRE(b && budget >= 1, GPGMP_E_INVALID, "bad argument");
+        GPGMP_REQUIRE(b, GPGMP_E_INVALID, "null argument");
+        b->eng->set_launch(b->eng->block(), b->eng->cpt(), ssa_lanes, sweep_lanes);
+    });
+}
+
+int gpgmp_batch_get_launch(gpgmp_batch *b, int32_t *block, int32_t *cpt, int32_t *ssa_lanes, int32_t *sweep_lanes) {
+    return guarded([&] {
+        GPGMP_REQUIRE(b && block && cpt && ssa_lanes && sweep_lanes, GPGMP_E_INVALID, "null argument");
+        *block = b->eng->block();
+        *cpt = b->eng->cpt();
+        *ssa_lanes = b->eng->grp();
+        *sweep_lanes = b->eng->grp_sweep();
+    });
+}
+
+int gpgmp_batch_set_event_budget(gpgmp_batch *b, uint32_t budget, int32_t below_active) {
+    return guarded([&] {
+        GPGMP_REQUIRE(b && budget >= 1 && below_active >= 0 && below_active <= 65, GPGMP_E_INVALID,
+                      "bad argument");
         GPGMP_HIP_CHECK(hipStreamSynchronize(b->stream.s));
-        b->eng->set_hot_budget(budget);
+        b->eng->set_hot_budget(budget, below_active);
     });
 }
 
@@ -407,7 +443,8 @@ int gpgmp_batch_reset(gpgmp_batch *b) {
     });
 }
 
-// Project-written autotuner: every candidate (block, cells per thread) runs the same
+// Project-written autotuner: every candidate (block, cells per thread, lanes per cell)
+// runs the same
 // trial steps from the same saved state with a fresh schedule; the fastest is kept and
 // the state and schedule are restored.  Results never depend on the shape (RNG keys
 // carry no tile or batch offset), only the speed does.
@@ -422,14 +459,16 @@ int gpgmp_batch_autotune(gpgmp_batch *b, int32_t trial_steps, int32_t *block_out
         GPGMP_HIP_CHECK(hipStreamSynchronize(st));
         static const int blocks[] = {64, 128, 256};
         static const int cpts[] = {1, 2, 4};
+        static const int grps[][2] = {{1, 1}, {1, 4}, {4, 4}};
         double best = 1e300;
-        int bb = e.block(), bc = e.cpt();
+        int bb = e.block(), bc = e.cpt(), bg = e.grp(), bgs = e.grp_sweep();
         hipEvent_t t0, t1;
         GPGMP_HIP_CHECK(hipEventCreate(&t0));
         GPGMP_HIP_CHECK(hipEventCreate(&t1));
+        for (const auto &g : grps)
         for (int blk : blocks)
             for (int c : cpts) {
-                e.set_launch(blk, c);
+                e.set_launch(blk, c, g[0], g[1]);
                 GPGMP_HIP_CHECK(hipMemcpyAsync(e.state(), saved.get(), n * sizeof(int32_t),
                                                hipMemcpyDeviceToDevice, st));
                 GmpRunner trial(e, b->dtx, b->dty, b->seed0s);
@@ -443,11 +482,13 @@ int gpgmp_batch_autotune(gpgmp_batch *b, int32_t trial_steps, int32_t *block_out
                     best = ms;
                     bb = blk;
                     bc = c;
+                    bg = g[0];
+                    bgs = g[1];
                 }
             }
         (void)hipEventDestroy(t0);
         (void)hipEventDestroy(t1);
-        e.set_launch(bb, bc);
+        e.set_launch(bb, bc, bg, bgs);
         GPGMP_HIP_CHECK(hipMemcpyAsync(e.state(), saved.get(), n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
         GPGMP_HIP_CHECK(hipStreamSynchronize(st));
         if (block_out) *block_out = bb;

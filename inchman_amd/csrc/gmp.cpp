@@ -231,7 +231,10 @@ GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vecto
     }
     // deferred busy cells: at most one entry per cell of the launch, capped at 2^20
     hot_.cap = (uint32_t)std::min<int64_t>(total_cells_, (int64_t)1 << 20);
-    hot_.budget = 32;  // measured best for the 256x256 Schnakenberg run (16..32)
+    // deferral of busy cells: after 16 events, once fewer than 16 lanes of the wave run
+    // (measured best on the 256x256 Schnakenberg run among budgets 8..64, thresholds 4..65)
+    hot_.budget = 16;
+    hot_.below_active = 16;
     hot_e_.alloc(hot_.cap);
     hot_count_.alloc(2);
     hot_.e = hot_e_.get();
@@ -244,27 +247,40 @@ GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vecto
     GPGMP_HIP_CHECK(hipMemsetAsync(lost_[0].get(), 0, lo * sizeof(int32_t), stream_));
     GPGMP_HIP_CHECK(hipMemsetAsync(lost_[1].get(), 0, lo * sizeof(int32_t), stream_));
     GPGMP_HIP_CHECK(hipMemsetAsync(state_.get(), 0, so * sizeof(int32_t), stream_));
+    // small grids leave most SIMDs with a single wave at one lane per cell: below 2^18
+    // cells the sweep-only launches give each cell 4 lanes (their particle loops split
+    // exactly); the SSA launches keep one lane per cell, whose serial event chain costs
+    // the least VALU work per event
+    grp_ = 1;
+    grp_sweep_ = (total_cells_ < ((int64_t)1 << 18)) ? 4 : 1;
     build_tiles();
 }
 
 void GmpEngine::build_tiles() {
-    std::vector<Tile> tiles;
-    const int per = block_ * cpt_;
-    for (size_t r = 0; r < reals_.size(); ++r) {
-        const int A = reals_[r].W * reals_[r].H;
-        for (int c = 0; c < A; c += per) tiles.push_back(Tile{(int32_t)r, (int32_t)c});
+    for (int k = 0; k < 2; ++k) {
+        std::vector<Tile> tiles;
+        const int per = block_ / (k ? 4 : 1) * cpt_;
+        for (size_t r = 0; r < reals_.size(); ++r) {
+            const int A = reals_[r].W * reals_[r].H;
+            for (int c = 0; c < A; c += per) tiles.push_back(Tile{(int32_t)r, (int32_t)c});
+        }
+        ntiles_[k] = (int)tiles.size();
+        tiles_dev_[k].alloc(tiles.size());
+        GPGMP_HIP_CHECK(hipMemcpyAsync(tiles_dev_[k].get(), tiles.data(), tiles.size() * sizeof(Tile),
+                                       hipMemcpyHostToDevice, stream_));
     }
-    ntiles_ = (int)tiles.size();
-    tiles_dev_.alloc(tiles.size());
-    GPGMP_HIP_CHECK(hipMemcpyAsync(tiles_dev_.get(), tiles.data(), tiles.size() * sizeof(Tile),
-                                   hipMemcpyHostToDevice, stream_));
     GPGMP_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
-void GmpEngine::set_launch(int block, int cpt) {
+void GmpEngine::set_launch(int block, int cpt, int grp, int grp_sweep) {
     GPGMP_REQUIRE(block >= 64 && block <= 256 && block % 64 == 0, GPGMP_E_INVALID,
                   "block must be 64, 128, 192 or 256");
     GPGMP_REQUIRE(cpt >= 1 && cpt <= 16, GPGMP_E_INVALID, "cells per thread must be in [1, 16]");
+    GPGMP_REQUIRE((grp == 1 || grp == 4) && (grp_sweep == 1 || grp_sweep == 4), GPGMP_E_INVALID,
+                  "lanes per cell must be 1 or 4");
+    ++shape_version_;
+    grp_ = grp;
+    grp_sweep_ = grp_sweep;
     if (block == block_ && cpt == cpt_) return;
     GPGMP_HIP_CHECK(hipStreamSynchronize(stream_));
     block_ = block;
@@ -313,8 +329,10 @@ void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cu
     a.lost_out = lost_out;
     a.prob = inh_ ? prob_[mode == 2 ? 1 : 0].get() : nullptr;
     a.reals = reals_dev_.get();
-    a.tiles = tiles_dev_.get();
-    a.ntiles = ntiles_;
+    a.grp = (mode == 2) ? grp_sweep_ : grp_;
+    a.grp_sweep = (a.grp == 4) ? 4 : grp_sweep_;
+    a.tiles = tiles_dev_[a.grp == 4].get();
+    a.ntiles = ntiles_[a.grp == 4];
     a.plan = plan;
     a.seeds_cur = seeds_cur;
     a.seeds_prev = seeds_prev;
@@ -343,6 +361,7 @@ struct GmpRunner::Chunk {
     hipEvent_t copied = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
+    uint64_t graph_version = 0;  // engine shape version the graph was captured with
     Chunk(int cap_, int nreal_) : cap(cap_), nreal(nreal_) {
         GPGMP_HIP_CHECK(hipHostMalloc((void **)&hplan, sizeof(LaunchPlan) * (2 * cap + 1)));
         GPGMP_HIP_CHECK(hipHostMalloc((void **)&hseed, sizeof(int32_t) * (size_t)(cap + 1) * nreal));
@@ -465,7 +484,7 @@ void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
         bool v;
         ~TimingRestore() { t = v; }
     } timing_restore{timing_, saved_timing};
-    if (!c.graph || c.graph_steps != nsteps) {
+    if (!c.graph || c.graph_steps != nsteps || c.graph_version != eng_.shape_version()) {
         if (c.graph) {
             GPGMP_HIP_CHECK(hipGraphExecDestroy(c.graph));
             c.graph = nullptr;
@@ -497,6 +516,7 @@ void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
         (void)hipStreamDestroy(cap);
         GPGMP_HIP_CHECK(ie);
         c.graph_steps = nsteps;
+        c.graph_version = eng_.shape_version();
     }
     GPGMP_HIP_CHECK(hipGraphLaunch(c.graph, eng_.stream()));
 }

@@ -153,12 +153,23 @@ class GmpEngine {
     hipStream_t stream() const { return stream_; }
     void set_stream(hipStream_t s) { stream_ = s; }
 
-    void set_launch(int block, int cpt);
+    // threads per block, cells per thread, lanes per cell (1 or 4) of the launches
+    // with an SSA (modes 0, 1) and of the sweep-only launches (mode 2)
+    void set_launch(int block, int cpt, int grp_ssa, int grp_sweep);
+    void set_launch(int block, int cpt) { set_launch(block, cpt, grp_, grp_sweep_); }
     // SSA events a lane runs before its cell moves to the hot-cell kernel
-    void set_hot_budget(uint32_t b) { hot_.budget = b; }
+    void set_hot_budget(uint32_t b, int below_active) {
+        hot_.budget = b;
+        hot_.below_active = below_active;
+        ++shape_version_;
+    }
+    // changes with every launch-shape or budget change (captured graphs bake them in)
+    uint64_t shape_version() const { return shape_version_; }
     uint32_t hot_budget() const { return hot_.budget; }
     int block() const { return block_; }
     int cpt() const { return cpt_; }
+    int grp() const { return grp_; }
+    int grp_sweep() const { return grp_sweep_; }
 
     void set_state(int r, const int32_t *host);
     void get_state(int r, int32_t *host);
@@ -186,12 +197,13 @@ class GmpEngine {
     DevBuf<int32_t> lost_[2];
     DevBuf<float2> prob_[2];
     DevBuf<RealInfo> reals_dev_;
-    DevBuf<Tile> tiles_dev_;
+    DevBuf<Tile> tiles_dev_[2];  // tiles for 1 and for 4 lanes per cell
     DevBuf<HotEntry> hot_e_;
     DevBuf<uint32_t> hot_count_;
     HotList hot_{};
-    int ntiles_ = 0;
-    int block_ = 256, cpt_ = 1;
+    int ntiles_[2] = {0, 0};
+    int block_ = 256, cpt_ = 1, grp_ = 1, grp_sweep_ = 1;
+    uint64_t shape_version_ = 0;
     int64_t total_cells_ = 0;
     hipStream_t stream_;
 };

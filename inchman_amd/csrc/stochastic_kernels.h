@@ -1,0 +1,718 @@
+// Stochastic GMP kernels for gfx950 (device code, included by the per-mode
+// translation units): Gillespie SSA and stochastic diffusion splitting (homogeneous,
+// and directionally split inhomogeneous drift-diffusion).
+//
+// One kernel, k_gmp_step, does the work of a host-loop launch for any number of
+// realizations (grids).  Per cell it
+//   1. gathers the outflow the neighbours computed in the previous launch
+//      (homogeneous_stochastic_updateState, StochasticHomogeneousDiffusion.cl:105-124;
+//      inhomogeneous_updateStateX/Y, StochasticInhomogeneousDriftDiffusion.cl:687-723),
+//      plus homogeneous source-boundary inflow (:128-183);
+//   2. runs the Gillespie direct method over [0, deltat) (performGillespie,
+//      Gillespie.cl:4-106) with the species counts held in VGPRs;
+//   3. computes this launch's outflow for the due species
+//      (homogeneous_stochastic_diffuse :8-103; inhomogeneous_diffuseX/Y :426-685),
+//      subtracts it and writes it to the outflow buffer.
+// Deferring step 3's gather into the next launch removes the reference's separate
+// updateState launches: a host-loop iteration is one launch (homogeneous) or two
+// (inhomogeneous: X sweeps, then Y sweeps).  RNG keys never contain a batch or
+// tile offset, so results do not depend on the launch shape or batch layout.
+#pragma once
+
+#include <hip/hip_ext.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "gpgmp_numeric.h"
+#include "stochastic_launch.h"
+#include "stochastic_types.h"
+
+namespace gpgmp {
+
+enum : int { MODE_HOM = 0, MODE_INH_X = 1, MODE_INH_Y = 2 };
+
+
+
+template <int S>
+__device__ __forceinline__ int32_t sel(const int32_t (&x)[S], int j) {
+    int32_t v = x[0];
+#pragma unroll
+    for (int k = 1; k < S; ++k) v = (j == k) ? x[k] : v;
+    return v;
+}
+
+// _h[i+1] = _h[0] + <law_i>; int literal -> int product then float add,
+// double literal -> double product chain rounded once (Solver.cpp:849-857).
+template <int S>
+__device__ __forceinline__ float add_law(const RxTable &T, int i, float h0, const int32_t (&x)[S]) {
+    const int nf = T.nf[i];
+    if ((T.lit_is_int >> i) & 1u) {
+        uint32_t v = (uint32_t)T.lit_int[i];
+        for (int f = 0; f < nf; ++f) v *= (uint32_t)(sel(x, T.fs[i][f]) - (int32_t)T.fo[i][f]);
+        return h0 + (float)(int32_t)v;
+    }
+    double v = T.lit_dbl[i];
+    for (int f = 0; f < nf; ++f) v = v * (double)(sel(x, T.fs[i][f]) - (int32_t)T.fo[i][f]);
+    return (float)((double)h0 + v);
+}
+
+// Cumulative hazards _h[1..R] and the total _h[0] (Solver.cpp:843-857).
+template <int S>
+__device__ __forceinline__ void hazards(const RxTable &T, const int32_t (&x)[S], float (&h)[kMaxReactions + 1]) {
+    h[0] = 0.f;
+#pragma unroll
+    for (int i = 0; i < kMaxReactions; ++i) {
+        if (i < T.R) {
+            h[i + 1] = add_law(T, i, h[0], x);
+            h[0] = h[i + 1];
+        }
+    }
+}
+
+// Reaction selection and stoichiometric update (Solver.cpp:867-903): reaction 0 fires
+// when r2 < h[1], reaction i when h[i] <= r2 < h[i+1]; nothing fires when rounding
+// puts r2 at or above h[R].
+template <int S>
+__device__ __forceinline__ void fire(const RxTable &T, const float (&h)[kMaxReactions + 1], float r2,
+                                     int32_t (&x)[S]) {
+#pragma unroll
+    for (int i = 0; i < kMaxReactions; ++i) {
+        if (i < T.R) {
+            const bool f = (i == 0) ? (r2 < h[1]) : ((r2 >= h[i]) && (r2 < h[i + 1]));
+#pragma unroll
+            for (int j = 0; j < S; ++j) x[j] += f ? (int32_t)T.net[i][j] : 0;
+        }
+    }
+}
+
+// performGillespie for one cell (Gillespie.cl:41-99): key {cell, 0xdecafbad+seed},
+// counter {++n, 0xf00dcafe}; r1 -> waiting time, r2 (times h0 in double) -> reaction.
+// Runs from (time, ctr) until deltat, or until `budget` events have been drawn and fewer
+// than `below` lanes of the wave still run (a wave of busy cells keeps going; a lone
+// busy cell stops holding its wave): returns false when it stops early (time < deltat),
+// so the cell can be resumed bit-exactly.
+template <int S>
+__device__ __forceinline__ bool ssa_cell(int32_t (&x)[S], uint32_t cell, int32_t seed, float deltat,
+                                         const RxTable &T, uint32_t budget, int below, float &time,
+                                         uint32_t &ctr) {
+    const uint32_t k1 = kKeyBase + (uint32_t)seed;
+    while (time < deltat) {
+        float h[kMaxReactions + 1];
+        hazards<S>(T, x, h);
+        if (h[0] == 0.f) return true;
+        if (ctr == kMaxEventsPerCall) return true;  // runaway guard (never reached by sane models)
+        if (ctr >= budget && __popcll(__ballot(1)) < below) return false;
+        ++ctr;
+        const u32x2 w = threefry2x32_20(ctr, kCtrHi, cell, k1);
+        const float r1 = u01(w.x);
+        const float r2 = (float)(u01_d(w.y) * (double)h[0]);
+        const float dt = log_f(1.f / r1) / h[0];
+        time += dt;
+        if (time < deltat) fire<S>(T, h, r2, x);
+    }
+    return true;
+}
+
+// Lane groups.  With G > 1, G consecutive lanes of a wave (G divides 64) share one
+// cell: the random words and logarithms of G successive SSA events are drawn at once
+// (lane g: counter ctr+1+g) and broadcast through DPP, the serial event chain runs in
+// all lanes of the group, and the particle loops split their Threefry calls over the
+// lanes (call k by lane k % G) with the counts summed -- integer sums, so the results
+// are the same bits as with one lane per cell.
+constexpr int qperm(int a, int b, int c, int d) { return a | (b << 2) | (c << 4) | (d << 6); }
+
+template <int G, int J>
+__device__ __forceinline__ int32_t grp_bcast(int32_t v) {
+    static_assert(G == 1 || G == 2 || G == 4, "lane groups of 1, 2 or 4");
+    if constexpr (G == 1) return v;
+    else if constexpr (G == 2) return __builtin_amdgcn_update_dpp(0, v, qperm(J, J, 2 + J, 2 + J), 0xf, 0xf, false);
+    else return __builtin_amdgcn_update_dpp(0, v, qperm(J, J, J, J), 0xf, 0xf, false);
+}
+
+template <int G, int J = 0>
+__device__ __forceinline__ void grp_gather(float v, double d, float (&vs)[G], double (&ds)[G]) {
+    if constexpr (J < G) {
+        vs[J] = __int_as_float(grp_bcast<G, J>(__float_as_int(v)));
+        const uint64_t b = (uint64_t)__double_as_longlong(d);
+        const uint32_t lo = (uint32_t)grp_bcast<G, J>((int32_t)(uint32_t)b);
+        const uint32_t hi = (uint32_t)grp_bcast<G, J>((int32_t)(uint32_t)(b >> 32));
+        ds[J] = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+        grp_gather<G, J + 1>(v, d, vs, ds);
+    }
+}
+
+template <int G>
+__device__ __forceinline__ int32_t grp_sum(int32_t v) {
+    if constexpr (G >= 2) v += __builtin_amdgcn_update_dpp(0, v, qperm(1, 0, 3, 2), 0xf, 0xf, false);
+    if constexpr (G >= 4) v += __builtin_amdgcn_update_dpp(0, v, qperm(2, 3, 0, 1), 0xf, 0xf, false);
+    return v;
+}
+
+// performGillespie for one cell with G lanes (identical event sequence to ssa_cell).
+template <int S, int G>
+__device__ __forceinline__ bool ssa_cell_g(int32_t (&x)[S], uint32_t cell, int32_t seed, float deltat,
+                                           const RxTable &T, uint32_t budget, int below, float &time,
+                                           uint32_t &ctr) {
+    if constexpr (G == 1) {
+        return ssa_cell<S>(x, cell, seed, deltat, T, budget, below, time, ctr);
+    } else {
+        const uint32_t k1 = kKeyBase + (uint32_t)seed;
+        const uint32_t g = threadIdx.x & (G - 1);
+        if (!(time < deltat)) return true;
+        while (true) {
+            const u32x2 w = threefry2x32_20(ctr + 1u + g, kCtrHi, cell, k1);
+            float Ls[G];
+            double Us[G];
+            grp_gather<G>(log_f(1.f / u01(w.x)), u01_d(w.y), Ls, Us);
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                float h[kMaxReactions + 1];
+                hazards<S>(T, x, h);
+                if (h[0] == 0.f) return true;
+                if (ctr == kMaxEventsPerCall) return true;  // runaway guard
+                if (ctr >= budget && __popcll(__ballot(1)) < below) return false;
+                ++ctr;
+                const float r2 = (float)(Us[j] * (double)h[0]);
+                const float dt = Ls[j] / h[0];
+                time += dt;
+                if (!(time < deltat)) return true;
+                fire<S>(T, h, r2, x);
+            }
+        }
+    }
+}
+
+// Per-particle categorical hop of homogeneous_stochastic_diffuse
+// (StochasticHomogeneousDiffusion.cl:68-92): counts of u in (stay,p1], (p1,p2],
+// (p2,p3], (p3,p4] are the hops to -x, +x, -y, +y.  Call k (counter k+1) serves
+// particles 2k (word x) and 2k+1 (word y).
+template <int G>
+__device__ __forceinline__ void hom_outflow(int n, uint32_t k0, uint32_t k1, const HomProbs &P,
+                                            int32_t (&cnt)[4]) {
+    int32_t le0 = 0, le1 = 0, le2 = 0, le3 = 0, le4 = 0;
+    const int calls = (n + 1) >> 1;
+    for (int k = (int)(threadIdx.x & (G - 1)); k < calls; k += G) {
+        const u32x2 w = threefry2x32_20((uint32_t)k + 1u, kCtrHi, k0, k1);
+        float u = u01(w.x);
+        le0 += u <= P.stay;
+        le1 += u <= P.p1;
+        le2 += u <= P.p2;
+        le3 += u <= P.p3;
+        le4 += u <= P.p4;
+        if (2 * k + 1 < n) {
+            u = u01(w.y);
+            le0 += u <= P.stay;
+            le1 += u <= P.p1;
+            le2 += u <= P.p2;
+            le3 += u <= P.p3;
+            le4 += u <= P.p4;
+        }
+    }
+    le0 = grp_sum<G>(le0);
+    le1 = grp_sum<G>(le1);
+    le2 = grp_sum<G>(le2);
+    le3 = grp_sum<G>(le3);
+    le4 = grp_sum<G>(le4);
+    cnt[0] = le1 - le0;
+    cnt[1] = le2 - le1;
+    cnt[2] = le3 - le2;
+    cnt[3] = le4 - le3;
+}
+
+// Per-particle Bernoulli hop of inhomogeneous_diffuseX/Y
+// (StochasticInhomogeneousDriftDiffusion.cl:524-547): dir = +[u <= p1], counted
+// when u <= p2.  pr = (min(p1,p2), p2): plus = #(u <= pr.x), minus = #(u <= pr.y) - plus.
+template <int G>
+__device__ __forceinline__ void inh_outflow(int n, uint32_t k0, uint32_t k1, float2 pr, int32_t &minus,
+                                            int32_t &plus) {
+    int32_t a = 0, b = 0;
+    const int calls = (n + 1) >> 1;
+    for (int k = (int)(threadIdx.x & (G - 1)); k < calls; k += G) {
+        const u32x2 w = threefry2x32_20((uint32_t)k + 1u, kCtrHi, k0, k1);
+        float u = u01(w.x);
+        a += u <= pr.x;
+        b += u <= pr.y;
+        if (2 * k + 1 < n) {
+            u = u01(w.y);
+            a += u <= pr.x;
+            b += u <= pr.y;
+        }
+    }
+    a = grp_sum<G>(a);
+    b = grp_sum<G>(b);
+    plus = a;
+    minus = b - a;
+}
+
+// homogeneous_stochastic_source_boundary (StochasticHomogeneousDiffusion.cl:128-183)
+// for one boundary position d: `count` draws with key {d, 0xdecafbad+seed}, each adding
+// a particle when stay < u <= p1.
+__device__ __forceinline__ int32_t source_inflow(int count, uint32_t d, uint32_t k1,
+                                                 const HomProbs &P) {
+    int32_t add = 0;
+    uint32_t ctr = 0;
+    for (int i = 0; i < count; i += 2) {
+        ++ctr;
+        const u32x2 w = threefry2x32_20(ctr, kCtrHi, d, k1);
+        float u = u01(w.x);
+        add += (u > P.stay) && (u <= P.p1);
+        if (i + 1 < count) {
+            u = u01(w.y);
+            add += (u > P.stay) && (u <= P.p1);
+        }
+    }
+    return add;
+}
+
+// Block -> tile with XCD-aware placement: hardware dispatches block b to XCD b % 8,
+// so XCD k gets a contiguous run of tiles and neighbour gathers stay in its L2.
+__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
+    const int per = (ntiles + 7) >> 3;
+    const int t = (b & 7) * per + (b >> 3);
+    return t;
+}
+
+
+// Outflow of species s from cell c for this launch's sweep, computed by a group of G
+// lanes (all lanes get the counts; the group's first lane writes them).  Returns the
+// particles that left.
+template <int MODE, int G>
+__device__ __forceinline__ int32_t cell_outflow(int32_t n, int s, int c, int W, int H, int A, int32_t seed,
+                                                const LaunchPlan &pl, int4 bm, const RealInfo &ri,
+                                                const float2 *__restrict__ prob, int32_t *__restrict__ lout,
+                                                const HomProbs &P) {
+    constexpr int ND = (MODE == MODE_HOM) ? 4 : 2;
+    const bool lead = (threadIdx.x & (G - 1)) == 0;
+    const int gx = c % W, gy = c / W;
+    const int m0 = (gx == 0) ? bm.x : 1, m1 = (gx == W - 1) ? bm.y : 1;
+    const int m2 = (gy == 0) ? bm.z : 1, m3 = (gy == H - 1) ? bm.w : 1;
+    const uint32_t k1 = kKeyBase + (uint32_t)seed + (uint32_t)pl.out_off[s];
+    int32_t *l = lout + (size_t)s * ND * A;
+    if constexpr (MODE == MODE_HOM) {
+        int32_t cnt[4];
+        hom_outflow<G>(n, (uint32_t)c, k1, P, cnt);
+        const int32_t l0 = m0 * cnt[0], l1 = m1 * cnt[1], l2 = m2 * cnt[2], l3 = m3 * cnt[3];
+        if (lead) {
+            l[0 * (size_t)A + c] = l0;
+            l[1 * (size_t)A + c] = l1;
+            l[2 * (size_t)A + c] = l2;
+            l[3 * (size_t)A + c] = l3;
+        }
+        return l0 + l1 + l2 + l3;
+    } else {
+        const float2 pr = prob[ri.prob_off + (size_t)s * A + c];
+        int32_t mi, pl_;
+        inh_outflow<G>(n, (uint32_t)(c + s * A), k1, pr, mi, pl_);
+        const int32_t mm = (MODE == MODE_INH_X) ? m0 : m2;
+        const int32_t mp = (MODE == MODE_INH_X) ? m1 : m3;
+        const int32_t lm = mm * mi, lp = mp * pl_;
+        if (lead) {
+            l[0 * (size_t)A + c] = lm;
+            l[1 * (size_t)A + c] = lp;
+        }
+        return lm + lp;
+    }
+}
+
+// G lanes per cell for the SSA and GS (= G, or 4 with G = 1) for the outflow.
+template <int S, int MODE, int G, int GS>
+__global__ __launch_bounds__(256) void k_gmp_step(
+    int32_t *__restrict__ state, const int32_t *__restrict__ lost_in, int32_t *__restrict__ lost_out,
+    const float2 *__restrict__ prob, const RealInfo *__restrict__ reals,
+    const Tile *__restrict__ tiles, int ntiles, const LaunchPlan *__restrict__ plan,
+    const int32_t *__restrict__ seeds_cur, const int32_t *__restrict__ seeds_prev, RxTable T,
+    HomProbs P, int4 bm, int cpt, HotList hot) {
+    // the grid has 8*ceil(ntiles/8) blocks so that the remap is a bijection
+    const int t = xcd_tile((int)blockIdx.x, ntiles);
+    if (t >= ntiles) return;
+    const Tile tile = tiles[t];
+    const RealInfo ri = reals[tile.r];
+    const LaunchPlan pl = *plan;
+    const int32_t seed = seeds_cur[tile.r];
+    const int W = ri.W, H = ri.H;
+    const int A = W * H;
+    constexpr int ND = (MODE == MODE_HOM) ? 4 : 2;
+    int32_t *st = state + ri.state_off;
+    const int32_t *lin = lost_in + ri.lost_off;
+    int32_t *lout = lost_out + ri.lost_off;
+    const bool any_src = (ri.src[0] | ri.src[1] | ri.src[2] | ri.src[3]) != 0;
+    const int g = (int)(threadIdx.x & (G - 1));        // lane in the cell's group
+    const int cpb = (int)blockDim.x / G;               // cells per block and round
+
+    for (int k = 0; k < cpt; ++k) {
+        const int c = tile.cell0 + k * cpb + (int)threadIdx.x / G;
+        if (c >= A) break;
+        const int gx = c % W, gy = c / W;
+        int32_t x[S];
+        // loads: every species for launches with an SSA, only touched species otherwise
+        const uint32_t touched = pl.gather_mask | pl.out_mask;
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            x[s] = (pl.deltat > 0.f || ((touched >> s) & 1u)) ? st[(size_t)s * A + c] : 0;
+
+        // 1. gather the previous launch's outflow
+        if (pl.gather_mask) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if (!((pl.gather_mask >> s) & 1u)) continue;
+                const int32_t *l = lin + (size_t)s * ND * A;
+                int32_t v = x[s];
+                if (MODE == MODE_HOM) {
+                    if (gx > 0) v += l[1 * (size_t)A + c - 1];
+                    if (gx < W - 1) v += l[0 * (size_t)A + c + 1];
+                    if (gy > 0) v += l[3 * (size_t)A + c - W];
+                    if (gy < H - 1) v += l[2 * (size_t)A + c + W];
+                } else if (MODE == MODE_INH_X) {  // gathers the Y outflow of the previous launch
+                    if (gy > 0) v += l[1 * (size_t)A + c - W];
+                    if (gy < H - 1) v += l[0 * (size_t)A + c + W];
+                } else {                          // gathers the X outflow of the previous launch
+                    if (gx > 0) v += l[1 * (size_t)A + c - 1];
+                    if (gx < W - 1) v += l[0 * (size_t)A + c + 1];
+                }
+                if (MODE == MODE_HOM && any_src) {
+                    const uint32_t k1s = kKeyBase + (uint32_t)seeds_prev[tile.r] + (uint32_t)pl.gather_off;
+                    if (ri.src[0] > 0 && gx == 0) v += source_inflow(ri.src[0], (uint32_t)gy, k1s, P);
+                    if (ri.src[1] > 0 && gx == W - 1) v += source_inflow(ri.src[1], (uint32_t)gy, k1s, P);
+                    if (ri.src[2] > 0 && gy == 0) v += source_inflow(ri.src[2], (uint32_t)gx, k1s, P);
+                    if (ri.src[3] > 0 && gy == H - 1) v += source_inflow(ri.src[3], (uint32_t)gx, k1s, P);
+                }
+                x[s] = v;
+            }
+        }
+
+        // 2. reactions.  A cell stops after hot.budget events and is handed to the
+        //    hot-cell kernel, so a few busy cells do not hold a whole wave (and the GPU).
+        bool deferred = false;
+        if (MODE != MODE_INH_Y && pl.deltat > 0.f && T.R > 0) {
+            float time = 0.f;
+            uint32_t ctr = 0;
+            const bool fin =
+                ssa_cell_g<S, G>(x, (uint32_t)c, seed, pl.deltat, T, hot.budget, hot.below_active, time, ctr);
+            // one list entry per deferred cell, pushed by the group's first lane
+            const uint64_t dm = __ballot(!fin && g == 0);
+            if (dm) {
+                const int lane = (int)(threadIdx.x & 63);
+                const int leader = __ffsll((unsigned long long)dm) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(hot.count, (uint32_t)__popcll(dm));
+                base = __shfl(base, leader, 64);
+                if (!fin) {
+                    const int first = lane - g;
+                    const uint32_t idx = base + (uint32_t)__popcll(dm & ((1ull << first) - 1ull));
+                    if (idx < hot.cap) {
+                        if (g == 0) hot.e[idx] = HotEntry{tile.r, c, time, ctr};
+                        deferred = true;
+                    } else {  // list full: finish the cell here
+                        ssa_cell_g<S, G>(x, (uint32_t)c, seed, pl.deltat, T, 0xffffffffu, 0, time, ctr);
+                    }
+                }
+            }
+        }
+
+        // 3. outflow of the due species (a deferred cell's comes from the hot-cell kernel)
+        if (pl.out_mask) {
+            if constexpr (G == 1 && GS == 4) {
+                // the wave's 64 cells in 4 rounds of 16, cell 16r+q by lanes 4q..4q+3;
+                // the counts go back to the lane that owns the cell
+                const int lane = (int)(threadIdx.x & 63);
+                const int base = c - lane;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (!((pl.out_mask >> s) & 1u)) continue;
+                    int32_t sub = 0;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int src = 16 * r + (lane >> 2);
+                        const int32_t xs = __shfl(x[s], src, 64);
+                        const int dfr = __shfl((int)deferred, src, 64);
+                        int32_t res = 0;
+                        if (!dfr)
+                            res = cell_outflow<MODE, 4>(xs, s, base + src, W, H, A, seed, pl, bm, ri, prob, lout, P);
+                        const int32_t back = __shfl(res, 4 * (lane & 15), 64);
+                        if ((lane >> 4) == r) sub = back;
+                    }
+                    x[s] -= sub;
+                }
+            } else if (!deferred) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (!((pl.out_mask >> s) & 1u)) continue;
+                    x[s] -= cell_outflow<MODE, G>(x[s], s, c, W, H, A, seed, pl, bm, ri, prob, lout, P);
+                }
+            }
+        }
+
+        if (g == 0) {
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                if (pl.deltat > 0.f || ((touched >> s) & 1u)) st[(size_t)s * A + c] = x[s];
+        }
+    }
+}
+
+__device__ __forceinline__ float readlane_f(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ double readlane_d(double v, int j) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), j);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ int32_t wave_sum(int32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Mask of the reactions whose selection interval holds r2 -- the conditions of fire;
+// cumulative hazards never decrease, so at most one bit is set.
+template <int S>
+__device__ __forceinline__ uint32_t fired(const RxTable &T, const float (&h)[kMaxReactions + 1], float r2) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < kMaxReactions; ++i) {
+        if (i < T.R) {
+            const bool f = (i == 0) ? (r2 < h[1]) : ((r2 >= h[i]) && (r2 < h[i + 1]));
+            m |= f ? (1u << i) : 0u;
+        }
+    }
+    return m;
+}
+
+// Net change of species s by the reactions of mask m (the update fire applies).
+__device__ __forceinline__ int32_t net_of(const RxTable &T, uint32_t m, int s) {
+    int32_t d = 0;
+#pragma unroll
+    for (int i = 0; i < kMaxReactions; ++i)
+        if (i < T.R) d += ((m >> i) & 1u) ? (int32_t)T.net[i][s] : 0;
+    return d;
+}
+
+__device__ __forceinline__ int32_t wave_excl_scan(int32_t v) {
+    const int lane = (int)(threadIdx.x & 63);
+    int32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+    }
+    return incl - v;
+}
+
+// Continues performGillespie (Gillespie.cl:41-99) of one cell from (time, ctr) with the
+// 64 lanes of a wave, speculatively: lane j draws event ctr+1+j and selects its reaction
+// with the hazards of the batch's first state; an exclusive scan of the selected net
+// changes gives every lane the state before its event; each lane then recomputes the
+// hazards of that state and its reaction.  Up to the first lane whose reaction differs
+// from the speculation (or whose total hazard is 0) every state is exact, and that lane's
+// own event is exact too, so all those events are the serial chain's.  The float clock
+// is then advanced event by event (t += log(1/r1)/h0, stop at t >= deltat) exactly as in
+// ssa_cell.  A batch commits at least one event; while the hazards change little per
+// event (busy cells) it commits tens.
+template <int S>
+__device__ __forceinline__ void ssa_wave_spec(int32_t (&x)[S], uint32_t cell, uint32_t k1, float deltat,
+                                              const RxTable &T, float &time, uint32_t &ctr) {
+    const int lane = (int)(threadIdx.x & 63);
+    if (!(time < deltat)) return;
+    while (true) {
+        float h[kMaxReactions + 1];
+        hazards<S>(T, x, h);
+        if (h[0] == 0.f) return;
+        if (ctr == kMaxEventsPerCall) return;
+        const u32x2 w = threefry2x32_20(ctr + 1u + (uint32_t)lane, kCtrHi, cell, k1);
+        const float L = log_f(1.f / u01(w.x));
+        const double U2 = u01_d(w.y);
+        const uint32_t ms = fired<S>(T, h, (float)(U2 * (double)h[0]));
+        int32_t xj[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) xj[s] = x[s] + wave_excl_scan(net_of(T, ms, s));
+        float hj[kMaxReactions + 1];
+        hazards<S>(T, xj, hj);
+        const uint32_t ma = fired<S>(T, hj, (float)(U2 * (double)hj[0]));
+        const bool zero = hj[0] == 0.f;
+        const uint64_t zb = __ballot(zero);
+        const uint64_t bad = __ballot(zero || ma != ms);
+        const int last = bad ? (__ffsll((unsigned long long)bad) - 1) : 63;
+        const float dt = L / hj[0];
+        int stop = -1;
+        for (int j = 0; j <= last; ++j) {
+            if ((zb >> j) & 1ull) { stop = j; break; }
+            if (ctr + (uint32_t)j == kMaxEventsPerCall) { stop = j; break; }
+            time += readlane_f(dt, j);
+            if (!(time < deltat)) { stop = j; break; }
+        }
+        if (stop >= 0) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) x[s] = __builtin_amdgcn_readlane(xj[s], stop);
+            return;
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s) x[s] = __builtin_amdgcn_readlane(xj[s] + net_of(T, ma, s), last);
+        ctr += (uint32_t)last + 1u;
+    }
+}
+
+// Hot-cell kernel: one wave per deferred cell.  The SSA continues from (time, ctr) with
+// ssa_wave_spec.  Then the cell's outflow for the due species is computed with the
+// particle loop split over the lanes (call k by lane k % 64) and the counts summed --
+// the same integers as the serial loop.  The last block to finish resets the list.
+template <int S, int MODE>
+__global__ __launch_bounds__(64) void k_gmp_hot(int32_t *__restrict__ state, int32_t *__restrict__ lost_out,
+                                                const float2 *__restrict__ prob, const RealInfo *__restrict__ reals,
+                                                const LaunchPlan *__restrict__ plan,
+                                                const int32_t *__restrict__ seeds_cur, RxTable T, HomProbs P,
+                                                int4 bm, HotList hot) {
+    const int lane = (int)threadIdx.x;
+    const uint32_t total = hot.count[0];
+    if (total == 0) return;
+    const uint32_t nh = total < hot.cap ? total : hot.cap;
+    const LaunchPlan pl = *plan;
+    constexpr int ND = (MODE == MODE_HOM) ? 4 : 2;
+    for (uint32_t i = blockIdx.x; i < nh; i += gridDim.x) {
+        const HotEntry he = hot.e[i];
+        const RealInfo ri = reals[he.r];
+        const int32_t seed = seeds_cur[he.r];
+        const int W = ri.W, H = ri.H, A = W * H, c = he.c;
+        int32_t *st = state + ri.state_off;
+        int32_t x[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) x[s] = __builtin_amdgcn_readfirstlane(st[(size_t)s * A + c]);
+        float time = he.time;
+        uint32_t ctr = he.ctr;
+        ssa_wave_spec<S>(x, (uint32_t)c, kKeyBase + (uint32_t)seed, pl.deltat, T, time, ctr);
+        if (pl.out_mask) {
+            const int gx = c % W, gy = c / W;
+            const int m0 = (gx == 0) ? bm.x : 1, m1 = (gx == W - 1) ? bm.y : 1;
+            const int m2 = (gy == 0) ? bm.z : 1, m3 = (gy == H - 1) ? bm.w : 1;
+            int32_t *lo = lost_out + ri.lost_off;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if (!((pl.out_mask >> s) & 1u)) continue;
+                const uint32_t k1o = kKeyBase + (uint32_t)seed + (uint32_t)pl.out_off[s];
+                const int n = x[s];
+                const int calls = (n + 1) >> 1;
+                int32_t *l = lo + (size_t)s * ND * A;
+                if (MODE == MODE_HOM) {
+                    int32_t le0 = 0, le1 = 0, le2 = 0, le3 = 0, le4 = 0;
+                    for (int k = lane; k < calls; k += 64) {
+                        const u32x2 ww = threefry2x32_20((uint32_t)k + 1u, kCtrHi, (uint32_t)c, k1o);
+                        float u = u01(ww.x);
+                        le0 += u <= P.stay; le1 += u <= P.p1; le2 += u <= P.p2; le3 += u <= P.p3; le4 += u <= P.p4;
+                        if (2 * k + 1 < n) {
+                            u = u01(ww.y);
+                            le0 += u <= P.stay; le1 += u <= P.p1; le2 += u <= P.p2; le3 += u <= P.p3; le4 += u <= P.p4;
+                        }
+                    }
+                    le0 = wave_sum(le0); le1 = wave_sum(le1); le2 = wave_sum(le2); le3 = wave_sum(le3);
+                    le4 = wave_sum(le4);
+                    const int32_t l0 = m0 * (le1 - le0), l1 = m1 * (le2 - le1), l2 = m2 * (le3 - le2),
+                                  l3 = m3 * (le4 - le3);
+                    if (lane == 0) {
+                        l[0 * (size_t)A + c] = l0;
+                        l[1 * (size_t)A + c] = l1;
+                        l[2 * (size_t)A + c] = l2;
+                        l[3 * (size_t)A + c] = l3;
+                    }
+                    x[s] -= l0 + l1 + l2 + l3;
+                } else {
+                    const float2 pr = prob[ri.prob_off + (size_t)s * A + c];
+                    int32_t a = 0, b = 0;
+                    for (int k = lane; k < calls; k += 64) {
+                        const u32x2 ww = threefry2x32_20((uint32_t)k + 1u, kCtrHi, (uint32_t)(c + s * A), k1o);
+                        float u = u01(ww.x);
+                        a += u <= pr.x;
+                        b += u <= pr.y;
+                        if (2 * k + 1 < n) {
+                            u = u01(ww.y);
+                            a += u <= pr.x;
+                            b += u <= pr.y;
+                        }
+                    }
+                    a = wave_sum(a);
+                    b = wave_sum(b);
+                    const int32_t lm = m0 * (b - a), lp = m1 * a;
+                    if (lane == 0) {
+                        l[0 * (size_t)A + c] = lm;
+                        l[1 * (size_t)A + c] = lp;
+                    }
+                    x[s] -= lm + lp;
+                }
+            }
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) st[(size_t)s * A + c] = x[s];
+        }
+    }
+    // every block read the count before adding itself here, so the last one may reset
+    if (lane == 0) {
+        __threadfence();
+        const uint32_t fin = atomicAdd(&hot.count[1], 1u);
+        if (fin == gridDim.x - 1) {
+            atomicExch(&hot.count[0], 0u);
+            atomicExch(&hot.count[1], 0u);
+            __threadfence();
+        }
+    }
+}
+
+
+template <int S, int MODE, int G, int GS>
+static hipError_t launch_sg(const StepLaunch &a) {
+    const int grid = 8 * ((a.ntiles + 7) / 8);
+    if (a.ev_start) {
+        // start/stop timestamps taken by the dispatch packet itself
+        hipExtLaunchKernelGGL((k_gmp_step<S, MODE, G, GS>), dim3(grid), dim3(a.block), 0, a.stream, a.ev_start,
+                              a.ev_stop, 0, a.state, a.lost_in, a.lost_out, a.prob, a.reals, a.tiles,
+                              a.ntiles, a.plan, a.seeds_cur, a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot);
+    } else {
+        hipLaunchKernelGGL((k_gmp_step<S, MODE, G, GS>), dim3(grid), dim3(a.block), 0, a.stream, a.state,
+                           a.lost_in, a.lost_out, a.prob, a.reals, a.tiles, a.ntiles, a.plan, a.seeds_cur,
+                           a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot);
+    }
+    return hipGetLastError();
+}
+
+template <int S, int MODE>
+static hipError_t launch_s(const StepLaunch &a) {
+    hipError_t e;
+    if (a.grp == 4) e = launch_sg<S, MODE, 4, 4>(a);
+    else if (a.grp != 1) return hipErrorInvalidValue;
+    else if constexpr (MODE == MODE_INH_Y) e = launch_sg<S, MODE, 1, 1>(a);  // no SSA: lanes = grp
+    else if (a.grp_sweep == 4) e = launch_sg<S, MODE, 1, 4>(a);
+    else e = launch_sg<S, MODE, 1, 1>(a);
+    if constexpr (MODE == MODE_INH_Y) {
+        return e;
+    } else {
+        if (e != hipSuccess) return e;
+        // the cells that used up their event budget
+        if (a.ev_hot_start) {
+            hipExtLaunchKernelGGL((k_gmp_hot<S, MODE>), dim3(a.hot_grid), dim3(64), 0, a.stream, a.ev_hot_start,
+                                  a.ev_hot_stop, 0, a.state, a.lost_out, a.prob, a.reals, a.plan, a.seeds_cur,
+                                  *a.T, *a.P, a.bm, a.hot);
+        } else {
+            hipLaunchKernelGGL((k_gmp_hot<S, MODE>), dim3(a.hot_grid), dim3(64), 0, a.stream, a.state,
+                               a.lost_out, a.prob, a.reals, a.plan, a.seeds_cur, *a.T, *a.P, a.bm, a.hot);
+        }
+        return hipGetLastError();
+    }
+}
+
+template <int MODE>
+static hipError_t launch_m(const StepLaunch &a) {
+    switch (a.T->S) {
+        case 1: return launch_s<1, MODE>(a);
+        case 2: return launch_s<2, MODE>(a);
+        case 3: return launch_s<3, MODE>(a);
+        case 4: return launch_s<4, MODE>(a);
+        case 5: return launch_s<5, MODE>(a);
+        case 6: return launch_s<6, MODE>(a);
+        case 7: return launch_s<7, MODE>(a);
+        case 8: return launch_s<8, MODE>(a);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace gpgmp

@@ -24,6 +24,8 @@ struct StepLaunch {
     int4 bm;                // boundary masks -x, +x, -y, +y
     int block;              // threads per block (64..256, multiple of 64)
     int cpt;                // cells per thread
+    int grp = 1;            // lanes per cell of the SSA (1 or 4)
+    int grp_sweep = 1;      // lanes per cell of the outflow (4, or grp)
     hipStream_t stream;
     hipEvent_t ev_start = nullptr;  // optional: timestamps of this dispatch
     hipEvent_t ev_stop = nullptr;
@@ -34,6 +36,10 @@ struct StepLaunch {
 };
 
 hipError_t launch_gmp_step(const StepLaunch &a);
+// per-mode instantiations (stochastic_hom.hip, stochastic_inh_x.hip, stochastic_inh_y.hip)
+hipError_t launch_gmp_hom(const StepLaunch &a);
+hipError_t launch_gmp_inh_x(const StepLaunch &a);
+hipError_t launch_gmp_inh_y(const StepLaunch &a);
 hipError_t launch_min_dt(const float *D, const float *r, int n, float h, float *blocks, float *out,
                          hipStream_t stream);
 hipError_t launch_inh_probs(float2 *out, const float *D, const float *r, float dt, float dx, int n,

@@ -66,7 +66,8 @@ struct HotList {
     HotEntry *e;
     uint32_t *count;  // [0] entries pushed, [1] blocks of the hot kernel finished
     uint32_t cap;
-    uint32_t budget;  // SSA events a lane runs before deferring its cell
+    uint32_t budget;  // SSA events a lane runs before it may defer its cell
+    int32_t below_active;  // ... which it does once fewer lanes of its wave still run
 };
 
 // Homogeneous diffusion thresholds (StochasticHomogeneousDiffusion.cl:58-62)

@@ -254,9 +254,22 @@ class Solver:
     def setLaunch(self, block, cells_per_thread):
         call("gpgmp_solver_set_launch", self.handle, int(block), int(cells_per_thread))
 
-    def setEventBudget(self, budget):
-        """SSA events per lane before a busy cell moves to the one-wave-per-cell kernel."""
-        call("gpgmp_solver_set_event_budget", self.handle, int(budget))
+    def setLanesPerCell(self, ssa_lanes, sweep_lanes=None):
+        """1 or 4 lanes per cell for the launches with an SSA and the sweep-only launches
+        (results identical; speed differs)."""
+        sweep_lanes = ssa_lanes if sweep_lanes is None else sweep_lanes
+        call("gpgmp_solver_set_lanes_per_cell", self.handle, int(ssa_lanes), int(sweep_lanes))
+
+    def getLaunch(self):
+        """(threads per block, cells per thread, SSA lanes per cell, sweep lanes per cell)."""
+        v = [C.c_int32() for _ in range(4)]
+        call("gpgmp_solver_get_launch", self.handle, *[C.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def setEventBudget(self, budget, below_active=16):
+        """A cell past `budget` SSA events moves to the one-wave-per-cell kernel once
+        fewer than `below_active` lanes of its wave still run (65: always)."""
+        call("gpgmp_solver_set_event_budget", self.handle, int(budget), int(below_active))
 
     def kernelTimes(self, on=True):
         """Accumulated HIP-event times (ms, launches) per kernel mode of the current
@@ -323,6 +336,19 @@ class Ensemble:
 
     def setLaunch(self, block, cells_per_thread):
         call("gpgmp_batch_set_launch", self.handle, int(block), int(cells_per_thread))
+
+    def setEventBudget(self, budget, below_active=16):
+        call("gpgmp_batch_set_event_budget", self.handle, int(budget), int(below_active))
+
+    def setLanesPerCell(self, ssa_lanes, sweep_lanes=None):
+        sweep_lanes = ssa_lanes if sweep_lanes is None else sweep_lanes
+        call("gpgmp_batch_set_lanes_per_cell", self.handle, int(ssa_lanes), int(sweep_lanes))
+
+    def getLaunch(self):
+        """(threads per block, cells per thread, SSA lanes per cell, sweep lanes per cell)."""
+        v = [C.c_int32() for _ in range(4)]
+        call("gpgmp_batch_get_launch", self.handle, *[C.byref(x) for x in v])
+        return tuple(x.value for x in v)
 
     def autotune(self, trial_steps=20):
         b = C.c_int32()

@@ -123,14 +123,25 @@ int gpgmp_solver_run(gpgmp_solver *s, int32_t steps, uint32_t seed0, float *sim_
 int gpgmp_solver_run_continue(gpgmp_solver *s, int32_t steps, int32_t graph_chunk, float *sim_time_out);
 /* launch shape of the solver's kernels: threads per block (64..256), cells per thread */
 int gpgmp_solver_set_launch(gpgmp_solver *s, int32_t block, int32_t cells_per_thread);
+/* Lanes per cell (1 or 4) of the launches with an SSA and of the sweep-only launches
+ * (default 1 and, below 2^18 cells, 4): with 4, four lanes share a cell's SSA (random
+ * words and logarithms of 4 events drawn at once) and its particle loops.  Results are
+ * identical for every launch shape.  No counterpart in the reference (a launch shape
+ * of this build's fused kernel). */
+int gpgmp_solver_set_lanes_per_cell(gpgmp_solver *s, int32_t ssa_lanes, int32_t sweep_lanes);
+/* current launch shape */
+int gpgmp_solver_get_launch(gpgmp_solver *s, int32_t *block, int32_t *cells_per_thread, int32_t *ssa_lanes,
+                            int32_t *sweep_lanes);
 /* Reads the accumulated per-launch HIP-event times of the current run's eager launches
  * (ms[4], n[4], algorithmic HBM bytes[4] by kernel: 0 homogeneous, 1 SSA + X sweeps,
  * 2 Y sweeps, 3 hot-cell continuation; may be NULL), then switches the timing on or off
  * for the following launches. */
 int gpgmp_solver_kernel_times(gpgmp_solver *s, int32_t on, double *ms, int64_t *n, double *bytes);
-/* SSA events a lane runs before its cell moves to the one-wave-per-cell continuation
- * kernel (default 32); results do not depend on it. */
-int gpgmp_solver_set_event_budget(gpgmp_solver *s, uint32_t budget);
+/* Deferral of busy cells: a cell whose SSA has drawn `budget` events (default 16) moves
+ * to the one-wave-per-cell continuation kernel once fewer than `below_active` lanes of
+ * its wave still run their SSA (default 16; 65 = always at the budget).  Results do not
+ * depend on either.  No counterpart in the reference. */
+int gpgmp_solver_set_event_budget(gpgmp_solver *s, uint32_t budget, int32_t below_active);
 int gpgmp_solver_synchronize(gpgmp_solver *s);
 
 /* ---- batched ensemble (no reference counterpart) ----------------------------------- */
@@ -145,8 +156,14 @@ int gpgmp_batch_set_state(gpgmp_batch *b, int32_t r, const int32_t *host_counts)
 int gpgmp_batch_get_state(gpgmp_batch *b, int32_t r, int32_t *host_counts);
 /* launch shape: threads per block (64..256) and cells per thread (1..16) */
 int gpgmp_batch_set_launch(gpgmp_batch *b, int32_t block, int32_t cells_per_thread);
-/* Times the candidate launch shapes on `trial_steps` steps from a copy of the current
- * state (restored afterwards) and keeps the fastest; reports it. */
+/* lanes per cell (1 or 4), as gpgmp_solver_set_lanes_per_cell */
+int gpgmp_batch_set_lanes_per_cell(gpgmp_batch *b, int32_t ssa_lanes, int32_t sweep_lanes);
+int gpgmp_batch_get_launch(gpgmp_batch *b, int32_t *block, int32_t *cells_per_thread, int32_t *ssa_lanes,
+                           int32_t *sweep_lanes);
+/* Times the candidate launch shapes (threads per block, cells per thread, lanes per
+ * cell) on `trial_steps` steps from a copy of the current state (restored afterwards)
+ * and keeps the fastest; reports block and cells per thread (gpgmp_batch_get_launch
+ * gives the lanes too). */
 int gpgmp_batch_autotune(gpgmp_batch *b, int32_t trial_steps, int32_t *block_out,
                          int32_t *cpt_out);
 /* Runs `steps` host-loop iterations of every realization.  graph_chunk > 0 captures
@@ -155,7 +172,7 @@ int gpgmp_batch_run(gpgmp_batch *b, int32_t steps, int32_t graph_chunk);
 /* Restarts the host loops at time 0 with the realizations' seed0 (state is kept). */
 int gpgmp_batch_reset(gpgmp_batch *b);
 int gpgmp_batch_kernel_times(gpgmp_batch *b, int32_t on, double *ms, int64_t *n, double *bytes);
-int gpgmp_batch_set_event_budget(gpgmp_batch *b, uint32_t budget);
+int gpgmp_batch_set_event_budget(gpgmp_batch *b, uint32_t budget, int32_t below_active);
 int gpgmp_batch_total_cells(gpgmp_batch *b, int64_t *cells);
 int gpgmp_batch_synchronize(gpgmp_batch *b);
 

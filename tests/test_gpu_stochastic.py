@@ -138,25 +138,36 @@ def test_ensemble_matches_plain(gpu):
     for r, st in enumerate(states):
         ens.setState(r, st)
     ens.setLaunch(128, 2)
+    ens.setLanesPerCell(4, 1)  # the plain solvers below run (1, 4) lanes per cell (small grids)
     ens.run(40, graph_chunk=0)
     ens.run(60, graph_chunk=20)
     for r, sd in enumerate(sides):
         pm = configs.plain_gmp_model(int(sd))
         p = pm.createSolver()
+        assert p.getLaunch()[2:] == (1, 4)
         p.synchronizeStateBufferToDevice(states[r])
         p.run(100, int(seeds[r]))
         assert np.array_equal(ens.getState(r), p.synchronizeStateBufferFromDevice()), r
 
 
-def test_launch_shape_invariance(gpu):
-    m = configs.plain_gmp_model(64)
-    st = configs.plain_gmp_state(64)
+@pytest.mark.parametrize("network", ["plain", "schnakenberg"])
+def test_launch_shape_invariance(gpu, network):
+    """Threads per block, cells per thread and lanes per cell (1: one lane runs a cell;
+    4: a lane group draws 4 SSA events at once and splits the particle loops) change
+    the speed only, never a count."""
+    if network == "plain":
+        m, st, steps = configs.plain_gmp_model(64), configs.plain_gmp_state(64), 50
+    else:
+        m, st, steps = configs.schnakenberg_model(64), configs.schnakenberg_state(64, seed=3), 80
     outs = []
-    for block, cpt in [(256, 1), (64, 4), (128, 3)]:
+    for block, cpt, lanes in [(256, 1, (1, 1)), (64, 4, (4, 4)), (128, 3, (1, 4)), (192, 2, (4, 1)),
+                              (256, 1, (4, 4))]:
         s = m.createSolver()
         s.setLaunch(block, cpt)
+        s.setLanesPerCell(*lanes)
+        assert s.getLaunch() == (block, cpt) + lanes
         s.synchronizeStateBufferToDevice(st)
-        s.run(50, 3)
+        s.run(steps, 3)
         outs.append(s.synchronizeStateBufferFromDevice())
     assert all(np.array_equal(outs[0], o) for o in outs[1:])
 
@@ -183,15 +194,18 @@ def test_min_timestep_device(gpu, oracle):
                 assert abs(np.float32(out.value) - want) <= np.spacing(want), (side, s)
 
 
-@pytest.mark.parametrize("budget", [1, 5, 64, 1 << 30])
-def test_event_budget_hot_cells(gpu, oracle, budget):
-    """Cells that exceed the per-lane SSA event budget continue in the one-wave-per-cell
-    kernel (parallel Threefry/log batches, lane-split particle loops): counts stay
-    bit-exact for any budget, including 1 (every active cell is deferred)."""
+@pytest.mark.parametrize("lanes", [1, 4])
+@pytest.mark.parametrize("budget,below", [(1, 65), (5, 65), (1, 8), (64, 65), (1 << 30, 8)])
+def test_event_budget_hot_cells(gpu, oracle, budget, below, lanes):
+    """Cells that exceed the SSA event budget continue in the one-wave-per-cell kernel
+    (speculative 64-event batches, lane-split particle loops): counts stay bit-exact for
+    any budget and deferral threshold, including budget 1 with below_active 65 (every
+    cell with an event is deferred), with 1 or 4 lanes per cell in the main kernel."""
     m = configs.schnakenberg_model(64)
     st = configs.schnakenberg_state(64, seed=21)
     s = m.createSolver()
-    s.setEventBudget(budget)
+    s.setLanesPerCell(lanes)
+    s.setEventBudget(budget, below)
     s.synchronizeStateBufferToDevice(st)
     t = s.run(120, 4)
     want, tw = oracle.run(m, st, 120, 4)
@@ -201,7 +215,8 @@ def test_event_budget_hot_cells(gpu, oracle, budget):
     mh = configs.plain_gmp_model(48)
     mh.source_mask = (0, 3, 0, 2)
     sh = mh.createSolver()
-    sh.setEventBudget(budget)
+    sh.setLanesPerCell(lanes)
+    sh.setEventBudget(budget, below)
     st2 = configs.plain_gmp_state(48)
     sh.synchronizeStateBufferToDevice(st2)
     sh.run(60, 8)
