@@ -146,6 +146,51 @@ def rdd_extra(reps=3):
             "forward_algorithmic_GBps": 20.0 * W * H * steps / (f * 1e-3) / 1e9}
 
 
+def det_diffusion_extra(side=8192, reps=20):
+    """The reference's deterministic stencils (homogeneous diffusion of 2 species; the
+    inhomogeneous drift-diffusion X and Y sweeps of one species) on a side x side fp32
+    grid, time-stepped by ping-pong between two state buffers of 2 x 256 MiB (larger than
+    the caches): achieved HBM GB/s from the algorithmic bytes (homogeneous: read + write
+    8 B per cell-species; inhomogeneous: state, D, r read and state written, 16 B per
+    cell) over HIP-event kernel time on the launching stream."""
+    import ctypes as C
+    import torch
+    from inchman_amd._lib import call
+
+    A = side * side
+    g = torch.Generator(device="cuda").manual_seed(3)
+    buf = [torch.rand(2 * A, device="cuda", generator=g) * 100 for _ in range(2)]
+    D = torch.rand(A, device="cuda", generator=g) * 0.9 + 0.1
+    r = (torch.rand(A, device="cuda", generator=g) - 0.5) * 0.6
+    Dh = (C.c_float * 2)(0.6, 0.2)
+    st = torch.cuda.current_stream()
+    sh = int(st.cuda_stream)
+    out = {"workload": "deterministic stencils, %dx%d fp32, ping-pong time steps" % (side, side)}
+
+    def timed(fn, nbytes):
+        for k in range(4):
+            fn(k)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(st)
+        for k in range(reps):
+            fn(k)
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        return {"ms_per_call": ms, "GBps": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+
+    out["homogeneous_2_species"] = timed(
+        lambda k: call("gpgmp_det_hom_diffuse_device", buf[k & 1].data_ptr(), buf[(k + 1) & 1].data_ptr(), side, side,
+                       2, Dh, 0.05, 1.0, sh), 8.0 * 2 * A)
+    for axis, name in ((0, "inhomogeneous_x"), (1, "inhomogeneous_y")):
+        out[name] = timed(lambda k: call("gpgmp_det_inhom_sweep_device", buf[k & 1].data_ptr(),
+                                         buf[(k + 1) & 1].data_ptr(), side, side, 0, axis, 0.1, D.data_ptr(),
+                                         r.data_ptr(), 1.0, sh), 16.0 * A)
+    return out
+
+
 def particles_extra(n=10_000_000, side=1024, steps=1000):
     """10^7 particles on a 1024x1024 periodic domain: move, stable radix sort, offsets,
     in-cell A+A->0 (p = 0.05 per pair per step)."""
@@ -178,7 +223,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--cpu-steps", type=int, default=3000)
     ap.add_argument("--no-extra", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -239,6 +284,7 @@ def main():
     if not args.no_extra:
         extra["ensemble"] = ensemble_extra()
         extra["rdd"] = rdd_extra()
+        extra["det_diffusion"] = det_diffusion_extra()
         extra["particles"] = particles_extra()
     out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     out["extra"] = extra

@@ -21,14 +21,17 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-30)))
 
 
-def test_det_hom_diffuse(gpu, oracle):
+@pytest.mark.parametrize("H,W", [(40, 48), (37, 50), (300, 1024), (5, 4)])
+def test_det_hom_diffuse(gpu, oracle, H, W):
+    """Streaming 4-cell form (W % 4 == 0, strips of 16 rows, ragged last strip) and the
+    one-cell form (other widths)."""
     rng = np.random.default_rng(1)
-    st = rng.uniform(1, 100, (2, 40, 48))
+    st = rng.uniform(1, 100, (2, H, W))
     D = np.array([0.6, 0.2], np.float32)
     dt, h = 0.05, 1.1
     x = _dev(st)
     y = torch.empty_like(x)
-    call("gpgmp_det_hom_diffuse_device", x.data_ptr(), y.data_ptr(), 48, 40, 2,
+    call("gpgmp_det_hom_diffuse_device", x.data_ptr(), y.data_ptr(), W, H, 2,
          D.ctypes.data_as(C.POINTER(C.c_float)), dt, h, None)
     torch.cuda.synchronize()
     want = oracle.det_hom_diffuse(st.astype(np.float32).astype(np.float64), D.astype(np.float64), dt,
@@ -36,15 +39,16 @@ def test_det_hom_diffuse(gpu, oracle):
     assert _rel(y.cpu().numpy(), want) <= 1e-5
 
 
+@pytest.mark.parametrize("H,W", [(32, 48), (37, 50), (300, 1024), (5, 4)])
 @pytest.mark.parametrize("axis", [0, 1])
-def test_det_inhom_sweep(gpu, oracle, axis):
+def test_det_inhom_sweep(gpu, oracle, axis, H, W):
     rng = np.random.default_rng(2 + axis)
-    st = rng.uniform(1, 100, (2, 32, 48)).astype(np.float32)
-    Dax = rng.uniform(0.1, 1.0, (2, 32, 48)).astype(np.float32)
-    rax = rng.uniform(-0.3, 0.3, (2, 32, 48)).astype(np.float32)
+    st = rng.uniform(1, 100, (2, H, W)).astype(np.float32)
+    Dax = rng.uniform(0.1, 1.0, (2, H, W)).astype(np.float32)
+    rax = rng.uniform(-0.3, 0.3, (2, H, W)).astype(np.float32)
     x, D, r = _dev(st), _dev(Dax), _dev(rax)
     y = x.clone()
-    call("gpgmp_det_inhom_sweep_device", x.data_ptr(), y.data_ptr(), 48, 32, 1, axis, 0.1, D.data_ptr(),
+    call("gpgmp_det_inhom_sweep_device", x.data_ptr(), y.data_ptr(), W, H, 1, axis, 0.1, D.data_ptr(),
          r.data_ptr(), 1.0, None)
     torch.cuda.synchronize()
     want = oracle.det_inhom_sweep(st.astype(np.float64), 1, axis, 0.1, Dax.astype(np.float64),
