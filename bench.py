@@ -191,6 +191,45 @@ def det_diffusion_extra(side=8192, reps=20):
     return out
 
 
+def individuals_extra(n=1 << 20, side=1024, steps=20, cpu_steps=2):
+    """A particle-based species (IndividualSolver): n particles with their own
+    diffusivities/drifts on a side x side reflective grid; each host-loop iteration does
+    the due X/Y hop sweeps, each followed by the stable sort, cell offsets, per-cell
+    constants and the time-step minimum (read back to the host, as the reference does)."""
+    import numpy as np
+    import torch
+    from inchman_amd.individuals import IndividualSpecies
+
+    rng = np.random.default_rng(8)
+    cells = rng.integers(0, side * side, n).astype(np.uint32)
+    props = np.stack([rng.uniform(0.1, 1.0, n), rng.uniform(0.1, 1.0, n), rng.uniform(-0.2, 0.2, n),
+                      rng.uniform(-0.2, 0.2, n)]).astype(np.float32)
+    g = IndividualSpecies(side, side, float(side), n)
+    g.setParticles(cells, props)
+    g.run(2, 1)  # warmup
+    g.setParticles(cells, props)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.run(steps, 21)
+    wall = time.perf_counter() - t0
+    sw = g.sweeps()
+    out = {"workload": "individual species, %d particles, %dx%d reflective cells, %d iterations" % (n, side, side, steps),
+           "sweeps": sw, "particle_hops_per_s": n * sw / wall, "iterations_per_s": steps / wall, "wall_s": wall}
+    from oracle import oracle
+    p = oracle.ind_params(side, side, float(side))
+    t1 = time.perf_counter()
+    oracle.ind_run(p, cells, np.arange(n, dtype=np.uint32), props, cpu_steps, 21)
+    cpu = time.perf_counter() - t1
+    g.setParticles(cells, props)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g.run(cpu_steps, 21)
+    gpu = time.perf_counter() - t2
+    out["cpu_oracle_1_thread_iterations_per_s"] = cpu_steps / cpu
+    out["gpu_iterations_per_s_same_sample"] = cpu_steps / gpu
+    return out
+
+
 def particles_extra(n=10_000_000, side=1024, steps=1000):
     """10^7 particles on a 1024x1024 periodic domain: move, stable radix sort, offsets,
     in-cell A+A->0 (p = 0.05 per pair per step)."""
@@ -286,6 +325,7 @@ def main():
         extra["rdd"] = rdd_extra()
         extra["det_diffusion"] = det_diffusion_extra()
         extra["particles"] = particles_extra()
+        extra["individuals"] = individuals_extra()
     out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     out["extra"] = extra
     print(json.dumps(out), flush=True)

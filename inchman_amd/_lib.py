@@ -68,6 +68,13 @@ class RddParams(C.Structure):
     ]
 
 
+class IndividualsDesc(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("physical_length", C.c_float),
+        ("boundary_mask", C.c_int32 * 4), ("species", C.c_int32),
+    ]
+
+
 class ParticleParams(C.Structure):
     _fields_ = [
         ("side", C.c_int32),
@@ -144,6 +151,17 @@ SIGNATURES = {
     "gpgmp_particles_count": [_P, _pI],
     "gpgmp_particles_step": [_P, _I, _I],
     "gpgmp_particles_processed": [_P, C.POINTER(C.c_uint64)],
+    "gpgmp_individuals_create": [C.POINTER(IndividualsDesc), _I, _P, C.POINTER(_P)],
+    "gpgmp_individuals_destroy": [_P],
+    "gpgmp_individuals_set": [_P, _P, _P, _I],
+    "gpgmp_individuals_time_steps": [_P, _pF, _pF],
+    "gpgmp_individuals_diffuse": [_P, _I, _F, _pI],
+    "gpgmp_individuals_update": [_P, _I, _pF, _pF],
+    "gpgmp_individuals_run": [_P, _I, _U, _pF],
+    "gpgmp_individuals_get": [_P, _P, _P, _P, _pI],
+    "gpgmp_individuals_constants": [_P, _P, _P, _P, _P],
+    "gpgmp_individuals_keys_device": [_P, _P, _P, _P],
+    "gpgmp_individuals_sweeps": [_P, C.POINTER(C.c_int64)],
 }
 
 _lib = None

@@ -105,6 +105,77 @@ GPGMP_HD double log_d(double y) {
 
 GPGMP_HD float log_f(float y) { return (float)log_d((double)y); }
 
+// exp(x) in double: x = k ln2 + r with |r| <= ln2/2 (ln2 split in two for an exact
+// k*ln2_hi), Taylor series of exp(r) to r^14 (truncation below 1e-17), scaled by 2^k.
+// IEEE add/mul/div only, so CPU and GPU builds give the same bits.  |x| <= 700.
+GPGMP_HD double exp_d(double x) {
+    const double k = __builtin_rint(x * 1.4426950408889634074);
+    const double r = (x - k * 6.93147180369123816490e-01) - k * 1.90821492927058770002e-10;
+    double p = 1.0 / 87178291200.0;
+    p = 1.0 / 6227020800.0 + r * p;
+    p = 1.0 / 479001600.0 + r * p;
+    p = 1.0 / 39916800.0 + r * p;
+    p = 1.0 / 3628800.0 + r * p;
+    p = 1.0 / 362880.0 + r * p;
+    p = 1.0 / 40320.0 + r * p;
+    p = 1.0 / 5040.0 + r * p;
+    p = 1.0 / 720.0 + r * p;
+    p = 1.0 / 120.0 + r * p;
+    p = 1.0 / 24.0 + r * p;
+    p = 1.0 / 6.0 + r * p;
+    p = 0.5 + r * p;
+    p = 1.0 + r * p;
+    p = 1.0 + r * p;
+    return __builtin_ldexp(p, (int)k);
+}
+
+// exp(x) - 1 for x >= 0 without cancellation: Taylor series below ln2/2 (to x^16),
+// exp_d(x) - 1 above.
+GPGMP_HD double expm1_d(double x) {
+    if (x >= 0.34657359027997264) return exp_d(x) - 1.0;
+    double p = 1.0 / 355687428096000.0;
+    p = 1.0 / 20922789888000.0 + x * p;
+    p = 1.0 / 1307674368000.0 + x * p;
+    p = 1.0 / 87178291200.0 + x * p;
+    p = 1.0 / 6227020800.0 + x * p;
+    p = 1.0 / 479001600.0 + x * p;
+    p = 1.0 / 39916800.0 + x * p;
+    p = 1.0 / 3628800.0 + x * p;
+    p = 1.0 / 362880.0 + x * p;
+    p = 1.0 / 40320.0 + x * p;
+    p = 1.0 / 5040.0 + x * p;
+    p = 1.0 / 720.0 + x * p;
+    p = 1.0 / 120.0 + x * p;
+    p = 1.0 / 24.0 + x * p;
+    p = 1.0 / 6.0 + x * p;
+    p = 0.5 + x * p;
+    return x + x * (x * p);
+}
+
+// Admissible time step of one cell for a particle-based species
+// (computeIndividualTimestepX/Y, StochasticInhomogeneousDriftDiffusion.cl:130-151): the
+// formula in double with tanh(a) = E2/(E2+2), E2 = expm1(2a), and sinh(a) = (E + E/(E+1))/2,
+// E = expm1(a), from expm1_d (so the CPU and GPU evaluate the same bits), rounded to
+// float.  An empty cell (D = r = 0) gives +inf.
+GPGMP_HD float ind_cell_dt(float Df, float rf, float hf) {
+    const double D = Df, r = rf, h = hf;
+    if (r == 0.0) return (float)(h * h / (2.0 * D) * (1.0 - 2.0 / 3.0));
+    if (D == 0.0) return (float)(h / __builtin_fabs(r));
+    const double eps = h * r / (2.0 * D);
+    const double a = __builtin_fabs(eps);
+    double th = 1.0, ish2 = 0.0;
+    if (!(a > 350.0)) {
+        const double e2 = expm1_d(2.0 * a), e1 = expm1_d(a);
+        th = e2 / (e2 + 2.0);
+        const double sh = (e1 + e1 / (e1 + 1.0)) * 0.5;
+        ish2 = 1.0 / (sh * sh);
+    }
+    if (eps < 0.0) th = -th;
+    double temp = 1.0 + (-1.0 / (th * eps) + ish2);
+    if (temp <= 0.0) temp = 1.0 / 3.0;
+    return (float)(h / r * th * temp);
+}
+
 // sin(pi t) in double: t reduced to [0,2), folded to [0,1/2], Taylor series to x^21
 GPGMP_HD double sinpi_d(double t) {
     double t2 = t - 2.0 * __builtin_floor(t * 0.5);

@@ -79,6 +79,7 @@ typedef struct gpgmp_solver gpgmp_solver;
 typedef struct gpgmp_batch gpgmp_batch;
 typedef struct gpgmp_rdd gpgmp_rdd;
 typedef struct gpgmp_particles gpgmp_particles;
+typedef struct gpgmp_individuals gpgmp_individuals;
 
 const char *gpgmp_last_error(void);
 int gpgmp_version(int32_t *major, int32_t *minor);
@@ -250,6 +251,48 @@ int gpgmp_particles_count(gpgmp_particles *ps, int32_t *n_out);
 int gpgmp_particles_step(gpgmp_particles *ps, int32_t step0, int32_t steps);
 /* Sum over the steps run so far of the particles each step processed. */
 int gpgmp_particles_processed(gpgmp_particles *ps, uint64_t *particle_steps);
+
+/* ---- individual (particle-based) species on the lattice (gpgmp::IndividualSolver) -- */
+/* One particle-based species of an inhomogeneous stochastic model: particles are cell
+ * keys with four properties each (DiffusivityX, DiffusivityY, DriftX, DriftY; the
+ * computeDiffusivityDrift hook is the identity, IndividualSolver.cl:690-694). */
+typedef struct {
+    int32_t width, height;       /* grid cells */
+    float physical_length;       /* domain side in um (cell width = header float of L/width) */
+    int32_t boundary_mask[4];    /* -x,+x,-y,+y: 0 reflective, 1 absorbing (the key leaves or wraps) */
+    int32_t species;             /* species index: RNG key = cell + species*width*height */
+} gpgmp_individuals_desc;
+/* IndividualSolver(...) buffers for `capacity` particles (IndividualSolver.cpp:37-395). */
+int gpgmp_individuals_create(const gpgmp_individuals_desc *d, int32_t capacity, void *stream,
+                             gpgmp_individuals **out);
+int gpgmp_individuals_destroy(gpgmp_individuals *g);
+/* Sets n particles (host): cells[n] in [0, W*H), props[4][n]; particle index i carries
+ * props[k][i].  Then sorts, computes the per-cell constants and both first time steps
+ * (InhomogeneousSolver.cpp:205-225 for an individual species). */
+int gpgmp_individuals_set(gpgmp_individuals *g, const uint32_t *cells, const float *props, int32_t n);
+/* Current time steps of the X and Y sweeps (m_diffusionDtX/Y). */
+int gpgmp_individuals_time_steps(gpgmp_individuals *g, float *dt_x, float *dt_y);
+/* IndividualSolver::diffuseX / diffuseY (IndividualSolver.cpp:416-456, kernels
+ * IndividualSolver.cl:750-1014) with time step dt and *seed, then *seed += 1 as
+ * InhomogeneousSolver::diffuse does (InhomogeneousSolver.cpp:268-269, 361-362). */
+int gpgmp_individuals_diffuse(gpgmp_individuals *g, int32_t axis, float dt, int32_t *seed);
+/* IndividualSolver::computeDiffusionConstants (IndividualSolver.cpp:1100-1153: sort,
+ * cell indices, constants kernel IndividualSolver.cl:654-747) and the minimum per-cell
+ * time step of the axes in `axes` (bit 0: X, bit 1: Y; InhomogeneousSolver.cpp:304-336). */
+int gpgmp_individuals_update(gpgmp_individuals *g, int32_t axes, float *dt_x, float *dt_y);
+/* DiffusionModel::oclSolve's loop (DiffusionModel.cpp:1467-1603) for this species alone
+ * (no reactions): `steps` iterations from seed0, X then Y sweep when due, each followed by
+ * the update of its time step (InhomogeneousSolver::diffuse, InhomogeneousSolver.cpp:254-427). */
+int gpgmp_individuals_run(gpgmp_individuals *g, int32_t steps, uint32_t seed0, float *sim_time_out);
+/* Sorted keys[n] and particle indices vals[n], per-cell counts[W*H] (any may be NULL),
+ * and n (IndividualSolver::getPositionList, IndividualSolver.h:36-297). */
+int gpgmp_individuals_get(gpgmp_individuals *g, uint32_t *keys, uint32_t *vals, int32_t *counts, int32_t *n);
+/* Per-cell max diffusivities and max-magnitude drifts of the last update (host, [W*H]). */
+int gpgmp_individuals_constants(gpgmp_individuals *g, float *dx, float *dy, float *rx, float *ry);
+/* Hop sweeps (X and Y) done since the particles were set. */
+int gpgmp_individuals_sweeps(gpgmp_individuals *g, int64_t *sweeps);
+/* Device pointers of the sorted keys, particle indices and CSR cell offsets [W*H+1]. */
+int gpgmp_individuals_keys_device(gpgmp_individuals *g, uint32_t **keys, uint32_t **vals, uint32_t **off);
 
 #ifdef __cplusplus
 }

@@ -135,6 +135,29 @@ typedef struct {
 int oracle_particle_step(const oracle_particle_params *p, int step, int n, float *x, float *y,
                          uint32_t *id, uint32_t *keys, uint32_t *off, int nthreads);
 
+/* ---- individual-species lattice hops (IndividualSolver) ---- */
+typedef struct {
+    int W, H;
+    float physical_length;
+    int32_t boundary_mask[4];  /* -x, +x, -y, +y: 0 reflective, 1 absorbing */
+    int species;               /* RNG key offset species * W * H */
+} oracle_ind_params;
+/* props: [4][cap] = DiffusivityX, DiffusivityY, DriftX, DriftY per particle index */
+void oracle_ind_sweep(const oracle_ind_params *p, uint32_t *keys, const uint32_t *vals, const uint32_t *left,
+                      const uint32_t *right, int n, const float *props, int cap, int axis, int32_t seed,
+                      float dt);
+void oracle_ind_constants(const oracle_ind_params *p, const uint32_t *keys, const uint32_t *vals,
+                          const uint32_t *left, const uint32_t *right, int n, const float *props, int cap,
+                          float *Dx, float *Dy, float *rx, float *ry, int32_t *count);
+void oracle_ind_update(const oracle_ind_params *p, uint32_t *keys, uint32_t *vals, int n, const float *props,
+                       int cap, uint32_t *left, uint32_t *right, float *Dx, float *Dy, float *rx, float *ry,
+                       int32_t *count, float *dtx, float *dty);
+double oracle_expd(double x);
+double oracle_expm1d(double x);
+float oracle_ind_cell_dt(float D, float r, float h);
+int oracle_ind_run(const oracle_ind_params *p, uint32_t *keys, uint32_t *vals, int n, const float *props, int cap,
+                   int steps, uint32_t seed0, int32_t *count, float *sim_time_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,11 @@ class ParticleParams(C.Structure):
                 ("p_annihilate", C.c_float), ("seed", C.c_uint32)]
 
 
+class IndParams(C.Structure):
+    _fields_ = [("W", C.c_int), ("H", C.c_int), ("physical_length", C.c_float), ("boundary_mask", C.c_int32 * 4),
+                ("species", C.c_int)]
+
+
 _lib = None
 
 
@@ -86,6 +91,18 @@ def lib():
         L.oracle_rdd_grad.restype = C.c_double
         L.oracle_rdd_loss.restype = C.c_double
         L.oracle_particle_step.restype = C.c_int
+        L.oracle_expd.restype = C.c_double
+        L.oracle_expd.argtypes = [C.c_double]
+        L.oracle_expm1d.restype = C.c_double
+        L.oracle_expm1d.argtypes = [C.c_double]
+        L.oracle_ind_cell_dt.restype = C.c_float
+        L.oracle_ind_cell_dt.argtypes = [C.c_float] * 3
+        L.oracle_ind_sweep.argtypes = [C.c_void_p] * 5 + [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int32,
+                                                          C.c_float]
+        L.oracle_ind_constants.argtypes = [C.c_void_p] * 5 + [C.c_int, C.c_void_p, C.c_int] + [C.c_void_p] * 5
+        L.oracle_ind_update.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_void_p, C.c_int] + [C.c_void_p] * 9
+        L.oracle_ind_run.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_uint32,
+                                                        C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -308,3 +325,65 @@ def particle_steps(p, step0, steps, x, y, ids, nthreads=0):
     for s in range(step0, step0 + steps):
         n = lib().oracle_particle_step(C.byref(p), s, n, _p(x), _p(y), _p(ids), _p(keys), _p(off), nthreads)
     return x[:n], y[:n], ids[:n], keys[:n], off
+
+
+# ---- individual-species lattice hops ---------------------------------------------------
+def ind_params(W, H, physical_length, bm=(0, 0, 0, 0), species=0):
+    p = IndParams()
+    p.W, p.H, p.physical_length, p.species = W, H, physical_length, species
+    for i in range(4):
+        p.boundary_mask[i] = bm[i]
+    return p
+
+
+def _props(props):
+    pr = np.ascontiguousarray(props, np.float32)
+    assert pr.ndim == 2 and pr.shape[0] == 4
+    return pr, pr.shape[1]
+
+
+def ind_sweep(p, keys, vals, left, right, props, axis, seed, dt):
+    k = np.ascontiguousarray(keys, np.uint32).copy()
+    v = np.ascontiguousarray(vals, np.uint32)
+    le = np.ascontiguousarray(left, np.uint32)
+    ri = np.ascontiguousarray(right, np.uint32)
+    pr, cap = _props(props)
+    lib().oracle_ind_sweep(C.byref(p), _p(k), _p(v), _p(le), _p(ri), k.size, _p(pr), cap, axis, seed, dt)
+    return k
+
+
+def ind_constants(p, keys, vals, left, right, props):
+    A = p.W * p.H
+    k = np.ascontiguousarray(keys, np.uint32)
+    v = np.ascontiguousarray(vals, np.uint32)
+    le = np.ascontiguousarray(left, np.uint32)
+    ri = np.ascontiguousarray(right, np.uint32)
+    pr, cap = _props(props)
+    out = [np.zeros(A, np.float32) for _ in range(4)] + [np.zeros(A, np.int32)]
+    lib().oracle_ind_constants(C.byref(p), _p(k), _p(v), _p(le), _p(ri), k.size, _p(pr), cap, *[_p(o) for o in out])
+    return out
+
+
+def ind_update(p, keys, vals, props):
+    A = p.W * p.H
+    k = np.ascontiguousarray(keys, np.uint32).copy()
+    v = np.ascontiguousarray(vals, np.uint32).copy()
+    pr, cap = _props(props)
+    le, ri = np.zeros(A, np.uint32), np.zeros(A, np.uint32)
+    out = [np.zeros(A, np.float32) for _ in range(4)] + [np.zeros(A, np.int32)]
+    dtx, dty = C.c_float(), C.c_float()
+    lib().oracle_ind_update(C.byref(p), _p(k), _p(v), k.size, _p(pr), cap, _p(le), _p(ri), *[_p(o) for o in out],
+                            C.byref(dtx), C.byref(dty))
+    return k, v, le, ri, out, dtx.value, dty.value
+
+
+def ind_run(p, keys, vals, props, steps, seed0):
+    A = p.W * p.H
+    k = np.ascontiguousarray(keys, np.uint32).copy()
+    v = np.ascontiguousarray(vals, np.uint32).copy()
+    pr, cap = _props(props)
+    count = np.zeros(A, np.int32)
+    t = C.c_float()
+    lib().oracle_ind_run(C.byref(p), _p(k), _p(v), k.size, _p(pr), cap, steps, C.c_uint32(seed0), _p(count),
+                         C.byref(t))
+    return k, v, count, t.value
