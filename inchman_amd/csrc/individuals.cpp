@@ -46,8 +46,10 @@ struct gpgmp_individuals {
     int A = 0, cap = 0, n = 0, key_bits = 32, nb = 1;
     float h = 0.f;
     hipStream_t s = nullptr;
-    DevBuf<uint32_t> keys, vals, keys2, vals2, off, hist, tot;
-    DevBuf<float> props, Dx, Dy, rx, ry, blk, dt_dev;
+    // keys: cell per particle; pos: positions 0..n-1 sorted along with the keys (the
+    // permutation); ids / props: particle index and properties in the keys' order
+    DevBuf<uint32_t> keys, pos, keys2, pos2, ids, ids2, off, hist, tot;
+    DevBuf<float> props, props2, Dx, Dy, rx, ry, blk, dt_dev;
     DevBuf<int32_t> count;
     float *dt_host = nullptr;  // pinned [2]
     float dtx = 0.f, dty = 0.f;
@@ -65,16 +67,19 @@ struct gpgmp_individuals {
     void update(int axes) {
         if (n > 0) {
             bool in_tmp = false;
-            GPGMP_HIP_CHECK(radix_sort_pairs(keys.get(), vals.get(), keys2.get(), vals2.get(), nullptr, n, key_bits,
+            GPGMP_HIP_CHECK(radix_sort_pairs(keys.get(), pos.get(), keys2.get(), pos2.get(), nullptr, n, key_bits,
                                              hist.get(), tot.get(), s, &in_tmp));
             if (in_tmp) {
-                GPGMP_HIP_CHECK(hipMemcpyAsync(keys.get(), keys2.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, s));
-                GPGMP_HIP_CHECK(hipMemcpyAsync(vals.get(), vals2.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+                std::swap(keys, keys2);
+                std::swap(pos, pos2);
             }
+            GPGMP_HIP_CHECK(launch_ind_permute(pos.get(), ids.get(), ids2.get(), props.get(), props2.get(), n, cap, s));
+            std::swap(ids, ids2);
+            std::swap(props, props2);
         }
         GPGMP_HIP_CHECK(launch_lower_bound(keys.get(), nullptr, n, A, off.get(), s));
-        GPGMP_HIP_CHECK(launch_ind_constants(vals.get(), off.get(), props.get(), cap, A, Dx.get(), Dy.get(),
-                                             rx.get(), ry.get(), count.get(), s));
+        GPGMP_HIP_CHECK(launch_ind_constants(off.get(), props.get(), cap, A, Dx.get(), Dy.get(), rx.get(), ry.get(),
+                                             count.get(), s));
         if (axes & 1) GPGMP_HIP_CHECK(launch_ind_dt(Dx.get(), rx.get(), A, h, blk.get(), nb, dt_dev.get(), s));
         if (axes & 2) GPGMP_HIP_CHECK(launch_ind_dt(Dy.get(), ry.get(), A, h, blk.get(), nb, dt_dev.get() + 1, s));
         if (axes) {
@@ -90,7 +95,7 @@ struct gpgmp_individuals {
     void sweep(int axis, float dt, int32_t seed) {
         ++sweeps;
         const int4 bm = make_int4(d.boundary_mask[0], d.boundary_mask[1], d.boundary_mask[2], d.boundary_mask[3]);
-        GPGMP_HIP_CHECK(launch_ind_sweep(axis, keys.get(), vals.get(), off.get(), props.get(), cap, d.width, d.height,
+        GPGMP_HIP_CHECK(launch_ind_sweep(axis, keys.get(), off.get(), props.get(), cap, d.width, d.height,
                                          (uint32_t)d.species * (uint32_t)A, kKeyBase + (uint32_t)seed, dt, h, bm, s));
     }
 };
@@ -123,13 +128,16 @@ int gpgmp_individuals_create(const gpgmp_individuals_desc *desc, int32_t capacit
         g->nb = std::min(1024, (g->A + 255) / 256);
         const size_t cp = (size_t)capacity;
         g->keys.alloc(cp);
-        g->vals.alloc(cp);
+        g->pos.alloc(cp);
         g->keys2.alloc(cp);
-        g->vals2.alloc(cp);
+        g->pos2.alloc(cp);
+        g->ids.alloc(cp);
+        g->ids2.alloc(cp);
         g->hist.alloc((size_t)256 * std::max(radix_tiles(capacity), 1));
         g->tot.alloc(256);
         g->off.alloc((size_t)g->A + 1);
         g->props.alloc(4 * cp);
+        g->props2.alloc(4 * cp);
         g->Dx.alloc(g->A);
         g->Dy.alloc(g->A);
         g->rx.alloc(g->A);
@@ -159,7 +167,8 @@ int gpgmp_individuals_set(gpgmp_individuals *g, const uint32_t *cells, const flo
         g->sweeps = 0;
         if (n > 0) {
             GPGMP_HIP_CHECK(hipMemcpyAsync(g->keys.get(), cells, (size_t)n * 4, hipMemcpyHostToDevice, g->s));
-            GPGMP_HIP_CHECK(hipMemcpyAsync(g->vals.get(), ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, g->s));
+            GPGMP_HIP_CHECK(hipMemcpyAsync(g->ids.get(), ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, g->s));
+            GPGMP_HIP_CHECK(hipMemcpyAsync(g->pos.get(), ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, g->s));
             for (int k = 0; k < 4; ++k)
                 GPGMP_HIP_CHECK(hipMemcpyAsync(g->props.get() + (size_t)k * g->cap, props + (size_t)k * n,
                                                (size_t)n * 4, hipMemcpyHostToDevice, g->s));
@@ -232,7 +241,7 @@ int gpgmp_individuals_get(gpgmp_individuals *g, uint32_t *keys, uint32_t *vals, 
         GPGMP_REQUIRE(g, GPGMP_E_INVALID, "null argument");
         GPGMP_REQUIRE(g->ready, GPGMP_E_STATE, "particles are unsorted: call update first");
         if (keys && g->n) GPGMP_HIP_CHECK(hipMemcpyAsync(keys, g->keys.get(), (size_t)g->n * 4, hipMemcpyDeviceToHost, g->s));
-        if (vals && g->n) GPGMP_HIP_CHECK(hipMemcpyAsync(vals, g->vals.get(), (size_t)g->n * 4, hipMemcpyDeviceToHost, g->s));
+        if (vals && g->n) GPGMP_HIP_CHECK(hipMemcpyAsync(vals, g->ids.get(), (size_t)g->n * 4, hipMemcpyDeviceToHost, g->s));
         if (count) GPGMP_HIP_CHECK(hipMemcpyAsync(count, g->count.get(), (size_t)g->A * 4, hipMemcpyDeviceToHost, g->s));
         GPGMP_HIP_CHECK(hipStreamSynchronize(g->s));
         if (n_out) *n_out = g->n;
@@ -263,7 +272,7 @@ int gpgmp_individuals_keys_device(gpgmp_individuals *g, uint32_t **keys, uint32_
     return guarded_i([&] {
         GPGMP_REQUIRE(g, GPGMP_E_INVALID, "null argument");
         if (keys) *keys = g->keys.get();
-        if (vals) *vals = g->vals.get();
+        if (vals) *vals = g->ids.get();
         if (off) *off = g->off.get();
     });
 }

@@ -2,8 +2,9 @@
 // species (IndividualSolver), whose particles carry their own diffusivity and drift and
 // hop between cells with the inhomogeneous drift-diffusion probabilities.
 //
-// Particles are (cell key, particle index) pairs sorted stably by key (the radix sort of
-// particles.hip); off[c] .. off[c+1] is cell c's range.  One thread per cell walks its
+// Particles are cell keys sorted stably (the radix sort of particles.hip), with their
+// particle index and four properties kept in the same order (k_ind_permute after every
+// sort), so every kernel reads them coalesced; off[c] .. off[c+1] is cell c's range.  One thread per cell walks its
 // range in sorted order, as individual_diffuseX/Y (IndividualSolver.cl:750-1014) walk
 // left[c] .. right[c]: RNG key {cell + species*A, 0xdecafbad+seed}, counter advanced per
 // particle of the cell, word x.  The reference rewrites keys in place while an empty
@@ -33,8 +34,7 @@ __device__ __forceinline__ void ind_hop_probs(float D, float r, float dt, float 
 // -/+1 (X) or -/+W (Y) times the direction's boundary mask (0 reflective: the particle
 // stays; 1 absorbing: the key leaves the cell range or wraps, as in the reference).
 template <int AXIS>
-__global__ __launch_bounds__(256) void k_ind_sweep(uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
-                                                   const uint32_t *__restrict__ off,
+__global__ __launch_bounds__(256) void k_ind_sweep(uint32_t *__restrict__ keys, const uint32_t *__restrict__ off,
                                                    const float *__restrict__ props, int cap, int W, int H,
                                                    uint32_t key_off, uint32_t k1, float dt, float dx, int4 bm) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -46,9 +46,8 @@ __global__ __launch_bounds__(256) void k_ind_sweep(uint32_t *__restrict__ keys, 
     const uint32_t b = off[c], e = off[c + 1];
     uint32_t ctr = 0;
     for (uint32_t i = b; i < e; ++i) {
-        const uint32_t idx = vals[i];
         float p1, p2;
-        ind_hop_probs(props[idx + (size_t)AXIS * cap], props[idx + (size_t)(2 + AXIS) * cap], dt, dx, p1, p2);
+        ind_hop_probs(props[i + (size_t)AXIS * cap], props[i + (size_t)(2 + AXIS) * cap], dt, dx, p1, p2);
         ++ctr;
         const u32x2 w = threefry2x32_20(ctr, kCtrHi, (uint32_t)c + key_off, k1);
         const float u = u01(w.x);
@@ -56,6 +55,22 @@ __global__ __launch_bounds__(256) void k_ind_sweep(uint32_t *__restrict__ keys, 
         if (u <= p2) key = (u <= p1) ? key + step * mhi : key - step * mlo;
         keys[i] = key;
     }
+}
+
+// After a sort of (key, position) pairs: carry the particle indices and properties into
+// the sorted order (perm[i] = the position the i-th sorted particle came from), and reset
+// the positions for the next sort.
+__global__ __launch_bounds__(256) void k_ind_permute(uint32_t *__restrict__ perm, const uint32_t *__restrict__ ids,
+                                                     uint32_t *__restrict__ ids_out,
+                                                     const float *__restrict__ props, float *__restrict__ props_out,
+                                                     int n, int cap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = perm[i];
+    ids_out[i] = ids[j];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) props_out[i + (size_t)k * cap] = props[j + (size_t)k * cap];
+    perm[i] = (uint32_t)i;
 }
 
 // OpenCL maxmag(x, y): the argument of larger magnitude, fmax on ties.
@@ -67,8 +82,7 @@ __device__ __forceinline__ float maxmag(float x, float y) {
 // computeDiffusionConstants (IndividualSolver.cl:654-747) with the identity drift /
 // diffusivity hook: per cell the largest diffusivities, the largest-magnitude drifts and
 // the particle count (written into the species' state).
-__global__ __launch_bounds__(256) void k_ind_constants(const uint32_t *__restrict__ vals,
-                                                       const uint32_t *__restrict__ off,
+__global__ __launch_bounds__(256) void k_ind_constants(const uint32_t *__restrict__ off,
                                                        const float *__restrict__ props, int cap, int A,
                                                        float *__restrict__ Dx, float *__restrict__ Dy,
                                                        float *__restrict__ rx, float *__restrict__ ry,
@@ -78,11 +92,10 @@ __global__ __launch_bounds__(256) void k_ind_constants(const uint32_t *__restric
     float dx = 0.f, dy = 0.f, qx = 0.f, qy = 0.f;
     const uint32_t b = off[c], e = off[c + 1];
     for (uint32_t i = b; i < e; ++i) {
-        const uint32_t idx = vals[i];
-        dx = fmaxf(props[idx], dx);
-        dy = fmaxf(props[idx + (size_t)cap], dy);
-        qx = maxmag(props[idx + 2 * (size_t)cap], qx);
-        qy = maxmag(props[idx + 3 * (size_t)cap], qy);
+        dx = fmaxf(props[i], dx);
+        dy = fmaxf(props[i + (size_t)cap], dy);
+        qx = maxmag(props[i + 2 * (size_t)cap], qx);
+        qy = maxmag(props[i + 3 * (size_t)cap], qy);
     }
     Dx[c] = dx;
     Dy[c] = dy;
@@ -121,22 +134,29 @@ __global__ __launch_bounds__(256) void k_ind_dt_reduce(const float *__restrict__
     if (threadIdx.x == 0) *out = m;
 }
 
-hipError_t launch_ind_sweep(int axis, uint32_t *keys, const uint32_t *vals, const uint32_t *off, const float *props,
-                            int cap, int W, int H, uint32_t key_off, uint32_t k1, float dt, float dx, int4 bm,
-                            hipStream_t s) {
+hipError_t launch_ind_sweep(int axis, uint32_t *keys, const uint32_t *off, const float *props, int cap, int W, int H,
+                            uint32_t key_off, uint32_t k1, float dt, float dx, int4 bm, hipStream_t s) {
     const int A = W * H;
     const dim3 g((A + 255) / 256), b(256);
     if (axis == 0)
-        hipLaunchKernelGGL(k_ind_sweep<0>, g, b, 0, s, keys, vals, off, props, cap, W, H, key_off, k1, dt, dx, bm);
+        hipLaunchKernelGGL(k_ind_sweep<0>, g, b, 0, s, keys, off, props, cap, W, H, key_off, k1, dt, dx, bm);
     else
-        hipLaunchKernelGGL(k_ind_sweep<1>, g, b, 0, s, keys, vals, off, props, cap, W, H, key_off, k1, dt, dx, bm);
+        hipLaunchKernelGGL(k_ind_sweep<1>, g, b, 0, s, keys, off, props, cap, W, H, key_off, k1, dt, dx, bm);
     return hipGetLastError();
 }
 
-hipError_t launch_ind_constants(const uint32_t *vals, const uint32_t *off, const float *props, int cap, int A,
-                                float *Dx, float *Dy, float *rx, float *ry, int32_t *count, hipStream_t s) {
-    hipLaunchKernelGGL(k_ind_constants, dim3((A + 255) / 256), dim3(256), 0, s, vals, off, props, cap, A, Dx, Dy, rx,
-                       ry, count);
+hipError_t launch_ind_constants(const uint32_t *off, const float *props, int cap, int A, float *Dx, float *Dy,
+                                float *rx, float *ry, int32_t *count, hipStream_t s) {
+    hipLaunchKernelGGL(k_ind_constants, dim3((A + 255) / 256), dim3(256), 0, s, off, props, cap, A, Dx, Dy, rx, ry,
+                       count);
+    return hipGetLastError();
+}
+
+hipError_t launch_ind_permute(uint32_t *perm, const uint32_t *ids, uint32_t *ids_out, const float *props,
+                              float *props_out, int n, int cap, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ind_permute, dim3((n + 255) / 256), dim3(256), 0, s, perm, ids, ids_out, props, props_out, n,
+                       cap);
     return hipGetLastError();
 }
 
