@@ -459,7 +459,7 @@ int gpgmp_batch_autotune(gpgmp_batch *b, int32_t trial_steps, int32_t *block_out
         GPGMP_HIP_CHECK(hipStreamSynchronize(st));
         static const int blocks[] = {64, 128, 256};
         static const int cpts[] = {1, 2, 4};
-        static const int grps[][2] = {{1, 1}, {1, 4}, {4, 4}};
+        static const int grps[][2] = {{1, 0}, {1, 1}, {1, 4}, {4, 4}, {4, 0}};
         double best = 1e300;
         int bb = e.block(), bc = e.cpt(), bg = e.grp(), bgs = e.grp_sweep();
         hipEvent_t t0, t1;

@@ -231,10 +231,10 @@ GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vecto
     }
     // deferred busy cells: at most one entry per cell of the launch, capped at 2^20
     hot_.cap = (uint32_t)std::min<int64_t>(total_cells_, (int64_t)1 << 20);
-    // deferral of busy cells: after 16 events, once fewer than 16 lanes of the wave run
-    // (measured best on the 256x256 Schnakenberg run among budgets 8..64, thresholds 4..65)
-    hot_.budget = 16;
-    hot_.below_active = 16;
+    // deferral of busy cells: after 8 events, once fewer than 24 lanes of the wave run
+    // (measured best on the 256x256 Schnakenberg run among budgets 2..64, thresholds 4..65)
+    hot_.budget = 8;
+    hot_.below_active = 24;
     hot_e_.alloc(hot_.cap);
     hot_count_.alloc(2);
     hot_.e = hot_e_.get();
@@ -247,12 +247,11 @@ GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vecto
     GPGMP_HIP_CHECK(hipMemsetAsync(lost_[0].get(), 0, lo * sizeof(int32_t), stream_));
     GPGMP_HIP_CHECK(hipMemsetAsync(lost_[1].get(), 0, lo * sizeof(int32_t), stream_));
     GPGMP_HIP_CHECK(hipMemsetAsync(state_.get(), 0, so * sizeof(int32_t), stream_));
-    // small grids leave most SIMDs with a single wave at one lane per cell: below 2^18
-    // cells the sweep-only launches give each cell 4 lanes (their particle loops split
-    // exactly); the SSA launches keep one lane per cell, whose serial event chain costs
-    // the least VALU work per event
+    // the SSA launches keep one lane per cell (the least VALU work per event); the
+    // particle loops of each wave's cells are flattened over all its lanes (measured best
+    // on the 256x256 Schnakenberg run; the batch autotuner may choose otherwise)
     grp_ = 1;
-    grp_sweep_ = (total_cells_ < ((int64_t)1 << 18)) ? 4 : 1;
+    grp_sweep_ = 0;
     build_tiles();
 }
 
@@ -276,8 +275,8 @@ void GmpEngine::set_launch(int block, int cpt, int grp, int grp_sweep) {
     GPGMP_REQUIRE(block >= 64 && block <= 256 && block % 64 == 0, GPGMP_E_INVALID,
                   "block must be 64, 128, 192 or 256");
     GPGMP_REQUIRE(cpt >= 1 && cpt <= 16, GPGMP_E_INVALID, "cells per thread must be in [1, 16]");
-    GPGMP_REQUIRE((grp == 1 || grp == 4) && (grp_sweep == 1 || grp_sweep == 4), GPGMP_E_INVALID,
-                  "lanes per cell must be 1 or 4");
+    GPGMP_REQUIRE((grp == 1 || grp == 4) && (grp_sweep == 0 || grp_sweep == 1 || grp_sweep == 4), GPGMP_E_INVALID,
+                  "lanes per cell must be 1 or 4 (SSA) and 0, 1 or 4 (sweeps)");
     ++shape_version_;
     grp_ = grp;
     grp_sweep_ = grp_sweep;
@@ -329,8 +328,13 @@ void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cu
     a.lost_out = lost_out;
     a.prob = inh_ ? prob_[mode == 2 ? 1 : 0].get() : nullptr;
     a.reals = reals_dev_.get();
-    a.grp = (mode == 2) ? grp_sweep_ : grp_;
-    a.grp_sweep = (a.grp == 4) ? 4 : grp_sweep_;
+    if (mode == 2) {  // no SSA: 16 cells per wave unless one lane per cell is asked for
+        a.grp = (grp_sweep_ == 1) ? 1 : 4;
+        a.grp_sweep = (grp_sweep_ == 0) ? 0 : a.grp;
+    } else {
+        a.grp = grp_;
+        a.grp_sweep = (grp_sweep_ == 0) ? 0 : (grp_ == 4 ? 4 : grp_sweep_);
+    }
     a.tiles = tiles_dev_[a.grp == 4].get();
     a.ntiles = ntiles_[a.grp == 4];
     a.plan = plan;

@@ -315,7 +315,144 @@ __device__ __forceinline__ int32_t cell_outflow(int32_t n, int s, int c, int W, 
     }
 }
 
-// G lanes per cell for the SSA and GS (= G, or 4 with G = 1) for the outflow.
+// Flattened outflow of one species for the cells of a wave (NC = 64/LPC cells, cell q
+// owned by lane q*LPC): the cells' Threefry calls are laid end to end and every lane
+// takes an equal slice of them (lane L: calls [L*T/64, (L+1)*T/64) of the wave's T), so
+// one busy cell no longer sets the wave's time.  A lane walks its slice cell by cell,
+// adding its partial counts into the wave's LDS counters when it leaves a cell; the
+// owners then turn the totals into the outflow.  Counts are integer sums, so they equal
+// the serial loop's.  Only the lanes of one wave touch its LDS slice.
+struct FlatLds {
+    int32_t start[4][65];       // exclusive scan of the calls per cell, [NC] = total
+    int32_t n[4][64];           // particles per cell
+    float2 pr[4][64];           // inhomogeneous (min(p1,p2), p2) per cell
+    uint32_t cnt[4][64][5];     // counters per cell
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int MODE, int LPC>
+__device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, int c, int s, int W, int H, int A,
+                                                  int32_t seed, const LaunchPlan &pl, int4 bm, const RealInfo &ri,
+                                                  const float2 *__restrict__ prob, int32_t *__restrict__ lout,
+                                                  const HomProbs &P, FlatLds &L) {
+    constexpr int NC = 64 / LPC;
+    constexpr int NCNT = (MODE == MODE_HOM) ? 5 : 2;
+    constexpr int ND = (MODE == MODE_HOM) ? 4 : 2;
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int q = lane / LPC;
+    const bool own = (lane % LPC) == 0;
+    const int base = c - q;  // the wave's first cell
+    const uint32_t k1 = kKeyBase + (uint32_t)seed + (uint32_t)pl.out_off[s];
+    const uint32_t koff = (MODE == MODE_HOM) ? 0u : (uint32_t)(s * A);
+    int32_t *st = L.start[wave];
+    const int32_t calls = (own && !deferred) ? ((xs + 1) >> 1) : 0;
+    int32_t incl = calls;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+    }
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
+    if (own) {
+        st[q] = incl - calls;
+        L.n[wave][q] = xs;
+        if (MODE != MODE_HOM) L.pr[wave][q] = prob[ri.prob_off + (size_t)s * A + c];
+#pragma unroll
+        for (int j = 0; j < NCNT; ++j) L.cnt[wave][q][j] = 0u;
+    }
+    if (lane == 0) st[NC] = (int32_t)total;
+    wave_lds_sync();
+    const uint32_t lo = (uint32_t)(((uint64_t)lane * total) >> 6);
+    const uint32_t hi = (uint32_t)(((uint64_t)(lane + 1) * total) >> 6);
+    if (lo < hi) {
+        int a = 0, b = NC;  // st[a] <= lo < st[b]
+        while (b - a > 1) {
+            const int m = (a + b) >> 1;
+            if ((uint32_t)st[m] <= lo) a = m;
+            else b = m;
+        }
+        int qq = a;
+        uint32_t end = (uint32_t)st[qq + 1];
+        uint32_t k = lo - (uint32_t)st[qq];
+        int32_t n = L.n[wave][qq];
+        float2 pr = (MODE != MODE_HOM) ? L.pr[wave][qq] : make_float2(0.f, 0.f);
+        uint32_t k0 = (uint32_t)(base + qq) + koff;
+        uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+        for (uint32_t f = lo; f < hi; ++f) {
+            if (f == end) {  // next cell with calls
+                atomicAdd(&L.cnt[wave][qq][0], c0);
+                atomicAdd(&L.cnt[wave][qq][1], c1);
+                if (MODE == MODE_HOM) {
+                    atomicAdd(&L.cnt[wave][qq][2], c2);
+                    atomicAdd(&L.cnt[wave][qq][3], c3);
+                    atomicAdd(&L.cnt[wave][qq][4], c4);
+                }
+                c0 = c1 = c2 = c3 = c4 = 0;
+                ++qq;
+                while ((uint32_t)st[qq + 1] == f) ++qq;
+                end = (uint32_t)st[qq + 1];
+                k = 0;
+                n = L.n[wave][qq];
+                if (MODE != MODE_HOM) pr = L.pr[wave][qq];
+                k0 = (uint32_t)(base + qq) + koff;
+            }
+            const u32x2 w = threefry2x32_20(k + 1u, kCtrHi, k0, k1);
+            const bool two = (int32_t)(2 * k + 1) < n;
+            const float u = u01(w.x), v = u01(w.y);
+            if (MODE == MODE_HOM) {
+                c0 += (u <= P.stay) + (two && v <= P.stay);
+                c1 += (u <= P.p1) + (two && v <= P.p1);
+                c2 += (u <= P.p2) + (two && v <= P.p2);
+                c3 += (u <= P.p3) + (two && v <= P.p3);
+                c4 += (u <= P.p4) + (two && v <= P.p4);
+            } else {
+                c0 += (u <= pr.x) + (two && v <= pr.x);
+                c1 += (u <= pr.y) + (two && v <= pr.y);
+            }
+            ++k;
+        }
+        atomicAdd(&L.cnt[wave][qq][0], c0);
+        atomicAdd(&L.cnt[wave][qq][1], c1);
+        if (MODE == MODE_HOM) {
+            atomicAdd(&L.cnt[wave][qq][2], c2);
+            atomicAdd(&L.cnt[wave][qq][3], c3);
+            atomicAdd(&L.cnt[wave][qq][4], c4);
+        }
+    }
+    wave_lds_sync();
+    if (own && !deferred) {
+        const int gx = c % W, gy = c / W;
+        const int m0 = (gx == 0) ? bm.x : 1, m1 = (gx == W - 1) ? bm.y : 1;
+        const int m2 = (gy == 0) ? bm.z : 1, m3 = (gy == H - 1) ? bm.w : 1;
+        int32_t *l = lout + (size_t)s * ND * A;
+        const uint32_t *cn = L.cnt[wave][q];
+        if (MODE == MODE_HOM) {
+            const int32_t l0 = m0 * (int32_t)(cn[1] - cn[0]), l1 = m1 * (int32_t)(cn[2] - cn[1]);
+            const int32_t l2 = m2 * (int32_t)(cn[3] - cn[2]), l3 = m3 * (int32_t)(cn[4] - cn[3]);
+            l[0 * (size_t)A + c] = l0;
+            l[1 * (size_t)A + c] = l1;
+            l[2 * (size_t)A + c] = l2;
+            l[3 * (size_t)A + c] = l3;
+            xs -= l0 + l1 + l2 + l3;
+        } else {
+            const int32_t mm = (MODE == MODE_INH_X) ? m0 : m2;
+            const int32_t mp = (MODE == MODE_INH_X) ? m1 : m3;
+            const int32_t lm = mm * (int32_t)(cn[1] - cn[0]), lp = mp * (int32_t)cn[0];
+            l[0 * (size_t)A + c] = lm;
+            l[1 * (size_t)A + c] = lp;
+            xs -= lm + lp;
+        }
+    }
+    wave_lds_sync();  // the slice is reused by the next species / cell round
+}
+
+// G lanes per cell for the SSA; the outflow by GS lanes per cell (GS = G, or 4 with
+// G = 1), or flattened over the wave (GS = 0).
 template <int S, int MODE, int G, int GS>
 __global__ __launch_bounds__(256) void k_gmp_step(
     int32_t *__restrict__ state, const int32_t *__restrict__ lost_in, int32_t *__restrict__ lost_out,
@@ -339,6 +476,8 @@ __global__ __launch_bounds__(256) void k_gmp_step(
     const bool any_src = (ri.src[0] | ri.src[1] | ri.src[2] | ri.src[3]) != 0;
     const int g = (int)(threadIdx.x & (G - 1));        // lane in the cell's group
     const int cpb = (int)blockDim.x / G;               // cells per block and round
+    __shared__ FlatLds flat_lds;
+    (void)flat_lds;
 
     for (int k = 0; k < cpt; ++k) {
         const int c = tile.cell0 + k * cpb + (int)threadIdx.x / G;
@@ -412,7 +551,14 @@ __global__ __launch_bounds__(256) void k_gmp_step(
 
         // 3. outflow of the due species (a deferred cell's comes from the hot-cell kernel)
         if (pl.out_mask) {
-            if constexpr (G == 1 && GS == 4) {
+            if constexpr (GS == 0) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (!((pl.out_mask >> s) & 1u)) continue;
+                    wave_flat_outflow<MODE, G>(x[s], deferred, c, s, W, H, A, seed, pl, bm, ri, prob, lout, P,
+                                               flat_lds);
+                }
+            } else if constexpr (G == 1 && GS == 4) {
                 // the wave's 64 cells in 4 rounds of 16, cell 16r+q by lanes 4q..4q+3;
                 // the counts go back to the lane that owns the cell
                 const int lane = (int)(threadIdx.x & 63);
@@ -678,8 +824,9 @@ static hipError_t launch_sg(const StepLaunch &a) {
 template <int S, int MODE>
 static hipError_t launch_s(const StepLaunch &a) {
     hipError_t e;
-    if (a.grp == 4) e = launch_sg<S, MODE, 4, 4>(a);
-    else if (a.grp != 1) return hipErrorInvalidValue;
+    if (a.grp != 1 && a.grp != 4) return hipErrorInvalidValue;
+    if (a.grp_sweep == 0) e = (a.grp == 4) ? launch_sg<S, MODE, 4, 0>(a) : launch_sg<S, MODE, 1, 0>(a);
+    else if (a.grp == 4) e = launch_sg<S, MODE, 4, 4>(a);
     else if constexpr (MODE == MODE_INH_Y) e = launch_sg<S, MODE, 1, 1>(a);  // no SSA: lanes = grp
     else if (a.grp_sweep == 4) e = launch_sg<S, MODE, 1, 4>(a);
     else e = launch_sg<S, MODE, 1, 1>(a);

@@ -25,7 +25,7 @@ struct StepLaunch {
     int block;              // threads per block (64..256, multiple of 64)
     int cpt;                // cells per thread
     int grp = 1;            // lanes per cell of the SSA (1 or 4)
-    int grp_sweep = 1;      // lanes per cell of the outflow (4, or grp)
+    int grp_sweep = 1;      // outflow: lanes per cell (4, or grp), or 0 = flattened over the wave
     hipStream_t stream;
     hipEvent_t ev_start = nullptr;  // optional: timestamps of this dispatch
     hipEvent_t ev_stop = nullptr;

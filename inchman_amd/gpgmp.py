@@ -266,7 +266,7 @@ class Solver:
         call("gpgmp_solver_get_launch", self.handle, *[C.byref(x) for x in v])
         return tuple(x.value for x in v)
 
-    def setEventBudget(self, budget, below_active=16):
+    def setEventBudget(self, budget, below_active=24):
         """A cell past `budget` SSA events moves to the one-wave-per-cell kernel once
         fewer than `below_active` lanes of its wave still run (65: always)."""
         call("gpgmp_solver_set_event_budget", self.handle, int(budget), int(below_active))
@@ -337,7 +337,7 @@ class Ensemble:
     def setLaunch(self, block, cells_per_thread):
         call("gpgmp_batch_set_launch", self.handle, int(block), int(cells_per_thread))
 
-    def setEventBudget(self, budget, below_active=16):
+    def setEventBudget(self, budget, below_active=24):
         call("gpgmp_batch_set_event_budget", self.handle, int(budget), int(below_active))
 
     def setLanesPerCell(self, ssa_lanes, sweep_lanes=None):

@@ -124,11 +124,11 @@ int gpgmp_solver_run(gpgmp_solver *s, int32_t steps, uint32_t seed0, float *sim_
 int gpgmp_solver_run_continue(gpgmp_solver *s, int32_t steps, int32_t graph_chunk, float *sim_time_out);
 /* launch shape of the solver's kernels: threads per block (64..256), cells per thread */
 int gpgmp_solver_set_launch(gpgmp_solver *s, int32_t block, int32_t cells_per_thread);
-/* Lanes per cell (1 or 4) of the launches with an SSA and of the sweep-only launches
- * (default 1 and, below 2^18 cells, 4): with 4, four lanes share a cell's SSA (random
- * words and logarithms of 4 events drawn at once) and its particle loops.  Results are
- * identical for every launch shape.  No counterpart in the reference (a launch shape
- * of this build's fused kernel). */
+/* Lanes per cell (1 or 4, default 1) of the launches with an SSA, and the sweep
+ * scheme (default 0): 1 or 4 lanes per cell, or 0 = the particle loops of a wave's cells
+ * flattened over all its lanes.  With 4 SSA lanes, four lanes share a cell's SSA (random words and logarithms of
+ * 4 events drawn at once).  Results are identical for every choice.  No counterpart in
+ * the reference (a launch shape of this build's fused kernel). */
 int gpgmp_solver_set_lanes_per_cell(gpgmp_solver *s, int32_t ssa_lanes, int32_t sweep_lanes);
 /* current launch shape */
 int gpgmp_solver_get_launch(gpgmp_solver *s, int32_t *block, int32_t *cells_per_thread, int32_t *ssa_lanes,
@@ -138,9 +138,9 @@ int gpgmp_solver_get_launch(gpgmp_solver *s, int32_t *block, int32_t *cells_per_
  * 2 Y sweeps, 3 hot-cell continuation; may be NULL), then switches the timing on or off
  * for the following launches. */
 int gpgmp_solver_kernel_times(gpgmp_solver *s, int32_t on, double *ms, int64_t *n, double *bytes);
-/* Deferral of busy cells: a cell whose SSA has drawn `budget` events (default 16) moves
+/* Deferral of busy cells: a cell whose SSA has drawn `budget` events (default 8) moves
  * to the one-wave-per-cell continuation kernel once fewer than `below_active` lanes of
- * its wave still run their SSA (default 16; 65 = always at the budget).  Results do not
+ * its wave still run their SSA (default 24; 65 = always at the budget).  Results do not
  * depend on either.  No counterpart in the reference. */
 int gpgmp_solver_set_event_budget(gpgmp_solver *s, uint32_t budget, int32_t below_active);
 int gpgmp_solver_synchronize(gpgmp_solver *s);

@@ -138,13 +138,13 @@ def test_ensemble_matches_plain(gpu):
     for r, st in enumerate(states):
         ens.setState(r, st)
     ens.setLaunch(128, 2)
-    ens.setLanesPerCell(4, 1)  # the plain solvers below run (1, 4) lanes per cell (small grids)
+    ens.setLanesPerCell(4, 1)  # the plain solvers below run the default (1 lane, flattened sweeps)
     ens.run(40, graph_chunk=0)
     ens.run(60, graph_chunk=20)
     for r, sd in enumerate(sides):
         pm = configs.plain_gmp_model(int(sd))
         p = pm.createSolver()
-        assert p.getLaunch()[2:] == (1, 4)
+        assert p.getLaunch()[2:] == (1, 0)
         p.synchronizeStateBufferToDevice(states[r])
         p.run(100, int(seeds[r]))
         assert np.array_equal(ens.getState(r), p.synchronizeStateBufferFromDevice()), r
@@ -153,7 +153,8 @@ def test_ensemble_matches_plain(gpu):
 @pytest.mark.parametrize("network", ["plain", "schnakenberg"])
 def test_launch_shape_invariance(gpu, network):
     """Threads per block, cells per thread and lanes per cell (1: one lane runs a cell;
-    4: a lane group draws 4 SSA events at once and splits the particle loops) change
+    4: a lane group draws 4 SSA events at once and splits the particle loops; sweep
+    lanes 0: the particle loops of a wave's cells flattened over all its lanes) change
     the speed only, never a count."""
     if network == "plain":
         m, st, steps = configs.plain_gmp_model(64), configs.plain_gmp_state(64), 50
@@ -161,7 +162,7 @@ def test_launch_shape_invariance(gpu, network):
         m, st, steps = configs.schnakenberg_model(64), configs.schnakenberg_state(64, seed=3), 80
     outs = []
     for block, cpt, lanes in [(256, 1, (1, 1)), (64, 4, (4, 4)), (128, 3, (1, 4)), (192, 2, (4, 1)),
-                              (256, 1, (4, 4))]:
+                              (256, 1, (4, 4)), (256, 1, (1, 0)), (128, 2, (4, 0)), (64, 3, (1, 0))]:
         s = m.createSolver()
         s.setLaunch(block, cpt)
         s.setLanesPerCell(*lanes)
@@ -194,7 +195,7 @@ def test_min_timestep_device(gpu, oracle):
                 assert abs(np.float32(out.value) - want) <= np.spacing(want), (side, s)
 
 
-@pytest.mark.parametrize("lanes", [1, 4])
+@pytest.mark.parametrize("lanes", [(1, 1), (4, 4), (1, 0)])
 @pytest.mark.parametrize("budget,below", [(1, 65), (5, 65), (1, 8), (64, 65), (1 << 30, 8)])
 def test_event_budget_hot_cells(gpu, oracle, budget, below, lanes):
     """Cells that exceed the SSA event budget continue in the one-wave-per-cell kernel
@@ -204,7 +205,7 @@ def test_event_budget_hot_cells(gpu, oracle, budget, below, lanes):
     m = configs.schnakenberg_model(64)
     st = configs.schnakenberg_state(64, seed=21)
     s = m.createSolver()
-    s.setLanesPerCell(lanes)
+    s.setLanesPerCell(*lanes)
     s.setEventBudget(budget, below)
     s.synchronizeStateBufferToDevice(st)
     t = s.run(120, 4)
@@ -215,7 +216,7 @@ def test_event_budget_hot_cells(gpu, oracle, budget, below, lanes):
     mh = configs.plain_gmp_model(48)
     mh.source_mask = (0, 3, 0, 2)
     sh = mh.createSolver()
-    sh.setLanesPerCell(lanes)
+    sh.setLanesPerCell(*lanes)
     sh.setEventBudget(budget, below)
     st2 = configs.plain_gmp_state(48)
     sh.synchronizeStateBufferToDevice(st2)
