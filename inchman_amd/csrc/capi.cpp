@@ -333,7 +333,7 @@ int gpgmp_solver_kernel_times(gpgmp_solver *s, int32_t on, double *ms, int64_t *
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
         GPGMP_REQUIRE(s->runner != nullptr, GPGMP_E_STATE, "no run");
         if (ms && n) s->runner->kernel_times(ms, n, bytes);
-        s->runner->set_timing(on != 0);
+        s->runner->set_timing(on);
     });
 }
 
@@ -501,7 +501,7 @@ int gpgmp_batch_kernel_times(gpgmp_batch *b, int32_t on, double *ms, int64_t *n,
         GPGMP_REQUIRE(b, GPGMP_E_INVALID, "null argument");
         if (!b->runner) b->runner.reset(new GmpRunner(*b->eng, b->dtx, b->dty, b->seed0s));
         if (ms && n) b->runner->kernel_times(ms, n, bytes);
-        b->runner->set_timing(on != 0);
+        b->runner->set_timing(on);
     });
 }
 

@@ -328,7 +328,7 @@ void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cu
     a.lost_out = lost_out;
     a.prob = inh_ ? prob_[mode == 2 ? 1 : 0].get() : nullptr;
     a.reals = reals_dev_.get();
-    if (mode == 2) {  // no SSA: 16 cells per wave unless one lane per cell is asked for
+    if (mode == 2 || mode == 3) {  // no SSA: 16 cells per wave unless one lane per cell is asked for
         a.grp = (grp_sweep_ == 1) ? 1 : 4;
         a.grp_sweep = (grp_sweep_ == 0) ? 0 : a.grp;
     } else {
@@ -358,7 +358,7 @@ void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cu
 // ------------------------------------------------------------------------------------
 struct GmpRunner::Chunk {
     int cap = 0, nreal = 0;
-    LaunchPlan *hplan = nullptr;   // pinned [2*cap + 1]
+    LaunchPlan *hplan = nullptr;   // pinned [3*cap + 1]
     int32_t *hseed = nullptr;      // pinned [(cap + 1) * nreal]
     DevBuf<LaunchPlan> dplan;
     DevBuf<int32_t> dseed;
@@ -367,9 +367,9 @@ struct GmpRunner::Chunk {
     int graph_steps = 0;
     uint64_t graph_version = 0;  // engine shape version the graph was captured with
     Chunk(int cap_, int nreal_) : cap(cap_), nreal(nreal_) {
-        GPGMP_HIP_CHECK(hipHostMalloc((void **)&hplan, sizeof(LaunchPlan) * (2 * cap + 1)));
+        GPGMP_HIP_CHECK(hipHostMalloc((void **)&hplan, sizeof(LaunchPlan) * (3 * cap + 1)));
         GPGMP_HIP_CHECK(hipHostMalloc((void **)&hseed, sizeof(int32_t) * (size_t)(cap + 1) * nreal));
-        dplan.alloc(2 * cap + 1);
+        dplan.alloc(3 * cap + 1);
         dseed.alloc((size_t)(cap + 1) * nreal);
         GPGMP_HIP_CHECK(hipEventCreateWithFlags(&copied, hipEventDisableTiming));
     }
@@ -416,7 +416,7 @@ static double launch_bytes(const GmpEngine &e, int mode, const LaunchPlan &p) {
 
 void GmpRunner::timed_launch(int mode, const LaunchPlan *plan, const LaunchPlan &hplan, const int32_t *scur,
                              const int32_t *sprev, const int32_t *lin, int32_t *lout) {
-    if (!timing_) {
+    if (!timing_ || (launch_no_[mode == 3 ? 4 : mode]++ % every_) != 0) {
         eng_.launch(mode, plan, scur, sprev, lin, lout);
         return;
     }
@@ -432,9 +432,10 @@ void GmpRunner::timed_launch(int mode, const LaunchPlan *plan, const LaunchPlan 
         ev_mode_[ev_used_] = m;
         return ev_pool_[ev_used_++];
     };
-    const std::pair<hipEvent_t, hipEvent_t> e = next_pair(mode);
-    kt_bytes_[mode] += launch_bytes(eng_, mode, hplan);
-    if (mode == 2) {
+    const int slot = (mode == 3) ? 4 : mode;
+    const std::pair<hipEvent_t, hipEvent_t> e = next_pair(slot);
+    kt_bytes_[slot] += launch_bytes(eng_, mode, hplan);
+    if (mode == 2 || mode == 3) {
         eng_.launch(mode, plan, scur, sprev, lin, lout, e.first, e.second);
     } else {
         const std::pair<hipEvent_t, hipEvent_t> h = next_pair(3);
@@ -457,7 +458,7 @@ void GmpRunner::collect_times() {
 // Issues the launches of one chunk whose plans/seeds are already in its host buffers.
 void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
     const int nreal = eng_.num_real();
-    const int lps = eng_.inh_ ? 2 : 1;  // launches per step
+    const int lps = eng_.inh_ ? (eng_.split_x() ? 3 : 2) : 1;  // launches per step
     GPGMP_HIP_CHECK(hipMemcpyAsync(c.dplan.get(), c.hplan, sizeof(LaunchPlan) * lps * nsteps,
                                    hipMemcpyHostToDevice, eng_.stream()));
     GPGMP_HIP_CHECK(hipMemcpyAsync(c.dseed.get(), c.hseed, sizeof(int32_t) * (size_t)(nsteps + 1) * nreal,
@@ -470,6 +471,12 @@ void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
             if (!eng_.inh_) {
                 const int p = (parity_ + j) & 1;
                 timed_launch(0, c.dplan.get() + j, c.hplan[j], scur, sprev, eng_.lost(p ^ 1), eng_.lost(p));
+            } else if (eng_.split_x()) {
+                timed_launch(1, c.dplan.get() + 3 * j, c.hplan[3 * j], scur, sprev, eng_.lost(1), eng_.lost(0));
+                timed_launch(3, c.dplan.get() + 3 * j + 1, c.hplan[3 * j + 1], scur, sprev, eng_.lost(1),
+                             eng_.lost(0));
+                timed_launch(2, c.dplan.get() + 3 * j + 2, c.hplan[3 * j + 2], scur, sprev, eng_.lost(0),
+                             eng_.lost(1));
             } else {
                 timed_launch(1, c.dplan.get() + 2 * j, c.hplan[2 * j], scur, sprev, eng_.lost(1), eng_.lost(0));
                 timed_launch(2, c.dplan.get() + 2 * j + 1, c.hplan[2 * j + 1], scur, sprev, eng_.lost(0),
@@ -559,8 +566,10 @@ void GmpRunner::run(int steps, int graph_chunk) {
                 p.out_mask = d.due_x;
                 for (int s = 0; s < kMaxSpecies; ++s) p.out_off[s] = d.off_x[s];
             } else {
-                LaunchPlan &px = c.hplan[2 * j];
-                LaunchPlan &py = c.hplan[2 * j + 1];
+                const bool split = eng_.split_x();
+                const int lps = split ? 3 : 2;
+                LaunchPlan &px = c.hplan[lps * j];
+                LaunchPlan &py = c.hplan[lps * j + lps - 1];
                 std::memset(&px, 0, sizeof px);
                 std::memset(&py, 0, sizeof py);
                 px.deltat = dts;
@@ -572,6 +581,13 @@ void GmpRunner::run(int steps, int graph_chunk) {
                 for (int s = 0; s < kMaxSpecies; ++s) {
                     px.out_off[s] = d.off_x[s];
                     py.out_off[s] = d.off_y[s];
+                }
+                if (split) {  // the X sweeps move from the SSA launch to their own
+                    LaunchPlan &pxs = c.hplan[lps * j + 1];
+                    pxs = px;
+                    pxs.deltat = 0.f;
+                    pxs.gather_mask = 0u;
+                    px.out_mask = 0u;
                 }
             }
             prev_ = d;

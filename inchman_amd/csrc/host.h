@@ -170,6 +170,9 @@ class GmpEngine {
     int cpt() const { return cpt_; }
     int grp() const { return grp_; }
     int grp_sweep() const { return grp_sweep_; }
+    // inhomogeneous: the SSA and the X sweeps in separate launches (the sweep then runs
+    // with 16 cells per wave, 4 waves per SIMD), as with the default lanes (1, 0)
+    bool split_x() const { return inh_ && grp_ == 1 && grp_sweep_ == 0; }
 
     void set_state(int r, const int32_t *host);
     void get_state(int r, int32_t *host);
@@ -177,7 +180,9 @@ class GmpEngine {
     // inhomogeneous probability table of realization r, species s, axis; D, r host [A]
     void set_probs(int r, int s, int axis, const float *D, const float *rr, float dt, float dx);
 
-    // one launch (all device pointers)
+    // one launch (all device pointers).  mode 0 homogeneous, 1 inhomogeneous X kernel
+    // (gather of the Y outflow, SSA, X sweeps), 2 Y kernel, 3 X kernel with the sweep
+    // layout and no SSA (the X sweeps of a split step)
     void launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cur, const int32_t *seeds_prev,
                 const int32_t *lost_in, int32_t *lost_out, hipEvent_t ev_start = nullptr,
                 hipEvent_t ev_stop = nullptr, hipEvent_t ev_hot_start = nullptr,
@@ -219,11 +224,15 @@ class GmpRunner {
     void run(int steps, int graph_chunk);
     float sim_time() const { return sched_.sim_time(); }
     // Per-launch HIP-event timing of the eager (non-graph) launches, by kernel mode
-    // (0 homogeneous, 1 SSA + X sweeps, 2 Y sweeps, 3 hot cells); accumulated over
-    // run() calls.
-    void set_timing(bool on) { timing_ = on; }
+    // (0 homogeneous, 1 SSA (+ X sweeps unless split), 2 Y sweeps, 3 hot cells, 4 X sweeps
+    // of a split step); accumulated over run() calls.
+    // time every `every`-th launch of each kernel (0: none)
+    void set_timing(int every) {
+        timing_ = every > 0;
+        every_ = every > 0 ? every : 1;
+    }
     void kernel_times(double *ms, int64_t *n, double *bytes) const {
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < 5; ++i) {
             ms[i] = kt_ms_[i];
             n[i] = kt_n_[i];
             if (bytes) bytes[i] = kt_bytes_[i];
@@ -246,13 +255,15 @@ class GmpRunner {
                       const int32_t *sprev, const int32_t *lin, int32_t *lout);
     void collect_times();
     bool timing_ = false;
+    int every_ = 1;
+    int64_t launch_no_[5] = {0, 0, 0, 0, 0};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
     std::vector<int> ev_mode_;
     size_t ev_used_ = 0;
     // index 3: the hot-cell kernel that follows every launch with an SSA
-    double kt_ms_[4] = {0, 0, 0, 0};
-    int64_t kt_n_[4] = {0, 0, 0, 0};
-    double kt_bytes_[4] = {0, 0, 0, 0};  // algorithmic HBM bytes of the timed launches
+    double kt_ms_[5] = {0, 0, 0, 0, 0};
+    int64_t kt_n_[5] = {0, 0, 0, 0, 0};
+    double kt_bytes_[5] = {0, 0, 0, 0, 0};  // algorithmic HBM bytes of the timed launches
     int chunk_cap_ = 0;
     int graph_steps_ = 0;
 };

@@ -1,6 +1,6 @@
 // Stochastic GMP: per-cell time steps, the inhomogeneous hop-probability tables and
 // the dispatch of k_gmp_step launches to the per-mode translation units
-// (stochastic_hom.hip, stochastic_inh_x.hip, stochastic_inh_y.hip, which instantiate
+// (stochastic_hom.hip, stochastic_inh_x.hip, stochastic_inh_y.hip, stochastic_inh_xs.hip, which instantiate
 // the kernels of stochastic_kernels.h).
 #include <hip/hip_runtime.h>
 
@@ -86,6 +86,7 @@ hipError_t launch_gmp_step(const StepLaunch &a) {
         case 0: return launch_gmp_hom(a);
         case 1: return launch_gmp_inh_x(a);
         case 2: return launch_gmp_inh_y(a);
+        case 3: return launch_gmp_inh_xs(a);
         default: return hipErrorInvalidValue;
     }
 }

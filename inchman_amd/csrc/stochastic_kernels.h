@@ -30,7 +30,11 @@
 
 namespace gpgmp {
 
-enum : int { MODE_HOM = 0, MODE_INH_X = 1, MODE_INH_Y = 2 };
+// MODE_INH_XS: the X sweeps alone (no SSA code, so the sweep-only kernel keeps a small
+// register footprint), launched after the SSA launch when the two are split.
+enum : int { MODE_HOM = 0, MODE_INH_X = 1, MODE_INH_Y = 2, MODE_INH_XS = 3 };
+constexpr bool mode_x(int m) { return m == MODE_INH_X || m == MODE_INH_XS; }
+constexpr bool mode_ssa(int m) { return m == MODE_HOM || m == MODE_INH_X; }
 
 
 
@@ -304,8 +308,8 @@ __device__ __forceinline__ int32_t cell_outflow(int32_t n, int s, int c, int W, 
         const float2 pr = prob[ri.prob_off + (size_t)s * A + c];
         int32_t mi, pl_;
         inh_outflow<G>(n, (uint32_t)(c + s * A), k1, pr, mi, pl_);
-        const int32_t mm = (MODE == MODE_INH_X) ? m0 : m2;
-        const int32_t mp = (MODE == MODE_INH_X) ? m1 : m3;
+        const int32_t mm = mode_x(MODE) ? m0 : m2;
+        const int32_t mp = mode_x(MODE) ? m1 : m3;
         const int32_t lm = mm * mi, lp = mp * pl_;
         if (lead) {
             l[0 * (size_t)A + c] = lm;
@@ -334,6 +338,74 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+// One cursor of a lane's slice of the flattened calls: the cell it is in, the call
+// index inside that cell and the partial counts, added into the wave's LDS counters when
+// the cursor leaves the cell.
+template <int MODE>
+struct FlatWalker {
+    int qq;
+    uint32_t end, k, k0;
+    int32_t n;
+    float2 pr;
+    uint32_t c[5];
+
+    __device__ __forceinline__ void load(const FlatLds &L, int wave, int base, uint32_t koff) {
+        n = L.n[wave][qq];
+        pr = (MODE != MODE_HOM) ? L.pr[wave][qq] : make_float2(0.f, 0.f);
+        k0 = (uint32_t)(base + qq) + koff;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) c[j] = 0u;
+    }
+    // position at flattened call f: the cell q with st[q] <= f < st[q+1]
+    __device__ __forceinline__ void start(uint32_t f, const int32_t *st, const FlatLds &L, int wave, int NC,
+                                          int base, uint32_t koff) {
+        int a = 0, b = NC;
+        while (b - a > 1) {
+            const int m = (a + b) >> 1;
+            if ((uint32_t)st[m] <= f) a = m;
+            else b = m;
+        }
+        qq = a;
+        end = (uint32_t)st[qq + 1];
+        k = f - (uint32_t)st[qq];
+        load(L, wave, base, koff);
+    }
+    __device__ __forceinline__ void flush(FlatLds &L, int wave) {
+        atomicAdd(&L.cnt[wave][qq][0], c[0]);
+        atomicAdd(&L.cnt[wave][qq][1], c[1]);
+        if (MODE == MODE_HOM) {
+            atomicAdd(&L.cnt[wave][qq][2], c[2]);
+            atomicAdd(&L.cnt[wave][qq][3], c[3]);
+            atomicAdd(&L.cnt[wave][qq][4], c[4]);
+        }
+    }
+    // f reached the end of the current cell: move to the next cell with calls
+    __device__ __forceinline__ void advance(uint32_t f, const int32_t *st, FlatLds &L, int wave, int base,
+                                            uint32_t koff) {
+        flush(L, wave);
+        ++qq;
+        while ((uint32_t)st[qq + 1] == f) ++qq;
+        end = (uint32_t)st[qq + 1];
+        k = 0;
+        load(L, wave, base, koff);
+    }
+    // particles 2k (word x) and 2k+1 (word y, if the cell has it)
+    __device__ __forceinline__ void count(const u32x2 &w, bool valid, const HomProbs &P) {
+        const bool two = valid && (int32_t)(2 * k + 1) < n;
+        const float u = u01(w.x), v = u01(w.y);
+        if (MODE == MODE_HOM) {
+            c[0] += (valid && u <= P.stay) + (two && v <= P.stay);
+            c[1] += (valid && u <= P.p1) + (two && v <= P.p1);
+            c[2] += (valid && u <= P.p2) + (two && v <= P.p2);
+            c[3] += (valid && u <= P.p3) + (two && v <= P.p3);
+            c[4] += (valid && u <= P.p4) + (two && v <= P.p4);
+        } else {
+            c[0] += (valid && u <= pr.x) + (two && v <= pr.x);
+            c[1] += (valid && u <= pr.y) + (two && v <= pr.y);
+        }
+    }
+};
 
 template <int MODE, int LPC>
 __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, int c, int s, int W, int H, int A,
@@ -370,59 +442,28 @@ __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, in
     const uint32_t lo = (uint32_t)(((uint64_t)lane * total) >> 6);
     const uint32_t hi = (uint32_t)(((uint64_t)(lane + 1) * total) >> 6);
     if (lo < hi) {
-        int a = 0, b = NC;  // st[a] <= lo < st[b]
-        while (b - a > 1) {
-            const int m = (a + b) >> 1;
-            if ((uint32_t)st[m] <= lo) a = m;
-            else b = m;
+        // two walkers over the two halves of the slice: their Threefry chains are
+        // independent, so one wave keeps two in flight
+        const uint32_t half = (hi - lo + 1) >> 1, midf = lo + half;
+        FlatWalker<MODE> wa, wb;
+        wa.start(lo, st, L, wave, NC, base, koff);
+        const bool has_b = midf < hi;
+        if (has_b) wb.start(midf, st, L, wave, NC, base, koff);
+        else wb = wa;
+        for (uint32_t i = 0; i < half; ++i) {
+            const uint32_t fa = lo + i, fb = midf + i;
+            const bool vb = fb < hi;
+            if (fa == wa.end) wa.advance(fa, st, L, wave, base, koff);
+            if (vb && fb == wb.end) wb.advance(fb, st, L, wave, base, koff);
+            const u32x2 ra = threefry2x32_20(wa.k + 1u, kCtrHi, wa.k0, k1);
+            const u32x2 rb = threefry2x32_20(wb.k + 1u, kCtrHi, wb.k0, k1);
+            wa.count(ra, true, P);
+            wb.count(rb, vb, P);
+            ++wa.k;
+            wb.k += vb ? 1u : 0u;
         }
-        int qq = a;
-        uint32_t end = (uint32_t)st[qq + 1];
-        uint32_t k = lo - (uint32_t)st[qq];
-        int32_t n = L.n[wave][qq];
-        float2 pr = (MODE != MODE_HOM) ? L.pr[wave][qq] : make_float2(0.f, 0.f);
-        uint32_t k0 = (uint32_t)(base + qq) + koff;
-        uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
-        for (uint32_t f = lo; f < hi; ++f) {
-            if (f == end) {  // next cell with calls
-                atomicAdd(&L.cnt[wave][qq][0], c0);
-                atomicAdd(&L.cnt[wave][qq][1], c1);
-                if (MODE == MODE_HOM) {
-                    atomicAdd(&L.cnt[wave][qq][2], c2);
-                    atomicAdd(&L.cnt[wave][qq][3], c3);
-                    atomicAdd(&L.cnt[wave][qq][4], c4);
-                }
-                c0 = c1 = c2 = c3 = c4 = 0;
-                ++qq;
-                while ((uint32_t)st[qq + 1] == f) ++qq;
-                end = (uint32_t)st[qq + 1];
-                k = 0;
-                n = L.n[wave][qq];
-                if (MODE != MODE_HOM) pr = L.pr[wave][qq];
-                k0 = (uint32_t)(base + qq) + koff;
-            }
-            const u32x2 w = threefry2x32_20(k + 1u, kCtrHi, k0, k1);
-            const bool two = (int32_t)(2 * k + 1) < n;
-            const float u = u01(w.x), v = u01(w.y);
-            if (MODE == MODE_HOM) {
-                c0 += (u <= P.stay) + (two && v <= P.stay);
-                c1 += (u <= P.p1) + (two && v <= P.p1);
-                c2 += (u <= P.p2) + (two && v <= P.p2);
-                c3 += (u <= P.p3) + (two && v <= P.p3);
-                c4 += (u <= P.p4) + (two && v <= P.p4);
-            } else {
-                c0 += (u <= pr.x) + (two && v <= pr.x);
-                c1 += (u <= pr.y) + (two && v <= pr.y);
-            }
-            ++k;
-        }
-        atomicAdd(&L.cnt[wave][qq][0], c0);
-        atomicAdd(&L.cnt[wave][qq][1], c1);
-        if (MODE == MODE_HOM) {
-            atomicAdd(&L.cnt[wave][qq][2], c2);
-            atomicAdd(&L.cnt[wave][qq][3], c3);
-            atomicAdd(&L.cnt[wave][qq][4], c4);
-        }
+        wa.flush(L, wave);
+        if (has_b) wb.flush(L, wave);
     }
     wave_lds_sync();
     if (own && !deferred) {
@@ -440,8 +481,8 @@ __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, in
             l[3 * (size_t)A + c] = l3;
             xs -= l0 + l1 + l2 + l3;
         } else {
-            const int32_t mm = (MODE == MODE_INH_X) ? m0 : m2;
-            const int32_t mp = (MODE == MODE_INH_X) ? m1 : m3;
+            const int32_t mm = mode_x(MODE) ? m0 : m2;
+            const int32_t mp = mode_x(MODE) ? m1 : m3;
             const int32_t lm = mm * (int32_t)(cn[1] - cn[0]), lp = mp * (int32_t)cn[0];
             l[0 * (size_t)A + c] = lm;
             l[1 * (size_t)A + c] = lp;
@@ -502,7 +543,7 @@ __global__ __launch_bounds__(256) void k_gmp_step(
                     if (gx < W - 1) v += l[0 * (size_t)A + c + 1];
                     if (gy > 0) v += l[3 * (size_t)A + c - W];
                     if (gy < H - 1) v += l[2 * (size_t)A + c + W];
-                } else if (MODE == MODE_INH_X) {  // gathers the Y outflow of the previous launch
+                } else if (mode_x(MODE)) {  // gathers the Y outflow of the previous launch
                     if (gy > 0) v += l[1 * (size_t)A + c - W];
                     if (gy < H - 1) v += l[0 * (size_t)A + c + W];
                 } else {                          // gathers the X outflow of the previous launch
@@ -523,7 +564,7 @@ __global__ __launch_bounds__(256) void k_gmp_step(
         // 2. reactions.  A cell stops after hot.budget events and is handed to the
         //    hot-cell kernel, so a few busy cells do not hold a whole wave (and the GPU).
         bool deferred = false;
-        if (MODE != MODE_INH_Y && pl.deltat > 0.f && T.R > 0) {
+        if (mode_ssa(MODE) && pl.deltat > 0.f && T.R > 0) {
             float time = 0.f;
             uint32_t ctr = 0;
             const bool fin =
@@ -827,10 +868,10 @@ static hipError_t launch_s(const StepLaunch &a) {
     if (a.grp != 1 && a.grp != 4) return hipErrorInvalidValue;
     if (a.grp_sweep == 0) e = (a.grp == 4) ? launch_sg<S, MODE, 4, 0>(a) : launch_sg<S, MODE, 1, 0>(a);
     else if (a.grp == 4) e = launch_sg<S, MODE, 4, 4>(a);
-    else if constexpr (MODE == MODE_INH_Y) e = launch_sg<S, MODE, 1, 1>(a);  // no SSA: lanes = grp
+    else if constexpr (!mode_ssa(MODE)) e = launch_sg<S, MODE, 1, 1>(a);  // no SSA: lanes = grp
     else if (a.grp_sweep == 4) e = launch_sg<S, MODE, 1, 4>(a);
     else e = launch_sg<S, MODE, 1, 1>(a);
-    if constexpr (MODE == MODE_INH_Y) {
+    if constexpr (!mode_ssa(MODE)) {
         return e;
     } else {
         if (e != hipSuccess) return e;

@@ -8,7 +8,8 @@
 namespace gpgmp {
 
 struct StepLaunch {
-    int mode;               // 0 homogeneous, 1 inhomogeneous X sweeps (+SSA), 2 inhomogeneous Y sweeps
+    int mode;               // 0 homogeneous, 1 inhomogeneous X sweeps (+SSA), 2 inhomogeneous Y sweeps,
+                            // 3 inhomogeneous X sweeps without SSA
     int32_t *state;
     const int32_t *lost_in;
     int32_t *lost_out;
@@ -40,6 +41,7 @@ hipError_t launch_gmp_step(const StepLaunch &a);
 hipError_t launch_gmp_hom(const StepLaunch &a);
 hipError_t launch_gmp_inh_x(const StepLaunch &a);
 hipError_t launch_gmp_inh_y(const StepLaunch &a);
+hipError_t launch_gmp_inh_xs(const StepLaunch &a);
 hipError_t launch_min_dt(const float *D, const float *r, int n, float h, float *blocks, float *out,
                          hipStream_t stream);
 hipError_t launch_inh_probs(float2 *out, const float *D, const float *r, float dt, float dx, int n,

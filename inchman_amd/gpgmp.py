@@ -272,11 +272,12 @@ class Solver:
         call("gpgmp_solver_set_event_budget", self.handle, int(budget), int(below_active))
 
     def kernelTimes(self, on=True):
-        """Accumulated HIP-event times (ms, launches) per kernel mode of the current
-        run, then switches per-launch timing on/off."""
-        ms = np.zeros(4, np.float64)
-        n = np.zeros(4, np.int64)
-        b = np.zeros(4, np.float64)
+        """Accumulated HIP-event times (ms, launches timed, algorithmic bytes) per kernel
+        of the current run, then sets the timing of the following launches: False/0 off,
+        True/1 every launch, k every k-th launch of each kernel."""
+        ms = np.zeros(5, np.float64)
+        n = np.zeros(5, np.int64)
+        b = np.zeros(5, np.float64)
         call("gpgmp_solver_kernel_times", self.handle, int(on), ms.ctypes.data_as(C.POINTER(C.c_double)),
              n.ctypes.data_as(C.POINTER(C.c_int64)), b.ctypes.data_as(C.POINTER(C.c_double)))
         return ms, n, b
@@ -363,9 +364,9 @@ class Ensemble:
         call("gpgmp_batch_reset", self.handle)
 
     def kernelTimes(self, on=True):
-        ms = np.zeros(4, np.float64)
-        n = np.zeros(4, np.int64)
-        b = np.zeros(4, np.float64)
+        ms = np.zeros(5, np.float64)
+        n = np.zeros(5, np.int64)
+        b = np.zeros(5, np.float64)
         call("gpgmp_batch_kernel_times", self.handle, int(on), ms.ctypes.data_as(C.POINTER(C.c_double)),
              n.ctypes.data_as(C.POINTER(C.c_int64)), b.ctypes.data_as(C.POINTER(C.c_double)))
         return ms, n, b
