@@ -44,7 +44,11 @@ def cpu_baseline(steps):
             "sample": "first %d host-loop iterations of the 256x256 Schnakenberg run (%.1f s)" % (steps, dt)}
 
 
-TIMING_EVERY = 16  # time every 16th launch of each kernel: per-launch events add dispatch overhead
+TIMING_EVERY = 128  # time 4 of every 128 steps: events on every launch add ~22% dispatch overhead
+
+
+def progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 def plain_run(solver, st, seed0, steps, timing):
@@ -287,7 +291,8 @@ def main():
     solver.run(args.warmup, seed0)
     solver.synchronize()
     # timed: K iterations from the initial state; dispatch-tied HIP events on the solver's
-    # stream around every 16th launch of each kernel (an unbiased sample of the launches)
+    # stream around every launch of 4 consecutive steps out of every 128 (after one more
+    # step with uncounted events: events right after untimed launches read skewed)
     wall, ms, n, b = plain_run(solver, st, seed0, args.steps, timing=TIMING_EVERY)
     cells = 256 * 256
     value = cells * args.steps / wall
@@ -295,9 +300,15 @@ def main():
     dom = int(max((0, 1, 2, 4), key=lambda i: ms[i] / max(n[i], 1) * max(n[i], 1)))
     avg_ms = ms[dom] / max(n[dom], 1)
     achieved = (b[dom] / max(n[dom], 1)) / (avg_ms * 1e-3) / 1e9
-    names = {0: "k_gmp_step<S,MODE_HOM>", 1: "k_gmp_step<2,MODE_INH_X,1,0> (gather + SSA)",
-             2: "k_gmp_step<2,MODE_INH_Y,4,0> (gather + Y sweeps)", 3: "k_gmp_hot<2,MODE_INH_X> (deferred busy cells)",
-             4: "k_gmp_step<2,MODE_INH_X,4,0> (X sweeps)"}
+    # kernel names as the profiler prints them (template arguments: species, mode, SSA or
+    # sweep lanes per cell, sweep scheme), for the default launch shape
+    g_ssa, g_sweep = solver.getLaunch()[2:4]
+    assert (g_ssa, g_sweep) == (1, 0), "names below assume the default lanes per cell (1, 0)"
+    names = {0: "k_gmp_step<2, 0, 1, 0> (homogeneous)",
+             1: "k_gmp_step<2, 1, 1, 0> (gather of Y outflow + SSA)",
+             2: "k_gmp_step<2, 2, 4, 0> (gather of X outflow + Y sweeps)",
+             3: "k_gmp_hot<2, 1> (deferred busy cells)",
+             4: "k_gmp_step<2, 3, 4, 0> (X sweeps)"}
     out = {
         "metric": METRIC,
         "value": value,
@@ -323,17 +334,19 @@ def main():
     }
     extra = {"kernel_avg_launch_ms": {names[i]: ms[i] / n[i] for i in range(5) if n[i]},
              "kernel_launches_timed": {names[i]: int(n[i]) for i in range(5) if n[i]},
-             "kernel_timing": "every %dth launch of each kernel in the timed region" % TIMING_EVERY}
+             "kernel_timing": "every launch of 4 consecutive steps out of every %d of the timed region"
+                               % TIMING_EVERY}
     # same sample as the CPU baseline, for a like-for-like ratio
     w2, _, _, _ = plain_run(solver, st, seed0, args.cpu_steps, timing=False)
     extra["gpu_on_cpu_sample_cell_steps_per_s"] = cells * args.cpu_steps / w2
+    progress("headline done: %.4g cell-steps/s" % value)
     if not args.no_extra:
-        extra["ensemble"] = ensemble_extra()
-        extra["rdd"] = rdd_extra()
-        extra["det_diffusion"] = det_diffusion_extra()
-        extra["particles"] = particles_extra()
-        extra["individuals"] = individuals_extra()
+        for key, fn in (("ensemble", ensemble_extra), ("rdd", rdd_extra), ("det_diffusion", det_diffusion_extra),
+                        ("particles", particles_extra), ("individuals", individuals_extra)):
+            extra[key] = fn()
+            progress(key + " done")
     out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    progress("cpu baseline done")
     out["extra"] = extra
     print(json.dumps(out), flush=True)
 

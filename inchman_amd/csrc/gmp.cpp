@@ -414,9 +414,11 @@ static double launch_bytes(const GmpEngine &e, int mode, const LaunchPlan &p) {
     return per_cell * (double)e.total_cells();
 }
 
+// how: 0 plain launch; 1 timed (dispatch-tied start/stop events, accumulated); 2 with
+// dispatch-tied events that are not accumulated (the first step of a timing window).
 void GmpRunner::timed_launch(int mode, const LaunchPlan *plan, const LaunchPlan &hplan, const int32_t *scur,
-                             const int32_t *sprev, const int32_t *lin, int32_t *lout) {
-    if (!timing_ || (launch_no_[mode == 3 ? 4 : mode]++ % every_) != 0) {
+                             const int32_t *sprev, const int32_t *lin, int32_t *lout, int how) {
+    if (how == 0) {
         eng_.launch(mode, plan, scur, sprev, lin, lout);
         return;
     }
@@ -432,13 +434,14 @@ void GmpRunner::timed_launch(int mode, const LaunchPlan *plan, const LaunchPlan 
         ev_mode_[ev_used_] = m;
         return ev_pool_[ev_used_++];
     };
+    const bool keep = how == 1;
     const int slot = (mode == 3) ? 4 : mode;
-    const std::pair<hipEvent_t, hipEvent_t> e = next_pair(slot);
-    kt_bytes_[slot] += launch_bytes(eng_, mode, hplan);
+    const std::pair<hipEvent_t, hipEvent_t> e = next_pair(keep ? slot : -1);
+    if (keep) kt_bytes_[slot] += launch_bytes(eng_, mode, hplan);
     if (mode == 2 || mode == 3) {
         eng_.launch(mode, plan, scur, sprev, lin, lout, e.first, e.second);
     } else {
-        const std::pair<hipEvent_t, hipEvent_t> h = next_pair(3);
+        const std::pair<hipEvent_t, hipEvent_t> h = next_pair(keep ? 3 : -1);
         eng_.launch(mode, plan, scur, sprev, lin, lout, e.first, e.second, h.first, h.second);
     }
 }
@@ -448,6 +451,7 @@ void GmpRunner::collect_times() {
     GPGMP_HIP_CHECK(hipStreamSynchronize(eng_.stream()));
     for (size_t i = 0; i < ev_used_; ++i) {
         float ms = 0.f;
+        if (ev_mode_[i] < 0) continue;
         GPGMP_HIP_CHECK(hipEventElapsedTime(&ms, ev_pool_[i].first, ev_pool_[i].second));
         kt_ms_[ev_mode_[i]] += ms;
         kt_n_[ev_mode_[i]] += 1;
@@ -468,19 +472,31 @@ void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
         for (int j = 0; j < nsteps; ++j) {
             const int32_t *scur = c.dseed.get() + (size_t)(j + 1) * nreal;
             const int32_t *sprev = c.dseed.get() + (size_t)j * nreal;
+            // timing windows: of every every_ steps the first kTimingWindow carry events,
+            // and the first step of a window is not counted
+            const int64_t sn = step_no_++;
+            int how = 0;
+            if (timing_) {
+                const int64_t r = sn % every_;
+                how = every_ == 1 ? 1 : r == 0 ? 2 : r < kTimingWindow ? 1 : 0;
+            }
+            const int how_last = how;
             if (!eng_.inh_) {
                 const int p = (parity_ + j) & 1;
-                timed_launch(0, c.dplan.get() + j, c.hplan[j], scur, sprev, eng_.lost(p ^ 1), eng_.lost(p));
+                timed_launch(0, c.dplan.get() + j, c.hplan[j], scur, sprev, eng_.lost(p ^ 1), eng_.lost(p),
+                             how_last);
             } else if (eng_.split_x()) {
-                timed_launch(1, c.dplan.get() + 3 * j, c.hplan[3 * j], scur, sprev, eng_.lost(1), eng_.lost(0));
+                timed_launch(1, c.dplan.get() + 3 * j, c.hplan[3 * j], scur, sprev, eng_.lost(1), eng_.lost(0),
+                             how);
                 timed_launch(3, c.dplan.get() + 3 * j + 1, c.hplan[3 * j + 1], scur, sprev, eng_.lost(1),
-                             eng_.lost(0));
+                             eng_.lost(0), how);
                 timed_launch(2, c.dplan.get() + 3 * j + 2, c.hplan[3 * j + 2], scur, sprev, eng_.lost(0),
-                             eng_.lost(1));
+                             eng_.lost(1), how_last);
             } else {
-                timed_launch(1, c.dplan.get() + 2 * j, c.hplan[2 * j], scur, sprev, eng_.lost(1), eng_.lost(0));
+                timed_launch(1, c.dplan.get() + 2 * j, c.hplan[2 * j], scur, sprev, eng_.lost(1), eng_.lost(0),
+                             how);
                 timed_launch(2, c.dplan.get() + 2 * j + 1, c.hplan[2 * j + 1], scur, sprev, eng_.lost(0),
-                             eng_.lost(1));
+                             eng_.lost(1), how_last);
             }
         }
     };

@@ -170,9 +170,9 @@ class GmpEngine {
     int cpt() const { return cpt_; }
     int grp() const { return grp_; }
     int grp_sweep() const { return grp_sweep_; }
-    // inhomogeneous: the SSA and the X sweeps in separate launches (the sweep then runs
-    // with 16 cells per wave, 4 waves per SIMD), as with the default lanes (1, 0)
-    bool split_x() const { return inh_ && grp_ == 1 && grp_sweep_ == 0; }
+    // inhomogeneous with flattened sweeps: the SSA and the X sweeps in separate launches
+    // (the sweep then runs with 16 cells per wave, 4 waves per SIMD)
+    bool split_x() const { return inh_ && grp_sweep_ == 0; }
 
     void set_state(int r, const int32_t *host);
     void get_state(int r, int32_t *host);
@@ -226,7 +226,11 @@ class GmpRunner {
     // Per-launch HIP-event timing of the eager (non-graph) launches, by kernel mode
     // (0 homogeneous, 1 SSA (+ X sweeps unless split), 2 Y sweeps, 3 hot cells, 4 X sweeps
     // of a split step); accumulated over run() calls.
-    // time every `every`-th launch of each kernel (0: none)
+    // every = 1: time every launch; every = k > 1: time the launches of kTimingWindow - 1
+    // consecutive steps out of every k steps, after one step whose events are not counted
+    // (dispatch-tied events on launches that directly follow untimed ones read skewed
+    // times, by up to 60% of a launch here; from the second timed step on they agree with
+    // the profiler's kernel trace) (0: none)
     void set_timing(int every) {
         timing_ = every > 0;
         every_ = every > 0 ? every : 1;
@@ -252,11 +256,12 @@ class GmpRunner {
     int parity_ = 0;
     std::vector<Chunk *> chunks_;
     void timed_launch(int mode, const LaunchPlan *plan, const LaunchPlan &hplan, const int32_t *scur,
-                      const int32_t *sprev, const int32_t *lin, int32_t *lout);
+                      const int32_t *sprev, const int32_t *lin, int32_t *lout, int how);
     void collect_times();
     bool timing_ = false;
     int every_ = 1;
-    int64_t launch_no_[5] = {0, 0, 0, 0, 0};
+    static constexpr int kTimingWindow = 5;
+    int64_t step_no_ = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
     std::vector<int> ev_mode_;
     size_t ev_used_ = 0;

@@ -692,12 +692,16 @@ __device__ __forceinline__ int32_t wave_excl_scan(int32_t v) {
 // 64 lanes of a wave, speculatively: lane j draws event ctr+1+j and selects its reaction
 // with the hazards of the batch's first state; an exclusive scan of the selected net
 // changes gives every lane the state before its event; each lane then recomputes the
-// hazards of that state and its reaction.  Up to the first lane whose reaction differs
-// from the speculation (or whose total hazard is 0) every state is exact, and that lane's
-// own event is exact too, so all those events are the serial chain's.  The float clock
-// is then advanced event by event (t += log(1/r1)/h0, stop at t >= deltat) exactly as in
-// ssa_cell.  A batch commits at least one event; while the hazards change little per
-// event (busy cells) it commits tens.
+// hazards of that state and its reaction.  Up to the first lane m whose reaction differs
+// from the speculation (or whose total hazard is 0) every state is exact, and lane m's
+// own reaction is exact too.  The recomputed reactions become the next speculation and
+// the pass repeats (each pass fixes at least one more event; a handful usually fix the
+// whole batch), reusing the batch's random words and logarithms.  The float clock before
+// every event of the exact prefix is then formed lane-parallel but in the serial order of
+// additions (t += log(1/r1)/h0), and the batch stops where ssa_cell would (t >= deltat,
+// zero hazard, runaway guard), so the committed events are the serial chain's.
+constexpr int kSpecPasses = 8;
+
 template <int S>
 __device__ __forceinline__ void ssa_wave_spec(int32_t (&x)[S], uint32_t cell, uint32_t k1, float deltat,
                                               const RxTable &T, float &time, uint32_t &ctr) {
@@ -711,30 +715,52 @@ __device__ __forceinline__ void ssa_wave_spec(int32_t (&x)[S], uint32_t cell, ui
         const u32x2 w = threefry2x32_20(ctr + 1u + (uint32_t)lane, kCtrHi, cell, k1);
         const float L = log_f(1.f / u01(w.x));
         const double U2 = u01_d(w.y);
-        const uint32_t ms = fired<S>(T, h, (float)(U2 * (double)h[0]));
+        uint32_t ms = fired<S>(T, h, (float)(U2 * (double)h[0]));
         int32_t xj[S];
-#pragma unroll
-        for (int s = 0; s < S; ++s) xj[s] = x[s] + wave_excl_scan(net_of(T, ms, s));
         float hj[kMaxReactions + 1];
-        hazards<S>(T, xj, hj);
-        const uint32_t ma = fired<S>(T, hj, (float)(U2 * (double)hj[0]));
-        const bool zero = hj[0] == 0.f;
-        const uint64_t zb = __ballot(zero);
-        const uint64_t bad = __ballot(zero || ma != ms);
-        const int last = bad ? (__ffsll((unsigned long long)bad) - 1) : 63;
-        const float dt = L / hj[0];
-        int stop = -1;
-        for (int j = 0; j <= last; ++j) {
-            if ((zb >> j) & 1ull) { stop = j; break; }
-            if (ctr + (uint32_t)j == kMaxEventsPerCall) { stop = j; break; }
-            time += readlane_f(dt, j);
-            if (!(time < deltat)) { stop = j; break; }
+        uint32_t ma = 0;
+        uint64_t zb = 0;
+        int last = 63;
+        for (int pass = 0; pass < kSpecPasses; ++pass) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) xj[s] = x[s] + wave_excl_scan(net_of(T, ms, s));
+            hazards<S>(T, xj, hj);
+            ma = fired<S>(T, hj, (float)(U2 * (double)hj[0]));
+            const bool zero = hj[0] == 0.f;
+            zb = __ballot(zero);
+            const uint64_t bad = __ballot(zero || ma != ms);
+            if (!bad) {
+                last = 63;
+                break;
+            }
+            last = __ffsll((unsigned long long)bad) - 1;  // lanes 0..last exact
+            if ((zb >> last) & 1ull) break;              // the chain ends at a zero hazard
+            ms = ma;
         }
-        if (stop >= 0) {
+        // the float clock before each event, in the serial order: lane j accumulates
+        // d_0, d_1, ..., d_{j-1} one after the other, so t_j is bit-identical to the
+        // serial chain's clock (no reassociation)
+        const float dt = L / hj[0];
+        float t = time;
+#pragma unroll
+        for (int k = 0; k < 63; ++k) {
+            const float dk = readlane_f(dt, k);
+            t = (lane > k) ? t + dk : t;
+        }
+        const float t_after = t + dt;
+        // stop before event j: zero total hazard, the runaway guard, or the clock reaching
+        // deltat (the event is then not fired)
+        const uint64_t upto = (last == 63) ? ~0ull : ((2ull << last) - 1ull);
+        uint64_t stopm = (__ballot(!(t_after < deltat)) | zb) & upto;
+        const uint32_t room = kMaxEventsPerCall - ctr;  // ctr < kMaxEventsPerCall here
+        if (room <= (uint32_t)last) stopm |= 1ull << room;
+        if (stopm) {
+            const int stop = __ffsll((unsigned long long)stopm) - 1;
 #pragma unroll
             for (int s = 0; s < S; ++s) x[s] = __builtin_amdgcn_readlane(xj[s], stop);
             return;
         }
+        time = readlane_f(t_after, last);
 #pragma unroll
         for (int s = 0; s < S; ++s) x[s] = __builtin_amdgcn_readlane(xj[s] + net_of(T, ma, s), last);
         ctr += (uint32_t)last + 1u;

@@ -274,7 +274,8 @@ class Solver:
     def kernelTimes(self, on=True):
         """Accumulated HIP-event times (ms, launches timed, algorithmic bytes) per kernel
         of the current run, then sets the timing of the following launches: False/0 off,
-        True/1 every launch, k every k-th launch of each kernel."""
+        True/1 every launch, k > 1 the launches of 4 consecutive steps out of every k (after
+        one more step whose events are not counted)."""
         ms = np.zeros(5, np.float64)
         n = np.zeros(5, np.int64)
         b = np.zeros(5, np.float64)

@@ -136,10 +136,12 @@ int gpgmp_solver_get_launch(gpgmp_solver *s, int32_t *block, int32_t *cells_per_
 /* Reads the accumulated per-launch HIP-event times of the current run's eager launches
  * (ms[5], n[5], algorithmic HBM bytes[5] by kernel: 0 homogeneous, 1 SSA (+ X sweeps
  * unless they run in their own launch), 2 Y sweeps, 3 hot-cell continuation, 4 X sweeps
- * in their own launch; may be NULL), then sets the timing of the
- * following launches: on = 0 off, on = k > 0 times every k-th launch of each kernel with
- * dispatch-tied HIP events (each timed launch costs a few microseconds of extra
- * dispatch overhead, so sampling keeps the run's own speed). */
+ * in their own launch; may be NULL), then sets the timing of the following steps: on = 0
+ * off, on = 1 every launch, on = k > 1 the launches of 4 consecutive host-loop steps out
+ * of every k, each window led by one more step whose events are not counted (events on
+ * launches right after untimed ones read skewed times).  Timing uses dispatch-tied HIP
+ * events; each timed launch adds a few microseconds of dispatch overhead, so windows keep
+ * the run's own speed. */
 int gpgmp_solver_kernel_times(gpgmp_solver *s, int32_t on, double *ms, int64_t *n, double *bytes);
 /* Deferral of busy cells: a cell whose SSA has drawn `budget` events (default 8) moves
  * to the one-wave-per-cell continuation kernel once fewer than `below_active` lanes of
