@@ -47,6 +47,13 @@ def cpu_baseline(steps):
 TIMING_EVERY = 128  # time 4 of every 128 steps: events on every launch add ~22% dispatch overhead
 
 
+# HBM bytes per dispatch of the headline step kernels (by timing slot: 1 SSA launch, 2 Y
+# sweeps, 3 hot cells, 4 X sweeps), from rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs
+# over the first 2000 iterations, scaled by a plain device copy's readings (x2.0 / x1.0);
+# commands and raw counters in profiles/gmp_step_schnakenberg256.md
+TRAFFIC_BYTES = {1: 1663439, 2: 1395485, 3: 125509, 4: 870951}
+
+
 def progress(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -326,7 +333,8 @@ def main():
                                "reference-exact Threefry streams",
                    "grid": [256, 256], "species": 2, "reactions": 4, "iterations": args.steps},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES.get(dom),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/gmp_step_schnakenberg256.md)",
                      "avg_launch_ms": avg_ms, "launches_timed": int(n[dom]),
                      "algorithmic_bytes_per_launch": b[dom] / max(n[dom], 1),
                      "note": "RNG-bound (one Threefry-2x32-20 per event and per particle pair): "
