@@ -206,6 +206,7 @@ GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vecto
         ri.state_off = so;
         ri.lost_off = lo;
         ri.prob_off = po;
+        ri.cell_off = (uint64_t)total_cells_;
         // source counts per boundary as HomogeneousSolver::diffuse passes them
         // (HomogeneousSolver.cpp:141-156: boundary 0 takes sourceMask[1])
         const int32_t *sm = m.desc.source_mask;
@@ -237,9 +238,12 @@ GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vecto
     hot_.below_active = 24;
     hot_e_.alloc(hot_.cap);
     hot_count_.alloc(2);
+    hot_flag_.alloc((size_t)total_cells_);
     hot_.e = hot_e_.get();
     hot_.count = hot_count_.get();
+    hot_.flag = nullptr;
     GPGMP_HIP_CHECK(hipMemsetAsync(hot_count_.get(), 0, 2 * sizeof(uint32_t), stream_));
+    GPGMP_HIP_CHECK(hipMemsetAsync(hot_flag_.get(), 0, (size_t)total_cells_, stream_));
     reals_dev_.alloc(reals_.size());
     GPGMP_HIP_CHECK(hipMemcpyAsync(reals_dev_.get(), reals_.data(), reals_.size() * sizeof(RealInfo),
                                    hipMemcpyHostToDevice, stream_));
@@ -320,7 +324,7 @@ void GmpEngine::set_probs(int r, int s, int axis, const float *D, const float *r
 void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cur,
                        const int32_t *seeds_prev, const int32_t *lost_in, int32_t *lost_out,
                        hipEvent_t ev_start, hipEvent_t ev_stop, hipEvent_t ev_hot_start,
-                       hipEvent_t ev_hot_stop) {
+                       hipEvent_t ev_hot_stop, bool fuse) {
     StepLaunch a{};
     a.mode = mode;
     a.state = state_.get();
@@ -350,6 +354,13 @@ void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cu
     a.ev_stop = ev_stop;
     a.hot = hot_;
     a.hot_grid = (int)std::min<uint32_t>(hot_.cap, 1024u);
+    if (fuse) {
+        GPGMP_REQUIRE(fused_hot() && (mode == 1 || mode == 3), GPGMP_E_INVALID, "hot cells fuse into split steps only");
+        // the SSA launch flags its deferred cells; the X-sweep launch (plan = SSA plan + 1)
+        // continues them beside the sweeps of the others
+        a.hot.flag = hot_flag_.get();
+        if (mode == 3) a.plan_ssa = plan - 1;
+    }
     a.ev_hot_start = ev_hot_start;
     a.ev_hot_stop = ev_hot_stop;
     GPGMP_HIP_CHECK(launch_gmp_step(a));
@@ -418,8 +429,9 @@ static double launch_bytes(const GmpEngine &e, int mode, const LaunchPlan &p) {
 // dispatch-tied events that are not accumulated (the first step of a timing window).
 void GmpRunner::timed_launch(int mode, const LaunchPlan *plan, const LaunchPlan &hplan, const int32_t *scur,
                              const int32_t *sprev, const int32_t *lin, int32_t *lout, int how) {
+    const bool fuse = eng_.fused_hot() && (mode == 1 || mode == 3);
     if (how == 0) {
-        eng_.launch(mode, plan, scur, sprev, lin, lout);
+        eng_.launch(mode, plan, scur, sprev, lin, lout, nullptr, nullptr, nullptr, nullptr, fuse);
         return;
     }
     // returns a copy: growing the pool would invalidate references
@@ -438,8 +450,8 @@ void GmpRunner::timed_launch(int mode, const LaunchPlan *plan, const LaunchPlan 
     const int slot = (mode == 3) ? 4 : mode;
     const std::pair<hipEvent_t, hipEvent_t> e = next_pair(keep ? slot : -1);
     if (keep) kt_bytes_[slot] += launch_bytes(eng_, mode, hplan);
-    if (mode == 2 || mode == 3) {
-        eng_.launch(mode, plan, scur, sprev, lin, lout, e.first, e.second);
+    if (mode == 2 || mode == 3 || fuse) {
+        eng_.launch(mode, plan, scur, sprev, lin, lout, e.first, e.second, nullptr, nullptr, fuse);
     } else {
         const std::pair<hipEvent_t, hipEvent_t> h = next_pair(keep ? 3 : -1);
         eng_.launch(mode, plan, scur, sprev, lin, lout, e.first, e.second, h.first, h.second);

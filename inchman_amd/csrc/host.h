@@ -173,6 +173,8 @@ class GmpEngine {
     // inhomogeneous with flattened sweeps: the SSA and the X sweeps in separate launches
     // (the sweep then runs with 16 cells per wave, 4 waves per SIMD)
     bool split_x() const { return inh_ && grp_sweep_ == 0; }
+    // split steps continue the deferred cells inside the X-sweep launch
+    bool fused_hot() const { return split_x(); }
 
     void set_state(int r, const int32_t *host);
     void get_state(int r, int32_t *host);
@@ -182,11 +184,13 @@ class GmpEngine {
 
     // one launch (all device pointers).  mode 0 homogeneous, 1 inhomogeneous X kernel
     // (gather of the Y outflow, SSA, X sweeps), 2 Y kernel, 3 X kernel with the sweep
-    // layout and no SSA (the X sweeps of a split step)
+    // layout and no SSA (the X sweeps of a split step).  fuse (modes 1 and 3 of a split
+    // step, plans adjacent): the SSA launch flags its deferred cells instead of launching
+    // the hot-cell kernel, and the X-sweep launch continues them.
     void launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cur, const int32_t *seeds_prev,
                 const int32_t *lost_in, int32_t *lost_out, hipEvent_t ev_start = nullptr,
                 hipEvent_t ev_stop = nullptr, hipEvent_t ev_hot_start = nullptr,
-                hipEvent_t ev_hot_stop = nullptr);
+                hipEvent_t ev_hot_stop = nullptr, bool fuse = false);
 
     int32_t *lost(int i) const { return lost_[i].get(); }
 
@@ -205,6 +209,7 @@ class GmpEngine {
     DevBuf<Tile> tiles_dev_[2];  // tiles for 1 and for 4 lanes per cell
     DevBuf<HotEntry> hot_e_;
     DevBuf<uint32_t> hot_count_;
+    DevBuf<uint8_t> hot_flag_;
     HotList hot_{};
     int ntiles_[2] = {0, 0};
     int block_ = 256, cpt_ = 1, grp_ = 1, grp_sweep_ = 1;
@@ -224,8 +229,8 @@ class GmpRunner {
     void run(int steps, int graph_chunk);
     float sim_time() const { return sched_.sim_time(); }
     // Per-launch HIP-event timing of the eager (non-graph) launches, by kernel mode
-    // (0 homogeneous, 1 SSA (+ X sweeps unless split), 2 Y sweeps, 3 hot cells, 4 X sweeps
-    // of a split step); accumulated over run() calls.
+    // (0 homogeneous, 1 SSA (+ X sweeps unless split), 2 Y sweeps, 3 hot cells in their own
+    // launch, 4 X sweeps + hot cells of a split step); accumulated over run() calls.
     // every = 1: time every launch; every = k > 1: time the launches of kTimingWindow - 1
     // consecutive steps out of every k steps, after one step whose events are not counted
     // (dispatch-tied events on launches that directly follow untimed ones read skewed

@@ -492,18 +492,15 @@ __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, in
     wave_lds_sync();  // the slice is reused by the next species / cell round
 }
 
-// G lanes per cell for the SSA; the outflow by GS lanes per cell (GS = G, or 4 with
-// G = 1), or flattened over the wave (GS = 0).
+// One tile of a step launch: G lanes per cell for the SSA; the outflow by GS lanes per
+// cell (GS = G, or 4 with G = 1), or flattened over the wave (GS = 0).
 template <int S, int MODE, int G, int GS>
-__global__ __launch_bounds__(256) void k_gmp_step(
-    int32_t *__restrict__ state, const int32_t *__restrict__ lost_in, int32_t *__restrict__ lost_out,
+__device__ __forceinline__ void step_tile(
+    int t, int32_t *__restrict__ state, const int32_t *__restrict__ lost_in, int32_t *__restrict__ lost_out,
     const float2 *__restrict__ prob, const RealInfo *__restrict__ reals,
-    const Tile *__restrict__ tiles, int ntiles, const LaunchPlan *__restrict__ plan,
-    const int32_t *__restrict__ seeds_cur, const int32_t *__restrict__ seeds_prev, RxTable T,
-    HomProbs P, int4 bm, int cpt, HotList hot) {
-    // the grid has 8*ceil(ntiles/8) blocks so that the remap is a bijection
-    const int t = xcd_tile((int)blockIdx.x, ntiles);
-    if (t >= ntiles) return;
+    const Tile *__restrict__ tiles, const LaunchPlan *__restrict__ plan,
+    const int32_t *__restrict__ seeds_cur, const int32_t *__restrict__ seeds_prev, const RxTable &T,
+    const HomProbs &P, int4 bm, int cpt, const HotList &hot) {
     const Tile tile = tiles[t];
     const RealInfo ri = reals[tile.r];
     const LaunchPlan pl = *plan;
@@ -563,7 +560,9 @@ __global__ __launch_bounds__(256) void k_gmp_step(
 
         // 2. reactions.  A cell stops after hot.budget events and is handed to the
         //    hot-cell kernel, so a few busy cells do not hold a whole wave (and the GPU).
+        //    An X-sweep launch that runs beside the hot cells skips the flagged ones.
         bool deferred = false;
+        if (MODE == MODE_INH_XS && hot.flag) deferred = hot.flag[ri.cell_off + c] != 0;
         if (mode_ssa(MODE) && pl.deltat > 0.f && T.R > 0) {
             float time = 0.f;
             uint32_t ctr = 0;
@@ -589,6 +588,8 @@ __global__ __launch_bounds__(256) void k_gmp_step(
                 }
             }
         }
+
+        if (mode_ssa(MODE) && hot.flag && g == 0) hot.flag[ri.cell_off + c] = deferred ? 1 : 0;
 
         // 3. outflow of the due species (a deferred cell's comes from the hot-cell kernel)
         if (pl.out_mask) {
@@ -630,12 +631,27 @@ __global__ __launch_bounds__(256) void k_gmp_step(
             }
         }
 
-        if (g == 0) {
+        // a deferred cell's state belongs to the hot-cell work once the SSA launch is done
+        if (g == 0 && !(MODE == MODE_INH_XS && deferred)) {
 #pragma unroll
             for (int s = 0; s < S; ++s)
                 if (pl.deltat > 0.f || ((touched >> s) & 1u)) st[(size_t)s * A + c] = x[s];
         }
     }
+}
+
+template <int S, int MODE, int G, int GS>
+__global__ __launch_bounds__(256) void k_gmp_step(
+    int32_t *__restrict__ state, const int32_t *__restrict__ lost_in, int32_t *__restrict__ lost_out,
+    const float2 *__restrict__ prob, const RealInfo *__restrict__ reals,
+    const Tile *__restrict__ tiles, int ntiles, const LaunchPlan *__restrict__ plan,
+    const int32_t *__restrict__ seeds_cur, const int32_t *__restrict__ seeds_prev, RxTable T,
+    HomProbs P, int4 bm, int cpt, HotList hot) {
+    // the grid has 8*ceil(ntiles/8) blocks so that the remap is a bijection
+    const int t = xcd_tile((int)blockIdx.x, ntiles);
+    if (t >= ntiles) return;
+    step_tile<S, MODE, G, GS>(t, state, lost_in, lost_out, prob, reals, tiles, plan, seeds_cur, seeds_prev, T, P,
+                              bm, cpt, hot);
 }
 
 __device__ __forceinline__ float readlane_f(float v, int j) {
@@ -767,23 +783,24 @@ __device__ __forceinline__ void ssa_wave_spec(int32_t (&x)[S], uint32_t cell, ui
     }
 }
 
-// Hot-cell kernel: one wave per deferred cell.  The SSA continues from (time, ctr) with
-// ssa_wave_spec.  Then the cell's outflow for the due species is computed with the
-// particle loop split over the lanes (call k by lane k % 64) and the counts summed --
-// the same integers as the serial loop.  The last block to finish resets the list.
+// Deferred busy cells, one wave per cell: worker `wk` of `nwk` takes list entries wk,
+// wk + nwk, ...  The SSA continues from (time, ctr) with ssa_wave_spec under the SSA
+// launch's plan `pss`.  Then the cell's outflow for the due species of `pout` is computed
+// with the particle loop split over the lanes (call k by lane k % 64) and the counts
+// summed -- the same integers as the serial loop.  The last worker to finish resets the
+// list.
 template <int S, int MODE>
-__global__ __launch_bounds__(64) void k_gmp_hot(int32_t *__restrict__ state, int32_t *__restrict__ lost_out,
-                                                const float2 *__restrict__ prob, const RealInfo *__restrict__ reals,
-                                                const LaunchPlan *__restrict__ plan,
-                                                const int32_t *__restrict__ seeds_cur, RxTable T, HomProbs P,
-                                                int4 bm, HotList hot) {
-    const int lane = (int)threadIdx.x;
+__device__ __forceinline__ void hot_cells(int wk, int nwk, int32_t *__restrict__ state, int32_t *__restrict__ lost_out,
+                                          const float2 *__restrict__ prob, const RealInfo *__restrict__ reals,
+                                          const LaunchPlan &pss, const LaunchPlan &pl,
+                                          const int32_t *__restrict__ seeds_cur, const RxTable &T, const HomProbs &P,
+                                          int4 bm, const HotList &hot) {
+    const int lane = (int)(threadIdx.x & 63);
     const uint32_t total = hot.count[0];
     if (total == 0) return;
     const uint32_t nh = total < hot.cap ? total : hot.cap;
-    const LaunchPlan pl = *plan;
     constexpr int ND = (MODE == MODE_HOM) ? 4 : 2;
-    for (uint32_t i = blockIdx.x; i < nh; i += gridDim.x) {
+    for (uint32_t i = (uint32_t)wk; i < nh; i += (uint32_t)nwk) {
         const HotEntry he = hot.e[i];
         const RealInfo ri = reals[he.r];
         const int32_t seed = seeds_cur[he.r];
@@ -794,7 +811,7 @@ __global__ __launch_bounds__(64) void k_gmp_hot(int32_t *__restrict__ state, int
         for (int s = 0; s < S; ++s) x[s] = __builtin_amdgcn_readfirstlane(st[(size_t)s * A + c]);
         float time = he.time;
         uint32_t ctr = he.ctr;
-        ssa_wave_spec<S>(x, (uint32_t)c, kKeyBase + (uint32_t)seed, pl.deltat, T, time, ctr);
+        ssa_wave_spec<S>(x, (uint32_t)c, kKeyBase + (uint32_t)seed, pss.deltat, T, time, ctr);
         if (pl.out_mask) {
             const int gx = c % W, gy = c / W;
             const int m0 = (gx == 0) ? bm.x : 1, m1 = (gx == W - 1) ? bm.y : 1;
@@ -859,16 +876,56 @@ __global__ __launch_bounds__(64) void k_gmp_hot(int32_t *__restrict__ state, int
             for (int s = 0; s < S; ++s) st[(size_t)s * A + c] = x[s];
         }
     }
-    // every block read the count before adding itself here, so the last one may reset
+    // every worker read the count before adding itself here, so the last one may reset
     if (lane == 0) {
         __threadfence();
         const uint32_t fin = atomicAdd(&hot.count[1], 1u);
-        if (fin == gridDim.x - 1) {
+        if (fin == (uint32_t)nwk - 1u) {
             atomicExch(&hot.count[0], 0u);
             atomicExch(&hot.count[1], 0u);
             __threadfence();
         }
     }
+}
+
+// Hot-cell kernel after the step launch (one wave per block).
+template <int S, int MODE>
+__global__ __launch_bounds__(64) void k_gmp_hot(int32_t *__restrict__ state, int32_t *__restrict__ lost_out,
+                                                const float2 *__restrict__ prob, const RealInfo *__restrict__ reals,
+                                                const LaunchPlan *__restrict__ plan,
+                                                const int32_t *__restrict__ seeds_cur, RxTable T, HomProbs P,
+                                                int4 bm, HotList hot) {
+    const LaunchPlan pl = *plan;
+    hot_cells<S, MODE>((int)blockIdx.x, (int)gridDim.x, state, lost_out, prob, reals, pl, pl, seeds_cur, T, P, bm,
+                       hot);
+}
+
+// X sweeps and the deferred busy cells in one launch, so the cells' serial event chains
+// run beside the sweeps instead of before them.  Blocks [0, nhot_blocks) are hot-cell
+// workers (one per wave; they continue the SSA under the SSA launch's plan `plan_ssa`,
+// then compute their cell's X outflow); the other blocks are the X-sweep tiles (16 cells
+// per wave, flattened), which skip the cells flagged by the SSA launch.  The two kinds
+// touch disjoint cells.  nhot_blocks is a multiple of 8, so the tiles keep their XCD
+// placement.
+template <int S>
+__global__ __launch_bounds__(256) void k_gmp_xs_hot(
+    int32_t *__restrict__ state, const int32_t *__restrict__ lost_in, int32_t *__restrict__ lost_out,
+    const float2 *__restrict__ prob, const RealInfo *__restrict__ reals,
+    const Tile *__restrict__ tiles, int ntiles, const LaunchPlan *__restrict__ plan,
+    const LaunchPlan *__restrict__ plan_ssa, const int32_t *__restrict__ seeds_cur,
+    const int32_t *__restrict__ seeds_prev, RxTable T, HomProbs P, int4 bm, int cpt, HotList hot,
+    int nhot_blocks) {
+    if ((int)blockIdx.x < nhot_blocks) {
+        const int wpb = (int)(blockDim.x >> 6);
+        const LaunchPlan pss = *plan_ssa, pl = *plan;
+        hot_cells<S, MODE_INH_X>((int)blockIdx.x * wpb + (int)(threadIdx.x >> 6), nhot_blocks * wpb, state,
+                                 lost_out, prob, reals, pss, pl, seeds_cur, T, P, bm, hot);
+        return;
+    }
+    const int t = xcd_tile((int)blockIdx.x - nhot_blocks, ntiles);
+    if (t >= ntiles) return;
+    step_tile<S, MODE_INH_XS, 4, 0>(t, state, lost_in, lost_out, prob, reals, tiles, plan, seeds_cur, seeds_prev,
+                                    T, P, bm, cpt, hot);
 }
 
 
@@ -888,9 +945,32 @@ static hipError_t launch_sg(const StepLaunch &a) {
     return hipGetLastError();
 }
 
+template <int S>
+static hipError_t launch_xs_hot(const StepLaunch &a) {
+    const int wpb = a.block / 64;
+    const int nhot = 8 * ((a.hot_grid + 8 * wpb - 1) / (8 * wpb));
+    const int grid = nhot + 8 * ((a.ntiles + 7) / 8);
+    if (a.ev_start) {
+        hipExtLaunchKernelGGL((k_gmp_xs_hot<S>), dim3(grid), dim3(a.block), 0, a.stream, a.ev_start, a.ev_stop, 0,
+                              a.state, a.lost_in, a.lost_out, a.prob, a.reals, a.tiles, a.ntiles, a.plan,
+                              a.plan_ssa, a.seeds_cur, a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot, nhot);
+    } else {
+        hipLaunchKernelGGL((k_gmp_xs_hot<S>), dim3(grid), dim3(a.block), 0, a.stream, a.state, a.lost_in,
+                           a.lost_out, a.prob, a.reals, a.tiles, a.ntiles, a.plan, a.plan_ssa, a.seeds_cur,
+                           a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot, nhot);
+    }
+    return hipGetLastError();
+}
+
 template <int S, int MODE>
 static hipError_t launch_s(const StepLaunch &a) {
     hipError_t e;
+    if constexpr (MODE == MODE_INH_XS) {
+        if (a.hot.flag) {
+            if (a.grp != 4 || a.grp_sweep != 0 || !a.plan_ssa) return hipErrorInvalidValue;
+            return launch_xs_hot<S>(a);
+        }
+    }
     if (a.grp != 1 && a.grp != 4) return hipErrorInvalidValue;
     if (a.grp_sweep == 0) e = (a.grp == 4) ? launch_sg<S, MODE, 4, 0>(a) : launch_sg<S, MODE, 1, 0>(a);
     else if (a.grp == 4) e = launch_sg<S, MODE, 4, 4>(a);
@@ -900,7 +980,8 @@ static hipError_t launch_s(const StepLaunch &a) {
     if constexpr (!mode_ssa(MODE)) {
         return e;
     } else {
-        if (e != hipSuccess) return e;
+        // the hot cells run inside the X-sweep launch when the SSA launch flags them
+        if (e != hipSuccess || a.hot.flag) return e;
         // the cells that used up their event budget
         if (a.ev_hot_start) {
             hipExtLaunchKernelGGL((k_gmp_hot<S, MODE>), dim3(a.hot_grid), dim3(64), 0, a.stream, a.ev_hot_start,

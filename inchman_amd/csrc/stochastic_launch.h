@@ -9,7 +9,8 @@ namespace gpgmp {
 
 struct StepLaunch {
     int mode;               // 0 homogeneous, 1 inhomogeneous X sweeps (+SSA), 2 inhomogeneous Y sweeps,
-                            // 3 inhomogeneous X sweeps without SSA
+                            // 3 inhomogeneous X sweeps without SSA (+ the deferred cells when
+                            // hot.flag is set)
     int32_t *state;
     const int32_t *lost_in;
     int32_t *lost_out;
@@ -32,6 +33,7 @@ struct StepLaunch {
     hipEvent_t ev_stop = nullptr;
     HotList hot{};                  // deferred busy cells (modes with an SSA)
     int hot_grid = 1024;            // waves of the hot-cell kernel
+    const LaunchPlan *plan_ssa = nullptr;  // X sweeps with the hot cells: the SSA launch's plan
     hipEvent_t ev_hot_start = nullptr;  // optional: timestamps of the hot-cell dispatch
     hipEvent_t ev_hot_stop = nullptr;
 };

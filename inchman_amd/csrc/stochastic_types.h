@@ -31,6 +31,7 @@ struct RealInfo {
     uint64_t state_off;   // int32 elements: state[state_off + s*A + cell]
     uint64_t lost_off;    // int32 elements into each lost buffer: + (s*ndir + d)*A + cell
     uint64_t prob_off;    // float2 elements into the inhomogeneous probability tables
+    uint64_t cell_off;    // cells of the realizations before this one
     int32_t W, H;
     int32_t src[4];       // source-boundary counts per boundary (0 = none)
 };
@@ -68,6 +69,10 @@ struct HotList {
     uint32_t cap;
     uint32_t budget;  // SSA events a lane runs before it may defer its cell
     int32_t below_active;  // ... which it does once fewer lanes of its wave still run
+    // per cell (cell_off + c): 1 if the cell is in the list.  Non-null only when the
+    // hot cells run inside the X-sweep launch: the SSA launch then writes every cell's
+    // flag and the sweep waves skip the flagged cells.
+    uint8_t *flag;
 };
 
 // Homogeneous diffusion thresholds (StochasticHomogeneousDiffusion.cl:58-62)

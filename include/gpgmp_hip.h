@@ -135,8 +135,9 @@ int gpgmp_solver_get_launch(gpgmp_solver *s, int32_t *block, int32_t *cells_per_
                             int32_t *sweep_lanes);
 /* Reads the accumulated per-launch HIP-event times of the current run's eager launches
  * (ms[5], n[5], algorithmic HBM bytes[5] by kernel: 0 homogeneous, 1 SSA (+ X sweeps
- * unless they run in their own launch), 2 Y sweeps, 3 hot-cell continuation, 4 X sweeps
- * in their own launch; may be NULL), then sets the timing of the following steps: on = 0
+ * unless they run in their own launch), 2 Y sweeps, 3 hot-cell continuation in its own
+ * launch, 4 X sweeps in their own launch, together with the hot-cell continuation; may be
+ * NULL), then sets the timing of the following steps: on = 0
  * off, on = 1 every launch, on = k > 1 the launches of 4 consecutive host-loop steps out
  * of every k, each window led by one more step whose events are not counted (events on
  * launches right after untimed ones read skewed times).  Timing uses dispatch-tied HIP
