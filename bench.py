@@ -48,10 +48,10 @@ TIMING_EVERY = 128  # time 4 of every 128 steps: events on every launch add ~22%
 
 
 # HBM bytes per dispatch of the headline step kernels (by timing slot: 1 SSA launch, 2 Y
-# sweeps, 3 hot cells, 4 X sweeps), from rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs
+# sweeps, 4 X sweeps + hot cells), from rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs
 # over the first 2000 iterations, scaled by a plain device copy's readings (x2.0 / x1.0);
 # commands and raw counters in profiles/gmp_step_schnakenberg256.md
-TRAFFIC_BYTES = {1: 1663439, 2: 1395485, 3: 125509, 4: 870951}
+TRAFFIC_BYTES = {1: 1738201, 2: 1394951, 4: 1042763}
 
 
 def progress(msg):

@@ -414,14 +414,16 @@ GmpRunner::~GmpRunner() {
 
 // Algorithmic HBM bytes of one launch: counts read and written (all species when the
 // SSA runs, else the touched ones), ND outflow words read per gathered species, ND
-// written per species with outflow, plus the (p1,p2) table of inhomogeneous sweeps.
+// written per species with outflow, plus the (p1,p2) table of inhomogeneous sweeps and
+// the hot-cell flag byte (written by the SSA launch, read by the X sweeps) of split steps.
 static double launch_bytes(const GmpEngine &e, int mode, const LaunchPlan &p) {
     const int S = e.S();
     const int nd = (mode == 0) ? 4 : 2;
     const int ng = __builtin_popcount(p.gather_mask), no = __builtin_popcount(p.out_mask);
     const bool ssa = (mode != 2) && p.deltat > 0.f && e.table_.R > 0;
     const int nstate = ssa ? S : __builtin_popcount(p.gather_mask | p.out_mask);
-    const double per_cell = 8.0 * nstate + 4.0 * nd * ng + 4.0 * nd * no + (mode != 0 ? 8.0 * no : 0.0);
+    const double flag = (e.fused_hot() && (mode == 1 || mode == 3)) ? 1.0 : 0.0;
+    const double per_cell = 8.0 * nstate + 4.0 * nd * ng + 4.0 * nd * no + (mode != 0 ? 8.0 * no : 0.0) + flag;
     return per_cell * (double)e.total_cells();
 }
 
