@@ -241,7 +241,52 @@ def individuals_extra(n=1 << 20, side=1024, steps=20, cpu_steps=2):
     gpu = time.perf_counter() - t2
     out["cpu_oracle_1_thread_iterations_per_s"] = cpu_steps / cpu
     out["gpu_iterations_per_s_same_sample"] = cpu_steps / gpu
+    out["maintain"] = maintain_extra(side)
     return out
+
+
+def maintain_extra(side=1024, calls=20):
+    """Population maintenance after a reaction step (IndividualSolver::maintainPopulation):
+    the per-cell goal alternates between two independent Poisson(1) fields on side x side
+    cells (about 2^20 particles, ~0.37 of them killed and as many created per call), goals
+    resident on the device, a newIndividuals rule drawing one RANDOM per new particle."""
+    import numpy as np
+    import torch
+    from inchman_amd.individuals import IndividualSpecies
+
+    A = side * side
+    rng = np.random.default_rng(12)
+    goals = [rng.poisson(1.0, A).astype(np.int32) for _ in range(2)]
+    n0 = int(goals[1].sum())
+    cells = np.repeat(np.arange(A, dtype=np.uint32), goals[1])
+    props = np.ones((4, n0), np.float32)
+    cap = 3 * A
+    g = IndividualSpecies(side, side, float(side), cap)
+    g.setParticles(cells, props)
+    dev = [torch.from_numpy(x).cuda() for x in goals]
+    rule = ((2, 0, 0, 0), (0.5, 0, 0, 0), (1.0, 0, 0, 0))
+    for k in range(2):
+        g.maintainPopulation(dev[k & 1], 100 + k, rule)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(calls):
+        g.maintainPopulation(dev[k & 1], 200 + k, rule)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # the oracle on one call of the same workload
+    from oracle import oracle
+    p = oracle.ind_params(side, side, float(side))
+    keys = np.concatenate([cells, np.full(cap - n0, 0xFFFFFFFF, np.uint32)])
+    vals = np.arange(cap, dtype=np.uint32)
+    pr = np.zeros((4, cap), np.float32)
+    t1 = time.perf_counter()
+    st, *_ = oracle.ind_maintain(p, keys, vals, n0, pr, goals[0], 7, oracle.new_rule(*rule))
+    cpu = time.perf_counter() - t1
+    assert st == 0
+    return {"workload": "maintainPopulation, %dx%d cells, goals alternating between two Poisson(1) fields "
+                        "(~%d particles), RANDOM rule" % (side, side, n0),
+            "ms_per_call": wall * 1e3 / calls, "cells_per_s": A * calls / wall,
+            "cpu_oracle_1_thread_ms_per_call": cpu * 1e3}
 
 
 def particles_extra(n=10_000_000, side=1024, steps=1000):

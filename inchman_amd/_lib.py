@@ -26,6 +26,12 @@ DETERMINISTIC_HOMOGENEOUS_RK4 = 2
 DETERMINISTIC_INHOMOGENEOUS_RK4 = 3
 
 
+class NewIndividualRule(C.Structure):
+    """gpgmp_new_individual_rule: per property (DiffusivityX, DiffusivityY, DriftX, DriftY)
+    mode 0 keep, 1 constant a, 2 a + b * RANDOM."""
+    _fields_ = [("mode", C.c_int32 * 4), ("a", C.c_double * 4), ("b", C.c_double * 4)]
+
+
 class GpgmpError(RuntimeError):
     """A negative status from the C ABI (the reference raised gpgmp::Error)."""
 
@@ -162,6 +168,9 @@ SIGNATURES = {
     "gpgmp_individuals_constants": [_P, _P, _P, _P, _P],
     "gpgmp_individuals_keys_device": [_P, _P, _P, _P],
     "gpgmp_individuals_sweeps": [_P, C.POINTER(C.c_int64)],
+    "gpgmp_individuals_maintain": [_P, _P, _I, _I, _P],
+    "gpgmp_individuals_get_props": [_P, _P],
+    "gpgmp_individuals_get_pool": [_P, _P, _P, _pI],
 }
 
 _lib = None

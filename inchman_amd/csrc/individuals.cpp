@@ -51,6 +51,14 @@ struct gpgmp_individuals {
     DevBuf<uint32_t> keys, pos, keys2, pos2, ids, ids2, off, hist, tot;
     DevBuf<float> props, props2, Dx, Dy, rx, ry, blk, dt_dev;
     DevBuf<int32_t> count;
+    // dead-particle pool (a stack: top = the front of the reference's dead list) and the
+    // scratch of a population maintenance
+    DevBuf<uint32_t> pool_ids, off2, nnew, nkill, mp_tiles, mp_err, mp_out;
+    DevBuf<float> pool_props;
+    DevBuf<uint8_t> dead;
+    DevBuf<int32_t> goal_dev;
+    int np = 0;  // pool size; n + np == cap
+    uint32_t *mp_host = nullptr;  // pinned [8]
     float *dt_host = nullptr;  // pinned [2]
     float dtx = 0.f, dty = 0.f;
     int64_t sweeps = 0;  // hop sweeps done since the particles were set
@@ -58,6 +66,74 @@ struct gpgmp_individuals {
     ~gpgmp_individuals() {
         (void)hipStreamSynchronize(s);
         if (dt_host) (void)hipHostFree(dt_host);
+        if (mp_host) (void)hipHostFree(mp_host);
+    }
+
+    // IndividualSolver::maintainPopulation (IndividualSolver.cpp:458-671): make cell c hold
+    // exactly goal[c] particles.  Kills follow the reference's pick loop (RNG key
+    // {cell, 0xdecafbad + seed}); new particles recycle dead ids from the pool front in cell
+    // order and take the newIndividuals rule (RANDOM keyed {cell, 0xdecafbad + seed + 1});
+    // the list stays sorted, so only the offsets and constants are rebuilt (no sort).
+    // One host synchronisation reads the totals, as the reference reads d_total/d_totalAdded.
+    void maintain(const int32_t *goal, bool on_device, int32_t seed, const gpgmp_new_individual_rule *rule) {
+        const int32_t *gd = goal;
+        if (!on_device) {
+            GPGMP_HIP_CHECK(hipMemcpyAsync(goal_dev.get(), goal, (size_t)A * 4, hipMemcpyHostToDevice, s));
+            gd = goal_dev.get();
+        }
+        GPGMP_HIP_CHECK(launch_ind_mp_count(off.get(), gd, A, dead.get(), nnew.get(), nkill.get(), mp_tiles.get(),
+                                            mp_err.get(), mp_out.get(), kKeyBase + (uint32_t)seed, s));
+        GPGMP_HIP_CHECK(hipMemcpyAsync(mp_host, mp_out.get(), 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s));
+        GPGMP_REQUIRE(mp_host[4] == 0, GPGMP_E_INVALID, "negative goal count");
+        const uint32_t total_goal = mp_host[0], total_new = mp_host[1], total_kill = mp_host[2],
+                       first_out = mp_host[3];
+        GPGMP_REQUIRE(total_new <= (uint32_t)np, GPGMP_E_CAPACITY,
+                      "required number of individuals is greater than the capacity");
+        const uint32_t n_new = (uint32_t)n - total_kill + total_new;
+        GPGMP_REQUIRE(n_new == total_goal + ((uint32_t)n - first_out), GPGMP_E_STATE,
+                      "population counts disagree");
+        MaintainArgs m{};
+        m.A = A;
+        m.cap = cap;
+        m.n = (uint32_t)n;
+        m.np = (uint32_t)np;
+        m.total_goal = total_goal;
+        m.total_new = total_new;
+        m.total_kill = total_kill;
+        m.first_outside = first_out;
+        m.off = off.get();
+        m.nnew = nnew.get();
+        m.nkill = nkill.get();
+        m.tiles = mp_tiles.get();
+        m.keys = keys.get();
+        m.ids = ids.get();
+        m.goal = gd;
+        m.dead = dead.get();
+        m.props = props.get();
+        m.keys2 = keys2.get();
+        m.ids2 = ids2.get();
+        m.pos = pos.get();
+        m.off2 = off2.get();
+        m.pool_ids = pool_ids.get();
+        m.props2 = props2.get();
+        m.pool_props = pool_props.get();
+        m.use_rule = rule != nullptr;
+        for (int k = 0; k < 4; ++k) {
+            m.mode[k] = rule ? rule->mode[k] : 0;
+            m.a[k] = rule ? rule->a[k] : 0.0;
+            m.b[k] = rule ? rule->b[k] : 0.0;
+        }
+        m.k1_new = kKeyBase + (uint32_t)seed + 1u;
+        GPGMP_HIP_CHECK(launch_ind_mp_apply(m, s));
+        std::swap(keys, keys2);
+        std::swap(ids, ids2);
+        std::swap(props, props2);
+        std::swap(off, off2);
+        n = (int)n_new;
+        np = np - (int)total_new + (int)total_kill;
+        GPGMP_HIP_CHECK(launch_ind_constants(off.get(), props.get(), cap, A, Dx.get(), Dy.get(), rx.get(), ry.get(),
+                                             count.get(), s));
     }
 
     // IndividualSolver::computeDiffusionConstants (IndividualSolver.cpp:1100-1153):
@@ -145,6 +221,18 @@ int gpgmp_individuals_create(const gpgmp_individuals_desc *desc, int32_t capacit
         g->count.alloc(g->A);
         g->blk.alloc(g->nb);
         g->dt_dev.alloc(2);
+        g->pool_ids.alloc(cp);
+        g->pool_props.alloc(4 * cp);
+        g->dead.alloc(cp);
+        g->off2.alloc((size_t)g->A + 1);
+        g->nnew.alloc(g->A);
+        g->nkill.alloc(g->A);
+        g->goal_dev.alloc(g->A);
+        g->mp_tiles.alloc((size_t)3 * mp_tiles(g->A));
+        g->mp_err.alloc(1);
+        g->mp_out.alloc(8);
+        GPGMP_HIP_CHECK(hipMemsetAsync(g->mp_err.get(), 0, sizeof(uint32_t), g->s));
+        GPGMP_HIP_CHECK(hipHostMalloc((void **)&g->mp_host, 8 * sizeof(uint32_t)));
         GPGMP_HIP_CHECK(hipHostMalloc((void **)&g->dt_host, 2 * sizeof(float)));
         *out = g.release();
     });
@@ -164,7 +252,22 @@ int gpgmp_individuals_set(gpgmp_individuals *g, const uint32_t *cells, const flo
         std::vector<uint32_t> ids(n);
         for (int i = 0; i < n; ++i) ids[i] = (uint32_t)i;
         g->n = n;
+        g->np = g->cap - n;
         g->sweeps = 0;
+        // the pool of unused ids n .. cap-1 with the reference's defaults (diffusivities 1,
+        // drifts 0, IndividualSolver.cpp:84-98), id n at the top
+        if (g->np > 0) {
+            std::vector<uint32_t> pid((size_t)g->np);
+            for (int j = 0; j < g->np; ++j) pid[j] = (uint32_t)(g->cap - 1 - j);
+            std::vector<float> pp((size_t)g->np);
+            GPGMP_HIP_CHECK(hipMemcpyAsync(g->pool_ids.get(), pid.data(), (size_t)g->np * 4, hipMemcpyHostToDevice, g->s));
+            for (int k = 0; k < 4; ++k) {
+                std::fill(pp.begin(), pp.end(), k < 2 ? 1.0f : 0.0f);
+                GPGMP_HIP_CHECK(hipMemcpyAsync(g->pool_props.get() + (size_t)k * g->cap, pp.data(), (size_t)g->np * 4,
+                                               hipMemcpyHostToDevice, g->s));
+                GPGMP_HIP_CHECK(hipStreamSynchronize(g->s));  // pp is reused
+            }
+        }
         if (n > 0) {
             GPGMP_HIP_CHECK(hipMemcpyAsync(g->keys.get(), cells, (size_t)n * 4, hipMemcpyHostToDevice, g->s));
             GPGMP_HIP_CHECK(hipMemcpyAsync(g->ids.get(), ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, g->s));
@@ -265,6 +368,49 @@ int gpgmp_individuals_sweeps(gpgmp_individuals *g, int64_t *sweeps) {
     return guarded_i([&] {
         GPGMP_REQUIRE(g && sweeps, GPGMP_E_INVALID, "null argument");
         *sweeps = g->sweeps;
+    });
+}
+
+int gpgmp_individuals_maintain(gpgmp_individuals *g, const int32_t *goal, int32_t goal_on_device, int32_t seed,
+                               const gpgmp_new_individual_rule *rule) {
+    return guarded_i([&] {
+        GPGMP_REQUIRE(g && goal, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(g->ready, GPGMP_E_STATE, "particles are unsorted: call update first");
+        if (rule)
+            for (int k = 0; k < 4; ++k)
+                GPGMP_REQUIRE(rule->mode[k] >= 0 && rule->mode[k] <= 2, GPGMP_E_INVALID, "rule modes are 0, 1 or 2");
+        g->maintain(goal, goal_on_device != 0, seed, rule);
+    });
+}
+
+int gpgmp_individuals_get_props(gpgmp_individuals *g, float *props) {
+    return guarded_i([&] {
+        GPGMP_REQUIRE(g && props, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(g->ready, GPGMP_E_STATE, "particles are unsorted: call update first");
+        for (int k = 0; k < 4 && g->n; ++k)
+            GPGMP_HIP_CHECK(hipMemcpyAsync(props + (size_t)k * g->n, g->props.get() + (size_t)k * g->cap,
+                                           (size_t)g->n * 4, hipMemcpyDeviceToHost, g->s));
+        GPGMP_HIP_CHECK(hipStreamSynchronize(g->s));
+    });
+}
+
+int gpgmp_individuals_get_pool(gpgmp_individuals *g, uint32_t *ids, float *props, int32_t *np_out) {
+    return guarded_i([&] {
+        GPGMP_REQUIRE(g, GPGMP_E_INVALID, "null argument");
+        const size_t np = (size_t)g->np;
+        if (ids && np) {
+            GPGMP_HIP_CHECK(hipMemcpyAsync(ids, g->pool_ids.get(), np * 4, hipMemcpyDeviceToHost, g->s));
+            GPGMP_HIP_CHECK(hipStreamSynchronize(g->s));
+            std::reverse(ids, ids + np);  // front (stack top) first
+        }
+        if (props && np) {
+            for (int k = 0; k < 4; ++k)
+                GPGMP_HIP_CHECK(hipMemcpyAsync(props + (size_t)k * np, g->pool_props.get() + (size_t)k * g->cap,
+                                               np * 4, hipMemcpyDeviceToHost, g->s));
+            GPGMP_HIP_CHECK(hipStreamSynchronize(g->s));
+            for (int k = 0; k < 4; ++k) std::reverse(props + (size_t)k * np, props + (size_t)(k + 1) * np);
+        }
+        if (np_out) *np_out = g->np;
     });
 }
 

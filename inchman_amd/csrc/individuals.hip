@@ -134,6 +134,269 @@ __global__ __launch_bounds__(256) void k_ind_dt_reduce(const float *__restrict__
     if (threadIdx.x == 0) *out = m;
 }
 
+// ---- population maintenance after a reaction step (IndividualSolver::maintainPopulation) ----
+//
+// The live particles [0, n) are sorted by cell with their ids and properties in the same
+// order; the pool of dead particles is a stack (pool_ids / pool_props, top = front of the
+// reference's dead list at positions n, n+1, ... of its key list).  One maintenance:
+//   k_ind_mp_cells    per cell: kills (the reference's pick loop), new-particle and kill
+//                     counts, per-tile sums of (goal, new, kills)
+//   k_ind_mp_tiles    exclusive scan of the tile sums, totals, the first out-of-grid index
+//   k_ind_mp_new      per cell: its new particles pop ids and properties off the pool top
+//                     in cell order and take the newIndividuals rule; written first in
+//                     the cell's new range
+//   k_ind_mp_scatter  per cell: survivors after the new particles, killed particles pushed
+//                     onto the pool in list order (the first killed ends on top), new CSR
+//                     offsets
+//   k_ind_mp_outside  live particles with keys outside the grid, after the last cell
+// The cells are tiles of MP_T consecutive cells, one block each; offsets come from a
+// block-wide scan plus the tile's scanned base, so no per-cell scan array is stored.
+constexpr int MP_T = 256;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// exclusive scan of one value per thread over the MP_T-thread block; sm: MP_T/64 words
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sm, uint32_t &total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) sm[w] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < MP_T / 64; ++i) {
+        base += (i < w) ? sm[i] : 0u;
+        tot += sm[i];
+    }
+    __syncthreads();
+    total = tot;
+    return base + inc - v;
+}
+
+// x, y, x, y, ... of successive Threefry calls with counter 1, 2, ... (maintainPopulation's
+// kill loop and the RANDOM macro, IndividualSolver.cl:1059-1075, 1262-1282)
+struct PairStream {
+    uint32_t k0, k1, ri, ctr;
+    u32x2 w;
+    __device__ PairStream(uint32_t a, uint32_t b) : k0(a), k1(b), ri(0), ctr(0), w{0, 0} {}
+    __device__ __forceinline__ uint32_t next() {
+        uint32_t r;
+        if (!(ri & 1u)) {
+            ++ctr;
+            w = threefry2x32_20(ctr, kCtrHi, k0, k1);
+            r = w.x;
+        } else {
+            r = w.y;
+        }
+        ++ri;
+        return r;
+    }
+};
+
+__global__ __launch_bounds__(MP_T) void k_ind_mp_cells(const uint32_t *__restrict__ off, const int32_t *__restrict__ goal,
+                                                       int A, uint8_t *__restrict__ dead, uint32_t *__restrict__ nnew,
+                                                       uint32_t *__restrict__ nkill, uint32_t *__restrict__ tiles,
+                                                       int nt, uint32_t *__restrict__ err, uint32_t k1) {
+    __shared__ uint32_t sm[MP_T / 64];
+    const int c = blockIdx.x * MP_T + threadIdx.x;
+    uint32_t g = 0, nw = 0, nk = 0;
+    if (c < A) {
+        int32_t gi = goal[c];
+        if (gi < 0) {
+            atomicOr(err, 1u);
+            gi = 0;
+        }
+        g = (uint32_t)gi;
+        const uint32_t b = off[c], is = off[c + 1] - b;
+        for (uint32_t i = 0; i < is; ++i) dead[b + i] = 0;
+        if (is > g) {
+            nk = is - g;
+            // the reference's pick: left + (uint)(u * is), u in double, redrawn while the
+            // pick is already dead; a pick rounded up to is (only for is > 2^21) is clamped
+            PairStream rs((uint32_t)c, k1);
+            for (uint32_t k = 0; k < nk; ++k) {
+                uint32_t r;
+                do {
+                    r = (uint32_t)(u01_d(rs.next()) * (double)is);
+                    r = r < is ? r : is - 1u;
+                } while (dead[b + r]);
+                dead[b + r] = 1;
+            }
+        } else {
+            nw = g - is;
+        }
+        nnew[c] = nw;
+        nkill[c] = nk;
+    }
+    uint32_t t0, t1, t2;
+    (void)block_excl_scan(g, sm, t0);
+    (void)block_excl_scan(nw, sm, t1);
+    (void)block_excl_scan(nk, sm, t2);
+    if (threadIdx.x == 0) {
+        tiles[blockIdx.x] = t0;
+        tiles[nt + blockIdx.x] = t1;
+        tiles[2 * nt + blockIdx.x] = t2;
+    }
+}
+
+// one block: exclusive scans of the three tile-sum rows in place; out[0..2] = totals,
+// out[3] = off[A] (first particle outside the grid), out[4] = the error flag (then reset)
+__global__ __launch_bounds__(MP_T) void k_ind_mp_tiles(uint32_t *__restrict__ tiles, int nt,
+                                                       const uint32_t *__restrict__ off, int A,
+                                                       uint32_t *__restrict__ err, uint32_t *__restrict__ out) {
+    __shared__ uint32_t sm[MP_T / 64];
+    for (int row = 0; row < 3; ++row) {
+        uint32_t *t = tiles + (size_t)row * nt;
+        uint32_t carry = 0;
+        for (int b = 0; b < nt; b += MP_T) {
+            const int i = b + threadIdx.x;
+            const uint32_t v = i < nt ? t[i] : 0u;
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan(v, sm, tot);
+            if (i < nt) t[i] = carry + ex;
+            carry += tot;
+        }
+        if (threadIdx.x == 0) out[row] = carry;
+    }
+    if (threadIdx.x == 0) {
+        out[3] = off[A];
+        out[4] = *err;  // negative-goal flag of this maintenance, cleared for the next
+        *err = 0u;
+    }
+}
+
+struct NewRule {
+    int32_t mode[4];
+    double a[4], b[4];
+};
+
+__global__ __launch_bounds__(MP_T) void k_ind_mp_new(const int32_t *__restrict__ goal, const uint32_t *__restrict__ nnew,
+                                                     const uint32_t *__restrict__ tiles, int nt, int A,
+                                                     const uint32_t *__restrict__ pool_ids,
+                                                     const float *__restrict__ pool_props, uint32_t np, int cap,
+                                                     uint32_t *__restrict__ keys2, uint32_t *__restrict__ ids2,
+                                                     float *__restrict__ props2, uint32_t *__restrict__ pos,
+                                                     NewRule rule, int use_rule, uint32_t k1) {
+    __shared__ uint32_t sm[MP_T / 64];
+    const int c = blockIdx.x * MP_T + threadIdx.x;
+    const uint32_t g = c < A ? (uint32_t)max(goal[c], 0) : 0u;
+    const uint32_t nw = c < A ? nnew[c] : 0u;
+    uint32_t t;
+    const uint32_t base = tiles[blockIdx.x] + block_excl_scan(g, sm, t);
+    const uint32_t q0 = tiles[nt + blockIdx.x] + block_excl_scan(nw, sm, t);
+    if (c >= A || nw == 0) return;
+    PairStream rs((uint32_t)c, k1);
+    for (uint32_t j = 0; j < nw; ++j) {
+        const uint32_t src = np - 1u - (q0 + j), dst = base + j;
+        keys2[dst] = (uint32_t)c;
+        ids2[dst] = pool_ids[src];
+        pos[dst] = dst;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float v = pool_props[src + (size_t)k * cap];
+            if (use_rule) {
+                if (rule.mode[k] == 1) v = (float)rule.a[k];
+                if (rule.mode[k] == 2) v = (float)(rule.a[k] + rule.b[k] * u01_d(rs.next()));
+            }
+            props2[dst + (size_t)k * cap] = v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(MP_T) void k_ind_mp_scatter(
+    const uint32_t *__restrict__ off, const uint8_t *__restrict__ dead, const int32_t *__restrict__ goal,
+    const uint32_t *__restrict__ nnew, const uint32_t *__restrict__ nkill, const uint32_t *__restrict__ tiles, int nt,
+    int A, const uint32_t *__restrict__ ids, const float *__restrict__ props, int cap, uint32_t *__restrict__ keys2,
+    uint32_t *__restrict__ ids2, float *__restrict__ props2, uint32_t *__restrict__ pos,
+    uint32_t *__restrict__ pool_ids, float *__restrict__ pool_props, uint32_t np_popped, uint32_t nkill_total,
+    uint32_t *__restrict__ off2) {
+    __shared__ uint32_t sm[MP_T / 64];
+    const int c = blockIdx.x * MP_T + threadIdx.x;
+    const uint32_t g = c < A ? (uint32_t)max(goal[c], 0) : 0u;
+    const uint32_t nk = c < A ? nkill[c] : 0u;
+    uint32_t t;
+    const uint32_t base = tiles[blockIdx.x] + block_excl_scan(g, sm, t);
+    const uint32_t kb = tiles[2 * nt + blockIdx.x] + block_excl_scan(nk, sm, t);
+    if (c >= A) return;
+    off2[c] = base;
+    if (c == A - 1) off2[A] = base + g;
+    const uint32_t b = off[c], e = off[c + 1];
+    uint32_t dst = base + nnew[c], r = kb;
+    for (uint32_t i = b; i < e; ++i) {
+        if (dead[i]) {
+            // killed particles in list order onto the pool, the first killed on top
+            const uint32_t pp = np_popped + (nkill_total - 1u - r);
+            ++r;
+            pool_ids[pp] = ids[i];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) pool_props[pp + (size_t)k * cap] = props[i + (size_t)k * cap];
+        } else {
+            keys2[dst] = (uint32_t)c;
+            ids2[dst] = ids[i];
+            pos[dst] = dst;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) props2[dst + (size_t)k * cap] = props[i + (size_t)k * cap];
+            ++dst;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ind_mp_outside(uint32_t first, uint32_t n, uint32_t dst0,
+                                                        const uint32_t *__restrict__ keys,
+                                                        const uint32_t *__restrict__ ids,
+                                                        const float *__restrict__ props, int cap,
+                                                        uint32_t *__restrict__ keys2, uint32_t *__restrict__ ids2,
+                                                        float *__restrict__ props2, uint32_t *__restrict__ pos) {
+    const uint32_t i = first + blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t dst = dst0 + (i - first);
+    keys2[dst] = keys[i];
+    ids2[dst] = ids[i];
+    pos[dst] = dst;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) props2[dst + (size_t)k * cap] = props[i + (size_t)k * cap];
+}
+
+int mp_tiles(int A) { return (A + MP_T - 1) / MP_T; }
+
+hipError_t launch_ind_mp_count(const uint32_t *off, const int32_t *goal, int A, uint8_t *dead, uint32_t *nnew,
+                               uint32_t *nkill, uint32_t *tiles, uint32_t *err, uint32_t *out, uint32_t k1,
+                               hipStream_t s) {
+    const int nt = mp_tiles(A);
+    hipLaunchKernelGGL(k_ind_mp_cells, dim3(nt), dim3(MP_T), 0, s, off, goal, A, dead, nnew, nkill, tiles, nt, err, k1);
+    hipLaunchKernelGGL(k_ind_mp_tiles, dim3(1), dim3(MP_T), 0, s, tiles, nt, off, A, err, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_ind_mp_apply(const MaintainArgs &m, hipStream_t s) {
+    const int nt = mp_tiles(m.A);
+    NewRule rule{};
+    for (int k = 0; k < 4; ++k) {
+        rule.mode[k] = m.mode[k];
+        rule.a[k] = m.a[k];
+        rule.b[k] = m.b[k];
+    }
+    if (m.total_new)
+        hipLaunchKernelGGL(k_ind_mp_new, dim3(nt), dim3(MP_T), 0, s, m.goal, m.nnew, m.tiles, nt, m.A, m.pool_ids,
+                           m.pool_props, m.np, m.cap, m.keys2, m.ids2, m.props2, m.pos, rule, m.use_rule, m.k1_new);
+    hipLaunchKernelGGL(k_ind_mp_scatter, dim3(nt), dim3(MP_T), 0, s, m.off, m.dead, m.goal, m.nnew, m.nkill, m.tiles,
+                       nt, m.A, m.ids, m.props, m.cap, m.keys2, m.ids2, m.props2, m.pos, m.pool_ids, m.pool_props,
+                       m.np - m.total_new, m.total_kill, m.off2);
+    if (m.n > m.first_outside) {
+        const uint32_t cnt = m.n - m.first_outside;
+        hipLaunchKernelGGL(k_ind_mp_outside, dim3((cnt + 255) / 256), dim3(256), 0, s, m.first_outside, m.n,
+                           m.total_goal, m.keys, m.ids, m.props, m.cap, m.keys2, m.ids2, m.props2, m.pos);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_ind_sweep(int axis, uint32_t *keys, const uint32_t *off, const float *props, int cap, int W, int H,
                             uint32_t key_off, uint32_t k1, float dt, float dx, int4 bm, hipStream_t s) {
     const int A = W * H;

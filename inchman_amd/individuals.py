@@ -6,6 +6,9 @@ Each particle is a cell key plus four properties (DiffusivityX, DiffusivityY, Dr
 DriftY).  A sweep hops every particle along one axis with the inhomogeneous
 drift-diffusion probabilities of its own properties; the constants update re-sorts the
 particles by cell and recomputes the per-cell maxima, counts and the next time step.
+After a reaction step, maintainPopulation makes every cell hold the species' new count
+(IndividualSolver::maintainPopulation, IndividualSolver.cpp:458-671): it kills randomly
+picked particles and creates new ones from the pool of dead ids.
 """
 import ctypes as C
 
@@ -100,3 +103,45 @@ class IndividualSpecies:
         out = [np.zeros(self.width * self.height, np.float32) for _ in range(4)]
         call("gpgmp_individuals_constants", self.handle, *[o.ctypes.data for o in out])
         return out
+
+    def maintainPopulation(self, goal, seed, rule=None):
+        """IndividualSolver::maintainPopulation(species, seed, d_state): afterwards cell c holds
+        goal[c] particles.  goal: host int32 [H][W] / [W*H], or a device pointer (int) / torch
+        tensor on the GPU (e.g. the species' slice of the solver state).  rule: None (keep the
+        recycled particle's properties) or ((mode0..3), (a0..3), (b0..3)) per property with mode
+        0 keep, 1 constant a, 2 a + b * RANDOM."""
+        r = None
+        if rule is not None:
+            r = _lib.NewIndividualRule()
+            modes, a, b = rule
+            for k in range(4):
+                r.mode[k], r.a[k], r.b[k] = int(modes[k]), float(a[k]), float(b[k])
+        if isinstance(goal, int):
+            ptr, on_dev = goal, 1
+        elif hasattr(goal, "data_ptr"):
+            assert goal.is_cuda and goal.is_contiguous() and goal.numel() == self.width * self.height
+            ptr, on_dev = goal.data_ptr(), 1
+        else:
+            g = np.ascontiguousarray(goal, np.int32).reshape(-1)
+            assert g.size == self.width * self.height
+            ptr, on_dev, keep = g.ctypes.data, 0, g  # noqa: F841 (keeps the buffer alive)
+        call("gpgmp_individuals_maintain", self.handle, ptr, on_dev, int(seed), C.byref(r) if r is not None else None)
+        npool = C.c_int32()
+        call("gpgmp_individuals_get_pool", self.handle, None, None, C.byref(npool))
+        self.n = self.capacity - npool.value
+
+    def getPropertiesList(self):
+        """Properties [4][n] of the live particles in sorted order."""
+        out = np.zeros((4, max(self.n, 1)), np.float32)
+        call("gpgmp_individuals_get_props", self.handle, out.ctypes.data)
+        return out[:, : self.n] if self.n else np.zeros((4, 0), np.float32)
+
+    def pool(self):
+        """(ids, props[4][np]) of the dead-particle pool, front first."""
+        npool = C.c_int32()
+        call("gpgmp_individuals_get_pool", self.handle, None, None, C.byref(npool))
+        ids = np.zeros(max(npool.value, 1), np.uint32)
+        props = np.zeros((4, max(npool.value, 1)), np.float32)
+        call("gpgmp_individuals_get_pool", self.handle, ids.ctypes.data, props.ctypes.data, C.byref(npool))
+        m = npool.value
+        return ids[:m], props.reshape(-1)[: 4 * m].reshape(4, m) if m else np.zeros((4, 0), np.float32)

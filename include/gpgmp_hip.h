@@ -300,6 +300,37 @@ int gpgmp_individuals_sweeps(gpgmp_individuals *g, int64_t *sweeps);
 /* Device pointers of the sorted keys, particle indices and CSR cell offsets [W*H+1]. */
 int gpgmp_individuals_keys_device(gpgmp_individuals *g, uint32_t **keys, uint32_t **vals, uint32_t **off);
 
+/* The model's newIndividuals method (DiffusionModel::getNewIndividualsMethod, pasted into
+ * updateNewParticles at IndividualSolver.cl:1284) as a rule per property (DiffusivityX,
+ * DiffusivityY, DriftX, DriftY): mode 0 keeps the recycled particle's stored value (an
+ * empty method), 1 sets the constant a, 2 sets a + b * RANDOM evaluated in double and stored
+ * as float, RANDOM being the reference's macro (IndividualSolver.cl:1262-1282), drawn in
+ * property order. */
+typedef struct {
+    int32_t mode[4];
+    double a[4], b[4];
+} gpgmp_new_individual_rule;
+/* IndividualSolver::maintainPopulation(speciesIndex, seed, d_state) (IndividualSolver.cpp:458-671,
+ * kernels IndividualSolver.cl:1017-1599), called by Solver::doReactions after the SSA for each
+ * individual species with seed++ (Solver.cpp:241-247): afterwards cell c holds exactly goal[c]
+ * particles (goal = the species' counts after the reactions, W*H int32, on the device if
+ * goal_on_device, e.g. the solver's state from gpgmp_solver_state_device, else host).
+ * Kills pick uniformly with the reference's loop (RNG key {cell, 0xdecafbad + seed}); new
+ * particles recycle dead ids from the front of the pool in cell order and go before the
+ * cell's survivors; `rule` (NULL = keep) sets their properties with RANDOM keyed
+ * {cell, 0xdecafbad + seed + 1}.  The list stays sorted; the per-cell counts and constants
+ * are refreshed, the time steps are not (as in the reference).  GPGMP_E_CAPACITY when the
+ * pool holds fewer ids than needed (nothing changes); GPGMP_E_INVALID for a negative goal. */
+int gpgmp_individuals_maintain(gpgmp_individuals *g, const int32_t *goal, int32_t goal_on_device, int32_t seed,
+                               const gpgmp_new_individual_rule *rule);
+/* Properties of the live particles in sorted order, props[4][n]
+ * (IndividualSolver::getPropertiesList, IndividualSolver.h:36-297). */
+int gpgmp_individuals_get_props(gpgmp_individuals *g, float *props);
+/* The dead-particle pool front first, i.e. the reference's key list after the live
+ * particles (positions n .. cap-1, key UINT_MAX): ids[cap-n] and props[4][cap-n] (either
+ * may be NULL), and its size. */
+int gpgmp_individuals_get_pool(gpgmp_individuals *g, uint32_t *ids, float *props, int32_t *np);
+
 #ifdef __cplusplus
 }
 #endif

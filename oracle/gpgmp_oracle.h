@@ -158,6 +158,16 @@ float oracle_ind_cell_dt(float D, float r, float h);
 int oracle_ind_run(const oracle_ind_params *p, uint32_t *keys, uint32_t *vals, int n, const float *props, int cap,
                    int steps, uint32_t seed0, int32_t *count, float *sim_time_out);
 
+/* ---- population maintenance of an individual species (IndividualSolver::maintainPopulation) ---- */
+typedef struct {
+    int32_t mode[4]; /* per property: 0 keep the recycled id's value, 1 constant a, 2 a + b * RANDOM */
+    double a[4], b[4];
+} oracle_new_rule;
+/* keys/vals: the full list [cap] (live sorted in [0, n), dead ids after); props [4][cap] by id;
+ * goal[W*H]; returns 0, -1 (pool too small: nothing changed) or -2 (negative goal). */
+int oracle_ind_maintain(const oracle_ind_params *p, uint32_t *keys, uint32_t *vals, int n, int cap, float *props,
+                        const int32_t *goal, int32_t seed, const oracle_new_rule *rule, int *n_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,10 @@ class IndParams(C.Structure):
                 ("species", C.c_int)]
 
 
+class NewRule(C.Structure):
+    _fields_ = [("mode", C.c_int32 * 4), ("a", C.c_double * 4), ("b", C.c_double * 4)]
+
+
 _lib = None
 
 
@@ -103,6 +107,9 @@ def lib():
         L.oracle_ind_update.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_void_p, C.c_int] + [C.c_void_p] * 9
         L.oracle_ind_run.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_uint32,
                                                         C.c_void_p, C.c_void_p]
+        L.oracle_ind_maintain.restype = C.c_int
+        L.oracle_ind_maintain.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int32,
+                                                             C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -387,3 +394,25 @@ def ind_run(p, keys, vals, props, steps, seed0):
     lib().oracle_ind_run(C.byref(p), _p(k), _p(v), k.size, _p(pr), cap, steps, C.c_uint32(seed0), _p(count),
                          C.byref(t))
     return k, v, count, t.value
+
+
+def new_rule(mode=(0, 0, 0, 0), a=(0, 0, 0, 0), b=(0, 0, 0, 0)):
+    r = NewRule()
+    for i in range(4):
+        r.mode[i], r.a[i], r.b[i] = int(mode[i]), float(a[i]), float(b[i])
+    return r
+
+
+def ind_maintain(p, keys, vals, n, props, goal, seed, rule=None):
+    """maintainPopulation on the reference's full list: keys/vals [cap] (live sorted in
+    [0, n), dead ids after), props [4][cap] by id.  Returns (status, keys, vals, props, n_new)."""
+    k = np.ascontiguousarray(keys, np.uint32).copy()
+    v = np.ascontiguousarray(vals, np.uint32).copy()
+    pr = np.ascontiguousarray(props, np.float32).copy()
+    assert pr.shape == (4, k.size) and v.size == k.size
+    g = np.ascontiguousarray(goal, np.int32)
+    assert g.size == p.W * p.H
+    n_out = C.c_int(n)
+    st = lib().oracle_ind_maintain(C.byref(p), _p(k), _p(v), n, k.size, _p(pr), _p(g), seed,
+                                   C.byref(rule) if rule is not None else None, C.byref(n_out))
+    return st, k, v, pr, n_out.value
