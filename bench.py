@@ -53,6 +53,13 @@ TIMING_EVERY = 128  # time 4 of every 128 steps: events on every launch add ~22%
 # commands and raw counters in profiles/gmp_step_schnakenberg256.md
 TRAFFIC_BYTES = {1: 1738201, 2: 1394951, 4: 1042763}
 
+# wave64 VALU instructions per dispatch of the same kernels (rocprofv3 SQ_INSTS_VALU summed
+# over the chip, averaged over launches 2..10001 of a full 10^4-iteration run; command in
+# profiles/gmp_step_schnakenberg256.md) and MI355X's VALU issue peak: 256 CUs x 4 SIMDs,
+# one wave64 32-bit VALU instruction per SIMD every 4 cycles (16 lanes), 2.4 GHz
+VALU_INSTS = {1: 1.86328e6, 2: 8.48323e6, 4: 9.78619e6}
+VALU_PEAK = 256 * 4 * 2.4e9 / 4
+
 
 def progress(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
@@ -383,9 +390,16 @@ def main():
                      "avg_launch_ms": avg_ms, "launches_timed": int(n[dom]),
                      "algorithmic_bytes_per_launch": b[dom] / max(n[dom], 1),
                      "note": "RNG-bound (one Threefry-2x32-20 per event and per particle pair): "
-                             "VALU busy is the binding resource, HBM bytes are small"},
+                             "VALU issue is the binding resource, HBM bytes are small",
+                     "valu": {"achieved": VALU_INSTS[dom] / (avg_ms * 1e-3), "peak": VALU_PEAK,
+                              "unit": "wave64 VALU instructions/s", "frac": VALU_INSTS[dom] / (avg_ms * 1e-3) / VALU_PEAK,
+                              "instructions_per_launch": VALU_INSTS[dom],
+                              "source": "rocprofv3 --pmc SQ_INSTS_VALU over a full run "
+                                        "(profiles/gmp_step_schnakenberg256.md) / this run's launch time"}},
     }
     extra = {"kernel_avg_launch_ms": {names[i]: ms[i] / n[i] for i in range(5) if n[i]},
+             "kernel_valu_frac": {names[i]: VALU_INSTS[i] / (ms[i] / n[i] * 1e-3) / VALU_PEAK
+                                  for i in VALU_INSTS if n[i]},
              "kernel_launches_timed": {names[i]: int(n[i]) for i in range(5) if n[i]},
              "kernel_timing": "every launch of 4 consecutive steps out of every %d of the timed region"
                                % TIMING_EVERY}
