@@ -212,6 +212,47 @@ def det_diffusion_extra(side=8192, reps=20):
     return out
 
 
+def aqss_extra(side=1024, dt=0.1, reps=5, cpu_side=64):
+    """alpha-QSS reactions (deterministic_performReaction_aqss) of the Schnakenberg network
+    in concentration form on side x side cells (u ~ U(50, 250), v ~ U(20, 120)) over dt,
+    with the reference's corrector factor and with the fixed one; the oracle's float
+    instance (one thread) on a cpu_side x cpu_side sample beside it."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from inchman_amd import configs
+    from inchman_amd._lib import AQSS_PADE_FIX, call
+    from oracle import oracle
+
+    m = configs.schnakenberg_model(side)
+    cm = m.compile()
+    rng = np.random.default_rng(2)
+    st = np.stack([rng.uniform(50, 250, (side, side)), rng.uniform(20, 120, (side, side))]).astype(np.float32)
+    src = torch.from_numpy(st).cuda()
+    x = src.clone()
+    sh = int(torch.cuda.current_stream().cuda_stream)
+    out = {"workload": "alpha-QSS, Schnakenberg (concentration form), %dx%d cells, dt %g" % (side, side, dt)}
+    for name, flags in (("reference_corrector", 0), ("fixed_corrector", AQSS_PADE_FIX)):
+        x.copy_(src)
+        call("gpgmp_det_aqss_device", cm.handle, x.data_ptr(), dt, flags, sh)  # warmup
+        torch.cuda.synchronize()
+        t = 0.0
+        for _ in range(reps):
+            x.copy_(src)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            call("gpgmp_det_aqss_device", cm.handle, x.data_ptr(), dt, flags, sh)
+            torch.cuda.synchronize()
+            t += time.perf_counter() - t0
+        out[name] = {"ms_per_call": t * 1e3 / reps, "cells_per_s": side * side * reps / t}
+    tbl = oracle.table_from_model(m)
+    t0 = time.perf_counter()
+    oracle.det_aqss(st[:, :cpu_side, :cpu_side], tbl, dt, np.float32, False)
+    cpu = time.perf_counter() - t0
+    out["cpu_oracle_1_thread_cells_per_s"] = cpu_side * cpu_side / cpu
+    return out
+
+
 def individuals_extra(n=1 << 20, side=1024, steps=20, cpu_steps=2):
     """A particle-based species (IndividualSolver): n particles with their own
     diffusivities/drifts on a side x side reflective grid; each host-loop iteration does
@@ -409,7 +450,8 @@ def main():
     progress("headline done: %.4g cell-steps/s" % value)
     if not args.no_extra:
         for key, fn in (("ensemble", ensemble_extra), ("rdd", rdd_extra), ("det_diffusion", det_diffusion_extra),
-                        ("particles", particles_extra), ("individuals", individuals_extra)):
+                        ("particles", particles_extra), ("individuals", individuals_extra),
+                        ("aqss", aqss_extra)):
             extra[key] = fn()
             progress(key + " done")
     out["cpu_baseline"] = cpu_baseline(args.cpu_steps)

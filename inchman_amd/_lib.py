@@ -24,6 +24,9 @@ STOCHASTIC_HOMOGENEOUS = 0
 STOCHASTIC_INHOMOGENEOUS = 1
 DETERMINISTIC_HOMOGENEOUS_RK4 = 2
 DETERMINISTIC_INHOMOGENEOUS_RK4 = 3
+DETERMINISTIC_HOMOGENEOUS_AQSS = 4
+DETERMINISTIC_INHOMOGENEOUS_AQSS = 5
+AQSS_PADE_FIX = 1
 
 
 class NewIndividualRule(C.Structure):
@@ -147,6 +150,8 @@ SIGNATURES = {
     "gpgmp_det_hom_diffuse_device": [_P, _P, _I, _I, _I, _pF, _F, _F, _P],
     "gpgmp_det_inhom_sweep_device": [_P, _P, _I, _I, _I, _I, _F, _P, _P, _F, _P],
     "gpgmp_det_rk4_device": [_P, _P, _F, _P],
+    "gpgmp_det_aqss_device": [_P, _P, _F, _I, _P],
+    "gpgmp_moments_x_device": [_P, _I, _I, _I, _P, _P],
     "gpgmp_sort_pairs_device": [_P, _P, _P, _P, _I, _I, _P],
     "gpgmp_cell_offsets_device": [_P, _I, _I, _P, _P],
     "gpgmp_cell_left_right_device": [_P, _I, _I, _P, _P, _P],

@@ -31,8 +31,20 @@ def sources():
     return sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
+def _newest_header():
+    hdrs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".h", ".hpp"))]
+    inc = os.path.join(HERE, "..", "include")
+    hdrs += [os.path.join(inc, f) for f in os.listdir(inc)]
+    return max(os.path.getmtime(h) for h in hdrs)
+
+
 def _compile(src):
     obj = os.path.join(BUILD, src + ".o")
+    # GPGMP_INCREMENTAL=1 (development only) reuses objects newer than their source and
+    # every header; the default compiles everything
+    if os.environ.get("GPGMP_INCREMENTAL") == "1" and os.path.exists(obj):
+        if os.path.getmtime(obj) > max(os.path.getmtime(os.path.join(CSRC, src)), _newest_header()):
+            return obj
     cmd = [HIPCC] + COMMON + ["-I", os.path.join(HERE, "..", "include")]
     cmd += DEVICE + (["-x", "hip"] if src.endswith(".hip") else [])
     cmd += ["-c", os.path.join(CSRC, src), "-o", obj]

@@ -209,6 +209,150 @@ __global__ void k_det_rk4(float *__restrict__ state, int A, RxTable T, float dt)
     for (int s = 0; s < S; ++s) state[(size_t)s * A + c] = x[s];
 }
 
+// deterministic_performReaction_aqss (Deterministic.cl:291-522): Mott & Oran's alpha-QSS
+// predictor-corrector over dtg, one thread per cell, every per-cell array of the reference
+// (d_ys, d_qs, d_rtaus, d_y1, d_scrarray and the state in LDS) in registers.  Production
+// q and destruction d per species as deterministic_rhs_ode_aqss computes them
+// (Solver.cpp:960-1010): 0. + |n|. * k * X * Y + ... in double, in reaction order, rounded
+// to float.  Float variables with the reference's double-literal promotions statement by
+// statement; fixed = 1 puts the predictor's Pade factor into the corrector, 0 keeps the
+// reference's `180.*+rtaub*(...)` (Deterministic.cl:423).  itermax = 2, so the stability
+// factor is the constant 0.01.  At most AQSS_MAX_ITER inner passes per cell; a cell that
+// reaches the cap is set to NaN.
+constexpr int AQSS_MAX_ITER = 1 << 16;
+
+template <int S>
+__device__ __forceinline__ void aqss_rhs(const RxTable &T, const float (&y)[S], float (&q)[S], float (&d)[S]) {
+    double qs[S], ds[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) qs[s] = ds[s] = 0.;
+    for (int i = 0; i < T.R; ++i) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int n = T.net[i][s];
+            if (n == 0) continue;
+            double term = (double)(n > 0 ? n : -n) * T.lit_dbl[i];
+            for (int f = 0; f < T.nf[i]; ++f) {
+                float v = y[0];
+#pragma unroll
+                for (int k = 1; k < S; ++k) v = (T.fs[i][f] == k) ? y[k] : v;
+                term = term * (double)v;
+            }
+            if (n > 0)
+                qs[s] = qs[s] + term;
+            else
+                ds[s] = ds[s] + term;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        q[s] = (float)qs[s];
+        d[s] = (float)ds[s];
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_det_aqss(float *__restrict__ state, int A, RxTable T, float dtg,
+                                                  int fixed) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= A) return;
+    const float epsmin = 1e-2f, sqreps = 0.5f, epscl = 100.f, epsmax = 10.f, tfd = 1.000008f, dtmin = 1e-15f,
+                ymin = 1e-20f;
+    float y[S], q[S], d[S], ys[S], qsv[S], rtaus[S], y1[S], scr[S];
+    float tn = 0.f;
+#pragma unroll
+    for (int i = 0; i < S; ++i) y[i] = fmaxf(state[(size_t)i * A + c], ymin);
+    aqss_rhs<S>(T, y, q, d);
+    float scrtch = 1e-25f;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const float ascr = fabsf(q[i]);
+        const float scr2 = copysignf(1.f / y[i], 0.1f * epsmin * ascr - d[i]);
+        const float scr1 = scr2 * d[i];
+        scrtch = fmaxf(fmaxf(scr1, -fabsf(ascr - d[i]) * scr2), scrtch);
+    }
+    float dt = fminf(sqreps / scrtch, dtg);
+    float ts, dtcurrent, eps;
+    int iters = 0;
+    bool capped = false;
+    while (dtg > tn * tfd && !capped) {
+        ts = tn;
+        dtcurrent = dt;
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            ys[i] = y[i];
+            qsv[i] = q[i];
+            rtaus[i] = dtcurrent * d[i] / y[i];
+        }
+        while (dtg > tn * tfd) {
+            if (++iters > AQSS_MAX_ITER) {
+                capped = true;
+                break;
+            }
+#pragma unroll
+            for (int i = 0; i < S; ++i) {
+                const float rtaui = dtcurrent * d[i] / y[i];
+                const float alpha = (float)((180. + rtaui * (60. + rtaui * (11. + rtaui))) /
+                                            (360. + rtaui * (60. + rtaui * (12. + rtaui))));
+                scr[i] = (float)((q[i] - d[i]) / (1. + alpha * rtaui));
+            }
+            for (int iteration = 0; iteration < 2; iteration++) {
+#pragma unroll
+                for (int i = 0; i < S; ++i) y[i] = fmaxf(ys[i] + dt * scr[i], ymin);
+                if (iteration == 0) {
+                    tn = ts + dt;
+#pragma unroll
+                    for (int i = 0; i < S; ++i) y1[i] = y[i];
+                }
+                aqss_rhs<S>(T, y, q, d);
+                eps = 1e-10f;
+#pragma unroll
+                for (int i = 0; i < S; ++i) {
+                    const float rtaub = (float)(0.5 * (rtaus[i] + dt * d[i] / y[i]));
+                    const double px = 60. + rtaub * (11. + rtaub);
+                    const double pd = 360. + rtaub * (60. + rtaub * (12. + rtaub));
+                    // `180.*+rtaub*(...)` parses as (180. * rtaub) * (...)
+                    const float alpha = fixed ? (float)((180. + rtaub * px) / pd) : (float)((180. * rtaub) * px / pd);
+                    const float qt = (float)(qsv[i] * (1. - alpha) + q[i] * alpha);
+                    const float pb = rtaub / dt;
+                    scr[i] = (float)((qt - ys[i] * pb) / (1. + alpha * rtaub));
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < S; ++i) {
+                const float scr2 = fmaxf(ys[i] + dt * scr[i], 0.f);
+                float scr1 = fabsf(scr2 - y1[i]);
+                y[i] = fmaxf(scr2, ymin);
+                if (0.25 * (ys[i] + y[i]) > ymin) {
+                    scr1 = scr1 / y[i];
+                    eps = fmaxf(0.5f * (scr1 + fminf(fabsf(q[i] - d[i]) / (q[i] + d[i] + 1e-30f), scr1)), eps);
+                }
+            }
+            eps = eps * epscl;
+            if (dt <= dtmin + 1e-16 * tn) {
+                dt = dtg - ts;
+                dt = fminf(dtmin, fabsf(dt));
+            }
+            const float stab = 0.01f;
+            if (eps <= epsmax && stab <= 1.) {
+                if (dtg <= tn * tfd) break;
+            } else {
+                tn = ts;
+            }
+            const float rteps = sqrtf(eps);
+            float dto = dt;
+            dt = fminf(fminf(dt * (1.0f / rteps + .005f), tfd * (dtg - tn)), dto / (stab + 0.001f));
+            if (eps <= epsmax && stab <= 1) break;
+            dto = dt / dto;
+#pragma unroll
+            for (int i = 0; i < S; ++i) rtaus[i] *= dto;
+        }
+        if (!capped) aqss_rhs<S>(T, y, q, d);
+    }
+#pragma unroll
+    for (int i = 0; i < S; ++i) state[(size_t)i * A + c] = capped ? __builtin_nanf("") : y[i];
+}
+
 // ---------------------------------------------------------------------------------------
 // Fused Schnakenberg reaction-diffusion step (no reference counterpart; the diffusion is
 // the FPE stencil of Deterministic.cl:71-136 with zero drift, both axes in one Jacobi
@@ -538,6 +682,22 @@ hipError_t launch_det_inhom_sweep(const float *in, float *out, const float *D, c
     else
         hipLaunchKernelGGL(k_det_inhom_sweep, dim3((W + 255) / 256, H), dim3(256), 0, s, in, out, D, r, W, H, axis,
                            dt_h2, dt_2h);
+    return hipGetLastError();
+}
+
+hipError_t launch_det_aqss(float *state, int A, const RxTable &T, float dt, int fixed, hipStream_t s) {
+    const dim3 g((A + 255) / 256), b(256);
+    switch (T.S) {
+        case 1: hipLaunchKernelGGL(k_det_aqss<1>, g, b, 0, s, state, A, T, dt, fixed); break;
+        case 2: hipLaunchKernelGGL(k_det_aqss<2>, g, b, 0, s, state, A, T, dt, fixed); break;
+        case 3: hipLaunchKernelGGL(k_det_aqss<3>, g, b, 0, s, state, A, T, dt, fixed); break;
+        case 4: hipLaunchKernelGGL(k_det_aqss<4>, g, b, 0, s, state, A, T, dt, fixed); break;
+        case 5: hipLaunchKernelGGL(k_det_aqss<5>, g, b, 0, s, state, A, T, dt, fixed); break;
+        case 6: hipLaunchKernelGGL(k_det_aqss<6>, g, b, 0, s, state, A, T, dt, fixed); break;
+        case 7: hipLaunchKernelGGL(k_det_aqss<7>, g, b, 0, s, state, A, T, dt, fixed); break;
+        case 8: hipLaunchKernelGGL(k_det_aqss<8>, g, b, 0, s, state, A, T, dt, fixed); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

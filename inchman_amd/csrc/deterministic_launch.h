@@ -17,6 +17,8 @@ hipError_t launch_det_hom_diffuse(const float *in, float *out, int W, int H, flo
 hipError_t launch_det_inhom_sweep(const float *in, float *out, const float *D, const float *r, int W, int H,
                                   int axis, float dt_h2, float dt_2h, hipStream_t s);
 hipError_t launch_det_rk4(float *state, int A, const RxTable &T, float dt, hipStream_t s);
+// alpha-QSS reactions over dt per cell; fixed = 1: the predictor's Pade factor in the corrector
+hipError_t launch_det_aqss(float *state, int A, const RxTable &T, float dt, int fixed, hipStream_t s);
 // ns = 1 or 2 steps per launch (2 needs W, H >= 3); umid/vmid (may be NULL) receive
 // the intermediate state of a two-step launch
 hipError_t launch_rdd_step(const float *u, const float *v, const float *D, float *umid, float *vmid, float *un,

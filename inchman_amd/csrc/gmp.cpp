@@ -58,7 +58,7 @@ Model compile_model(const gpgmp_model_desc &d) {
     GPGMP_REQUIRE((int64_t)d.width * d.height <= (int64_t)1 << 28, GPGMP_E_CAPACITY, "grid too large");
     GPGMP_REQUIRE(d.physical_length > 0.f && std::isfinite(d.physical_length), GPGMP_E_INVALID,
                   "physical_length must be positive");
-    GPGMP_REQUIRE(d.solver_type >= 0 && d.solver_type <= 3, GPGMP_E_INVALID, "unknown solver type");
+    GPGMP_REQUIRE(d.solver_type >= 0 && d.solver_type <= 5, GPGMP_E_INVALID, "unknown solver type");
     for (int b = 0; b < 4; ++b)
         GPGMP_REQUIRE(d.boundary_mask[b] == 0 || d.boundary_mask[b] == 1, GPGMP_E_INVALID,
                       "boundary masks must be 0 or 1");

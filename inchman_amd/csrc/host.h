@@ -79,11 +79,12 @@ struct Model {
     float cell_height = 0.f;  // PhysicalCellHeight
     bool inhomogeneous() const {
         return desc.solver_type == GPGMP_STOCHASTIC_INHOMOGENEOUS ||
-               desc.solver_type == GPGMP_DETERMINISTIC_INHOMOGENEOUS_RK4;
+               desc.solver_type == GPGMP_DETERMINISTIC_INHOMOGENEOUS_RK4 ||
+               desc.solver_type == GPGMP_DETERMINISTIC_INHOMOGENEOUS_AQSS;
     }
     bool deterministic() const {
-        return desc.solver_type == GPGMP_DETERMINISTIC_HOMOGENEOUS_RK4 ||
-               desc.solver_type == GPGMP_DETERMINISTIC_INHOMOGENEOUS_RK4;
+        return desc.solver_type >= GPGMP_DETERMINISTIC_HOMOGENEOUS_RK4 &&
+               desc.solver_type <= GPGMP_DETERMINISTIC_INHOMOGENEOUS_AQSS;
     }
 };
 

@@ -90,6 +90,16 @@ int gpgmp_det_rk4_device(const gpgmp_model *m, float *state, float dt, void *str
     });
 }
 
+int gpgmp_det_aqss_device(const gpgmp_model *m, float *state, float dt, int32_t flags, void *stream) {
+    return guarded_det([&] {
+        GPGMP_REQUIRE(m && state, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE((flags & ~GPGMP_AQSS_PADE_FIX) == 0, GPGMP_E_INVALID, "unknown flags");
+        const int A = m->m.desc.width * m->m.desc.height;
+        if (m->m.table.R == 0) return;
+        GPGMP_HIP_CHECK(launch_det_aqss(state, A, m->m.table, dt, flags & GPGMP_AQSS_PADE_FIX, (hipStream_t)stream));
+    });
+}
+
 int gpgmp_rdd_create(const gpgmp_rdd_params *p, int32_t steps, int32_t ckpt_every, void *stream, gpgmp_rdd **out) {
     return guarded_det([&] {
         GPGMP_REQUIRE(p && out, GPGMP_E_INVALID, "null argument");

@@ -46,6 +46,8 @@ extern "C" {
 #define GPGMP_STOCHASTIC_INHOMOGENEOUS 1
 #define GPGMP_DETERMINISTIC_HOMOGENEOUS_RK4 2
 #define GPGMP_DETERMINISTIC_INHOMOGENEOUS_RK4 3
+#define GPGMP_DETERMINISTIC_HOMOGENEOUS_AQSS 4
+#define GPGMP_DETERMINISTIC_INHOMOGENEOUS_AQSS 5
 
 /* One reaction: stoichiometry per species and the rate constant
  * (Reaction(id, Real rate, ...) Reaction.cpp:39-56; law generated at :119-138). */
@@ -225,6 +227,23 @@ int gpgmp_det_inhom_sweep_device(const float *in, float *out, int32_t W, int32_t
                                  int32_t axis, float dt, const float *D_axis, const float *r_axis,
                                  float h, void *stream);
 int gpgmp_det_rk4_device(const gpgmp_model *m, float *state, float dt, void *stream);
+/* deterministic_performReaction_aqss (Deterministic.cl:291-522, called by Solver::doReactions
+ * for the alpha-QSS solver types, Solver.cpp:257-263): Mott & Oran's alpha-QSS
+ * predictor-corrector over dt per cell on the fp32 state [S][H][W] in place, production and
+ * destruction of deterministic_rhs_ode_aqss (Deterministic.cl:241-287).  flags 0 keeps the
+ * reference's corrector factor `180.*+rtaub*(...)` (Deterministic.cl:423);
+ * GPGMP_AQSS_PADE_FIX uses the predictor's Pade factor there too.  A cell that needs more
+ * than 2^16 inner passes (a step size that underflows, e.g. a cell at zero under the
+ * reference corrector) is set to NaN. */
+#define GPGMP_AQSS_PADE_FIX 1
+int gpgmp_det_aqss_device(const gpgmp_model *m, float *state, float dt, int32_t flags, void *stream);
+/* inhomogeneous_computeMomentsX + inhomogeneous_reduceMomentsBlocks
+ * (StochasticInhomogeneousDriftDiffusion.cl:5-82, InhomogeneousSolver.cpp:288-299): for
+ * species `species` of the int32 state [S][H][W] on the device, sums[0] = sum of x * n(x, y)
+ * (x = cell column) and sums[1] = sum of n(x, y), written to the device array sums[2]
+ * (stream-ordered).  Exact 64-bit integer sums instead of the reference's float tree. */
+int gpgmp_moments_x_device(const int32_t *state, int32_t W, int32_t H, int32_t species, int64_t *sums,
+                           void *stream);
 
 /* ---- individual particles (IndividualSolver sort / cell indices) ------------------- */
 /* Stable LSD radix sort of n (key, value) pairs by the 32-bit key, device pointers;

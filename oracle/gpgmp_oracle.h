@@ -121,6 +121,12 @@ double oracle_rdd_grad(const oracle_rdd_params *p, const double *D, const double
                        const double *v0, const double *ut, const double *vt, int steps,
                        int ckpt_every, double *gD, double *gk, int nthreads);
 
+/* alpha-QSS reactions over dtg per cell (deterministic_performReaction_aqss); state [S][H][W];
+ * fixed = 1: the predictor's Pade factor in the corrector too (0: the reference's).
+ * Returns the largest inner-pass count of a cell, -1 if a cell hit the pass cap. */
+int oracle_det_aqss_f32(float *state, int W, int H, const oracle_reaction_table *t, float dtg, int fixed);
+int oracle_det_aqss_f64(double *state, int W, int H, const oracle_reaction_table *t, double dtg, int fixed);
+
 /* ---- individual particles ---- */
 void oracle_stable_sort_pairs(uint32_t *keys, uint32_t *vals, int n);
 void oracle_cell_offsets(const uint32_t *sorted_keys, int n, int ncells, uint32_t *off);

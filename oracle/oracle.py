@@ -107,6 +107,10 @@ def lib():
         L.oracle_ind_update.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_void_p, C.c_int] + [C.c_void_p] * 9
         L.oracle_ind_run.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_uint32,
                                                         C.c_void_p, C.c_void_p]
+        L.oracle_det_aqss_f32.restype = C.c_int
+        L.oracle_det_aqss_f32.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_float, C.c_int]
+        L.oracle_det_aqss_f64.restype = C.c_int
+        L.oracle_det_aqss_f64.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_int]
         L.oracle_ind_maintain.restype = C.c_int
         L.oracle_ind_maintain.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int32,
                                                              C.c_void_p, C.c_void_p]
@@ -416,3 +420,16 @@ def ind_maintain(p, keys, vals, n, props, goal, seed, rule=None):
     st = lib().oracle_ind_maintain(C.byref(p), _p(k), _p(v), n, k.size, _p(pr), _p(g), seed,
                                    C.byref(rule) if rule is not None else None, C.byref(n_out))
     return st, k, v, pr, n_out.value
+
+
+def det_aqss(state, table, dtg, dtype=np.float32, fixed=False):
+    """alpha-QSS reactions over dtg on state [S][H][W] in float (the reference's
+    promotions) or double; fixed: the predictor's Pade factor in the corrector too.
+    Returns (new state, largest inner-pass count)."""
+    st = np.ascontiguousarray(state, dtype).copy()
+    S, H, W = st.shape
+    if dtype == np.float32:
+        it = lib().oracle_det_aqss_f32(_p(st), W, H, C.byref(table), float(dtg), int(fixed))
+    else:
+        it = lib().oracle_det_aqss_f64(_p(st), W, H, C.byref(table), float(dtg), int(fixed))
+    return st, it

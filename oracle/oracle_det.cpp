@@ -112,6 +112,172 @@ extern "C" void oracle_det_rk4(double *state, int W, int H, const oracle_reactio
 }
 
 // ---------------------------------------------------------------------------------------
+// deterministic_performReaction_aqss (Deterministic.cl:291-522): the alpha-QSS integrator
+// of Mott & Oran (CHEMEQ2) per cell over dtg, with the production / destruction split
+// q_s, d_s of deterministic_rhs_ode_aqss (Deterministic.cl:241-287) as Solver.cpp:960-1010
+// generates it: q_s = 0. + |n|. * k * X * Y + ... over the reactions that produce s (net
+// stoichiometry n > 0), d_s the same over those that consume it, left to right in double
+// (the "1." multipliers make the whole expression double), stored in the real type R.
+//
+// R = float reproduces the reference's promotions statement by statement: Real variables
+// are float, every subexpression with an unsuffixed literal (180., 0.5*, 1e-16*, ...) is
+// evaluated in double and rounded on assignment; no contraction, IEEE division and sqrt
+// (OpenCL allows 2.5 / 3 ulp: parity unpinned for those roundings).  R = double is the
+// same algorithm in double, for the closed-form checks.
+//
+// Kept from the reference as it computes (fixed = 0): the corrector's Pade factor is written
+// `180.*+rtaub*(...)` (Deterministic.cl:423), i.e. 180 * rtaub * (60 + ...), not
+// 180 + rtaub * (...) as in the predictor (:388); fixed = 1 uses the predictor's form
+// (Mott & Oran's alpha) in the corrector too.  itermax = 2, so the stability factor stays
+// 0.01 and the ym1 / ym2 history (written only for it) is not kept.
+// Not in the reference: at most kAqssMaxIter passes of the inner loop per cell (a cell
+// whose step size underflows would otherwise never finish); such a cell is set to NaN.
+// The GPU build has the same cap.
+// Pinned by: the birth-death closed form A(t) = k/g + (A0 - k/g) e^{-g t}, the equilibrium
+// of a stiff reversible pair, and the fp64 RK4 oracle on the Schnakenberg network.
+namespace {
+constexpr int kAqssMaxIter = 1 << 16;
+
+template <typename R>
+void aqss_rhs(const oracle_reaction_table *t, const R *y, R *q, R *d) {
+    const int S = t->num_species, NR = t->num_reactions;
+    for (int s = 0; s < S; ++s) {
+        double qs = 0., ds = 0.;
+        for (int i = 0; i < NR; ++i) {
+            const int n = t->net[i][s];
+            if (n == 0) continue;
+            double term = (double)std::abs(n) * t->lit_dbl[i];
+            for (int f = 0; f < t->n_factors[i]; ++f) term = term * (double)y[t->factor_species[i][f]];
+            if (n > 0)
+                qs = qs + term;
+            else
+                ds = ds + term;
+        }
+        q[s] = (R)qs;
+        d[s] = (R)ds;
+    }
+}
+
+template <typename R>
+int aqss_cell(const oracle_reaction_table *t, R *state, R dtg, int fixed) {
+    const int S = t->num_species;
+    const R epsmin = (R)1e-2, sqreps = (R)0.5, epscl = (R)100., epsmax = (R)10., tfd = (R)1.000008,
+            dtmin = (R)1e-15, ymin = (R)1e-20;
+    const int itermax = 2;
+    R y[ORACLE_MAX_SPECIES], q[ORACLE_MAX_SPECIES], d[ORACLE_MAX_SPECIES], ys[ORACLE_MAX_SPECIES],
+        qs[ORACLE_MAX_SPECIES], rtaus[ORACLE_MAX_SPECIES], y1[ORACLE_MAX_SPECIES], scr[ORACLE_MAX_SPECIES];
+    R tn = (R)0.;
+    for (int i = 0; i < S; ++i) y[i] = std::fmax(state[i], ymin);
+    aqss_rhs<R>(t, y, q, d);
+    R scrtch = (R)1e-25;
+    for (int i = 0; i < S; ++i) {
+        const R ascr = std::fabs(q[i]);
+        const R scr2 = std::copysign((R)1. / y[i], (R)0.1 * epsmin * ascr - d[i]);
+        const R scr1 = scr2 * d[i];
+        scrtch = std::fmax(std::fmax(scr1, -std::fabs(ascr - d[i]) * scr2), scrtch);
+    }
+    R dt = std::fmin(sqreps / scrtch, dtg);
+    R ts, dtcurrent, eps;
+    int iters = 0;
+    while (dtg > tn * tfd) {
+        ts = tn;
+        dtcurrent = dt;
+        for (int i = 0; i < S; ++i) {
+            ys[i] = y[i];
+            qs[i] = q[i];
+            rtaus[i] = dtcurrent * d[i] / y[i];
+        }
+        while (dtg > tn * tfd) {
+            if (++iters > kAqssMaxIter) {
+                for (int i = 0; i < S; ++i) state[i] = (R)NAN;
+                return -1;
+            }
+            for (int i = 0; i < S; ++i) {  // predictor
+                const R rtaui = dtcurrent * d[i] / y[i];
+                const R alpha = (R)((180. + rtaui * (60. + rtaui * (11. + rtaui))) /
+                                    (360. + rtaui * (60. + rtaui * (12. + rtaui))));
+                scr[i] = (R)((q[i] - d[i]) / (1. + alpha * rtaui));
+            }
+            for (int iteration = 0; iteration < itermax; iteration++) {
+                for (int i = 0; i < S; ++i) y[i] = std::fmax(ys[i] + dt * scr[i], ymin);
+                if (iteration == 0) {
+                    tn = ts + dt;
+                    for (int i = 0; i < S; ++i) y1[i] = y[i];
+                }
+                aqss_rhs<R>(t, y, q, d);
+                eps = (R)1e-10;
+                for (int i = 0; i < S; ++i) {  // corrector
+                    const R rtaub = (R)(0.5 * (rtaus[i] + dt * d[i] / y[i]));
+                    const R alpha = fixed ? (R)((180. + rtaub * (60. + rtaub * (11. + rtaub))) /
+                                                (360. + rtaub * (60. + rtaub * (12. + rtaub))))
+                                          : (R)((180. * +rtaub * (60. + rtaub * (11. + rtaub))) /
+                                                (360. + rtaub * (60. + rtaub * (12. + rtaub))));
+                    const R qt = (R)(qs[i] * (1. - alpha) + q[i] * alpha);
+                    const R pb = rtaub / dt;
+                    scr[i] = (R)((qt - ys[i] * pb) / (1. + alpha * rtaub));
+                }
+            }
+            for (int i = 0; i < S; ++i) {  // convergence
+                const R scr2 = std::fmax(ys[i] + dt * scr[i], (R)0.);
+                R scr1 = std::fabs(scr2 - y1[i]);
+                y[i] = std::fmax(scr2, ymin);
+                if (0.25 * (ys[i] + y[i]) > ymin) {
+                    scr1 = scr1 / y[i];
+                    eps = std::fmax((R)0.5 * (scr1 + std::fmin(std::fabs(q[i] - d[i]) / (q[i] + d[i] + (R)1e-30), scr1)),
+                                    eps);
+                }
+            }
+            eps = eps * epscl;
+            if (dt <= dtmin + 1e-16 * tn) {
+                dt = dtg - ts;
+                dt = std::fmin(dtmin, std::fabs(dt));
+            }
+            const R stab = (R)0.01;
+            if (eps <= epsmax && stab <= 1.) {
+                if (dtg <= tn * tfd) break;
+            } else {
+                tn = ts;
+            }
+            const R rteps = std::sqrt(eps);
+            R dto = dt;
+            dt = std::fmin(std::fmin(dt * ((R)1.0 / rteps + (R).005), tfd * (dtg - tn)), dto / (stab + (R)0.001));
+            if (eps <= epsmax && stab <= 1) break;
+            dto = dt / dto;
+            for (int i = 0; i < S; ++i) rtaus[i] *= dto;
+        }
+        aqss_rhs<R>(t, y, q, d);
+    }
+    for (int i = 0; i < S; ++i) state[i] = y[i];
+    return iters;
+}
+
+template <typename R>
+int aqss_grid(R *state, int W, int H, const oracle_reaction_table *t, R dtg, int fixed) {
+    const size_t A = (size_t)W * H;
+    const int S = t->num_species;
+    int worst = 0;
+    for (size_t c = 0; c < A; ++c) {
+        R x[ORACLE_MAX_SPECIES];
+        for (int s = 0; s < S; ++s) x[s] = state[c + s * A];
+        const int it = aqss_cell<R>(t, x, dtg, fixed);
+        worst = (it < 0 || worst < 0) ? -1 : std::max(worst, it);
+        for (int s = 0; s < S; ++s) state[c + s * A] = x[s];
+    }
+    return worst;
+}
+}  // namespace
+
+// Returns the largest number of inner passes of any cell, or -1 if a cell hit the cap.
+extern "C" int oracle_det_aqss_f32(float *state, int W, int H, const oracle_reaction_table *t, float dtg,
+                                   int fixed) {
+    return aqss_grid<float>(state, W, H, t, dtg, fixed);
+}
+extern "C" int oracle_det_aqss_f64(double *state, int W, int H, const oracle_reaction_table *t, double dtg,
+                                   int fixed) {
+    return aqss_grid<double>(state, W, H, t, dtg, fixed);
+}
+
+// ---------------------------------------------------------------------------------------
 // Fused Schnakenberg reaction-drift-diffusion step (no reference counterpart: the
 // reference computes no reactions in deterministic-inhomogeneous mode,
 // Solver.cpp:107-171,250-266).  Diffusion follows the FPE stencil of

@@ -223,3 +223,22 @@ def test_event_budget_hot_cells(gpu, oracle, budget, below, lanes):
     sh.run(60, 8)
     want2, _ = oracle.run(mh, st2, 60, 8)
     assert np.array_equal(sh.synchronizeStateBufferFromDevice(), want2)
+
+
+def test_moments_x(gpu):
+    """inhomogeneous_computeMomentsX + reduceMomentsBlocks: sum of x * n and of n for one
+    species, exact, against numpy on odd and even grids."""
+    import torch
+
+    from inchman_amd._lib import call
+
+    rng = np.random.default_rng(17)
+    for S, H, W, sp in ((2, 256, 256, 1), (3, 37, 50, 0), (1, 1, 1, 0), (2, 300, 1024, 1)):
+        st = rng.integers(0, 5000, (S, H, W)).astype(np.int32)
+        x = torch.from_numpy(st).cuda()
+        sums = torch.zeros(2, dtype=torch.int64, device="cuda")
+        call("gpgmp_moments_x_device", x.data_ptr(), W, H, sp, sums.data_ptr(), None)
+        torch.cuda.synchronize()
+        n = st[sp].astype(np.int64)
+        want = [int((n * np.arange(W)[None, :]).sum()), int(n.sum())]
+        assert sums.cpu().tolist() == want

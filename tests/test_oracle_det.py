@@ -112,3 +112,69 @@ def test_rdd_gradient_finite_differences(oracle, ckpt):
         fd = (loss_of(D + e * d, k) - loss_of(D - e * d, k)) / (2 * e)
         an = float((gD * d).sum())
         assert abs(fd - an) <= 1e-7 * abs(fd), (fd, an)
+
+
+# ---- alpha-QSS (deterministic_performReaction_aqss) ------------------------------------
+def _aqss(oracle, t, st, dt, dtype, fixed):
+    return oracle.det_aqss(st, t, dt, dtype, fixed)
+
+
+def test_aqss_birth_death(oracle):
+    """0 -> A (k), A -> 0 (g): A(t) = k/g + (A0 - k/g) e^{-g t}.  With the predictor's Pade
+    factor in the corrector (Mott & Oran's alpha) the integrator is accurate to its error
+    control; the reference's corrector factor (180.*+rtaub*..., Deterministic.cl:423) is
+    kept by default and lands about 11.6 % off at dt = 1.5, in both precisions."""
+    t = oracle.compile_reactions(1, [2.0, 0.5], [[0], [1]], [[1], [0]])
+    st = np.full((1, 4, 4), 10.0)
+    for dt, tol in ((0.1, 1e-7), (1.5, 2e-6), (5.0, 1e-4)):
+        want = 4.0 + (10.0 - 4.0) * np.exp(-0.5 * dt)
+        for dtype in (np.float64, np.float32):
+            out, it = _aqss(oracle, t, st, dt, dtype, True)
+            assert it > 0
+            np.testing.assert_allclose(out, want, rtol=tol)
+    want = 4.0 + (10.0 - 4.0) * np.exp(-0.5 * 1.5)
+    for dtype in (np.float64, np.float32):
+        out, _ = _aqss(oracle, t, st, 1.5, dtype, False)
+        assert abs(out[0, 0, 0] / want - 1.0 - 0.11573) < 2e-4
+
+
+def test_aqss_constant_production(oracle):
+    """0 -> A only: q constant, d = 0, so both corrector forms integrate it exactly."""
+    t = oracle.compile_reactions(1, [2.5], [[0]], [[1]])
+    st = np.full((1, 2, 2), 3.0)
+    for fixed in (True, False):
+        out, _ = _aqss(oracle, t, st, 4.0, np.float64, fixed)
+        np.testing.assert_allclose(out, 3.0 + 2.5 * 4.0, rtol=1e-12)
+
+
+def test_aqss_stiff_equilibrium(oracle):
+    """A <-> B with kf = 1000, kb = 10 relaxes to A = 100 kb / (kf + kb)."""
+    t = oracle.compile_reactions(2, [1000.0, 10.0], [[1, 0], [0, 1]], [[0, 1], [1, 0]])
+    st = np.zeros((2, 2, 2))
+    st[0] = 100.0
+    out, it = _aqss(oracle, t, st, 1.0, np.float64, True)
+    np.testing.assert_allclose(out[0], 100 * 10 / 1010, rtol=1e-2)
+    np.testing.assert_allclose(out[1], 100 * 1000 / 1010, rtol=1e-2)
+
+
+def test_aqss_schnakenberg_against_rk4(oracle):
+    """The Schnakenberg network (concentration form) against the fp64 RK4 oracle with 200
+    calls of 20 substeps; the float instance follows the double one step for step."""
+    from oracle.oracle import lib, _p
+    import ctypes as C
+
+    t = oracle.compile_reactions(2, [1.0, 3.0, 1e-4, 1.0], [[0, 0], [0, 0], [2, 1], [1, 0]],
+                                 [[1, 0], [0, 1], [3, 0], [0, 0]])
+    rng = np.random.default_rng(0)
+    st = np.stack([rng.uniform(50, 250, (8, 8)), rng.uniform(20, 120, (8, 8))])
+    for dt, tol in ((0.01, 2e-5), (0.1, 5e-3), (1.0, 2e-2)):
+        ref = st.copy()
+        for _ in range(200):
+            lib().oracle_det_rk4(_p(ref), 8, 8, C.byref(t), C.c_double(dt / 200))
+        o64, it64 = _aqss(oracle, t, st, dt, np.float64, True)
+        o32, it32 = _aqss(oracle, t, st, dt, np.float32, True)
+        assert it64 == it32
+        assert np.max(np.abs(o64 - ref) / ref) <= tol
+        assert np.max(np.abs(o32 - o64) / o64) <= 1e-5
+        r64, _ = _aqss(oracle, t, st, dt, np.float64, False)
+        assert np.max(np.abs(r64 - ref) / ref) > np.max(np.abs(o64 - ref) / ref)
