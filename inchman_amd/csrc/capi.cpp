@@ -52,6 +52,19 @@ struct gpgmp_solver {
     DevBuf<int32_t> seed_d;
     std::unique_ptr<GmpRunner> runner;
     uint32_t runner_seed0 = 0;
+    // output events of the host loop: interval, max_time (infinity = off), and up to
+    // max_dumps state snapshots kept on the device with their times
+    float out_interval = __builtin_inff(), max_time = __builtin_inff();
+    int32_t max_dumps = 0;
+    DevBuf<int32_t> dumps;
+    std::vector<float> dump_times;
+    void record_dump(float t) {
+        const size_t n = eng->state_elems();
+        if ((int32_t)dump_times.size() < max_dumps)
+            GPGMP_HIP_CHECK(hipMemcpyAsync(dumps.get() + dump_times.size() * n, eng->state(), n * sizeof(int32_t),
+                                           hipMemcpyDeviceToDevice, eng->stream()));
+        dump_times.push_back(t);
+    }
 };
 
 struct gpgmp_batch {
@@ -281,6 +294,13 @@ int gpgmp_solver_run(gpgmp_solver *s, int32_t steps, uint32_t seed0, float *sim_
         s->runner.reset();
         s->runner.reset(new GmpRunner(*s->eng, s->dtx, s->dty, {seed0}));
         s->runner_seed0 = seed0;
+        if (std::isfinite(s->out_interval) || std::isfinite(s->max_time)) {
+            // DiffusionModel::oclSolve writes the initial state before its loop
+            // (DiffusionModel.cpp:1442), then at output events
+            s->dump_times.clear();
+            s->record_dump(0.f);
+            s->runner->set_output(s->out_interval, s->max_time, [s](float t) { s->record_dump(t); });
+        }
         s->runner->run(steps, 0);
         if (sim_time_out) *sim_time_out = s->runner->sim_time();
     });
@@ -292,6 +312,36 @@ int gpgmp_solver_run_continue(gpgmp_solver *s, int32_t steps, int32_t graph_chun
         GPGMP_REQUIRE(s->runner != nullptr, GPGMP_E_STATE, "no run to continue");
         s->runner->run(steps, graph_chunk);
         if (sim_time_out) *sim_time_out = s->runner->sim_time();
+    });
+}
+
+int gpgmp_solver_set_output(gpgmp_solver *s, float interval, float max_time, int32_t max_dumps) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && max_dumps >= 0, GPGMP_E_INVALID, "bad argument");
+        GPGMP_REQUIRE(!std::isnan(interval) && !std::isnan(max_time), GPGMP_E_INVALID, "NaN interval or max_time");
+        s->out_interval = interval > 0.f ? interval : __builtin_inff();
+        s->max_time = max_time > 0.f ? max_time : __builtin_inff();
+        s->max_dumps = max_dumps;
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s->eng->stream()));
+        s->dumps.alloc((size_t)max_dumps * s->eng->state_elems());
+        s->dump_times.clear();
+    });
+}
+
+int gpgmp_solver_get_output(gpgmp_solver *s, int32_t *counts, float *times, int32_t *n_stored, int32_t *n_total,
+                            int64_t *iterations) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        const int32_t total = (int32_t)s->dump_times.size();
+        const int32_t stored = total < s->max_dumps ? total : s->max_dumps;
+        if (counts && stored)
+            GPGMP_HIP_CHECK(hipMemcpyAsync(counts, s->dumps.get(), (size_t)stored * s->eng->state_elems() * 4,
+                                           hipMemcpyDeviceToHost, s->eng->stream()));
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s->eng->stream()));
+        if (times) std::copy(s->dump_times.begin(), s->dump_times.begin() + stored, times);
+        if (n_stored) *n_stored = stored;
+        if (n_total) *n_total = total;
+        if (iterations) *iterations = s->runner ? s->runner->iterations() : 0;
     });
 }
 

@@ -161,8 +161,24 @@ StepDecision Schedule::next() {
             mindt = next[i];
             minspec = i;
         }
+    // an output time before the next diffusion makes this an output event: reactions up
+    // to it, no diffusion; the state is written once the clock has reached it, which is
+    // the iteration after (DiffusionModel.cpp:1519-1527, 1553-1572)
+    const float mindt_out = out_last_ + out_interval_;
+    const bool output_event = mindt_out < mindt;
+    if (output_event) mindt = mindt_out;
     d.deltat = mindt - sim_time_;
     int32_t off = rx_ ? 1 : 0;  // Solver::doReactions increments the seed
+    if (output_event) {
+        if (sim_time_ >= out_last_ + out_interval_) {
+            d.dump = true;
+            d.dump_time = mindt;
+            out_last_ += out_interval_;
+        }
+        d.off_end = off;
+        sim_time_ = mindt;
+        return d;
+    }
     for (int i = 0; i < S_; ++i) {
         if (!(i == minspec || next[i] <= mindt)) continue;
         if (!(next[i] > 0)) continue;
@@ -578,13 +594,18 @@ void GmpRunner::run(int steps, int graph_chunk) {
     }
     parity_ = 0;  // nothing is pending at entry
     int done = 0, ci = 0;
-    while (done < steps) {
-        const int n = std::min(cap, steps - done);
+    while (done < steps && !sched_.finished()) {
+        const int nmax = std::min(cap, steps - done);
+        int n = 0;
+        bool dump = false;
+        float dump_time = 0.f;
         Chunk &c = *chunks_[ci];
         GPGMP_HIP_CHECK(hipEventSynchronize(c.copied));  // host buffers free again
         for (int r = 0; r < nreal; ++r) c.hseed[r] = prev_seed_[r];
-        for (int j = 0; j < n; ++j) {
+        for (int j = 0; j < nmax; ++j) {
+            if (j > 0 && sched_.finished()) break;  // the loop's simTime < maxTime test
             const StepDecision d = sched_.next();
+            n = j + 1;
             for (int r = 0; r < nreal; ++r) c.hseed[(size_t)(j + 1) * nreal + r] = seeds_[r].next();
             const float dts = (eng_.table_.R > 0) ? d.deltat : 0.f;
             if (!eng_.inh_) {
@@ -622,12 +643,19 @@ void GmpRunner::run(int steps, int graph_chunk) {
             }
             prev_ = d;
             has_prev_ = true;
+            if (d.dump && dump_) {  // end the chunk here: the copy follows its launches
+                dump = true;
+                dump_time = d.dump_time;
+                break;
+            }
         }
         for (int r = 0; r < nreal; ++r) prev_seed_[r] = c.hseed[(size_t)n * nreal + r];
         issue(c, n, use_graph && n == cap);
         if (!eng_.inh_) parity_ = (parity_ + n) & 1;
         done += n;
+        iterations_ += n;
         ci ^= 1;
+        if (dump) dump_(dump_time);
     }
     // final gather of the last launch's outflow (and source boundaries)
     if (has_prev_) {

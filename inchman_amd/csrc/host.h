@@ -7,6 +7,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/gpgmp_hip.h"
@@ -120,6 +121,8 @@ struct StepDecision {
     int32_t off_x[kMaxSpecies] = {};
     int32_t off_y[kMaxSpecies] = {};
     int32_t off_end = 0;            // seed offset after the iteration
+    bool dump = false;              // an output event that writes the state (at dump_time)
+    float dump_time = 0.f;
 };
 
 // The multi-rate schedule: which species diffuse when.  It depends only on the time
@@ -131,6 +134,13 @@ class Schedule {
     StepDecision next();
     float sim_time() const { return sim_time_; }
     std::vector<float> next_diffusion_time() const;
+    // Output events every `interval` of simulation time (DiffusionModel.cpp:1519-1572) and
+    // the loop's stop at max_time (:1479-1480); infinity disables either.
+    void set_output(float interval, float max_time) {
+        out_interval_ = interval;
+        max_time_ = max_time;
+    }
+    bool finished() const { return !(sim_time_ < max_time_); }
 
   private:
     bool inh_;
@@ -138,6 +148,7 @@ class Schedule {
     bool rx_;
     std::vector<float> dtx_, dty_, nx_, ny_;
     float sim_time_ = 0.f;
+    float out_interval_ = __builtin_inff(), out_last_ = 0.f, max_time_ = __builtin_inff();
 };
 
 // Device engine of the stochastic solvers: a set of realizations of one network.
@@ -229,6 +240,14 @@ class GmpRunner {
     ~GmpRunner();
     void run(int steps, int graph_chunk);
     float sim_time() const { return sched_.sim_time(); }
+    // Output events and the max_time stop of the reference loop; `dump` is called, after
+    // the launches of each output iteration that writes (the state is then fully gathered),
+    // with the simulation time of the dump, to enqueue a copy of the state on the stream.
+    void set_output(float interval, float max_time, std::function<void(float)> dump) {
+        sched_.set_output(interval, max_time);
+        dump_ = std::move(dump);
+    }
+    int64_t iterations() const { return iterations_; }
     // Per-launch HIP-event timing of the eager (non-graph) launches, by kernel mode
     // (0 homogeneous, 1 SSA (+ X sweeps unless split), 2 Y sweeps, 3 hot cells in their own
     // launch, 4 X sweeps + hot cells of a split step); accumulated over run() calls.
@@ -261,6 +280,8 @@ class GmpRunner {
     std::vector<int32_t> prev_seed_;
     int parity_ = 0;
     std::vector<Chunk *> chunks_;
+    std::function<void(float)> dump_;
+    int64_t iterations_ = 0;
     void timed_launch(int mode, const LaunchPlan *plan, const LaunchPlan &hplan, const int32_t *scur,
                       const int32_t *sprev, const int32_t *lin, int32_t *lout, int how);
     void collect_times();

@@ -251,6 +251,23 @@ class Solver:
         call("gpgmp_solver_run_continue", self.handle, int(steps), int(graph_chunk), C.byref(t))
         return t.value
 
+    def setOutput(self, interval, max_time=0.0, max_dumps=64):
+        """Output events every `interval` of simulation time (DiffusionModel::outputInterval)
+        and the stop at max_time (DiffusionModel::maxTime); <= 0 disables either.  Runs then
+        keep up to max_dumps device snapshots (t = 0 and each output write)."""
+        call("gpgmp_solver_set_output", self.handle, C.c_float(interval), C.c_float(max_time), int(max_dumps))
+        self._max_dumps = int(max_dumps)
+
+    def getOutput(self):
+        """(counts [n][S][H][W], times [n], writes including dropped ones, iterations run)."""
+        ns, nt, it = C.c_int32(), C.c_int32(), C.c_int64()
+        call("gpgmp_solver_get_output", self.handle, None, None, C.byref(ns), C.byref(nt), C.byref(it))
+        counts = np.zeros((max(ns.value, 1),) + self.shape, np.int32)
+        times = np.zeros(max(ns.value, 1), np.float32)
+        call("gpgmp_solver_get_output", self.handle, _iptr(counts), _fptr(times), C.byref(ns), C.byref(nt),
+             C.byref(it))
+        return counts[: ns.value], times[: ns.value], nt.value, it.value
+
     def setLaunch(self, block, cells_per_thread):
         call("gpgmp_solver_set_launch", self.handle, int(block), int(cells_per_thread))
 

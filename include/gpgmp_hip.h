@@ -118,9 +118,21 @@ int gpgmp_solver_next_diffusion_time(gpgmp_solver *s, float *out);
 /* Per-species time steps: homogeneous dt[s] (both arrays), inhomogeneous dtX/dtY. */
 int gpgmp_solver_time_steps(gpgmp_solver *s, float *dt_x, float *dt_y);
 /* DiffusionModel::oclSolve main loop (DiffusionModel.cpp:1467-1603) for `steps`
- * iterations from seed0 (boost::mt19937(seed0), uniform_int<>(0,INT_MAX)), with no
- * output/alert events, as fused launches without host synchronisation. */
+ * iterations from seed0 (boost::mt19937(seed0), uniform_int<>(0,INT_MAX)), with the
+ * output events and max_time stop set by gpgmp_solver_set_output (none by default; no
+ * alert events), as fused launches without host synchronisation. */
 int gpgmp_solver_run(gpgmp_solver *s, int32_t steps, uint32_t seed0, float *sim_time_out);
+/* Output of the host loop (DiffusionModel.cpp:1442, 1519-1572, 1479-1480): with interval > 0
+ * every run takes the reference's output iterations (a reaction step up to the output time
+ * and, once the clock has reached it, a zero-length one that writes; both consume a seed)
+ * and keeps a device snapshot of the state at t = 0 and at each write, up to max_dumps
+ * of them (writeAllSpecies, DiffusionModel.cpp:1096-1296, without the file); with
+ * max_time > 0 the loop stops once the simulation time reaches it.  <= 0 disables either. */
+int gpgmp_solver_set_output(gpgmp_solver *s, float interval, float max_time, int32_t max_dumps);
+/* The snapshots of the last run: counts[n_stored][S][H][W] and times[n_stored] (either may
+ * be NULL), n_total = writes including those beyond max_dumps, and the iterations run. */
+int gpgmp_solver_get_output(gpgmp_solver *s, int32_t *counts, float *times, int32_t *n_stored, int32_t *n_total,
+                            int64_t *iterations);
 /* Continues the loop of the last gpgmp_solver_run for `steps` more iterations;
  * graph_chunk > 0 replays chunks of that many iterations as a HIP graph. */
 int gpgmp_solver_run_continue(gpgmp_solver *s, int32_t steps, int32_t graph_chunk, float *sim_time_out);

@@ -98,6 +98,12 @@ typedef struct {
 
 int oracle_run(const oracle_model *m, const oracle_reaction_table *tab, int32_t *state,
                int steps, uint32_t seed0, int nthreads, float *sim_time_out);
+/* the same loop with output events every `interval` and the stop at max_time (infinity:
+ * off); snapshots of the state [S][A] at t = 0 and each write into dumps[max_dumps];
+ * returns the number of writes */
+int oracle_run_output(const oracle_model *m, const oracle_reaction_table *t, int32_t *state, int steps,
+                      uint32_t seed0, int nthreads, float interval, float max_time, int32_t *dumps,
+                      float *dump_times, int max_dumps, int64_t *iterations, float *sim_time_out);
 
 /* ---- deterministic (FP64 Jacobi) ---- */
 void oracle_det_hom_diffuse(double *state, int W, int H, int S, const double *D, double dt,

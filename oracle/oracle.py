@@ -111,6 +111,9 @@ def lib():
         L.oracle_det_aqss_f32.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_float, C.c_int]
         L.oracle_det_aqss_f64.restype = C.c_int
         L.oracle_det_aqss_f64.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_int]
+        L.oracle_run_output.restype = C.c_int
+        L.oracle_run_output.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_uint32, C.c_int, C.c_float, C.c_float,
+                                                           C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.oracle_ind_maintain.restype = C.c_int
         L.oracle_ind_maintain.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int32,
                                                              C.c_void_p, C.c_void_p]
@@ -220,6 +223,44 @@ def inhomogeneous_dt(D, r, h):
     D = np.ascontiguousarray(D, np.float32)
     r = np.ascontiguousarray(r, np.float32)
     return lib().oracle_inhomogeneous_dt(_p(D), _p(r), D.size, float(h))
+
+
+def _model_struct(model):
+    S = model.numSpecies()
+    m = Model()
+    m.W, m.H, m.S = model.width, model.height, S
+    m.physical_length, m.p0 = model.physical_length, model.p0
+    m.inhomogeneous = 1 if model.solver == 1 else 0
+    diff = np.array([sp.diffusionConstant for sp in model.species], np.float32)
+    keep = [diff]
+    m.diffusion = diff.ctypes.data_as(C.POINTER(C.c_float))
+    if model.fields is not None:
+        for name, arr in zip(("Dx", "Dy", "rx", "ry"), model.fields):
+            a = np.ascontiguousarray(arr, np.float32)
+            keep.append(a)
+            setattr(m, name, a.ctypes.data_as(C.POINTER(C.c_float)))
+    for i in range(4):
+        m.boundary_mask[i] = model.boundary_mask[i]
+        m.source_mask[i] = model.source_mask[i]
+    m.max_time = model.max_time
+    return m, keep
+
+
+def run_output(model, state, steps, seed0, interval, max_time=float("inf"), max_dumps=64, nthreads=0):
+    """The host loop with output events every `interval` and the stop at max_time.
+    Returns (final state, sim time, iterations, dumps [n][S][H][W], dump times, writes)."""
+    st = np.ascontiguousarray(state, np.int32).copy()
+    m, keep = _model_struct(model)  # noqa: F841 (keeps the field arrays alive)
+    t = table_from_model(model)
+    dumps = np.zeros((max_dumps,) + st.shape, np.int32)
+    times = np.zeros(max_dumps, np.float32)
+    sim = C.c_float()
+    it = C.c_int64()
+    w = lib().oracle_run_output(C.byref(m), C.byref(t), _p(st), int(steps), C.c_uint32(seed0), nthreads,
+                                C.c_float(interval), C.c_float(max_time), _p(dumps), _p(times), int(max_dumps),
+                                C.byref(it), C.byref(sim))
+    k = min(w, max_dumps)
+    return st, sim.value, it.value, dumps[:k], times[:k], w
 
 
 def run(model, state, steps, seed0, nthreads=0):

@@ -416,8 +416,16 @@ extern "C" void oracle_inhom_sweep(int32_t *state, int W, int H, int species, in
 // and InhomogeneousSolver::diffuse / getNextDiffusionTime
 // (InhomogeneousSolver.cpp:238-427).  No output or alert events.
 // Pinned by: the per-kernel checks above plus the statistical end-to-end tests.
-extern "C" int oracle_run(const oracle_model *m, const oracle_reaction_table *t, int32_t *state,
-                          int steps, uint32_t seed0, int nthreads, float *sim_time_out) {
+// DiffusionModel::oclSolve's loop (DiffusionModel.cpp:1479-1603) with its output events
+// (:1519-1572): an output time before the next diffusion makes the iteration a reaction
+// step up to it without diffusion, and the state is written once the clock has reached it
+// (the next iteration, of length zero); the loop stops once simTime reaches max_time.  The
+// initial state is written first (:1442).  interval / max_time = infinity disables them.
+// dumps: [max_dumps][S][A]; returns the number of writes (stored or not).
+extern "C" int oracle_run_output(const oracle_model *m, const oracle_reaction_table *t, int32_t *state,
+                                 int steps, uint32_t seed0, int nthreads, float interval, float max_time,
+                                 int32_t *dumps, float *dump_times, int max_dumps, int64_t *iterations,
+                                 float *sim_time_out) {
     const int S = m->S, A = m->W * m->H;
     const float h = oracle_header_float(m->physical_length / (float)m->W);
     const float hy = oracle_header_float(m->physical_length / (float)m->H);
@@ -449,8 +457,18 @@ extern "C" int oracle_run(const oracle_model *m, const oracle_reaction_table *t,
     next_times(next);
     std::vector<int32_t> seeds(steps > 0 ? steps : 1);
     oracle_mt19937_seeds(seed0, steps, seeds.data());
-    float simTime = 0.f;
-    for (int n = 0; n < steps; ++n) {
+    float simTime = 0.f, outputLast = 0.f;
+    int writes = 0;
+    auto write = [&](float time) {
+        if (writes < max_dumps) {
+            std::copy(state, state + (size_t)S * A, dumps + (size_t)writes * S * A);
+            dump_times[writes] = time;
+        }
+        ++writes;
+    };
+    if (std::isfinite(interval) || std::isfinite(max_time)) write(0.f);
+    int n = 0;
+    for (; n < steps && simTime < max_time; ++n) {
         int32_t seed = seeds[n];
         float mindt = next[0];
         int minspec = 0;
@@ -459,10 +477,21 @@ extern "C" int oracle_run(const oracle_model *m, const oracle_reaction_table *t,
                 mindt = next[i];
                 minspec = i;
             }
+        const float mindtOutput = outputLast + interval;
+        const bool outputEvent = mindtOutput < mindt;
+        if (outputEvent) mindt = mindtOutput;
         const float deltat = mindt - simTime;
         if (t->num_reactions > 0) {
             oracle_ssa(state, m->W, m->H, t, seed, deltat, nthreads);
             seed = (int32_t)((uint32_t)seed + 1u);
+        }
+        if (outputEvent) {
+            if (simTime >= outputLast + interval) {
+                write(mindt);
+                outputLast += interval;
+            }
+            simTime = mindt;
+            continue;
         }
         for (int i = 0; i < S; ++i) {
             if (!(i == minspec || next[i] <= mindt)) continue;
@@ -489,6 +518,14 @@ extern "C" int oracle_run(const oracle_model *m, const oracle_reaction_table *t,
         next_times(next);
         simTime = mindt;
     }
+    if (iterations) *iterations = n;
     if (sim_time_out) *sim_time_out = simTime;
+    return writes;
+}
+
+extern "C" int oracle_run(const oracle_model *m, const oracle_reaction_table *t, int32_t *state,
+                          int steps, uint32_t seed0, int nthreads, float *sim_time_out) {
+    oracle_run_output(m, t, state, steps, seed0, nthreads, INFINITY, INFINITY, nullptr, nullptr, 0, nullptr,
+                      sim_time_out);
     return 0;
 }

@@ -242,3 +242,38 @@ def test_moments_x(gpu):
         n = st[sp].astype(np.int64)
         want = [int((n * np.arange(W)[None, :]).sum()), int(n.sum())]
         assert sums.cpu().tolist() == want
+
+
+@pytest.mark.parametrize("case", ["homogeneous", "inhomogeneous", "max_time", "overflow"])
+def test_output_events(gpu, oracle, case):
+    """The host loop with the reference's output events (a reaction step up to each output
+    time, then a zero-length iteration that writes) and its max_time stop, on the device,
+    against the oracle: final counts, clock, iterations, every snapshot and its time; a
+    run split by runContinue gives the same snapshots."""
+    inf = float("inf")
+    if case == "inhomogeneous":
+        m, st = configs.schnakenberg_model(64), configs.schnakenberg_state(64, seed=3)
+        steps, interval, max_time, md = 200, 0.5, 0.0, 64
+    else:
+        m, st = configs.plain_gmp_model(32), configs.plain_gmp_state(32)
+        steps, interval, max_time, md = {"homogeneous": (300, 0.1, 0.0, 64), "max_time": (2000, 0.3, 3.0, 64),
+                                         "overflow": (300, 0.05, 0.0, 4)}[case]
+    s = m.createSolver()
+    s.synchronizeStateBufferToDevice(st)
+    s.setOutput(interval, max_time, md)
+    if case == "homogeneous":
+        s.run(steps // 2, 21)
+        t = s.runContinue(steps - steps // 2)
+    else:
+        t = s.run(steps, 21)
+    counts, times, total, iters = s.getOutput()
+    fin = s.synchronizeStateBufferFromDevice()
+    want, wt, wit, dumps, wtimes, writes = oracle.run_output(m, st, steps, 21, interval,
+                                                             max_time if max_time > 0 else inf, max_dumps=md)
+    assert t == wt and iters == wit and total == writes
+    assert np.array_equal(fin, want)
+    assert np.array_equal(times, wtimes) and np.array_equal(counts, dumps)
+    if case == "max_time":
+        assert iters < steps and t >= max_time
+    if case == "overflow":
+        assert total > md and counts.shape[0] == md

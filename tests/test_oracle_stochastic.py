@@ -150,3 +150,62 @@ def test_schedule_seeds_and_times(oracle):
     assert t in times
     assert t <= float(np.float32(40 * 0.025)) + 1e-6
     assert (out >= 0).all()
+
+
+# ---- output events of the host loop ---------------------------------------------------
+def _schedule(dts, steps, interval, max_time):
+    """DiffusionModel::oclSolve's event choice (DiffusionModel.cpp:1479-1603) for a
+    homogeneous model in float32: returns (iterations, sim time, write times, the
+    iteration index after which each write happened)."""
+    f = np.float32
+    dts = [f(d) for d in dts]
+    nxt = list(dts)
+    sim, last, iv = f(0), f(0), f(interval)
+    writes, at = [0.0], [0]
+    n = 0
+    while n < steps and sim < f(max_time):
+        n += 1
+        mindt, minspec = nxt[0], 0
+        for i in range(1, len(nxt)):
+            if nxt[i] < mindt:
+                mindt, minspec = nxt[i], i
+        mo = f(last + iv)
+        if mo < mindt:
+            if sim >= f(last + iv):
+                writes.append(float(mo))
+                at.append(n)
+                last = f(last + iv)
+            sim = mo
+            continue
+        for i in range(len(nxt)):
+            if (i == minspec or nxt[i] <= mindt) and nxt[i] > 0:
+                nxt[i] = f(nxt[i] + dts[i])
+        sim = mindt
+    return n, float(sim), writes, at
+
+
+def test_output_events_schedule(oracle):
+    """The oracle's loop with output events against a restatement of the reference's event
+    choice: iterations, clock and write times; each write holds the state after its
+    iteration (a shorter run ends in the same state); without outputs the loop is the plain
+    one."""
+    from inchman_amd import configs
+
+    m = configs.plain_gmp_model(16)
+    st = configs.plain_gmp_state(16)
+    dts = [oracle.homogeneous_dt(m.p0, m.physical_length / m.width, sp.diffusionConstant) for sp in m.species]
+    for steps, interval, max_time in ((120, 0.1, float("inf")), (400, 0.37, 2.0), (60, 1000.0, float("inf"))):
+        fin, sim, it, dumps, times, w = oracle.run_output(m, st, steps, 9, interval, max_time, max_dumps=64)
+        n, sim_ref, writes, at = _schedule(dts, steps, interval, max_time)
+        assert it == n and np.float32(sim) == np.float32(sim_ref)
+        assert w == len(writes) and list(np.float32(times)) == list(np.float32(writes))
+        np.testing.assert_array_equal(dumps[0], st)
+        for k in sorted({1, len(at) // 2, len(at) - 1} & set(range(1, len(at)))):
+            part, *_ = oracle.run_output(m, st, at[k], 9, interval, max_time, max_dumps=1)
+            np.testing.assert_array_equal(dumps[k], part)
+        if max_time < float("inf"):
+            assert it < steps and sim >= max_time
+    fin, sim, it, *_ = oracle.run_output(m, st, 50, 9, float("inf"), float("inf"), max_dumps=1)
+    plain, sim2 = oracle.run(m, st, 50, 9)
+    np.testing.assert_array_equal(fin, plain)
+    assert sim == sim2 and it == 50
