@@ -178,6 +178,9 @@ SIGNATURES = {
     "gpgmp_individuals_maintain": [_P, _P, _I, _I, _P],
     "gpgmp_individuals_get_props": [_P, _P],
     "gpgmp_individuals_get_pool": [_P, _P, _P, _pI],
+    "gpgmp_individuals_bind_counts": [_P, _P],
+    "gpgmp_individuals_synchronize": [_P],
+    "gpgmp_solver_attach_individuals": [_P, _I, _P, _P],
 }
 
 _lib = None

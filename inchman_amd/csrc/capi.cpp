@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -58,6 +59,15 @@ struct gpgmp_solver {
     int32_t max_dumps = 0;
     DevBuf<int32_t> dumps;
     std::vector<float> dump_times;
+    // individual (particle-based) species, by species index: their counts in the state
+    // are the particles' (gpgmp_individuals_bind_counts), reactions are followed by the
+    // population maintenance and their diffusion is the particles' hops
+    struct Individuals {
+        gpgmp_individuals *g;
+        bool has_rule;
+        gpgmp_new_individual_rule rule;
+    };
+    std::map<int, Individuals> inds;
     void record_dump(float t) {
         const size_t n = eng->state_elems();
         if ((int32_t)dump_times.size() < max_dumps)
@@ -218,6 +228,11 @@ void api_launch(gpgmp_solver *s, int mode, const LaunchPlan &p, int32_t seed_cur
 }
 }  // namespace
 
+// status of a C-ABI call made inside a guarded body
+static void ok(int rc) {
+    if (rc != GPGMP_OK) throw Error(rc, gpgmp_last_error());
+}
+
 int gpgmp_solver_do_reactions(gpgmp_solver *s, float dt, int32_t *seed) {
     return guarded([&] {
         GPGMP_REQUIRE(s && seed, GPGMP_E_INVALID, "null argument");
@@ -227,6 +242,35 @@ int gpgmp_solver_do_reactions(gpgmp_solver *s, float dt, int32_t *seed) {
             api_launch(s, s->m.inhomogeneous() ? 1 : 0, p, *seed, *seed, s->eng->lost(1), s->eng->lost(0));
         }
         *seed = (int32_t)((uint32_t)*seed + 1u);  // Solver.cpp:239
+        // Solver.cpp:241-247: maintainPopulation(ii, seed++, d_state) per individual species
+        if (!s->inds.empty()) GPGMP_HIP_CHECK(hipStreamSynchronize(s->eng->stream()));
+        const int A = s->m.desc.width * s->m.desc.height;
+        for (auto &kv : s->inds) {
+            ok(gpgmp_individuals_maintain(kv.second.g, s->eng->state() + (size_t)kv.first * A, 1, *seed,
+                                          kv.second.has_rule ? &kv.second.rule : nullptr));
+            ok(gpgmp_individuals_synchronize(kv.second.g));
+            *seed = (int32_t)((uint32_t)*seed + 1u);
+        }
+    });
+}
+
+int gpgmp_solver_attach_individuals(gpgmp_solver *s, int32_t species, gpgmp_individuals *g,
+                                    const gpgmp_new_individual_rule *rule) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && g, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(s->m.desc.solver_type == GPGMP_STOCHASTIC_INHOMOGENEOUS, GPGMP_E_UNSUPPORTED,
+                      "individual species belong to the stochastic inhomogeneous solver");
+        GPGMP_REQUIRE(species >= 0 && species < s->m.desc.num_species, GPGMP_E_INVALID, "species out of range");
+        const int A = s->m.desc.width * s->m.desc.height;
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s->eng->stream()));
+        ok(gpgmp_individuals_bind_counts(g, s->eng->state() + (size_t)species * A));
+        float dx = 0.f, dy = 0.f;  // InhomogeneousSolver.cpp:205-225: the first steps from the particles
+        ok(gpgmp_individuals_time_steps(g, &dx, &dy));
+        s->dtx[species] = s->nx[species] = dx;
+        s->dty[species] = s->ny[species] = dy;
+        gpgmp_solver::Individuals a{g, rule != nullptr, {}};
+        if (rule) a.rule = *rule;
+        s->inds[species] = a;
     });
 }
 
@@ -245,6 +289,26 @@ int gpgmp_solver_diffuse(gpgmp_solver *s, int32_t species, float t, int32_t *see
             api_launch(s, 0, g, *seed, *seed, s->eng->lost(0), s->eng->lost(1));
             s->nx[species] += s->dtx[species];
             s->ny[species] = s->nx[species];
+            return;
+        }
+        auto it = s->inds.find(species);
+        if (it != s->inds.end()) {
+            // InhomogeneousSolver::diffuse for an individual species (InhomogeneousSolver.cpp:264-269,
+            // 304-343, 355-420): the hop sweep, seed++, then the constants and the new time step,
+            // which is the one added to the next diffusion time
+            gpgmp_individuals *g = it->second.g;
+            GPGMP_HIP_CHECK(hipStreamSynchronize(s->eng->stream()));
+            for (int axis = 0; axis < 2; ++axis) {
+                float &next = axis ? s->ny[species] : s->nx[species];
+                float &dt = axis ? s->dty[species] : s->dtx[species];
+                if (!(next <= t)) continue;
+                ok(gpgmp_individuals_diffuse(g, axis, dt, seed));
+                float dnew = 0.f;
+                ok(gpgmp_individuals_update(g, 1 << axis, axis ? nullptr : &dnew, axis ? &dnew : nullptr));
+                dt = dnew;
+                next += dt;
+            }
+            ok(gpgmp_individuals_synchronize(g));
             return;
         }
         if (s->nx[species] <= t) {
@@ -291,6 +355,8 @@ int gpgmp_solver_time_steps(gpgmp_solver *s, float *dt_x, float *dt_y) {
 int gpgmp_solver_run(gpgmp_solver *s, int32_t steps, uint32_t seed0, float *sim_time_out) {
     return guarded([&] {
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(s->inds.empty(), GPGMP_E_UNSUPPORTED,
+                      "models with individual species run through the per-call API (do_reactions / diffuse)");
         s->runner.reset();
         s->runner.reset(new GmpRunner(*s->eng, s->dtx, s->dty, {seed0}));
         s->runner_seed0 = seed0;

@@ -58,6 +58,11 @@ struct gpgmp_individuals {
     DevBuf<uint8_t> dead;
     DevBuf<int32_t> goal_dev;
     int np = 0;  // pool size; n + np == cap
+    int32_t *count_bound = nullptr;  // a device slice that mirrors the per-cell counts (solver state)
+    void publish_counts() {
+        if (count_bound)
+            GPGMP_HIP_CHECK(hipMemcpyAsync(count_bound, count.get(), (size_t)A * 4, hipMemcpyDeviceToDevice, s));
+    }
     uint32_t *mp_host = nullptr;  // pinned [8]
     float *dt_host = nullptr;  // pinned [2]
     float dtx = 0.f, dty = 0.f;
@@ -134,6 +139,7 @@ struct gpgmp_individuals {
         np = np - (int)total_new + (int)total_kill;
         GPGMP_HIP_CHECK(launch_ind_constants(off.get(), props.get(), cap, A, Dx.get(), Dy.get(), rx.get(), ry.get(),
                                              count.get(), s));
+        publish_counts();
     }
 
     // IndividualSolver::computeDiffusionConstants (IndividualSolver.cpp:1100-1153):
@@ -156,6 +162,7 @@ struct gpgmp_individuals {
         GPGMP_HIP_CHECK(launch_lower_bound(keys.get(), nullptr, n, A, off.get(), s));
         GPGMP_HIP_CHECK(launch_ind_constants(off.get(), props.get(), cap, A, Dx.get(), Dy.get(), rx.get(), ry.get(),
                                              count.get(), s));
+        publish_counts();
         if (axes & 1) GPGMP_HIP_CHECK(launch_ind_dt(Dx.get(), rx.get(), A, h, blk.get(), nb, dt_dev.get(), s));
         if (axes & 2) GPGMP_HIP_CHECK(launch_ind_dt(Dy.get(), ry.get(), A, h, blk.get(), nb, dt_dev.get() + 1, s));
         if (axes) {
@@ -380,6 +387,22 @@ int gpgmp_individuals_maintain(gpgmp_individuals *g, const int32_t *goal, int32_
             for (int k = 0; k < 4; ++k)
                 GPGMP_REQUIRE(rule->mode[k] >= 0 && rule->mode[k] <= 2, GPGMP_E_INVALID, "rule modes are 0, 1 or 2");
         g->maintain(goal, goal_on_device != 0, seed, rule);
+    });
+}
+
+int gpgmp_individuals_bind_counts(gpgmp_individuals *g, int32_t *dev_counts) {
+    return guarded_i([&] {
+        GPGMP_REQUIRE(g, GPGMP_E_INVALID, "null argument");
+        g->count_bound = dev_counts;
+        if (g->ready) g->publish_counts();
+        GPGMP_HIP_CHECK(hipStreamSynchronize(g->s));
+    });
+}
+
+int gpgmp_individuals_synchronize(gpgmp_individuals *g) {
+    return guarded_i([&] {
+        GPGMP_REQUIRE(g, GPGMP_E_INVALID, "null argument");
+        GPGMP_HIP_CHECK(hipStreamSynchronize(g->s));
     });
 }
 

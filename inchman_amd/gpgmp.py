@@ -251,6 +251,19 @@ class Solver:
         call("gpgmp_solver_run_continue", self.handle, int(steps), int(graph_chunk), C.byref(t))
         return t.value
 
+    def attachIndividualSpecies(self, species, individuals, rule=None):
+        """Makes `species` particle-based: `individuals` (an IndividualSpecies on this grid with
+        that species index) holds its particles; doReactions then maintains their population
+        and diffuse hops them.  rule: None or ((mode0..3), (a0..3), (b0..3))."""
+        r = None
+        if rule is not None:
+            r = _lib.NewIndividualRule()
+            for k in range(4):
+                r.mode[k], r.a[k], r.b[k] = int(rule[0][k]), float(rule[1][k]), float(rule[2][k])
+        idx = species.index if isinstance(species, Species) else int(species)
+        call("gpgmp_solver_attach_individuals", self.handle, idx, individuals.handle, C.byref(r) if r else None)
+        self._individuals = individuals  # keeps it alive as long as the solver
+
     def setOutput(self, interval, max_time=0.0, max_dumps=64):
         """Output events every `interval` of simulation time (DiffusionModel::outputInterval)
         and the stop at max_time (DiffusionModel::maxTime); <= 0 disables either.  Runs then

@@ -90,8 +90,14 @@ class IndividualSpecies:
         call("gpgmp_individuals_sweeps", self.handle, C.byref(v))
         return v.value
 
+    def _refresh_n(self):
+        npool = C.c_int32()
+        call("gpgmp_individuals_get_pool", self.handle, None, None, C.byref(npool))
+        self.n = self.capacity - npool.value
+
     def getPositionList(self):
         """(sorted cell keys, particle indices, per-cell counts)."""
+        self._refresh_n()
         keys = np.zeros(max(self.n, 1), np.uint32)
         vals = np.zeros(max(self.n, 1), np.uint32)
         cnt = np.zeros(self.width * self.height, np.int32)
@@ -132,6 +138,7 @@ class IndividualSpecies:
 
     def getPropertiesList(self):
         """Properties [4][n] of the live particles in sorted order."""
+        self._refresh_n()
         out = np.zeros((4, max(self.n, 1)), np.float32)
         call("gpgmp_individuals_get_props", self.handle, out.ctypes.data)
         return out[:, : self.n] if self.n else np.zeros((4, 0), np.float32)

@@ -83,6 +83,17 @@ typedef struct gpgmp_rdd gpgmp_rdd;
 typedef struct gpgmp_particles gpgmp_particles;
 typedef struct gpgmp_individuals gpgmp_individuals;
 
+/* The model's newIndividuals method (DiffusionModel::getNewIndividualsMethod, pasted into
+ * updateNewParticles at IndividualSolver.cl:1284) as a rule per property (DiffusivityX,
+ * DiffusivityY, DriftX, DriftY): mode 0 keeps the recycled particle's stored value (an
+ * empty method), 1 sets the constant a, 2 sets a + b * RANDOM evaluated in double and stored
+ * as float, RANDOM being the reference's macro (IndividualSolver.cl:1262-1282), drawn in
+ * property order. */
+typedef struct {
+    int32_t mode[4];
+    double a[4], b[4];
+} gpgmp_new_individual_rule;
+
 const char *gpgmp_last_error(void);
 int gpgmp_version(int32_t *major, int32_t *minor);
 
@@ -115,6 +126,17 @@ int gpgmp_solver_diffuse(gpgmp_solver *s, int32_t species, float t, int32_t *see
 /* Solver::getNextDiffusionTime() (Solver.h:68; HomogeneousSolver.h:42,
  * InhomogeneousSolver.cpp:238-252): out[num_species]. */
 int gpgmp_solver_next_diffusion_time(gpgmp_solver *s, float *out);
+/* Makes `species` an individual (particle-based) species of this stochastic inhomogeneous
+ * solver (Solver(..., hasIndividualSpecies), IndividualSolver.cpp:37-130): its counts in the
+ * state become the particles' (gpgmp_individuals_bind_counts), gpgmp_solver_do_reactions
+ * follows the SSA with gpgmp_individuals_maintain(seed++) for each individual species in
+ * index order (Solver.cpp:241-247, `rule` = the model's newIndividuals method, NULL = keep),
+ * and gpgmp_solver_diffuse hops its particles, each sweep followed by the constants and the
+ * new time step (InhomogeneousSolver.cpp:254-427).  `g` must have the solver's grid, its
+ * species index and its particles set; it stays caller-owned.  Such models run through this
+ * per-call API (gpgmp_solver_run refuses them), synchronising once per call. */
+int gpgmp_solver_attach_individuals(gpgmp_solver *s, int32_t species, gpgmp_individuals *g,
+                                    const gpgmp_new_individual_rule *rule);
 /* Per-species time steps: homogeneous dt[s] (both arrays), inhomogeneous dtX/dtY. */
 int gpgmp_solver_time_steps(gpgmp_solver *s, float *dt_x, float *dt_y);
 /* DiffusionModel::oclSolve main loop (DiffusionModel.cpp:1467-1603) for `steps`
@@ -331,16 +353,6 @@ int gpgmp_individuals_sweeps(gpgmp_individuals *g, int64_t *sweeps);
 /* Device pointers of the sorted keys, particle indices and CSR cell offsets [W*H+1]. */
 int gpgmp_individuals_keys_device(gpgmp_individuals *g, uint32_t **keys, uint32_t **vals, uint32_t **off);
 
-/* The model's newIndividuals method (DiffusionModel::getNewIndividualsMethod, pasted into
- * updateNewParticles at IndividualSolver.cl:1284) as a rule per property (DiffusivityX,
- * DiffusivityY, DriftX, DriftY): mode 0 keeps the recycled particle's stored value (an
- * empty method), 1 sets the constant a, 2 sets a + b * RANDOM evaluated in double and stored
- * as float, RANDOM being the reference's macro (IndividualSolver.cl:1262-1282), drawn in
- * property order. */
-typedef struct {
-    int32_t mode[4];
-    double a[4], b[4];
-} gpgmp_new_individual_rule;
 /* IndividualSolver::maintainPopulation(speciesIndex, seed, d_state) (IndividualSolver.cpp:458-671,
  * kernels IndividualSolver.cl:1017-1599), called by Solver::doReactions after the SSA for each
  * individual species with seed++ (Solver.cpp:241-247): afterwards cell c holds exactly goal[c]
@@ -354,6 +366,12 @@ typedef struct {
  * pool holds fewer ids than needed (nothing changes); GPGMP_E_INVALID for a negative goal. */
 int gpgmp_individuals_maintain(gpgmp_individuals *g, const int32_t *goal, int32_t goal_on_device, int32_t seed,
                                const gpgmp_new_individual_rule *rule);
+/* Mirror the per-cell counts into dev_counts[W*H] (device; NULL stops) after every update and
+ * maintenance, as computeDiffusionConstants writes the species' slice of d_state
+ * (IndividualSolver.cl:700-747); copies the current counts now. */
+int gpgmp_individuals_bind_counts(gpgmp_individuals *g, int32_t *dev_counts);
+/* Waits for the handle's queued work. */
+int gpgmp_individuals_synchronize(gpgmp_individuals *g);
 /* Properties of the live particles in sorted order, props[4][n]
  * (IndividualSolver::getPropertiesList, IndividualSolver.h:36-297). */
 int gpgmp_individuals_get_props(gpgmp_individuals *g, float *props);
