@@ -216,7 +216,7 @@ class GmpEngine {
     std::vector<RealInfo> reals_;
     DevBuf<int32_t> state_;
     DevBuf<int32_t> lost_[2];
-    DevBuf<float2> prob_[2];
+    DevBuf<int4> prob_[2];
     DevBuf<RealInfo> reals_dev_;
     DevBuf<Tile> tiles_dev_[2];  // tiles for 1 and for 4 lanes per cell
     DevBuf<HotEntry> hot_e_;

@@ -6,15 +6,38 @@
 
 #include <algorithm>
 
+#include "gpgmp_numeric.h"
 #include "stochastic_launch.h"
 #include "stochastic_types.h"
 
 namespace gpgmp {
 
+// Exact integer form of the test u <= p, u = u01(w) = float(w ^ 2^31) * 2^-32: u01 is
+// non-decreasing in m = w ^ 2^31, so {w : u01(w) <= p} = {m <= T} with T the largest such
+// m (binary search over the same u01), i.e. (int32)w <= (int32)(T ^ 2^31) (signed order of
+// w is the unsigned order of m).  p < 0 or NaN admits no draw: none = 1 (the threshold
+// then admits only m = 0, and the count is zeroed per cell afterwards).
+__device__ __forceinline__ int32_t hop_threshold(float p, int32_t &none) {
+    none = 0;
+    if (!(p >= 0.f)) {
+        none = 1;
+        return (int32_t)0x80000000u;
+    }
+    uint32_t lo = 0u, hi = 0xffffffffu;  // u01 at m = lo is 0 <= p
+    if (u01(hi ^ 0x80000000u) <= p) return (int32_t)(hi ^ 0x80000000u);
+    while (hi - lo > 1u) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (u01(mid ^ 0x80000000u) <= p) lo = mid;
+        else hi = mid;
+    }
+    return (int32_t)(lo ^ 0x80000000u);
+}
+
 // Inhomogeneous hop probabilities (StochasticInhomogeneousDriftDiffusion.cl:518-520 /
-// :648-650) with the reference's float/double promotions; stored as (min(p1,p2), p2).
-// They depend only on the fields and the sweep's dt, so they are tabulated once.
-__global__ void k_inh_probs(float2 *__restrict__ out, const float *__restrict__ D,
+// :648-650) with the reference's float/double promotions, stored as the thresholds of
+// (min(p1,p2), p2) with their none-mask (bit 0, bit 1).  They depend only on the fields
+// and the sweep's dt, so they are tabulated once.
+__global__ void k_inh_probs(int4 *__restrict__ out, const float *__restrict__ D,
                             const float *__restrict__ r, float dt, float dx, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -26,7 +49,10 @@ __global__ void k_inh_probs(float2 *__restrict__ out, const float *__restrict__ 
     const double den = 2. * (double)Di + (double)rrdt;
     const float p1 = (float)((1. - (double)ps) * (1. + (double)dxr / den) / 2.);
     const float p2 = (float)((double)p1 + (1. - (double)ps) * (1. - (double)dxr / den) / 2.);
-    out[i] = make_float2((p1 <= p2) ? p1 : p2, p2);
+    int32_t n1, n2;
+    const int32_t t1 = hop_threshold((p1 <= p2) ? p1 : p2, n1);
+    const int32_t t2 = hop_threshold(p2, n2);
+    out[i] = make_int4(t1, t2, n1 | (n2 << 1), 0);
 }
 
 // Per-cell admissible time step of the stochastic inhomogeneous solver
@@ -100,7 +126,7 @@ hipError_t launch_min_dt(const float *D, const float *r, int n, float h, float *
     return hipGetLastError();
 }
 
-hipError_t launch_inh_probs(float2 *out, const float *D, const float *r, float dt, float dx, int n,
+hipError_t launch_inh_probs(int4 *out, const float *D, const float *r, float dt, float dx, int n,
                             hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_inh_probs, dim3((n + 255) / 256), dim3(256), 0, stream, out, D, r, dt, dx, n);

@@ -226,25 +226,30 @@ __device__ __forceinline__ void hom_outflow(int n, uint32_t k0, uint32_t k1, con
 
 // Per-particle Bernoulli hop of inhomogeneous_diffuseX/Y
 // (StochasticInhomogeneousDriftDiffusion.cl:524-547): dir = +[u <= p1], counted
-// when u <= p2.  pr = (min(p1,p2), p2): plus = #(u <= pr.x), minus = #(u <= pr.y) - plus.
+// when u <= p2.  With (min(p1,p2), p2) as exact integer thresholds (k_inh_probs):
+// plus = #(w <= x), minus = #(w <= y) - plus, then the per-cell none-mask fix-up.
+__device__ __forceinline__ void hop_fixup(const int4 &pr, int32_t &a, int32_t &b) {
+    if (pr.z & 1) a = 0;
+    if (pr.z & 2) b = 0;
+}
+
 template <int G>
-__device__ __forceinline__ void inh_outflow(int n, uint32_t k0, uint32_t k1, float2 pr, int32_t &minus,
+__device__ __forceinline__ void inh_outflow(int n, uint32_t k0, uint32_t k1, int4 pr, int32_t &minus,
                                             int32_t &plus) {
     int32_t a = 0, b = 0;
     const int calls = (n + 1) >> 1;
     for (int k = (int)(threadIdx.x & (G - 1)); k < calls; k += G) {
         const u32x2 w = threefry2x32_20((uint32_t)k + 1u, kCtrHi, k0, k1);
-        float u = u01(w.x);
-        a += u <= pr.x;
-        b += u <= pr.y;
+        a += (int32_t)w.x <= pr.x;
+        b += (int32_t)w.x <= pr.y;
         if (2 * k + 1 < n) {
-            u = u01(w.y);
-            a += u <= pr.x;
-            b += u <= pr.y;
+            a += (int32_t)w.y <= pr.x;
+            b += (int32_t)w.y <= pr.y;
         }
     }
     a = grp_sum<G>(a);
     b = grp_sum<G>(b);
+    hop_fixup(pr, a, b);
     plus = a;
     minus = b - a;
 }
@@ -284,7 +289,7 @@ __device__ __forceinline__ int xcd_tile(int b, int ntiles) {
 template <int MODE, int G>
 __device__ __forceinline__ int32_t cell_outflow(int32_t n, int s, int c, int W, int H, int A, int32_t seed,
                                                 const LaunchPlan &pl, int4 bm, const RealInfo &ri,
-                                                const float2 *__restrict__ prob, int32_t *__restrict__ lout,
+                                                const int4 *__restrict__ prob, int32_t *__restrict__ lout,
                                                 const HomProbs &P) {
     constexpr int ND = (MODE == MODE_HOM) ? 4 : 2;
     const bool lead = (threadIdx.x & (G - 1)) == 0;
@@ -305,7 +310,7 @@ __device__ __forceinline__ int32_t cell_outflow(int32_t n, int s, int c, int W, 
         }
         return l0 + l1 + l2 + l3;
     } else {
-        const float2 pr = prob[ri.prob_off + (size_t)s * A + c];
+        const int4 pr = prob[ri.prob_off + (size_t)s * A + c];
         int32_t mi, pl_;
         inh_outflow<G>(n, (uint32_t)(c + s * A), k1, pr, mi, pl_);
         const int32_t mm = mode_x(MODE) ? m0 : m2;
@@ -329,7 +334,7 @@ __device__ __forceinline__ int32_t cell_outflow(int32_t n, int s, int c, int W, 
 struct FlatLds {
     int32_t start[4][65];       // exclusive scan of the calls per cell, [NC] = total
     int32_t n[4][64];           // particles per cell
-    float2 pr[4][64];           // inhomogeneous (min(p1,p2), p2) per cell
+    int2 pr[4][64];             // inhomogeneous thresholds of (min(p1,p2), p2) per cell
     uint32_t cnt[4][64][5];     // counters per cell
 };
 
@@ -347,12 +352,12 @@ struct FlatWalker {
     int qq;
     uint32_t end, k, k0;
     int32_t n;
-    float2 pr;
+    int2 pr;
     uint32_t c[5];
 
     __device__ __forceinline__ void load(const FlatLds &L, int wave, int base, uint32_t koff) {
         n = L.n[wave][qq];
-        pr = (MODE != MODE_HOM) ? L.pr[wave][qq] : make_float2(0.f, 0.f);
+        pr = (MODE != MODE_HOM) ? L.pr[wave][qq] : make_int2(0, 0);
         k0 = (uint32_t)(base + qq) + koff;
 #pragma unroll
         for (int j = 0; j < 5; ++j) c[j] = 0u;
@@ -393,14 +398,15 @@ struct FlatWalker {
     // particles 2k (word x) and 2k+1 (word y, if the cell has it)
     __device__ __forceinline__ void count(const u32x2 &w, bool valid, const HomProbs &P) {
         const bool two = valid && (int32_t)(2 * k + 1) < n;
-        const float u = u01(w.x), v = u01(w.y);
         if (MODE == MODE_HOM) {
+            const float u = u01(w.x), v = u01(w.y);
             c[0] += (valid && u <= P.stay) + (two && v <= P.stay);
             c[1] += (valid && u <= P.p1) + (two && v <= P.p1);
             c[2] += (valid && u <= P.p2) + (two && v <= P.p2);
             c[3] += (valid && u <= P.p3) + (two && v <= P.p3);
             c[4] += (valid && u <= P.p4) + (two && v <= P.p4);
         } else {
+            const int32_t u = (int32_t)w.x, v = (int32_t)w.y;
             c[0] += (valid && u <= pr.x) + (two && v <= pr.x);
             c[1] += (valid && u <= pr.y) + (two && v <= pr.y);
         }
@@ -410,7 +416,7 @@ struct FlatWalker {
 template <int MODE, int LPC>
 __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, int c, int s, int W, int H, int A,
                                                   int32_t seed, const LaunchPlan &pl, int4 bm, const RealInfo &ri,
-                                                  const float2 *__restrict__ prob, int32_t *__restrict__ lout,
+                                                  const int4 *__restrict__ prob, int32_t *__restrict__ lout,
                                                   const HomProbs &P, FlatLds &L) {
     constexpr int NC = 64 / LPC;
     constexpr int NCNT = (MODE == MODE_HOM) ? 5 : 2;
@@ -430,10 +436,15 @@ __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, in
         if (lane >= off) incl += t;
     }
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
+    int32_t none = 0;  // the cell's none-mask (thresholds below every draw)
     if (own) {
         st[q] = incl - calls;
         L.n[wave][q] = xs;
-        if (MODE != MODE_HOM) L.pr[wave][q] = prob[ri.prob_off + (size_t)s * A + c];
+        if (MODE != MODE_HOM) {
+            const int4 pr4 = prob[ri.prob_off + (size_t)s * A + c];
+            L.pr[wave][q] = make_int2(pr4.x, pr4.y);
+            none = pr4.z;
+        }
 #pragma unroll
         for (int j = 0; j < NCNT; ++j) L.cnt[wave][q][j] = 0u;
     }
@@ -483,7 +494,9 @@ __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, in
         } else {
             const int32_t mm = mode_x(MODE) ? m0 : m2;
             const int32_t mp = mode_x(MODE) ? m1 : m3;
-            const int32_t lm = mm * (int32_t)(cn[1] - cn[0]), lp = mp * (int32_t)cn[0];
+            int32_t a = (int32_t)cn[0], b = (int32_t)cn[1];
+            hop_fixup(make_int4(0, 0, none, 0), a, b);
+            const int32_t lm = mm * (b - a), lp = mp * a;
             l[0 * (size_t)A + c] = lm;
             l[1 * (size_t)A + c] = lp;
             xs -= lm + lp;
@@ -497,7 +510,7 @@ __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, in
 template <int S, int MODE, int G, int GS>
 __device__ __forceinline__ void step_tile(
     int t, int32_t *__restrict__ state, const int32_t *__restrict__ lost_in, int32_t *__restrict__ lost_out,
-    const float2 *__restrict__ prob, const RealInfo *__restrict__ reals,
+    const int4 *__restrict__ prob, const RealInfo *__restrict__ reals,
     const Tile *__restrict__ tiles, const LaunchPlan *__restrict__ plan,
     const int32_t *__restrict__ seeds_cur, const int32_t *__restrict__ seeds_prev, const RxTable &T,
     const HomProbs &P, int4 bm, int cpt, const HotList &hot) {
@@ -643,7 +656,7 @@ __device__ __forceinline__ void step_tile(
 template <int S, int MODE, int G, int GS>
 __global__ __launch_bounds__(256) void k_gmp_step(
     int32_t *__restrict__ state, const int32_t *__restrict__ lost_in, int32_t *__restrict__ lost_out,
-    const float2 *__restrict__ prob, const RealInfo *__restrict__ reals,
+    const int4 *__restrict__ prob, const RealInfo *__restrict__ reals,
     const Tile *__restrict__ tiles, int ntiles, const LaunchPlan *__restrict__ plan,
     const int32_t *__restrict__ seeds_cur, const int32_t *__restrict__ seeds_prev, RxTable T,
     HomProbs P, int4 bm, int cpt, HotList hot) {
@@ -791,7 +804,7 @@ __device__ __forceinline__ void ssa_wave_spec(int32_t (&x)[S], uint32_t cell, ui
 // list.
 template <int S, int MODE>
 __device__ __forceinline__ void hot_cells(int wk, int nwk, int32_t *__restrict__ state, int32_t *__restrict__ lost_out,
-                                          const float2 *__restrict__ prob, const RealInfo *__restrict__ reals,
+                                          const int4 *__restrict__ prob, const RealInfo *__restrict__ reals,
                                           const LaunchPlan &pss, const LaunchPlan &pl,
                                           const int32_t *__restrict__ seeds_cur, const RxTable &T, const HomProbs &P,
                                           int4 bm, const HotList &hot) {
@@ -847,21 +860,20 @@ __device__ __forceinline__ void hot_cells(int wk, int nwk, int32_t *__restrict__
                     }
                     x[s] -= l0 + l1 + l2 + l3;
                 } else {
-                    const float2 pr = prob[ri.prob_off + (size_t)s * A + c];
+                    const int4 pr = prob[ri.prob_off + (size_t)s * A + c];
                     int32_t a = 0, b = 0;
                     for (int k = lane; k < calls; k += 64) {
                         const u32x2 ww = threefry2x32_20((uint32_t)k + 1u, kCtrHi, (uint32_t)(c + s * A), k1o);
-                        float u = u01(ww.x);
-                        a += u <= pr.x;
-                        b += u <= pr.y;
+                        a += (int32_t)ww.x <= pr.x;
+                        b += (int32_t)ww.x <= pr.y;
                         if (2 * k + 1 < n) {
-                            u = u01(ww.y);
-                            a += u <= pr.x;
-                            b += u <= pr.y;
+                            a += (int32_t)ww.y <= pr.x;
+                            b += (int32_t)ww.y <= pr.y;
                         }
                     }
                     a = wave_sum(a);
                     b = wave_sum(b);
+                    hop_fixup(pr, a, b);
                     const int32_t lm = m0 * (b - a), lp = m1 * a;
                     if (lane == 0) {
                         l[0 * (size_t)A + c] = lm;
@@ -891,7 +903,7 @@ __device__ __forceinline__ void hot_cells(int wk, int nwk, int32_t *__restrict__
 // Hot-cell kernel after the step launch (one wave per block).
 template <int S, int MODE>
 __global__ __launch_bounds__(64) void k_gmp_hot(int32_t *__restrict__ state, int32_t *__restrict__ lost_out,
-                                                const float2 *__restrict__ prob, const RealInfo *__restrict__ reals,
+                                                const int4 *__restrict__ prob, const RealInfo *__restrict__ reals,
                                                 const LaunchPlan *__restrict__ plan,
                                                 const int32_t *__restrict__ seeds_cur, RxTable T, HomProbs P,
                                                 int4 bm, HotList hot) {
@@ -910,7 +922,7 @@ __global__ __launch_bounds__(64) void k_gmp_hot(int32_t *__restrict__ state, int
 template <int S>
 __global__ __launch_bounds__(256) void k_gmp_xs_hot(
     int32_t *__restrict__ state, const int32_t *__restrict__ lost_in, int32_t *__restrict__ lost_out,
-    const float2 *__restrict__ prob, const RealInfo *__restrict__ reals,
+    const int4 *__restrict__ prob, const RealInfo *__restrict__ reals,
     const Tile *__restrict__ tiles, int ntiles, const LaunchPlan *__restrict__ plan,
     const LaunchPlan *__restrict__ plan_ssa, const int32_t *__restrict__ seeds_cur,
     const int32_t *__restrict__ seeds_prev, RxTable T, HomProbs P, int4 bm, int cpt, HotList hot,

@@ -14,7 +14,7 @@ struct StepLaunch {
     int32_t *state;
     const int32_t *lost_in;
     int32_t *lost_out;
-    const float2 *prob;     // inhomogeneous probability table of this launch's axis
+    const int4 *prob;       // inhomogeneous hop thresholds of this launch's axis
     const RealInfo *reals;
     const Tile *tiles;
     int ntiles;
@@ -46,7 +46,7 @@ hipError_t launch_gmp_inh_y(const StepLaunch &a);
 hipError_t launch_gmp_inh_xs(const StepLaunch &a);
 hipError_t launch_min_dt(const float *D, const float *r, int n, float h, float *blocks, float *out,
                          hipStream_t stream);
-hipError_t launch_inh_probs(float2 *out, const float *D, const float *r, float dt, float dx, int n,
+hipError_t launch_inh_probs(int4 *out, const float *D, const float *r, float dt, float dx, int n,
                             hipStream_t stream);
 
 }  // namespace gpgmp

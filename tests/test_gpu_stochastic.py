@@ -277,3 +277,40 @@ def test_output_events(gpu, oracle, case):
         assert iters < steps and t >= max_time
     if case == "overflow":
         assert total > md and counts.shape[0] == md
+
+
+@pytest.mark.parametrize("lanes", [(1, 0), (4, 4), (1, 1)])
+@pytest.mark.parametrize("kind", ["mixed", "pure_drift"])
+def test_extreme_hop_probabilities(gpu, oracle, kind, lanes):
+    """Fields whose hop probabilities leave [0, 1): cells with strong negative drift under a
+    time step set by fast-diffusing cells (p1 < 0: no draw goes +), pure-drift cells
+    (p2 = 1: every particle hops), zero fields; the whole host loop bit-exact against the
+    oracle (the device compares draws with exact integer thresholds and a none-mask)."""
+    from inchman_amd.gpgmp import STOCHASTIC_INHOMOGENEOUS, DiffusionModel
+
+    W = 32
+    rng = np.random.default_rng(8)
+    m = DiffusionModel(W, W, physical_length=float(W), p0=0.15, solver=STOCHASTIC_INHOMOGENEOUS)
+    a = m.addSpecies("A", 0.0)
+    m.addReaction(5.0, products={a: 1})
+    m.addReaction(0.1, reactants={a: 1})
+    D = np.zeros((1, W, W), np.float32)
+    r = np.zeros((1, W, W), np.float32)
+    if kind == "mixed":
+        D[0, :, : W // 2] = 3.33
+        D[0, :, W // 2:] = 0.001
+        r[0, :, W // 2:] = rng.choice([-0.5, 0.5, -2.0], (W, W // 2))
+        D[0, ::5, ::3] = 0.0
+    else:
+        r[0] = rng.choice([-0.5, 0.5], (W, W))
+    m.setDriftDiffusivityFields(D, D.copy(), r, -r)
+    st = rng.poisson(30, (1, W, W)).astype(np.int32)
+    s = m.createSolver()
+    s.setLanesPerCell(*lanes)
+    s.synchronizeStateBufferToDevice(st)
+    t = s.run(40, 17)
+    got = s.synchronizeStateBufferFromDevice()
+    want, tw = oracle.run(m, st, 40, 17)
+    assert t == tw
+    assert np.array_equal(got, want)
+    assert not np.array_equal(want, st)
