@@ -608,9 +608,11 @@ void GmpRunner::run(int steps, int graph_chunk) {
             n = j + 1;
             for (int r = 0; r < nreal; ++r) c.hseed[(size_t)(j + 1) * nreal + r] = seeds_[r].next();
             const float dts = (eng_.table_.R > 0) ? d.deltat : 0.f;
+            const int32_t spread = eng_.spread() ? 1 : 0;
             if (!eng_.inh_) {
                 LaunchPlan &p = c.hplan[j];
                 std::memset(&p, 0, sizeof p);
+                p.interleave = spread;
                 p.deltat = dts;
                 p.gather_mask = has_prev_ ? prev_.due_x : 0u;
                 p.gather_off = has_prev_ ? (eng_.table_.R > 0 ? 1 : 0) : 0;
@@ -623,6 +625,8 @@ void GmpRunner::run(int steps, int graph_chunk) {
                 LaunchPlan &py = c.hplan[lps * j + lps - 1];
                 std::memset(&px, 0, sizeof px);
                 std::memset(&py, 0, sizeof py);
+                px.interleave = spread;
+                py.interleave = spread;
                 px.deltat = dts;
                 px.gather_mask = has_prev_ ? prev_.due_y : 0u;
                 px.out_mask = d.due_x;

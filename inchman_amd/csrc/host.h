@@ -187,6 +187,12 @@ class GmpEngine {
     bool split_x() const { return inh_ && grp_sweep_ == 0; }
     // split steps continue the deferred cells inside the X-sweep launch
     bool fused_hot() const { return split_x(); }
+    // flattened sweeps spread each wave's 16 cells over the grid (stride A/16)
+    bool spread() const { return spread_; }
+    void set_spread(bool on) {
+        spread_ = on;
+        ++shape_version_;
+    }
 
     void set_state(int r, const int32_t *host);
     void get_state(int r, int32_t *host);
@@ -225,6 +231,7 @@ class GmpEngine {
     HotList hot_{};
     int ntiles_[2] = {0, 0};
     int block_ = 256, cpt_ = 1, grp_ = 1, grp_sweep_ = 1;
+    bool spread_ = true;
     uint64_t shape_version_ = 0;
     int64_t total_cells_ = 0;
     hipStream_t stream_;

@@ -349,7 +349,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // the cursor leaves the cell.
 template <int MODE>
 struct FlatWalker {
-    int qq;
+    int qq, stride;
     uint32_t end, k, k0;
     int32_t n;
     int2 pr;
@@ -358,7 +358,7 @@ struct FlatWalker {
     __device__ __forceinline__ void load(const FlatLds &L, int wave, int base, uint32_t koff) {
         n = L.n[wave][qq];
         pr = (MODE != MODE_HOM) ? L.pr[wave][qq] : make_int2(0, 0);
-        k0 = (uint32_t)(base + qq) + koff;
+        k0 = (uint32_t)(base + qq * stride) + koff;
 #pragma unroll
         for (int j = 0; j < 5; ++j) c[j] = 0u;
     }
@@ -414,7 +414,8 @@ struct FlatWalker {
 };
 
 template <int MODE, int LPC>
-__device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, int c, int s, int W, int H, int A,
+__device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, int c, int stride, int s, int W, int H,
+                                                  int A,
                                                   int32_t seed, const LaunchPlan &pl, int4 bm, const RealInfo &ri,
                                                   const int4 *__restrict__ prob, int32_t *__restrict__ lout,
                                                   const HomProbs &P, FlatLds &L) {
@@ -424,7 +425,7 @@ __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, in
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
     const int q = lane / LPC;
     const bool own = (lane % LPC) == 0;
-    const int base = c - q;  // the wave's first cell
+    const int base = c - q * stride;  // the cell of the wave's slot 0; slot q is base + q * stride
     const uint32_t k1 = kKeyBase + (uint32_t)seed + (uint32_t)pl.out_off[s];
     const uint32_t koff = (MODE == MODE_HOM) ? 0u : (uint32_t)(s * A);
     int32_t *st = L.start[wave];
@@ -457,6 +458,8 @@ __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, in
         // independent, so one wave keeps two in flight
         const uint32_t half = (hi - lo + 1) >> 1, midf = lo + half;
         FlatWalker<MODE> wa, wb;
+        wa.stride = stride;
+        wb.stride = stride;
         wa.start(lo, st, L, wave, NC, base, koff);
         const bool has_b = midf < hi;
         if (has_b) wb.start(midf, st, L, wave, NC, base, koff);
@@ -530,9 +533,15 @@ __device__ __forceinline__ void step_tile(
     __shared__ FlatLds flat_lds;
     (void)flat_lds;
 
+    // flattened sweeps with 16 cells per wave may spread a wave's cells over the grid
+    // (stride A/16), so that waves covering dense and sparse regions even out; per-cell
+    // results do not depend on the mapping (RNG keys are per cell)
+    constexpr bool kSpread = (GS == 0 && G == 4);
+    const int stride = (kSpread && pl.interleave) ? (A >> 4) : 1;
     for (int k = 0; k < cpt; ++k) {
-        const int c = tile.cell0 + k * cpb + (int)threadIdx.x / G;
-        if (c >= A) break;
+        const int lc = tile.cell0 + k * cpb + (int)threadIdx.x / G;
+        if (lc >= A) break;
+        const int c = (stride > 1) ? (lc & 15) * stride + (lc >> 4) : lc;
         const int gx = c % W, gy = c / W;
         int32_t x[S];
         // loads: every species for launches with an SSA, only touched species otherwise
@@ -610,8 +619,8 @@ __device__ __forceinline__ void step_tile(
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     if (!((pl.out_mask >> s) & 1u)) continue;
-                    wave_flat_outflow<MODE, G>(x[s], deferred, c, s, W, H, A, seed, pl, bm, ri, prob, lout, P,
-                                               flat_lds);
+                    wave_flat_outflow<MODE, G>(x[s], deferred, c, stride, s, W, H, A, seed, pl, bm, ri, prob, lout,
+                                               P, flat_lds);
                 }
             } else if constexpr (G == 1 && GS == 4) {
                 // the wave's 64 cells in 4 rounds of 16, cell 16r+q by lanes 4q..4q+3;

@@ -46,6 +46,8 @@ struct alignas(16) LaunchPlan {
     uint32_t out_mask;      // species whose outflow is computed in this launch
     int32_t gather_off;     // source-boundary seed offset, relative to the previous step's seed
     int32_t out_off[kMaxSpecies];
+    int32_t interleave;     // flattened sweeps, 16 cells per wave: wave p's slot q is cell
+                            // q*(A/16) + p (spread over the grid) instead of 16p + q
 };
 
 // A tile: cells [cell0, cell0 + cells) of realization r.

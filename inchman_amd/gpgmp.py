@@ -281,6 +281,11 @@ class Solver:
              C.byref(it))
         return counts[: ns.value], times[: ns.value], nt.value, it.value
 
+    def setSweepSpread(self, on=True):
+        """Flattened sweeps: spread each wave's 16 cells over the grid (default) or keep them
+        consecutive; results are identical."""
+        call("gpgmp_solver_set_sweep_spread", self.handle, int(bool(on)))
+
     def setLaunch(self, block, cells_per_thread):
         call("gpgmp_solver_set_launch", self.handle, int(block), int(cells_per_thread))
 

@@ -314,3 +314,30 @@ def test_extreme_hop_probabilities(gpu, oracle, kind, lanes):
     assert t == tw
     assert np.array_equal(got, want)
     assert not np.array_equal(want, st)
+
+
+@pytest.mark.parametrize("spread", [False, True])
+@pytest.mark.parametrize("lanes", [(1, 0), (4, 0)])
+def test_sweep_spread(gpu, oracle, spread, lanes):
+    """Flattened sweeps with each wave's 16 cells consecutive or spread over the grid
+    (stride W*H/16), on a non-square grid, the inhomogeneous and the homogeneous solver:
+    counts and clock bit-exact against the oracle either way."""
+    m = configs.schnakenberg_model(96, 48)
+    st = configs.schnakenberg_state(96, 48, seed=4)
+    s = m.createSolver()
+    s.setLanesPerCell(*lanes)
+    s.setSweepSpread(spread)
+    s.synchronizeStateBufferToDevice(st)
+    t = s.run(150, 6)
+    want, tw = oracle.run(m, st, 150, 6)
+    assert t == tw
+    assert np.array_equal(s.synchronizeStateBufferFromDevice(), want)
+    mh = configs.plain_gmp_model(64, 32)
+    sh = mh.createSolver()
+    sh.setLanesPerCell(*lanes)
+    sh.setSweepSpread(spread)
+    st2 = configs.plain_gmp_state(64, 32)
+    sh.synchronizeStateBufferToDevice(st2)
+    sh.run(80, 3)
+    want2, _ = oracle.run(mh, st2, 80, 3)
+    assert np.array_equal(sh.synchronizeStateBufferFromDevice(), want2)
