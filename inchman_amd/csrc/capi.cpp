@@ -411,6 +411,14 @@ int gpgmp_solver_get_output(gpgmp_solver *s, int32_t *counts, float *times, int3
     });
 }
 
+int gpgmp_solver_set_predict_busy(gpgmp_solver *s, int32_t on) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s->eng->stream()));
+        s->eng->set_predict(on != 0);
+    });
+}
+
 int gpgmp_solver_set_sweep_spread(gpgmp_solver *s, int32_t on) {
     return guarded([&] {
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");

@@ -252,14 +252,17 @@ GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vecto
     // (measured best on the 256x256 Schnakenberg run among budgets 2..64, thresholds 4..65)
     hot_.budget = 8;
     hot_.below_active = 24;
-    hot_e_.alloc(hot_.cap);
-    hot_count_.alloc(2);
+    hot_e_.alloc(2 * (size_t)hot_.cap);  // two lists when busy cells are predicted
+    hot_count_.alloc(4);
     hot_flag_.alloc((size_t)total_cells_);
+    hot_pred_.alloc(2 * (size_t)total_cells_);
     hot_.e = hot_e_.get();
     hot_.count = hot_count_.get();
     hot_.flag = nullptr;
-    GPGMP_HIP_CHECK(hipMemsetAsync(hot_count_.get(), 0, 2 * sizeof(uint32_t), stream_));
+    hot_.pred = hot_pred_.get();
+    hot_.pred_stride = (uint32_t)total_cells_;
     GPGMP_HIP_CHECK(hipMemsetAsync(hot_flag_.get(), 0, (size_t)total_cells_, stream_));
+    reset_hot();
     reals_dev_.alloc(reals_.size());
     GPGMP_HIP_CHECK(hipMemcpyAsync(reals_dev_.get(), reals_.data(), reals_.size() * sizeof(RealInfo),
                                    hipMemcpyHostToDevice, stream_));
@@ -376,10 +379,17 @@ void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cu
         // continues them beside the sweeps of the others
         a.hot.flag = hot_flag_.get();
         if (mode == 3) a.plan_ssa = plan - 1;
+        if (mode == 1) a.predict = predict() ? 1 : 0;
     }
     a.ev_hot_start = ev_hot_start;
     a.ev_hot_stop = ev_hot_stop;
     GPGMP_HIP_CHECK(launch_gmp_step(a));
+}
+
+// empty hot lists and no predicted cells (a run starts from here)
+void GmpEngine::reset_hot() {
+    GPGMP_HIP_CHECK(hipMemsetAsync(hot_count_.get(), 0, 4 * sizeof(uint32_t), stream_));
+    GPGMP_HIP_CHECK(hipMemsetAsync(hot_pred_.get(), 0, 2 * (size_t)total_cells_, stream_));
 }
 
 // ------------------------------------------------------------------------------------
@@ -417,6 +427,7 @@ GmpRunner::GmpRunner(GmpEngine &eng, const std::vector<float> &dtx, const std::v
     GPGMP_REQUIRE((int)seed0s.size() == eng.num_real(), GPGMP_E_INVALID, "one seed0 per realization");
     for (uint32_t s0 : seed0s) seeds_.emplace_back(s0);
     prev_seed_.assign(eng.num_real(), 0);
+    eng_.reset_hot();
 }
 
 GmpRunner::~GmpRunner() {
@@ -627,6 +638,13 @@ void GmpRunner::run(int steps, int graph_chunk) {
                 std::memset(&py, 0, sizeof py);
                 px.interleave = spread;
                 py.interleave = spread;
+                // split steps predict busy cells from the previous step's hot list
+                px.predict = eng_.predict() ? 1 : 0;
+                px.hot_parity = (int32_t)(plan_step_ & 1);
+                // the Y launch empties the hot list the X-sweep launch consumed (or, when
+                // predicting, the one this step's SSA launch read)
+                py.reset_hot = eng_.fused_hot() ? 1 : 0;
+                py.hot_parity = px.predict ? (px.hot_parity ^ 1) : 0;
                 px.deltat = dts;
                 px.gather_mask = has_prev_ ? prev_.due_y : 0u;
                 px.out_mask = d.due_x;
@@ -647,6 +665,7 @@ void GmpRunner::run(int steps, int graph_chunk) {
             }
             prev_ = d;
             has_prev_ = true;
+            ++plan_step_;
             if (d.dump && dump_) {  // end the chunk here: the copy follows its launches
                 dump = true;
                 dump_time = d.dump_time;

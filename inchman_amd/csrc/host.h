@@ -187,6 +187,14 @@ class GmpEngine {
     bool split_x() const { return inh_ && grp_sweep_ == 0; }
     // split steps continue the deferred cells inside the X-sweep launch
     bool fused_hot() const { return split_x(); }
+    // split steps with one SSA lane per cell: the SSA launch runs the previous step's busy
+    // cells in worker waves
+    bool predict() const { return predict_ && fused_hot() && grp_ == 1; }
+    void set_predict(bool on) {
+        predict_ = on;
+        ++shape_version_;
+    }
+    void reset_hot();
     // flattened sweeps spread each wave's 16 cells over the grid (stride A/16)
     bool spread() const { return spread_; }
     void set_spread(bool on) {
@@ -228,10 +236,12 @@ class GmpEngine {
     DevBuf<HotEntry> hot_e_;
     DevBuf<uint32_t> hot_count_;
     DevBuf<uint8_t> hot_flag_;
+    DevBuf<uint8_t> hot_pred_;
     HotList hot_{};
     int ntiles_[2] = {0, 0};
     int block_ = 256, cpt_ = 1, grp_ = 1, grp_sweep_ = 1;
     bool spread_ = true;
+    bool predict_ = false;
     uint64_t shape_version_ = 0;
     int64_t total_cells_ = 0;
     hipStream_t stream_;
@@ -289,6 +299,7 @@ class GmpRunner {
     std::vector<Chunk *> chunks_;
     std::function<void(float)> dump_;
     int64_t iterations_ = 0;
+    int64_t plan_step_ = 0;  // steps planned so far (hot-list parity)
     void timed_launch(int mode, const LaunchPlan *plan, const LaunchPlan &hplan, const int32_t *scur,
                       const int32_t *sprev, const int32_t *lin, int32_t *lout, int how);
     void collect_times();

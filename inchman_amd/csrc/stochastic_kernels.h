@@ -508,6 +508,19 @@ __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, in
     wave_lds_sync();  // the slice is reused by the next species / cell round
 }
 
+// Events above which a cell counts as busy for the next step (one speculative batch):
+// such cells are run whole by worker waves of the next SSA launch.
+constexpr uint32_t kPredictEvents = 64;
+
+// The hot list of one parity: with predicted busy cells two lists alternate by step (the
+// SSA launch of step i reads list (i-1)&1 and fills list i&1).
+__device__ __forceinline__ HotList hot_view(const HotList &h, int parity) {
+    HotList v = h;
+    v.e = h.e + (size_t)parity * h.cap;
+    v.count = h.count + 2 * parity;
+    return v;
+}
+
 // One tile of a step launch: G lanes per cell for the SSA; the outflow by GS lanes per
 // cell (GS = G, or 4 with G = 1), or flattened over the wave (GS = 0).
 template <int S, int MODE, int G, int GS>
@@ -516,10 +529,19 @@ __device__ __forceinline__ void step_tile(
     const int4 *__restrict__ prob, const RealInfo *__restrict__ reals,
     const Tile *__restrict__ tiles, const LaunchPlan *__restrict__ plan,
     const int32_t *__restrict__ seeds_cur, const int32_t *__restrict__ seeds_prev, const RxTable &T,
-    const HomProbs &P, int4 bm, int cpt, const HotList &hot) {
+    const HomProbs &P, int4 bm, int cpt, const HotList &hot_all) {
     const Tile tile = tiles[t];
     const RealInfo ri = reals[tile.r];
     const LaunchPlan pl = *plan;
+    if (MODE == MODE_INH_Y && pl.reset_hot && t == 0 && threadIdx.x == 0) {
+        const HotList v = hot_view(hot_all, pl.hot_parity);
+        v.count[0] = 0u;
+        v.count[1] = 0u;
+    }
+    const bool predict = mode_ssa(MODE) && pl.predict && hot_all.flag;
+    const HotList hot = hot_view(hot_all, predict ? pl.hot_parity : 0);
+    const uint8_t *pred_prev = predict ? hot_all.pred + (size_t)(pl.hot_parity ^ 1) * hot_all.pred_stride : nullptr;
+    uint8_t *pred_next = predict ? hot_all.pred + (size_t)pl.hot_parity * hot_all.pred_stride : nullptr;
     const int32_t seed = seeds_cur[tile.r];
     const int W = ri.W, H = ri.H;
     const int A = W * H;
@@ -542,6 +564,11 @@ __device__ __forceinline__ void step_tile(
         const int lc = tile.cell0 + k * cpb + (int)threadIdx.x / G;
         if (lc >= A) break;
         const int c = (stride > 1) ? (lc & 15) * stride + (lc >> 4) : lc;
+        // a cell busy last step is run whole (gather + SSA) by a worker wave of this launch
+        if (predict && pred_prev[ri.cell_off + c]) {
+            if (g == 0) hot.flag[ri.cell_off + c] = 0;
+            continue;
+        }
         const int gx = c % W, gy = c / W;
         int32_t x[S];
         // loads: every species for launches with an SSA, only touched species otherwise
@@ -612,6 +639,7 @@ __device__ __forceinline__ void step_tile(
         }
 
         if (mode_ssa(MODE) && hot.flag && g == 0) hot.flag[ri.cell_off + c] = deferred ? 1 : 0;
+        if (predict && g == 0) pred_next[ri.cell_off + c] = deferred ? 1 : 0;
 
         // 3. outflow of the due species (a deferred cell's comes from the hot-cell kernel)
         if (pl.out_mask) {
@@ -811,12 +839,15 @@ __device__ __forceinline__ void ssa_wave_spec(int32_t (&x)[S], uint32_t cell, ui
 // with the particle loop split over the lanes (call k by lane k % 64) and the counts
 // summed -- the same integers as the serial loop.  The last worker to finish resets the
 // list.
+// finish: the last worker to finish empties the list (its own kernel); in the X-sweep
+// launch of a split step the following Y launch empties it instead (one thread), so the
+// workers do not serialise on a shared counter.
 template <int S, int MODE>
 __device__ __forceinline__ void hot_cells(int wk, int nwk, int32_t *__restrict__ state, int32_t *__restrict__ lost_out,
                                           const int4 *__restrict__ prob, const RealInfo *__restrict__ reals,
                                           const LaunchPlan &pss, const LaunchPlan &pl,
                                           const int32_t *__restrict__ seeds_cur, const RxTable &T, const HomProbs &P,
-                                          int4 bm, const HotList &hot) {
+                                          int4 bm, const HotList &hot, bool finish) {
     const int lane = (int)(threadIdx.x & 63);
     const uint32_t total = hot.count[0];
     if (total == 0) return;
@@ -824,6 +855,7 @@ __device__ __forceinline__ void hot_cells(int wk, int nwk, int32_t *__restrict__
     constexpr int ND = (MODE == MODE_HOM) ? 4 : 2;
     for (uint32_t i = (uint32_t)wk; i < nh; i += (uint32_t)nwk) {
         const HotEntry he = hot.e[i];
+        if (he.time < 0.f) continue;  // a predicted cell, already run whole by the SSA launch
         const RealInfo ri = reals[he.r];
         const int32_t seed = seeds_cur[he.r];
         const int W = ri.W, H = ri.H, A = W * H, c = he.c;
@@ -834,6 +866,9 @@ __device__ __forceinline__ void hot_cells(int wk, int nwk, int32_t *__restrict__
         float time = he.time;
         uint32_t ctr = he.ctr;
         ssa_wave_spec<S>(x, (uint32_t)c, kKeyBase + (uint32_t)seed, pss.deltat, T, time, ctr);
+        // the whole count is known now: busy cells are predicted for the next step
+        if (pss.predict && lane == 0)
+            hot.pred[(size_t)pss.hot_parity * hot.pred_stride + ri.cell_off + c] = ctr > kPredictEvents ? 1 : 0;
         if (pl.out_mask) {
             const int gx = c % W, gy = c / W;
             const int m0 = (gx == 0) ? bm.x : 1, m1 = (gx == W - 1) ? bm.y : 1;
@@ -898,13 +933,69 @@ __device__ __forceinline__ void hot_cells(int wk, int nwk, int32_t *__restrict__
         }
     }
     // every worker read the count before adding itself here, so the last one may reset
-    if (lane == 0) {
+    if (finish && lane == 0) {
         __threadfence();
         const uint32_t fin = atomicAdd(&hot.count[1], 1u);
         if (fin == (uint32_t)nwk - 1u) {
             atomicExch(&hot.count[0], 0u);
             atomicExch(&hot.count[1], 0u);
             __threadfence();
+        }
+    }
+}
+
+// Predicted busy cells (SSA launch of a split step): the cells of the other parity's list
+// whose prediction flag is set (more than kPredictEvents events last step) are run whole
+// by one wave each -- the gather of the Y outflow and the SSA from t = 0 with the
+// speculative batches -- while the tile lanes skip exactly those cells.  A cell still busy
+// goes into this step's list as a predicted entry (time -1) with its flag set, so the
+// next SSA launch runs it the same way.  The flag of a cell deferred by a tile lane is set
+// by that lane and decided by the X-sweep launch's worker once the cell's count is known.
+template <int S>
+__device__ __forceinline__ void pred_cells(int wk, int nwk, int32_t *__restrict__ state,
+                                           const int32_t *__restrict__ lost_in, const RealInfo *__restrict__ reals,
+                                           const LaunchPlan &pl, const int32_t *__restrict__ seeds_cur,
+                                           const RxTable &T, const HotList &hot_all) {
+    const int lane = (int)(threadIdx.x & 63);
+    const HotList prev = hot_view(hot_all, pl.hot_parity ^ 1);
+    const HotList cur = hot_view(hot_all, pl.hot_parity);
+    uint8_t *pred_next = hot_all.pred + (size_t)pl.hot_parity * hot_all.pred_stride;
+    const uint8_t *pred_prev = hot_all.pred + (size_t)(pl.hot_parity ^ 1) * hot_all.pred_stride;
+    const uint32_t total = prev.count[0];
+    const uint32_t nh = total < prev.cap ? total : prev.cap;
+    for (uint32_t i = (uint32_t)wk; i < nh; i += (uint32_t)nwk) {
+        const HotEntry he = prev.e[i];
+        const RealInfo ri = reals[he.r];
+        if (!pred_prev[ri.cell_off + he.c]) continue;  // listed, but not predicted: a tile lane runs it
+        const int32_t seed = seeds_cur[he.r];
+        const int W = ri.W, H = ri.H, A = W * H, c = he.c;
+        const int gy = c / W;
+        int32_t *st = state + ri.state_off;
+        const int32_t *lin = lost_in + ri.lost_off;
+        int32_t x[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            int32_t v = st[(size_t)s * A + c];
+            if ((pl.gather_mask >> s) & 1u) {  // the Y outflow of the previous launch
+                const int32_t *l = lin + (size_t)s * 2 * A;
+                if (gy > 0) v += l[1 * (size_t)A + c - W];
+                if (gy < H - 1) v += l[0 * (size_t)A + c + W];
+            }
+            x[s] = __builtin_amdgcn_readfirstlane(v);
+        }
+        float time = 0.f;
+        uint32_t ctr = 0;
+        if (pl.deltat > 0.f && T.R > 0) ssa_wave_spec<S>(x, (uint32_t)c, kKeyBase + (uint32_t)seed, pl.deltat, T, time, ctr);
+        if (lane == 0) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) st[(size_t)s * A + c] = x[s];
+            bool busy = ctr > kPredictEvents;
+            if (busy) {
+                const uint32_t idx = atomicAdd(cur.count, 1u);
+                if (idx < cur.cap) cur.e[idx] = HotEntry{he.r, c, -1.f, 0u};
+                else busy = false;
+            }
+            pred_next[ri.cell_off + c] = busy ? 1 : 0;
         }
     }
 }
@@ -918,7 +1009,7 @@ __global__ __launch_bounds__(64) void k_gmp_hot(int32_t *__restrict__ state, int
                                                 int4 bm, HotList hot) {
     const LaunchPlan pl = *plan;
     hot_cells<S, MODE>((int)blockIdx.x, (int)gridDim.x, state, lost_out, prob, reals, pl, pl, seeds_cur, T, P, bm,
-                       hot);
+                       hot, true);
 }
 
 // X sweeps and the deferred busy cells in one launch, so the cells' serial event chains
@@ -939,8 +1030,9 @@ __global__ __launch_bounds__(256) void k_gmp_xs_hot(
     if ((int)blockIdx.x < nhot_blocks) {
         const int wpb = (int)(blockDim.x >> 6);
         const LaunchPlan pss = *plan_ssa, pl = *plan;
+        const int par = pss.predict ? pss.hot_parity : 0;
         hot_cells<S, MODE_INH_X>((int)blockIdx.x * wpb + (int)(threadIdx.x >> 6), nhot_blocks * wpb, state,
-                                 lost_out, prob, reals, pss, pl, seeds_cur, T, P, bm, hot);
+                                 lost_out, prob, reals, pss, pl, seeds_cur, T, P, bm, hot_view(hot, par), false);
         return;
     }
     const int t = xcd_tile((int)blockIdx.x - nhot_blocks, ntiles);
@@ -949,6 +1041,46 @@ __global__ __launch_bounds__(256) void k_gmp_xs_hot(
                                     T, P, bm, cpt, hot);
 }
 
+
+// SSA launch of a split step with predicted busy cells: blocks [0, npred_blocks) are the
+// worker waves of pred_cells, the others the SSA tiles (one lane per cell), which skip the
+// predicted cells.  npred_blocks is a multiple of 8, so the tiles keep their XCD placement.
+template <int S>
+__global__ __launch_bounds__(256) void k_gmp_ssa_pred(
+    int32_t *__restrict__ state, const int32_t *__restrict__ lost_in, int32_t *__restrict__ lost_out,
+    const int4 *__restrict__ prob, const RealInfo *__restrict__ reals,
+    const Tile *__restrict__ tiles, int ntiles, const LaunchPlan *__restrict__ plan,
+    const int32_t *__restrict__ seeds_cur, const int32_t *__restrict__ seeds_prev, RxTable T, HomProbs P, int4 bm,
+    int cpt, HotList hot, int npred_blocks) {
+    if ((int)blockIdx.x < npred_blocks) {
+        const int wpb = (int)(blockDim.x >> 6);
+        const LaunchPlan pl = *plan;
+        pred_cells<S>((int)blockIdx.x * wpb + (int)(threadIdx.x >> 6), npred_blocks * wpb, state, lost_in, reals, pl,
+                      seeds_cur, T, hot);
+        return;
+    }
+    const int t = xcd_tile((int)blockIdx.x - npred_blocks, ntiles);
+    if (t >= ntiles) return;
+    step_tile<S, MODE_INH_X, 1, 0>(t, state, lost_in, lost_out, prob, reals, tiles, plan, seeds_cur, seeds_prev, T,
+                                   P, bm, cpt, hot);
+}
+
+template <int S>
+static hipError_t launch_ssa_pred(const StepLaunch &a) {
+    const int wpb = a.block / 64;
+    const int npred = 8 * ((a.hot_grid + 8 * wpb - 1) / (8 * wpb));
+    const int grid = npred + 8 * ((a.ntiles + 7) / 8);
+    if (a.ev_start) {
+        hipExtLaunchKernelGGL((k_gmp_ssa_pred<S>), dim3(grid), dim3(a.block), 0, a.stream, a.ev_start, a.ev_stop, 0,
+                              a.state, a.lost_in, a.lost_out, a.prob, a.reals, a.tiles, a.ntiles, a.plan,
+                              a.seeds_cur, a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot, npred);
+    } else {
+        hipLaunchKernelGGL((k_gmp_ssa_pred<S>), dim3(grid), dim3(a.block), 0, a.stream, a.state, a.lost_in,
+                           a.lost_out, a.prob, a.reals, a.tiles, a.ntiles, a.plan, a.seeds_cur, a.seeds_prev, *a.T,
+                           *a.P, a.bm, a.cpt, a.hot, npred);
+    }
+    return hipGetLastError();
+}
 
 template <int S, int MODE, int G, int GS>
 static hipError_t launch_sg(const StepLaunch &a) {
@@ -990,6 +1122,12 @@ static hipError_t launch_s(const StepLaunch &a) {
         if (a.hot.flag) {
             if (a.grp != 4 || a.grp_sweep != 0 || !a.plan_ssa) return hipErrorInvalidValue;
             return launch_xs_hot<S>(a);
+        }
+    }
+    if constexpr (MODE == MODE_INH_X) {
+        if (a.predict) {
+            if (!a.hot.flag || a.grp != 1 || a.grp_sweep != 0 || !a.hot.pred) return hipErrorInvalidValue;
+            return launch_ssa_pred<S>(a);
         }
     }
     if (a.grp != 1 && a.grp != 4) return hipErrorInvalidValue;

@@ -34,6 +34,7 @@ struct StepLaunch {
     HotList hot{};                  // deferred busy cells (modes with an SSA)
     int hot_grid = 1024;            // waves of the hot-cell kernel
     const LaunchPlan *plan_ssa = nullptr;  // X sweeps with the hot cells: the SSA launch's plan
+    int predict = 0;        // SSA launch of a split step: run last step's busy cells in worker waves
     hipEvent_t ev_hot_start = nullptr;  // optional: timestamps of the hot-cell dispatch
     hipEvent_t ev_hot_stop = nullptr;
 };

@@ -48,6 +48,10 @@ struct alignas(16) LaunchPlan {
     int32_t out_off[kMaxSpecies];
     int32_t interleave;     // flattened sweeps, 16 cells per wave: wave p's slot q is cell
                             // q*(A/16) + p (spread over the grid) instead of 16p + q
+    int32_t predict;        // split steps: the SSA launch runs last step's busy cells in
+                            // worker waves (hot lists alternate by step parity)
+    int32_t hot_parity;     // this step's hot list (0 or 1) when predicting
+    int32_t reset_hot;      // Y launch of a split step: empty hot list hot_parity (one thread)
 };
 
 // A tile: cells [cell0, cell0 + cells) of realization r.
@@ -75,6 +79,10 @@ struct HotList {
     // hot cells run inside the X-sweep launch: the SSA launch then writes every cell's
     // flag and the sweep waves skip the flagged cells.
     uint8_t *flag;
+    // predicted busy cells: two lists (e[0..cap), e[cap..2cap); count[2p], count[2p+1])
+    // and pred[p * pred_stride + cell_off + c] = 1 if the cell is in list p
+    uint8_t *pred;
+    uint32_t pred_stride;
 };
 
 // Homogeneous diffusion thresholds (StochasticHomogeneousDiffusion.cl:58-62)

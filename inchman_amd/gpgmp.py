@@ -281,6 +281,11 @@ class Solver:
              C.byref(it))
         return counts[: ns.value], times[: ns.value], nt.value, it.value
 
+    def setPredictBusy(self, on=True):
+        """Split steps: run last step's busy cells in worker waves of the SSA launch (default)
+        or defer them to the X-sweep launch as they turn up; results are identical."""
+        call("gpgmp_solver_set_predict_busy", self.handle, int(bool(on)))
+
     def setSweepSpread(self, on=True):
         """Flattened sweeps: spread each wave's 16 cells over the grid (default) or keep them
         consecutive; results are identical."""

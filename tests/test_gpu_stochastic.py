@@ -341,3 +341,24 @@ def test_sweep_spread(gpu, oracle, spread, lanes):
     sh.run(80, 3)
     want2, _ = oracle.run(mh, st2, 80, 3)
     assert np.array_equal(sh.synchronizeStateBufferFromDevice(), want2)
+
+
+@pytest.mark.parametrize("predict", [True, False])
+@pytest.mark.parametrize("budget,below", [(1, 65), (8, 24), (3, 40)])
+def test_predict_busy_cells(gpu, oracle, predict, budget, below):
+    """Split steps with the previous step's busy cells run whole in worker waves of the SSA
+    launch (hot lists alternating by step parity) or deferred as they turn up: counts and
+    clock bit-exact against the oracle, for eager launches and for HIP-graph chunks
+    continued across runs (the parity lives in the uploaded plans)."""
+    m = configs.schnakenberg_model(64)
+    st = configs.schnakenberg_state(64, seed=23)
+    s = m.createSolver()
+    s.setPredictBusy(predict)
+    s.setEventBudget(budget, below)
+    s.synchronizeStateBufferToDevice(st)
+    s.run(61, 9)
+    s.runContinue(40, 10)
+    t = s.runContinue(19)
+    want, tw = oracle.run(m, st, 120, 9)
+    assert t == tw
+    assert np.array_equal(s.synchronizeStateBufferFromDevice(), want)
