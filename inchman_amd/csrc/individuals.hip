@@ -15,6 +15,7 @@
 
 #include "gpgmp_numeric.h"
 #include "individuals_launch.h"
+#include "wave_ops.h"
 
 namespace gpgmp {
 
@@ -153,15 +154,7 @@ __global__ __launch_bounds__(256) void k_ind_dt_reduce(const float *__restrict__
 // block-wide scan plus the tile's scanned base, so no per-cell scan array is stored.
 constexpr int MP_T = 256;
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) { return wave_incl_scan_u32(v); }
 
 // exclusive scan of one value per thread over the MP_T-thread block; sm: MP_T/64 words
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sm, uint32_t &total) {

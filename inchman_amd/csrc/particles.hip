@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gpgmp_numeric.h"
+#include "wave_ops.h"
 
 namespace gpgmp {
 
@@ -42,12 +43,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t *__res
 // Block-wide exclusive scan of one value per thread (256 threads); returns the total.
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *smem /*[RS_WAVES]*/, uint32_t &total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
+    const uint32_t x = wave_incl_scan_u32(v);
     if (lane == 63) smem[wave] = x;
     __syncthreads();
     uint32_t wbase = 0, tot = 0;

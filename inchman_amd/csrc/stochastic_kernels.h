@@ -27,6 +27,7 @@
 #include "gpgmp_numeric.h"
 #include "stochastic_launch.h"
 #include "stochastic_types.h"
+#include "wave_ops.h"
 
 namespace gpgmp {
 
@@ -430,12 +431,7 @@ __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, in
     const uint32_t koff = (MODE == MODE_HOM) ? 0u : (uint32_t)(s * A);
     int32_t *st = L.start[wave];
     const int32_t calls = (own && !deferred) ? ((xs + 1) >> 1) : 0;
-    int32_t incl = calls;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int32_t t = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += t;
-    }
+    const int32_t incl = wave_incl_scan_i32(calls);
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
     int32_t none = 0;  // the cell's none-mask (thresholds below every draw)
     if (own) {
@@ -713,11 +709,7 @@ __device__ __forceinline__ double readlane_d(double v, int j) {
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), j);
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
-__device__ __forceinline__ int32_t wave_sum(int32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
+__device__ __forceinline__ int32_t wave_sum(int32_t v) { return wave_sum_i32(v); }
 
 // Mask of the reactions whose selection interval holds r2 -- the conditions of fire;
 // cumulative hazards never decrease, so at most one bit is set.
@@ -743,16 +735,7 @@ __device__ __forceinline__ int32_t net_of(const RxTable &T, uint32_t m, int s) {
     return d;
 }
 
-__device__ __forceinline__ int32_t wave_excl_scan(int32_t v) {
-    const int lane = (int)(threadIdx.x & 63);
-    int32_t incl = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int32_t t = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += t;
-    }
-    return incl - v;
-}
+__device__ __forceinline__ int32_t wave_excl_scan(int32_t v) { return wave_incl_scan_i32(v) - v; }
 
 // Continues performGillespie (Gillespie.cl:41-99) of one cell from (time, ctr) with the
 // 64 lanes of a wave, speculatively: lane j draws event ctr+1+j and selects its reaction
