@@ -101,6 +101,9 @@ Model compile_model(const gpgmp_model_desc &d) {
         }
         T.nf[i] = (uint8_t)nf;
     }
+    T.compact = (R <= kCompactReactions) ? 1 : 0;
+    for (int i = 0; i < R; ++i)
+        if (T.nf[i] > kCompactFactors) T.compact = 0;
     // StayProbability = (1.-p0) printed "%.6e" (Solver.cpp:505); the thresholds follow
     // StochasticHomogeneousDiffusion.cl:58-62.
     {

@@ -63,8 +63,53 @@ __device__ __forceinline__ float add_law(const RxTable &T, int i, float h0, cons
 }
 
 // Cumulative hazards _h[1..R] and the total _h[0] (Solver.cpp:843-857).
+// Compact tables: every law of the first kCompactReactions is formed at once in straight-line
+// code.  A factor past the law's nf is 1, which leaves the int and double product chains
+// exactly as the general path's; reactions i >= R leave the cumulative sum unchanged.  The
+// table entries sit at constant offsets of the kernel argument (scalar registers), and
+// the independent products overlap: a lone wave's event costs a fraction of the table
+// walk with per-factor branches and loads.
+template <int S>
+__device__ __forceinline__ void hazards_compact(const RxTable &T, const int32_t (&x)[S],
+                                                float (&h)[kMaxReactions + 1]) {
+    int32_t d[kCompactReactions][kCompactFactors];
+#pragma unroll
+    for (int f = 0; f < kCompactFactors; ++f)
+#pragma unroll
+        for (int i = 0; i < kCompactReactions; ++i)
+            d[i][f] = (f < (int)T.nf[i]) ? sel(x, T.fs[i][f]) - (int32_t)T.fo[i][f] : 1;
+    double vd[kCompactReactions];
+#pragma unroll
+    for (int i = 0; i < kCompactReactions; ++i) vd[i] = T.lit_dbl[i];
+#pragma unroll
+    for (int f = 0; f < kCompactFactors; ++f)
+#pragma unroll
+        for (int i = 0; i < kCompactReactions; ++i) vd[i] = vd[i] * (double)d[i][f];
+    uint32_t vi[kCompactReactions] = {};
+    if (T.lit_is_int) {
+#pragma unroll
+        for (int i = 0; i < kCompactReactions; ++i) vi[i] = (uint32_t)T.lit_int[i];
+#pragma unroll
+        for (int f = 0; f < kCompactFactors; ++f)
+#pragma unroll
+            for (int i = 0; i < kCompactReactions; ++i) vi[i] *= (uint32_t)d[i][f];
+    }
+    float t = 0.f;
+#pragma unroll
+    for (int i = 0; i < kCompactReactions; ++i) {
+        const float hi = ((T.lit_is_int >> i) & 1u) ? t + (float)(int32_t)vi[i] : (float)((double)t + vd[i]);
+        t = (i < T.R) ? hi : t;
+        h[i + 1] = t;
+    }
+    h[0] = t;
+}
+
 template <int S>
 __device__ __forceinline__ void hazards(const RxTable &T, const int32_t (&x)[S], float (&h)[kMaxReactions + 1]) {
+    if (T.compact) {
+        hazards_compact<S>(T, x, h);
+        return;
+    }
     h[0] = 0.f;
 #pragma unroll
     for (int i = 0; i < kMaxReactions; ++i) {
@@ -81,6 +126,15 @@ __device__ __forceinline__ void hazards(const RxTable &T, const int32_t (&x)[S],
 template <int S>
 __device__ __forceinline__ void fire(const RxTable &T, const float (&h)[kMaxReactions + 1], float r2,
                                      int32_t (&x)[S]) {
+    if (T.compact) {
+#pragma unroll
+        for (int i = 0; i < kCompactReactions; ++i) {
+            const bool f = (i < T.R) & ((i == 0) ? (r2 < h[1]) : ((r2 >= h[i]) & (r2 < h[i + 1])));
+#pragma unroll
+            for (int j = 0; j < S; ++j) x[j] += f ? (int32_t)T.net[i][j] : 0;
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < kMaxReactions; ++i) {
         if (i < T.R) {
@@ -716,6 +770,14 @@ __device__ __forceinline__ int32_t wave_sum(int32_t v) { return wave_sum_i32(v);
 template <int S>
 __device__ __forceinline__ uint32_t fired(const RxTable &T, const float (&h)[kMaxReactions + 1], float r2) {
     uint32_t m = 0;
+    if (T.compact) {
+#pragma unroll
+        for (int i = 0; i < kCompactReactions; ++i) {
+            const bool f = (i < T.R) & ((i == 0) ? (r2 < h[1]) : ((r2 >= h[i]) & (r2 < h[i + 1])));
+            m |= f ? (1u << i) : 0u;
+        }
+        return m;
+    }
 #pragma unroll
     for (int i = 0; i < kMaxReactions; ++i) {
         if (i < T.R) {
@@ -729,6 +791,11 @@ __device__ __forceinline__ uint32_t fired(const RxTable &T, const float (&h)[kMa
 // Net change of species s by the reactions of mask m (the update fire applies).
 __device__ __forceinline__ int32_t net_of(const RxTable &T, uint32_t m, int s) {
     int32_t d = 0;
+    if (T.compact) {  // fired sets no bit at or above R
+#pragma unroll
+        for (int i = 0; i < kCompactReactions; ++i) d += ((m >> i) & 1u) ? (int32_t)T.net[i][s] : 0;
+        return d;
+    }
 #pragma unroll
     for (int i = 0; i < kMaxReactions; ++i)
         if (i < T.R) d += ((m >> i) & 1u) ? (int32_t)T.net[i][s] : 0;

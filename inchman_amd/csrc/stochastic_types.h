@@ -8,6 +8,10 @@ namespace gpgmp {
 constexpr int kMaxSpecies = 8;
 constexpr int kMaxReactions = 16;
 constexpr int kMaxFactors = 6;
+// Small models (at most 4 reactions of at most 3 factors each) take a branch-free form of
+// the hazard and selection code in the event loops.
+constexpr int kCompactReactions = 4;
+constexpr int kCompactFactors = 3;
 // Upper bound on SSA events per cell per reaction step: a runaway model (hazards so
 // large that t += dt stops advancing in float) ends instead of spinning forever.
 constexpr uint32_t kMaxEventsPerCall = 1u << 24;
@@ -24,6 +28,7 @@ struct RxTable {
     uint8_t fs[kMaxReactions][kMaxFactors];  // factor species
     uint8_t fo[kMaxReactions][kMaxFactors];  // factor offset m of (X - m)
     int8_t net[kMaxReactions][kMaxSpecies];  // product - reactant stoichiometry
+    int32_t compact;  // R <= kCompactReactions and every nf <= kCompactFactors
 };
 
 // One realization (grid) of a launch.  The plain solver is a batch of one.

@@ -362,3 +362,63 @@ def test_predict_busy_cells(gpu, oracle, predict, budget, below):
     want, tw = oracle.run(m, st, 120, 9)
     assert t == tw
     assert np.array_equal(s.synchronizeStateBufferFromDevice(), want)
+
+
+def _law_model(kind, side=32):
+    """Reaction tables on either side of the compact form (at most 4 reactions of at most
+    3 factors): five reactions, a fourth-order law, int and double rate literals mixed,
+    and a single reaction (three unused compact slots)."""
+    m = configs.DiffusionModel(side, side, physical_length=side, p0=0.15, solver=configs.STOCHASTIC_HOMOGENEOUS)
+    a, b, c = m.addSpecies("A", 1.0), m.addSpecies("B", 0.5), m.addSpecies("C", 0.1)
+    if kind == "five":
+        m.addReaction(1.0, products={a: 1})
+        m.addReaction(1.0, products={b: 1})
+        m.addReaction(0.01, reactants={a: 1, b: 1}, products={c: 1})
+        m.addReaction(0.1, reactants={c: 1})
+        m.addReaction(2e-4, reactants={a: 2, c: 1}, products={b: 1})
+    elif kind == "order4":
+        m.addReaction(20, products={a: 1})
+        m.addReaction(1e-6, reactants={a: 4}, products={b: 1})
+        m.addReaction(0.5, reactants={b: 1}, products={c: 2})
+    elif kind == "mixed":
+        m.addReaction(5, products={a: 1})
+        m.addReaction(2, reactants={a: 1})
+        m.addReaction(1e-3, reactants={a: 2, b: 1}, products={c: 1})
+        m.addReaction(3, reactants={c: 1}, products={b: 1})
+    else:
+        m.addReaction(0.3, reactants={a: 1}, products={b: 1})
+    return m
+
+
+@pytest.mark.parametrize("kind", ["five", "order4", "mixed", "single"])
+@pytest.mark.parametrize("budget", [None, (2, 65)])
+def test_ssa_law_forms(gpu, oracle, kind, budget):
+    """The general table walk (more than 4 reactions, or a law of more than 3 factors) and
+    the compact straight-line form give the oracle's counts, in the one-lane SSA and in
+    the one-wave-per-cell continuation of busy cells (budget 2: nearly every cell)."""
+    m = _law_model(kind)
+    st = configs.plain_gmp_state(32, seed=11)
+    s = _solver(m)
+    if budget:
+        s.setEventBudget(*budget)
+    s.synchronizeStateBufferToDevice(st)
+    s.doReactions(0.2, 41)
+    got = s.synchronizeStateBufferFromDevice()
+    want = oracle.ssa(st, oracle.table_from_model(m), 41, 0.2)
+    assert np.array_equal(got, want)
+    assert not np.array_equal(got, st)
+
+
+def test_run_schnakenberg_five_reactions(gpu, oracle):
+    """The split inhomogeneous step (SSA launch, X sweeps with busy cells, Y sweeps) with a
+    table outside the compact form."""
+    m = configs.schnakenberg_model(64)
+    m.addReaction(0.05, reactants={m.species[1]: 1}, id="V_out")
+    st = configs.schnakenberg_state(64, seed=4)
+    s = _solver(m)
+    s.synchronizeStateBufferToDevice(st)
+    t_gpu = s.run(60, 9)
+    got = s.synchronizeStateBufferFromDevice()
+    want, t_cpu = oracle.run(m, st, 60, 9)
+    assert t_gpu == t_cpu
+    assert np.array_equal(got, want)
