@@ -105,6 +105,37 @@ GPGMP_HD double log_d(double y) {
 
 GPGMP_HD float log_f(float y) { return (float)log_d((double)y); }
 
+// log_f for y >= 1 or y = +inf (the SSA's log(1/r1), r1 in [0, 1]) without branches: the
+// same operation sequence as log_d on a normal y (no subnormal scaling is needed at
+// y >= 1), the reduction m > sqrt(2) as a select, and +inf selected at the end -- the
+// same bits as log_f on that domain, with nothing to diverge on.
+GPGMP_HD float log_f_ge1(float yf) {
+    const double y = (double)yf;
+    uint64_t b;
+    __builtin_memcpy(&b, &y, 8);
+    int e = (int)((b >> 52) & 0x7ff) - 1023;
+    b = (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+    double m;
+    __builtin_memcpy(&m, &b, 8);
+    const bool big = m > 1.4142135623730951;
+    m = big ? m * 0.5 : m;
+    e = big ? e + 1 : e;
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double p = 1.0 / 15.0;
+    p = 1.0 / 13.0 + z * p;
+    p = 1.0 / 11.0 + z * p;
+    p = 1.0 / 9.0 + z * p;
+    p = 1.0 / 7.0 + z * p;
+    p = 1.0 / 5.0 + z * p;
+    p = 1.0 / 3.0 + z * p;
+    p = z * p;
+    const double two_s = 2.0 * s;
+    const float r = (float)((double)e * 0.6931471805599453094 + (two_s + two_s * p));
+    return (y == __builtin_inf()) ? __builtin_inff() : r;
+}
+
 // exp(x) in double: x = k ln2 + r with |r| <= ln2/2 (ln2 split in two for an exact
 // k*ln2_hi), Taylor series of exp(r) to r^14 (truncation below 1e-17), scaled by 2^k.
 // IEEE add/mul/div only, so CPU and GPU builds give the same bits.  |x| <= 700.

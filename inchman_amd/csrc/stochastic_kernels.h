@@ -156,19 +156,27 @@ __device__ __forceinline__ bool ssa_cell(int32_t (&x)[S], uint32_t cell, int32_t
                                          const RxTable &T, uint32_t budget, int below, float &time,
                                          uint32_t &ctr) {
     const uint32_t k1 = kKeyBase + (uint32_t)seed;
+    if (!(time < deltat)) return true;
+    // The draws of an event depend on its counter only, so they are formed one event
+    // ahead: the next event's Threefry and log chain overlaps this event's hazards and
+    // selection (one extra draw per call, never used).
+    u32x2 w = threefry2x32_20(ctr + 1u, kCtrHi, cell, k1);
+    float L = log_f_ge1(1.f / u01(w.x));  // log(1/r1)
+    double U2 = u01_d(w.y);
     while (time < deltat) {
+        w = threefry2x32_20(ctr + 2u, kCtrHi, cell, k1);
         float h[kMaxReactions + 1];
         hazards<S>(T, x, h);
         if (h[0] == 0.f) return true;
         if (ctr == kMaxEventsPerCall) return true;  // runaway guard (never reached by sane models)
         if (ctr >= budget && __popcll(__ballot(1)) < below) return false;
         ++ctr;
-        const u32x2 w = threefry2x32_20(ctr, kCtrHi, cell, k1);
-        const float r1 = u01(w.x);
-        const float r2 = (float)(u01_d(w.y) * (double)h[0]);
-        const float dt = log_f(1.f / r1) / h[0];
+        const float r2 = (float)(U2 * (double)h[0]);
+        const float dt = L / h[0];
         time += dt;
         if (time < deltat) fire<S>(T, h, r2, x);
+        L = log_f_ge1(1.f / u01(w.x));
+        U2 = u01_d(w.y);
     }
     return true;
 }
