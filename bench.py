@@ -406,9 +406,9 @@ def main():
     assert (g_ssa, g_sweep) == (1, 0), "names below assume the default lanes per cell (1, 0)"
     names = {0: "k_gmp_step<2, 0, 1, 0> (homogeneous)",
              1: "k_gmp_step<2, 1, 1, 0> (gather of Y outflow + SSA)",
-             2: "k_gmp_step<2, 2, 4, 0> (gather of X outflow + Y sweeps)",
+             2: "k_gmp_step<2, 2, 8, 0> (gather of X outflow + Y sweeps)",
              3: "k_gmp_hot<2, 1> (deferred busy cells; not launched on this shape)",
-             4: "k_gmp_xs_hot<2> (X sweeps + deferred busy cells)"}
+             4: "k_gmp_xs_hot<2, 8> (X sweeps + deferred busy cells)"}
     out = {
         "metric": METRIC,
         "value": value,

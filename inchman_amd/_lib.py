@@ -121,6 +121,7 @@ SIGNATURES = {
     "gpgmp_solver_run": [_P, _I, _U, _pF],
     "gpgmp_solver_set_output": [_P, _F, _F, _I],
     "gpgmp_solver_set_sweep_spread": [_P, _I],
+    "gpgmp_solver_set_sweep_wave_cells": [_P, _I],
     "gpgmp_solver_set_predict_busy": [_P, _I],
     "gpgmp_solver_get_output": [_P, _P, _P, _pI, _pI, C.POINTER(C.c_int64)],
     "gpgmp_solver_run_continue": [_P, _I, _I, _pF],

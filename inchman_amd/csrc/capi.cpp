@@ -427,6 +427,14 @@ int gpgmp_solver_set_sweep_spread(gpgmp_solver *s, int32_t on) {
     });
 }
 
+int gpgmp_solver_set_sweep_wave_cells(gpgmp_solver *s, int32_t cells) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s->eng->stream()));
+        s->eng->set_sweep_wave_cells(cells);
+    });
+}
+
 int gpgmp_solver_set_launch(gpgmp_solver *s, int32_t block, int32_t cpt) {
     return guarded([&] {
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");

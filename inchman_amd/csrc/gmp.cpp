@@ -282,9 +282,9 @@ GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vecto
 }
 
 void GmpEngine::build_tiles() {
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < 3; ++k) {
         std::vector<Tile> tiles;
-        const int per = block_ / (k ? 4 : 1) * cpt_;
+        const int per = block_ / (k == 0 ? 1 : (k == 1 ? 4 : 8)) * cpt_;
         for (size_t r = 0; r < reals_.size(); ++r) {
             const int A = reals_[r].W * reals_[r].H;
             for (int c = 0; c < A; c += per) tiles.push_back(Tile{(int32_t)r, (int32_t)c});
@@ -354,15 +354,16 @@ void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cu
     a.lost_out = lost_out;
     a.prob = inh_ ? prob_[mode == 2 ? 1 : 0].get() : nullptr;
     a.reals = reals_dev_.get();
-    if (mode == 2 || mode == 3) {  // no SSA: 16 cells per wave unless one lane per cell is asked for
-        a.grp = (grp_sweep_ == 1) ? 1 : 4;
+    if (mode == 2 || mode == 3) {  // no SSA: 16 (or 8) cells per wave unless one lane per cell is asked for
+        a.grp = (grp_sweep_ == 1) ? 1 : ((grp_sweep_ == 0 && sweep_wave_cells_ == 8) ? 8 : 4);
         a.grp_sweep = (grp_sweep_ == 0) ? 0 : a.grp;
     } else {
         a.grp = grp_;
         a.grp_sweep = (grp_sweep_ == 0) ? 0 : (grp_ == 4 ? 4 : grp_sweep_);
     }
-    a.tiles = tiles_dev_[a.grp == 4].get();
-    a.ntiles = ntiles_[a.grp == 4];
+    const int tk = (a.grp == 8) ? 2 : (a.grp == 4 ? 1 : 0);
+    a.tiles = tiles_dev_[tk].get();
+    a.ntiles = ntiles_[tk];
     a.plan = plan;
     a.seeds_cur = seeds_cur;
     a.seeds_prev = seeds_prev;

@@ -201,6 +201,14 @@ class GmpEngine {
         spread_ = on;
         ++shape_version_;
     }
+    // cells per wave of the flattened launches without an SSA (8, the default: twice the
+    // waves of 16, which evens out the X-sweep launch beside the busy cells)
+    int sweep_wave_cells() const { return sweep_wave_cells_; }
+    void set_sweep_wave_cells(int n) {
+        GPGMP_REQUIRE(n == 8 || n == 16, GPGMP_E_INVALID, "cells per sweep wave must be 8 or 16");
+        sweep_wave_cells_ = n;
+        ++shape_version_;
+    }
 
     void set_state(int r, const int32_t *host);
     void get_state(int r, int32_t *host);
@@ -232,15 +240,16 @@ class GmpEngine {
     DevBuf<int32_t> lost_[2];
     DevBuf<int4> prob_[2];
     DevBuf<RealInfo> reals_dev_;
-    DevBuf<Tile> tiles_dev_[2];  // tiles for 1 and for 4 lanes per cell
+    DevBuf<Tile> tiles_dev_[3];  // tiles for 1, 4 and 8 lanes per cell
     DevBuf<HotEntry> hot_e_;
     DevBuf<uint32_t> hot_count_;
     DevBuf<uint8_t> hot_flag_;
     DevBuf<uint8_t> hot_pred_;
     HotList hot_{};
-    int ntiles_[2] = {0, 0};
+    int ntiles_[3] = {0, 0, 0};
     int block_ = 256, cpt_ = 1, grp_ = 1, grp_sweep_ = 1;
     bool spread_ = true;
+    int sweep_wave_cells_ = 8;
     bool predict_ = false;
     uint64_t shape_version_ = 0;
     int64_t total_cells_ = 0;

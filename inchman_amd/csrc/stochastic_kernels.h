@@ -613,15 +613,17 @@ __device__ __forceinline__ void step_tile(
     __shared__ FlatLds flat_lds;
     (void)flat_lds;
 
-    // flattened sweeps with 16 cells per wave may spread a wave's cells over the grid
-    // (stride A/16), so that waves covering dense and sparse regions even out; per-cell
-    // results do not depend on the mapping (RNG keys are per cell)
-    constexpr bool kSpread = (GS == 0 && G == 4);
-    const int stride = (kSpread && pl.interleave) ? (A >> 4) : 1;
+    // flattened sweeps (64/G cells per wave) may spread a wave's cells over the grid
+    // (stride A/(64/G), a bijection when 64/G divides A), so that waves covering dense and
+    // sparse regions even out; per-cell results do not depend on the mapping (RNG keys are
+    // per cell)
+    constexpr bool kSpread = (GS == 0 && G >= 4);
+    constexpr int NCW = 64 / G;
+    const int stride = (kSpread && pl.interleave && A % NCW == 0) ? A / NCW : 1;
     for (int k = 0; k < cpt; ++k) {
         const int lc = tile.cell0 + k * cpb + (int)threadIdx.x / G;
         if (lc >= A) break;
-        const int c = (stride > 1) ? (lc & 15) * stride + (lc >> 4) : lc;
+        const int c = (stride > 1) ? (lc % NCW) * stride + lc / NCW : lc;
         // a cell busy last step is run whole (gather + SSA) by a worker wave of this launch
         if (predict && pred_prev[ri.cell_off + c]) {
             if (g == 0) hot.flag[ri.cell_off + c] = 0;
@@ -670,7 +672,8 @@ __device__ __forceinline__ void step_tile(
         //    An X-sweep launch that runs beside the hot cells skips the flagged ones.
         bool deferred = false;
         if (MODE == MODE_INH_XS && hot.flag) deferred = hot.flag[ri.cell_off + c] != 0;
-        if (mode_ssa(MODE) && pl.deltat > 0.f && T.R > 0) {
+        if constexpr (mode_ssa(MODE)) {
+          if (pl.deltat > 0.f && T.R > 0) {
             float time = 0.f;
             uint32_t ctr = 0;
             const bool fin =
@@ -694,6 +697,7 @@ __device__ __forceinline__ void step_tile(
                     }
                 }
             }
+          }
         }
 
         if (mode_ssa(MODE) && hot.flag && g == 0) hot.flag[ri.cell_off + c] = deferred ? 1 : 0;
@@ -1077,7 +1081,7 @@ __global__ __launch_bounds__(64) void k_gmp_hot(int32_t *__restrict__ state, int
 // per wave, flattened), which skip the cells flagged by the SSA launch.  The two kinds
 // touch disjoint cells.  nhot_blocks is a multiple of 8, so the tiles keep their XCD
 // placement.
-template <int S>
+template <int S, int G>
 __global__ __launch_bounds__(256) void k_gmp_xs_hot(
     int32_t *__restrict__ state, const int32_t *__restrict__ lost_in, int32_t *__restrict__ lost_out,
     const int4 *__restrict__ prob, const RealInfo *__restrict__ reals,
@@ -1095,7 +1099,7 @@ __global__ __launch_bounds__(256) void k_gmp_xs_hot(
     }
     const int t = xcd_tile((int)blockIdx.x - nhot_blocks, ntiles);
     if (t >= ntiles) return;
-    step_tile<S, MODE_INH_XS, 4, 0>(t, state, lost_in, lost_out, prob, reals, tiles, plan, seeds_cur, seeds_prev,
+    step_tile<S, MODE_INH_XS, G, 0>(t, state, lost_in, lost_out, prob, reals, tiles, plan, seeds_cur, seeds_prev,
                                     T, P, bm, cpt, hot);
 }
 
@@ -1156,21 +1160,26 @@ static hipError_t launch_sg(const StepLaunch &a) {
     return hipGetLastError();
 }
 
-template <int S>
-static hipError_t launch_xs_hot(const StepLaunch &a) {
+template <int S, int G>
+static hipError_t launch_xs_hot_g(const StepLaunch &a) {
     const int wpb = a.block / 64;
     const int nhot = 8 * ((a.hot_grid + 8 * wpb - 1) / (8 * wpb));
     const int grid = nhot + 8 * ((a.ntiles + 7) / 8);
     if (a.ev_start) {
-        hipExtLaunchKernelGGL((k_gmp_xs_hot<S>), dim3(grid), dim3(a.block), 0, a.stream, a.ev_start, a.ev_stop, 0,
+        hipExtLaunchKernelGGL((k_gmp_xs_hot<S, G>), dim3(grid), dim3(a.block), 0, a.stream, a.ev_start, a.ev_stop, 0,
                               a.state, a.lost_in, a.lost_out, a.prob, a.reals, a.tiles, a.ntiles, a.plan,
                               a.plan_ssa, a.seeds_cur, a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot, nhot);
     } else {
-        hipLaunchKernelGGL((k_gmp_xs_hot<S>), dim3(grid), dim3(a.block), 0, a.stream, a.state, a.lost_in,
+        hipLaunchKernelGGL((k_gmp_xs_hot<S, G>), dim3(grid), dim3(a.block), 0, a.stream, a.state, a.lost_in,
                            a.lost_out, a.prob, a.reals, a.tiles, a.ntiles, a.plan, a.plan_ssa, a.seeds_cur,
                            a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot, nhot);
     }
     return hipGetLastError();
+}
+
+template <int S>
+static hipError_t launch_xs_hot(const StepLaunch &a) {
+    return (a.grp == 8) ? launch_xs_hot_g<S, 8>(a) : launch_xs_hot_g<S, 4>(a);
 }
 
 template <int S, int MODE>
@@ -1178,7 +1187,7 @@ static hipError_t launch_s(const StepLaunch &a) {
     hipError_t e;
     if constexpr (MODE == MODE_INH_XS) {
         if (a.hot.flag) {
-            if (a.grp != 4 || a.grp_sweep != 0 || !a.plan_ssa) return hipErrorInvalidValue;
+            if ((a.grp != 4 && a.grp != 8) || a.grp_sweep != 0 || !a.plan_ssa) return hipErrorInvalidValue;
             return launch_xs_hot<S>(a);
         }
     }
@@ -1187,6 +1196,10 @@ static hipError_t launch_s(const StepLaunch &a) {
             if (!a.hot.flag || a.grp != 1 || a.grp_sweep != 0 || !a.hot.pred) return hipErrorInvalidValue;
             return launch_ssa_pred<S>(a);
         }
+    }
+    if (a.grp == 8) {  // flattened sweeps over 8 cells per wave: launches without an SSA
+        if constexpr (mode_ssa(MODE)) return hipErrorInvalidValue;
+        else return (a.grp_sweep == 0) ? launch_sg<S, MODE, 8, 0>(a) : hipErrorInvalidValue;
     }
     if (a.grp != 1 && a.grp != 4) return hipErrorInvalidValue;
     if (a.grp_sweep == 0) e = (a.grp == 4) ? launch_sg<S, MODE, 4, 0>(a) : launch_sg<S, MODE, 1, 0>(a);

@@ -282,9 +282,14 @@ class Solver:
         return counts[: ns.value], times[: ns.value], nt.value, it.value
 
     def setPredictBusy(self, on=True):
-        """Split steps: run last step's busy cells in worker waves of the SSA launch (default)
-        or defer them to the X-sweep launch as they turn up; results are identical."""
+        """Split steps: run last step's busy cells in worker waves of the SSA launch, or
+        defer them to the X-sweep launch as they turn up (default); results are identical."""
         call("gpgmp_solver_set_predict_busy", self.handle, int(bool(on)))
+
+    def setSweepWaveCells(self, cells=8):
+        """Flattened X/Y sweep launches: 8 (default) or 16 cells per wave; results are
+        identical."""
+        call("gpgmp_solver_set_sweep_wave_cells", self.handle, int(cells))
 
     def setSweepSpread(self, on=True):
         """Flattened sweeps: spread each wave's 16 cells over the grid (default) or keep them

@@ -163,11 +163,16 @@ int gpgmp_solver_run_continue(gpgmp_solver *s, int32_t steps, int32_t graph_chun
  * grid (stride W*H/16) so that waves over dense and sparse regions even out; off keeps 16
  * consecutive cells.  Results are identical either way (no reference counterpart). */
 int gpgmp_solver_set_sweep_spread(gpgmp_solver *s, int32_t on);
-/* Split steps (one lane per cell, flattened sweeps): on (default) runs the cells that were
- * busy in the previous step (more SSA events than the budget) whole in worker waves of the
- * SSA launch, beside the cheap cells, instead of deferring them to the X-sweep launch.
- * Results are identical either way (no reference counterpart). */
+/* Split steps (one lane per cell, flattened sweeps): on runs the cells that were busy in
+ * the previous step (more SSA events than the budget) whole in worker waves of the SSA
+ * launch, beside the cheap cells; off (default, faster on the 256x256 Schnakenberg run)
+ * defers them to the X-sweep launch as they turn up.  Results are identical either way
+ * (no reference counterpart). */
 int gpgmp_solver_set_predict_busy(gpgmp_solver *s, int32_t on);
+/* Flattened launches without an SSA (X and Y sweeps): 8 (default) or 16 cells per wave.
+ * 8 gives twice the waves (256x256 Schnakenberg: X sweeps + busy cells 24.2 -> 23.3 us).
+ * Results are identical either way (no reference counterpart). */
+int gpgmp_solver_set_sweep_wave_cells(gpgmp_solver *s, int32_t cells);
 int gpgmp_solver_set_launch(gpgmp_solver *s, int32_t block, int32_t cells_per_thread);
 /* Lanes per cell (1 or 4, default 1) of the launches with an SSA, and the sweep
  * scheme (default 0): 1 or 4 lanes per cell, or 0 = the particle loops of a wave's cells

@@ -422,3 +422,20 @@ def test_run_schnakenberg_five_reactions(gpu, oracle):
     want, t_cpu = oracle.run(m, st, 60, 9)
     assert t_gpu == t_cpu
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("W,H", [(48, 32), (64, 48)])
+@pytest.mark.parametrize("cells", [16, 8])
+def test_sweep_wave_cells(gpu, oracle, W, H, cells):
+    """Flattened X/Y sweeps over 16 or 8 cells per wave (spread over the grid); the host
+    loop matches the oracle either way."""
+    m = configs.schnakenberg_model(W, H)
+    st = configs.schnakenberg_state(W, H, seed=6)
+    s = _solver(m)
+    s.setSweepWaveCells(cells)
+    s.synchronizeStateBufferToDevice(st)
+    t_gpu = s.run(120, 13)
+    got = s.synchronizeStateBufferFromDevice()
+    want, t_cpu = oracle.run(m, st, 120, 13)
+    assert t_gpu == t_cpu
+    assert np.array_equal(got, want)
