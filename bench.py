@@ -73,7 +73,9 @@ def plain_run(solver, st, seed0, steps, timing):
     solver.kernelTimes(on=timing)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    solver.runContinue(steps, 0)
+    # HIP-graph chunks of `timing` steps: each chunk's timing window runs eagerly with its
+    # events, the rest of the chunk replays a graph
+    solver.runContinue(steps, timing)
     solver.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0

@@ -406,6 +406,7 @@ struct GmpRunner::Chunk {
     hipEvent_t copied = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
+    int graph_from = 0;            // first step of the chunk in the graph (eager timed prefix)
     uint64_t graph_version = 0;  // engine shape version the graph was captured with
     Chunk(int cap_, int nreal_) : cap(cap_), nreal(nreal_) {
         GPGMP_HIP_CHECK(hipHostMalloc((void **)&hplan, sizeof(LaunchPlan) * (3 * cap + 1)));
@@ -513,8 +514,8 @@ void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
     GPGMP_HIP_CHECK(hipMemcpyAsync(c.dseed.get(), c.hseed, sizeof(int32_t) * (size_t)(nsteps + 1) * nreal,
                                    hipMemcpyHostToDevice, eng_.stream()));
     GPGMP_HIP_CHECK(hipEventRecord(c.copied, eng_.stream()));
-    auto body = [&]() {
-        for (int j = 0; j < nsteps; ++j) {
+    auto body = [&](int j0, int j1) {
+        for (int j = j0; j < j1; ++j) {
             const int32_t *scur = c.dseed.get() + (size_t)(j + 1) * nreal;
             const int32_t *sprev = c.dseed.get() + (size_t)j * nreal;
             // timing windows: of every every_ steps the first kTimingWindow carry events,
@@ -546,9 +547,33 @@ void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
         }
     };
     if (!use_graph) {
-        body();
+        body(0, nsteps);
         return;
     }
+    // With timing on, the chunk's event-carrying steps run eagerly when they form a prefix
+    // of the chunk (a chunk length that is a multiple of the timing period keeps them
+    // there) and the rest of the chunk replays a graph; otherwise the chunk runs eagerly.
+    int k = 0;
+    if (timing_) {
+        auto timed = [&](int64_t sn) { return every_ == 1 || (sn % every_) < kTimingWindow; };
+        while (k < nsteps && timed(step_no_ + k)) ++k;
+        for (int j = k; j < nsteps; ++j) {
+            if (timed(step_no_ + j)) {
+                body(0, nsteps);
+                return;
+            }
+        }
+    }
+    body(0, k);
+    if (k == nsteps) return;
+    // a replay issues no host code, so the step counter of the timing windows is advanced
+    // here for the graph's steps
+    const int64_t sn_graph = step_no_;
+    struct StepRestore {
+        int64_t &n;
+        int64_t v;
+        ~StepRestore() { n = v; }
+    } step_restore{step_no_, sn_graph + (nsteps - k)};
     const bool saved_timing = timing_;
     timing_ = false;  // events are not captured into graphs
     struct TimingRestore {
@@ -556,7 +581,7 @@ void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
         bool v;
         ~TimingRestore() { t = v; }
     } timing_restore{timing_, saved_timing};
-    if (!c.graph || c.graph_steps != nsteps || c.graph_version != eng_.shape_version()) {
+    if (!c.graph || c.graph_steps != nsteps || c.graph_from != k || c.graph_version != eng_.shape_version()) {
         if (c.graph) {
             GPGMP_HIP_CHECK(hipGraphExecDestroy(c.graph));
             c.graph = nullptr;
@@ -574,7 +599,7 @@ void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
         hipGraph_t g = nullptr;
         GPGMP_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
         try {
-            body();
+            body(k, nsteps);
         } catch (...) {
             (void)hipStreamEndCapture(cap, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -588,6 +613,7 @@ void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
         (void)hipStreamDestroy(cap);
         GPGMP_HIP_CHECK(ie);
         c.graph_steps = nsteps;
+        c.graph_from = k;
         c.graph_version = eng_.shape_version();
     }
     GPGMP_HIP_CHECK(hipGraphLaunch(c.graph, eng_.stream()));

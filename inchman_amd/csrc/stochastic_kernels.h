@@ -151,22 +151,29 @@ __device__ __forceinline__ void fire(const RxTable &T, const float (&h)[kMaxReac
 // than `below` lanes of the wave still run (a wave of busy cells keeps going; a lone
 // busy cell stops holding its wave): returns false when it stops early (time < deltat),
 // so the cell can be resumed bit-exactly.
-template <int S>
-__device__ __forceinline__ bool ssa_cell(int32_t (&x)[S], uint32_t cell, int32_t seed, float deltat,
-                                         const RxTable &T, uint32_t budget, int below, float &time,
-                                         uint32_t &ctr) {
+template <int S, bool COMPACT>
+__device__ __forceinline__ bool ssa_cell_t(int32_t (&x)[S], uint32_t cell, int32_t seed, float deltat,
+                                           const RxTable &T, uint32_t budget, int below, float &time,
+                                           uint32_t &ctr) {
     const uint32_t k1 = kKeyBase + (uint32_t)seed;
     if (!(time < deltat)) return true;
     // The draws of an event depend on its counter only, so they are formed one event
-    // ahead: the next event's Threefry and log chain overlaps this event's hazards and
-    // selection (one extra draw per call, never used).
+    // ahead (one extra draw per call, never used).  With a compact table the next event's
+    // Threefry and log chain and this event's hazards are one branch-free block; the
+    // empty asm below needs the next draws before the loop's exits, so the compiler keeps
+    // them in that block (instead of sinking them past the exits) and interleaves the two
+    // independent chains.
     u32x2 w = threefry2x32_20(ctr + 1u, kCtrHi, cell, k1);
     float L = log_f_ge1(1.f / u01(w.x));  // log(1/r1)
     double U2 = u01_d(w.y);
     while (time < deltat) {
         w = threefry2x32_20(ctr + 2u, kCtrHi, cell, k1);
+        const float Ln = log_f_ge1(1.f / u01(w.x));
+        const double U2n = u01_d(w.y);
         float h[kMaxReactions + 1];
-        hazards<S>(T, x, h);
+        if constexpr (COMPACT) hazards_compact<S>(T, x, h);
+        else hazards<S>(T, x, h);
+        asm volatile("" ::"v"(Ln), "v"(U2n));
         if (h[0] == 0.f) return true;
         if (ctr == kMaxEventsPerCall) return true;  // runaway guard (never reached by sane models)
         if (ctr >= budget && __popcll(__ballot(1)) < below) return false;
@@ -175,10 +182,18 @@ __device__ __forceinline__ bool ssa_cell(int32_t (&x)[S], uint32_t cell, int32_t
         const float dt = L / h[0];
         time += dt;
         if (time < deltat) fire<S>(T, h, r2, x);
-        L = log_f_ge1(1.f / u01(w.x));
-        U2 = u01_d(w.y);
+        L = Ln;
+        U2 = U2n;
     }
     return true;
+}
+
+template <int S>
+__device__ __forceinline__ bool ssa_cell(int32_t (&x)[S], uint32_t cell, int32_t seed, float deltat,
+                                         const RxTable &T, uint32_t budget, int below, float &time,
+                                         uint32_t &ctr) {
+    return T.compact ? ssa_cell_t<S, true>(x, cell, seed, deltat, T, budget, below, time, ctr)
+                     : ssa_cell_t<S, false>(x, cell, seed, deltat, T, budget, below, time, ctr);
 }
 
 // Lane groups.  With G > 1, G consecutive lanes of a wave (G divides 64) share one

@@ -127,6 +127,28 @@ def test_run_continue_graph(gpu):
     assert np.array_equal(b.synchronizeStateBufferFromDevice(), ref)
 
 
+def test_graph_chunks_with_timing_windows(gpu):
+    """Timing windows (4 counted steps of every 32) inside graph chunks of 32: the windows
+    run eagerly at each chunk's start and the rest replays a graph; counts are the eager
+    run's and every window is timed."""
+    m = configs.schnakenberg_model(64)
+    st = configs.schnakenberg_state(64)
+    a = _solver(m)
+    a.synchronizeStateBufferToDevice(st)
+    a.run(200, 4)
+    ref = a.synchronizeStateBufferFromDevice()
+    b = _solver(m)
+    b.synchronizeStateBufferToDevice(st)
+    b.run(0, 4)
+    b.kernelTimes(32)
+    b.runContinue(200, graph_chunk=32)
+    ms, n, _ = b.kernelTimes(0)
+    assert np.array_equal(b.synchronizeStateBufferFromDevice(), ref)
+    # 200 steps = 6 chunks of 32 + 8 eager steps; each window of 5 counts 4 steps
+    assert n[1] == n[2] == n[4] == 7 * 4
+    assert (ms[[1, 2, 4]] > 0).all()
+
+
 def test_ensemble_matches_plain(gpu):
     m = configs.plain_gmp_model(32)
     sides = np.array([16, 32, 48, 64, 80, 128, 96], np.int32)
