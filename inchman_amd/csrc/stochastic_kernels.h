@@ -652,24 +652,29 @@ __device__ __forceinline__ void step_tile(
         for (int s = 0; s < S; ++s)
             x[s] = (pl.deltat > 0.f || ((touched >> s) & 1u)) ? st[(size_t)s * A + c] : 0;
 
-        // 1. gather the previous launch's outflow
+        // 1. gather the previous launch's outflow.  Every neighbour word is loaded from a
+        //    clamped (in-grid) address and the edge lanes add 0 instead, so the loads of all
+        //    species go out together rather than one divergent branch and wait each; the
+        //    integer sum is the same in any order.
         if (pl.gather_mask) {
+            const int cxm = (gx > 0) ? c - 1 : c, cxp = (gx < W - 1) ? c + 1 : c;
+            const int cym = (gy > 0) ? c - W : c, cyp = (gy < H - 1) ? c + W : c;
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 if (!((pl.gather_mask >> s) & 1u)) continue;
                 const int32_t *l = lin + (size_t)s * ND * A;
                 int32_t v = x[s];
                 if (MODE == MODE_HOM) {
-                    if (gx > 0) v += l[1 * (size_t)A + c - 1];
-                    if (gx < W - 1) v += l[0 * (size_t)A + c + 1];
-                    if (gy > 0) v += l[3 * (size_t)A + c - W];
-                    if (gy < H - 1) v += l[2 * (size_t)A + c + W];
+                    const int32_t a0 = l[1 * (size_t)A + cxm], a1 = l[0 * (size_t)A + cxp];
+                    const int32_t a2 = l[3 * (size_t)A + cym], a3 = l[2 * (size_t)A + cyp];
+                    v += ((gx > 0) ? a0 : 0) + ((gx < W - 1) ? a1 : 0) + ((gy > 0) ? a2 : 0) +
+                         ((gy < H - 1) ? a3 : 0);
                 } else if (mode_x(MODE)) {  // gathers the Y outflow of the previous launch
-                    if (gy > 0) v += l[1 * (size_t)A + c - W];
-                    if (gy < H - 1) v += l[0 * (size_t)A + c + W];
+                    const int32_t a2 = l[1 * (size_t)A + cym], a3 = l[0 * (size_t)A + cyp];
+                    v += ((gy > 0) ? a2 : 0) + ((gy < H - 1) ? a3 : 0);
                 } else {                          // gathers the X outflow of the previous launch
-                    if (gx > 0) v += l[1 * (size_t)A + c - 1];
-                    if (gx < W - 1) v += l[0 * (size_t)A + c + 1];
+                    const int32_t a0 = l[1 * (size_t)A + cxm], a1 = l[0 * (size_t)A + cxp];
+                    v += ((gx > 0) ? a0 : 0) + ((gx < W - 1) ? a1 : 0);
                 }
                 if (MODE == MODE_HOM && any_src) {
                     const uint32_t k1s = kKeyBase + (uint32_t)seeds_prev[tile.r] + (uint32_t)pl.gather_off;
