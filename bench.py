@@ -214,6 +214,50 @@ def det_diffusion_extra(side=8192, reps=20):
     return out
 
 
+def rk4_moments_extra(side=1024, dt=0.01, reps=20, moment_side=8192):
+    """RK4 reactions (deterministic_performReaction_rk4) of the Schnakenberg network in
+    concentration form on side x side cells over dt; the X moments of one species
+    (inhomogeneous_computeMomentsX + reduceMomentsBlocks) of an int32 2-species state of
+    moment_side x moment_side cells (4 B read per cell of the species)."""
+    import numpy as np
+    import torch
+    from inchman_amd import configs
+    from inchman_amd._lib import call
+
+    m = configs.schnakenberg_model(side)
+    cm = m.compile()
+    rng = np.random.default_rng(3)
+    st = np.stack([rng.uniform(50, 250, (side, side)), rng.uniform(20, 120, (side, side))]).astype(np.float32)
+    x = torch.from_numpy(st).cuda()
+    sh = int(torch.cuda.current_stream().cuda_stream)
+    call("gpgmp_det_rk4_device", cm.handle, x.data_ptr(), dt, sh)  # warmup
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call("gpgmp_det_rk4_device", cm.handle, x.data_ptr(), dt, sh)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    out = {"rk4": {"workload": "RK4 reactions, Schnakenberg (concentration form), %dx%d cells, dt %g"
+                   % (side, side, dt), "ms_per_call": t * 1e3 / reps, "cells_per_s": side * side * reps / t,
+                   "algorithmic_GBps": 16.0 * side * side * reps / t / 1e9}}
+    W = H = moment_side
+    state = torch.randint(0, 1000, (2, H, W), dtype=torch.int32, device="cuda")
+    sums = torch.zeros(2, dtype=torch.int64, device="cuda")
+    call("gpgmp_moments_x_device", state.data_ptr(), W, H, 1, sums.data_ptr(), sh)  # warmup
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call("gpgmp_moments_x_device", state.data_ptr(), W, H, 1, sums.data_ptr(), sh)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    n = state[1].to(torch.int64)
+    ok = sums.tolist() == [int((n * torch.arange(W, device="cuda")[None, :]).sum()), int(n.sum())]
+    out["moments_x"] = {"workload": "X moments of one species, %dx%d int32 cells" % (W, H),
+                        "ms_per_call": t * 1e3 / reps, "GBps": 4.0 * W * H * reps / t / 1e9,
+                        "hbm_frac": 4.0 * W * H * reps / t / 1e9 / HBM_PEAK_GBS, "exact": ok}
+    return out
+
+
 def aqss_extra(side=1024, dt=0.1, reps=5, cpu_side=64):
     """alpha-QSS reactions (deterministic_performReaction_aqss) of the Schnakenberg network
     in concentration form on side x side cells (u ~ U(50, 250), v ~ U(20, 120)) over dt,
@@ -453,7 +497,7 @@ def main():
     if not args.no_extra:
         for key, fn in (("ensemble", ensemble_extra), ("rdd", rdd_extra), ("det_diffusion", det_diffusion_extra),
                         ("particles", particles_extra), ("individuals", individuals_extra),
-                        ("aqss", aqss_extra)):
+                        ("aqss", aqss_extra), ("rk4_moments", rk4_moments_extra)):
             extra[key] = fn()
             progress(key + " done")
     out["cpu_baseline"] = cpu_baseline(args.cpu_steps)

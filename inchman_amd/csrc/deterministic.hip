@@ -180,25 +180,55 @@ __device__ __forceinline__ void rk4_rhs(const RxTable &T, const float (&x)[S], f
     }
 }
 
+// The same right-hand side for compact tables (at most kCompactReactions reactions of at
+// most kCompactFactors factors) in straight-line code: an unused factor is 1 (law * 1 is
+// law), and a reaction at or past R leaves d unchanged (selected, not added), so the float
+// operations on d are the table walk's.
 template <int S>
-__global__ void k_det_rk4(float *__restrict__ state, int A, RxTable T, float dt) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= A) return;
+__device__ __forceinline__ void rk4_rhs_compact(const RxTable &T, const float (&x)[S], float (&d)[S]) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) d[s] = 0.f;
+#pragma unroll
+    for (int i = 0; i < kCompactReactions; ++i) {
+        float law = (float)T.lit_dbl[i];
+#pragma unroll
+        for (int f = 0; f < kCompactFactors; ++f) {
+            float v = x[0];
+#pragma unroll
+            for (int k = 1; k < S; ++k) v = (T.fs[i][f] == k) ? x[k] : v;
+            law *= (f < (int)T.nf[i]) ? v : 1.f;
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const float ds = d[s] + (float)T.net[i][s] * law;
+            d[s] = (i < T.R) ? ds : d[s];
+        }
+    }
+}
+
+template <int S, bool COMPACT>
+__device__ __forceinline__ void rk4_rhs_t(const RxTable &T, const float (&x)[S], float (&d)[S]) {
+    if constexpr (COMPACT) rk4_rhs_compact<S>(T, x, d);
+    else rk4_rhs<S>(T, x, d);
+}
+
+template <int S, bool COMPACT>
+__device__ __forceinline__ void rk4_cell(float *__restrict__ state, int A, int c, const RxTable &T, float dt) {
     float x[S], k1[S], k2[S], k3[S], k4[S], tmp[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) x[s] = state[(size_t)s * A + c];
     const float h = dt / 20.f;
     for (int n = 0; n < 20; ++n) {
-        rk4_rhs<S>(T, x, k1);
+        rk4_rhs_t<S, COMPACT>(T, x, k1);
 #pragma unroll
         for (int s = 0; s < S; ++s) tmp[s] = x[s] + h / 2.f * k1[s];
-        rk4_rhs<S>(T, tmp, k2);
+        rk4_rhs_t<S, COMPACT>(T, tmp, k2);
 #pragma unroll
         for (int s = 0; s < S; ++s) tmp[s] = x[s] + h / 2.f * k2[s];
-        rk4_rhs<S>(T, tmp, k3);
+        rk4_rhs_t<S, COMPACT>(T, tmp, k3);
 #pragma unroll
         for (int s = 0; s < S; ++s) tmp[s] = x[s] + h * k3[s];
-        rk4_rhs<S>(T, tmp, k4);
+        rk4_rhs_t<S, COMPACT>(T, tmp, k4);
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             x[s] += h / 6.f * (k1[s] + 2.f * k2[s] + 2.f * k3[s] + k4[s]);
@@ -207,6 +237,14 @@ __global__ void k_det_rk4(float *__restrict__ state, int A, RxTable T, float dt)
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) state[(size_t)s * A + c] = x[s];
+}
+
+template <int S>
+__global__ void k_det_rk4(float *__restrict__ state, int A, RxTable T, float dt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= A) return;
+    if (T.compact) rk4_cell<S, true>(state, A, c, T, dt);
+    else rk4_cell<S, false>(state, A, c, T, dt);
 }
 
 // deterministic_performReaction_aqss (Deterministic.cl:291-522): Mott & Oran's alpha-QSS

@@ -58,8 +58,14 @@ def test_det_inhom_sweep(gpu, oracle, axis, H, W):
     assert np.array_equal(got[0], st[0])
 
 
-def test_det_rk4(gpu, oracle):
+@pytest.mark.parametrize("table", ["compact", "general"])
+def test_det_rk4(gpu, oracle, table):
+    """RK4 reactions against the fp64 oracle: the plain GMP network (4 reactions, the
+    straight-line right-hand side) and the same with a fifth reaction (the table walk)."""
     m = configs.plain_gmp_model(32)
+    if table == "general":
+        a, b, c = m.species
+        m.addReaction(2e-4, reactants={a: 2, c: 1}, products={b: 1})
     cm = m.compile()
     rng = np.random.default_rng(4)
     st = rng.uniform(0, 60, (3, 32, 32)).astype(np.float32)
