@@ -88,8 +88,12 @@ def test_det_aqss(gpu, oracle, fixed):
     from inchman_amd._lib import AQSS_PADE_FIX
 
     rng = np.random.default_rng(40 + fixed)
+    five = configs.plain_gmp_model(32)  # a fifth reaction: the table walk, not the compact form
+    a, b, c = five.species
+    five.addReaction(2e-4, reactants={a: 2, c: 1}, products={b: 1})
     for m, scale, dts in ((configs.plain_gmp_model(32), 60.0, (0.05, 0.5, 2.0)),
-                          (configs.schnakenberg_model(32), 250.0, (0.01, 0.1, 1.0))):
+                          (configs.schnakenberg_model(32), 250.0, (0.01, 0.1, 1.0)),
+                          (five, 60.0, (0.05, 0.5))):
         S = m.numSpecies()
         st = rng.uniform(0, scale, (S, 32, 32)).astype(np.float32)
         st[:, ::7, ::5] = 0.0
