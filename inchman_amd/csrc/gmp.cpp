@@ -364,6 +364,10 @@ void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cu
     const int tk = (a.grp == 8) ? 2 : (a.grp == 4 ? 1 : 0);
     a.tiles = tiles_dev_[tk].get();
     a.ntiles = ntiles_[tk];
+    if (reals_.size() == 1) {  // the tiles of build_tiles(), computed in the kernel
+        a.one.r = reals_[0];
+        a.one.per = block_ / (tk == 0 ? 1 : (tk == 1 ? 4 : 8)) * cpt_;
+    }
     a.plan = plan;
     a.seeds_cur = seeds_cur;
     a.seeds_prev = seeds_prev;

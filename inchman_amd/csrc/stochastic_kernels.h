@@ -602,9 +602,17 @@ __device__ __forceinline__ void step_tile(
     const int4 *__restrict__ prob, const RealInfo *__restrict__ reals,
     const Tile *__restrict__ tiles, const LaunchPlan *__restrict__ plan,
     const int32_t *__restrict__ seeds_cur, const int32_t *__restrict__ seeds_prev, const RxTable &T,
-    const HomProbs &P, int4 bm, int cpt, const HotList &hot_all) {
-    const Tile tile = tiles[t];
-    const RealInfo ri = reals[tile.r];
+    const HomProbs &P, int4 bm, int cpt, const HotList &hot_all, const OneReal &one) {
+    Tile tile;
+    RealInfo ri;
+    if (one.per > 0) {
+        tile.r = 0;
+        tile.cell0 = t * one.per;
+        ri = one.r;
+    } else {
+        tile = tiles[t];
+        ri = reals[tile.r];
+    }
     const LaunchPlan pl = *plan;
     if (MODE == MODE_INH_Y && pl.reset_hot && t == 0 && threadIdx.x == 0) {
         const HotList v = hot_view(hot_all, pl.hot_parity);
@@ -778,12 +786,12 @@ __global__ __launch_bounds__(256) void k_gmp_step(
     const int4 *__restrict__ prob, const RealInfo *__restrict__ reals,
     const Tile *__restrict__ tiles, int ntiles, const LaunchPlan *__restrict__ plan,
     const int32_t *__restrict__ seeds_cur, const int32_t *__restrict__ seeds_prev, RxTable T,
-    HomProbs P, int4 bm, int cpt, HotList hot) {
+    HomProbs P, int4 bm, int cpt, HotList hot, OneReal one) {
     // the grid has 8*ceil(ntiles/8) blocks so that the remap is a bijection
     const int t = xcd_tile((int)blockIdx.x, ntiles);
     if (t >= ntiles) return;
     step_tile<S, MODE, G, GS>(t, state, lost_in, lost_out, prob, reals, tiles, plan, seeds_cur, seeds_prev, T, P,
-                              bm, cpt, hot);
+                              bm, cpt, hot, one);
 }
 
 __device__ __forceinline__ float readlane_f(float v, int j) {
@@ -1108,7 +1116,7 @@ __global__ __launch_bounds__(256) void k_gmp_xs_hot(
     const Tile *__restrict__ tiles, int ntiles, const LaunchPlan *__restrict__ plan,
     const LaunchPlan *__restrict__ plan_ssa, const int32_t *__restrict__ seeds_cur,
     const int32_t *__restrict__ seeds_prev, RxTable T, HomProbs P, int4 bm, int cpt, HotList hot,
-    int nhot_blocks) {
+    int nhot_blocks, OneReal one) {
     if ((int)blockIdx.x < nhot_blocks) {
         const int wpb = (int)(blockDim.x >> 6);
         const LaunchPlan pss = *plan_ssa, pl = *plan;
@@ -1120,7 +1128,7 @@ __global__ __launch_bounds__(256) void k_gmp_xs_hot(
     const int t = xcd_tile((int)blockIdx.x - nhot_blocks, ntiles);
     if (t >= ntiles) return;
     step_tile<S, MODE_INH_XS, G, 0>(t, state, lost_in, lost_out, prob, reals, tiles, plan, seeds_cur, seeds_prev,
-                                    T, P, bm, cpt, hot);
+                                    T, P, bm, cpt, hot, one);
 }
 
 
@@ -1144,7 +1152,7 @@ __global__ __launch_bounds__(256) void k_gmp_ssa_pred(
     const int t = xcd_tile((int)blockIdx.x - npred_blocks, ntiles);
     if (t >= ntiles) return;
     step_tile<S, MODE_INH_X, 1, 0>(t, state, lost_in, lost_out, prob, reals, tiles, plan, seeds_cur, seeds_prev, T,
-                                   P, bm, cpt, hot);
+                                   P, bm, cpt, hot, OneReal{});
 }
 
 template <int S>
@@ -1171,11 +1179,11 @@ static hipError_t launch_sg(const StepLaunch &a) {
         // start/stop timestamps taken by the dispatch packet itself
         hipExtLaunchKernelGGL((k_gmp_step<S, MODE, G, GS>), dim3(grid), dim3(a.block), 0, a.stream, a.ev_start,
                               a.ev_stop, 0, a.state, a.lost_in, a.lost_out, a.prob, a.reals, a.tiles,
-                              a.ntiles, a.plan, a.seeds_cur, a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot);
+                              a.ntiles, a.plan, a.seeds_cur, a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot, a.one);
     } else {
         hipLaunchKernelGGL((k_gmp_step<S, MODE, G, GS>), dim3(grid), dim3(a.block), 0, a.stream, a.state,
                            a.lost_in, a.lost_out, a.prob, a.reals, a.tiles, a.ntiles, a.plan, a.seeds_cur,
-                           a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot);
+                           a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot, a.one);
     }
     return hipGetLastError();
 }
@@ -1188,11 +1196,12 @@ static hipError_t launch_xs_hot_g(const StepLaunch &a) {
     if (a.ev_start) {
         hipExtLaunchKernelGGL((k_gmp_xs_hot<S, G>), dim3(grid), dim3(a.block), 0, a.stream, a.ev_start, a.ev_stop, 0,
                               a.state, a.lost_in, a.lost_out, a.prob, a.reals, a.tiles, a.ntiles, a.plan,
-                              a.plan_ssa, a.seeds_cur, a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot, nhot);
+                              a.plan_ssa, a.seeds_cur, a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot, nhot,
+                              a.one);
     } else {
         hipLaunchKernelGGL((k_gmp_xs_hot<S, G>), dim3(grid), dim3(a.block), 0, a.stream, a.state, a.lost_in,
                            a.lost_out, a.prob, a.reals, a.tiles, a.ntiles, a.plan, a.plan_ssa, a.seeds_cur,
-                           a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot, nhot);
+                           a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot, nhot, a.one);
     }
     return hipGetLastError();
 }

@@ -37,6 +37,7 @@ struct StepLaunch {
     int predict = 0;        // SSA launch of a split step: run last step's busy cells in worker waves
     hipEvent_t ev_hot_start = nullptr;  // optional: timestamps of the hot-cell dispatch
     hipEvent_t ev_hot_stop = nullptr;
+    OneReal one{};          // one.per > 0: a single realization, tiles of one.per cells computed
 };
 
 hipError_t launch_gmp_step(const StepLaunch &a);

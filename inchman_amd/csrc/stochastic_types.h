@@ -65,6 +65,14 @@ struct Tile {
     int32_t cell0;
 };
 
+// A launch over a single realization (per > 0): tile t is cells [t*per, (t+1)*per) of
+// realization 0, computed instead of loaded, and the realization's info comes by value, so
+// a wave starts its state loads without two dependent loads (tile, then realization).
+struct OneReal {
+    RealInfo r;
+    int32_t per;
+};
+
 // A cell whose SSA used up its per-lane event budget in k_gmp_step: the hot-cell
 // kernel continues it from (time, ctr) and then computes its outflow.
 struct HotEntry {
