@@ -183,7 +183,7 @@ class GmpEngine {
     int grp() const { return grp_; }
     int grp_sweep() const { return grp_sweep_; }
     // inhomogeneous with flattened sweeps: the SSA and the X sweeps in separate launches
-    // (the sweep then runs with 16 cells per wave, 4 waves per SIMD)
+    // (the sweeps then run with 8 or 16 cells per wave, sweep_wave_cells())
     bool split_x() const { return inh_ && grp_sweep_ == 0; }
     // split steps continue the deferred cells inside the X-sweep launch
     bool fused_hot() const { return split_x(); }
@@ -195,7 +195,7 @@ class GmpEngine {
         ++shape_version_;
     }
     void reset_hot();
-    // flattened sweeps spread each wave's 16 cells over the grid (stride A/16)
+    // flattened sweeps spread each wave's n cells over the grid (stride A/n)
     bool spread() const { return spread_; }
     void set_spread(bool on) {
         spread_ = on;

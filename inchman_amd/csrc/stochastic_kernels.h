@@ -1105,7 +1105,7 @@ __global__ __launch_bounds__(64) void k_gmp_hot(int32_t *__restrict__ state, int
 // X sweeps and the deferred busy cells in one launch, so the cells' serial event chains
 // run beside the sweeps instead of before them.  Blocks [0, nhot_blocks) are hot-cell
 // workers (one per wave; they continue the SSA under the SSA launch's plan `plan_ssa`,
-// then compute their cell's X outflow); the other blocks are the X-sweep tiles (16 cells
+// then compute their cell's X outflow); the other blocks are the X-sweep tiles (64/G cells
 // per wave, flattened), which skip the cells flagged by the SSA launch.  The two kinds
 // touch disjoint cells.  nhot_blocks is a multiple of 8, so the tiles keep their XCD
 // placement.

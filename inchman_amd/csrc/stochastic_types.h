@@ -51,8 +51,8 @@ struct alignas(16) LaunchPlan {
     uint32_t out_mask;      // species whose outflow is computed in this launch
     int32_t gather_off;     // source-boundary seed offset, relative to the previous step's seed
     int32_t out_off[kMaxSpecies];
-    int32_t interleave;     // flattened sweeps, 16 cells per wave: wave p's slot q is cell
-                            // q*(A/16) + p (spread over the grid) instead of 16p + q
+    int32_t interleave;     // flattened sweeps, n = 64/G cells per wave: wave p's slot q is
+                            // cell q*(A/n) + p (spread over the grid) instead of n*p + q
     int32_t predict;        // split steps: the SSA launch runs last step's busy cells in
                             // worker waves (hot lists alternate by step parity)
     int32_t hot_parity;     // this step's hot list (0 or 1) when predicting

@@ -159,8 +159,8 @@ int gpgmp_solver_get_output(gpgmp_solver *s, int32_t *counts, float *times, int3
  * graph_chunk > 0 replays chunks of that many iterations as a HIP graph. */
 int gpgmp_solver_run_continue(gpgmp_solver *s, int32_t steps, int32_t graph_chunk, float *sim_time_out);
 /* launch shape of the solver's kernels: threads per block (64..256), cells per thread */
-/* Flattened sweeps (16 cells per wave): on (default) spreads each wave's cells over the
- * grid (stride W*H/16) so that waves over dense and sparse regions even out; off keeps 16
+/* Flattened sweeps (n = 8 or 16 cells per wave): on (default) spreads each wave's cells over
+ * the grid (stride W*H/n) so that waves over dense and sparse regions even out; off keeps n
  * consecutive cells.  Results are identical either way (no reference counterpart). */
 int gpgmp_solver_set_sweep_spread(gpgmp_solver *s, int32_t on);
 /* Split steps (one lane per cell, flattened sweeps): on runs the cells that were busy in
