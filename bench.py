@@ -419,8 +419,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the solver on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        print("bench.py measures one GPU: the solver runs on a single MI355X "
+              "(multi-GPU sharding is not built)", file=sys.stderr)
         sys.exit(2)
 
     import torch
