@@ -51,13 +51,13 @@ TIMING_EVERY = 128  # time 4 of every 128 steps: events on every launch add ~22%
 # sweeps, 4 X sweeps + hot cells), from rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs
 # over the first 2000 iterations, scaled by a plain device copy's readings (x2.0 / x1.0);
 # commands and raw counters in profiles/gmp_step_schnakenberg256.md
-TRAFFIC_BYTES = {1: 1638328, 2: 1657398, 4: 1283427}
+TRAFFIC_BYTES = {1: 1634513, 2: 1635917, 4: 1275432}
 
 # wave64 VALU instructions per dispatch of the same kernels (rocprofv3 SQ_INSTS_VALU summed
 # over the chip, averaged over launches 2..10001 of a full 10^4-iteration run; command in
 # profiles/gmp_step_schnakenberg256.md) and MI355X's VALU issue peak: 256 CUs x 4 SIMDs,
 # one wave64 32-bit VALU instruction per SIMD every 4 cycles (16 lanes), 2.4 GHz
-VALU_INSTS = {1: 1.69648e6, 2: 9.01716e6, 4: 9.97543e6}
+VALU_INSTS = {1: 1.69444e6, 2: 8.53646e6, 4: 9.49486e6}
 VALU_PEAK = 256 * 4 * 2.4e9 / 4
 
 

@@ -526,31 +526,19 @@ __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, in
     wave_lds_sync();
     const uint32_t lo = (uint32_t)(((uint64_t)lane * total) >> 6);
     const uint32_t hi = (uint32_t)(((uint64_t)(lane + 1) * total) >> 6);
+    // one walker per lane over its slice (with 8 cells per wave, 8 waves per SIMD keep
+    // enough Threefry chains in flight; a second interleaved walker per lane measured slower)
     if (lo < hi) {
-        // two walkers over the two halves of the slice: their Threefry chains are
-        // independent, so one wave keeps two in flight
-        const uint32_t half = (hi - lo + 1) >> 1, midf = lo + half;
-        FlatWalker<MODE> wa, wb;
+        FlatWalker<MODE> wa;
         wa.stride = stride;
-        wb.stride = stride;
         wa.start(lo, st, L, wave, NC, base, koff);
-        const bool has_b = midf < hi;
-        if (has_b) wb.start(midf, st, L, wave, NC, base, koff);
-        else wb = wa;
-        for (uint32_t i = 0; i < half; ++i) {
-            const uint32_t fa = lo + i, fb = midf + i;
-            const bool vb = fb < hi;
-            if (fa == wa.end) wa.advance(fa, st, L, wave, base, koff);
-            if (vb && fb == wb.end) wb.advance(fb, st, L, wave, base, koff);
+        for (uint32_t f = lo; f < hi; ++f) {
+            if (f == wa.end) wa.advance(f, st, L, wave, base, koff);
             const u32x2 ra = threefry2x32_20(wa.k + 1u, kCtrHi, wa.k0, k1);
-            const u32x2 rb = threefry2x32_20(wb.k + 1u, kCtrHi, wb.k0, k1);
             wa.count(ra, true, P);
-            wb.count(rb, vb, P);
             ++wa.k;
-            wb.k += vb ? 1u : 0u;
         }
         wa.flush(L, wave);
-        if (has_b) wb.flush(L, wave);
     }
     wave_lds_sync();
     if (own && !deferred) {
