@@ -125,9 +125,11 @@ SIGNATURES = {
     "gpgmp_solver_set_predict_busy": [_P, _I],
     "gpgmp_solver_get_output": [_P, _P, _P, _pI, _pI, C.POINTER(C.c_int64)],
     "gpgmp_solver_run_continue": [_P, _I, _I, _pF],
+    "gpgmp_solver_prepare_run": [_P, _I, _I],
     "gpgmp_solver_set_launch": [_P, _I, _I],
     "gpgmp_solver_kernel_times": [_P, _I, _pD, C.POINTER(C.c_int64), _pD],
     "gpgmp_solver_set_event_budget": [_P, _U, _I],
+    "gpgmp_solver_set_hot_workers": [_P, _F, _I, _I],
     "gpgmp_solver_set_lanes_per_cell": [_P, _I, _I],
     "gpgmp_solver_get_launch": [_P, _pI, _pI, _pI, _pI],
     "gpgmp_batch_set_lanes_per_cell": [_P, _I, _I],
@@ -141,6 +143,7 @@ SIGNATURES = {
     "gpgmp_batch_set_launch": [_P, _I, _I],
     "gpgmp_batch_autotune": [_P, _I, _pI, _pI],
     "gpgmp_batch_run": [_P, _I, _I],
+    "gpgmp_batch_prepare_run": [_P, _I, _I],
     "gpgmp_batch_reset": [_P],
     "gpgmp_batch_kernel_times": [_P, _I, _pD, C.POINTER(C.c_int64), _pD],
     "gpgmp_batch_total_cells": [_P, C.POINTER(C.c_int64)],

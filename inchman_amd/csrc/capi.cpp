@@ -381,6 +381,14 @@ int gpgmp_solver_run_continue(gpgmp_solver *s, int32_t steps, int32_t graph_chun
     });
 }
 
+int gpgmp_solver_prepare_run(gpgmp_solver *s, int32_t steps, int32_t graph_chunk) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(s->runner != nullptr, GPGMP_E_STATE, "no run to continue");
+        s->runner->prepare(steps, graph_chunk);
+    });
+}
+
 int gpgmp_solver_set_output(gpgmp_solver *s, float interval, float max_time, int32_t max_dumps) {
     return guarded([&] {
         GPGMP_REQUIRE(s && max_dumps >= 0, GPGMP_E_INVALID, "bad argument");
@@ -465,6 +473,14 @@ int gpgmp_solver_set_event_budget(gpgmp_solver *s, uint32_t budget, int32_t belo
                       "bad argument");
         GPGMP_HIP_CHECK(hipStreamSynchronize(s->stream.s));
         s->eng->set_hot_budget(budget, below_active);
+    });
+}
+
+int gpgmp_solver_set_hot_workers(gpgmp_solver *s, float lane_events, int32_t lane_waves, int32_t wave_waves) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s->stream.s));
+        s->eng->set_hot_workers(lane_events, lane_waves, wave_waves);
     });
 }
 
@@ -573,6 +589,14 @@ int gpgmp_batch_run(gpgmp_batch *b, int32_t steps, int32_t graph_chunk) {
         GPGMP_REQUIRE(b, GPGMP_E_INVALID, "null argument");
         if (!b->runner) b->runner.reset(new GmpRunner(*b->eng, b->dtx, b->dty, b->seed0s));
         b->runner->run(steps, graph_chunk);
+    });
+}
+
+int gpgmp_batch_prepare_run(gpgmp_batch *b, int32_t steps, int32_t graph_chunk) {
+    return guarded([&] {
+        GPGMP_REQUIRE(b, GPGMP_E_INVALID, "null argument");
+        if (!b->runner) b->runner.reset(new GmpRunner(*b->eng, b->dtx, b->dty, b->seed0s));
+        b->runner->prepare(steps, graph_chunk);
     });
 }
 

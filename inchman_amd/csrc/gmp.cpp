@@ -256,7 +256,8 @@ GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vecto
     hot_.budget = 8;
     hot_.below_active = 24;
     hot_e_.alloc(2 * (size_t)hot_.cap);  // two lists when busy cells are predicted
-    hot_count_.alloc(4);
+    hot_lane_.alloc((size_t)hot_.cap);
+    hot_count_.alloc(8);
     hot_flag_.alloc((size_t)total_cells_);
     hot_pred_.alloc(2 * (size_t)total_cells_);
     hot_.e = hot_e_.get();
@@ -264,6 +265,7 @@ GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vecto
     hot_.flag = nullptr;
     hot_.pred = hot_pred_.get();
     hot_.pred_stride = (uint32_t)total_cells_;
+    hot_.lane_count = hot_count_.get() + 4;  // the lane list's count (entries only when fused)
     GPGMP_HIP_CHECK(hipMemsetAsync(hot_flag_.get(), 0, (size_t)total_cells_, stream_));
     reset_hot();
     reals_dev_.alloc(reals_.size());
@@ -380,12 +382,19 @@ void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cu
     a.ev_start = ev_start;
     a.ev_stop = ev_stop;
     a.hot = hot_;
-    a.hot_grid = (int)std::min<uint32_t>(hot_.cap, 1024u);
+    a.hot_grid = (int)std::min<uint32_t>(hot_.cap, (uint32_t)hot_waves_);
     if (fuse) {
         GPGMP_REQUIRE(fused_hot() && (mode == 1 || mode == 3), GPGMP_E_INVALID, "hot cells fuse into split steps only");
+        GPGMP_REQUIRE(num_real() == 1, GPGMP_E_INVALID, "hot cells fuse into single-realization launches only");
         // the SSA launch flags its deferred cells; the X-sweep launch (plan = SSA plan + 1)
         // continues them beside the sweeps of the others
         a.hot.flag = hot_flag_.get();
+        if (!predict() && lane_events_ > 0.f) {  // modest tails to the lane list
+            a.hot.lane_e = hot_lane_.get();
+            a.hot.lane_cap = hot_.cap;
+            a.hot.lane_events = lane_events_;
+            a.lane_grid = lane_waves_;
+        }
         if (mode == 3) a.plan_ssa = plan - 1;
         if (mode == 1) a.predict = predict() ? 1 : 0;
     }
@@ -396,7 +405,7 @@ void GmpEngine::launch(int mode, const LaunchPlan *plan, const int32_t *seeds_cu
 
 // empty hot lists and no predicted cells (a run starts from here)
 void GmpEngine::reset_hot() {
-    GPGMP_HIP_CHECK(hipMemsetAsync(hot_count_.get(), 0, 4 * sizeof(uint32_t), stream_));
+    GPGMP_HIP_CHECK(hipMemsetAsync(hot_count_.get(), 0, 8 * sizeof(uint32_t), stream_));
     GPGMP_HIP_CHECK(hipMemsetAsync(hot_pred_.get(), 0, 2 * (size_t)total_cells_, stream_));
 }
 
@@ -408,10 +417,20 @@ struct GmpRunner::Chunk {
     DevBuf<LaunchPlan> dplan;
     DevBuf<int32_t> dseed;
     hipEvent_t copied = nullptr;
-    hipGraphExec_t graph = nullptr;
-    int graph_steps = 0;
-    int graph_from = 0;            // first step of the chunk in the graph (eager timed prefix)
-    uint64_t graph_version = 0;  // engine shape version the graph was captured with
+    // Captured graphs of this chunk's buffers.  A graph bakes in the launches of steps
+    // [from, steps) of the chunk and, for homogeneous models, the outflow buffers those
+    // steps read and write, which alternate with the step's parity: an output event can
+    // end a chunk after an odd number of steps, so a graph is only reused at the parity
+    // it was captured at.
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        int steps = 0, from = 0, parity = 0;
+        uint64_t version = 0;  // engine shape version the graph was captured with
+        uint64_t used = 0;     // last use (eviction order)
+    };
+    static constexpr size_t kMaxGraphs = 8;
+    std::vector<Graph> graphs;
+    uint64_t uses = 0;
     Chunk(int cap_, int nreal_) : cap(cap_), nreal(nreal_) {
         GPGMP_HIP_CHECK(hipHostMalloc((void **)&hplan, sizeof(LaunchPlan) * (3 * cap + 1)));
         GPGMP_HIP_CHECK(hipHostMalloc((void **)&hseed, sizeof(int32_t) * (size_t)(cap + 1) * nreal));
@@ -419,8 +438,26 @@ struct GmpRunner::Chunk {
         dseed.alloc((size_t)(cap + 1) * nreal);
         GPGMP_HIP_CHECK(hipEventCreateWithFlags(&copied, hipEventDisableTiming));
     }
+    Graph *find(int steps, int from, int parity, uint64_t version) {
+        for (Graph &g : graphs)
+            if (g.steps == steps && g.from == from && g.parity == parity && g.version == version) {
+                g.used = ++uses;
+                return &g;
+            }
+        return nullptr;
+    }
+    void insert(const Graph &g) {
+        if (graphs.size() == kMaxGraphs) {
+            auto lru = std::min_element(graphs.begin(), graphs.end(),
+                                        [](const Graph &a, const Graph &b) { return a.used < b.used; });
+            (void)hipGraphExecDestroy(lru->exec);
+            graphs.erase(lru);
+        }
+        graphs.push_back(g);
+        graphs.back().used = ++uses;
+    }
     ~Chunk() {
-        if (graph) (void)hipGraphExecDestroy(graph);
+        for (Graph &g : graphs) (void)hipGraphExecDestroy(g.exec);
         if (copied) {
             (void)hipEventSynchronize(copied);
             (void)hipEventDestroy(copied);
@@ -510,14 +547,18 @@ void GmpRunner::collect_times() {
 }
 
 // Issues the launches of one chunk whose plans/seeds are already in its host buffers.
-void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
+// dry: issues nothing; only captures (and caches) the graph a real issue of this chunk
+// would replay.
+void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph, bool dry) {
     const int nreal = eng_.num_real();
     const int lps = eng_.inh_ ? (eng_.split_x() ? 3 : 2) : 1;  // launches per step
-    GPGMP_HIP_CHECK(hipMemcpyAsync(c.dplan.get(), c.hplan, sizeof(LaunchPlan) * lps * nsteps,
-                                   hipMemcpyHostToDevice, eng_.stream()));
-    GPGMP_HIP_CHECK(hipMemcpyAsync(c.dseed.get(), c.hseed, sizeof(int32_t) * (size_t)(nsteps + 1) * nreal,
-                                   hipMemcpyHostToDevice, eng_.stream()));
-    GPGMP_HIP_CHECK(hipEventRecord(c.copied, eng_.stream()));
+    if (!dry) {
+        GPGMP_HIP_CHECK(hipMemcpyAsync(c.dplan.get(), c.hplan, sizeof(LaunchPlan) * lps * nsteps,
+                                       hipMemcpyHostToDevice, eng_.stream()));
+        GPGMP_HIP_CHECK(hipMemcpyAsync(c.dseed.get(), c.hseed, sizeof(int32_t) * (size_t)(nsteps + 1) * nreal,
+                                       hipMemcpyHostToDevice, eng_.stream()));
+        GPGMP_HIP_CHECK(hipEventRecord(c.copied, eng_.stream()));
+    }
     auto body = [&](int j0, int j1) {
         for (int j = j0; j < j1; ++j) {
             const int32_t *scur = c.dseed.get() + (size_t)(j + 1) * nreal;
@@ -550,26 +591,24 @@ void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
             }
         }
     };
-    if (!use_graph) {
-        body(0, nsteps);
-        return;
-    }
     // With timing on, the chunk's event-carrying steps run eagerly when they form a prefix
     // of the chunk (a chunk length that is a multiple of the timing period keeps them
     // there) and the rest of the chunk replays a graph; otherwise the chunk runs eagerly.
+    // A graph of fewer than kMinGraphSteps steps is not worth its capture.
     int k = 0;
-    if (timing_) {
+    if (use_graph && timing_) {
         auto timed = [&](int64_t sn) { return every_ == 1 || (sn % every_) < kTimingWindow; };
         while (k < nsteps && timed(step_no_ + k)) ++k;
-        for (int j = k; j < nsteps; ++j) {
-            if (timed(step_no_ + j)) {
-                body(0, nsteps);
-                return;
-            }
-        }
+        for (int j = k; j < nsteps; ++j)
+            if (timed(step_no_ + j)) use_graph = false;
     }
-    body(0, k);
-    if (k == nsteps) return;
+    if (!use_graph || nsteps - k < kMinGraphSteps) {
+        if (dry) step_no_ += nsteps;
+        else body(0, nsteps);
+        return;
+    }
+    if (dry) step_no_ += k;
+    else body(0, k);
     // a replay issues no host code, so the step counter of the timing windows is advanced
     // here for the graph's steps
     const int64_t sn_graph = step_no_;
@@ -585,11 +624,10 @@ void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
         bool v;
         ~TimingRestore() { t = v; }
     } timing_restore{timing_, saved_timing};
-    if (!c.graph || c.graph_steps != nsteps || c.graph_from != k || c.graph_version != eng_.shape_version()) {
-        if (c.graph) {
-            GPGMP_HIP_CHECK(hipGraphExecDestroy(c.graph));
-            c.graph = nullptr;
-        }
+    // homogeneous steps read and write the outflow buffers of their parity
+    const int par = eng_.inh_ ? 0 : parity_;
+    Chunk::Graph *gr = c.find(nsteps, k, par, eng_.shape_version());
+    if (!gr) {
         // capture on a private stream, then replay on the engine's stream
         hipStream_t cap;
         GPGMP_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
@@ -612,18 +650,61 @@ void GmpRunner::issue(Chunk &c, int nsteps, bool use_graph) {
         }
         GPGMP_HIP_CHECK(hipStreamEndCapture(cap, &g));
         eng_.set_stream(saved);
-        const hipError_t ie = hipGraphInstantiate(&c.graph, g, nullptr, nullptr, 0);
+        Chunk::Graph ng;
+        const hipError_t ie = hipGraphInstantiate(&ng.exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         (void)hipStreamDestroy(cap);
         GPGMP_HIP_CHECK(ie);
-        c.graph_steps = nsteps;
-        c.graph_from = k;
-        c.graph_version = eng_.shape_version();
+        ng.steps = nsteps;
+        ng.from = k;
+        ng.parity = par;
+        ng.version = eng_.shape_version();
+        c.insert(ng);
+        gr = &c.graphs.back();
     }
-    GPGMP_HIP_CHECK(hipGraphLaunch(c.graph, eng_.stream()));
+    if (!dry) GPGMP_HIP_CHECK(hipGraphLaunch(gr->exec, eng_.stream()));
 }
 
-void GmpRunner::run(int steps, int graph_chunk) {
+// Captures the graphs that run(steps, graph_chunk) will replay, without running anything:
+// the host loop is planned on a copy of the runner's state, which is restored afterwards.
+// A run of a few steps then replays its chunk instead of capturing it inside the run.
+void GmpRunner::prepare(int steps, int graph_chunk) {
+    GPGMP_REQUIRE(steps >= 0, GPGMP_E_INVALID, "steps must be non-negative");
+    if (graph_chunk <= 0 || steps == 0) return;
+    const Schedule sched = sched_;
+    const std::vector<SeedStream> seeds = seeds_;
+    const StepDecision prev = prev_;
+    const bool has_prev = has_prev_;
+    const std::vector<int32_t> prev_seed = prev_seed_;
+    const int parity = parity_;
+    const int64_t iterations = iterations_, plan_step = plan_step_, step_no = step_no_;
+    struct Restore {
+        GmpRunner &r;
+        const Schedule &sc;
+        const std::vector<SeedStream> &se;
+        const StepDecision &pv;
+        bool hp;
+        const std::vector<int32_t> &ps;
+        int pa;
+        int64_t it, pl, sn;
+        ~Restore() {
+            r.sched_ = sc;
+            r.seeds_ = se;
+            r.prev_ = pv;
+            r.has_prev_ = hp;
+            r.prev_seed_ = ps;
+            r.parity_ = pa;
+            r.iterations_ = it;
+            r.plan_step_ = pl;
+            r.step_no_ = sn;
+        }
+    } restore{*this, sched, seeds, prev, has_prev, prev_seed, parity, iterations, plan_step, step_no};
+    run_impl(steps, graph_chunk, true);
+}
+
+void GmpRunner::run(int steps, int graph_chunk) { run_impl(steps, graph_chunk, false); }
+
+void GmpRunner::run_impl(int steps, int graph_chunk, bool dry) {
     GPGMP_REQUIRE(steps >= 0, GPGMP_E_INVALID, "steps must be non-negative");
     const int nreal = eng_.num_real();
     const bool use_graph = graph_chunk > 0;
@@ -707,13 +788,14 @@ void GmpRunner::run(int steps, int graph_chunk) {
             }
         }
         for (int r = 0; r < nreal; ++r) prev_seed_[r] = c.hseed[(size_t)n * nreal + r];
-        issue(c, n, use_graph && n == cap);
+        issue(c, n, use_graph, dry);
         if (!eng_.inh_) parity_ = (parity_ + n) & 1;
         done += n;
         iterations_ += n;
         ci ^= 1;
-        if (dump) dump_(dump_time);
+        if (dump && !dry) dump_(dump_time);
     }
+    if (dry) return;
     // final gather of the last launch's outflow (and source boundaries)
     if (has_prev_) {
         Chunk &c = *chunks_[ci];

@@ -195,6 +195,22 @@ class GmpEngine {
         ++shape_version_;
     }
     void reset_hot();
+    // Workers of the deferred cells in the X-sweep launch of a split step: a cell whose
+    // expected remaining events are at most lane_events is resumed by one lane (lane_waves
+    // worker waves, 64 cells each per round), the others by one wave each (wave_waves
+    // worker waves); lane_events <= 0 sends every cell to the wave workers.
+    void set_hot_workers(float lane_events, int lane_waves, int wave_waves) {
+        GPGMP_REQUIRE(lane_waves >= 8 && wave_waves >= 8 && lane_waves <= 65536 && wave_waves <= 65536,
+                      GPGMP_E_INVALID, "worker waves must be in [8, 65536]");
+        GPGMP_REQUIRE(!(lane_events != lane_events), GPGMP_E_INVALID, "lane_events is NaN");
+        lane_events_ = lane_events;
+        lane_waves_ = lane_waves;
+        hot_waves_ = wave_waves;
+        ++shape_version_;
+    }
+    float lane_events() const { return lane_events_; }
+    int lane_waves() const { return lane_waves_; }
+    int hot_waves() const { return hot_waves_; }
     // flattened sweeps spread each wave's n cells over the grid (stride A/n)
     bool spread() const { return spread_; }
     void set_spread(bool on) {
@@ -242,6 +258,7 @@ class GmpEngine {
     DevBuf<RealInfo> reals_dev_;
     DevBuf<Tile> tiles_dev_[3];  // tiles for 1, 4 and 8 lanes per cell
     DevBuf<HotEntry> hot_e_;
+    DevBuf<HotEntry> hot_lane_;
     DevBuf<uint32_t> hot_count_;
     DevBuf<uint8_t> hot_flag_;
     DevBuf<uint8_t> hot_pred_;
@@ -251,6 +268,9 @@ class GmpEngine {
     bool spread_ = true;
     int sweep_wave_cells_ = 8;
     bool predict_ = false;
+    float lane_events_ = 16.f;
+    int lane_waves_ = 1024;
+    int hot_waves_ = 1024;
     uint64_t shape_version_ = 0;
     int64_t total_cells_ = 0;
     hipStream_t stream_;
@@ -265,6 +285,9 @@ class GmpRunner {
               const std::vector<uint32_t> &seed0s);
     ~GmpRunner();
     void run(int steps, int graph_chunk);
+    // Captures the HIP graphs run(steps, graph_chunk) would replay from the current state of
+    // the loop, without running anything (the loop's state is unchanged).
+    void prepare(int steps, int graph_chunk);
     float sim_time() const { return sched_.sim_time(); }
     // Output events and the max_time stop of the reference loop; `dump` is called, after
     // the launches of each output iteration that writes (the state is then fully gathered),
@@ -297,7 +320,9 @@ class GmpRunner {
 
   private:
     struct Chunk;
-    void issue(Chunk &c, int nlaunch_steps, bool use_graph);
+    void issue(Chunk &c, int nlaunch_steps, bool use_graph, bool dry);
+    void run_impl(int steps, int graph_chunk, bool dry);
+    static constexpr int kMinGraphSteps = 2;
     GmpEngine &eng_;
     Schedule sched_;
     std::vector<SeedStream> seeds_;

@@ -150,11 +150,12 @@ __device__ __forceinline__ void fire(const RxTable &T, const float (&h)[kMaxReac
 // Runs from (time, ctr) until deltat, or until `budget` events have been drawn and fewer
 // than `below` lanes of the wave still run (a wave of busy cells keeps going; a lone
 // busy cell stops holding its wave): returns false when it stops early (time < deltat),
-// so the cell can be resumed bit-exactly.
+// so the cell can be resumed bit-exactly; `rate` is then the total hazard h0 of the state
+// it stopped in (h0 * (deltat - time) estimates the events it still has to draw).
 template <int S, bool COMPACT>
 __device__ __forceinline__ bool ssa_cell_t(int32_t (&x)[S], uint32_t cell, int32_t seed, float deltat,
                                            const RxTable &T, uint32_t budget, int below, float &time,
-                                           uint32_t &ctr) {
+                                           uint32_t &ctr, float &rate) {
     const uint32_t k1 = kKeyBase + (uint32_t)seed;
     if (!(time < deltat)) return true;
     // The draws of an event depend on its counter only, so they are formed one event
@@ -176,7 +177,10 @@ __device__ __forceinline__ bool ssa_cell_t(int32_t (&x)[S], uint32_t cell, int32
         asm volatile("" ::"v"(Ln), "v"(U2n));
         if (h[0] == 0.f) return true;
         if (ctr == kMaxEventsPerCall) return true;  // runaway guard (never reached by sane models)
-        if (ctr >= budget && __popcll(__ballot(1)) < below) return false;
+        if (ctr >= budget && __popcll(__ballot(1)) < below) {
+            rate = h[0];
+            return false;
+        }
         ++ctr;
         const float r2 = (float)(U2 * (double)h[0]);
         const float dt = L / h[0];
@@ -191,9 +195,9 @@ __device__ __forceinline__ bool ssa_cell_t(int32_t (&x)[S], uint32_t cell, int32
 template <int S>
 __device__ __forceinline__ bool ssa_cell(int32_t (&x)[S], uint32_t cell, int32_t seed, float deltat,
                                          const RxTable &T, uint32_t budget, int below, float &time,
-                                         uint32_t &ctr) {
-    return T.compact ? ssa_cell_t<S, true>(x, cell, seed, deltat, T, budget, below, time, ctr)
-                     : ssa_cell_t<S, false>(x, cell, seed, deltat, T, budget, below, time, ctr);
+                                         uint32_t &ctr, float &rate) {
+    return T.compact ? ssa_cell_t<S, true>(x, cell, seed, deltat, T, budget, below, time, ctr, rate)
+                     : ssa_cell_t<S, false>(x, cell, seed, deltat, T, budget, below, time, ctr, rate);
 }
 
 // Lane groups.  With G > 1, G consecutive lanes of a wave (G divides 64) share one
@@ -235,9 +239,9 @@ __device__ __forceinline__ int32_t grp_sum(int32_t v) {
 template <int S, int G>
 __device__ __forceinline__ bool ssa_cell_g(int32_t (&x)[S], uint32_t cell, int32_t seed, float deltat,
                                            const RxTable &T, uint32_t budget, int below, float &time,
-                                           uint32_t &ctr) {
+                                           uint32_t &ctr, float &rate) {
     if constexpr (G == 1) {
-        return ssa_cell<S>(x, cell, seed, deltat, T, budget, below, time, ctr);
+        return ssa_cell<S>(x, cell, seed, deltat, T, budget, below, time, ctr, rate);
     } else {
         const uint32_t k1 = kKeyBase + (uint32_t)seed;
         const uint32_t g = threadIdx.x & (G - 1);
@@ -253,7 +257,10 @@ __device__ __forceinline__ bool ssa_cell_g(int32_t (&x)[S], uint32_t cell, int32
                 hazards<S>(T, x, h);
                 if (h[0] == 0.f) return true;
                 if (ctr == kMaxEventsPerCall) return true;  // runaway guard
-                if (ctr >= budget && __popcll(__ballot(1)) < below) return false;
+                if (ctr >= budget && __popcll(__ballot(1)) < below) {
+                    rate = h[0];
+                    return false;
+                }
                 ++ctr;
                 const float r2 = (float)(Us[j] * (double)h[0]);
                 const float dt = Ls[j] / h[0];
@@ -412,6 +419,7 @@ __device__ __forceinline__ int32_t cell_outflow(int32_t n, int s, int c, int W, 
 struct FlatLds {
     int32_t start[4][65];       // exclusive scan of the calls per cell, [NC] = total
     int32_t n[4][64];           // particles per cell
+    int32_t cell[4][64];        // the cell of each slot
     int2 pr[4][64];             // inhomogeneous thresholds of (min(p1,p2), p2) per cell
     uint32_t cnt[4][64][5];     // counters per cell
 };
@@ -427,22 +435,22 @@ __device__ __forceinline__ void wave_lds_sync() {
 // the cursor leaves the cell.
 template <int MODE>
 struct FlatWalker {
-    int qq, stride;
+    int qq;
     uint32_t end, k, k0;
     int32_t n;
     int2 pr;
     uint32_t c[5];
 
-    __device__ __forceinline__ void load(const FlatLds &L, int wave, int base, uint32_t koff) {
+    __device__ __forceinline__ void load(const FlatLds &L, int wave, uint32_t koff) {
         n = L.n[wave][qq];
         pr = (MODE != MODE_HOM) ? L.pr[wave][qq] : make_int2(0, 0);
-        k0 = (uint32_t)(base + qq * stride) + koff;
+        k0 = (uint32_t)L.cell[wave][qq] + koff;
 #pragma unroll
         for (int j = 0; j < 5; ++j) c[j] = 0u;
     }
     // position at flattened call f: the cell q with st[q] <= f < st[q+1]
     __device__ __forceinline__ void start(uint32_t f, const int32_t *st, const FlatLds &L, int wave, int NC,
-                                          int base, uint32_t koff) {
+                                          uint32_t koff) {
         int a = 0, b = NC;
         while (b - a > 1) {
             const int m = (a + b) >> 1;
@@ -452,7 +460,7 @@ struct FlatWalker {
         qq = a;
         end = (uint32_t)st[qq + 1];
         k = f - (uint32_t)st[qq];
-        load(L, wave, base, koff);
+        load(L, wave, koff);
     }
     __device__ __forceinline__ void flush(FlatLds &L, int wave) {
         atomicAdd(&L.cnt[wave][qq][0], c[0]);
@@ -464,14 +472,13 @@ struct FlatWalker {
         }
     }
     // f reached the end of the current cell: move to the next cell with calls
-    __device__ __forceinline__ void advance(uint32_t f, const int32_t *st, FlatLds &L, int wave, int base,
-                                            uint32_t koff) {
+    __device__ __forceinline__ void advance(uint32_t f, const int32_t *st, FlatLds &L, int wave, uint32_t koff) {
         flush(L, wave);
         ++qq;
         while ((uint32_t)st[qq + 1] == f) ++qq;
         end = (uint32_t)st[qq + 1];
         k = 0;
-        load(L, wave, base, koff);
+        load(L, wave, koff);
     }
     // particles 2k (word x) and 2k+1 (word y, if the cell has it)
     __device__ __forceinline__ void count(const u32x2 &w, bool valid, const HomProbs &P) {
@@ -492,8 +499,7 @@ struct FlatWalker {
 };
 
 template <int MODE, int LPC>
-__device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, int c, int stride, int s, int W, int H,
-                                                  int A,
+__device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, int c, int s, int W, int H, int A,
                                                   int32_t seed, const LaunchPlan &pl, int4 bm, const RealInfo &ri,
                                                   const int4 *__restrict__ prob, int32_t *__restrict__ lout,
                                                   const HomProbs &P, FlatLds &L) {
@@ -503,7 +509,6 @@ __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, in
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
     const int q = lane / LPC;
     const bool own = (lane % LPC) == 0;
-    const int base = c - q * stride;  // the cell of the wave's slot 0; slot q is base + q * stride
     const uint32_t k1 = kKeyBase + (uint32_t)seed + (uint32_t)pl.out_off[s];
     const uint32_t koff = (MODE == MODE_HOM) ? 0u : (uint32_t)(s * A);
     int32_t *st = L.start[wave];
@@ -514,6 +519,7 @@ __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, in
     if (own) {
         st[q] = incl - calls;
         L.n[wave][q] = xs;
+        L.cell[wave][q] = c;
         if (MODE != MODE_HOM) {
             const int4 pr4 = prob[ri.prob_off + (size_t)s * A + c];
             L.pr[wave][q] = make_int2(pr4.x, pr4.y);
@@ -530,10 +536,9 @@ __device__ __forceinline__ void wave_flat_outflow(int32_t &xs, bool deferred, in
     // enough Threefry chains in flight; a second interleaved walker per lane measured slower)
     if (lo < hi) {
         FlatWalker<MODE> wa;
-        wa.stride = stride;
-        wa.start(lo, st, L, wave, NC, base, koff);
+        wa.start(lo, st, L, wave, NC, koff);
         for (uint32_t f = lo; f < hi; ++f) {
-            if (f == wa.end) wa.advance(f, st, L, wave, base, koff);
+            if (f == wa.end) wa.advance(f, st, L, wave, koff);
             const u32x2 ra = threefry2x32_20(wa.k + 1u, kCtrHi, wa.k0, k1);
             wa.count(ra, true, P);
             ++wa.k;
@@ -582,6 +587,26 @@ __device__ __forceinline__ HotList hot_view(const HotList &h, int parity) {
     return v;
 }
 
+// Appends one entry per lane group whose lanes want it (the group's first lane writes
+// it) to the list (e, count, cap); returns true in the lanes whose entry found room.  Every
+// lane of the wave calls it.
+__device__ __forceinline__ bool push_hot(bool want, int g, HotEntry *e, uint32_t *count, uint32_t cap,
+                                         const HotEntry &he) {
+    const uint64_t dm = __ballot(want && g == 0);
+    if (!dm) return false;
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __ffsll((unsigned long long)dm) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(dm));
+    base = __shfl(base, leader, 64);
+    if (!want) return false;
+    const int first = lane - g;
+    const uint32_t idx = base + (uint32_t)__popcll(dm & ((1ull << first) - 1ull));
+    if (idx >= cap) return false;
+    if (g == 0) e[idx] = he;
+    return true;
+}
+
 // One tile of a step launch: G lanes per cell for the SSA; the outflow by GS lanes per
 // cell (GS = G, or 4 with G = 1), or flattened over the wave (GS = 0).
 template <int S, int MODE, int G, int GS>
@@ -590,7 +615,7 @@ __device__ __forceinline__ void step_tile(
     const int4 *__restrict__ prob, const RealInfo *__restrict__ reals,
     const Tile *__restrict__ tiles, const LaunchPlan *__restrict__ plan,
     const int32_t *__restrict__ seeds_cur, const int32_t *__restrict__ seeds_prev, const RxTable &T,
-    const HomProbs &P, int4 bm, int cpt, const HotList &hot_all, const OneReal &one) {
+    const HomProbs &P, int4 bm, int cpt, const HotList &hot_all, const OneReal &one, FlatLds &flat_lds) {
     Tile tile;
     RealInfo ri;
     if (one.per > 0) {
@@ -606,6 +631,7 @@ __device__ __forceinline__ void step_tile(
         const HotList v = hot_view(hot_all, pl.hot_parity);
         v.count[0] = 0u;
         v.count[1] = 0u;
+        if (hot_all.lane_count) hot_all.lane_count[0] = 0u;
     }
     const bool predict = mode_ssa(MODE) && pl.predict && hot_all.flag;
     const HotList hot = hot_view(hot_all, predict ? pl.hot_parity : 0);
@@ -621,8 +647,6 @@ __device__ __forceinline__ void step_tile(
     const bool any_src = (ri.src[0] | ri.src[1] | ri.src[2] | ri.src[3]) != 0;
     const int g = (int)(threadIdx.x & (G - 1));        // lane in the cell's group
     const int cpb = (int)blockDim.x / G;               // cells per block and round
-    __shared__ FlatLds flat_lds;
-    (void)flat_lds;
 
     // flattened sweeps (64/G cells per wave) may spread a wave's cells over the grid
     // (stride A/(64/G), a bijection when 64/G divides A), so that waves covering dense and
@@ -690,29 +714,20 @@ __device__ __forceinline__ void step_tile(
         if (MODE == MODE_INH_XS && hot.flag) deferred = hot.flag[ri.cell_off + c] != 0;
         if constexpr (mode_ssa(MODE)) {
           if (pl.deltat > 0.f && T.R > 0) {
-            float time = 0.f;
+            float time = 0.f, rate = 0.f;
             uint32_t ctr = 0;
-            const bool fin =
-                ssa_cell_g<S, G>(x, (uint32_t)c, seed, pl.deltat, T, hot.budget, hot.below_active, time, ctr);
-            // one list entry per deferred cell, pushed by the group's first lane
-            const uint64_t dm = __ballot(!fin && g == 0);
-            if (dm) {
-                const int lane = (int)(threadIdx.x & 63);
-                const int leader = __ffsll((unsigned long long)dm) - 1;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(hot.count, (uint32_t)__popcll(dm));
-                base = __shfl(base, leader, 64);
-                if (!fin) {
-                    const int first = lane - g;
-                    const uint32_t idx = base + (uint32_t)__popcll(dm & ((1ull << first) - 1ull));
-                    if (idx < hot.cap) {
-                        if (g == 0) hot.e[idx] = HotEntry{tile.r, c, time, ctr};
-                        deferred = true;
-                    } else {  // list full: finish the cell here
-                        ssa_cell_g<S, G>(x, (uint32_t)c, seed, pl.deltat, T, 0xffffffffu, 0, time, ctr);
-                    }
-                }
-            }
+            const bool fin = ssa_cell_g<S, G>(x, (uint32_t)c, seed, pl.deltat, T, hot.budget, hot.below_active,
+                                              time, ctr, rate);
+            // a deferred cell with a modest expected tail goes to the lane list, the others
+            // (and any the lane list has no room for) to the wave list; a cell neither list
+            // can take is finished here
+            const HotEntry he{tile.r, c, time, ctr};
+            const bool to_lane = !fin && hot.lane_e != nullptr && rate * (pl.deltat - time) <= hot.lane_events;
+            const bool in_lane = push_hot(to_lane, g, hot.lane_e, hot.lane_count, hot.lane_cap, he);
+            const bool in_wave = push_hot(!fin && !in_lane, g, hot.e, hot.count, hot.cap, he);
+            deferred = in_lane || in_wave;
+            if (!fin && !deferred)
+                ssa_cell_g<S, G>(x, (uint32_t)c, seed, pl.deltat, T, 0xffffffffu, 0, time, ctr, rate);
           }
         }
 
@@ -725,8 +740,8 @@ __device__ __forceinline__ void step_tile(
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     if (!((pl.out_mask >> s) & 1u)) continue;
-                    wave_flat_outflow<MODE, G>(x[s], deferred, c, stride, s, W, H, A, seed, pl, bm, ri, prob, lout,
-                                               P, flat_lds);
+                    wave_flat_outflow<MODE, G>(x[s], deferred, c, s, W, H, A, seed, pl, bm, ri, prob, lout, P,
+                                               flat_lds);
                 }
             } else if constexpr (G == 1 && GS == 4) {
                 // the wave's 64 cells in 4 rounds of 16, cell 16r+q by lanes 4q..4q+3;
@@ -776,10 +791,11 @@ __global__ __launch_bounds__(256) void k_gmp_step(
     const int32_t *__restrict__ seeds_cur, const int32_t *__restrict__ seeds_prev, RxTable T,
     HomProbs P, int4 bm, int cpt, HotList hot, OneReal one) {
     // the grid has 8*ceil(ntiles/8) blocks so that the remap is a bijection
+    __shared__ FlatLds flat_lds;
     const int t = xcd_tile((int)blockIdx.x, ntiles);
     if (t >= ntiles) return;
     step_tile<S, MODE, G, GS>(t, state, lost_in, lost_out, prob, reals, tiles, plan, seeds_cur, seeds_prev, T, P,
-                              bm, cpt, hot, one);
+                              bm, cpt, hot, one, flat_lds);
 }
 
 __device__ __forceinline__ float readlane_f(float v, int j) {
@@ -1022,6 +1038,52 @@ __device__ __forceinline__ void hot_cells(int wk, int nwk, int32_t *__restrict__
     }
 }
 
+// Deferred cells of the lane list, one per lane: worker wave wk of nwk takes the entries
+// [64 (wk + j nwk), 64 (wk + j nwk + 1)), j = 0, 1, ...; each lane resumes its cell's SSA
+// from (time, ctr) to deltat with the one-lane event loop (no further deferral: the list
+// holds the cells with a modest expected tail), then the wave computes the X outflow of
+// its cells for the due species of `pl`, flattened over its lanes.  The lane list serves
+// one realization: hot cells run inside the X-sweep launch of an inhomogeneous solver,
+// which is never batched.
+template <int S>
+__device__ __forceinline__ void hot_lane_cells(int wk, int nwk, int32_t *__restrict__ state,
+                                               int32_t *__restrict__ lost_out, const int4 *__restrict__ prob,
+                                               const RealInfo &ri, const LaunchPlan &pss, const LaunchPlan &pl,
+                                               int32_t seed, const RxTable &T, const HomProbs &P, int4 bm,
+                                               const HotList &hot, FlatLds &L) {
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t total = __builtin_amdgcn_readfirstlane(hot.lane_count[0]);
+    const uint32_t nh = total < hot.lane_cap ? total : hot.lane_cap;
+    const int W = ri.W, H = ri.H, A = W * H;
+    int32_t *st = state + ri.state_off;
+    for (uint32_t b = (uint32_t)wk * 64u; b < nh; b += (uint32_t)nwk * 64u) {
+        const uint32_t i = b + (uint32_t)lane;
+        const bool valid = i < nh;
+        const HotEntry he = valid ? hot.lane_e[i] : HotEntry{0, 0, 0.f, 0u};
+        const int c = he.c;
+        int32_t x[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) x[s] = valid ? st[(size_t)s * A + c] : 0;
+        if (valid) {
+            float time = he.time, rate = 0.f;
+            uint32_t ctr = he.ctr;
+            (void)ssa_cell<S>(x, (uint32_t)c, seed, pss.deltat, T, 0xffffffffu, 0, time, ctr, rate);
+        }
+        if (pl.out_mask) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if (!((pl.out_mask >> s) & 1u)) continue;
+                wave_flat_outflow<MODE_INH_X, 1>(x[s], !valid, c, s, W, H, A, seed, pl, bm, ri, prob, lost_out + ri.lost_off,
+                                                 P, L);
+            }
+        }
+        if (valid) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) st[(size_t)s * A + c] = x[s];
+        }
+    }
+}
+
 // Predicted busy cells (SSA launch of a split step): the cells of the other parity's list
 // whose prediction flag is set (more than kPredictEvents events last step) are run whole
 // by one wave each -- the gather of the Y outflow and the SSA from t = 0 with the
@@ -1091,32 +1153,43 @@ __global__ __launch_bounds__(64) void k_gmp_hot(int32_t *__restrict__ state, int
 }
 
 // X sweeps and the deferred busy cells in one launch, so the cells' serial event chains
-// run beside the sweeps instead of before them.  Blocks [0, nhot_blocks) are hot-cell
-// workers (one per wave; they continue the SSA under the SSA launch's plan `plan_ssa`,
-// then compute their cell's X outflow); the other blocks are the X-sweep tiles (64/G cells
-// per wave, flattened), which skip the cells flagged by the SSA launch.  The two kinds
-// touch disjoint cells.  nhot_blocks is a multiple of 8, so the tiles keep their XCD
-// placement.
+// run beside the sweeps instead of before them.  Blocks [0, nlane_blocks) are the lane-list
+// workers (64 cells per wave), blocks [nlane_blocks, nlane_blocks + nhot_blocks) the
+// wave-list workers (one cell per wave); both continue the SSA under the SSA launch's plan
+// `plan_ssa`, then compute their cells' X outflow.  The other blocks are the X-sweep tiles
+// (64/G cells per wave, flattened), which skip the cells flagged by the SSA launch.  The
+// kinds touch disjoint cells.  Both worker counts are multiples of 8 blocks, so the tiles
+// keep their XCD placement.  The lane and
+// wave workers read the reaction table from two copies of it (T2 and T): with a single
+// argument the compiler shares its loads between the two kinds of workers, and the kernel
+// then needs 217 VGPRs and spills (two copies: 126, 4 waves per SIMD as before).
 template <int S, int G>
-__global__ __launch_bounds__(256) void k_gmp_xs_hot(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_gmp_xs_hot(
     int32_t *__restrict__ state, const int32_t *__restrict__ lost_in, int32_t *__restrict__ lost_out,
     const int4 *__restrict__ prob, const RealInfo *__restrict__ reals,
     const Tile *__restrict__ tiles, int ntiles, const LaunchPlan *__restrict__ plan,
     const LaunchPlan *__restrict__ plan_ssa, const int32_t *__restrict__ seeds_cur,
-    const int32_t *__restrict__ seeds_prev, RxTable T, HomProbs P, int4 bm, int cpt, HotList hot,
-    int nhot_blocks, OneReal one) {
-    if ((int)blockIdx.x < nhot_blocks) {
-        const int wpb = (int)(blockDim.x >> 6);
+    const int32_t *__restrict__ seeds_prev, RxTable T, RxTable T2, HomProbs P, int4 bm, int cpt, HotList hot,
+    int nlane_blocks, int nhot_blocks, OneReal one) {
+    __shared__ FlatLds flat_lds;
+    const int wpb = (int)(blockDim.x >> 6);
+    if ((int)blockIdx.x < nlane_blocks) {
         const LaunchPlan pss = *plan_ssa, pl = *plan;
-        const int par = pss.predict ? pss.hot_parity : 0;
-        hot_cells<S, MODE_INH_X>((int)blockIdx.x * wpb + (int)(threadIdx.x >> 6), nhot_blocks * wpb, state,
-                                 lost_out, prob, reals, pss, pl, seeds_cur, T, P, bm, hot_view(hot, par), false);
+        hot_lane_cells<S>((int)blockIdx.x * wpb + (int)(threadIdx.x >> 6), nlane_blocks * wpb, state, lost_out, prob,
+                          reals[0], pss, pl, seeds_cur[0], T2, P, bm, hot, flat_lds);
         return;
     }
-    const int t = xcd_tile((int)blockIdx.x - nhot_blocks, ntiles);
+    if ((int)blockIdx.x < nlane_blocks + nhot_blocks) {
+        const LaunchPlan pss = *plan_ssa, pl = *plan;
+        const int par = pss.predict ? pss.hot_parity : 0;
+        hot_cells<S, MODE_INH_X>(((int)blockIdx.x - nlane_blocks) * wpb + (int)(threadIdx.x >> 6), nhot_blocks * wpb,
+                                 state, lost_out, prob, reals, pss, pl, seeds_cur, T, P, bm, hot_view(hot, par), false);
+        return;
+    }
+    const int t = xcd_tile((int)blockIdx.x - nlane_blocks - nhot_blocks, ntiles);
     if (t >= ntiles) return;
     step_tile<S, MODE_INH_XS, G, 0>(t, state, lost_in, lost_out, prob, reals, tiles, plan, seeds_cur, seeds_prev,
-                                    T, P, bm, cpt, hot, one);
+                                    T, P, bm, cpt, hot, one, flat_lds);
 }
 
 
@@ -1130,6 +1203,7 @@ __global__ __launch_bounds__(256) void k_gmp_ssa_pred(
     const Tile *__restrict__ tiles, int ntiles, const LaunchPlan *__restrict__ plan,
     const int32_t *__restrict__ seeds_cur, const int32_t *__restrict__ seeds_prev, RxTable T, HomProbs P, int4 bm,
     int cpt, HotList hot, int npred_blocks) {
+    __shared__ FlatLds flat_lds;
     if ((int)blockIdx.x < npred_blocks) {
         const int wpb = (int)(blockDim.x >> 6);
         const LaunchPlan pl = *plan;
@@ -1140,7 +1214,7 @@ __global__ __launch_bounds__(256) void k_gmp_ssa_pred(
     const int t = xcd_tile((int)blockIdx.x - npred_blocks, ntiles);
     if (t >= ntiles) return;
     step_tile<S, MODE_INH_X, 1, 0>(t, state, lost_in, lost_out, prob, reals, tiles, plan, seeds_cur, seeds_prev, T,
-                                   P, bm, cpt, hot, OneReal{});
+                                   P, bm, cpt, hot, OneReal{}, flat_lds);
 }
 
 template <int S>
@@ -1179,17 +1253,19 @@ static hipError_t launch_sg(const StepLaunch &a) {
 template <int S, int G>
 static hipError_t launch_xs_hot_g(const StepLaunch &a) {
     const int wpb = a.block / 64;
+    // both worker counts are multiples of 8 blocks, so the tiles keep their XCD placement
+    const int nlane = a.hot.lane_e ? 8 * ((a.lane_grid + 8 * wpb - 1) / (8 * wpb)) : 0;
     const int nhot = 8 * ((a.hot_grid + 8 * wpb - 1) / (8 * wpb));
-    const int grid = nhot + 8 * ((a.ntiles + 7) / 8);
+    const int grid = nlane + nhot + 8 * ((a.ntiles + 7) / 8);
     if (a.ev_start) {
         hipExtLaunchKernelGGL((k_gmp_xs_hot<S, G>), dim3(grid), dim3(a.block), 0, a.stream, a.ev_start, a.ev_stop, 0,
                               a.state, a.lost_in, a.lost_out, a.prob, a.reals, a.tiles, a.ntiles, a.plan,
-                              a.plan_ssa, a.seeds_cur, a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot, nhot,
-                              a.one);
+                              a.plan_ssa, a.seeds_cur, a.seeds_prev, *a.T, *a.T, *a.P, a.bm, a.cpt, a.hot, nlane,
+                              nhot, a.one);
     } else {
         hipLaunchKernelGGL((k_gmp_xs_hot<S, G>), dim3(grid), dim3(a.block), 0, a.stream, a.state, a.lost_in,
                            a.lost_out, a.prob, a.reals, a.tiles, a.ntiles, a.plan, a.plan_ssa, a.seeds_cur,
-                           a.seeds_prev, *a.T, *a.P, a.bm, a.cpt, a.hot, nhot, a.one);
+                           a.seeds_prev, *a.T, *a.T, *a.P, a.bm, a.cpt, a.hot, nlane, nhot, a.one);
     }
     return hipGetLastError();
 }

@@ -33,6 +33,7 @@ struct StepLaunch {
     hipEvent_t ev_stop = nullptr;
     HotList hot{};                  // deferred busy cells (modes with an SSA)
     int hot_grid = 1024;            // waves of the hot-cell kernel
+    int lane_grid = 0;              // lane-list worker waves (X sweeps with the hot cells)
     const LaunchPlan *plan_ssa = nullptr;  // X sweeps with the hot cells: the SSA launch's plan
     int predict = 0;        // SSA launch of a split step: run last step's busy cells in worker waves
     hipEvent_t ev_hot_start = nullptr;  // optional: timestamps of the hot-cell dispatch

@@ -96,6 +96,15 @@ struct HotList {
     // and pred[p * pred_stride + cell_off + c] = 1 if the cell is in list p
     uint8_t *pred;
     uint32_t pred_stride;
+    // Lane list (hot cells fused into the X-sweep launch, one realization, no prediction):
+    // a deferred cell whose expected remaining events h0 * (deltat - time) are at most
+    // lane_events goes here instead of e and is resumed by one lane of a worker wave
+    // (64 such cells per wave) -- the cheapest VALU work per event for the many modest
+    // tails of a dense step -- while the busier ones keep one wave each.  Null: no lane list.
+    HotEntry *lane_e;
+    uint32_t *lane_count;  // [0] entries pushed
+    uint32_t lane_cap;
+    float lane_events;
 };
 
 // Homogeneous diffusion thresholds (StochasticHomogeneousDiffusion.cl:58-62)

@@ -251,6 +251,10 @@ class Solver:
         call("gpgmp_solver_run_continue", self.handle, int(steps), int(graph_chunk), C.byref(t))
         return t.value
 
+    def prepareRun(self, steps, graph_chunk):
+        """Captures the HIP graphs runContinue(steps, graph_chunk) will replay (runs nothing)."""
+        call("gpgmp_solver_prepare_run", self.handle, int(steps), int(graph_chunk))
+
     def attachIndividualSpecies(self, species, individuals, rule=None):
         """Makes `species` particle-based: `individuals` (an IndividualSpecies on this grid with
         that species index) holds its particles; doReactions then maintains their population
@@ -315,6 +319,11 @@ class Solver:
         """A cell past `budget` SSA events moves to the one-wave-per-cell kernel once
         fewer than `below_active` lanes of its wave still run (65: always)."""
         call("gpgmp_solver_set_event_budget", self.handle, int(budget), int(below_active))
+
+    def setHotWorkers(self, lane_events=16.0, lane_waves=1024, wave_waves=1024):
+        """Deferred-cell workers of split steps: tails of at most lane_events expected events
+        resume one per lane, busier cells one per wave (results do not depend on these)."""
+        call("gpgmp_solver_set_hot_workers", self.handle, float(lane_events), int(lane_waves), int(wave_waves))
 
     def kernelTimes(self, on=True):
         """Accumulated HIP-event times (ms, launches timed, algorithmic bytes) per kernel
@@ -405,6 +414,10 @@ class Ensemble:
 
     def run(self, steps, graph_chunk=0):
         call("gpgmp_batch_run", self.handle, int(steps), int(graph_chunk))
+
+    def prepareRun(self, steps, graph_chunk):
+        """Captures the HIP graphs run(steps, graph_chunk) will replay (runs nothing)."""
+        call("gpgmp_batch_prepare_run", self.handle, int(steps), int(graph_chunk))
 
     def reset(self):
         call("gpgmp_batch_reset", self.handle)

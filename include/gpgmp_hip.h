@@ -158,6 +158,12 @@ int gpgmp_solver_get_output(gpgmp_solver *s, int32_t *counts, float *times, int3
 /* Continues the loop of the last gpgmp_solver_run for `steps` more iterations;
  * graph_chunk > 0 replays chunks of that many iterations as a HIP graph. */
 int gpgmp_solver_run_continue(gpgmp_solver *s, int32_t steps, int32_t graph_chunk, float *sim_time_out);
+/* Captures the HIP graphs that gpgmp_solver_run_continue(s, steps, graph_chunk) will replay,
+ * without running any step (the loop, state and seeds are unchanged), so that a following
+ * run of a few steps replays graphs instead of capturing them while it runs.  Graphs are
+ * kept per chunk buffer, chunk length and outflow-buffer parity.  No counterpart in the
+ * reference (its loop launches every kernel eagerly, Kernel.h:248-272). */
+int gpgmp_solver_prepare_run(gpgmp_solver *s, int32_t steps, int32_t graph_chunk);
 /* launch shape of the solver's kernels: threads per block (64..256), cells per thread */
 /* Flattened sweeps (n = 8 or 16 cells per wave): on (default) spreads each wave's cells over
  * the grid (stride W*H/n) so that waves over dense and sparse regions even out; off keeps n
@@ -199,6 +205,14 @@ int gpgmp_solver_kernel_times(gpgmp_solver *s, int32_t on, double *ms, int64_t *
  * its wave still run their SSA (default 24; 65 = always at the budget).  Results do not
  * depend on either.  No counterpart in the reference. */
 int gpgmp_solver_set_event_budget(gpgmp_solver *s, uint32_t budget, int32_t below_active);
+/* Workers of the deferred cells inside the X-sweep launch of a split step (the default
+ * inhomogeneous launch shape): a deferred cell whose expected remaining events
+ * (h0 * (dt - t) when it was deferred) are at most lane_events (default 16) is resumed by
+ * one lane of one of lane_waves worker waves (default 1024; 64 cells per wave and round),
+ * the others by one wave each of wave_waves (default 1024) with speculative event batches;
+ * lane_events <= 0 sends every deferred cell to the wave workers.  Results do not depend
+ * on any of them.  No counterpart in the reference. */
+int gpgmp_solver_set_hot_workers(gpgmp_solver *s, float lane_events, int32_t lane_waves, int32_t wave_waves);
 int gpgmp_solver_synchronize(gpgmp_solver *s);
 
 /* ---- batched ensemble (no reference counterpart) ----------------------------------- */
@@ -226,6 +240,8 @@ int gpgmp_batch_autotune(gpgmp_batch *b, int32_t trial_steps, int32_t *block_out
 /* Runs `steps` host-loop iterations of every realization.  graph_chunk > 0 captures
  * chunks of that many iterations in a HIP graph that is replayed. */
 int gpgmp_batch_run(gpgmp_batch *b, int32_t steps, int32_t graph_chunk);
+/* As gpgmp_solver_prepare_run, for gpgmp_batch_run(b, steps, graph_chunk). */
+int gpgmp_batch_prepare_run(gpgmp_batch *b, int32_t steps, int32_t graph_chunk);
 /* Restarts the host loops at time 0 with the realizations' seed0 (state is kept). */
 int gpgmp_batch_reset(gpgmp_batch *b);
 int gpgmp_batch_kernel_times(gpgmp_batch *b, int32_t on, double *ms, int64_t *n, double *bytes);
