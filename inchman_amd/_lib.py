@@ -161,6 +161,7 @@ SIGNATURES = {
     "gpgmp_det_aqss_device": [_P, _P, _F, _I, _P],
     "gpgmp_moments_x_device": [_P, _I, _I, _I, _P, _P],
     "gpgmp_sort_pairs_device": [_P, _P, _P, _P, _I, _I, _P],
+    "gpgmp_sort_pairs_device_noalloc": [_P, _P, _P, _P, _I, _I, _P],
     "gpgmp_cell_offsets_device": [_P, _I, _I, _P, _P],
     "gpgmp_cell_left_right_device": [_P, _I, _I, _P, _P, _P],
     "gpgmp_particles_create": [C.POINTER(ParticleParams), _I, _P, C.POINTER(_P)],

@@ -315,6 +315,15 @@ int gpgmp_moments_x_device(const int32_t *state, int32_t W, int32_t H, int32_t s
  * IndividualSolver.cpp:673-808).  key_bits limits the passes (32 = full key). */
 int gpgmp_sort_pairs_device(uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp,
                             int32_t n, int32_t key_bits, void *stream);
+/* gpgmp_sort_pairs_device without any device allocation: no counterpart in the reference;
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  The per-tile digit histograms live in a library-owned device array
+ * of 4 MiB + 1 KiB per GPU (held until the process ends), so n is at most 2^24
+ * (GPGMP_E_CAPACITY above); keys_tmp/vals_tmp are still the caller's n-element buffers.
+ * Stream-ordered, no host synchronisation.  Every call on a GPU shares that array: calls
+ * that overlap on one GPU (other streams or threads) race. */
+int gpgmp_sort_pairs_device_noalloc(uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp,
+                                    int32_t n, int32_t key_bits, void *stream);
 /* CSR offsets off[c] = #(key < c), c = 0..ncells, of sorted keys. */
 int gpgmp_cell_offsets_device(const uint32_t *sorted_keys, int32_t n, int32_t ncells, uint32_t *off,
                               void *stream);
