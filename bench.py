@@ -28,9 +28,37 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "cell-steps/s at 1 GPU, batched vs plain; HBM GB/s; adjoint grad/s at 1e-4 rel"
 
 
-def cpu_baseline(steps):
+def _host_info():
+    """CPU model, thread count and the OpenCL devices of the host (the reference's kernels
+    would need a CPU OpenCL device; ROCm's OpenCL exposes GPU agents only)."""
+    import subprocess
+
+    info = {"cpu_model": None, "cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["cpu_model"] = line.split(":", 1)[1].strip()
+    except Exception as e:  # noqa: BLE001
+        info["cpu_model"] = "unknown (%s)" % type(e).__name__
+    try:
+        out = subprocess.run(["clinfo"], capture_output=True, text=True, timeout=60).stdout
+        types = [l.split(":", 1)[1].strip() for l in out.splitlines() if "Device Type:" in l]
+        plats = [l.split(":", 1)[1].strip() for l in out.splitlines() if l.strip().startswith("Platform Name:")]
+        cpu = [t for t in types if "CPU" in t]
+        info["opencl"] = {"platforms": sorted(set(plats)), "device_types": types,
+                          "cpu_device": bool(cpu),
+                          "reference_on_cpu_opencl": "available" if cpu else
+                          "unavailable: no CPU OpenCL device (clinfo lists %s)" % (", ".join(types) or "no device")}
+    except Exception as e:  # noqa: BLE001
+        info["opencl"] = {"reference_on_cpu_opencl": "unavailable: clinfo failed (%s)" % type(e).__name__}
+    return info
+
+
+def cpu_baseline(steps, single_steps):
     """The CPU oracle (a restatement of the reference kernels, OpenMP over cells) on the
-    first `steps` iterations of the headline workload."""
+    first `steps` iterations of the headline workload with every host thread, and on the
+    first `single_steps` iterations with one thread."""
     from oracle import oracle
     from inchman_amd import configs
 
@@ -40,24 +68,31 @@ def cpu_baseline(steps):
     t0 = time.perf_counter()
     oracle.run(m, st, steps, 5, nthreads=threads)
     dt = time.perf_counter() - t0
-    return {"value": 65536 * steps / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
-            "sample": "first %d host-loop iterations of the 256x256 Schnakenberg run (%.1f s)" % (steps, dt)}
+    t1 = time.perf_counter()
+    oracle.run(m, st, single_steps, 5, nthreads=1)
+    d1 = time.perf_counter() - t1
+    out = {"value": 65536 * steps / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+           "sample": "first %d host-loop iterations of the 256x256 Schnakenberg run (%.1f s)" % (steps, dt),
+           "single_thread": {"value": 65536 * single_steps / d1, "unit": "cell-steps/s", "cores": 1,
+                             "sample": "first %d host-loop iterations of the same run (%.1f s)" % (single_steps, d1)}}
+    out.update(_host_info())
+    return out
 
 
 TIMING_EVERY = 128  # time 4 of every 128 steps: events on every launch add ~22% dispatch overhead
+GRAPH_CHUNK = 128   # HIP-graph chunk of the timed run (the timing windows run eagerly)
 
+# Counters of the timed launches (the 4 counted steps of every window of TIMING_EVERY in the
+# timed region), per dispatch, by timing slot (1 SSA launch, 2 Y sweeps, 4 X sweeps + busy
+# cells), for the bench windows that were profiled: SQ_INSTS_VALU (wave64 VALU instructions,
+# summed over the chip) and HBM bytes (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE KiB, the factors a
+# plain device copy reads).  Keyed by the number of timed steps (the timed region always
+# starts from the initial state, so the window is fixed by it).  Commands, raw counters and
+# the launch selection: profiles/gmp_step_schnakenberg256_counters.md.
+WINDOW_COUNTERS = {}
 
-# HBM bytes per dispatch of the headline step kernels (by timing slot: 1 SSA launch, 2 Y
-# sweeps, 4 X sweeps + hot cells), from rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs
-# over the first 2000 iterations, scaled by a plain device copy's readings (x2.0 / x1.0);
-# commands and raw counters in profiles/gmp_step_schnakenberg256.md
-TRAFFIC_BYTES = {1: 1634513, 2: 1635917, 4: 1275432}
-
-# wave64 VALU instructions per dispatch of the same kernels (rocprofv3 SQ_INSTS_VALU summed
-# over the chip, averaged over launches 2..10001 of a full 10^4-iteration run; command in
-# profiles/gmp_step_schnakenberg256.md) and MI355X's VALU issue peak: 256 CUs x 4 SIMDs,
-# one wave64 32-bit VALU instruction per SIMD every 4 cycles (16 lanes), 2.4 GHz
-VALU_INSTS = {1: 1.69444e6, 2: 8.53646e6, 4: 9.49486e6}
+# MI355X's VALU issue peak: 256 CUs x 4 SIMDs, one wave64 32-bit VALU instruction per SIMD
+# every 4 cycles (16 lanes), 2.4 GHz
 VALU_PEAK = 256 * 4 * 2.4e9 / 4
 
 
@@ -65,17 +100,20 @@ def progress(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def plain_run(solver, st, seed0, steps, timing):
+def plain_run(solver, st, seed0, steps, timing, chunk=GRAPH_CHUNK):
     import torch
 
     solver.synchronizeStateBufferToDevice(st)
     solver.run(0, seed0)
     solver.kernelTimes(on=timing)
+    # the graphs the run replays are captured here, ahead of the timed region (nothing runs)
+    solver.prepareRun(steps, chunk)
+    solver.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    # HIP-graph chunks of `timing` steps: each chunk's timing window runs eagerly with its
-    # events, the rest of the chunk replays a graph
-    solver.runContinue(steps, timing)
+    # HIP-graph chunks: each chunk's timing window runs eagerly with its events, the rest of
+    # the chunk replays a graph
+    solver.runContinue(steps, chunk)
     solver.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -83,7 +121,12 @@ def plain_run(solver, st, seed0, steps, timing):
     return wall, ms, n, b
 
 
-def ensemble_extra(steps=200):
+def ensemble_extra(steps=5000, sample=32, chunk=50):
+    """The batched ensemble on its configuration (512 realizations of the plain GMP network,
+    sides log-uniform in [64, 512], seeds s_r = Threefry(r, 0xBA7C), `steps` host-loop
+    iterations), autotuned launch shape, HIP-graph chunks captured ahead; against the plain
+    solver run realization by realization on a stratified sample (every 512/sample-th
+    realization in size order) for the same number of steps."""
     import numpy as np
     import torch
     from inchman_amd import Ensemble, configs
@@ -100,35 +143,37 @@ def ensemble_extra(steps=200):
     for r, st in enumerate(states):
         ens.setState(r, st)
     ens.reset()
-    ens.run(20, 0)
+    ens.prepareRun(steps, chunk)
     ens.synchronize()
     cells = ens.totalCells()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ens.run(steps, 50)
+    ens.run(steps, chunk)
     ens.synchronize()
     wall = time.perf_counter() - t0
     batched = cells * steps / wall
-    # plain solver, one realization after the other, on a slice spanning the sizes
-    idx = np.argsort(sides)[:: max(1, n // 24)]
+    idx = np.argsort(sides, kind="stable")[n // (2 * sample)::n // sample]
     tot, cs = 0.0, 0
     for r in idx:
-        pm = configs.plain_gmp_model(int(sides[r]))
-        p = pm.createSolver()
+        p = configs.plain_gmp_model(int(sides[r])).createSolver()
         p.synchronizeStateBufferToDevice(states[r])
-        p.run(20, int(seeds[r]))
+        p.run(0, int(seeds[r]))
+        p.prepareRun(steps, chunk)
         p.synchronize()
         t0 = time.perf_counter()
-        p.runContinue(steps, 50)
+        p.runContinue(steps, chunk)
         p.synchronize()
         tot += time.perf_counter() - t0
         cs += int(sides[r]) ** 2 * steps
     plain = cs / tot
-    return {"workload": "512 realizations of the plain GMP network, sides 64..512 (sum %d cells), %d steps"
-                        % (cells, steps),
-            "batched_cell_steps_per_s": batched, "plain_cell_steps_per_s": plain,
+    return {"workload": "512 realizations of the plain GMP network, sides 64..512 (sum %d cells), %d steps, "
+                        "seeds Threefry(r, 0xBA7C)" % (cells, steps),
+            "batched_cell_steps_per_s": batched, "batched_wall_s": wall, "plain_cell_steps_per_s": plain,
+            "plain_sample": "%d realizations (every %dth by size, sides %d..%d, sum %d cells), %d steps each"
+                            % (len(idx), n // sample, int(sides[idx].min()), int(sides[idx].max()),
+                               int((sides[idx].astype(np.int64) ** 2).sum()), steps),
             "batched_over_plain": batched / plain, "launch": {"block": block, "cells_per_thread": cpt},
-            "graph_chunk": 50}
+            "graph_chunk": chunk}
 
 
 def rdd_extra(reps=3):
@@ -416,6 +461,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=3000)
+    ap.add_argument("--cpu-single-steps", type=int, default=60)
     ap.add_argument("--no-extra", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -443,7 +489,7 @@ def main():
     cells = 256 * 256
     value = cells * args.steps / wall
     # the step kernels (index 3 is the hot-cell continuation), by total time in the run
-    dom = int(max((0, 1, 2, 4), key=lambda i: ms[i] / max(n[i], 1) * max(n[i], 1)))
+    dom = int(max((0, 1, 2, 4), key=lambda i: ms[i]))
     avg_ms = ms[dom] / max(n[dom], 1)
     achieved = (b[dom] / max(n[dom], 1)) / (avg_ms * 1e-3) / 1e9
     # kernel names as the profiler prints them (template arguments: species, mode, SSA or
@@ -455,6 +501,30 @@ def main():
              2: "k_gmp_step<2, 2, 8, 0> (gather of X outflow + Y sweeps)",
              3: "k_gmp_hot<2, 1> (deferred busy cells; not launched on this shape)",
              4: "k_gmp_xs_hot<2, 8> (X sweeps + deferred busy cells)"}
+    # VALU issue against its peak, from the counters of this window's timed launches (the
+    # kernels are RNG-bound: one Threefry-2x32-20 per SSA event and per particle pair); the
+    # HBM roofline of the same launches beside it
+    win = WINDOW_COUNTERS.get(args.steps)
+    valu = {}
+    for i in (1, 2, 4):
+        if win and i in win["valu"] and n[i]:
+            valu[i] = win["valu"][i] / (ms[i] / n[i] * 1e-3)
+    frac = {i: v / VALU_PEAK for i, v in valu.items()}
+    bad = {names[i]: f for i, f in frac.items() if f > 1.0}
+    if bad:
+        progress("VALU fraction above 1 (counter window does not match the timed launches): %s" % bad)
+    roof = {"bound": "valu", "kernel": names[dom],
+            "achieved": valu.get(dom), "peak": VALU_PEAK, "unit": "wave64 VALU instructions/s",
+            "frac": frac.get(dom),
+            "traffic": win["hbm_bytes"].get(dom) if win else None,
+            "avg_launch_ms": avg_ms, "launches_timed": int(n[dom]),
+            "instructions_per_launch": win["valu"].get(dom) if win else None,
+            "counters": (win["source"] if win else
+                         "no counter profile of a %d-step timed window: VALU fraction not reported" % args.steps),
+            "hbm": {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                    "algorithmic_bytes_per_launch": b[dom] / max(n[dom], 1),
+                    "traffic_bytes_per_launch": win["hbm_bytes"].get(dom) if win else None},
+            "consistent": not bad}
     out = {
         "metric": METRIC,
         "value": value,
@@ -471,37 +541,29 @@ def main():
         "config": {"workload": "stochastic inhomogeneous drift-diffusion, Schnakenberg, 256x256, "
                                "reference-exact Threefry streams",
                    "grid": [256, 256], "species": 2, "reactions": 4, "iterations": args.steps},
-        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES.get(dom),
-                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/gmp_step_schnakenberg256.md)",
-                     "avg_launch_ms": avg_ms, "launches_timed": int(n[dom]),
-                     "algorithmic_bytes_per_launch": b[dom] / max(n[dom], 1),
-                     "note": "RNG-bound (one Threefry-2x32-20 per event and per particle pair): "
-                             "VALU issue is the binding resource, HBM bytes are small",
-                     "valu": {"achieved": VALU_INSTS[dom] / (avg_ms * 1e-3), "peak": VALU_PEAK,
-                              "unit": "wave64 VALU instructions/s", "frac": VALU_INSTS[dom] / (avg_ms * 1e-3) / VALU_PEAK,
-                              "instructions_per_launch": VALU_INSTS[dom],
-                              "source": "rocprofv3 --pmc SQ_INSTS_VALU over a full run "
-                                        "(profiles/gmp_step_schnakenberg256.md) / this run's launch time"}},
+        "roofline": roof,
     }
     extra = {"kernel_avg_launch_ms": {names[i]: ms[i] / n[i] for i in range(5) if n[i]},
-             "kernel_valu_frac": {names[i]: VALU_INSTS[i] / (ms[i] / n[i] * 1e-3) / VALU_PEAK
-                                  for i in VALU_INSTS if n[i]},
+             "kernel_valu_frac": {names[i]: f for i, f in frac.items()},
+             "kernel_hbm_frac": {names[i]: (b[i] / n[i]) / (ms[i] / n[i] * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                 for i in range(5) if n[i]},
              "kernel_launches_timed": {names[i]: int(n[i]) for i in range(5) if n[i]},
              "kernel_timing": "every launch of 4 consecutive steps out of every %d of the timed region"
                                % TIMING_EVERY}
-    # same sample as the CPU baseline, for a like-for-like ratio
-    w2, _, _, _ = plain_run(solver, st, seed0, args.cpu_steps, timing=False)
-    extra["gpu_on_cpu_sample_cell_steps_per_s"] = cells * args.cpu_steps / w2
     progress("headline done: %.4g cell-steps/s" % value)
+    if args.cpu_steps > 0:
+        # same sample as the CPU baseline, for a like-for-like ratio
+        w2, _, _, _ = plain_run(solver, st, seed0, args.cpu_steps, timing=False)
+        extra["gpu_on_cpu_sample_cell_steps_per_s"] = cells * args.cpu_steps / w2
     if not args.no_extra:
         for key, fn in (("ensemble", ensemble_extra), ("rdd", rdd_extra), ("det_diffusion", det_diffusion_extra),
                         ("particles", particles_extra), ("individuals", individuals_extra),
                         ("aqss", aqss_extra), ("rk4_moments", rk4_moments_extra)):
             extra[key] = fn()
             progress(key + " done")
-    out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
-    progress("cpu baseline done")
+    if args.cpu_steps > 0:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.cpu_single_steps)
+        progress("cpu baseline done")
     out["extra"] = extra
     print(json.dumps(out), flush=True)
 

@@ -126,6 +126,8 @@ SIGNATURES = {
     "gpgmp_solver_get_output": [_P, _P, _P, _pI, _pI, C.POINTER(C.c_int64)],
     "gpgmp_solver_run_continue": [_P, _I, _I, _pF],
     "gpgmp_solver_prepare_run": [_P, _I, _I],
+    "gpgmp_solver_skip": [_P, _I, _pF],
+    "gpgmp_solver_individuals": [_P, _I, C.POINTER(C.c_void_p)],
     "gpgmp_solver_set_launch": [_P, _I, _I],
     "gpgmp_solver_kernel_times": [_P, _I, _pD, C.POINTER(C.c_int64), _pD],
     "gpgmp_solver_set_event_budget": [_P, _U, _I],
@@ -154,7 +156,9 @@ SIGNATURES = {
     "gpgmp_rdd_set_rates": [_P, _pF],
     "gpgmp_rdd_set_steps_per_launch": [_P, _I],
     "gpgmp_rdd_forward": [_P, _P, _P, _P, _I],
-    "gpgmp_rdd_backward": [_P, _P, _P, _P, _P, _P],
+    "gpgmp_rdd_backward": [_P, C.c_uint64, _P, _P, _P, _P, _P],
+    "gpgmp_rdd_set_rates_device": [_P, _P],
+    "gpgmp_rdd_generation": [_P, C.POINTER(C.c_uint64)],
     "gpgmp_det_hom_diffuse_device": [_P, _P, _I, _I, _I, _pF, _F, _F, _P],
     "gpgmp_det_inhom_sweep_device": [_P, _P, _I, _I, _I, _I, _F, _P, _P, _F, _P],
     "gpgmp_det_rk4_device": [_P, _P, _F, _P],

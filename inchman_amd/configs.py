@@ -90,8 +90,31 @@ def ensemble_sides(n=512, lo=64, hi=512, seed=2024):
     return (16 * np.maximum(1, np.round(s / 16))).astype(np.int32)
 
 
-def ensemble_seeds(n=512, seed=0xBA7C):
-    return np.random.default_rng(seed).integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+def threefry2x32_20(c0, c1, k0, k1):
+    """Threefry-2x32 with 20 rounds (Salmon et al., SC'11: rotations 13 15 26 6 17 29 16 24,
+    key parity 0x1BD11BDA, a key injection after every fourth round), vectorised over numpy
+    uint32 arrays; the generator of the solver kernels, used here to derive seeds."""
+    M = np.uint64(0xFFFFFFFF)
+    c0, c1, k0, k1 = (np.asarray(v, np.uint64) & M for v in (c0, c1, k0, k1))
+    ks = (k0, k1, (k0 ^ k1 ^ np.uint64(0x1BD11BDA)) & M)
+    rot = (13, 15, 26, 6, 17, 29, 16, 24)
+    x0, x1 = (c0 + ks[0]) & M, (c1 + ks[1]) & M
+    for r in range(20):
+        x0 = (x0 + x1) & M
+        s = rot[r % 8]
+        x1 = ((x1 << np.uint64(s)) | (x1 >> np.uint64(32 - s))) & M
+        x1 ^= x0
+        if r % 4 == 3:
+            j = r // 4 + 1
+            x0 = (x0 + ks[j % 3]) & M
+            x1 = (x1 + ks[(j + 1) % 3] + np.uint64(j)) & M
+    return x0.astype(np.uint32), x1.astype(np.uint32)
+
+
+def ensemble_seeds(n=512, key=0xBA7C):
+    """Realization seeds s_r = word 0 of Threefry-2x32-20 at counter (r, 0), key (0xBA7C, 0)."""
+    r = np.arange(n, dtype=np.uint64)
+    return threefry2x32_20(r, 0, key, 0)[0]
 
 
 def ensemble_state(side, r, seed=12345):
@@ -102,19 +125,27 @@ def ensemble_state(side, r, seed=12345):
 RDD_D0 = (0.05, 1.0)
 
 
-def rdd_problem(width=1024, height=None, steps=2000, seed=7, target_seed=11):
+def rdd_problem(width=1024, height=None, steps=2000, seed=7, target_seed=11, field="smoothed_lognormal"):
     """Schnakenberg (k1..k4 of the stochastic configuration) in concentration form on a
-    seeded log-normal diffusivity field: D = exp(0.3 g), g a unit-variance Gaussian
-    random field with correlation length 8 cells (white noise smoothed by a Gaussian
-    of sigma 8, rescaled).  dt = 0.2 h^2 / (4 max_s D0_s max D).  Initial u, v near
-    200 / 75; the target is a seeded field of the magnitude the run reaches."""
+    seeded diffusivity field.  field "smoothed_lognormal" (the configured one):
+    D = G_8 * exp(N(0, 0.3^2)), i.e. per-cell log-normal noise smoothed by a Gaussian of
+    sigma 8 cells; "lognormal_grf": D = exp(0.3 g) with g a unit-variance Gaussian random
+    field of correlation length 8 cells (white noise smoothed, then rescaled), a rougher
+    field.  dt = 0.2 h^2 / (4 max_s D0_s max D).  Initial u, v near 200 / 75; the target is
+    a seeded field of the magnitude the run reaches."""
     from scipy.ndimage import gaussian_filter
 
     height = width if height is None else height
     rng = np.random.default_rng(seed)
-    g = gaussian_filter(rng.standard_normal((height, width)), 8.0, mode="reflect")
-    g = (g - g.mean()) / g.std()
-    D = np.exp(0.3 * g).astype(np.float32)
+    noise = rng.standard_normal((height, width))
+    if field == "smoothed_lognormal":
+        D = gaussian_filter(np.exp(0.3 * noise), 8.0, mode="reflect").astype(np.float32)
+    elif field == "lognormal_grf":
+        g = gaussian_filter(noise, 8.0, mode="reflect")
+        g = (g - g.mean()) / g.std()
+        D = np.exp(0.3 * g).astype(np.float32)
+    else:
+        raise ValueError("unknown field %r" % (field,))
     dt = float(np.float32(0.2 / (4.0 * max(RDD_D0) * float(D.max()))))
     u0 = (200.0 + np.sqrt(200.0) * rng.standard_normal((height, width))).astype(np.float32)
     v0 = (75.0 + np.sqrt(75.0) * rng.standard_normal((height, width))).astype(np.float32)
@@ -122,4 +153,4 @@ def rdd_problem(width=1024, height=None, steps=2000, seed=7, target_seed=11):
     ut = (1.0 + 0.1 * trng.standard_normal((height, width))).astype(np.float32)
     vt = (60.0 + 5.0 * trng.standard_normal((height, width))).astype(np.float32)
     return {"D": D, "u0": u0, "v0": v0, "ut": ut, "vt": vt, "dt": dt, "steps": steps,
-            "k": SCHNAKENBERG_K, "D0": RDD_D0, "h": 1.0}
+            "k": SCHNAKENBERG_K, "D0": RDD_D0, "h": 1.0, "field": field}

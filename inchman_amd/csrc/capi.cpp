@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "det_solver.h"
 #include "host.h"
 
 using namespace gpgmp;
@@ -47,7 +48,8 @@ struct StreamHolder {
 struct gpgmp_solver {
     Model m;
     StreamHolder stream;
-    std::unique_ptr<GmpEngine> eng;
+    std::unique_ptr<GmpEngine> eng;  // stochastic types
+    std::unique_ptr<DetEngine> det;  // deterministic types
     std::vector<float> dtx, dty, nx, ny;
     DevBuf<LaunchPlan> plan_d;
     DevBuf<int32_t> seed_d;
@@ -124,16 +126,22 @@ int gpgmp_model_rate_literals(const gpgmp_model *m, int32_t *is_int, double *val
 
 // ---- plain solver -------------------------------------------------------------------
 int gpgmp_solver_create(const gpgmp_model *model, void *stream, uint32_t flags, gpgmp_solver **out) {
-    (void)flags;
     return guarded([&] {
         GPGMP_REQUIRE(model && out, GPGMP_E_INVALID, "null argument");
         *out = nullptr;
-        GPGMP_REQUIRE(!model->m.deterministic(), GPGMP_E_UNSUPPORTED,
-                      "deterministic solver types are served by the gpgmp_det_* and gpgmp_rdd_* calls");
+        GPGMP_REQUIRE((flags & ~(GPGMP_SOLVER_AQSS_PADE_FIX | GPGMP_SOLVER_REFERENCE_QUIRKS)) == 0, GPGMP_E_INVALID,
+                      "unknown solver flags");
         std::unique_ptr<gpgmp_solver> s(new gpgmp_solver());
         s->m = model->m;
         s->m.desc.reactions = s->m.reactions.empty() ? nullptr : s->m.reactions.data();
         s->stream.init(stream);
+        if (s->m.deterministic()) {
+            s->det.reset(new DetEngine(s->m, flags, s->stream.s));
+            s->dtx = s->det->dt_x();
+            s->dty = s->det->dt_y();
+            *out = s.release();
+            return;
+        }
         const gpgmp_model_desc &d = s->m.desc;
         const int S = d.num_species;
         s->eng.reset(new GmpEngine(s->m, {d.width}, {d.height}, s->stream.s));
@@ -183,9 +191,17 @@ int gpgmp_solver_destroy(gpgmp_solver *s) {
     return GPGMP_OK;
 }
 
+// the stochastic engine of a handle (the calls that only stochastic types have)
+static GmpEngine &stoch(gpgmp_solver *s) {
+    GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+    GPGMP_REQUIRE(s->eng != nullptr, GPGMP_E_UNSUPPORTED, "a call of the stochastic solver types");
+    return *s->eng;
+}
+
 int gpgmp_solver_set_state(gpgmp_solver *s, const int32_t *host) {
     return guarded([&] {
         GPGMP_REQUIRE(s && host, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(!s->det, GPGMP_E_STATE, "deterministic solvers hold fp32 concentrations");
         s->eng->set_state(0, host);
     });
 }
@@ -193,24 +209,31 @@ int gpgmp_solver_set_state(gpgmp_solver *s, const int32_t *host) {
 int gpgmp_solver_get_state(gpgmp_solver *s, int32_t *host) {
     return guarded([&] {
         GPGMP_REQUIRE(s && host, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(!s->det, GPGMP_E_STATE, "deterministic solvers hold fp32 concentrations");
         s->eng->get_state(0, host);
     });
 }
 
-int gpgmp_solver_set_state_f32(gpgmp_solver *, const float *) {
-    g_last_error = "stochastic solvers hold int32 counts";
-    return GPGMP_E_UNSUPPORTED;
+int gpgmp_solver_set_state_f32(gpgmp_solver *s, const float *host) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && host, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(s->det != nullptr, GPGMP_E_STATE, "stochastic solvers hold int32 counts");
+        s->det->set_state(host);
+    });
 }
 
-int gpgmp_solver_get_state_f32(gpgmp_solver *, float *) {
-    g_last_error = "stochastic solvers hold int32 counts";
-    return GPGMP_E_UNSUPPORTED;
+int gpgmp_solver_get_state_f32(gpgmp_solver *s, float *host) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && host, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(s->det != nullptr, GPGMP_E_STATE, "stochastic solvers hold int32 counts");
+        s->det->get_state(host);
+    });
 }
 
 int gpgmp_solver_state_device(gpgmp_solver *s, void **dev_ptr) {
     return guarded([&] {
         GPGMP_REQUIRE(s && dev_ptr, GPGMP_E_INVALID, "null argument");
-        *dev_ptr = s->eng->state();
+        *dev_ptr = s->det ? (void *)s->det->state() : (void *)s->eng->state();
     });
 }
 
@@ -236,6 +259,11 @@ static void ok(int rc) {
 int gpgmp_solver_do_reactions(gpgmp_solver *s, float dt, int32_t *seed) {
     return guarded([&] {
         GPGMP_REQUIRE(s && seed, GPGMP_E_INVALID, "null argument");
+        if (s->det) {  // Solver.cpp:250-266: the integrator; the seed is not used
+            s->det->do_reactions(dt);
+            GPGMP_HIP_CHECK(hipStreamSynchronize(s->stream.s));
+            return;
+        }
         if (s->m.table.R > 0) {
             LaunchPlan p{};
             p.deltat = dt;
@@ -274,10 +302,24 @@ int gpgmp_solver_attach_individuals(gpgmp_solver *s, int32_t species, gpgmp_indi
     });
 }
 
+int gpgmp_solver_individuals(gpgmp_solver *s, int32_t species, gpgmp_individuals **out) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && out, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(species >= 0 && species < s->m.desc.num_species, GPGMP_E_INVALID, "species out of range");
+        auto it = s->inds.find(species);
+        *out = (it == s->inds.end()) ? nullptr : it->second.g;
+    });
+}
+
 int gpgmp_solver_diffuse(gpgmp_solver *s, int32_t species, float t, int32_t *seed) {
     return guarded([&] {
         GPGMP_REQUIRE(s && seed, GPGMP_E_INVALID, "null argument");
         GPGMP_REQUIRE(species >= 0 && species < s->m.desc.num_species, GPGMP_E_INVALID, "species out of range");
+        if (s->det) {  // the deterministic stencils; the seed is not used
+            s->det->diffuse(species, t);
+            GPGMP_HIP_CHECK(hipStreamSynchronize(s->stream.s));
+            return;
+        }
         const uint32_t bit = 1u << species;
         if (!s->m.inhomogeneous()) {
             // homogeneous_stochastic_diffuse + updateState (+ sources) with the same seed
@@ -337,6 +379,11 @@ int gpgmp_solver_diffuse(gpgmp_solver *s, int32_t species, float t, int32_t *see
 int gpgmp_solver_next_diffusion_time(gpgmp_solver *s, float *out) {
     return guarded([&] {
         GPGMP_REQUIRE(s && out, GPGMP_E_INVALID, "null argument");
+        if (s->det) {
+            const std::vector<float> nt = s->det->next_diffusion_time();
+            std::copy(nt.begin(), nt.end(), out);
+            return;
+        }
         for (int i = 0; i < s->m.desc.num_species; ++i)
             out[i] = (!s->m.inhomogeneous()) ? s->nx[i] : ((s->nx[i] < s->ny[i]) ? s->nx[i] : s->ny[i]);
     });
@@ -355,6 +402,12 @@ int gpgmp_solver_time_steps(gpgmp_solver *s, float *dt_x, float *dt_y) {
 int gpgmp_solver_run(gpgmp_solver *s, int32_t steps, uint32_t seed0, float *sim_time_out) {
     return guarded([&] {
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        if (s->det) {  // no seeds in the deterministic loop
+            s->det->reset();
+            s->det->run(steps);
+            if (sim_time_out) *sim_time_out = s->det->sim_time();
+            return;
+        }
         GPGMP_REQUIRE(s->inds.empty(), GPGMP_E_UNSUPPORTED,
                       "models with individual species run through the per-call API (do_reactions / diffuse)");
         s->runner.reset();
@@ -375,14 +428,30 @@ int gpgmp_solver_run(gpgmp_solver *s, int32_t steps, uint32_t seed0, float *sim_
 int gpgmp_solver_run_continue(gpgmp_solver *s, int32_t steps, int32_t graph_chunk, float *sim_time_out) {
     return guarded([&] {
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        if (s->det) {
+            s->det->run(steps);
+            if (sim_time_out) *sim_time_out = s->det->sim_time();
+            return;
+        }
         GPGMP_REQUIRE(s->runner != nullptr, GPGMP_E_STATE, "no run to continue");
         s->runner->run(steps, graph_chunk);
         if (sim_time_out) *sim_time_out = s->runner->sim_time();
     });
 }
 
+int gpgmp_solver_skip(gpgmp_solver *s, int32_t steps, float *sim_time_out) {
+    return guarded([&] {
+        (void)stoch(s);
+        GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(s->runner != nullptr, GPGMP_E_STATE, "no run to continue");
+        s->runner->skip(steps);
+        if (sim_time_out) *sim_time_out = s->runner->sim_time();
+    });
+}
+
 int gpgmp_solver_prepare_run(gpgmp_solver *s, int32_t steps, int32_t graph_chunk) {
     return guarded([&] {
+        (void)stoch(s);
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
         GPGMP_REQUIRE(s->runner != nullptr, GPGMP_E_STATE, "no run to continue");
         s->runner->prepare(steps, graph_chunk);
@@ -391,6 +460,7 @@ int gpgmp_solver_prepare_run(gpgmp_solver *s, int32_t steps, int32_t graph_chunk
 
 int gpgmp_solver_set_output(gpgmp_solver *s, float interval, float max_time, int32_t max_dumps) {
     return guarded([&] {
+        (void)stoch(s);
         GPGMP_REQUIRE(s && max_dumps >= 0, GPGMP_E_INVALID, "bad argument");
         GPGMP_REQUIRE(!std::isnan(interval) && !std::isnan(max_time), GPGMP_E_INVALID, "NaN interval or max_time");
         s->out_interval = interval > 0.f ? interval : __builtin_inff();
@@ -405,6 +475,7 @@ int gpgmp_solver_set_output(gpgmp_solver *s, float interval, float max_time, int
 int gpgmp_solver_get_output(gpgmp_solver *s, int32_t *counts, float *times, int32_t *n_stored, int32_t *n_total,
                             int64_t *iterations) {
     return guarded([&] {
+        (void)stoch(s);
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
         const int32_t total = (int32_t)s->dump_times.size();
         const int32_t stored = total < s->max_dumps ? total : s->max_dumps;
@@ -421,6 +492,7 @@ int gpgmp_solver_get_output(gpgmp_solver *s, int32_t *counts, float *times, int3
 
 int gpgmp_solver_set_predict_busy(gpgmp_solver *s, int32_t on) {
     return guarded([&] {
+        (void)stoch(s);
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
         GPGMP_HIP_CHECK(hipStreamSynchronize(s->eng->stream()));
         s->eng->set_predict(on != 0);
@@ -429,6 +501,7 @@ int gpgmp_solver_set_predict_busy(gpgmp_solver *s, int32_t on) {
 
 int gpgmp_solver_set_sweep_spread(gpgmp_solver *s, int32_t on) {
     return guarded([&] {
+        (void)stoch(s);
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
         GPGMP_HIP_CHECK(hipStreamSynchronize(s->eng->stream()));
         s->eng->set_spread(on != 0);
@@ -437,6 +510,7 @@ int gpgmp_solver_set_sweep_spread(gpgmp_solver *s, int32_t on) {
 
 int gpgmp_solver_set_sweep_wave_cells(gpgmp_solver *s, int32_t cells) {
     return guarded([&] {
+        (void)stoch(s);
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
         GPGMP_HIP_CHECK(hipStreamSynchronize(s->eng->stream()));
         s->eng->set_sweep_wave_cells(cells);
@@ -445,6 +519,7 @@ int gpgmp_solver_set_sweep_wave_cells(gpgmp_solver *s, int32_t cells) {
 
 int gpgmp_solver_set_launch(gpgmp_solver *s, int32_t block, int32_t cpt) {
     return guarded([&] {
+        (void)stoch(s);
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
         s->eng->set_launch(block, cpt);
     });
@@ -452,6 +527,7 @@ int gpgmp_solver_set_launch(gpgmp_solver *s, int32_t block, int32_t cpt) {
 
 int gpgmp_solver_set_lanes_per_cell(gpgmp_solver *s, int32_t ssa_lanes, int32_t sweep_lanes) {
     return guarded([&] {
+        (void)stoch(s);
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
         s->eng->set_launch(s->eng->block(), s->eng->cpt(), ssa_lanes, sweep_lanes);
     });
@@ -459,6 +535,7 @@ int gpgmp_solver_set_lanes_per_cell(gpgmp_solver *s, int32_t ssa_lanes, int32_t 
 
 int gpgmp_solver_get_launch(gpgmp_solver *s, int32_t *block, int32_t *cpt, int32_t *ssa_lanes, int32_t *sweep_lanes) {
     return guarded([&] {
+        (void)stoch(s);
         GPGMP_REQUIRE(s && block && cpt && ssa_lanes && sweep_lanes, GPGMP_E_INVALID, "null argument");
         *block = s->eng->block();
         *cpt = s->eng->cpt();
@@ -469,6 +546,7 @@ int gpgmp_solver_get_launch(gpgmp_solver *s, int32_t *block, int32_t *cpt, int32
 
 int gpgmp_solver_set_event_budget(gpgmp_solver *s, uint32_t budget, int32_t below_active) {
     return guarded([&] {
+        (void)stoch(s);
         GPGMP_REQUIRE(s && budget >= 1 && below_active >= 0 && below_active <= 65, GPGMP_E_INVALID,
                       "bad argument");
         GPGMP_HIP_CHECK(hipStreamSynchronize(s->stream.s));
@@ -478,6 +556,7 @@ int gpgmp_solver_set_event_budget(gpgmp_solver *s, uint32_t budget, int32_t belo
 
 int gpgmp_solver_set_hot_workers(gpgmp_solver *s, float lane_events, int32_t lane_waves, int32_t wave_waves) {
     return guarded([&] {
+        (void)stoch(s);
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
         GPGMP_HIP_CHECK(hipStreamSynchronize(s->stream.s));
         s->eng->set_hot_workers(lane_events, lane_waves, wave_waves);
@@ -486,6 +565,7 @@ int gpgmp_solver_set_hot_workers(gpgmp_solver *s, float lane_events, int32_t lan
 
 int gpgmp_solver_kernel_times(gpgmp_solver *s, int32_t on, double *ms, int64_t *n, double *bytes) {
     return guarded([&] {
+        (void)stoch(s);
         GPGMP_REQUIRE(s, GPGMP_E_INVALID, "null argument");
         GPGMP_REQUIRE(s->runner != nullptr, GPGMP_E_STATE, "no run");
         if (ms && n) s->runner->kernel_times(ms, n, bytes);

@@ -410,6 +410,15 @@ struct RddK {
 
 constexpr int RT_X = 64, RT_Y = 16;
 
+// rate constants k1..k4 from a device array (rates set on the device by the caller, so the
+// host never waits for them)
+__device__ __forceinline__ void rdd_rates(RddK &p, const float *kdev) {
+    p.k1 = kdev[0];
+    p.k2 = kdev[1];
+    p.k3 = kdev[2];
+    p.k4 = kdev[3];
+}
+
 // mirror index for g in [-(n-1), 2(n-1)]; halo cells of a partial tile that lie further
 // out feed no in-grid result and are only clamped to a valid address
 __device__ __forceinline__ int reflect(int g, int n) {
@@ -434,7 +443,9 @@ template <int NS>
 __global__ __launch_bounds__(256) void k_rdd_step(const float *__restrict__ u, const float *__restrict__ v,
                                                   const float *__restrict__ D, float *__restrict__ umid,
                                                   float *__restrict__ vmid, float *__restrict__ un,
-                                                  float *__restrict__ vn, int W, int H, RddK p) {
+                                                  float *__restrict__ vn, int W, int H, RddK p,
+                                                  const float *__restrict__ kdev) {
+    if (kdev) rdd_rates(p, kdev);
     constexpr int LX = RT_X + 2 * NS, LY = RT_Y + 2 * NS;
     __shared__ float sd[LY][LX];
     __shared__ float su[2][LY][LX];
@@ -554,7 +565,8 @@ __global__ __launch_bounds__(256) void k_rdd_adjoint(
     const float *__restrict__ lu, const float *__restrict__ lv, const float *__restrict__ u1,
     const float *__restrict__ v1, const float *__restrict__ u0, const float *__restrict__ v0,
     const float *__restrict__ D, float *__restrict__ lun, float *__restrict__ lvn, float *__restrict__ gD,
-    double *__restrict__ gk_tiles, int W, int H, RddK p) {
+    double *__restrict__ gk_tiles, int W, int H, RddK p, const float *__restrict__ kdev) {
+    if (kdev) rdd_rates(p, kdev);
     constexpr int LX = RT_X + 2 * NS, LY = RT_Y + 2 * NS;
     constexpr int MX = RT_X + 2, MY = RT_Y + 2;  // ring of the intermediate lambda
     __shared__ float slu[LY][LX];
@@ -775,11 +787,11 @@ hipError_t launch_rdd_step(const float *u, const float *v, const float *D, float
     const RddK k = make_k(c);
     const dim3 g = rdd_grid(W, H), b(256);
     if (ns == 2) {
-        if (e0) hipExtLaunchKernelGGL(k_rdd_step<2>, g, b, 0, s, e0, e1, 0, u, v, D, umid, vmid, un, vn, W, H, k);
-        else hipLaunchKernelGGL(k_rdd_step<2>, g, b, 0, s, u, v, D, umid, vmid, un, vn, W, H, k);
+        if (e0) hipExtLaunchKernelGGL(k_rdd_step<2>, g, b, 0, s, e0, e1, 0, u, v, D, umid, vmid, un, vn, W, H, k, c.kdev);
+        else hipLaunchKernelGGL(k_rdd_step<2>, g, b, 0, s, u, v, D, umid, vmid, un, vn, W, H, k, c.kdev);
     } else {
-        if (e0) hipExtLaunchKernelGGL(k_rdd_step<1>, g, b, 0, s, e0, e1, 0, u, v, D, umid, vmid, un, vn, W, H, k);
-        else hipLaunchKernelGGL(k_rdd_step<1>, g, b, 0, s, u, v, D, umid, vmid, un, vn, W, H, k);
+        if (e0) hipExtLaunchKernelGGL(k_rdd_step<1>, g, b, 0, s, e0, e1, 0, u, v, D, umid, vmid, un, vn, W, H, k, c.kdev);
+        else hipLaunchKernelGGL(k_rdd_step<1>, g, b, 0, s, u, v, D, umid, vmid, un, vn, W, H, k, c.kdev);
     }
     return hipGetLastError();
 }
@@ -790,9 +802,11 @@ hipError_t launch_rdd_adjoint(const float *lu, const float *lv, const float *u1,
     const RddK k = make_k(c);
     const dim3 g = rdd_grid(W, H), b(256);
     if (ns == 2)
-        hipLaunchKernelGGL(k_rdd_adjoint<2>, g, b, 0, s, lu, lv, u1, v1, u0, v0, D, lun, lvn, gD, gk_tiles, W, H, k);
+        hipLaunchKernelGGL(k_rdd_adjoint<2>, g, b, 0, s, lu, lv, u1, v1, u0, v0, D, lun, lvn, gD, gk_tiles, W, H, k,
+                           c.kdev);
     else
-        hipLaunchKernelGGL(k_rdd_adjoint<1>, g, b, 0, s, lu, lv, u1, v1, u0, v0, D, lun, lvn, gD, gk_tiles, W, H, k);
+        hipLaunchKernelGGL(k_rdd_adjoint<1>, g, b, 0, s, lu, lv, u1, v1, u0, v0, D, lun, lvn, gD, gk_tiles, W, H, k,
+                           c.kdev);
     return hipGetLastError();
 }
 

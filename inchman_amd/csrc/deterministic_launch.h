@@ -11,6 +11,7 @@ namespace gpgmp {
 struct RddConsts {
     float dt, cu, cv;  // c_s = D0_s / h^2
     float k[4];
+    const float *kdev = nullptr;  // non-null: the kernels read k1..k4 from this device array instead
 };
 
 hipError_t launch_det_hom_diffuse(const float *in, float *out, int W, int H, float coef, hipStream_t s);

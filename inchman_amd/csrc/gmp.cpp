@@ -704,6 +704,20 @@ void GmpRunner::prepare(int steps, int graph_chunk) {
 
 void GmpRunner::run(int steps, int graph_chunk) { run_impl(steps, graph_chunk, false); }
 
+// Advances the loop by `steps` iterations without launching anything: the schedule, the
+// seed streams and the iteration count move on as if they had run, and nothing is pending
+// (the caller's state is the state after them, e.g. a snapshot of an earlier run).
+void GmpRunner::skip(int steps) {
+    GPGMP_REQUIRE(steps >= 0, GPGMP_E_INVALID, "steps must be non-negative");
+    GPGMP_REQUIRE(!has_prev_, GPGMP_E_STATE, "outflow of a run is pending");
+    for (int j = 0; j < steps && !sched_.finished(); ++j) {
+        (void)sched_.next();
+        for (size_t r = 0; r < seeds_.size(); ++r) prev_seed_[r] = seeds_[r].next();
+        ++plan_step_;
+        ++iterations_;
+    }
+}
+
 void GmpRunner::run_impl(int steps, int graph_chunk, bool dry) {
     GPGMP_REQUIRE(steps >= 0, GPGMP_E_INVALID, "steps must be non-negative");
     const int nreal = eng_.num_real();

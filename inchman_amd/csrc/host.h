@@ -288,6 +288,8 @@ class GmpRunner {
     // Captures the HIP graphs run(steps, graph_chunk) would replay from the current state of
     // the loop, without running anything (the loop's state is unchanged).
     void prepare(int steps, int graph_chunk);
+    // Advances the loop by `steps` iterations without running them (nothing pending after).
+    void skip(int steps);
     float sim_time() const { return sched_.sim_time(); }
     // Output events and the max_time stop of the reference loop; `dump` is called, after
     // the launches of each output iteration that writes (the state is then fully gathered),

@@ -40,8 +40,10 @@ struct gpgmp_rdd {
     DevBuf<double> gk_tiles; // per-tile dL/dk accumulators
     int ntiles = 0;
     bool saved = false;
+    uint64_t generation = 0;  // saving forward runs so far; the last one is what backward differentiates
     int ns = 2;  // steps per launch (temporal blocking)
     RddConsts c{};
+    DevBuf<float> kdev;  // rates set on the device (gpgmp_rdd_set_rates_device)
     ~gpgmp_rdd() {
         (void)hipStreamSynchronize(stream);
     }
@@ -120,6 +122,7 @@ int gpgmp_rdd_create(const gpgmp_rdd_params *p, int32_t steps, int32_t ckpt_ever
         r->ntiles = rdd_num_tiles(p->width, p->height);
         r->gk_tiles.alloc((size_t)r->ntiles * 4);
         r->c = rdd_consts(*p);
+        r->c.kdev = nullptr;
         r->ns = (p->width >= 3 && p->height >= 3) ? 2 : 1;
         *out = r.release();
     });
@@ -136,6 +139,23 @@ int gpgmp_rdd_set_rates(gpgmp_rdd *r, const float *k) {
         for (int i = 0; i < 4; ++i) r->p.k[i] = k[i];
         r->c = rdd_consts(r->p);
         r->saved = false;
+    });
+}
+
+int gpgmp_rdd_set_rates_device(gpgmp_rdd *r, const float *k_dev) {
+    return guarded_det([&] {
+        GPGMP_REQUIRE(r && k_dev, GPGMP_E_INVALID, "null argument");
+        if (!r->kdev.get()) r->kdev.alloc(4);
+        GPGMP_HIP_CHECK(hipMemcpyAsync(r->kdev.get(), k_dev, 4 * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
+        r->c.kdev = r->kdev.get();
+        r->saved = false;
+    });
+}
+
+int gpgmp_rdd_generation(gpgmp_rdd *r, uint64_t *generation) {
+    return guarded_det([&] {
+        GPGMP_REQUIRE(r && generation, GPGMP_E_INVALID, "null argument");
+        *generation = r->saved ? r->generation : 0;
     });
 }
 
@@ -176,13 +196,17 @@ int gpgmp_rdd_forward(gpgmp_rdd *r, const float *D, float *u, float *v, int32_t 
             GPGMP_HIP_CHECK(hipMemcpyAsync(v, bv[1], A * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
         }
         r->saved = save != 0;
+        if (save) ++r->generation;
     });
 }
 
-int gpgmp_rdd_backward(gpgmp_rdd *r, const float *D, float *lu, float *lv, float *gD, float *gk) {
+int gpgmp_rdd_backward(gpgmp_rdd *r, uint64_t generation, const float *D, float *lu, float *lv, float *gD,
+                       float *gk) {
     return guarded_det([&] {
         GPGMP_REQUIRE(r && D && lu && lv && gD && gk, GPGMP_E_INVALID, "null argument");
         GPGMP_REQUIRE(r->saved, GPGMP_E_STATE, "backward needs a forward run with save != 0");
+        GPGMP_REQUIRE(generation == 0 || generation == r->generation, GPGMP_E_STATE,
+                      "the saved forward is not the one asked for (another forward ran since)");
         const int W = r->p.width, H = r->p.height;
         const size_t A = r->A;
         GPGMP_HIP_CHECK(launch_fill_f32(gD, A, 0.f, r->stream));

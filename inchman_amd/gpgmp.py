@@ -25,6 +25,15 @@ from ._lib import call, lib
 
 STOCHASTIC_HOMOGENEOUS = _lib.STOCHASTIC_HOMOGENEOUS
 STOCHASTIC_INHOMOGENEOUS = _lib.STOCHASTIC_INHOMOGENEOUS
+DETERMINISTIC_HOMOGENEOUS_RK4 = _lib.DETERMINISTIC_HOMOGENEOUS_RK4
+DETERMINISTIC_INHOMOGENEOUS_RK4 = _lib.DETERMINISTIC_INHOMOGENEOUS_RK4
+DETERMINISTIC_HOMOGENEOUS_AQSS = _lib.DETERMINISTIC_HOMOGENEOUS_AQSS
+DETERMINISTIC_INHOMOGENEOUS_AQSS = _lib.DETERMINISTIC_INHOMOGENEOUS_AQSS
+DETERMINISTIC_TYPES = (DETERMINISTIC_HOMOGENEOUS_RK4, DETERMINISTIC_INHOMOGENEOUS_RK4,
+                       DETERMINISTIC_HOMOGENEOUS_AQSS, DETERMINISTIC_INHOMOGENEOUS_AQSS)
+# gpgmp_solver_create flags
+SOLVER_AQSS_PADE_FIX = 1
+SOLVER_REFERENCE_QUIRKS = 2
 
 
 def _fptr(a):
@@ -135,9 +144,11 @@ class DiffusionModel:
         call("gpgmp_model_create", C.byref(d), C.byref(h))
         return CompiledModel(h, self)
 
-    def createSolver(self, stream=None):
+    def createSolver(self, stream=None, flags=0):
         """Factory of DiffusionModel::oclSolve (DiffusionModel.cpp:1422-1440)."""
         cm = self.compile()
+        if self.solver in DETERMINISTIC_TYPES:
+            return DeterministicSolver(cm, stream, flags)
         cls = HomogeneousSolver if self.solver == STOCHASTIC_HOMOGENEOUS else InhomogeneousSolver
         return cls(cm, stream)
 
@@ -175,11 +186,11 @@ def _stream_ptr(stream):
 class Solver:
     """gpgmp::Solver (Solver.h:15-239) over the HIP library."""
 
-    def __init__(self, compiled, stream=None):
+    def __init__(self, compiled, stream=None, flags=0):
         self._cm = compiled
         self.model = compiled.model
         h = C.c_void_p()
-        call("gpgmp_solver_create", compiled.handle, _stream_ptr(stream), 0, C.byref(h))
+        call("gpgmp_solver_create", compiled.handle, _stream_ptr(stream), int(flags), C.byref(h))
         self.handle = h
 
     def close(self):
@@ -254,6 +265,13 @@ class Solver:
     def prepareRun(self, steps, graph_chunk):
         """Captures the HIP graphs runContinue(steps, graph_chunk) will replay (runs nothing)."""
         call("gpgmp_solver_prepare_run", self.handle, int(steps), int(graph_chunk))
+
+    def skip(self, steps):
+        """Advances the loop by `steps` iterations without running them (the state set on the
+        device must be the state after them); returns the simulation time."""
+        t = C.c_float()
+        call("gpgmp_solver_skip", self.handle, int(steps), C.byref(t))
+        return t.value
 
     def attachIndividualSpecies(self, species, individuals, rule=None):
         """Makes `species` particle-based: `individuals` (an IndividualSpecies on this grid with
@@ -347,6 +365,21 @@ class HomogeneousSolver(Solver):
 
 class InhomogeneousSolver(Solver):
     """InhomogeneousSolver (InhomogeneousSolver.h:15-162)."""
+
+
+class DeterministicSolver(Solver):
+    """The deterministic solver types (Solver(..., detState, ..., deterministic=true)): fp32
+    concentrations [S][H][W]; doReactions runs RK4 or alpha-QSS, diffuse the reference's
+    stencils; neither uses the seed (returned unchanged)."""
+
+    def synchronizeStateBufferToDevice(self, state):
+        a = np.ascontiguousarray(state, dtype=np.float32).reshape(self.shape)
+        call("gpgmp_solver_set_state_f32", self.handle, _fptr(a))
+
+    def synchronizeStateBufferFromDevice(self):
+        a = np.empty(self.shape, np.float32)
+        call("gpgmp_solver_get_state_f32", self.handle, _fptr(a))
+        return a
 
 
 class Ensemble:

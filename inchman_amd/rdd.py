@@ -55,6 +55,17 @@ class ReactionDiffusion:
         kk = (C.c_float * 4)(*[float(x) for x in k])
         call("gpgmp_rdd_set_rates", self.handle, kk)
 
+    def set_rates_device(self, k):
+        """Rates k1..k4 from a 4-element fp32 CUDA tensor, copied on the stream (no host wait)."""
+        assert k.is_cuda and k.dtype == torch.float32 and k.is_contiguous() and k.numel() == 4
+        call("gpgmp_rdd_set_rates_device", self.handle, k.data_ptr())
+
+    def generation(self):
+        """Generation of the saved forward run (0: none)."""
+        g = C.c_uint64()
+        call("gpgmp_rdd_generation", self.handle, C.byref(g))
+        return g.value
+
     def set_steps_per_launch(self, ns):
         """1 or 2 Euler steps per kernel launch (results are bit-identical)."""
         call("gpgmp_rdd_set_steps_per_launch", self.handle, int(ns))
@@ -65,36 +76,44 @@ class ReactionDiffusion:
         call("gpgmp_rdd_forward", self.handle, D.data_ptr(), u.data_ptr(), v.data_ptr(), int(save))
         return u, v
 
-    def backward(self, D, lu, lv):
-        """Adjoint of the last saved forward; lu, lv (in place): dL/du_T -> dL/du_0."""
+    def backward(self, D, lu, lv, generation=0):
+        """Adjoint of the last saved forward; lu, lv (in place): dL/du_T -> dL/du_0.  A
+        non-zero generation must be that forward's (else GpgmpError: another forward ran)."""
         self._check(D, lu, lv)
         gD = torch.empty_like(D)
         gk = torch.empty(4, dtype=torch.float32, device=D.device)
-        call("gpgmp_rdd_backward", self.handle, D.data_ptr(), lu.data_ptr(), lv.data_ptr(), gD.data_ptr(),
-             gk.data_ptr())
+        call("gpgmp_rdd_backward", self.handle, int(generation), D.data_ptr(), lu.data_ptr(), lv.data_ptr(),
+             gD.data_ptr(), gk.data_ptr())
         return gD, gk, lu, lv
 
 
 class RDDFunction(torch.autograd.Function):
-    """(D, k, u0, v0) -> (u_T, v_T); k is a 4-element tensor of rate constants."""
+    """(D, k, u0, v0) -> (u_T, v_T); k is a 4-element tensor of rate constants.
+
+    The rates go to the solver as a device tensor (no host synchronisation).  The handle
+    keeps the checkpoints of one forward only: backward checks that the handle's saved
+    forward is still this one (another forward on the same handle before this backward
+    raises instead of differentiating the wrong run)."""
 
     @staticmethod
     def forward(ctx, D, k, u0, v0, solver):
-        solver.set_rates(k.detach().cpu().tolist())
+        kd = k.detach().to(device=D.device, dtype=torch.float32).contiguous()
+        solver.set_rates_device(kd)
         Dc = D.detach().contiguous()
         u = u0.detach().clone().contiguous()
         v = v0.detach().clone().contiguous()
         solver.forward(Dc, u, v, save=True)
         ctx.solver = solver
-        ctx.save_for_backward(Dc, k)
+        ctx.generation = solver.generation()
+        ctx.save_for_backward(Dc, k, kd)
         return u, v
 
     @staticmethod
     def backward(ctx, gu, gv):
-        Dc, k = ctx.saved_tensors
+        Dc, k, _ = ctx.saved_tensors
         lu = (gu if gu is not None else torch.zeros_like(Dc)).detach().clone().contiguous()
         lv = (gv if gv is not None else torch.zeros_like(Dc)).detach().clone().contiguous()
-        gD, gk, gu0, gv0 = ctx.solver.backward(Dc, lu, lv)
+        gD, gk, gu0, gv0 = ctx.solver.backward(Dc, lu, lv, generation=ctx.generation)
         return gD, gk.to(k.device, k.dtype), gu0, gv0, None
 
 

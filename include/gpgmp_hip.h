@@ -107,14 +107,36 @@ int gpgmp_model_destroy(gpgmp_model *m);
 int gpgmp_model_rate_literals(const gpgmp_model *m, int32_t *is_int, double *value);
 
 /* ---- plain solver (gpgmp::Solver and subclasses) --------------------------------- */
+/* Flags of gpgmp_solver_create. */
+/* deterministic alpha-QSS types: the predictor's Pade factor in the corrector too
+ * (see gpgmp_det_aqss_device) */
+#define GPGMP_SOLVER_AQSS_PADE_FIX 1u
+/* deterministic types: reproduce the reference where it departs from the model: the
+ * inhomogeneous deterministic types run no reactions (Solver.cpp:250-266 reacts for the
+ * homogeneous types only) and a homogeneous diffuse(species) call moves every species
+ * (homogeneous_deterministic_diffuse, Deterministic.cl:32, called once per due species by
+ * the host loop).  Without it, every deterministic type reacts and diffuse moves the one
+ * species asked for. */
+#define GPGMP_SOLVER_REFERENCE_QUIRKS 2u
 /* Solver::Solver + HomogeneousSolver/InhomogeneousSolver constructors
- * (Solver.cpp:18-172, HomogeneousSolver.cpp:8-125, InhomogeneousSolver.cpp:10-213). */
+ * (Solver.cpp:18-172, HomogeneousSolver.cpp:8-125, InhomogeneousSolver.cpp:10-213), for every
+ * solver type.  Stochastic types hold int32 counts (gpgmp_solver_set_state/get_state), the
+ * deterministic types fp32 concentrations (the reference's detState,
+ * gpgmp_solver_set_state_f32/get_state_f32): doReactions runs RK4 or alpha-QSS over dt
+ * (Solver.cpp:250-266), diffuse the reference's stencils (Jacobi) with its time steps --
+ * homogeneous dt = min_s 1e-2 h^2 / (4 D_s) for every species (HomogeneousSolver.cpp:97-121),
+ * inhomogeneous the per-cell minimum of the stochastic types per axis
+ * (InhomogeneousSolver.cpp:189-212, 282-286, 374-378) -- and neither touches the seed.
+ * flags: GPGMP_SOLVER_* (others: GPGMP_E_INVALID). */
 int gpgmp_solver_create(const gpgmp_model *m, void *stream, uint32_t flags, gpgmp_solver **out);
 int gpgmp_solver_destroy(gpgmp_solver *s);
 /* Solver::synchronizeStateBufferToDevice / FromDevice (Solver.h:85-90). */
 int gpgmp_solver_set_state(gpgmp_solver *s, const int32_t *host_counts);
 int gpgmp_solver_get_state(gpgmp_solver *s, int32_t *host_counts);
-int gpgmp_solver_set_state_f32(gpgmp_solver *s, const float *host_conc);   /* deterministic */
+/* The same for the deterministic types' fp32 concentrations [S][H][W] (Solver.h:85-90 with
+ * the detState vector); GPGMP_E_STATE on a stochastic solver (and the int32 calls on a
+ * deterministic one). */
+int gpgmp_solver_set_state_f32(gpgmp_solver *s, const float *host_conc);
 int gpgmp_solver_get_state_f32(gpgmp_solver *s, float *host_conc);
 /* Device-resident state buffer (int32 or float, [S][H][W]); valid until destroy. */
 int gpgmp_solver_state_device(gpgmp_solver *s, void **dev_ptr);
@@ -137,6 +159,10 @@ int gpgmp_solver_next_diffusion_time(gpgmp_solver *s, float *out);
  * per-call API (gpgmp_solver_run refuses them), synchronising once per call. */
 int gpgmp_solver_attach_individuals(gpgmp_solver *s, int32_t species, gpgmp_individuals *g,
                                     const gpgmp_new_individual_rule *rule);
+/* Solver::getIndividualSolver() (Solver.h:109, used by writeAllSpecies,
+ * DiffusionModel.cpp:1186-1250): the handle attached to `species` by
+ * gpgmp_solver_attach_individuals, or NULL. */
+int gpgmp_solver_individuals(gpgmp_solver *s, int32_t species, gpgmp_individuals **out);
 /* Per-species time steps: homogeneous dt[s] (both arrays), inhomogeneous dtX/dtY. */
 int gpgmp_solver_time_steps(gpgmp_solver *s, float *dt_x, float *dt_y);
 /* DiffusionModel::oclSolve main loop (DiffusionModel.cpp:1467-1603) for `steps`
@@ -158,6 +184,13 @@ int gpgmp_solver_get_output(gpgmp_solver *s, int32_t *counts, float *times, int3
 /* Continues the loop of the last gpgmp_solver_run for `steps` more iterations;
  * graph_chunk > 0 replays chunks of that many iterations as a HIP graph. */
 int gpgmp_solver_run_continue(gpgmp_solver *s, int32_t steps, int32_t graph_chunk, float *sim_time_out);
+/* Advances the loop of the last gpgmp_solver_run by `steps` iterations without running
+ * them: the schedule, the seed stream (seed_n of DiffusionModel.cpp:1482) and the iteration
+ * count move on, and the state is left as the caller set it -- it must be the state after
+ * those iterations (a snapshot of an earlier run), so that gpgmp_solver_run_continue resumes
+ * the loop from there.  Skipped iterations write no output.  No counterpart in the
+ * reference. */
+int gpgmp_solver_skip(gpgmp_solver *s, int32_t steps, float *sim_time_out);
 /* Captures the HIP graphs that gpgmp_solver_run_continue(s, steps, graph_chunk) will replay,
  * without running any step (the loop, state and seeds are unchanged), so that a following
  * run of a few steps replays graphs instead of capturing them while it runs.  Graphs are
@@ -268,6 +301,13 @@ int gpgmp_rdd_create(const gpgmp_rdd_params *p, int32_t steps, int32_t ckpt_ever
 int gpgmp_rdd_destroy(gpgmp_rdd *r);
 /* Replaces the rate constants k1..k4 (host floats) for the following forward runs. */
 int gpgmp_rdd_set_rates(gpgmp_rdd *r, const float *k);
+/* The same from a device array k_dev[4], copied on the handle's stream: the host never waits
+ * for the rates (a torch tensor of rates stays on the GPU); the kernels read them from the
+ * handle's copy until gpgmp_rdd_set_rates is called again. */
+int gpgmp_rdd_set_rates_device(gpgmp_rdd *r, const float *k_dev);
+/* Generation of the saved forward run (1, 2, ... for each forward with save != 0; 0 when no
+ * saved forward exists or the rates changed since). */
+int gpgmp_rdd_generation(gpgmp_rdd *r, uint64_t *generation);
 /* Euler steps per kernel launch, 1 or 2 (default 2 when both sides are >= 3): two steps
  * per launch keep the intermediate state in LDS (2-cell halo) and give results
  * bit-identical to one step per launch.  No counterpart in the reference (a launch
@@ -278,8 +318,11 @@ int gpgmp_rdd_set_steps_per_launch(gpgmp_rdd *r, int32_t ns);
 int gpgmp_rdd_forward(gpgmp_rdd *r, const float *D, float *u, float *v, int32_t save);
 /* Adjoint of the last saved forward: lu/lv (device) carry dL/du_T, dL/dv_T in and
  * dL/du_0, dL/dv_0 out; outputs gD = dL/dD (device, [H][W]) and gk = dL/dk1..k4
- * (device, 4 floats). */
-int gpgmp_rdd_backward(gpgmp_rdd *r, const float *D, float *lu, float *lv, float *gD, float *gk);
+ * (device, 4 floats).  generation (from gpgmp_rdd_generation after that forward) must be
+ * the saved forward's, else GPGMP_E_STATE (a later forward replaced its checkpoints);
+ * 0 skips the check. */
+int gpgmp_rdd_backward(gpgmp_rdd *r, uint64_t generation, const float *D, float *lu, float *lv, float *gD,
+                       float *gk);
 
 /* Deterministic stencils and RK4 of Deterministic.cl on device fp32 buffers, Jacobi
  * (in != out): homogeneous_deterministic_diffuse (Deterministic.cl:17-65) for every

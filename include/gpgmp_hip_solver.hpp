@@ -36,17 +36,22 @@ typedef float Real;
 
 class HipSolver {
   public:
-    // Solver(DiffusionModel&, std::vector<int>* runState, ...) (Solver.h:33-36): the
-    // host state vector stays caller-owned; synchronize* copy it to/from the device.
-    HipSolver(const gpgmp_model_desc &desc, std::vector<int> *runState, void *stream = nullptr)
-        : m_runState(runState), m_numSpecies(desc.num_species) {
+    // Solver(DiffusionModel&, std::vector<int>* runState, std::vector<Real>* detState, ...)
+    // (Solver.h:33-36): the host state vectors stay caller-owned; synchronize* copy the one
+    // of the solver type (int32 counts for the stochastic types, fp32 concentrations for
+    // the deterministic ones) to/from the device.  flags: GPGMP_SOLVER_*.
+    HipSolver(const gpgmp_model_desc &desc, std::vector<int> *runState, std::vector<Real> *detState = nullptr,
+              void *stream = nullptr, uint32_t flags = 0u)
+        : m_runState(runState), m_detState(detState), m_numSpecies(desc.num_species),
+          m_deterministic(desc.solver_type >= GPGMP_DETERMINISTIC_HOMOGENEOUS_RK4) {
         check(gpgmp_model_create(&desc, &m_model));
-        int rc = gpgmp_solver_create(m_model, stream, 0u, &m_solver);
+        int rc = gpgmp_solver_create(m_model, stream, flags, &m_solver);
         if (rc != GPGMP_OK) {
             gpgmp_model_destroy(m_model);
             check(rc);
         }
-        if (m_runState && !m_runState->empty()) synchronizeStateBufferToDevice();
+        if (!m_deterministic && m_runState && !m_runState->empty()) synchronizeStateBufferToDevice();
+        if (m_deterministic && m_detState && !m_detState->empty()) synchronizeStateBufferToDevice();
     }
     ~HipSolver() {
         gpgmp_solver_destroy(m_solver);
@@ -74,8 +79,21 @@ class HipSolver {
         return out;
     }
     // Solver::synchronizeStateBufferFromDevice / ToDevice (Solver.h:85-90)
-    void synchronizeStateBufferFromDevice() { check(gpgmp_solver_get_state(m_solver, m_runState->data())); }
-    void synchronizeStateBufferToDevice() { check(gpgmp_solver_set_state(m_solver, m_runState->data())); }
+    void synchronizeStateBufferFromDevice() {
+        if (m_deterministic) check(gpgmp_solver_get_state_f32(m_solver, m_detState->data()));
+        else check(gpgmp_solver_get_state(m_solver, m_runState->data()));
+    }
+    void synchronizeStateBufferToDevice() {
+        if (m_deterministic) check(gpgmp_solver_set_state_f32(m_solver, m_detState->data()));
+        else check(gpgmp_solver_set_state(m_solver, m_runState->data()));
+    }
+    // Solver::getIndividualSolver() (Solver.h:109): the particle handle of an individual
+    // species attached with gpgmp_solver_attach_individuals (one per species here), NULL if none
+    gpgmp_individuals *getIndividualSolver(int speciesIndex) {
+        gpgmp_individuals *g = nullptr;
+        check(gpgmp_solver_individuals(m_solver, speciesIndex, &g));
+        return g;
+    }
     // DiffusionModel::oclSolve's loop on the device (DiffusionModel.cpp:1467-1603)
     Real run(int steps, unsigned seed0) {
         float t = 0.f;
@@ -86,7 +104,9 @@ class HipSolver {
 
   private:
     std::vector<int> *m_runState;
+    std::vector<Real> *m_detState;
     int m_numSpecies;
+    bool m_deterministic;
     gpgmp_model *m_model = nullptr;
     gpgmp_solver *m_solver = nullptr;
 };

@@ -104,6 +104,10 @@ int oracle_run(const oracle_model *m, const oracle_reaction_table *tab, int32_t 
 int oracle_run_output(const oracle_model *m, const oracle_reaction_table *t, int32_t *state, int steps,
                       uint32_t seed0, int nthreads, float interval, float max_time, int32_t *dumps,
                       float *dump_times, int max_dumps, int64_t *iterations, float *sim_time_out);
+/* the loop from iteration `skip` on: the first skip iterations advance only the schedule and
+ * the seed stream, `state` being the state after them; then `steps` iterations run */
+int oracle_run_from(const oracle_model *m, const oracle_reaction_table *t, int32_t *state, int skip, int steps,
+                    uint32_t seed0, int nthreads, float *sim_time_out);
 
 /* ---- deterministic (FP64 Jacobi) ---- */
 void oracle_det_hom_diffuse(double *state, int W, int H, int S, const double *D, double dt,

@@ -92,6 +92,8 @@ def lib():
                                          C.c_float, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_int]
         L.oracle_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_int,
                                  C.c_void_p]
+        L.oracle_run_from.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_int,
+                                      C.c_void_p]
         L.oracle_rdd_grad.restype = C.c_double
         L.oracle_rdd_loss.restype = C.c_double
         L.oracle_particle_step.restype = C.c_int
@@ -263,30 +265,20 @@ def run_output(model, state, steps, seed0, interval, max_time=float("inf"), max_
     return st, sim.value, it.value, dumps[:k], times[:k], w
 
 
-def run(model, state, steps, seed0, nthreads=0):
+def run(model, state, steps, seed0, nthreads=0, skip=0):
     """The host loop (DiffusionModel::oclSolve) on the oracle; model is a
-    inchman_amd.gpgmp.DiffusionModel."""
+    inchman_amd.gpgmp.DiffusionModel.  skip > 0: `state` is the state after `skip`
+    iterations of the loop from seed0, which only advance the schedule and the seeds here,
+    and `steps` more iterations run."""
     st = np.ascontiguousarray(state, np.int32).copy()
-    S = model.numSpecies()
-    m = Model()
-    m.W, m.H, m.S = model.width, model.height, S
-    m.physical_length, m.p0 = model.physical_length, model.p0
-    m.inhomogeneous = 1 if model.solver == 1 else 0
-    diff = np.array([sp.diffusionConstant for sp in model.species], np.float32)
-    keep = [diff]
-    m.diffusion = diff.ctypes.data_as(C.POINTER(C.c_float))
-    if model.fields is not None:
-        for name, arr in zip(("Dx", "Dy", "rx", "ry"), model.fields):
-            a = np.ascontiguousarray(arr, np.float32)
-            keep.append(a)
-            setattr(m, name, a.ctypes.data_as(C.POINTER(C.c_float)))
-    for i in range(4):
-        m.boundary_mask[i] = model.boundary_mask[i]
-        m.source_mask[i] = model.source_mask[i]
-    m.max_time = model.max_time
+    m, keep = _model_struct(model)  # noqa: F841 (keeps the field arrays alive)
     t = table_from_model(model)
     sim = C.c_float()
-    lib().oracle_run(C.byref(m), C.byref(t), _p(st), int(steps), C.c_uint32(seed0), nthreads, C.byref(sim))
+    if skip:
+        lib().oracle_run_from(C.byref(m), C.byref(t), _p(st), int(skip), int(steps), C.c_uint32(seed0), nthreads,
+                              C.byref(sim))
+    else:
+        lib().oracle_run(C.byref(m), C.byref(t), _p(st), int(steps), C.c_uint32(seed0), nthreads, C.byref(sim))
     return st, sim.value
 
 
@@ -474,3 +466,80 @@ def det_aqss(state, table, dtg, dtype=np.float32, fixed=False):
     else:
         it = lib().oracle_det_aqss_f64(_p(st), W, H, C.byref(table), float(dtg), int(fixed))
     return st, it
+
+
+def det_rk4(state, table, dt):
+    """deterministic_performReaction_rk4 (Deterministic.cl:143-238) in double."""
+    st = np.ascontiguousarray(state, np.float64).copy()
+    S, H, W = st.shape
+    lib().oracle_det_rk4(_p(st), W, H, C.byref(table), C.c_double(dt))
+    return st
+
+
+def det_run(model, state, steps, flags=0):
+    """DiffusionModel::oclSolve's loop (DiffusionModel.cpp:1479-1603) with a deterministic
+    solver, in double: the multi-rate schedule of the stochastic loop with the deterministic
+    time steps (homogeneous: dt = min_s (float)1e-2 h^2/(4 D_s) for every species,
+    HomogeneousSolver.cpp:97-121; inhomogeneous: the per-cell minimum per axis,
+    InhomogeneousSolver.cpp:189-212), doReactions = RK4 or alpha-QSS over the iteration's dt
+    (Solver.cpp:250-266), diffuse = the Jacobi stencil of the due species and axes
+    (HomogeneousSolver.cpp:157-160, InhomogeneousSolver.cpp:282-286, 374-378).  flags bit 1:
+    the reference's own behaviour (no reactions for the inhomogeneous types; a homogeneous
+    diffuse call moves every species); bit 0: the alpha-QSS corrector with the predictor's
+    Pade factor.  Returns (state fp64 [S][H][W], simulation time as the float clock)."""
+    f32 = np.float32
+    st = np.ascontiguousarray(state, np.float64).copy()
+    S, H, W = st.shape
+    A = W * H
+    inh = model.solver in (3, 5)
+    aqss = model.solver in (4, 5)
+    quirks = bool(flags & 2)
+    tab = table_from_model(model)
+    R = len(model.reactions)
+    if not inh:
+        h = f32(model.physical_length) / f32(model.width)
+        dts = []
+        for sp in model.species:
+            D = f32(sp.diffusionConstant)
+            dts.append(f32(f32(f32(f32(1e-2) * h) * h) / f32(f32(4.0) * D)) if D > 0 else f32(model.max_time))
+        common = min(dts)
+        dtx = [common] * S
+        dty = dtx
+        D_h = np.array([sp.diffusionConstant for sp in model.species], np.float32).astype(np.float64)
+    else:
+        Dx, Dy, rx, ry = (np.ascontiguousarray(a, np.float32) for a in model.fields)
+        hx = header_float(model.physical_length / model.width)
+        hy = header_float(model.physical_length / model.height)
+        dtx = [f32(inhomogeneous_dt(Dx[s], rx[s], hx)) for s in range(S)]
+        dty = [f32(inhomogeneous_dt(Dy[s], ry[s], hx)) for s in range(S)]
+    nx, ny = list(dtx), list(dty)
+    sim = f32(0.0)
+    for _ in range(steps):
+        nxt = [nx[s] if not inh else min(nx[s], ny[s]) for s in range(S)]
+        mindt, minspec = nxt[0], 0
+        for i in range(1, S):
+            if nxt[i] < mindt:
+                mindt, minspec = nxt[i], i
+        deltat = f32(mindt - sim)
+        if R > 0 and deltat > 0 and not (inh and quirks):
+            if aqss:
+                st, _ = det_aqss(st, tab, float(deltat), np.float64, bool(flags & 1))
+            else:
+                st = det_rk4(st, tab, float(deltat))
+        for i in range(S):
+            if not (i == minspec or nxt[i] <= mindt) or not (nxt[i] > 0):
+                continue
+            if not inh:
+                for k in (range(S) if quirks else (i,)):
+                    if D_h[k] > 0:
+                        st[k:k + 1] = det_hom_diffuse(st[k:k + 1], D_h[k:k + 1], float(dtx[i]), float(h))
+                nx[i] = f32(nx[i] + dtx[i])
+            else:
+                if nx[i] <= mindt:
+                    st = det_inhom_sweep(st, i, 0, float(dtx[i]), Dx.astype(np.float64), rx.astype(np.float64), hx)
+                    nx[i] = f32(nx[i] + dtx[i])
+                if ny[i] <= mindt:
+                    st = det_inhom_sweep(st, i, 1, float(dty[i]), Dy.astype(np.float64), ry.astype(np.float64), hy)
+                    ny[i] = f32(ny[i] + dty[i])
+        sim = mindt
+    return st, float(sim)

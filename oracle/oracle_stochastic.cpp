@@ -422,10 +422,13 @@ extern "C" void oracle_inhom_sweep(int32_t *state, int W, int H, int species, in
 // (the next iteration, of length zero); the loop stops once simTime reaches max_time.  The
 // initial state is written first (:1442).  interval / max_time = infinity disables them.
 // dumps: [max_dumps][S][A]; returns the number of writes (stored or not).
-extern "C" int oracle_run_output(const oracle_model *m, const oracle_reaction_table *t, int32_t *state,
-                                 int steps, uint32_t seed0, int nthreads, float interval, float max_time,
-                                 int32_t *dumps, float *dump_times, int max_dumps, int64_t *iterations,
-                                 float *sim_time_out) {
+// skip: the first `skip` iterations advance the schedule and the seed stream only (the
+// state passed in is the state after them, e.g. a snapshot of an earlier run); the other
+// `steps` iterations run.
+static int run_loop(const oracle_model *m, const oracle_reaction_table *t, int32_t *state, int skip, int steps,
+                    uint32_t seed0, int nthreads, float interval, float max_time, int32_t *dumps, float *dump_times,
+                    int max_dumps, int64_t *iterations, float *sim_time_out) {
+    steps += skip;
     const int S = m->S, A = m->W * m->H;
     const float h = oracle_header_float(m->physical_length / (float)m->W);
     const float hy = oracle_header_float(m->physical_length / (float)m->H);
@@ -481,8 +484,9 @@ extern "C" int oracle_run_output(const oracle_model *m, const oracle_reaction_ta
         const bool outputEvent = mindtOutput < mindt;
         if (outputEvent) mindt = mindtOutput;
         const float deltat = mindt - simTime;
+        const bool run = n >= skip;
         if (t->num_reactions > 0) {
-            oracle_ssa(state, m->W, m->H, t, seed, deltat, nthreads);
+            if (run) oracle_ssa(state, m->W, m->H, t, seed, deltat, nthreads);
             seed = (int32_t)((uint32_t)seed + 1u);
         }
         if (outputEvent) {
@@ -497,19 +501,22 @@ extern "C" int oracle_run_output(const oracle_model *m, const oracle_reaction_ta
             if (!(i == minspec || next[i] <= mindt)) continue;
             if (!(next[i] > 0)) continue;
             if (!m->inhomogeneous) {
-                oracle_hom_diffuse(state, m->W, m->H, i, seed, stay, m->boundary_mask, m->source_mask,
-                                   nthreads);
+                if (run)
+                    oracle_hom_diffuse(state, m->W, m->H, i, seed, stay, m->boundary_mask, m->source_mask,
+                                       nthreads);
                 nh[i] += dth[i];
             } else {
                 if (nx[i] <= mindt) {
-                    oracle_inhom_sweep(state, m->W, m->H, i, 0, seed, dtx[i], m->Dx, m->rx, h,
-                                       m->boundary_mask, nthreads);
+                    if (run)
+                        oracle_inhom_sweep(state, m->W, m->H, i, 0, seed, dtx[i], m->Dx, m->rx, h,
+                                           m->boundary_mask, nthreads);
                     seed = (int32_t)((uint32_t)seed + 1u);
                     nx[i] += dtx[i];
                 }
                 if (ny[i] <= mindt) {
-                    oracle_inhom_sweep(state, m->W, m->H, i, 1, seed, dty[i], m->Dy, m->ry, hy,
-                                       m->boundary_mask, nthreads);
+                    if (run)
+                        oracle_inhom_sweep(state, m->W, m->H, i, 1, seed, dty[i], m->Dy, m->ry, hy,
+                                           m->boundary_mask, nthreads);
                     seed = (int32_t)((uint32_t)seed + 1u);
                     ny[i] += dty[i];
                 }
@@ -523,9 +530,24 @@ extern "C" int oracle_run_output(const oracle_model *m, const oracle_reaction_ta
     return writes;
 }
 
+extern "C" int oracle_run_output(const oracle_model *m, const oracle_reaction_table *t, int32_t *state,
+                                 int steps, uint32_t seed0, int nthreads, float interval, float max_time,
+                                 int32_t *dumps, float *dump_times, int max_dumps, int64_t *iterations,
+                                 float *sim_time_out) {
+    return run_loop(m, t, state, 0, steps, seed0, nthreads, interval, max_time, dumps, dump_times, max_dumps,
+                    iterations, sim_time_out);
+}
+
 extern "C" int oracle_run(const oracle_model *m, const oracle_reaction_table *t, int32_t *state,
                           int steps, uint32_t seed0, int nthreads, float *sim_time_out) {
-    oracle_run_output(m, t, state, steps, seed0, nthreads, INFINITY, INFINITY, nullptr, nullptr, 0, nullptr,
-                      sim_time_out);
+    run_loop(m, t, state, 0, steps, seed0, nthreads, INFINITY, INFINITY, nullptr, nullptr, 0, nullptr, sim_time_out);
+    return 0;
+}
+
+// The loop from iteration `skip` on: state is the state after `skip` iterations.
+extern "C" int oracle_run_from(const oracle_model *m, const oracle_reaction_table *t, int32_t *state, int skip,
+                               int steps, uint32_t seed0, int nthreads, float *sim_time_out) {
+    run_loop(m, t, state, skip, steps, seed0, nthreads, INFINITY, INFINITY, nullptr, nullptr, 0, nullptr,
+             sim_time_out);
     return 0;
 }

@@ -228,3 +228,33 @@ def test_rdd_two_cell_sides(gpu, oracle):
     assert _rel(u, uw) <= 1e-5 and _rel(v, vw) <= 1e-5
     _, gDw, gkw = oracle.rdd_grad(p, prob["D"], prob["u0"], prob["v0"], prob["ut"], prob["vt"], 9, 4)
     assert np.linalg.norm(gD - gDw) / np.linalg.norm(gDw) <= 1e-4
+
+
+def test_rdd_autograd_refuses_a_replaced_forward(gpu):
+    """The handle keeps one forward's checkpoints: a second simulate() on the same handle
+    before the first one's backward makes that backward raise instead of differentiating
+    the second run; the rates of a CUDA tensor reach the solver without a host copy and
+    give the same gradients as host rates."""
+    from inchman_amd import GpgmpError
+    from inchman_amd.rdd import simulate
+
+    prob = configs.rdd_problem(64, steps=50)
+    r = _rdd(prob, steps=50, ckpt=10)
+    k1 = torch.tensor(prob["k"], dtype=torch.float32, device="cuda", requires_grad=True)
+    D = _dev(prob["D"]).requires_grad_(True)
+    u, v = simulate(r, D, k1, _dev(prob["u0"]), _dev(prob["v0"]))
+    k2 = torch.tensor([1.5, 3.0, 1e-4, 1.0], dtype=torch.float32, device="cuda", requires_grad=True)
+    u2, v2 = simulate(r, D, k2, _dev(prob["u0"]), _dev(prob["v0"]))
+    with pytest.raises((GpgmpError, RuntimeError), match="saved forward"):
+        (u.sum() + v.sum()).backward()
+    (u2.sum() + v2.sum()).backward()
+    assert k2.grad is not None and torch.isfinite(k2.grad).all()
+    # device rates == host rates
+    r3 = _rdd(prob, steps=50, ckpt=10)
+    r3.set_rates([1.5, 3.0, 1e-4, 1.0])
+    D3, uu, vv = _dev(prob["D"]), _dev(prob["u0"]), _dev(prob["v0"])
+    r3.forward(D3, uu, vv, save=True)
+    gD, gk, _, _ = r3.backward(D3, torch.ones_like(uu), torch.ones_like(vv), generation=r3.generation())
+    torch.cuda.synchronize()
+    assert torch.equal(u2.detach(), uu) and torch.equal(v2.detach(), vv)
+    assert torch.allclose(k2.grad, gk, rtol=1e-6) and torch.allclose(D.grad, gD, rtol=1e-5, atol=1e-12)

@@ -209,3 +209,18 @@ def test_output_events_schedule(oracle):
     plain, sim2 = oracle.run(m, st, 50, 9)
     np.testing.assert_array_equal(fin, plain)
     assert sim == sim2 and it == 50
+
+
+def test_oracle_skip_resumes_the_loop(oracle):
+    """The oracle's loop resumed from a snapshot (the first iterations advance only the
+    schedule and the seed stream) equals the uninterrupted loop: counts and clock."""
+    from inchman_amd import configs
+
+    for m, st in ((configs.schnakenberg_model(32), configs.schnakenberg_state(32, seed=2)),
+                  (configs.plain_gmp_model(32), configs.plain_gmp_state(32))):
+        full, t_full = oracle.run(m, st, 70, 3)
+        mid, _ = oracle.run(m, st, 45, 3)
+        res, t_res = oracle.run(m, mid, 25, 3, skip=45)
+        assert t_res == t_full
+        assert np.array_equal(res, full)
+        assert not np.array_equal(mid, full)
