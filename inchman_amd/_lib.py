@@ -175,6 +175,7 @@ SIGNATURES = {
     "gpgmp_particles_count": [_P, _pI],
     "gpgmp_particles_step": [_P, _I, _I],
     "gpgmp_particles_processed": [_P, C.POINTER(C.c_uint64)],
+    "gpgmp_particles_set_sorted_step": [_P, _I],
     "gpgmp_individuals_create": [C.POINTER(IndividualsDesc), _I, _P, C.POINTER(_P)],
     "gpgmp_individuals_destroy": [_P],
     "gpgmp_individuals_set": [_P, _P, _P, _I],

@@ -231,22 +231,17 @@ GPGMP_HD double sinpi_d(double t) {
     return sign * (x + x * (z * p));
 }
 
-// Gaussian pair by the Marsaglia polar method; key {id, seed}, counter {step, attempt}
+// Gaussian pair by the Box-Muller transform of one Threefry call (key {id, seed}, counter
+// {step, 0}): u1 = (w.x + 1/2) 2^-32 and u2 = (w.y + 1/2) 2^-32 in (0, 1),
+// g_x = sqrt(-2 ln u1) cos(2 pi u2), g_y = sqrt(-2 ln u1) sin(2 pi u2), with cos(2 pi u2) =
+// sin(pi (2 u2 + 1/2)); no rejection loop, so the lanes of a wave never wait for each other.
 GPGMP_HD void gauss2(uint32_t id, uint32_t seed, uint32_t step, double *gx, double *gy) {
-    for (uint32_t j = 0; j < 64u; ++j) {
-        const u32x2 w = threefry2x32_20(step, j, id, seed);
-        const double a = 2.0 * (((double)w.x + 0.5) * 2.3283064365386962890625e-10) - 1.0;
-        const double b = 2.0 * (((double)w.y + 0.5) * 2.3283064365386962890625e-10) - 1.0;
-        const double s = a * a + b * b;
-        if (s < 1.0 && s > 0.0) {
-            const double f = __builtin_sqrt(-2.0 * log_d(s) / s);
-            *gx = a * f;
-            *gy = b * f;
-            return;
-        }
-    }
-    *gx = 0.0;
-    *gy = 0.0;
+    const u32x2 w = threefry2x32_20(step, 0u, id, seed);
+    const double u1 = ((double)w.x + 0.5) * 2.3283064365386962890625e-10;
+    const double u2 = ((double)w.y + 0.5) * 2.3283064365386962890625e-10;
+    const double r = __builtin_sqrt(-2.0 * log_d(u1));
+    *gx = r * sinpi_d(2.0 * u2 + 0.5);
+    *gy = r * sinpi_d(2.0 * u2);
 }
 
 }  // namespace gpgmp

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <memory>
 
 #include "host.h"
@@ -36,15 +37,26 @@ struct gpgmp_particles {
     gpgmp_particle_params p{};
     int capacity = 0;
     hipStream_t stream = nullptr;
+    // current list (x, y, id, key_out with offsets off) and the buffers of the general step
     DevBuf<float> x, y, x2, y2;
     DevBuf<uint32_t> id, id2, key, val, key2, val2, key_out, keep, off, tile_sum, hist, digit_total;
-    DevBuf<int> dn;  // [0] live count, [1] next count
+    // step on the sorted list: the next list (xa, ya, ida, keya, off2) and per-cell work
+    DevBuf<float> xa, ya;
+    DevBuf<uint32_t> ida, keya, off2, cnt, pre, tot, kmask, surv, tile_cells, far_cnt, far_in;
+    DevBuf<uint8_t> dir, far_list;
+    DevBuf<int> dn;  // live count in slot cur, the next count in the other
+    int cur = 0;
+    int far_slot = 0;
     DevBuf<unsigned long long> processed;  // particle-steps done
-    int *n_host = nullptr;  // pinned
+    int *n_host = nullptr;  // pinned: [0] synchronous reads, [1] the asynchronous one
+    hipEvent_t n_ev = nullptr;
+    bool n_pending = false;
     int n_ub = 0;
     bool stepped = false;  // keys/offsets exist once a step has sorted the list
+    bool sorted_ok = false;  // moves stay within the 3x3 neighbourhood all but never
     ~gpgmp_particles() {
         (void)hipStreamSynchronize(stream);
+        if (n_ev) (void)hipEventDestroy(n_ev);
         if (n_host) (void)hipHostFree(n_host);
     }
     ParticleBuffers buffers() {
@@ -53,16 +65,45 @@ struct gpgmp_particles {
         b.id = id.get(); b.id2 = id2.get(); b.key = key.get(); b.val = val.get();
         b.key2 = key2.get(); b.val2 = val2.get(); b.key_out = key_out.get(); b.keep = keep.get();
         b.off = off.get(); b.tile_sum = tile_sum.get(); b.hist = hist.get(); b.digit_total = digit_total.get();
-        b.d_n = dn.get(); b.d_n2 = dn.get() + 1;
+        b.d_n = dn.get() + cur; b.d_n2 = dn.get() + (cur ^ 1);
         b.processed = processed.get();
         b.n_ub = n_ub;
         return b;
     }
+    SortedStepBuffers sorted_buffers() {
+        SortedStepBuffers b;
+        b.x = x.get(); b.y = y.get(); b.id = id.get(); b.key = key_out.get(); b.off = off.get();
+        b.x2 = x2.get(); b.y2 = y2.get(); b.dir = dir.get();
+        b.cnt = cnt.get(); b.pre = pre.get(); b.tot = tot.get(); b.kmask = kmask.get(); b.surv = surv.get();
+        b.tile_sum = tile_cells.get(); b.off_new = off2.get();
+        b.far_cnt = far_cnt.get() + far_slot; b.far_cnt_next = far_cnt.get() + (far_slot ^ 1);
+        b.far_in = far_in.get(); b.far_list = far_list.get();
+        b.n_cur = dn.get() + cur; b.n_next = dn.get() + (cur ^ 1);
+        b.processed = processed.get();
+        b.xo = xa.get(); b.yo = ya.get(); b.ido = ida.get(); b.keyo = keya.get();
+        b.n_ub = n_ub;
+        b.capacity = capacity;
+        return b;
+    }
     int read_n() {
-        GPGMP_HIP_CHECK(hipMemcpyAsync(n_host, dn.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
+        GPGMP_HIP_CHECK(hipMemcpyAsync(n_host, dn.get() + cur, sizeof(int), hipMemcpyDeviceToHost, stream));
         GPGMP_HIP_CHECK(hipStreamSynchronize(stream));
         n_ub = *n_host;
+        n_pending = false;
         return *n_host;
+    }
+    // The live count only shrinks: grids are sized by an upper bound that a copy in the
+    // background lowers now and then, so no step waits for the device.
+    void poll_n() {
+        if (n_pending && hipEventQuery(n_ev) == hipSuccess) {
+            n_ub = std::min(n_ub, n_host[1]);
+            n_pending = false;
+        }
+        if (!n_pending) {
+            GPGMP_HIP_CHECK(hipMemcpyAsync(n_host + 1, dn.get() + cur, sizeof(int), hipMemcpyDeviceToHost, stream));
+            GPGMP_HIP_CHECK(hipEventRecord(n_ev, stream));
+            n_pending = true;
+        }
     }
 };
 
@@ -132,11 +173,36 @@ int gpgmp_particles_create(const gpgmp_particle_params *p, int32_t capacity, voi
         ps->tile_sum.alloc(nt);
         ps->hist.alloc((size_t)256 * nt);
         ps->digit_total.alloc(256);
+        // the step on the sorted list: moves of at most one cell per axis unless a Gaussian
+        // displacement of 6 sigma plus the largest drift moves a whole cell (then the far
+        // list, exact but slow when long, would fill; those parameters take the radix step)
+        const double sigma = std::sqrt(2.0 * (double)p->D * (double)p->dt);
+        ps->sorted_ok = p->side >= 3 && 6.0 * sigma + std::fabs((double)p->vortex_amp) * (double)p->dt < 1.0;
+        if (ps->sorted_ok) {
+            ps->xa.alloc(n);
+            ps->ya.alloc(n);
+            ps->ida.alloc(n);
+            ps->keya.alloc(n);
+            ps->dir.alloc(n);
+            ps->far_list.alloc(n * far_entry_bytes());
+            ps->off2.alloc(ncells + 1);
+            ps->cnt.alloc(9 * ncells);
+            ps->pre.alloc(9 * ncells);
+            ps->tot.alloc(ncells);
+            ps->kmask.alloc(ncells);
+            ps->surv.alloc(ncells);
+            ps->tile_cells.alloc((ncells + 255) / 256);
+            ps->far_cnt.alloc(2);
+            ps->far_in.alloc(ncells);
+            GPGMP_HIP_CHECK(hipMemsetAsync(ps->far_in.get(), 0, ncells * sizeof(uint32_t), ps->stream));
+            GPGMP_HIP_CHECK(hipMemsetAsync(ps->far_cnt.get(), 0, 2 * sizeof(uint32_t), ps->stream));
+        }
         ps->dn.alloc(2);
         ps->processed.alloc(1);
         GPGMP_HIP_CHECK(hipMemsetAsync(ps->processed.get(), 0, sizeof(unsigned long long), ps->stream));
-        GPGMP_HIP_CHECK(hipHostMalloc((void **)&ps->n_host, sizeof(int)));
-        *ps->n_host = 0;
+        GPGMP_HIP_CHECK(hipHostMalloc((void **)&ps->n_host, 2 * sizeof(int)));
+        ps->n_host[0] = ps->n_host[1] = 0;
+        GPGMP_HIP_CHECK(hipEventCreateWithFlags(&ps->n_ev, hipEventDisableTiming));
         GPGMP_HIP_CHECK(hipMemsetAsync(ps->dn.get(), 0, 2 * sizeof(int), ps->stream));
         GPGMP_HIP_CHECK(hipStreamSynchronize(ps->stream));
         *out = ps.release();
@@ -162,6 +228,9 @@ int gpgmp_particles_set(gpgmp_particles *ps, const float *x, const float *y, con
             GPGMP_HIP_CHECK(hipMemcpyAsync(ps->y.get(), y, (size_t)n * 4, hipMemcpyHostToDevice, s));
             GPGMP_HIP_CHECK(hipMemcpyAsync(ps->id.get(), id, (size_t)n * 4, hipMemcpyHostToDevice, s));
         }
+        GPGMP_HIP_CHECK(hipStreamSynchronize(s));  // no copy of the count still in flight
+        ps->n_pending = false;
+        ps->cur = 0;
         *ps->n_host = n;
         GPGMP_HIP_CHECK(hipMemcpyAsync(ps->dn.get(), ps->n_host, sizeof(int), hipMemcpyHostToDevice, s));
         GPGMP_HIP_CHECK(hipStreamSynchronize(s));
@@ -194,10 +263,46 @@ int gpgmp_particles_get(gpgmp_particles *ps, float *x, float *y, uint32_t *id, u
         }
         if (off) {
             // offsets of the stored (compacted, sorted) keys
-            GPGMP_HIP_CHECK(launch_lower_bound(ps->key_out.get(), ps->dn.get(), n, (int)ncells, ps->off.get(), s));
+            GPGMP_HIP_CHECK(launch_lower_bound(ps->key_out.get(), ps->dn.get() + ps->cur, n, (int)ncells, ps->off.get(), s));
             GPGMP_HIP_CHECK(hipMemcpyAsync(off, ps->off.get(), (ncells + 1) * 4, hipMemcpyDeviceToHost, s));
         }
         GPGMP_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int gpgmp_particles_set_sorted_step(gpgmp_particles *ps, int32_t mode) {
+    return guarded_p([&] {
+        GPGMP_REQUIRE(ps && mode >= 0 && mode <= 2, GPGMP_E_INVALID, "mode must be 0, 1 or 2");
+        GPGMP_REQUIRE(mode != 2 || ps->p.side >= 3, GPGMP_E_INVALID, "the sorted-list step needs side >= 3");
+        GPGMP_HIP_CHECK(hipStreamSynchronize(ps->stream));
+        if (mode == 2 && !ps->xa.get()) {
+            const size_t n = ps->capacity, ncells = (size_t)ps->p.side * ps->p.side;
+            ps->xa.alloc(n);
+            ps->ya.alloc(n);
+            ps->ida.alloc(n);
+            ps->keya.alloc(n);
+            ps->dir.alloc(n);
+            ps->far_list.alloc(n * far_entry_bytes());
+            ps->off2.alloc(ncells + 1);
+            ps->cnt.alloc(9 * ncells);
+            ps->pre.alloc(9 * ncells);
+            ps->tot.alloc(ncells);
+            ps->kmask.alloc(ncells);
+            ps->surv.alloc(ncells);
+            ps->tile_cells.alloc((ncells + 255) / 256);
+            ps->far_cnt.alloc(2);
+            ps->far_in.alloc(ncells);
+            GPGMP_HIP_CHECK(hipMemsetAsync(ps->far_in.get(), 0, ncells * sizeof(uint32_t), ps->stream));
+            GPGMP_HIP_CHECK(hipMemsetAsync(ps->far_cnt.get(), 0, 2 * sizeof(uint32_t), ps->stream));
+            GPGMP_HIP_CHECK(hipStreamSynchronize(ps->stream));
+        }
+        if (mode == 0) ps->sorted_ok = false;
+        if (mode == 2) ps->sorted_ok = true;
+        if (mode == 1) {
+            const double sigma = std::sqrt(2.0 * (double)ps->p.D * (double)ps->p.dt);
+            ps->sorted_ok = ps->xa.get() && ps->p.side >= 3 &&
+                            6.0 * sigma + std::fabs((double)ps->p.vortex_amp) * (double)ps->p.dt < 1.0;
+        }
     });
 }
 
@@ -223,12 +328,28 @@ int gpgmp_particles_step(gpgmp_particles *ps, int32_t step0, int32_t steps) {
         a.seed = ps->p.seed;
         a.key_bits = bits_for((uint64_t)ps->p.side * ps->p.side);
         for (int k = 0; k < steps; ++k) {
-            // the live count only shrinks: grids sized by the last value read back
-            if (k % 8 == 0) ps->read_n();
             if (ps->n_ub == 0) break;
-            ParticleBuffers b = ps->buffers();
-            GPGMP_HIP_CHECK(particle_step(a, step0 + k, b, ps->stream));
+            if (ps->stepped && ps->sorted_ok) {
+                // the list is sorted by cell with its offsets: counts and one scatter
+                GPGMP_HIP_CHECK(particle_step_sorted(a, step0 + k, ps->sorted_buffers(), ps->stream));
+                std::swap(ps->x, ps->xa);
+                std::swap(ps->y, ps->ya);
+                std::swap(ps->id, ps->ida);
+                std::swap(ps->key_out, ps->keya);
+                std::swap(ps->off, ps->off2);
+                ps->cur ^= 1;
+                ps->far_slot ^= 1;
+            } else {
+                // a list in any order: radix sort, offsets, annihilation, compaction
+                ParticleBuffers b = ps->buffers();
+                GPGMP_HIP_CHECK(particle_step(a, step0 + k, b, ps->stream));
+                // offsets of the survivors, which the next step on the sorted list starts from
+                GPGMP_HIP_CHECK(launch_lower_bound(ps->key_out.get(), ps->dn.get() + ps->cur, 0,
+                                                   (int)((size_t)ps->p.side * ps->p.side), ps->off.get(),
+                                                   ps->stream));
+            }
             ps->stepped = true;
+            ps->poll_n();
         }
     });
 }

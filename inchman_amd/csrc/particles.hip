@@ -296,6 +296,296 @@ __global__ void k_finish_step(const int *n_next, int *n, unsigned long long *pro
     *n = *n_next;
 }
 
+// ---- particle step on a list already sorted by cell ---------------------------------------
+// After a step the list is sorted by cell (stably), with CSR offsets.  A particle of the
+// next step moves from its cell o to a cell c that is, but for a vanishing fraction, one of
+// the 3x3 torus neighbours of o, so the stable order by the new cell is fixed by counts:
+// with n(o->c) the particles of o moving to c, a particle of o is at rank
+//   r = sum over the sources o' of c with key(o') < key(o) of n(o'->c)
+//     + #(earlier particles of o moving to c) + #(particles from outside the 3x3 into c
+//       with a smaller index)
+// among the particles of c, which is the position std::stable_sort gives it (the list is
+// sorted by old key, so a smaller source key means a smaller index).  Particles that leave
+// the 3x3 neighbourhood go on a far list and are ranked the same way.  No radix pass and no
+// gather: every particle is read and written once, in nearly sequential order.
+//
+// Direction codes: d = (dy + 1) * 3 + (dx + 1) for a move by (dx, dy) in {-1, 0, 1}^2 on the
+// torus (side >= 3), kFar otherwise.
+constexpr int kFar = 9;
+
+struct FarEntry {
+    uint32_t i;  // index in the old order
+    uint32_t c;  // new cell
+};
+
+__device__ __forceinline__ int torus_step(int a, int b, int L) {
+    int d = b - a;
+    if (d == L - 1) d = -1;
+    else if (d == -(L - 1)) d = 1;
+    return d;
+}
+
+__device__ __forceinline__ int wrap1(int v, int L) {  // v in [-1, L]
+    v += (v < 0) ? L : 0;
+    return v - ((v >= L) ? L : 0);
+}
+
+// cell (cx, cy) moved by direction d
+__device__ __forceinline__ uint32_t moved_cell(int cx, int cy, int d, int L) {
+    return (uint32_t)(wrap1(cy + (d / 3 - 1), L) * L + wrap1(cx + (d % 3 - 1), L));
+}
+
+// 1. move (as k_move) and classify the move; far moves are listed (order-free atomics: the
+//    list is only ever counted, never walked in order)
+__global__ void k_coh_move(const float *__restrict__ x, const float *__restrict__ y, const uint32_t *__restrict__ id,
+                           float *__restrict__ x2, float *__restrict__ y2, uint8_t *__restrict__ dir, const int *d_n,
+                           int step, PartConsts p, uint32_t *__restrict__ far_cnt, FarEntry *__restrict__ far_list,
+                           uint32_t *__restrict__ far_in) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *d_n) return;
+    double gx, gy;
+    gauss2(id[i], p.seed, (uint32_t)step, &gx, &gy);
+    const float xo = x[i], yo = y[i];
+    const double xd = xo, yd = yo;
+    const double L = p.L;
+    const double vx = -p.amp * sinpi_d(2.0 * yd / L);
+    const double vy = p.amp * sinpi_d(2.0 * xd / L);
+    const float xn = wrap_pos(xd + vx * p.dt + p.sigma * gx, L);
+    const float yn = wrap_pos(yd + vy * p.dt + p.sigma * gy, L);
+    x2[i] = xn;
+    y2[i] = yn;
+    const int side = p.side;
+    const int cx = (int)xn, cy = (int)yn;
+    const int dx = torus_step((int)xo, cx, side);  // the old cell is the old position's (the key)
+    const int dy = torus_step((int)yo, cy, side);
+    const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1;
+    dir[i] = (uint8_t)(near ? (dy + 1) * 3 + (dx + 1) : kFar);
+    if (!near) {
+        const uint32_t c = (uint32_t)(cy * side + cx);
+        const uint32_t k = atomicAdd(far_cnt, 1u);
+        far_list[k] = FarEntry{(uint32_t)i, c};
+        atomicAdd(&far_in[c], 1u);
+    }
+}
+
+// 2. per old cell: its particles by direction, cnt[d][o] (every entry written)
+__global__ void k_coh_count(const uint32_t *__restrict__ off, const uint8_t *__restrict__ dir, int A,
+                            uint32_t *__restrict__ cnt) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= A) return;
+    const uint32_t b = off[o], e = off[o + 1];
+    uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0, c6 = 0, c7 = 0, c8 = 0;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t d = dir[i];
+        c0 += d == 0; c1 += d == 1; c2 += d == 2; c3 += d == 3; c4 += d == 4;
+        c5 += d == 5; c6 += d == 6; c7 += d == 7; c8 += d == 8;
+    }
+    const size_t sA = (size_t)A;
+    cnt[0 * sA + o] = c0; cnt[1 * sA + o] = c1; cnt[2 * sA + o] = c2;
+    cnt[3 * sA + o] = c3; cnt[4 * sA + o] = c4; cnt[5 * sA + o] = c5;
+    cnt[6 * sA + o] = c6; cnt[7 * sA + o] = c7; cnt[8 * sA + o] = c8;
+}
+
+__device__ __forceinline__ bool pair_killed(int step, uint32_t j, uint32_t c, const PartConsts &p) {
+    const u32x2 w = threefry2x32_20((uint32_t)step, j, c, ~p.seed);
+    return u01(w.x) < p.p_ann;
+}
+
+// 3. per new cell: the particles arriving from each source (pre[d][c] = the arrivals from
+//    the sources with a smaller key than direction d's), the total, the A+A->0 pair draws
+//    (first 32 pairs as a mask) and the survivors, summed per 256-cell tile
+__global__ __launch_bounds__(256) void k_coh_cells(const uint32_t *__restrict__ cnt, uint32_t *__restrict__ far_in,
+                                                   int A, int side, int step, PartConsts p, uint32_t *__restrict__ pre,
+                                                   uint32_t *__restrict__ tot, uint32_t *__restrict__ kmask,
+                                                   uint32_t *__restrict__ surv, uint32_t *__restrict__ tile_sum,
+                                                   uint32_t *__restrict__ far_cnt_next) {
+    __shared__ uint32_t sm[RS_WAVES];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0) *far_cnt_next = 0u;  // the far list of the next step starts empty
+    uint32_t S = 0;
+    if (c < A) {
+        const int cy = c / side, cx = c - cy * side;
+        uint32_t key[9], n[9];
+        uint32_t T = 0;
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            key[d] = moved_cell(cx, cy, 8 - d, side);  // the source whose move d lands on c
+            n[d] = cnt[(size_t)d * A + key[d]];
+            T += n[d];
+        }
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            uint32_t q = 0;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) q += (key[e] < key[d]) ? n[e] : 0u;
+            pre[(size_t)d * A + c] = q;
+        }
+        const uint32_t fin = far_in[c];
+        if (fin) far_in[c] = 0u;  // counted: clear for the next step
+        T += fin;
+        uint32_t mask = 0, kills = 0;
+        const uint32_t pairs = T >> 1;
+        for (uint32_t j = 0; j < pairs; ++j) {
+            if (pair_killed(step, j, (uint32_t)c, p)) {
+                ++kills;
+                if (j < 32) mask |= 1u << j;
+            }
+        }
+        S = T - 2 * kills;
+        tot[c] = T;
+        kmask[c] = mask;
+        surv[c] = S;
+    }
+    uint32_t total;
+    (void)block_exclusive_scan(S, sm, total);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+// 4. exclusive scan of the tile sums (one block of 1024 threads, 4 tiles per thread per
+//    round); the new count goes to the other count slot
+__global__ __launch_bounds__(1024) void k_coh_scan(uint32_t *__restrict__ tile_sum, int ntiles,
+                                                    const int *__restrict__ n_cur, int *__restrict__ n_next,
+                                                    unsigned long long *processed) {
+    __shared__ uint32_t wsum[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (int c0 = 0; c0 < ntiles; c0 += 4096) {
+        const int i0 = c0 + 4 * (int)threadIdx.x;
+        uint32_t v[4], t = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[k] = (i0 + k < ntiles) ? tile_sum[i0 + k] : 0u;
+            t += v[k];
+        }
+        const uint32_t incl = wave_incl_scan_u32(t);
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        uint32_t wbase = 0, all = 0;
+#pragma unroll
+        for (int w = 0; w < 16; ++w) {
+            const uint32_t sw = wsum[w];
+            wbase += (w < wave) ? sw : 0u;
+            all += sw;
+        }
+        __syncthreads();
+        uint32_t run = carry + wbase + incl - t;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k < ntiles) tile_sum[i0 + k] = run;
+            run += v[k];
+        }
+        carry += all;
+    }
+    if (threadIdx.x == 0) {
+        *processed += (unsigned long long)*n_cur;
+        *n_next = (int)carry;
+    }
+}
+
+// 5. survivors' CSR offsets off[c] (and off[A] = the total)
+__global__ __launch_bounds__(256) void k_coh_offsets(const uint32_t *__restrict__ surv,
+                                                     const uint32_t *__restrict__ tile_off, int A,
+                                                     uint32_t *__restrict__ off) {
+    __shared__ uint32_t sm[RS_WAVES];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t S = (c < A) ? surv[c] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(S, sm, total);
+    if (c < A) off[c] = tile_off[blockIdx.x] + ex;
+    if (c == A - 1) off[A] = tile_off[blockIdx.x] + ex + S;
+}
+
+// 6. every particle to its place in the new order; the killed pairs drop out.  A particle's
+//    rank among the earlier particles of its cell with its move class is counted over the
+//    move classes of the block's particles and the 256 before them, staged in LDS (a cell
+//    reaching further back counts the rest from global memory).  A thread takes CS_PER
+//    particles (a block CS_BLOCK * CS_PER consecutive ones) and issues their loads together:
+//    the kernel is bound by memory latency, not bandwidth, so each wave keeps several
+//    particles' loads in flight.
+constexpr int CS_BLOCK = 256, CS_PER = 4, CS_SPAN = CS_BLOCK * CS_PER, CS_BACK = 256;
+
+__global__ __launch_bounds__(CS_BLOCK) void k_coh_scatter(
+    const float *__restrict__ x2, const float *__restrict__ y2, const uint32_t *__restrict__ id,
+    const uint32_t *__restrict__ key, const uint8_t *__restrict__ dir, const uint32_t *__restrict__ off_old,
+    const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ pre, const uint32_t *__restrict__ tot,
+    const uint32_t *__restrict__ kmask, const uint32_t *__restrict__ off_new, const uint32_t *__restrict__ far_cnt,
+    const FarEntry *__restrict__ far_list, const int *d_n, int A, int side, int step, PartConsts p,
+    float *__restrict__ xo, float *__restrict__ yo, uint32_t *__restrict__ ido, uint32_t *__restrict__ keyo,
+    uint32_t cap) {
+    __shared__ uint8_t sdir[CS_BACK + CS_SPAN];
+    const int n = *d_n;
+    const int b0 = blockIdx.x * CS_SPAN;
+    const int w0 = b0 - CS_BACK;  // first index staged
+    for (int k = threadIdx.x; k < CS_BACK + CS_SPAN; k += CS_BLOCK) {
+        const int j = w0 + k;
+        sdir[k] = (j >= 0 && j < n) ? dir[j] : (uint8_t)kFar;
+    }
+    __syncthreads();
+    const uint32_t nf = *far_cnt;
+    float xv[CS_PER], yv[CS_PER];
+    uint32_t idv[CS_PER], o[CS_PER], d[CS_PER], c[CS_PER];
+    // every load that depends only on the index, then every one that depends only on the
+    // cells, so a wave waits on two rounds of memory latency for its CS_PER particles
+#pragma unroll
+    for (int q = 0; q < CS_PER; ++q) {
+        const int i = b0 + q * CS_BLOCK + (int)threadIdx.x;
+        const bool v = i < n;
+        xv[q] = v ? x2[i] : 0.f;
+        yv[q] = v ? y2[i] : 0.f;
+        idv[q] = v ? id[i] : 0u;
+        o[q] = v ? key[i] : 0u;
+        d[q] = v ? sdir[CS_BACK + q * CS_BLOCK + threadIdx.x] : (uint32_t)kFar;
+    }
+    int s0[CS_PER];
+    uint32_t pr[CS_PER], T[CS_PER], m[CS_PER], base[CS_PER];
+#pragma unroll
+    for (int q = 0; q < CS_PER; ++q) {
+        const int oy = (int)(o[q] / (uint32_t)side), ox = (int)o[q] - oy * side;
+        c[q] = (d[q] < 9) ? moved_cell(ox, oy, (int)d[q], side) : (uint32_t)((int)yv[q] * side + (int)xv[q]);
+        s0[q] = (int)off_old[o[q]];
+        pr[q] = (d[q] < 9) ? pre[(size_t)d[q] * A + c[q]] : 0u;
+        T[q] = tot[c[q]];
+        m[q] = kmask[c[q]];
+        base[q] = off_new[c[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < CS_PER; ++q) {
+        const int i = b0 + q * CS_BLOCK + (int)threadIdx.x;
+        if (i >= n) continue;
+        uint32_t r;
+        if (d[q] < 9) {
+            uint32_t k = 0;
+            for (int j = (s0[q] > w0 ? s0[q] : w0); j < i; ++j) k += sdir[j - w0] == d[q];
+            for (int j = s0[q]; j < w0; ++j) k += dir[j] == d[q];
+            r = pr[q] + k;
+        } else {
+            const int cy = (int)c[q] / side, cx = (int)c[q] - cy * side;
+            r = 0;
+            for (int e = 0; e < 9; ++e) {
+                const uint32_t s = moved_cell(cx, cy, 8 - e, side);
+                if (s < o[q]) r += cnt[(size_t)e * A + s];
+            }
+        }
+        for (uint32_t f = 0; f < nf; ++f) {
+            const FarEntry fe = far_list[f];
+            r += (fe.c == c[q] && fe.i < (uint32_t)i) ? 1u : 0u;
+        }
+        const uint32_t j = r >> 1;
+        const bool partner = (r | 1u) < T[q];
+        bool killed = false;
+        if (partner) killed = (j < 32) ? ((m[q] >> j) & 1u) != 0 : pair_killed(step, j, c[q], p);
+        if (killed) continue;
+        uint32_t before = (uint32_t)__popc(m[q] & ((j < 32) ? ((1u << j) - 1u) : 0xffffffffu));
+        for (uint32_t jj = 32; jj < j; ++jj) before += pair_killed(step, jj, c[q], p) ? 1u : 0u;
+        const uint32_t pos = base[q] + r - 2u * before;
+        if (pos >= cap) continue;  // never for a consistent list; keeps every store inside the buffers
+        xo[pos] = xv[q];
+        yo[pos] = yv[q];
+        ido[pos] = idv[q];
+        keyo[pos] = c[q];
+    }
+}
+
 }  // namespace gpgmp
 
 // ---------------------------------------------------------------------------------------
@@ -347,6 +637,28 @@ static PartConsts consts(const ParticleStepArgs &a) {
     p.seed = a.seed;
     return p;
 }
+
+hipError_t particle_step_sorted(const ParticleStepArgs &a, int step, const SortedStepBuffers &b, hipStream_t s) {
+    const PartConsts p = consts(a);
+    const int A = a.side * a.side;
+    const dim3 blk(256);
+    const dim3 gN((b.n_ub + 255) / 256 > 0 ? (b.n_ub + 255) / 256 : 1), gA((A + 255) / 256);
+    const int ntiles = (A + 255) / 256;
+    hipLaunchKernelGGL(k_coh_move, gN, blk, 0, s, b.x, b.y, b.id, b.x2, b.y2, b.dir, b.n_cur, step, p, b.far_cnt,
+                       (FarEntry *)b.far_list, b.far_in);
+    hipLaunchKernelGGL(k_coh_count, gA, blk, 0, s, b.off, b.dir, A, b.cnt);
+    hipLaunchKernelGGL(k_coh_cells, gA, blk, 0, s, b.cnt, b.far_in, A, a.side, step, p, b.pre, b.tot, b.kmask, b.surv,
+                       b.tile_sum, b.far_cnt_next);
+    hipLaunchKernelGGL(k_coh_scan, dim3(1), dim3(1024), 0, s, b.tile_sum, ntiles, b.n_cur, b.n_next, b.processed);
+    hipLaunchKernelGGL(k_coh_offsets, gA, blk, 0, s, b.surv, b.tile_sum, A, b.off_new);
+    hipLaunchKernelGGL(k_coh_scatter, dim3((b.n_ub + CS_SPAN - 1) / CS_SPAN > 0 ? (b.n_ub + CS_SPAN - 1) / CS_SPAN : 1),
+                       dim3(CS_BLOCK), 0, s, b.x2, b.y2, b.id, b.key, b.dir, b.off, b.cnt, b.pre, b.tot, b.kmask,
+                       b.off_new, b.far_cnt, (const FarEntry *)b.far_list, b.n_cur, A, a.side, step, p, b.xo, b.yo,
+                       b.ido, b.keyo, (uint32_t)b.capacity);
+    return hipGetLastError();
+}
+
+size_t far_entry_bytes() { return sizeof(FarEntry); }
 
 hipError_t particle_step(const ParticleStepArgs &a, int step, ParticleBuffers &b, hipStream_t s) {
     const PartConsts p = consts(a);

@@ -24,7 +24,29 @@ struct ParticleBuffers {
     int n_ub;         // host upper bound of the live count (grid sizes)
 };
 
+// Buffers of a step on a list already sorted by cell (the current list x, y, id, key with
+// its offsets off; the new list goes to xo, yo, ido, keyo with offsets off_new).
+struct SortedStepBuffers {
+    const float *x, *y;
+    const uint32_t *id, *key, *off;
+    float *x2, *y2;           // moved positions, old order
+    uint8_t *dir;             // move class per particle
+    uint32_t *cnt, *pre;      // [9][A]
+    uint32_t *tot, *kmask, *surv, *tile_sum, *off_new;
+    uint32_t *far_cnt, *far_cnt_next, *far_in;
+    void *far_list;           // far_entry_bytes() per entry, capacity entries
+    const int *n_cur;
+    int *n_next;
+    unsigned long long *processed;
+    float *xo, *yo;
+    uint32_t *ido, *keyo;
+    int n_ub;
+    int capacity;             // elements of every per-particle buffer
+};
+
 int radix_tiles(int n);
+size_t far_entry_bytes();
+hipError_t particle_step_sorted(const ParticleStepArgs &a, int step, const SortedStepBuffers &b, hipStream_t s);
 hipError_t radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *ktmp, uint32_t *vtmp, const int *d_n,
                             int n_ub, int key_bits, uint32_t *hist, uint32_t *digit_total, hipStream_t s,
                             bool *result_in_tmp);

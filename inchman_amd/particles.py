@@ -46,6 +46,11 @@ class ParticleSystem:
     def step(self, step0, steps):
         call("gpgmp_particles_step", self.handle, int(step0), int(steps))
 
+    def setSortedStep(self, mode=1):
+        """0: radix step every time; 1: the step on the sorted list when moves stay within the
+        neighbour cells; 2: that step always (results are identical)."""
+        call("gpgmp_particles_set_sorted_step", self.handle, int(mode))
+
     def processed(self):
         """Particle-steps done so far (sum of the live count at each step)."""
         v = C.c_uint64()

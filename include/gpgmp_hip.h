@@ -390,6 +390,13 @@ int gpgmp_particles_get(gpgmp_particles *ps, float *x, float *y, uint32_t *id, u
 int gpgmp_particles_count(gpgmp_particles *ps, int32_t *n_out);
 /* `steps` steps starting at step index `step0`: move, stable sort, offsets, annihilate. */
 int gpgmp_particles_step(gpgmp_particles *ps, int32_t step0, int32_t steps);
+/* Which step runs once the list is sorted by cell (after the first step): 1 (default) the
+ * step on the sorted list -- per-cell counts of the moves to the 3x3 neighbour cells and one
+ * scatter of every particle to its stable-order place, with moves beyond the neighbours on an
+ * exact but slow far list -- when 6 sigma + |v| dt < 1 cell, else the radix step; 0 always the
+ * radix step (move, stable LSD radix sort, offsets, annihilation, compaction); 2 always the
+ * sorted-list step.  Results are identical (no reference counterpart). */
+int gpgmp_particles_set_sorted_step(gpgmp_particles *ps, int32_t mode);
 /* Sum over the steps run so far of the particles each step processed. */
 int gpgmp_particles_processed(gpgmp_particles *ps, uint64_t *particle_steps);
 

@@ -83,26 +83,28 @@ double sinpi_d(double t) {
     return sign * (x + x * (z * p));
 }
 
-// Gaussian pair by the Marsaglia polar method on Threefry words, key {id, seed},
-// counter {step, attempt}.
+// Gaussian pair by the Box-Muller transform on one Threefry call, key {id, seed}, counter
+// {step, 0}: u1, u2 = (w + 1/2) 2^-32, g = sqrt(-2 ln u1) (cos 2 pi u2, sin 2 pi u2) with
+// cos(2 pi u2) = sin(pi (2 u2 + 1/2)).  Pinned by: mean 0, variance 1 and independence of the
+// pair (test_oracle_particles), libm for log and sin.
 void gauss2(uint32_t id, uint32_t seed, uint32_t step, double *gx, double *gy) {
     const double inv32 = 2.32830643653869628906e-010;
-    for (uint32_t j = 0; j < 64; ++j) {
-        uint32_t w0, w1;
-        oracle_threefry2x32_20(step, j, id, seed, &w0, &w1);
-        const double a = 2.0 * (((double)w0 + 0.5) * inv32) - 1.0;
-        const double b = 2.0 * (((double)w1 + 0.5) * inv32) - 1.0;
-        const double s = a * a + b * b;
-        if (s < 1.0 && s > 0.0) {
-            const double f = std::sqrt(-2.0 * oracle_logd(s) / s);
-            *gx = a * f;
-            *gy = b * f;
-            return;
-        }
-    }
-    *gx = 0.0;
-    *gy = 0.0;
+    uint32_t w0, w1;
+    oracle_threefry2x32_20(step, 0u, id, seed, &w0, &w1);
+    const double u1 = ((double)w0 + 0.5) * inv32;
+    const double u2 = ((double)w1 + 0.5) * inv32;
+    const double r = std::sqrt(-2.0 * oracle_logd(u1));
+    *gx = r * sinpi_d(2.0 * u2 + 0.5);
+    *gy = r * sinpi_d(2.0 * u2);
 }
+
+}  // namespace
+
+extern "C" void oracle_gauss2(uint32_t id, uint32_t seed, uint32_t step, double *gx, double *gy) {
+    gauss2(id, seed, step, gx, gy);
+}
+
+namespace {
 
 inline float wrap_pos(double xd, double L) {
     xd = xd - L * std::floor(xd / L);

@@ -60,3 +60,26 @@ def test_particle_annihilation_rate(oracle):
     removed = n - len(x1)
     assert removed % 2 == 0
     assert abs(removed / 2 - 0.05 * n / 2) < 5 * np.sqrt(0.05 * n / 2)
+
+
+def test_gauss2_is_box_muller(oracle):
+    """The particle Gaussian is the Box-Muller transform of one Threefry call: against libm on
+    the same words (a swapped sine/cosine, a missing factor or a wrong word would differ),
+    and the pair is standard normal and uncorrelated."""
+    import math
+
+    rng = np.random.default_rng(3)
+    ids = rng.integers(0, 2**32, 2000, dtype=np.uint64)
+    g = []
+    for k, i in enumerate(ids):
+        gx, gy = oracle.gauss2(int(i), 1234567, k % 7)
+        w0, w1 = oracle.threefry(k % 7, 0, int(i), 1234567)
+        u1, u2 = (w0 + 0.5) * 2.0**-32, (w1 + 0.5) * 2.0**-32
+        r = math.sqrt(-2.0 * math.log(u1))
+        assert abs(gx - r * math.cos(2 * math.pi * u2)) <= 1e-12 * max(1.0, abs(gx))
+        assert abs(gy - r * math.sin(2 * math.pi * u2)) <= 1e-12 * max(1.0, abs(gy))
+        g.append((gx, gy))
+    g = np.array(g)
+    assert np.all(np.abs(g.mean(axis=0)) < 0.1)
+    assert np.all(np.abs(g.var(axis=0) - 1.0) < 0.1)
+    assert abs(np.corrcoef(g[:, 0], g[:, 1])[0, 1]) < 0.1
