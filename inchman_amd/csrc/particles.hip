@@ -85,27 +85,61 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scan_totals(uint32_t *__re
     digit_total[threadIdx.x] = ex;
 }
 
-// Stable scatter of one pass.  Items are taken in rounds j of 256 consecutive keys
-// (index tile*4096 + j*256 + thread), so (round, wave, lane) order is index order.
+// Payload words carried with the keys through a pass (NP of them).
+struct RadixPayload {
+    const uint32_t *in[3];
+    uint32_t *out[3];
+};
+
+// Stable scatter of one pass, staged in LDS.  A tile of 4096 keys (16 per thread, in registers
+// with their payload) is first counted by digit, then ranked in rounds j of 256 consecutive
+// keys (index tile*4096 + j*256 + thread, so (round, wave, lane) order is index order): the
+// lanes of a wave holding the same digit are found with eight __ballot calls, a key's rank
+// is the popcount of its lower peers plus the counts of the earlier waves and rounds, and it
+// is written to LDS at its digit's place in the tile.  The tile, now sorted by digit, is
+// written out in LDS order: consecutive threads write consecutive places of one digit's run,
+// so the stores coalesce instead of landing one word at a time all over the output.
+template <int NP>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
-    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint32_t *__restrict__ kout,
-    uint32_t *__restrict__ vout, const int *d_n, int n_host, int shift, const uint32_t *__restrict__ hist,
-    const uint32_t *__restrict__ digit_base, int ntiles) {
+    const uint32_t *__restrict__ kin, uint32_t *__restrict__ kout, RadixPayload pl, const int *d_n, int n_host,
+    int shift, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ digit_base, int ntiles) {
+    __shared__ uint32_t skey[RS_TILE];
+    __shared__ uint32_t spay[NP][RS_TILE];
+    __shared__ uint32_t tstart[256];  // the tile's first place of each digit
+    __shared__ uint32_t gbase[256];   // output place of the tile's first key of each digit
     __shared__ uint32_t run[256];
-    __shared__ uint32_t base[256];
     __shared__ uint32_t wc[RS_WAVES][256];
+    __shared__ uint32_t sm[RS_WAVES];
     const int n = load_n(d_n, n_host);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    run[threadIdx.x] = 0;
-    base[threadIdx.x] = digit_base[threadIdx.x] + hist[(size_t)threadIdx.x * ntiles + blockIdx.x];
     const size_t tbase = (size_t)blockIdx.x * RS_TILE;
+    uint32_t key[RS_ITEMS], pay[NP][RS_ITEMS];
+    run[threadIdx.x] = 0;
+    gbase[threadIdx.x] = digit_base[threadIdx.x] + hist[(size_t)threadIdx.x * ntiles + blockIdx.x];
+    __syncthreads();
+#pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
         const size_t i = tbase + (size_t)j * RS_THREADS + threadIdx.x;
         const bool valid = i < (size_t)n;
-        const uint32_t key = valid ? kin[i] : 0u;
-        const uint32_t val = valid ? vin[i] : 0u;
-        const uint32_t d = (key >> shift) & 255u;
+        key[j] = valid ? kin[i] : 0u;
+#pragma unroll
+        for (int q = 0; q < NP; ++q) pay[q][j] = valid ? pl.in[q][i] : 0u;
+        if (valid) atomicAdd(&run[(key[j] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    {
+        uint32_t tot;
+        const uint32_t c = run[threadIdx.x];
+        tstart[threadIdx.x] = block_exclusive_scan(c, sm, tot);
+        run[threadIdx.x] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const size_t i = tbase + (size_t)j * RS_THREADS + threadIdx.x;
+        const bool valid = i < (size_t)n;
+        const uint32_t d = (key[j] >> shift) & 255u;
 #pragma unroll
         for (int w = 0; w < RS_WAVES; ++w) wc[w][threadIdx.x] = 0;
         __syncthreads();
@@ -132,11 +166,21 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
         }
         __syncthreads();
         if (valid) {
-            const uint32_t pos = base[d] + wc[wave][d] + rank;
-            kout[pos] = key;
-            vout[pos] = val;
+            const uint32_t lp = tstart[d] + wc[wave][d] + rank;
+            skey[lp] = key[j];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) spay[q][lp] = pay[q][j];
         }
-        __syncthreads();
+        __syncthreads();  // wc is cleared by the next round
+    }
+    const int cnt = (n - (int)tbase < RS_TILE) ? n - (int)tbase : RS_TILE;
+    for (int lp = threadIdx.x; lp < cnt; lp += RS_THREADS) {
+        const uint32_t k = skey[lp];
+        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t pos = gbase[d] + ((uint32_t)lp - tstart[d]);
+        kout[pos] = k;
+#pragma unroll
+        for (int q = 0; q < NP; ++q) pl.out[q][pos] = spay[q][lp];
     }
 }
 
@@ -184,9 +228,9 @@ __device__ __forceinline__ float wrap_pos(double xd, double L) {
 }
 
 // Move: x += v_x dt + sigma g_x with v = A(-sin(2 pi y/L), sin(2 pi x/L)), periodic;
-// writes the cell key and the identity payload for the sort.
+// writes the cell key for the sort.
 __global__ void k_move(float *__restrict__ x, float *__restrict__ y, const uint32_t *__restrict__ id,
-                       uint32_t *__restrict__ keys, uint32_t *__restrict__ vals, const int *d_n, int step,
+                       uint32_t *__restrict__ keys, const int *d_n, int step,
                        PartConsts p) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *d_n) return;
@@ -201,19 +245,6 @@ __global__ void k_move(float *__restrict__ x, float *__restrict__ y, const uint3
     x[i] = xn;
     y[i] = yn;
     keys[i] = (uint32_t)((int)yn * p.side + (int)xn);
-    vals[i] = (uint32_t)i;
-}
-
-// Gather of the positions and identities into sorted order.
-__global__ void k_gather(const uint32_t *__restrict__ perm, const float *__restrict__ x, const float *__restrict__ y,
-                         const uint32_t *__restrict__ id, float *__restrict__ xs, float *__restrict__ ys,
-                         uint32_t *__restrict__ ids, const int *d_n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *d_n) return;
-    const uint32_t j = perm[i];
-    xs[i] = x[j];
-    ys[i] = y[j];
-    ids[i] = id[j];
 }
 
 // A+A->0: the disjoint consecutive pairs (off[c]+2j, off[c]+2j+1) of cell c are removed
@@ -595,24 +626,49 @@ namespace gpgmp {
 
 int radix_tiles(int n) { return (n + RS_TILE - 1) / RS_TILE; }
 
-hipError_t radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *ktmp, uint32_t *vtmp, const int *d_n,
-                            int n_ub, int key_bits, uint32_t *hist, uint32_t *digit_total, hipStream_t s,
-                            bool *result_in_tmp) {
+// LSD passes of 8-bit digits over key_bits bits, ping-ponging the keys and np payload arrays
+// between the given and the tmp buffers; *result_in_tmp tells where the result is.
+hipError_t radix_sort_payload(uint32_t *keys, uint32_t *ktmp, uint32_t *const *pay, uint32_t *const *ptmp, int np,
+                              const int *d_n, int n_ub, int key_bits, uint32_t *hist, uint32_t *digit_total,
+                              hipStream_t s, bool *result_in_tmp) {
     const int nt = std::max(radix_tiles(n_ub), 1);
-    uint32_t *ki = keys, *vi = vals, *ko = ktmp, *vo = vtmp;
+    uint32_t *ki = keys, *ko = ktmp;
+    uint32_t *pi[3] = {nullptr, nullptr, nullptr}, *po[3] = {nullptr, nullptr, nullptr};
+    for (int q = 0; q < np; ++q) {
+        pi[q] = pay[q];
+        po[q] = ptmp[q];
+    }
     int passes = 0;
     for (int shift = 0; shift < key_bits; shift += 8) {
         hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(RS_THREADS), 0, s, ki, d_n, n_ub, shift, hist, nt);
         hipLaunchKernelGGL(k_radix_scan_rows, dim3(256), dim3(RS_THREADS), 0, s, hist, nt, digit_total);
         hipLaunchKernelGGL(k_radix_scan_totals, dim3(1), dim3(RS_THREADS), 0, s, digit_total);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nt), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, d_n, n_ub, shift, hist,
-                           digit_total, nt);
+        RadixPayload pl;
+        for (int q = 0; q < 3; ++q) {
+            pl.in[q] = pi[q];
+            pl.out[q] = po[q];
+        }
+        if (np == 1)
+            hipLaunchKernelGGL(k_radix_scatter<1>, dim3(nt), dim3(RS_THREADS), 0, s, ki, ko, pl, d_n, n_ub, shift, hist,
+                               digit_total, nt);
+        else if (np == 3)
+            hipLaunchKernelGGL(k_radix_scatter<3>, dim3(nt), dim3(RS_THREADS), 0, s, ki, ko, pl, d_n, n_ub, shift, hist,
+                               digit_total, nt);
+        else
+            return hipErrorInvalidValue;
         std::swap(ki, ko);
-        std::swap(vi, vo);
+        for (int q = 0; q < np; ++q) std::swap(pi[q], po[q]);
         ++passes;
     }
     *result_in_tmp = (passes & 1) != 0;
     return hipGetLastError();
+}
+
+hipError_t radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *ktmp, uint32_t *vtmp, const int *d_n,
+                            int n_ub, int key_bits, uint32_t *hist, uint32_t *digit_total, hipStream_t s,
+                            bool *result_in_tmp) {
+    uint32_t *pay[1] = {vals}, *ptmp[1] = {vtmp};
+    return radix_sort_payload(keys, ktmp, pay, ptmp, 1, d_n, n_ub, key_bits, hist, digit_total, s, result_in_tmp);
 }
 
 hipError_t launch_lower_bound(const uint32_t *keys, const int *d_n, int n_host, int ncells, uint32_t *off,
@@ -665,16 +721,21 @@ hipError_t particle_step(const ParticleStepArgs &a, int step, ParticleBuffers &b
     const int n_ub = b.n_ub;
     const int ncells = a.side * a.side;
     const dim3 gN((n_ub + 255) / 256 > 0 ? (n_ub + 255) / 256 : 1), blk(256);
-    // 1. move
-    hipLaunchKernelGGL(k_move, gN, blk, 0, s, b.x, b.y, b.id, b.key, b.val, b.d_n, step, p);
-    // 2. stable sort of (cell key, index)
+    // 1. move (positions in place) and cell keys
+    hipLaunchKernelGGL(k_move, gN, blk, 0, s, b.x, b.y, b.id, b.key, b.d_n, step, p);
+    // 2. stable sort by cell key carrying the positions and ids; an odd number of passes
+    //    (an all-zero digit pass above the key is a stable copy) leaves the result in the
+    //    second buffers, from which the compaction writes back into the first ones
+    int bits = a.key_bits;
+    if ((((bits + 7) / 8) & 1) == 0) bits += 8;
     bool in_tmp = false;
-    hipError_t e = radix_sort_pairs(b.key, b.val, b.key2, b.val2, b.d_n, n_ub, a.key_bits, b.hist, b.digit_total, s,
-                                    &in_tmp);
+    uint32_t *pay[3] = {(uint32_t *)b.x, (uint32_t *)b.y, b.id};
+    uint32_t *ptmp[3] = {(uint32_t *)b.x2, (uint32_t *)b.y2, b.id2};
+    hipError_t e = radix_sort_payload(b.key, b.key2, pay, ptmp, 3, b.d_n, n_ub, bits, b.hist, b.digit_total, s,
+                                      &in_tmp);
     if (e != hipSuccess) return e;
-    uint32_t *sk = in_tmp ? b.key2 : b.key;
-    uint32_t *sv = in_tmp ? b.val2 : b.val;
-    hipLaunchKernelGGL(k_gather, gN, blk, 0, s, sv, b.x, b.y, b.id, b.x2, b.y2, b.id2, b.d_n);
+    if (!in_tmp) return hipErrorInvalidValue;
+    uint32_t *sk = b.key2;
     // 3. offsets
     hipLaunchKernelGGL(k_lower_bound, dim3((ncells + 1 + 255) / 256), dim3(256), 0, s, sk, b.d_n, 0, ncells, b.off);
     // 4. annihilation flags
