@@ -89,7 +89,16 @@ GRAPH_CHUNK = 128   # HIP-graph chunk of the timed run (the timing windows run e
 # plain device copy reads).  Keyed by the number of timed steps (the timed region always
 # starts from the initial state, so the window is fixed by it).  Commands, raw counters and
 # the launch selection: profiles/gmp_step_schnakenberg256_counters.md.
-WINDOW_COUNTERS = {}
+WINDOW_COUNTERS = {
+    20: {"valu": {1: 7734881.5, 2: 2255003.0, 4: 3276716.0},
+         "hbm_bytes": {1: 1847424, 2: 1635872, 4: 2938256},
+         "source": "rocprofv3 --pmc SQ_INSTS_VALU / FETCH_SIZE / WRITE_SIZE over bench.py --steps 20 --warmup 5: "
+                   "the 4 counted launches of each kernel (timed iterations 1..4)"},
+    10000: {"valu": {1: 1779384.7, 2: 8454541.7, 4: 9159980.5},
+            "hbm_bytes": {1: 1670766, 2: 1666009, 4: 1668537},
+            "source": "rocprofv3 --pmc SQ_INSTS_VALU / FETCH_SIZE / WRITE_SIZE over bench.py --steps 10000 --warmup 100: "
+                      "the 316 counted launches of each kernel (timed iterations t with t % 128 in 1..4)"},
+}
 
 # MI355X's VALU issue peak: 256 CUs x 4 SIMDs, one wave64 32-bit VALU instruction per SIMD
 # every 4 cycles (16 lanes), 2.4 GHz
