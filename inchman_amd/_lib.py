@@ -26,6 +26,7 @@ DETERMINISTIC_HOMOGENEOUS_RK4 = 2
 DETERMINISTIC_INHOMOGENEOUS_RK4 = 3
 DETERMINISTIC_HOMOGENEOUS_AQSS = 4
 DETERMINISTIC_INHOMOGENEOUS_AQSS = 5
+STOCHASTIC_INHOMOGENEOUS_FPE = 6
 AQSS_PADE_FIX = 1
 
 
@@ -107,6 +108,8 @@ SIGNATURES = {
     "gpgmp_model_create": [C.POINTER(ModelDesc), C.POINTER(_P)],
     "gpgmp_model_destroy": [_P],
     "gpgmp_model_rate_literals": [_P, _pI, _pD],
+    "gpgmp_model_add_reaction_compartment": [_P, _I, _I, _I, _I, _I],
+    "gpgmp_model_localized": [_P, _pU],
     "gpgmp_solver_create": [_P, _P, _U, C.POINTER(_P)],
     "gpgmp_solver_destroy": [_P],
     "gpgmp_solver_set_state": [_P, _pI],

@@ -124,6 +124,25 @@ int gpgmp_model_rate_literals(const gpgmp_model *m, int32_t *is_int, double *val
     });
 }
 
+int gpgmp_model_add_reaction_compartment(gpgmp_model *m, int32_t reaction, int32_t x0, int32_t y0, int32_t x1,
+                                          int32_t y1) {
+    return guarded([&] {
+        GPGMP_REQUIRE(m, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(reaction >= 0 && reaction < m->m.table.R, GPGMP_E_INVALID, "no such reaction");
+        GPGMP_REQUIRE(m->m.compartments.size() < 4096, GPGMP_E_CAPACITY, "too many compartments");
+        m->m.compartments.push_back({reaction, x0, y0, x1, y1});
+        m->m.table.localized |= 1u << reaction;
+        m->m.table.compact = 0;  // the localized law is formed by the general table walk
+    });
+}
+
+int gpgmp_model_localized(const gpgmp_model *m, uint32_t *mask) {
+    return guarded([&] {
+        GPGMP_REQUIRE(m && mask, GPGMP_E_INVALID, "null argument");
+        *mask = m->m.table.localized;
+    });
+}
+
 // ---- plain solver -------------------------------------------------------------------
 int gpgmp_solver_create(const gpgmp_model *model, void *stream, uint32_t flags, gpgmp_solver **out) {
     return guarded([&] {
@@ -156,15 +175,18 @@ int gpgmp_solver_create(const gpgmp_model *model, void *stream, uint32_t flags, 
         } else {
             // InhomogeneousSolver.cpp:189-212: per species and axis, the minimum admissible
             // step over the cells (both axes use PhysicalCellWidth, as the reference's
-            // computeIndividualTimestepY does)
+            // computeIndividualTimestepY does), the Fokker-Planck form's for the fpe type
             const int A = d.width * d.height;
+            const bool fpe = s->m.fpe();
+            auto cell_dt = [&](float D, float r) {
+                return fpe ? fpe_cell_dt(D, r, s->m.cell_width, s->m.probs.stay)
+                           : inhomogeneous_cell_dt(D, r, s->m.cell_width);
+            };
             for (int i = 0; i < S; ++i) {
                 float mx = INFINITY, my = INFINITY;
                 for (int c = 0; c < A; ++c) {
-                    mx = std::min(mx, inhomogeneous_cell_dt(s->m.Dx[(size_t)i * A + c], s->m.rx[(size_t)i * A + c],
-                                                            s->m.cell_width));
-                    my = std::min(my, inhomogeneous_cell_dt(s->m.Dy[(size_t)i * A + c], s->m.ry[(size_t)i * A + c],
-                                                            s->m.cell_width));
+                    mx = std::min(mx, cell_dt(s->m.Dx[(size_t)i * A + c], s->m.rx[(size_t)i * A + c]));
+                    my = std::min(my, cell_dt(s->m.Dy[(size_t)i * A + c], s->m.ry[(size_t)i * A + c]));
                 }
                 s->dtx[i] = mx;
                 s->dty[i] = my;
@@ -286,8 +308,8 @@ int gpgmp_solver_attach_individuals(gpgmp_solver *s, int32_t species, gpgmp_indi
                                     const gpgmp_new_individual_rule *rule) {
     return guarded([&] {
         GPGMP_REQUIRE(s && g, GPGMP_E_INVALID, "null argument");
-        GPGMP_REQUIRE(s->m.desc.solver_type == GPGMP_STOCHASTIC_INHOMOGENEOUS, GPGMP_E_UNSUPPORTED,
-                      "individual species belong to the stochastic inhomogeneous solver");
+        GPGMP_REQUIRE(s->m.desc.solver_type == GPGMP_STOCHASTIC_INHOMOGENEOUS || s->m.fpe(), GPGMP_E_UNSUPPORTED,
+                      "individual species belong to the stochastic inhomogeneous solvers");
         GPGMP_REQUIRE(species >= 0 && species < s->m.desc.num_species, GPGMP_E_INVALID, "species out of range");
         const int A = s->m.desc.width * s->m.desc.height;
         GPGMP_HIP_CHECK(hipStreamSynchronize(s->eng->stream()));

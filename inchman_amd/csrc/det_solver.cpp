@@ -20,6 +20,14 @@ DetEngine::DetEngine(const Model &m, uint32_t flags, hipStream_t stream) : m_(m)
     A_ = (size_t)W_ * H_;
     x_.alloc((size_t)S_ * A_);
     tmp_.alloc(A_);
+    m_.table.act = nullptr;
+    if (m_.table.localized) {
+        std::vector<uint16_t> act(A_);
+        reaction_act(m_, W_, H_, act.data());
+        act_.alloc(A_);
+        GPGMP_HIP_CHECK(hipMemcpy(act_.get(), act.data(), A_ * sizeof(uint16_t), hipMemcpyHostToDevice));
+        m_.table.act = act_.get();
+    }
     GPGMP_HIP_CHECK(hipMemsetAsync(x_.get(), 0, (size_t)S_ * A_ * sizeof(float), stream_));
     dtx_.assign(S_, 0.f);
     dty_.assign(S_, 0.f);

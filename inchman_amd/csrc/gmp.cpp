@@ -46,6 +46,29 @@ float inhomogeneous_cell_dt(float Df, float rf, float hf) {
     return (float)ts;
 }
 
+// Per-cell step of the stochastic_inhomogeneous_fpe solver
+// (StochasticInhomogeneousDriftDiffusion.cl:123-126, 268-271):
+// (Real)StayProbability/6./(diffHere/(deltax*deltax) + fabs(rx/(2.*deltax))) -- a float
+// quotient D/h^2 plus a double |r|/(2h), the header float StayProbability over 6. in double.
+float fpe_cell_dt(float Df, float rf, float hf, float stay) {
+    const float dD = Df / (hf * hf);
+    const double den = (double)dD + std::fabs((double)rf / (2. * (double)hf));
+    return (float)((double)stay / 6. / den);
+}
+
+void reaction_act(const Model &m, int W, int H, uint16_t *out) {
+    uint16_t base = 0;
+    for (int i = 0; i < m.table.R; ++i)
+        if (!((m.table.localized >> i) & 1u)) base |= (uint16_t)(1u << i);
+    for (int64_t c = 0; c < (int64_t)W * H; ++c) out[c] = base;
+    for (const auto &cp : m.compartments) {
+        const int x0 = std::max(cp[1], 0), y0 = std::max(cp[2], 0);
+        const int x1 = std::min(cp[3], W - 1), y1 = std::min(cp[4], H - 1);
+        for (int y = y0; y <= y1; ++y)
+            for (int x = x0; x <= x1; ++x) out[(int64_t)y * W + x] |= (uint16_t)(1u << cp[0]);
+    }
+}
+
 Model compile_model(const gpgmp_model_desc &d) {
     Model m;
     m.desc = d;
@@ -58,7 +81,8 @@ Model compile_model(const gpgmp_model_desc &d) {
     GPGMP_REQUIRE((int64_t)d.width * d.height <= (int64_t)1 << 28, GPGMP_E_CAPACITY, "grid too large");
     GPGMP_REQUIRE(d.physical_length > 0.f && std::isfinite(d.physical_length), GPGMP_E_INVALID,
                   "physical_length must be positive");
-    GPGMP_REQUIRE(d.solver_type >= 0 && d.solver_type <= 5, GPGMP_E_INVALID, "unknown solver type");
+    GPGMP_REQUIRE(d.solver_type >= 0 && d.solver_type <= GPGMP_STOCHASTIC_INHOMOGENEOUS_FPE, GPGMP_E_INVALID,
+                  "unknown solver type");
     for (int b = 0; b < 4; ++b)
         GPGMP_REQUIRE(d.boundary_mask[b] == 0 || d.boundary_mask[b] == 1, GPGMP_E_INVALID,
                       "boundary masks must be 0 or 1");
@@ -210,7 +234,7 @@ StepDecision Schedule::next() {
 // ------------------------------------------------------------------------------------
 GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vector<int> &H,
                      hipStream_t stream)
-    : table_(m.table), probs_(m.probs), inh_(m.inhomogeneous()), stream_(stream) {
+    : table_(m.table), probs_(m.probs), inh_(m.inhomogeneous()), fpe_(m.fpe()), stream_(stream) {
     bm_ = make_int4(m.desc.boundary_mask[0], m.desc.boundary_mask[1], m.desc.boundary_mask[2],
                     m.desc.boundary_mask[3]);
     const int S = table_.S;
@@ -241,6 +265,16 @@ GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vecto
         po += A * S;
         total_cells_ += (int64_t)A;
         reals_.push_back(ri);
+    }
+    if (table_.localized) {
+        std::vector<uint16_t> act((size_t)total_cells_);
+        for (size_t r = 0; r < reals_.size(); ++r)
+            reaction_act(m, reals_[r].W, reals_[r].H, act.data() + reals_[r].cell_off);
+        act_.alloc(act.size());
+        GPGMP_HIP_CHECK(hipMemcpy(act_.get(), act.data(), act.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+        table_.act = act_.get();
+    } else {
+        table_.act = nullptr;
     }
     state_.alloc(so);
     lost_[0].alloc(lo);
@@ -341,7 +375,7 @@ void GmpEngine::set_probs(int r, int s, int axis, const float *D, const float *r
     GPGMP_HIP_CHECK(hipMemcpyAsync(dD.get(), D, A * sizeof(float), hipMemcpyHostToDevice, stream_));
     GPGMP_HIP_CHECK(hipMemcpyAsync(dr.get(), rr, A * sizeof(float), hipMemcpyHostToDevice, stream_));
     GPGMP_HIP_CHECK(launch_inh_probs(prob_[axis].get() + ri.prob_off + (size_t)s * A, dD.get(), dr.get(),
-                                     dt, dx, A, stream_));
+                                     dt, dx, A, fpe_ ? 1 : 0, stream_));
     GPGMP_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 

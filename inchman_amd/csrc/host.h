@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <random>
 #include <stdexcept>
@@ -78,11 +79,18 @@ struct Model {
     HomProbs probs{};
     float cell_width = 0.f;   // PhysicalCellWidth as printed into the header
     float cell_height = 0.f;  // PhysicalCellHeight
+    // compartments of the localized reactions: {reaction, x0, y0, x1, y1} with inclusive
+    // cell bounds (Reaction::addCompartment, Compartment::isInCompartment)
+    std::vector<std::array<int32_t, 5>> compartments;
     bool inhomogeneous() const {
         return desc.solver_type == GPGMP_STOCHASTIC_INHOMOGENEOUS ||
+               desc.solver_type == GPGMP_STOCHASTIC_INHOMOGENEOUS_FPE ||
                desc.solver_type == GPGMP_DETERMINISTIC_INHOMOGENEOUS_RK4 ||
                desc.solver_type == GPGMP_DETERMINISTIC_INHOMOGENEOUS_AQSS;
     }
+    // stochastic_inhomogeneous_fpe: the inhomogeneous solver with the Fokker-Planck
+    // discretisation's time step and transition rates
+    bool fpe() const { return desc.solver_type == GPGMP_STOCHASTIC_INHOMOGENEOUS_FPE; }
     bool deterministic() const {
         return desc.solver_type >= GPGMP_DETERMINISTIC_HOMOGENEOUS_RK4 &&
                desc.solver_type <= GPGMP_DETERMINISTIC_INHOMOGENEOUS_AQSS;
@@ -90,9 +98,13 @@ struct Model {
 };
 
 Model compile_model(const gpgmp_model_desc &d);
+// Active-reaction bits per cell of a W x H grid (bit i: reaction i not localized, or the
+// cell inside one of its compartments; Reaction::reactionMask, Reaction.cpp:216-235).
+void reaction_act(const Model &m, int W, int H, uint16_t *out);
 float header_float(float v);
 float homogeneous_dt(float p0, float h, float D);
 float inhomogeneous_cell_dt(float D, float r, float h);
+float fpe_cell_dt(float D, float r, float h, float stay);
 
 // boost::mt19937(seed0) + uniform_int<>(0, INT_MAX), seed_n = draw_n + n
 // (DiffusionModel.cpp:1467-1484); the draw is eng() >> 1 for this range.
@@ -248,10 +260,12 @@ class GmpEngine {
     HomProbs probs_{};
     int4 bm_{};
     bool inh_ = false;
+    bool fpe_ = false;
 
   private:
     void build_tiles();
     std::vector<RealInfo> reals_;
+    DevBuf<uint16_t> act_;  // active-reaction bits per cell (localized reactions only)
     DevBuf<int32_t> state_;
     DevBuf<int32_t> lost_[2];
     DevBuf<int4> prob_[2];

@@ -37,18 +37,31 @@ __device__ __forceinline__ int32_t hop_threshold(float p, int32_t &none) {
 // :648-650) with the reference's float/double promotions, stored as the thresholds of
 // (min(p1,p2), p2) with their none-mask (bit 0, bit 1).  They depend only on the fields
 // and the sweep's dt, so they are tabulated once.
+// fpe != 0: the stochastic_inhomogeneous_fpe solver's Fokker-Planck transition rates
+// (computeIndividualTimestepX/Y, StochasticInhomogeneousDriftDiffusion.cl:191-197,341-347:
+// D/dx^2 +- r/(2. dx), float quotient plus a double term, stored as float) times the sweep's
+// dt in float (inhomogeneous_diffuseX/Y, :491-500, :626-633).
 __global__ void k_inh_probs(int4 *__restrict__ out, const float *__restrict__ D,
-                            const float *__restrict__ r, float dt, float dx, int n) {
+                            const float *__restrict__ r, float dt, float dx, int n, int fpe) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float Di = D[i], ri = r[i];
-    const float rrdt = (ri * ri) * dt;
-    const float dx2 = dx * dx;
-    const float dxr = dx * ri;
-    const float ps = (float)(1. - ((double)dt * (2. * (double)Di + (double)rrdt)) / (double)dx2);
-    const double den = 2. * (double)Di + (double)rrdt;
-    const float p1 = (float)((1. - (double)ps) * (1. + (double)dxr / den) / 2.);
-    const float p2 = (float)((double)p1 + (1. - (double)ps) * (1. - (double)dxr / den) / 2.);
+    float p1, p2;
+    if (fpe) {
+        const float dD = Di / (dx * dx);
+        const double dr = (double)ri / (2. * (double)dx);
+        const float up = (float)((double)dD + dr), down = (float)((double)dD - dr);
+        p1 = up * dt;
+        p2 = p1 + down * dt;
+    } else {
+        const float rrdt = (ri * ri) * dt;
+        const float dx2 = dx * dx;
+        const float dxr = dx * ri;
+        const float ps = (float)(1. - ((double)dt * (2. * (double)Di + (double)rrdt)) / (double)dx2);
+        const double den = 2. * (double)Di + (double)rrdt;
+        p1 = (float)((1. - (double)ps) * (1. + (double)dxr / den) / 2.);
+        p2 = (float)((double)p1 + (1. - (double)ps) * (1. - (double)dxr / den) / 2.);
+    }
     int32_t n1, n2;
     const int32_t t1 = hop_threshold((p1 <= p2) ? p1 : p2, n1);
     const int32_t t2 = hop_threshold(p2, n2);
@@ -126,10 +139,10 @@ hipError_t launch_min_dt(const float *D, const float *r, int n, float h, float *
     return hipGetLastError();
 }
 
-hipError_t launch_inh_probs(int4 *out, const float *D, const float *r, float dt, float dx, int n,
+hipError_t launch_inh_probs(int4 *out, const float *D, const float *r, float dt, float dx, int n, int fpe,
                             hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_inh_probs, dim3((n + 255) / 256), dim3(256), 0, stream, out, D, r, dt, dx, n);
+    hipLaunchKernelGGL(k_inh_probs, dim3((n + 255) / 256), dim3(256), 0, stream, out, D, r, dt, dx, n, fpe);
     return hipGetLastError();
 }
 

@@ -49,9 +49,23 @@ __device__ __forceinline__ int32_t sel(const int32_t (&x)[S], int j) {
 
 // _h[i+1] = _h[0] + <law_i>; int literal -> int product then float add,
 // double literal -> double product chain rounded once (Solver.cpp:849-857).
+// A localized reaction's law is `w * <law_i>` with the float mask value w of the cell
+// (Solver.cpp:853-854): the chain then starts at w, so an int literal's product is formed in
+// float ((w * lit) * X * (X - 1) ...) and a double literal's in double from (double)w * lit.
 template <int S>
-__device__ __forceinline__ float add_law(const RxTable &T, int i, float h0, const int32_t (&x)[S]) {
+__device__ __forceinline__ float add_law(const RxTable &T, int i, float h0, const int32_t (&x)[S], uint32_t act) {
     const int nf = T.nf[i];
+    if ((T.localized >> i) & 1u) {
+        const float w = ((act >> i) & 1u) ? 1.f : 0.f;
+        if ((T.lit_is_int >> i) & 1u) {
+            float v = w * (float)T.lit_int[i];
+            for (int f = 0; f < nf; ++f) v = v * (float)(sel(x, T.fs[i][f]) - (int32_t)T.fo[i][f]);
+            return h0 + v;
+        }
+        double v = (double)w * T.lit_dbl[i];
+        for (int f = 0; f < nf; ++f) v = v * (double)(sel(x, T.fs[i][f]) - (int32_t)T.fo[i][f]);
+        return (float)((double)h0 + v);
+    }
     if ((T.lit_is_int >> i) & 1u) {
         uint32_t v = (uint32_t)T.lit_int[i];
         for (int f = 0; f < nf; ++f) v *= (uint32_t)(sel(x, T.fs[i][f]) - (int32_t)T.fo[i][f]);
@@ -104,8 +118,10 @@ __device__ __forceinline__ void hazards_compact(const RxTable &T, const int32_t 
     h[0] = t;
 }
 
+// act: the cell's active-reaction bits (RxTable::act; ignored unless T.localized).
 template <int S>
-__device__ __forceinline__ void hazards(const RxTable &T, const int32_t (&x)[S], float (&h)[kMaxReactions + 1]) {
+__device__ __forceinline__ void hazards(const RxTable &T, const int32_t (&x)[S], float (&h)[kMaxReactions + 1],
+                                        uint32_t act) {
     if (T.compact) {
         hazards_compact<S>(T, x, h);
         return;
@@ -114,7 +130,7 @@ __device__ __forceinline__ void hazards(const RxTable &T, const int32_t (&x)[S],
 #pragma unroll
     for (int i = 0; i < kMaxReactions; ++i) {
         if (i < T.R) {
-            h[i + 1] = add_law(T, i, h[0], x);
+            h[i + 1] = add_law(T, i, h[0], x, act);
             h[0] = h[i + 1];
         }
     }
@@ -155,7 +171,7 @@ __device__ __forceinline__ void fire(const RxTable &T, const float (&h)[kMaxReac
 template <int S, bool COMPACT>
 __device__ __forceinline__ bool ssa_cell_t(int32_t (&x)[S], uint32_t cell, int32_t seed, float deltat,
                                            const RxTable &T, uint32_t budget, int below, float &time,
-                                           uint32_t &ctr, float &rate) {
+                                           uint32_t &ctr, float &rate, uint32_t act) {
     const uint32_t k1 = kKeyBase + (uint32_t)seed;
     if (!(time < deltat)) return true;
     // The draws of an event depend on its counter only, so they are formed one event
@@ -173,7 +189,7 @@ __device__ __forceinline__ bool ssa_cell_t(int32_t (&x)[S], uint32_t cell, int32
         const double U2n = u01_d(w.y);
         float h[kMaxReactions + 1];
         if constexpr (COMPACT) hazards_compact<S>(T, x, h);
-        else hazards<S>(T, x, h);
+        else hazards<S>(T, x, h, act);
         asm volatile("" ::"v"(Ln), "v"(U2n));
         if (h[0] == 0.f) return true;
         if (ctr == kMaxEventsPerCall) return true;  // runaway guard (never reached by sane models)
@@ -195,9 +211,14 @@ __device__ __forceinline__ bool ssa_cell_t(int32_t (&x)[S], uint32_t cell, int32
 template <int S>
 __device__ __forceinline__ bool ssa_cell(int32_t (&x)[S], uint32_t cell, int32_t seed, float deltat,
                                          const RxTable &T, uint32_t budget, int below, float &time,
-                                         uint32_t &ctr, float &rate) {
-    return T.compact ? ssa_cell_t<S, true>(x, cell, seed, deltat, T, budget, below, time, ctr, rate)
-                     : ssa_cell_t<S, false>(x, cell, seed, deltat, T, budget, below, time, ctr, rate);
+                                         uint32_t &ctr, float &rate, uint32_t act) {
+    return T.compact ? ssa_cell_t<S, true>(x, cell, seed, deltat, T, budget, below, time, ctr, rate, act)
+                     : ssa_cell_t<S, false>(x, cell, seed, deltat, T, budget, below, time, ctr, rate, act);
+}
+
+// The active-reaction bits of cell `gc` (cells of all realizations) for the hazards.
+__device__ __forceinline__ uint32_t cell_act(const RxTable &T, uint64_t gc) {
+    return T.localized ? (uint32_t)T.act[gc] : 0xffffu;
 }
 
 // Lane groups.  With G > 1, G consecutive lanes of a wave (G divides 64) share one
@@ -239,9 +260,9 @@ __device__ __forceinline__ int32_t grp_sum(int32_t v) {
 template <int S, int G>
 __device__ __forceinline__ bool ssa_cell_g(int32_t (&x)[S], uint32_t cell, int32_t seed, float deltat,
                                            const RxTable &T, uint32_t budget, int below, float &time,
-                                           uint32_t &ctr, float &rate) {
+                                           uint32_t &ctr, float &rate, uint32_t act) {
     if constexpr (G == 1) {
-        return ssa_cell<S>(x, cell, seed, deltat, T, budget, below, time, ctr, rate);
+        return ssa_cell<S>(x, cell, seed, deltat, T, budget, below, time, ctr, rate, act);
     } else {
         const uint32_t k1 = kKeyBase + (uint32_t)seed;
         const uint32_t g = threadIdx.x & (G - 1);
@@ -254,7 +275,7 @@ __device__ __forceinline__ bool ssa_cell_g(int32_t (&x)[S], uint32_t cell, int32
 #pragma unroll
             for (int j = 0; j < G; ++j) {
                 float h[kMaxReactions + 1];
-                hazards<S>(T, x, h);
+                hazards<S>(T, x, h, act);
                 if (h[0] == 0.f) return true;
                 if (ctr == kMaxEventsPerCall) return true;  // runaway guard
                 if (ctr >= budget && __popcll(__ballot(1)) < below) {
@@ -716,8 +737,9 @@ __device__ __forceinline__ void step_tile(
           if (pl.deltat > 0.f && T.R > 0) {
             float time = 0.f, rate = 0.f;
             uint32_t ctr = 0;
+            const uint32_t act = cell_act(T, ri.cell_off + (uint64_t)c);
             const bool fin = ssa_cell_g<S, G>(x, (uint32_t)c, seed, pl.deltat, T, hot.budget, hot.below_active,
-                                              time, ctr, rate);
+                                              time, ctr, rate, act);
             // a deferred cell with a modest expected tail goes to the lane list, the others
             // (and any the lane list has no room for) to the wave list; a cell neither list
             // can take is finished here
@@ -727,7 +749,7 @@ __device__ __forceinline__ void step_tile(
             const bool in_wave = push_hot(!fin && !in_lane, g, hot.e, hot.count, hot.cap, he);
             deferred = in_lane || in_wave;
             if (!fin && !deferred)
-                ssa_cell_g<S, G>(x, (uint32_t)c, seed, pl.deltat, T, 0xffffffffu, 0, time, ctr, rate);
+                ssa_cell_g<S, G>(x, (uint32_t)c, seed, pl.deltat, T, 0xffffffffu, 0, time, ctr, rate, act);
           }
         }
 
@@ -864,12 +886,12 @@ constexpr int kSpecPasses = 8;
 
 template <int S>
 __device__ __forceinline__ void ssa_wave_spec(int32_t (&x)[S], uint32_t cell, uint32_t k1, float deltat,
-                                              const RxTable &T, float &time, uint32_t &ctr) {
+                                              const RxTable &T, float &time, uint32_t &ctr, uint32_t act) {
     const int lane = (int)(threadIdx.x & 63);
     if (!(time < deltat)) return;
     while (true) {
         float h[kMaxReactions + 1];
-        hazards<S>(T, x, h);
+        hazards<S>(T, x, h, act);
         if (h[0] == 0.f) return;
         if (ctr == kMaxEventsPerCall) return;
         const u32x2 w = threefry2x32_20(ctr + 1u + (uint32_t)lane, kCtrHi, cell, k1);
@@ -884,7 +906,7 @@ __device__ __forceinline__ void ssa_wave_spec(int32_t (&x)[S], uint32_t cell, ui
         for (int pass = 0; pass < kSpecPasses; ++pass) {
 #pragma unroll
             for (int s = 0; s < S; ++s) xj[s] = x[s] + wave_excl_scan(net_of(T, ms, s));
-            hazards<S>(T, xj, hj);
+            hazards<S>(T, xj, hj, act);
             ma = fired<S>(T, hj, (float)(U2 * (double)hj[0]));
             const bool zero = hj[0] == 0.f;
             zb = __ballot(zero);
@@ -959,7 +981,8 @@ __device__ __forceinline__ void hot_cells(int wk, int nwk, int32_t *__restrict__
         for (int s = 0; s < S; ++s) x[s] = __builtin_amdgcn_readfirstlane(st[(size_t)s * A + c]);
         float time = he.time;
         uint32_t ctr = he.ctr;
-        ssa_wave_spec<S>(x, (uint32_t)c, kKeyBase + (uint32_t)seed, pss.deltat, T, time, ctr);
+        ssa_wave_spec<S>(x, (uint32_t)c, kKeyBase + (uint32_t)seed, pss.deltat, T, time, ctr,
+                         cell_act(T, ri.cell_off + (uint64_t)c));
         // the whole count is known now: busy cells are predicted for the next step
         if (pss.predict && lane == 0)
             hot.pred[(size_t)pss.hot_parity * hot.pred_stride + ri.cell_off + c] = ctr > kPredictEvents ? 1 : 0;
@@ -1067,7 +1090,8 @@ __device__ __forceinline__ void hot_lane_cells(int wk, int nwk, int32_t *__restr
         if (valid) {
             float time = he.time, rate = 0.f;
             uint32_t ctr = he.ctr;
-            (void)ssa_cell<S>(x, (uint32_t)c, seed, pss.deltat, T, 0xffffffffu, 0, time, ctr, rate);
+            (void)ssa_cell<S>(x, (uint32_t)c, seed, pss.deltat, T, 0xffffffffu, 0, time, ctr, rate,
+                              cell_act(T, ri.cell_off + (uint64_t)c));
         }
         if (pl.out_mask) {
 #pragma unroll
@@ -1125,7 +1149,9 @@ __device__ __forceinline__ void pred_cells(int wk, int nwk, int32_t *__restrict_
         }
         float time = 0.f;
         uint32_t ctr = 0;
-        if (pl.deltat > 0.f && T.R > 0) ssa_wave_spec<S>(x, (uint32_t)c, kKeyBase + (uint32_t)seed, pl.deltat, T, time, ctr);
+        if (pl.deltat > 0.f && T.R > 0)
+            ssa_wave_spec<S>(x, (uint32_t)c, kKeyBase + (uint32_t)seed, pl.deltat, T, time, ctr,
+                             cell_act(T, ri.cell_off + (uint64_t)c));
         if (lane == 0) {
 #pragma unroll
             for (int s = 0; s < S; ++s) st[(size_t)s * A + c] = x[s];

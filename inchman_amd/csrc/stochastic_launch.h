@@ -49,7 +49,7 @@ hipError_t launch_gmp_inh_y(const StepLaunch &a);
 hipError_t launch_gmp_inh_xs(const StepLaunch &a);
 hipError_t launch_min_dt(const float *D, const float *r, int n, float h, float *blocks, float *out,
                          hipStream_t stream);
-hipError_t launch_inh_probs(int4 *out, const float *D, const float *r, float dt, float dx, int n,
+hipError_t launch_inh_probs(int4 *out, const float *D, const float *r, float dt, float dx, int n, int fpe,
                             hipStream_t stream);
 
 }  // namespace gpgmp

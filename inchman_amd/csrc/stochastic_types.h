@@ -29,6 +29,12 @@ struct RxTable {
     uint8_t fo[kMaxReactions][kMaxFactors];  // factor offset m of (X - m)
     int8_t net[kMaxReactions][kMaxSpecies];  // product - reactant stoichiometry
     int32_t compact;  // R <= kCompactReactions and every nf <= kCompactFactors
+    // Localized reactions (Reaction::addCompartment, Reaction.cpp:209-235): bit i set when
+    // reaction i carries a compartment mask; act[cell] (cells of all realizations, cell_off
+    // + c) has bit i set where reaction i's mask w(x, y) is 1.  act is read only when
+    // localized != 0, and compact is then 0.
+    uint32_t localized;
+    const uint16_t *act;
 };
 
 // One realization (grid) of a launch.  The plain solver is a batch of one.

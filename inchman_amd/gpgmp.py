@@ -25,6 +25,7 @@ from ._lib import call, lib
 
 STOCHASTIC_HOMOGENEOUS = _lib.STOCHASTIC_HOMOGENEOUS
 STOCHASTIC_INHOMOGENEOUS = _lib.STOCHASTIC_INHOMOGENEOUS
+STOCHASTIC_INHOMOGENEOUS_FPE = _lib.STOCHASTIC_INHOMOGENEOUS_FPE
 DETERMINISTIC_HOMOGENEOUS_RK4 = _lib.DETERMINISTIC_HOMOGENEOUS_RK4
 DETERMINISTIC_INHOMOGENEOUS_RK4 = _lib.DETERMINISTIC_INHOMOGENEOUS_RK4
 DETERMINISTIC_HOMOGENEOUS_AQSS = _lib.DETERMINISTIC_HOMOGENEOUS_AQSS
@@ -57,14 +58,48 @@ class Species:
         self.diffusionConstant = float(diffusion_constant)
 
 
+class Compartment:
+    """Rectangular compartment (Compartment.h:50-158): inclusive cell bounds
+    x0 <= x <= x1, y0 <= y <= y1 (setBounds / isInCompartment)."""
+
+    def __init__(self, id, x0=0, y0=0, x1=0, y1=0):
+        self.id = id
+        self.setBounds(x0, y0, x1, y1)
+
+    def setBounds(self, x0, y0, x1, y1):
+        self.bounds = (int(x0), int(y0), int(x1), int(y1))
+
+    def getBounds(self):
+        return self.bounds
+
+    def isInCompartment(self, x, y, z=0):
+        x0, y0, x1, y1 = self.bounds
+        return x0 <= x <= x1 and y0 <= y <= y1
+
+
 class Reaction:
-    """Mass-action reaction (Reaction.h): rate and stoichiometry maps by species."""
+    """Mass-action reaction (Reaction.h): rate and stoichiometry maps by species, and the
+    compartments of a localized reaction (Reaction::addCompartment)."""
 
     def __init__(self, rate, reactants=None, products=None, id=None):
         self.rate = float(rate)
         self.reactants = dict(reactants or {})
         self.products = dict(products or {})
         self.id = id
+        self.compartments = []
+
+    # Reaction::addCompartment (Reaction.cpp:209-212)
+    def addCompartment(self, compartment):
+        self.compartments.append(compartment)
+
+    def isLocalized(self):
+        return bool(self.compartments)
+
+    # Reaction::reactionMask (Reaction.cpp:216-235)
+    def reactionMask(self, x, y, z=0):
+        if not self.compartments:
+            return 1.0
+        return 1.0 if any(c.isInCompartment(x, y, z) for c in self.compartments) else 0.0
 
 
 class DiffusionModel:
@@ -142,7 +177,31 @@ class DiffusionModel:
         d, keep = self._desc()
         h = C.c_void_p()
         call("gpgmp_model_create", C.byref(d), C.byref(h))
-        return CompiledModel(h, self)
+        cm = CompiledModel(h, self)
+        for i, r in enumerate(self.reactions):
+            for cp in r.compartments:
+                call("gpgmp_model_add_reaction_compartment", h, i, *cp.getBounds())
+        return cm
+
+    def hasLocalizedReactions(self):
+        return any(r.isLocalized() for r in self.reactions)
+
+    def reactionActiveBits(self, width=None, height=None):
+        """Per cell [H][W] uint16: bit i set where reaction i's mask is 1 (the reaction
+        mask Solver::oclCreateReactionMask uploads, Solver.cpp:358-392)."""
+        W = self.width if width is None else width
+        H = self.height if height is None else height
+        act = np.zeros((H, W), np.uint16)
+        for i, r in enumerate(self.reactions):
+            if not r.compartments:
+                act |= np.uint16(1 << i)
+                continue
+            for cp in r.compartments:
+                x0, y0, x1, y1 = cp.getBounds()
+                x0, y0, x1, y1 = max(x0, 0), max(y0, 0), min(x1, W - 1), min(y1, H - 1)
+                if x0 <= x1 and y0 <= y1:
+                    act[y0:y1 + 1, x0:x1 + 1] |= np.uint16(1 << i)
+        return act
 
     def createSolver(self, stream=None, flags=0):
         """Factory of DiffusionModel::oclSolve (DiffusionModel.cpp:1422-1440)."""

@@ -48,6 +48,11 @@ extern "C" {
 #define GPGMP_DETERMINISTIC_INHOMOGENEOUS_RK4 3
 #define GPGMP_DETERMINISTIC_HOMOGENEOUS_AQSS 4
 #define GPGMP_DETERMINISTIC_INHOMOGENEOUS_AQSS 5
+/* stochastic_inhomogeneous_fpe (DiffusionModel.h:49, DiffusionModel.cpp:257): the inhomogeneous
+ * solver with the Fokker-Planck discretisation -- per-cell step StayProbability/6/(D/h^2 +
+ * |r|/(2h)) and hop rates D/h^2 +- r/(2h) times the sweep's dt
+ * (StochasticInhomogeneousDriftDiffusion.cl:123-126,191-197,268-271,341-347,491-500,626-633) */
+#define GPGMP_STOCHASTIC_INHOMOGENEOUS_FPE 6
 
 /* One reaction: stoichiometry per species and the rate constant
  * (Reaction(id, Real rate, ...) Reaction.cpp:39-56; law generated at :119-138). */
@@ -105,6 +110,18 @@ int gpgmp_model_destroy(gpgmp_model *m);
 /* Compiled per-reaction rate literal as the reference prints it (Reaction.cpp:122):
  * is_int[i], value[i]. */
 int gpgmp_model_rate_literals(const gpgmp_model *m, int32_t *is_int, double *value);
+/* Reaction::addCompartment (Reaction.cpp:209-212) with the compartment's bounds
+ * (Compartment::setBounds, Compartment.cpp:27-32): reaction `reaction` becomes localized, and
+ * its hazard and deterministic rate are multiplied in every cell by the reaction mask
+ * w(x, y) = 1 inside any of its compartments (x0 <= x <= x1, y0 <= y <= y1,
+ * Compartment::isInCompartment, Compartment.h:152-158), 0 elsewhere (Reaction::reactionMask,
+ * Reaction.cpp:216-235; Solver::oclCreateReactionMask, Solver.cpp:358-392; hazards
+ * Solver.cpp:853-854, RK4 Solver.cpp:947-948, alpha-QSS Solver.cpp:987-1000).  Applies to
+ * solvers created from the model afterwards. */
+int gpgmp_model_add_reaction_compartment(gpgmp_model *m, int32_t reaction, int32_t x0, int32_t y0, int32_t x1,
+                                         int32_t y1);
+/* Bit i set when reaction i is localized (Reaction::isLocalized, Reaction.h:196). */
+int gpgmp_model_localized(const gpgmp_model *m, uint32_t *mask);
 
 /* ---- plain solver (gpgmp::Solver and subclasses) --------------------------------- */
 /* Flags of gpgmp_solver_create. */

@@ -53,6 +53,11 @@ typedef struct {
     int factor_offset[ORACLE_MAX_REACTIONS][ORACLE_MAX_FACTORS];
     /* net stoichiometric change (product - reactant) */
     int net[ORACLE_MAX_REACTIONS][ORACLE_MAX_SPECIES];
+    /* localized reactions (Reaction::addCompartment, Reaction.cpp:209-235): bit i set when
+     * reaction i is localized; act[cell] (x + y*W) has bit i set where its mask w(x,y) is 1.
+     * act is read only when localized != 0. */
+    uint32_t localized;
+    const uint16_t *act;
 } oracle_reaction_table;
 
 /* ---- primitives ---- */
@@ -74,8 +79,13 @@ float oracle_header_float(float v);
 float oracle_homogeneous_dt(float p0, float h, float D);
 float oracle_inhomogeneous_cell_dt(float D, float r, float h);
 float oracle_inhomogeneous_dt(const float *D, const float *r, int n, float h);
+/* stochastic_inhomogeneous_fpe: per-cell step and its minimum (stay: StayProbability) */
+float oracle_fpe_cell_dt(float D, float r, float h, float stay);
+float oracle_fpe_dt(const float *D, const float *r, int n, float h, float stay);
 
 /* ---- stochastic kernels ---- */
+/* cumulative hazards h[0..R] (h[0] = total) of state x[S] in cell `cell` */
+void oracle_hazards(const oracle_reaction_table *tab, const int32_t *x, int cell, float *h);
 void oracle_ssa(int32_t *state, int W, int H, const oracle_reaction_table *tab,
                 int32_t seed, float deltat, int nthreads);
 void oracle_hom_diffuse(int32_t *state, int W, int H, int species, int32_t seed,
@@ -84,13 +94,18 @@ void oracle_hom_diffuse(int32_t *state, int W, int H, int species, int32_t seed,
 void oracle_inhom_sweep(int32_t *state, int W, int H, int species, int axis, int32_t seed,
                         float dt, const float *D_axis, const float *r_axis, float h,
                         const int *boundary_mask, int nthreads);
+/* the same sweep with the Fokker-Planck transition rates (stochastic_inhomogeneous_fpe) */
+void oracle_fpe_sweep(int32_t *state, int W, int H, int species, int axis, int32_t seed,
+                      float dt, const float *D_axis, const float *r_axis, float h,
+                      const int *boundary_mask, int nthreads);
 
 /* ---- host schedule (DiffusionModel::oclSolve) ---- */
 typedef struct {
     int W, H, S;
     float physical_length;
     float p0;
-    int inhomogeneous;          /* 0: stochastic_homogeneous, 1: stochastic_inhomogeneous */
+    int inhomogeneous;          /* 0: stochastic_homogeneous, 1: stochastic_inhomogeneous,
+                                   2: stochastic_inhomogeneous_fpe */
     const float *diffusion;     /* [S] (homogeneous) */
     const float *Dx, *Dy, *rx, *ry; /* [S][H][W] (inhomogeneous) */
     int boundary_mask[4];

@@ -23,6 +23,7 @@ class RxTable(C.Structure):
         ("n_factors", C.c_int * MAXR),
         ("factor_species", (C.c_int * MAXF) * MAXR), ("factor_offset", (C.c_int * MAXF) * MAXR),
         ("net", (C.c_int * MAXS) * MAXR),
+        ("localized", C.c_uint32), ("act", C.POINTER(C.c_uint16)),
     ]
 
 
@@ -90,6 +91,10 @@ def lib():
                                          C.c_void_p, C.c_void_p, C.c_int]
         L.oracle_inhom_sweep.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int32,
                                          C.c_float, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_int]
+        L.oracle_fpe_cell_dt.restype = C.c_float
+        L.oracle_fpe_cell_dt.argtypes = [C.c_float] * 4
+        L.oracle_fpe_sweep.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int32,
+                                       C.c_float, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_int]
         L.oracle_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_int,
                                  C.c_void_p]
         L.oracle_run_from.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_int,
@@ -179,7 +184,36 @@ def table_from_model(model):
             rs[i, sp.index] += n
         for sp, n in r.products.items():
             ps[i, sp.index] += n
-    return compile_reactions(S, [r.rate for r in model.reactions], rs, ps)
+    t = compile_reactions(S, [r.rate for r in model.reactions], rs, ps)
+    localized = 0
+    for i, r in enumerate(model.reactions):
+        if getattr(r, "compartments", None):
+            localized |= 1 << i
+    if localized:
+        # the cells' active-reaction bits from the reactions' compartments
+        # (Reaction::reactionMask, Reaction.cpp:216-235), x + y*W order
+        act = np.zeros((model.height, model.width), np.uint16)
+        for i, r in enumerate(model.reactions):
+            if not (localized >> i) & 1:
+                act |= np.uint16(1 << i)
+                continue
+            for cp in r.compartments:
+                x0, y0, x1, y1 = cp.getBounds()
+                for y in range(max(y0, 0), min(y1, model.height - 1) + 1):
+                    for x in range(max(x0, 0), min(x1, model.width - 1) + 1):
+                        act[y, x] |= np.uint16(1 << i)
+        t._act = np.ascontiguousarray(act.ravel())
+        t.localized = localized
+        t.act = t._act.ctypes.data_as(C.POINTER(C.c_uint16))
+    return t
+
+
+def hazards(table, x, cell=0):
+    """Cumulative hazards [h0, h1..hR] of the state x[S] in cell `cell`."""
+    xs = np.ascontiguousarray(x, np.int32)
+    h = np.zeros(table.num_reactions + 1, np.float32)
+    lib().oracle_hazards(C.byref(table), _p(xs), int(cell), h.ctypes.data_as(C.POINTER(C.c_float)))
+    return h
 
 
 def ssa(state, table, seed, deltat, nthreads=0):
@@ -209,6 +243,21 @@ def inhom_sweep(state, species, axis, seed, dt, Dax, rax, h, bm=(0, 0, 0, 0), nt
     return st
 
 
+def fpe_cell_dt(D, r, h, stay):
+    return lib().oracle_fpe_cell_dt(float(D), float(r), float(h), float(stay))
+
+
+def fpe_sweep(state, species, axis, seed, dt, Dax, rax, h, bm=(0, 0, 0, 0), nthreads=0):
+    st = np.ascontiguousarray(state, np.int32).copy()
+    S, H, W = st.shape
+    D = np.ascontiguousarray(Dax, np.float32)
+    r = np.ascontiguousarray(rax, np.float32)
+    b = np.array(bm, np.int32)
+    lib().oracle_fpe_sweep(_p(st), W, H, species, axis, int(seed), float(dt), _p(D), _p(r), float(h),
+                           _p(b), nthreads)
+    return st
+
+
 def stay_probability(p0):
     return lib().oracle_stay_probability(float(p0))
 
@@ -232,7 +281,8 @@ def _model_struct(model):
     m = Model()
     m.W, m.H, m.S = model.width, model.height, S
     m.physical_length, m.p0 = model.physical_length, model.p0
-    m.inhomogeneous = 1 if model.solver == 1 else 0
+    # 0 homogeneous, 1 inhomogeneous, 2 inhomogeneous with the Fokker-Planck discretisation
+    m.inhomogeneous = {1: 1, 6: 2}.get(model.solver, 0)
     diff = np.array([sp.diffusionConstant for sp in model.species], np.float32)
     keep = [diff]
     m.diffusion = diff.ctypes.data_as(C.POINTER(C.c_float))

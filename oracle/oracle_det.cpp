@@ -79,10 +79,13 @@ extern "C" void oracle_det_rk4(double *state, int W, int H, const oracle_reactio
                                double dt) {
     const size_t A = (size_t)W * H;
     const int S = t->num_species, R = t->num_reactions;
+    size_t cell = 0;
+    // a localized reaction's rate is multiplied by the cell's mask value (Solver.cpp:947-948)
     auto rhs = [&](const double *x, double *d) {
         for (int s = 0; s < S; ++s) d[s] = 0.0;
         for (int i = 0; i < R; ++i) {
             double law = t->lit_dbl[i];
+            if ((t->localized >> i) & 1u) law *= ((t->act[cell] >> i) & 1u) ? 1.0 : 0.0;
             for (int f = 0; f < t->n_factors[i]; ++f) law *= x[t->factor_species[i][f]];
             for (int s = 0; s < S; ++s)
                 if (t->net[i][s] != 0) d[s] += (double)t->net[i][s] * law;
@@ -91,6 +94,7 @@ extern "C" void oracle_det_rk4(double *state, int W, int H, const oracle_reactio
     const int nmax = 20;
     const double h = dt / nmax;
     for (size_t c = 0; c < A; ++c) {
+        cell = c;
         double x[ORACLE_MAX_SPECIES], k1[ORACLE_MAX_SPECIES], k2[ORACLE_MAX_SPECIES],
             k3[ORACLE_MAX_SPECIES], k4[ORACLE_MAX_SPECIES], tmp[ORACLE_MAX_SPECIES];
         for (int s = 0; s < S; ++s) x[s] = state[c + s * A];
@@ -138,15 +142,20 @@ extern "C" void oracle_det_rk4(double *state, int W, int H, const oracle_reactio
 namespace {
 constexpr int kAqssMaxIter = 1 << 16;
 
+// deterministic_rhs_ode_aqss as generated (Solver.cpp:960-1010): per species, production and
+// destruction sums `0. + |n|. * [w *] <law> + ...` in double, w the cell's mask value of a
+// localized reaction (act: the cell's active bits).
 template <typename R>
-void aqss_rhs(const oracle_reaction_table *t, const R *y, R *q, R *d) {
+void aqss_rhs(const oracle_reaction_table *t, const R *y, R *q, R *d, uint32_t act) {
     const int S = t->num_species, NR = t->num_reactions;
     for (int s = 0; s < S; ++s) {
         double qs = 0., ds = 0.;
         for (int i = 0; i < NR; ++i) {
             const int n = t->net[i][s];
             if (n == 0) continue;
-            double term = (double)std::abs(n) * t->lit_dbl[i];
+            double term = (double)std::abs(n);
+            if ((t->localized >> i) & 1u) term = term * (((act >> i) & 1u) ? 1.0 : 0.0);
+            term = term * t->lit_dbl[i];
             for (int f = 0; f < t->n_factors[i]; ++f) term = term * (double)y[t->factor_species[i][f]];
             if (n > 0)
                 qs = qs + term;
@@ -159,7 +168,7 @@ void aqss_rhs(const oracle_reaction_table *t, const R *y, R *q, R *d) {
 }
 
 template <typename R>
-int aqss_cell(const oracle_reaction_table *t, R *state, R dtg, int fixed) {
+int aqss_cell(const oracle_reaction_table *t, R *state, R dtg, int fixed, uint32_t act) {
     const int S = t->num_species;
     const R epsmin = (R)1e-2, sqreps = (R)0.5, epscl = (R)100., epsmax = (R)10., tfd = (R)1.000008,
             dtmin = (R)1e-15, ymin = (R)1e-20;
@@ -168,7 +177,7 @@ int aqss_cell(const oracle_reaction_table *t, R *state, R dtg, int fixed) {
         qs[ORACLE_MAX_SPECIES], rtaus[ORACLE_MAX_SPECIES], y1[ORACLE_MAX_SPECIES], scr[ORACLE_MAX_SPECIES];
     R tn = (R)0.;
     for (int i = 0; i < S; ++i) y[i] = std::fmax(state[i], ymin);
-    aqss_rhs<R>(t, y, q, d);
+    aqss_rhs<R>(t, y, q, d, act);
     R scrtch = (R)1e-25;
     for (int i = 0; i < S; ++i) {
         const R ascr = std::fabs(q[i]);
@@ -204,7 +213,7 @@ int aqss_cell(const oracle_reaction_table *t, R *state, R dtg, int fixed) {
                     tn = ts + dt;
                     for (int i = 0; i < S; ++i) y1[i] = y[i];
                 }
-                aqss_rhs<R>(t, y, q, d);
+                aqss_rhs<R>(t, y, q, d, act);
                 eps = (R)1e-10;
                 for (int i = 0; i < S; ++i) {  // corrector
                     const R rtaub = (R)(0.5 * (rtaus[i] + dt * d[i] / y[i]));
@@ -245,7 +254,7 @@ int aqss_cell(const oracle_reaction_table *t, R *state, R dtg, int fixed) {
             dto = dt / dto;
             for (int i = 0; i < S; ++i) rtaus[i] *= dto;
         }
-        aqss_rhs<R>(t, y, q, d);
+        aqss_rhs<R>(t, y, q, d, act);
     }
     for (int i = 0; i < S; ++i) state[i] = y[i];
     return iters;
@@ -259,7 +268,8 @@ int aqss_grid(R *state, int W, int H, const oracle_reaction_table *t, R dtg, int
     for (size_t c = 0; c < A; ++c) {
         R x[ORACLE_MAX_SPECIES];
         for (int s = 0; s < S; ++s) x[s] = state[c + s * A];
-        const int it = aqss_cell<R>(t, x, dtg, fixed);
+        const uint32_t act = t->localized ? (uint32_t)t->act[c] : 0xffffffffu;
+        const int it = aqss_cell<R>(t, x, dtg, fixed, act);
         worst = (it < 0 || worst < 0) ? -1 : std::max(worst, it);
         for (int s = 0; s < S; ++s) state[c + s * A] = x[s];
     }

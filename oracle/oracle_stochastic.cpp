@@ -184,6 +184,24 @@ extern "C" float oracle_inhomogeneous_cell_dt(float Df, float rf, float hf) {
     return (float)ts;
 }
 
+// stochastic_inhomogeneous_fpe per-cell step (StochasticInhomogeneousDriftDiffusion.cl:
+// 123-126 for X, 268-271 for Y, both with PhysicalCellWidth): the generated expression
+// (Real)StayProbability/6./(diffHere/(deltax*deltax)+fabs(rx/(2.*deltax))) under C's
+// promotions -- D/(h*h) in float, |r/(2.*h)| in double, the sum and the quotient in double.
+// Pinned by: the pure-diffusion value stay*h^2/(6D) and the pure-drift value stay*h/(3|r|)
+// (tests/test_oracle_fpe.py), and the rates' closed forms.
+extern "C" float oracle_fpe_cell_dt(float D, float r, float h, float stay) {
+    const float dterm = D / (h * h);
+    const double den = (double)dterm + std::fabs((double)r / (2. * (double)h));
+    return (float)((double)stay / 6. / den);
+}
+
+extern "C" float oracle_fpe_dt(const float *D, const float *r, int n, float h, float stay) {
+    float m = INFINITY;
+    for (int i = 0; i < n; ++i) m = std::min(m, oracle_fpe_cell_dt(D[i], r[i], h, stay));
+    return m;
+}
+
 // Minimum over cells (computeIndividualTimestepX/Y + reduceBlocks,
 // StochasticInhomogeneousDriftDiffusion.cl:88-423).
 extern "C" float oracle_inhomogeneous_dt(const float *D, const float *r, int n, float h) {
@@ -196,7 +214,23 @@ namespace {
 
 // _h[i+1] = _h[0] + <law>  (Solver.cpp:843-857, law Reaction.cpp:119-138):
 // int literal -> int arithmetic then float add; double literal -> double chain.
-inline float add_law(const oracle_reaction_table *t, int i, float h0, const int32_t *x) {
+// A localized reaction's generated hazard is `_h[0] + read_imagef(mask).x * <law>`
+// (Solver.cpp:853-854): C evaluates (((w * lit) * X) * (X-1)) ... left to right, so with the
+// float mask value w an int literal's chain is float and a double literal's is double.
+inline float add_law(const oracle_reaction_table *t, int i, float h0, const int32_t *x, int cell) {
+    if ((t->localized >> i) & 1u) {
+        const float w = ((t->act[cell] >> i) & 1u) ? 1.0f : 0.0f;
+        if (t->lit_is_int[i]) {
+            float v = w * (float)t->lit_int[i];
+            for (int f = 0; f < t->n_factors[i]; ++f)
+                v = v * (float)(x[t->factor_species[i][f]] - t->factor_offset[i][f]);
+            return h0 + v;
+        }
+        double v = (double)w * t->lit_dbl[i];
+        for (int f = 0; f < t->n_factors[i]; ++f)
+            v = v * (double)(x[t->factor_species[i][f]] - t->factor_offset[i][f]);
+        return (float)((double)h0 + v);
+    }
     if (t->lit_is_int[i]) {
         uint32_t v = (uint32_t)t->lit_int[i];
         for (int f = 0; f < t->n_factors[i]; ++f)
@@ -220,6 +254,18 @@ int threads_or_default(int n) {
 
 }  // namespace
 
+// The cumulative hazards _h[0..R] of one state x in cell `cell` (the hazard block of
+// performGillespie, Solver.cpp:843-857).
+// Pinned by: laws evaluated by hand in the tests (int/double literals, localized reactions
+// with the mask's float chain against the unlocalized int chain, mask 0).
+extern "C" void oracle_hazards(const oracle_reaction_table *t, const int32_t *x, int cell, float *h) {
+    h[0] = 0.f;
+    for (int i = 0; i < t->num_reactions; ++i) {
+        h[i + 1] = add_law(t, i, h[0], x, cell);
+        h[0] = h[i + 1];
+    }
+}
+
 // performGillespie (Gillespie.cl:4-106) for every cell; the kernel is launched
 // over (W,H) with key {cell, 0xdecafbad+seed}, counter {++n, 0xf00dcafe}.
 // Pinned by: birth-death Poisson moments, A+B steady state, brute force on tiny grids.
@@ -239,7 +285,7 @@ extern "C" void oracle_ssa(int32_t *state, int W, int H, const oracle_reaction_t
         while (time < deltat) {
             h[0] = 0.f;
             for (int i = 0; i < R; ++i) {
-                h[i + 1] = add_law(t, i, h[0], x);
+                h[i + 1] = add_law(t, i, h[0], x, c);
                 h[0] = h[i + 1];
             }
             if (h[0] == 0) break;
@@ -346,13 +392,9 @@ extern "C" void oracle_hom_diffuse(int32_t *state, int W, int H, int species, in
     }
 }
 
-// inhomogeneous_diffuseX/Y + inhomogeneous_updateStateX/Y
-// (StochasticInhomogeneousDriftDiffusion.cl:426-723); key {cell + s*A, 0xdecafbad+seed};
-// probabilities at :518-520 / :648-650 with the reference's float/double promotions.
-// Pinned by: conservation, per-sweep mean r dt and variance 2 D dt closed forms.
-extern "C" void oracle_inhom_sweep(int32_t *state, int W, int H, int species, int axis,
-                                   int32_t seed, float dt, const float *Dax, const float *rax,
-                                   float dxh, const int *bm, int nthreads) {
+// One X or Y sweep with the inhomogeneous (fpe = false) or Fokker-Planck hop thresholds.
+static void sweep_impl(int32_t *state, int W, int H, int species, int axis, int32_t seed, float dt,
+                       const float *Dax, const float *rax, float dxh, const int *bm, int nthreads, bool fpe) {
     const int A = W * H;
     int32_t *st = state + (size_t)species * A;
     const float *Ds = Dax + (size_t)species * A;
@@ -368,13 +410,24 @@ extern "C" void oracle_inhom_sweep(int32_t *state, int W, int H, int species, in
         mask[2] = (gy == 0) ? bm[2] : 1;
         mask[3] = (gy == H - 1) ? bm[3] : 1;
         const float D = Ds[c], r = rs[c];
-        const float rrdt = (r * r) * dt;
-        const float dx2 = dxh * dxh;
-        const float dxr = dxh * r;
-        const float ps = (float)(1. - ((double)dt * (2. * (double)D + (double)rrdt)) / (double)dx2);
-        const double den = 2. * (double)D + (double)rrdt;
-        const float p1 = (float)((1. - (double)ps) * (1. + (double)dxr / den) / 2.);
-        const float p2 = (float)((double)p1 + (1. - (double)ps) * (1. - (double)dxr / den) / 2.);
+        float p1, p2;
+        if (fpe) {
+            // transition rates (…DriftDiffusion.cl:191-197 / 341-347, stored float4) and the
+            // thresholds p1 = rate_plus*dt, p2 = p1 + rate_minus*dt in float (:491-500 / :626-633)
+            const float dD = D / (dxh * dxh);
+            const float up = (float)((double)dD + (double)r / (2. * (double)dxh));
+            const float down = (float)((double)dD - (double)r / (2. * (double)dxh));
+            p1 = up * dt;
+            p2 = p1 + down * dt;
+        } else {
+            const float rrdt = (r * r) * dt;
+            const float dx2 = dxh * dxh;
+            const float dxr = dxh * r;
+            const float ps = (float)(1. - ((double)dt * (2. * (double)D + (double)rrdt)) / (double)dx2);
+            const double den = 2. * (double)D + (double)rrdt;
+            p1 = (float)((1. - (double)ps) * (1. + (double)dxr / den) / 2.);
+            p2 = (float)((double)p1 + (1. - (double)ps) * (1. - (double)dxr / den) / 2.);
+        }
         const int base = axis ? 2 : 0;
         int l[4] = {0, 0, 0, 0};
         uint32_t ctr = 0, w0 = 0, w1 = 0;
@@ -411,6 +464,24 @@ extern "C" void oracle_inhom_sweep(int32_t *state, int W, int H, int species, in
     }
 }
 
+// inhomogeneous_diffuseX/Y + inhomogeneous_updateStateX/Y
+// (StochasticInhomogeneousDriftDiffusion.cl:426-723); key {cell + s*A, 0xdecafbad+seed};
+// probabilities at :518-520 / :648-650 with the reference's float/double promotions.
+// Pinned by: conservation, per-sweep mean r dt and variance 2 D dt closed forms.
+extern "C" void oracle_inhom_sweep(int32_t *state, int W, int H, int species, int axis, int32_t seed, float dt,
+                                   const float *Dax, const float *rax, float dxh, const int *bm, int nthreads) {
+    sweep_impl(state, W, H, species, axis, seed, dt, Dax, rax, dxh, bm, nthreads, false);
+}
+
+// The same sweep of the stochastic_inhomogeneous_fpe solver (the GPGMP_SOLVER_STOCHASTIC_
+// INHOMOGENEOUS_FPE branches of inhomogeneous_diffuseX/Y, :491-500, :626-633).
+// Pinned by: conservation, the per-sweep mean hop balance (D/h^2 + r/2h - D/h^2 + r/2h) dt h
+// = r dt and second moment 2 D dt (tests/test_oracle_fpe.py).
+extern "C" void oracle_fpe_sweep(int32_t *state, int W, int H, int species, int axis, int32_t seed, float dt,
+                                 const float *Dax, const float *rax, float dxh, const int *bm, int nthreads) {
+    sweep_impl(state, W, H, species, axis, seed, dt, Dax, rax, dxh, bm, nthreads, true);
+}
+
 // DiffusionModel::oclSolve host loop (DiffusionModel.cpp:1414-1620) with the
 // solver semantics of HomogeneousSolver::diffuse (HomogeneousSolver.cpp:132-164)
 // and InhomogeneousSolver::diffuse / getNextDiffusionTime
@@ -443,8 +514,13 @@ static int run_loop(const oracle_model *m, const oracle_reaction_table *t, int32
         }
     } else {
         for (int s = 0; s < S; ++s) {
-            dtx[s] = oracle_inhomogeneous_dt(m->Dx + (size_t)s * A, m->rx + (size_t)s * A, A, h);
-            dty[s] = oracle_inhomogeneous_dt(m->Dy + (size_t)s * A, m->ry + (size_t)s * A, A, h);
+            if (m->inhomogeneous == 2) {
+                dtx[s] = oracle_fpe_dt(m->Dx + (size_t)s * A, m->rx + (size_t)s * A, A, h, stay);
+                dty[s] = oracle_fpe_dt(m->Dy + (size_t)s * A, m->ry + (size_t)s * A, A, h, stay);
+            } else {
+                dtx[s] = oracle_inhomogeneous_dt(m->Dx + (size_t)s * A, m->rx + (size_t)s * A, A, h);
+                dty[s] = oracle_inhomogeneous_dt(m->Dy + (size_t)s * A, m->ry + (size_t)s * A, A, h);
+            }
             nx[s] = dtx[s];
             ny[s] = dty[s];
         }
@@ -508,15 +584,15 @@ static int run_loop(const oracle_model *m, const oracle_reaction_table *t, int32
             } else {
                 if (nx[i] <= mindt) {
                     if (run)
-                        oracle_inhom_sweep(state, m->W, m->H, i, 0, seed, dtx[i], m->Dx, m->rx, h,
-                                           m->boundary_mask, nthreads);
+                        sweep_impl(state, m->W, m->H, i, 0, seed, dtx[i], m->Dx, m->rx, h, m->boundary_mask,
+                                   nthreads, m->inhomogeneous == 2);
                     seed = (int32_t)((uint32_t)seed + 1u);
                     nx[i] += dtx[i];
                 }
                 if (ny[i] <= mindt) {
                     if (run)
-                        oracle_inhom_sweep(state, m->W, m->H, i, 1, seed, dty[i], m->Dy, m->ry, hy,
-                                           m->boundary_mask, nthreads);
+                        sweep_impl(state, m->W, m->H, i, 1, seed, dty[i], m->Dy, m->ry, hy, m->boundary_mask,
+                                   nthreads, m->inhomogeneous == 2);
                     seed = (int32_t)((uint32_t)seed + 1u);
                     ny[i] += dty[i];
                 }
