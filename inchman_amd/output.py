@@ -2,9 +2,9 @@
 
 The reference writes one HDF5 file per run set (DiffusionModel::writeAllSpecies and
 writeSingleTimeHDF5, DiffusionModel.cpp:934-1296; layout in doc/source/hdf5.rst) and reads
-it back with python/gpgmp/io.py:read_gmp_hdf5 (io.py:189-291).  No HDF5 library is part of
-this build, so the same tree is stored as a NumPy ``.npz`` archive whose keys are the HDF5
-paths:
+it back with python/gpgmp/io.py:read_gmp_hdf5 (io.py:189-291).  inchman_amd.hdf5 writes that
+file through the system's HDF5 C library; this module stores the same tree as a NumPy
+``.npz`` archive whose keys are the HDF5 paths, for hosts without libhdf5:
 
     Model/numRuns               int32[1]
     Model/numSpecies            int32[1]
