@@ -110,6 +110,11 @@ SIGNATURES = {
     "gpgmp_model_rate_literals": [_P, _pI, _pD],
     "gpgmp_model_add_reaction_compartment": [_P, _I, _I, _I, _I, _I],
     "gpgmp_model_localized": [_P, _pU],
+    "gpgmp_model_set_species_ids": [_P, C.POINTER(C.c_char_p)],
+    "gpgmp_model_set_parameter": [_P, C.c_char_p, _F],
+    "gpgmp_model_set_kinetic_law": [_P, _I, C.c_char_p],
+    "gpgmp_model_law_hazard": [_P, _I, _pI, _I, _I, _F, _U, _I, _pF],
+    "gpgmp_model_law_rhs": [_P, _I, _I, _pF, _I, _I, _U, _pF],
     "gpgmp_solver_create": [_P, _P, _U, C.POINTER(_P)],
     "gpgmp_solver_destroy": [_P],
     "gpgmp_solver_set_state": [_P, _pI],

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "det_solver.h"
+#include "law_compile.h"
 #include "host.h"
 
 using namespace gpgmp;
@@ -133,6 +134,58 @@ int gpgmp_model_add_reaction_compartment(gpgmp_model *m, int32_t reaction, int32
         m->m.compartments.push_back({reaction, x0, y0, x1, y1});
         m->m.table.localized |= 1u << reaction;
         m->m.table.compact = 0;  // the localized law is formed by the general table walk
+    });
+}
+
+int gpgmp_model_set_species_ids(gpgmp_model *m, const char *const *ids) {
+    return guarded([&] {
+        GPGMP_REQUIRE(m && ids, GPGMP_E_INVALID, "null argument");
+        std::vector<std::string> v;
+        for (int s = 0; s < m->m.desc.num_species; ++s) {
+            GPGMP_REQUIRE(ids[s] && ids[s][0], GPGMP_E_INVALID, "empty species id");
+            v.push_back(ids[s]);
+        }
+        m->m.species_ids = v;
+    });
+}
+
+int gpgmp_model_set_parameter(gpgmp_model *m, const char *name, float value) {
+    return guarded([&] {
+        GPGMP_REQUIRE(m && name && name[0], GPGMP_E_INVALID, "null argument");
+        m->m.params[name] = value;
+    });
+}
+
+int gpgmp_model_set_kinetic_law(gpgmp_model *m, int32_t reaction, const char *formula) {
+    return guarded([&] {
+        GPGMP_REQUIRE(m && formula, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(reaction >= 0 && reaction < m->m.table.R, GPGMP_E_INVALID, "no such reaction");
+        Model trial = m->m;
+        trial.custom_law.resize((size_t)trial.table.R);
+        trial.custom_law[(size_t)reaction] = formula;
+        RxTable T = trial.table;
+        (void)build_law_programs(trial, T);  // throws with the reason if the formula does not compile
+        m->m.custom_law = trial.custom_law;
+    });
+}
+
+int gpgmp_model_law_hazard(const gpgmp_model *m, int32_t reaction, const int32_t *x, int32_t gx, int32_t gy,
+                           float h0, uint32_t act, int32_t how, float *out) {
+    return guarded([&] {
+        GPGMP_REQUIRE(m && x && out, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(reaction >= 0 && reaction < m->m.table.R, GPGMP_E_INVALID, "no such reaction");
+        GPGMP_REQUIRE(how >= 0 && how <= 2, GPGMP_E_INVALID, "how must be 0, 1 or 2");
+        *out = law_hazard_host(m->m, reaction, x, gx, gy, h0, act, how);
+    });
+}
+
+int gpgmp_model_law_rhs(const gpgmp_model *m, int32_t species, int32_t which, const float *y, int32_t gx,
+                        int32_t gy, uint32_t act, float *out) {
+    return guarded([&] {
+        GPGMP_REQUIRE(m && y && out, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(species >= 0 && species < m->m.table.S && which >= 0 && which <= 2, GPGMP_E_INVALID,
+                      "bad species or expression");
+        (void)law_rhs_host(m->m, species, which, y, gx, gy, act, out);
     });
 }
 

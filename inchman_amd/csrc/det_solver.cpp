@@ -21,6 +21,16 @@ DetEngine::DetEngine(const Model &m, uint32_t flags, hipStream_t stream) : m_(m)
     x_.alloc((size_t)S_ * A_);
     tmp_.alloc(A_);
     m_.table.act = nullptr;
+    m_.table.grid_w = W_;
+    {
+        const std::vector<uint32_t> code = build_law_programs(m_, m_.table);
+        m_.table.prog = nullptr;
+        if (!code.empty()) {
+            prog_.alloc(code.size());
+            GPGMP_HIP_CHECK(hipMemcpy(prog_.get(), code.data(), code.size() * 4, hipMemcpyHostToDevice));
+            m_.table.prog = prog_.get();
+        }
+    }
     if (m_.table.localized) {
         std::vector<uint16_t> act(A_);
         reaction_act(m_, W_, H_, act.data());

@@ -44,6 +44,7 @@ class DetEngine {
     size_t A_ = 0;
     DevBuf<float> x_, tmp_, Dx_, Dy_, rx_, ry_;
     DevBuf<uint16_t> act_;  // active-reaction bits per cell (localized reactions only)
+    DevBuf<uint32_t> prog_;  // kinetic-law programs (custom laws only)
     std::vector<float> dtx_, dty_, nx_, ny_;
     float sim_time_ = 0.f;
     int64_t iterations_ = 0;

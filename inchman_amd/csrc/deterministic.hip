@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gpgmp_numeric.h"
+#include "law.h"
 #include "stochastic_types.h"
 
 namespace gpgmp {
@@ -165,8 +166,16 @@ __global__ __launch_bounds__(256) void k_det_inhom_sweep4(const float *__restric
 // followed by a clamp at 0.  In place: each cell touches only itself.
 // A localized reaction's law is multiplied by the cell's mask value w (0 or 1,
 // Solver.cpp:947-948): act holds the cell's active-reaction bits (RxTable::act).
-template <int S>
-__device__ __forceinline__ void rk4_rhs(const RxTable &T, const float (&x)[S], float (&d)[S], uint32_t act) {
+// A model with custom laws takes each species' generated derivative expression as a program
+// (law.h; RxTable::det_rk).
+template <int S, bool CL>
+__device__ __forceinline__ void rk4_rhs(const RxTable &T, const float (&x)[S], float (&d)[S], const LawCell &lc) {
+    if constexpr (CL) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) d[s] = law_call<S>(T.prog + T.det_rk[s], T.det_rk_t[s], x, lc);
+        return;
+    }
+    const uint32_t act = lc.act;
 #pragma unroll
     for (int s = 0; s < S; ++s) d[s] = 0.f;
     for (int i = 0; i < T.R; ++i) {
@@ -209,10 +218,10 @@ __device__ __forceinline__ void rk4_rhs_compact(const RxTable &T, const float (&
     }
 }
 
-template <int S, bool COMPACT>
-__device__ __forceinline__ void rk4_rhs_t(const RxTable &T, const float (&x)[S], float (&d)[S], uint32_t act) {
+template <int S, bool COMPACT, bool CL>
+__device__ __forceinline__ void rk4_rhs_t(const RxTable &T, const float (&x)[S], float (&d)[S], const LawCell &act) {
     if constexpr (COMPACT) rk4_rhs_compact<S>(T, x, d);
-    else rk4_rhs<S>(T, x, d, act);
+    else rk4_rhs<S, CL>(T, x, d, act);
 }
 
 // The 20 substep increments are accumulated with a compensation term per species (Kahan
@@ -221,9 +230,9 @@ __device__ __forceinline__ void rk4_rhs_t(const RxTable &T, const float (&x)[S],
 // state of 64, ulp 7.6e-6: 16.4 ulps rounded to 16 each substep, a 2.4% loss of the call's
 // increment), and the call ends within one rounding of the exact sum of its increments.  The
 // right-hand sides see the rounded state, as the reference's do.
-template <int S, bool COMPACT>
+template <int S, bool COMPACT, bool CL>
 __device__ __forceinline__ void rk4_cell(float *__restrict__ state, int A, int c, const RxTable &T, float dt) {
-    const uint32_t act = T.localized ? (uint32_t)T.act[c] : 0xffffu;
+    const LawCell act{c % T.grid_w, c / T.grid_w, T.localized ? (uint32_t)T.act[c] : 0xffffu, 0.f};
     float x[S], k1[S], k2[S], k3[S], k4[S], tmp[S], comp[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -232,16 +241,16 @@ __device__ __forceinline__ void rk4_cell(float *__restrict__ state, int A, int c
     }
     const float h = dt / 20.f;
     for (int n = 0; n < 20; ++n) {
-        rk4_rhs_t<S, COMPACT>(T, x, k1, act);
+        rk4_rhs_t<S, COMPACT, CL>(T, x, k1, act);
 #pragma unroll
         for (int s = 0; s < S; ++s) tmp[s] = x[s] + h / 2.f * k1[s];
-        rk4_rhs_t<S, COMPACT>(T, tmp, k2, act);
+        rk4_rhs_t<S, COMPACT, CL>(T, tmp, k2, act);
 #pragma unroll
         for (int s = 0; s < S; ++s) tmp[s] = x[s] + h / 2.f * k2[s];
-        rk4_rhs_t<S, COMPACT>(T, tmp, k3, act);
+        rk4_rhs_t<S, COMPACT, CL>(T, tmp, k3, act);
 #pragma unroll
         for (int s = 0; s < S; ++s) tmp[s] = x[s] + h * k3[s];
-        rk4_rhs_t<S, COMPACT>(T, tmp, k4, act);
+        rk4_rhs_t<S, COMPACT, CL>(T, tmp, k4, act);
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const float inc = h / 6.f * (k1[s] + 2.f * k2[s] + 2.f * k3[s] + k4[s]);
@@ -259,12 +268,13 @@ __device__ __forceinline__ void rk4_cell(float *__restrict__ state, int A, int c
     for (int s = 0; s < S; ++s) state[(size_t)s * A + c] = x[s];
 }
 
-template <int S>
+// CL: the instance with the law programs of custom laws (RxTable::det_custom).
+template <int S, bool CL>
 __global__ void k_det_rk4(float *__restrict__ state, int A, RxTable T, float dt) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= A) return;
-    if (T.compact) rk4_cell<S, true>(state, A, c, T, dt);
-    else rk4_cell<S, false>(state, A, c, T, dt);
+    if (!CL && T.compact) rk4_cell<S, true, false>(state, A, c, T, dt);
+    else rk4_cell<S, false, CL>(state, A, c, T, dt);
 }
 
 // deterministic_performReaction_aqss (Deterministic.cl:291-522): Mott & Oran's alpha-QSS
@@ -281,9 +291,18 @@ constexpr int AQSS_MAX_ITER = 1 << 16;
 
 // A localized reaction's term is |n|. * w * <law> (Solver.cpp:987-1000): the double chain
 // starts at |n| * w for the cell's mask value w (0 or 1).
-template <int S>
+template <int S, bool CL>
 __device__ __forceinline__ void aqss_rhs(const RxTable &T, const float (&y)[S], float (&q)[S], float (&d)[S],
-                                         uint32_t act) {
+                                         const LawCell &lc) {
+    if constexpr (CL) {  // the generated sums as programs (RxTable::det_q, det_d)
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            q[s] = law_call<S>(T.prog + T.det_q[s], T.det_q_t[s], y, lc);
+            d[s] = law_call<S>(T.prog + T.det_d[s], T.det_d_t[s], y, lc);
+        }
+        return;
+    }
+    const uint32_t act = lc.act;
     double qs[S], ds[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) qs[s] = ds[s] = 0.;
@@ -314,19 +333,19 @@ __device__ __forceinline__ void aqss_rhs(const RxTable &T, const float (&y)[S], 
     }
 }
 
-template <int S>
+template <int S, bool CL>
 __global__ __launch_bounds__(256) void k_det_aqss(float *__restrict__ state, int A, RxTable T, float dtg,
                                                   int fixed) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= A) return;
-    const uint32_t act = T.localized ? (uint32_t)T.act[c] : 0xffffu;
+    const LawCell act{c % T.grid_w, c / T.grid_w, T.localized ? (uint32_t)T.act[c] : 0xffffu, 0.f};
     const float epsmin = 1e-2f, sqreps = 0.5f, epscl = 100.f, epsmax = 10.f, tfd = 1.000008f, dtmin = 1e-15f,
                 ymin = 1e-20f;
     float y[S], q[S], d[S], ys[S], qsv[S], rtaus[S], y1[S], scr[S];
     float tn = 0.f;
 #pragma unroll
     for (int i = 0; i < S; ++i) y[i] = fmaxf(state[(size_t)i * A + c], ymin);
-    aqss_rhs<S>(T, y, q, d, act);
+    aqss_rhs<S, CL>(T, y, q, d, act);
     float scrtch = 1e-25f;
 #pragma unroll
     for (int i = 0; i < S; ++i) {
@@ -368,7 +387,7 @@ __global__ __launch_bounds__(256) void k_det_aqss(float *__restrict__ state, int
 #pragma unroll
                     for (int i = 0; i < S; ++i) y1[i] = y[i];
                 }
-                aqss_rhs<S>(T, y, q, d, act);
+                aqss_rhs<S, CL>(T, y, q, d, act);
                 eps = 1e-10f;
 #pragma unroll
                 for (int i = 0; i < S; ++i) {
@@ -411,7 +430,7 @@ __global__ __launch_bounds__(256) void k_det_aqss(float *__restrict__ state, int
 #pragma unroll
             for (int i = 0; i < S; ++i) rtaus[i] *= dto;
         }
-        if (!capped) aqss_rhs<S>(T, y, q, d, act);
+        if (!capped) aqss_rhs<S, CL>(T, y, q, d, act);
     }
 #pragma unroll
     for (int i = 0; i < S; ++i) state[(size_t)i * A + c] = capped ? __builtin_nanf("") : y[i];
@@ -763,15 +782,16 @@ hipError_t launch_det_inhom_sweep(const float *in, float *out, const float *D, c
 
 hipError_t launch_det_aqss(float *state, int A, const RxTable &T, float dt, int fixed, hipStream_t s) {
     const dim3 g((A + 255) / 256), b(256);
+    const bool cl = T.det_custom != 0;
     switch (T.S) {
-        case 1: hipLaunchKernelGGL(k_det_aqss<1>, g, b, 0, s, state, A, T, dt, fixed); break;
-        case 2: hipLaunchKernelGGL(k_det_aqss<2>, g, b, 0, s, state, A, T, dt, fixed); break;
-        case 3: hipLaunchKernelGGL(k_det_aqss<3>, g, b, 0, s, state, A, T, dt, fixed); break;
-        case 4: hipLaunchKernelGGL(k_det_aqss<4>, g, b, 0, s, state, A, T, dt, fixed); break;
-        case 5: hipLaunchKernelGGL(k_det_aqss<5>, g, b, 0, s, state, A, T, dt, fixed); break;
-        case 6: hipLaunchKernelGGL(k_det_aqss<6>, g, b, 0, s, state, A, T, dt, fixed); break;
-        case 7: hipLaunchKernelGGL(k_det_aqss<7>, g, b, 0, s, state, A, T, dt, fixed); break;
-        case 8: hipLaunchKernelGGL(k_det_aqss<8>, g, b, 0, s, state, A, T, dt, fixed); break;
+        case 1: hipLaunchKernelGGL((cl ? k_det_aqss<1, true> : k_det_aqss<1, false>), g, b, 0, s, state, A, T, dt, fixed); break;
+        case 2: hipLaunchKernelGGL((cl ? k_det_aqss<2, true> : k_det_aqss<2, false>), g, b, 0, s, state, A, T, dt, fixed); break;
+        case 3: hipLaunchKernelGGL((cl ? k_det_aqss<3, true> : k_det_aqss<3, false>), g, b, 0, s, state, A, T, dt, fixed); break;
+        case 4: hipLaunchKernelGGL((cl ? k_det_aqss<4, true> : k_det_aqss<4, false>), g, b, 0, s, state, A, T, dt, fixed); break;
+        case 5: hipLaunchKernelGGL((cl ? k_det_aqss<5, true> : k_det_aqss<5, false>), g, b, 0, s, state, A, T, dt, fixed); break;
+        case 6: hipLaunchKernelGGL((cl ? k_det_aqss<6, true> : k_det_aqss<6, false>), g, b, 0, s, state, A, T, dt, fixed); break;
+        case 7: hipLaunchKernelGGL((cl ? k_det_aqss<7, true> : k_det_aqss<7, false>), g, b, 0, s, state, A, T, dt, fixed); break;
+        case 8: hipLaunchKernelGGL((cl ? k_det_aqss<8, true> : k_det_aqss<8, false>), g, b, 0, s, state, A, T, dt, fixed); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -779,15 +799,16 @@ hipError_t launch_det_aqss(float *state, int A, const RxTable &T, float dt, int 
 
 hipError_t launch_det_rk4(float *state, int A, const RxTable &T, float dt, hipStream_t s) {
     const dim3 g((A + 255) / 256), b(256);
+    const bool cl = T.det_custom != 0;
     switch (T.S) {
-        case 1: hipLaunchKernelGGL(k_det_rk4<1>, g, b, 0, s, state, A, T, dt); break;
-        case 2: hipLaunchKernelGGL(k_det_rk4<2>, g, b, 0, s, state, A, T, dt); break;
-        case 3: hipLaunchKernelGGL(k_det_rk4<3>, g, b, 0, s, state, A, T, dt); break;
-        case 4: hipLaunchKernelGGL(k_det_rk4<4>, g, b, 0, s, state, A, T, dt); break;
-        case 5: hipLaunchKernelGGL(k_det_rk4<5>, g, b, 0, s, state, A, T, dt); break;
-        case 6: hipLaunchKernelGGL(k_det_rk4<6>, g, b, 0, s, state, A, T, dt); break;
-        case 7: hipLaunchKernelGGL(k_det_rk4<7>, g, b, 0, s, state, A, T, dt); break;
-        case 8: hipLaunchKernelGGL(k_det_rk4<8>, g, b, 0, s, state, A, T, dt); break;
+        case 1: hipLaunchKernelGGL((cl ? k_det_rk4<1, true> : k_det_rk4<1, false>), g, b, 0, s, state, A, T, dt); break;
+        case 2: hipLaunchKernelGGL((cl ? k_det_rk4<2, true> : k_det_rk4<2, false>), g, b, 0, s, state, A, T, dt); break;
+        case 3: hipLaunchKernelGGL((cl ? k_det_rk4<3, true> : k_det_rk4<3, false>), g, b, 0, s, state, A, T, dt); break;
+        case 4: hipLaunchKernelGGL((cl ? k_det_rk4<4, true> : k_det_rk4<4, false>), g, b, 0, s, state, A, T, dt); break;
+        case 5: hipLaunchKernelGGL((cl ? k_det_rk4<5, true> : k_det_rk4<5, false>), g, b, 0, s, state, A, T, dt); break;
+        case 6: hipLaunchKernelGGL((cl ? k_det_rk4<6, true> : k_det_rk4<6, false>), g, b, 0, s, state, A, T, dt); break;
+        case 7: hipLaunchKernelGGL((cl ? k_det_rk4<7, true> : k_det_rk4<7, false>), g, b, 0, s, state, A, T, dt); break;
+        case 8: hipLaunchKernelGGL((cl ? k_det_rk4<8, true> : k_det_rk4<8, false>), g, b, 0, s, state, A, T, dt); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

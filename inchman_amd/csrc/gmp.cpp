@@ -266,6 +266,15 @@ GmpEngine::GmpEngine(const Model &m, const std::vector<int> &W, const std::vecto
         total_cells_ += (int64_t)A;
         reals_.push_back(ri);
     }
+    {
+        const std::vector<uint32_t> code = build_law_programs(m, table_);
+        table_.prog = nullptr;
+        if (!code.empty()) {
+            prog_.alloc(code.size());
+            GPGMP_HIP_CHECK(hipMemcpy(prog_.get(), code.data(), code.size() * 4, hipMemcpyHostToDevice));
+            table_.prog = prog_.get();
+        }
+    }
     if (table_.localized) {
         std::vector<uint16_t> act((size_t)total_cells_);
         for (size_t r = 0; r < reals_.size(); ++r)

@@ -9,6 +9,7 @@
 #include <stdexcept>
 #include <string>
 #include <functional>
+#include <map>
 #include <vector>
 
 #include "../../include/gpgmp_hip.h"
@@ -82,6 +83,12 @@ struct Model {
     // compartments of the localized reactions: {reaction, x0, y0, x1, y1} with inclusive
     // cell bounds (Reaction::addCompartment, Compartment::isInCompartment)
     std::vector<std::array<int32_t, 5>> compartments;
+    // names the custom laws use: Species::id per species index, the user parameters
+    // (DiffusionModel::setParameter, a Real each), and per reaction its custom law
+    // (Reaction::setCustomLaw; empty: the law generated from the stoichiometry)
+    std::vector<std::string> species_ids;
+    std::map<std::string, float> params;
+    std::vector<std::string> custom_law;
     bool inhomogeneous() const {
         return desc.solver_type == GPGMP_STOCHASTIC_INHOMOGENEOUS ||
                desc.solver_type == GPGMP_STOCHASTIC_INHOMOGENEOUS_FPE ||
@@ -98,6 +105,19 @@ struct Model {
 };
 
 Model compile_model(const gpgmp_model_desc &d);
+// Applies the model's custom laws to a copy of its reaction table (law.cpp): mass-action
+// shaped laws become table entries, the others programs; a model with any custom law also
+// gets the deterministic right-hand sides as programs.  Returns the program words (to be
+// uploaded; T.prog is set by the caller) -- empty when no reaction needs a program.
+std::vector<uint32_t> build_law_programs(const Model &m, RxTable &T);
+inline bool has_custom_laws(const Model &m) {
+    for (const auto &c : m.custom_law)
+        if (!c.empty()) return true;
+    return false;
+}
+// The species id of index s (Species::id), "S<s>" when none was given.
+std::string species_id(const Model &m, int s);
+
 // Active-reaction bits per cell of a W x H grid (bit i: reaction i not localized, or the
 // cell inside one of its compartments; Reaction::reactionMask, Reaction.cpp:216-235).
 void reaction_act(const Model &m, int W, int H, uint16_t *out);
@@ -266,6 +286,7 @@ class GmpEngine {
     void build_tiles();
     std::vector<RealInfo> reals_;
     DevBuf<uint16_t> act_;  // active-reaction bits per cell (localized reactions only)
+    DevBuf<uint32_t> prog_;  // kinetic-law programs (custom laws only)
     DevBuf<int32_t> state_;
     DevBuf<int32_t> lost_[2];
     DevBuf<int4> prob_[2];

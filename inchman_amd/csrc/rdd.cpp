@@ -88,8 +88,9 @@ int gpgmp_det_rk4_device(const gpgmp_model *m, float *state, float dt, void *str
         GPGMP_REQUIRE(m && state, GPGMP_E_INVALID, "null argument");
         const int A = m->m.desc.width * m->m.desc.height;
         if (m->m.table.R == 0) return;
-        GPGMP_REQUIRE(!m->m.table.localized, GPGMP_E_UNSUPPORTED,
-                      "localized reactions need a deterministic solver (it owns the reaction mask)");
+        GPGMP_REQUIRE(!m->m.table.localized && !has_custom_laws(m->m), GPGMP_E_UNSUPPORTED,
+                      "localized reactions and custom laws need a deterministic solver (it owns the mask and "
+                      "the law programs)");
         GPGMP_HIP_CHECK(launch_det_rk4(state, A, m->m.table, dt, (hipStream_t)stream));
     });
 }
@@ -100,8 +101,9 @@ int gpgmp_det_aqss_device(const gpgmp_model *m, float *state, float dt, int32_t 
         GPGMP_REQUIRE((flags & ~GPGMP_AQSS_PADE_FIX) == 0, GPGMP_E_INVALID, "unknown flags");
         const int A = m->m.desc.width * m->m.desc.height;
         if (m->m.table.R == 0) return;
-        GPGMP_REQUIRE(!m->m.table.localized, GPGMP_E_UNSUPPORTED,
-                      "localized reactions need a deterministic solver (it owns the reaction mask)");
+        GPGMP_REQUIRE(!m->m.table.localized && !has_custom_laws(m->m), GPGMP_E_UNSUPPORTED,
+                      "localized reactions and custom laws need a deterministic solver (it owns the mask and "
+                      "the law programs)");
         GPGMP_HIP_CHECK(launch_det_aqss(state, A, m->m.table, dt, flags & GPGMP_AQSS_PADE_FIX, (hipStream_t)stream));
     });
 }

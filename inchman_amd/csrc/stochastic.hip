@@ -121,11 +121,13 @@ __global__ __launch_bounds__(256) void k_min_reduce(const float *__restrict__ in
 hipError_t launch_gmp_step(const StepLaunch &a) {
     if (a.ntiles <= 0) return hipSuccess;
     if (a.block <= 0 || a.block > 256 || (a.block % 64) != 0 || a.cpt <= 0) return hipErrorInvalidValue;
+    // localized reactions, float literals or law programs: the extended instances
+    const bool cl = (a.T->custom | a.T->localized | a.T->lit_is_float) != 0;
     switch (a.mode) {
-        case 0: return launch_gmp_hom(a);
-        case 1: return launch_gmp_inh_x(a);
+        case 0: return cl ? launch_gmp_hom_cl(a) : launch_gmp_hom(a);
+        case 1: return cl ? launch_gmp_inh_x_cl(a) : launch_gmp_inh_x(a);
         case 2: return launch_gmp_inh_y(a);
-        case 3: return launch_gmp_inh_xs(a);
+        case 3: return cl ? launch_gmp_inh_xs_cl(a) : launch_gmp_inh_xs(a);
         default: return hipErrorInvalidValue;
     }
 }

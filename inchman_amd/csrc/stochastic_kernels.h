@@ -25,11 +25,26 @@
 #include <algorithm>
 
 #include "gpgmp_numeric.h"
+#include "law.h"
 #include "stochastic_launch.h"
 #include "stochastic_types.h"
 #include "wave_ops.h"
 
+// Two instantiations of every kernel below: the plain one (int and double rate literals of the
+// reaction table) and the extended one for models with localized reactions, float literals or
+// custom laws (law programs) -- needs_extended_laws().  The plain kernels carry none of the
+// extended code, so they keep their registers; GPGMP_EXTENDED_LAWS selects which one a
+// translation unit builds, each in its own inline namespace so their symbols stay apart.
+#ifndef GPGMP_EXTENDED_LAWS
+#define GPGMP_EXTENDED_LAWS 0
+#endif
+
 namespace gpgmp {
+#if GPGMP_EXTENDED_LAWS
+inline namespace extended_laws {
+#else
+inline namespace table_laws {
+#endif
 
 // MODE_INH_XS: the X sweeps alone (no SSA code, so the sweep-only kernel keeps a small
 // register footprint), launched after the SSA launch when the two are split.
@@ -47,25 +62,52 @@ __device__ __forceinline__ int32_t sel(const int32_t (&x)[S], int j) {
     return v;
 }
 
-// _h[i+1] = _h[0] + <law_i>; int literal -> int product then float add,
-// double literal -> double product chain rounded once (Solver.cpp:849-857).
-// A localized reaction's law is `w * <law_i>` with the float mask value w of the cell
+// The cell a hazard is evaluated in: its active-reaction bits (RxTable::act) and, for the
+// law programs, its grid coordinates (GridX, GridY of the generated kernel).
+struct CellCtx {
+    uint32_t act;
+    int32_t gx, gy;
+};
+
+// _h[i+1] = _h[0] + <law_i>; int literal -> int product then float add, double literal ->
+// double product chain rounded once (Solver.cpp:849-857).
+#if GPGMP_EXTENDED_LAWS
+// Extended: a float literal (a user parameter, `const Real k`) -> float product chain; a
+// localized reaction's law is `w * <law_i>` with the float mask value w of the cell
 // (Solver.cpp:853-854): the chain then starts at w, so an int literal's product is formed in
-// float ((w * lit) * X * (X - 1) ...) and a double literal's in double from (double)w * lit.
+// float ((w * lit) * X * (X - 1) ...) and a double literal's in double from (double)w * lit; a
+// custom law of another shape is its program (law.h), which forms the whole assignment
+// `_h[0] + [w *] <law>`.
 template <int S>
-__device__ __forceinline__ float add_law(const RxTable &T, int i, float h0, const int32_t (&x)[S], uint32_t act) {
+__device__ __forceinline__ float add_law(const RxTable &T, int i, float h0, const int32_t (&x)[S], const CellCtx &cx) {
+    if ((T.custom >> i) & 1u)
+        return law_call<S>(T.prog + T.prog_off[i], T.prog_type[i], x, LawCell{cx.gx, cx.gy, cx.act, h0});
     const int nf = T.nf[i];
-    if ((T.localized >> i) & 1u) {
-        const float w = ((act >> i) & 1u) ? 1.f : 0.f;
-        if ((T.lit_is_int >> i) & 1u) {
+    const bool loc = (T.localized >> i) & 1u;
+    const float w = ((cx.act >> i) & 1u) ? 1.f : 0.f;
+    if ((T.lit_is_int >> i) & 1u) {
+        if (loc) {
             float v = w * (float)T.lit_int[i];
             for (int f = 0; f < nf; ++f) v = v * (float)(sel(x, T.fs[i][f]) - (int32_t)T.fo[i][f]);
             return h0 + v;
         }
-        double v = (double)w * T.lit_dbl[i];
-        for (int f = 0; f < nf; ++f) v = v * (double)(sel(x, T.fs[i][f]) - (int32_t)T.fo[i][f]);
-        return (float)((double)h0 + v);
+        uint32_t v = (uint32_t)T.lit_int[i];
+        for (int f = 0; f < nf; ++f) v *= (uint32_t)(sel(x, T.fs[i][f]) - (int32_t)T.fo[i][f]);
+        return h0 + (float)(int32_t)v;
     }
+    if ((T.lit_is_float >> i) & 1u) {
+        float v = loc ? w * T.lit_flt[i] : T.lit_flt[i];
+        for (int f = 0; f < nf; ++f) v = v * (float)(sel(x, T.fs[i][f]) - (int32_t)T.fo[i][f]);
+        return h0 + v;
+    }
+    double v = loc ? (double)w * T.lit_dbl[i] : T.lit_dbl[i];
+    for (int f = 0; f < nf; ++f) v = v * (double)(sel(x, T.fs[i][f]) - (int32_t)T.fo[i][f]);
+    return (float)((double)h0 + v);
+}
+#else
+template <int S>
+__device__ __forceinline__ float add_law(const RxTable &T, int i, float h0, const int32_t (&x)[S], const CellCtx &) {
+    const int nf = T.nf[i];
     if ((T.lit_is_int >> i) & 1u) {
         uint32_t v = (uint32_t)T.lit_int[i];
         for (int f = 0; f < nf; ++f) v *= (uint32_t)(sel(x, T.fs[i][f]) - (int32_t)T.fo[i][f]);
@@ -75,6 +117,7 @@ __device__ __forceinline__ float add_law(const RxTable &T, int i, float h0, cons
     for (int f = 0; f < nf; ++f) v = v * (double)(sel(x, T.fs[i][f]) - (int32_t)T.fo[i][f]);
     return (float)((double)h0 + v);
 }
+#endif
 
 // Cumulative hazards _h[1..R] and the total _h[0] (Solver.cpp:843-857).
 // Compact tables: every law of the first kCompactReactions is formed at once in straight-line
@@ -108,26 +151,40 @@ __device__ __forceinline__ void hazards_compact(const RxTable &T, const int32_t 
 #pragma unroll
             for (int i = 0; i < kCompactReactions; ++i) vi[i] *= (uint32_t)d[i][f];
     }
+    float vf[kCompactReactions] = {};
+    if (GPGMP_EXTENDED_LAWS && T.lit_is_float) {  // an unused factor is 1.f: v * 1.f is v
+#pragma unroll
+        for (int i = 0; i < kCompactReactions; ++i) vf[i] = T.lit_flt[i];
+#pragma unroll
+        for (int f = 0; f < kCompactFactors; ++f)
+#pragma unroll
+            for (int i = 0; i < kCompactReactions; ++i) vf[i] = vf[i] * (float)d[i][f];
+    }
     float t = 0.f;
 #pragma unroll
     for (int i = 0; i < kCompactReactions; ++i) {
-        const float hi = ((T.lit_is_int >> i) & 1u) ? t + (float)(int32_t)vi[i] : (float)((double)t + vd[i]);
+        const float hi = ((T.lit_is_int >> i) & 1u)                            ? t + (float)(int32_t)vi[i]
+                         : (GPGMP_EXTENDED_LAWS && ((T.lit_is_float >> i) & 1u)) ? t + vf[i]
+                                                                                 : (float)((double)t + vd[i]);
         t = (i < T.R) ? hi : t;
         h[i + 1] = t;
     }
     h[0] = t;
 }
 
-// act: the cell's active-reaction bits (RxTable::act; ignored unless T.localized).
 template <int S>
 __device__ __forceinline__ void hazards(const RxTable &T, const int32_t (&x)[S], float (&h)[kMaxReactions + 1],
-                                        uint32_t act) {
+                                        const CellCtx &act) {
     if (T.compact) {
         hazards_compact<S>(T, x, h);
         return;
     }
     h[0] = 0.f;
+#if GPGMP_EXTENDED_LAWS
+#pragma unroll 1
+#else
 #pragma unroll
+#endif
     for (int i = 0; i < kMaxReactions; ++i) {
         if (i < T.R) {
             h[i + 1] = add_law(T, i, h[0], x, act);
@@ -171,7 +228,7 @@ __device__ __forceinline__ void fire(const RxTable &T, const float (&h)[kMaxReac
 template <int S, bool COMPACT>
 __device__ __forceinline__ bool ssa_cell_t(int32_t (&x)[S], uint32_t cell, int32_t seed, float deltat,
                                            const RxTable &T, uint32_t budget, int below, float &time,
-                                           uint32_t &ctr, float &rate, uint32_t act) {
+                                           uint32_t &ctr, float &rate, const CellCtx &act) {
     const uint32_t k1 = kKeyBase + (uint32_t)seed;
     if (!(time < deltat)) return true;
     // The draws of an event depend on its counter only, so they are formed one event
@@ -211,14 +268,20 @@ __device__ __forceinline__ bool ssa_cell_t(int32_t (&x)[S], uint32_t cell, int32
 template <int S>
 __device__ __forceinline__ bool ssa_cell(int32_t (&x)[S], uint32_t cell, int32_t seed, float deltat,
                                          const RxTable &T, uint32_t budget, int below, float &time,
-                                         uint32_t &ctr, float &rate, uint32_t act) {
+                                         uint32_t &ctr, float &rate, const CellCtx &act) {
     return T.compact ? ssa_cell_t<S, true>(x, cell, seed, deltat, T, budget, below, time, ctr, rate, act)
                      : ssa_cell_t<S, false>(x, cell, seed, deltat, T, budget, below, time, ctr, rate, act);
 }
 
-// The active-reaction bits of cell `gc` (cells of all realizations) for the hazards.
-__device__ __forceinline__ uint32_t cell_act(const RxTable &T, uint64_t gc) {
-    return T.localized ? (uint32_t)T.act[gc] : 0xffffu;
+// The hazards' view of cell c of a W-wide grid, gc its index among the cells of all
+// realizations (RxTable::act).
+__device__ __forceinline__ CellCtx cell_ctx(const RxTable &T, uint64_t gc, int c, int W) {
+#if GPGMP_EXTENDED_LAWS
+    return CellCtx{T.localized ? (uint32_t)T.act[gc] : 0xffffu, c % W, c / W};
+#else
+    (void)T, (void)gc, (void)c, (void)W;
+    return CellCtx{0xffffu, 0, 0};
+#endif
 }
 
 // Lane groups.  With G > 1, G consecutive lanes of a wave (G divides 64) share one
@@ -260,7 +323,7 @@ __device__ __forceinline__ int32_t grp_sum(int32_t v) {
 template <int S, int G>
 __device__ __forceinline__ bool ssa_cell_g(int32_t (&x)[S], uint32_t cell, int32_t seed, float deltat,
                                            const RxTable &T, uint32_t budget, int below, float &time,
-                                           uint32_t &ctr, float &rate, uint32_t act) {
+                                           uint32_t &ctr, float &rate, const CellCtx &act) {
     if constexpr (G == 1) {
         return ssa_cell<S>(x, cell, seed, deltat, T, budget, below, time, ctr, rate, act);
     } else {
@@ -737,7 +800,7 @@ __device__ __forceinline__ void step_tile(
           if (pl.deltat > 0.f && T.R > 0) {
             float time = 0.f, rate = 0.f;
             uint32_t ctr = 0;
-            const uint32_t act = cell_act(T, ri.cell_off + (uint64_t)c);
+            const CellCtx act = cell_ctx(T, ri.cell_off + (uint64_t)c, c, W);
             const bool fin = ssa_cell_g<S, G>(x, (uint32_t)c, seed, pl.deltat, T, hot.budget, hot.below_active,
                                               time, ctr, rate, act);
             // a deferred cell with a modest expected tail goes to the lane list, the others
@@ -886,7 +949,7 @@ constexpr int kSpecPasses = 8;
 
 template <int S>
 __device__ __forceinline__ void ssa_wave_spec(int32_t (&x)[S], uint32_t cell, uint32_t k1, float deltat,
-                                              const RxTable &T, float &time, uint32_t &ctr, uint32_t act) {
+                                              const RxTable &T, float &time, uint32_t &ctr, const CellCtx &act) {
     const int lane = (int)(threadIdx.x & 63);
     if (!(time < deltat)) return;
     while (true) {
@@ -982,7 +1045,7 @@ __device__ __forceinline__ void hot_cells(int wk, int nwk, int32_t *__restrict__
         float time = he.time;
         uint32_t ctr = he.ctr;
         ssa_wave_spec<S>(x, (uint32_t)c, kKeyBase + (uint32_t)seed, pss.deltat, T, time, ctr,
-                         cell_act(T, ri.cell_off + (uint64_t)c));
+                         cell_ctx(T, ri.cell_off + (uint64_t)c, c, W));
         // the whole count is known now: busy cells are predicted for the next step
         if (pss.predict && lane == 0)
             hot.pred[(size_t)pss.hot_parity * hot.pred_stride + ri.cell_off + c] = ctr > kPredictEvents ? 1 : 0;
@@ -1091,7 +1154,7 @@ __device__ __forceinline__ void hot_lane_cells(int wk, int nwk, int32_t *__restr
             float time = he.time, rate = 0.f;
             uint32_t ctr = he.ctr;
             (void)ssa_cell<S>(x, (uint32_t)c, seed, pss.deltat, T, 0xffffffffu, 0, time, ctr, rate,
-                              cell_act(T, ri.cell_off + (uint64_t)c));
+                              cell_ctx(T, ri.cell_off + (uint64_t)c, c, W));
         }
         if (pl.out_mask) {
 #pragma unroll
@@ -1151,7 +1214,7 @@ __device__ __forceinline__ void pred_cells(int wk, int nwk, int32_t *__restrict_
         uint32_t ctr = 0;
         if (pl.deltat > 0.f && T.R > 0)
             ssa_wave_spec<S>(x, (uint32_t)c, kKeyBase + (uint32_t)seed, pl.deltat, T, time, ctr,
-                             cell_act(T, ri.cell_off + (uint64_t)c));
+                             cell_ctx(T, ri.cell_off + (uint64_t)c, c, W));
         if (lane == 0) {
 #pragma unroll
             for (int s = 0; s < S; ++s) st[(size_t)s * A + c] = x[s];
@@ -1359,4 +1422,5 @@ static hipError_t launch_m(const StepLaunch &a) {
     }
 }
 
+}  // inline namespace
 }  // namespace gpgmp

@@ -47,6 +47,10 @@ hipError_t launch_gmp_hom(const StepLaunch &a);
 hipError_t launch_gmp_inh_x(const StepLaunch &a);
 hipError_t launch_gmp_inh_y(const StepLaunch &a);
 hipError_t launch_gmp_inh_xs(const StepLaunch &a);
+// the extended instances: localized reactions, float literals, law programs (stochastic_*_cl.hip)
+hipError_t launch_gmp_hom_cl(const StepLaunch &a);
+hipError_t launch_gmp_inh_x_cl(const StepLaunch &a);
+hipError_t launch_gmp_inh_xs_cl(const StepLaunch &a);
 hipError_t launch_min_dt(const float *D, const float *r, int n, float h, float *blocks, float *out,
                          hipStream_t stream);
 hipError_t launch_inh_probs(int4 *out, const float *D, const float *r, float dt, float dx, int n, int fpe,

@@ -22,8 +22,10 @@ constexpr uint32_t kMaxEventsPerCall = 1u << 24;
 struct RxTable {
     int32_t S, R;
     uint32_t lit_is_int;               // bit i: rate literal of reaction i is an int literal
+    uint32_t lit_is_float;             // bit i: ... a float (a user parameter, `const Real k`)
     int32_t lit_int[kMaxReactions];
-    double lit_dbl[kMaxReactions];
+    float lit_flt[kMaxReactions];
+    double lit_dbl[kMaxReactions];     // every literal's value as a double
     uint8_t nf[kMaxReactions];         // number of falling-factorial factors
     uint8_t fs[kMaxReactions][kMaxFactors];  // factor species
     uint8_t fo[kMaxReactions][kMaxFactors];  // factor offset m of (X - m)
@@ -35,6 +37,20 @@ struct RxTable {
     // localized != 0, and compact is then 0.
     uint32_t localized;
     const uint16_t *act;
+    // Custom laws that are not of the mass-action shape (law.h): bit i set when reaction i's
+    // hazard `_h[0] + [mask *] <law>` is the program at prog + prog_off[i] (result type
+    // prog_type[i]); compact is then 0.
+    uint32_t custom;
+    const uint32_t *prog;
+    uint16_t prog_off[kMaxReactions];
+    uint8_t prog_type[kMaxReactions];
+    // Deterministic right-hand sides as programs (det_custom != 0: a model with custom laws):
+    // per species the RK4 derivative `0. +- n. * [mask *] <law> ...` (Solver.cpp:918-955) and
+    // the alpha-QSS production and destruction sums (Solver.cpp:960-1010).
+    int32_t det_custom;
+    int32_t grid_w;  // deterministic kernels: grid width (GridX / GridY of the programs)
+    uint16_t det_rk[kMaxSpecies], det_q[kMaxSpecies], det_d[kMaxSpecies];
+    uint8_t det_rk_t[kMaxSpecies], det_q_t[kMaxSpecies], det_d_t[kMaxSpecies];
 };
 
 // One realization (grid) of a launch.  The plain solver is a batch of one.

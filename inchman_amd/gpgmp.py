@@ -87,6 +87,11 @@ class Reaction:
         self.products = dict(products or {})
         self.id = id
         self.compartments = []
+        self.customLaw = None
+
+    # Reaction::setCustomLaw (Reaction.cpp:108-111): the law as C expression text
+    def setCustomLaw(self, formula):
+        self.customLaw = str(formula) if formula else None
 
     # Reaction::addCompartment (Reaction.cpp:209-212)
     def addCompartment(self, compartment):
@@ -118,7 +123,12 @@ class DiffusionModel:
         self.max_time = float(max_time)
         self.species = []
         self.reactions = []
+        self.parameters = {}  # DiffusionModel::setParameter: name -> value (a Real)
         self.fields = None  # (Dx, Dy, rx, ry) each [S][H][W] float32
+
+    # DiffusionModel::setParameter (a name custom laws may use)
+    def setParameter(self, name, value):
+        self.parameters[str(name)] = float(value)
 
     # DiffusionModel::addSpecies
     def addSpecies(self, id, diffusion_constant=0.0):
@@ -178,9 +188,16 @@ class DiffusionModel:
         h = C.c_void_p()
         call("gpgmp_model_create", C.byref(d), C.byref(h))
         cm = CompiledModel(h, self)
+        ids = (C.c_char_p * max(len(self.species), 1))(*[sp.id.encode() for sp in self.species])
+        call("gpgmp_model_set_species_ids", h, ids)
+        for name, value in self.parameters.items():
+            call("gpgmp_model_set_parameter", h, name.encode(), C.c_float(value))
         for i, r in enumerate(self.reactions):
             for cp in r.compartments:
                 call("gpgmp_model_add_reaction_compartment", h, i, *cp.getBounds())
+        for i, r in enumerate(self.reactions):
+            if r.customLaw:
+                call("gpgmp_model_set_kinetic_law", h, i, r.customLaw.encode())
         return cm
 
     def hasLocalizedReactions(self):

@@ -120,6 +120,34 @@ int gpgmp_model_rate_literals(const gpgmp_model *m, int32_t *is_int, double *val
  * solvers created from the model afterwards. */
 int gpgmp_model_add_reaction_compartment(gpgmp_model *m, int32_t reaction, int32_t x0, int32_t y0, int32_t x1,
                                          int32_t y1);
+/* Species::id of every species (ids[num_species]), the names custom laws use. */
+int gpgmp_model_set_species_ids(gpgmp_model *m, const char *const *ids);
+/* DiffusionModel::setParameter(id, value) (a Real): a name custom laws may use; it enters the
+ * laws as the generated `const Real <id> = <value>;` does (Solver.cpp:834-841). */
+int gpgmp_model_set_parameter(gpgmp_model *m, const char *name, float value);
+/* Reaction::setCustomLaw (Reaction.cpp:108-111): the kinetic law of reaction `reaction` as the
+ * C expression text of its formula (libSBML's formula syntax: species ids, parameters,
+ * PhysicalCellWidth/Height, PhysicalModelWidth/Height, StayProbability, GridX/GridY, numbers,
+ * + - * /, parentheses, pow exp log log10 sqrt fabs abs floor ceil).  It replaces the generated
+ * law in the SSA hazards (`_h[0] + [mask *] <law>`, Solver.cpp:843-857) and in the
+ * deterministic right-hand sides (Solver.cpp:918-1010), typed as C types the pasted text: int
+ * counts, float parameters, int and double literals.  Laws of the mass-action shape (a constant
+ * times species factors X or (X - m)) are evaluated by the reaction table, the others by a
+ * compiled stack program.  Set the species ids and parameters first; the formula is checked
+ * here (GPGMP_E_INVALID with the reason). */
+int gpgmp_model_set_kinetic_law(gpgmp_model *m, int32_t reaction, const char *formula);
+/* The value the kernels give reaction `reaction`'s hazard assignment `_h[0] + [mask *] <law>`
+ * for counts x[S] in cell (gx, gy) with running total h0 and active-reaction bits act,
+ * evaluated on the host: how 0 as the solver evaluates it, 1 as a law program, 2 by the
+ * reaction table (mass-action shapes only).  No counterpart in the reference (its laws are
+ * compiled kernel text); for checking the law compiler. */
+int gpgmp_model_law_hazard(const gpgmp_model *m, int32_t reaction, const int32_t *x, int32_t gx, int32_t gy,
+                           float h0, uint32_t act, int32_t how, float *out);
+/* The same for species `species`'s deterministic expression (which 0: the RK4 derivative,
+ * 1: the alpha-QSS production, 2: its destruction) of concentrations y[S]; models with custom
+ * laws only. */
+int gpgmp_model_law_rhs(const gpgmp_model *m, int32_t species, int32_t which, const float *y, int32_t gx,
+                        int32_t gy, uint32_t act, float *out);
 /* Bit i set when reaction i is localized (Reaction::isLocalized, Reaction.h:196). */
 int gpgmp_model_localized(const gpgmp_model *m, uint32_t *mask);
 

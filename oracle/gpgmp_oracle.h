@@ -58,7 +58,20 @@ typedef struct {
      * act is read only when localized != 0. */
     uint32_t localized;
     const uint16_t *act;
+    /* custom kinetic laws (Reaction::setCustomLaw): when set, the reference's generated hazard
+     * and right-hand-side code for the model, compiled by a C++ compiler from the laws' formula
+     * text (oracle/lawgen.py); act = the cell's active-reaction bits */
+    void (*hazards_fn)(const int32_t *x, int gx, int gy, uint32_t act, float *h);
+    void (*rk_fn)(const double *x, int gx, int gy, uint32_t act, double *deriv);
+    void (*aqss_f32_fn)(const float *y, int gx, int gy, uint32_t act, float *q, float *d);
+    void (*aqss_f64_fn)(const double *y, int gx, int gy, uint32_t act, double *q, double *d);
+    int grid_w; /* cells per row (GridX / GridY of the custom laws) */
 } oracle_reaction_table;
+
+/* the laws' math functions (exp/log/log10 = fn 0/1/2 in double; pow), the same bits as the
+ * HIP library's */
+double oracle_law_fn_d(int fn, double x);
+double oracle_law_pow_d(double x, double y);
 
 /* ---- primitives ---- */
 void oracle_threefry2x32_20(uint32_t c0, uint32_t c1, uint32_t k0, uint32_t k1,

@@ -1,0 +1,216 @@
+"""Custom kinetic laws in the oracle: the reference's generated kernel code for a model,
+written as C++ and compiled with the host compiler.
+
+TEST INFRASTRUCTURE ONLY (used through oracle.table_from_model by tests/ and bench.py's
+cpu_baseline leg).  The reference pastes every reaction's law text into its kernel source
+and lets the OpenCL compiler type it:
+
+    const Real <param> = <value>;                      (Solver.cpp:834-841)
+    #define <species> _localState[...]                 (int counts / Real concentrations)
+    _h[0] = 0.;
+    _h[i+1] = _h[0] + [mask *] <law_i>; _h[0] = _h[i+1];          (Solver.cpp:843-857)
+    _deriv[s] = 0. + n. * [mask *] <law> - m. * ...;              (Solver.cpp:918-955)
+    _q[s] = 0. + n. * [mask *] <law> ...; _d[s] = 0. + ...;       (Solver.cpp:960-1010)
+
+Here the same lines are generated from the same texts and compiled by g++ (no contraction),
+so the C typing of the formulas (int, float and double literals; float parameters; int or
+floating species) is the C compiler's, independent of the HIP library's own law compiler.
+The laws' math functions call the oracle's exp/log/pow (oracle_law_fn_d, oracle_law_pow_d);
+OpenCL's built-ins are implementation-defined (parity unpinned for them).
+"""
+import ctypes as C
+import hashlib
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+PRELUDE = r"""
+#include <stdint.h>
+#include <type_traits>
+extern "C" double oracle_law_fn_d(int fn, double x);
+extern "C" double oracle_law_pow_d(double x, double y);
+namespace lawfns {
+template <class A> struct dbl { static const bool v = std::is_same<A, double>::value; };
+template <class A, class B> using pow_t = typename std::conditional<dbl<A>::v || dbl<B>::v, double, float>::type;
+template <class A> using fn_t = typename std::conditional<dbl<A>::v, double, float>::type;
+template <class A, class B> static inline pow_t<A, B> pow(A a, B b) {
+    if (dbl<A>::v || dbl<B>::v) return (pow_t<A, B>)oracle_law_pow_d((double)a, (double)b);
+    return (pow_t<A, B>)(float)oracle_law_pow_d((double)(float)a, (double)(float)b);
+}
+template <class A> static inline fn_t<A> fnx(int f, A a) {
+    if (dbl<A>::v) return (fn_t<A>)oracle_law_fn_d(f, (double)a);
+    return (fn_t<A>)(float)oracle_law_fn_d(f, (double)(float)a);
+}
+template <class A> static inline fn_t<A> exp(A a) { return fnx(0, a); }
+template <class A> static inline fn_t<A> log(A a) { return fnx(1, a); }
+template <class A> static inline fn_t<A> log10(A a) { return fnx(2, a); }
+static inline float sqrt(float a) { return __builtin_sqrtf(a); }
+static inline double sqrt(double a) { return __builtin_sqrt(a); }
+static inline float sqrt(int a) { return __builtin_sqrtf((float)a); }
+static inline float fabs(float a) { return __builtin_fabsf(a); }
+static inline double fabs(double a) { return __builtin_fabs(a); }
+static inline float fabs(int a) { return __builtin_fabsf((float)a); }
+static inline float floor(float a) { return __builtin_floorf(a); }
+static inline double floor(double a) { return __builtin_floor(a); }
+static inline float floor(int a) { return __builtin_floorf((float)a); }
+static inline float ceil(float a) { return __builtin_ceilf(a); }
+static inline double ceil(double a) { return __builtin_ceil(a); }
+static inline float ceil(int a) { return __builtin_ceilf((float)a); }
+static inline int abs(int a) { return (a < 0) ? (int)(0u - (unsigned)a) : a; }
+static inline float abs(float a) { return __builtin_fabsf(a); }
+static inline double abs(double a) { return __builtin_fabs(a); }
+}
+using namespace lawfns;
+"""
+
+
+def _flit(v):
+    """A float's exact value as a C++ hex-float literal of type float."""
+    return float(np.float32(v)).hex() + "f"
+
+
+def _counts(model, stoich):
+    """{species index: count} of a reaction's reactant or product map (keys Species or index)."""
+    out = {}
+    for sp, n in (stoich or {}).items():
+        i = sp.index if hasattr(sp, "index") else int(sp)
+        out[i] = out.get(i, 0) + int(n)
+    return out
+
+
+def law_texts(model, deterministic):
+    """Per reaction the law text the reference pastes: the custom law, or the generated one
+    (Reaction.cpp:119-162, species in index order)."""
+    out = []
+    for r in model.reactions:
+        if getattr(r, "customLaw", None):
+            out.append(r.customLaw)
+            continue
+        t = "%g" % np.float32(r.rate)
+        rc = _counts(model, r.reactants)
+        for sp in model.species:
+            n = rc.get(sp.index, 0)
+            for k in range(n):
+                t += (" * " + sp.id) if (k == 0 or deterministic) else " * (%s-%d)" % (sp.id, k)
+        out.append(t)
+    return out
+
+
+def source(model, header):
+    """The generated C++ of a model's laws.  header: the header constants as floats."""
+    S = len(model.species)
+    ids = [sp.id for sp in model.species]
+    net = [[_counts(model, r.products).get(s, 0) - _counts(model, r.reactants).get(s, 0) for s in range(S)]
+           for r in model.reactions]
+    loc = [bool(getattr(r, "compartments", None)) for r in model.reactions]
+    params = dict(getattr(model, "parameters", {}) or {})
+    src = [PRELUDE]
+
+    def consts(real):
+        lines = ["    const %s %s = %s;" % (real, k, _flit(v) if real == "float" else repr(float(np.float32(v))))
+                 for k, v in header.items()]
+        # `const Real k = <value>;` with the parameter printed with 6 significant digits
+        lines += ["    const %s %s = %s;" % (real, k, "%g" % np.float32(v)) for k, v in params.items()]
+        lines += ["    (void)%s;" % k for k in list(header) + list(params)]
+        return lines
+
+    def masks():
+        return ["    const float _mask%d = ((_act >> %d) & 1u) ? 1.f : 0.f; (void)_mask%d;" % (i, i, i)
+                for i in range(len(model.reactions))]
+
+    # hazards: int counts
+    sl = law_texts(model, False)
+    src.append('extern "C" void law_hazards(const int32_t *_x, int GridX, int GridY, uint32_t _act, float *_h) {')
+    src.append("    (void)GridX; (void)GridY;")
+    src += consts("float") + masks()
+    src += ["#define %s (_x[%d])" % (ids[s], s) for s in range(S)]
+    src.append("    _h[0] = 0.;")
+    for i, law in enumerate(sl):
+        m = "_mask%d * " % i if loc[i] else ""
+        src.append("    _h[%d] = _h[0] + %s%s; _h[0] = _h[%d];" % (i + 1, m, law, i + 1))
+    src += ["#undef %s" % ids[s] for s in range(S)]
+    src.append("}")
+    dl = law_texts(model, True)
+
+    def rhs_terms(s, sign_filter):
+        terms = []
+        for i, law in enumerate(dl):
+            n = net[i][s]
+            if n == 0 or (sign_filter == 1 and n < 0) or (sign_filter == -1 and n > 0):
+                continue
+            m = "_mask%d * " % i if loc[i] else ""
+            op = " + " if (n > 0 or sign_filter != 0) else " - "
+            terms.append("%s%d. * %s%s" % (op, abs(n), m, law))
+        return "0." + "".join(terms)
+
+    for name, real in (("law_rk_f64", "double"),):
+        src.append('extern "C" void %s(const %s *_x, int GridX, int GridY, uint32_t _act, %s *_deriv) {'
+                   % (name, real, real))
+        src.append("    (void)GridX; (void)GridY;")
+        src += consts(real) + masks()
+        src += ["#define %s (_x[%d])" % (ids[s], s) for s in range(S)]
+        src += ["    _deriv[%d] = %s;" % (s, rhs_terms(s, 0)) for s in range(S)]
+        src += ["#undef %s" % ids[s] for s in range(S)]
+        src.append("}")
+    for name, real in (("law_aqss_f32", "float"), ("law_aqss_f64", "double")):
+        src.append('extern "C" void %s(const %s *_y, int GridX, int GridY, uint32_t _act, %s *_q, %s *_d) {'
+                   % (name, real, real, real))
+        src.append("    (void)GridX; (void)GridY;")
+        src += consts(real) + masks()
+        src += ["#define %s (_y[%d])" % (ids[s], s) for s in range(S)]
+        for s in range(S):
+            src.append("    _q[%d] = %s;" % (s, rhs_terms(s, 1)))
+            src.append("    _d[%d] = %s;" % (s, rhs_terms(s, -1)))
+        src += ["#undef %s" % ids[s] for s in range(S)]
+        src.append("}")
+    return "\n".join(src) + "\n"
+
+
+_loaded = {}
+
+
+def build(src):
+    key = hashlib.sha1(src.encode()).hexdigest()[:16]
+    if key in _loaded:
+        return _loaded[key]
+    d = os.path.join(tempfile.gettempdir(), "gpgmp_lawgen")
+    os.makedirs(d, exist_ok=True)
+    cpp = os.path.join(d, key + ".cpp")
+    so = os.path.join(d, key + ".so")
+    if not os.path.exists(so):
+        with open(cpp, "w") as f:
+            f.write(src)
+        tmp = so + ".%d" % os.getpid()
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-shared", "-fPIC",
+                        cpp, "-o", tmp, "-L", HERE, "-loracle", "-Wl,-rpath," + HERE], check=True)
+        os.replace(tmp, so)
+    lib = C.CDLL(so)
+    _loaded[key] = lib
+    return lib
+
+
+def attach(table, model):
+    """Points the oracle table's custom-law slots at the compiled code of `model`."""
+    from . import oracle as o
+
+    L = o.lib()
+    W = model.width
+    header = {
+        "PhysicalCellWidth": L.oracle_header_float(C.c_float(np.float32(model.physical_length) / np.float32(W))),
+        "PhysicalCellHeight": L.oracle_header_float(C.c_float(np.float32(model.physical_length) /
+                                                              np.float32(model.height))),
+        "PhysicalModelWidth": L.oracle_header_float(C.c_float(model.physical_length)),
+        "PhysicalModelHeight": L.oracle_header_float(C.c_float(model.physical_length)),
+        "StayProbability": o.stay_probability(model.p0),
+    }
+    lib = build(source(model, header))
+    table._lawlib = lib
+    for field, fn in (("hazards_fn", "law_hazards"), ("rk_fn", "law_rk_f64"), ("aqss_f32_fn", "law_aqss_f32"),
+                      ("aqss_f64_fn", "law_aqss_f64")):
+        setattr(table, field, C.cast(getattr(lib, fn), C.c_void_p).value)
+    table.grid_w = W
+    return table

@@ -24,6 +24,8 @@ class RxTable(C.Structure):
         ("factor_species", (C.c_int * MAXF) * MAXR), ("factor_offset", (C.c_int * MAXF) * MAXR),
         ("net", (C.c_int * MAXS) * MAXR),
         ("localized", C.c_uint32), ("act", C.POINTER(C.c_uint16)),
+        ("hazards_fn", C.c_void_p), ("rk_fn", C.c_void_p), ("aqss_f32_fn", C.c_void_p), ("aqss_f64_fn", C.c_void_p),
+        ("grid_w", C.c_int),
     ]
 
 
@@ -205,6 +207,11 @@ def table_from_model(model):
         t._act = np.ascontiguousarray(act.ravel())
         t.localized = localized
         t.act = t._act.ctypes.data_as(C.POINTER(C.c_uint16))
+    t.grid_w = model.width
+    if any(getattr(r, "customLaw", None) for r in model.reactions):
+        from . import lawgen
+
+        lawgen.attach(t, model)
     return t
 
 

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #ifdef _OPENMP
@@ -81,7 +82,7 @@ extern "C" void oracle_det_rk4(double *state, int W, int H, const oracle_reactio
     const int S = t->num_species, R = t->num_reactions;
     size_t cell = 0;
     // a localized reaction's rate is multiplied by the cell's mask value (Solver.cpp:947-948)
-    auto rhs = [&](const double *x, double *d) {
+    std::function<void(const double *, double *)> rhs = [&](const double *x, double *d) {
         for (int s = 0; s < S; ++s) d[s] = 0.0;
         for (int i = 0; i < R; ++i) {
             double law = t->lit_dbl[i];
@@ -93,6 +94,12 @@ extern "C" void oracle_det_rk4(double *state, int W, int H, const oracle_reactio
     };
     const int nmax = 20;
     const double h = dt / nmax;
+    if (t->rk_fn) {
+        rhs = [&](const double *x, double *d) {
+            const int Wd = t->grid_w > 0 ? t->grid_w : W;
+            t->rk_fn(x, (int)(cell % Wd), (int)(cell / Wd), t->localized ? (uint32_t)t->act[cell] : 0xffffffffu, d);
+        };
+    }
     for (size_t c = 0; c < A; ++c) {
         cell = c;
         double x[ORACLE_MAX_SPECIES], k1[ORACLE_MAX_SPECIES], k2[ORACLE_MAX_SPECIES],
@@ -146,7 +153,23 @@ constexpr int kAqssMaxIter = 1 << 16;
 // destruction sums `0. + |n|. * [w *] <law> + ...` in double, w the cell's mask value of a
 // localized reaction (act: the cell's active bits).
 template <typename R>
-void aqss_rhs(const oracle_reaction_table *t, const R *y, R *q, R *d, uint32_t act) {
+void aqss_custom(const oracle_reaction_table *t, const R *y, R *q, R *d, uint32_t act, size_t cell);
+template <>
+void aqss_custom<float>(const oracle_reaction_table *t, const float *y, float *q, float *d, uint32_t act, size_t c) {
+    t->aqss_f32_fn(y, (int)(c % t->grid_w), (int)(c / t->grid_w), act, q, d);
+}
+template <>
+void aqss_custom<double>(const oracle_reaction_table *t, const double *y, double *q, double *d, uint32_t act,
+                         size_t c) {
+    t->aqss_f64_fn(y, (int)(c % t->grid_w), (int)(c / t->grid_w), act, q, d);
+}
+
+template <typename R>
+void aqss_rhs(const oracle_reaction_table *t, const R *y, R *q, R *d, uint32_t act, size_t cell) {
+    if ((sizeof(R) == 4 && t->aqss_f32_fn) || (sizeof(R) == 8 && t->aqss_f64_fn)) {
+        aqss_custom<R>(t, y, q, d, act, cell);
+        return;
+    }
     const int S = t->num_species, NR = t->num_reactions;
     for (int s = 0; s < S; ++s) {
         double qs = 0., ds = 0.;
@@ -168,7 +191,7 @@ void aqss_rhs(const oracle_reaction_table *t, const R *y, R *q, R *d, uint32_t a
 }
 
 template <typename R>
-int aqss_cell(const oracle_reaction_table *t, R *state, R dtg, int fixed, uint32_t act) {
+int aqss_cell(const oracle_reaction_table *t, R *state, R dtg, int fixed, uint32_t act, size_t cell) {
     const int S = t->num_species;
     const R epsmin = (R)1e-2, sqreps = (R)0.5, epscl = (R)100., epsmax = (R)10., tfd = (R)1.000008,
             dtmin = (R)1e-15, ymin = (R)1e-20;
@@ -177,7 +200,7 @@ int aqss_cell(const oracle_reaction_table *t, R *state, R dtg, int fixed, uint32
         qs[ORACLE_MAX_SPECIES], rtaus[ORACLE_MAX_SPECIES], y1[ORACLE_MAX_SPECIES], scr[ORACLE_MAX_SPECIES];
     R tn = (R)0.;
     for (int i = 0; i < S; ++i) y[i] = std::fmax(state[i], ymin);
-    aqss_rhs<R>(t, y, q, d, act);
+    aqss_rhs<R>(t, y, q, d, act, cell);
     R scrtch = (R)1e-25;
     for (int i = 0; i < S; ++i) {
         const R ascr = std::fabs(q[i]);
@@ -213,7 +236,7 @@ int aqss_cell(const oracle_reaction_table *t, R *state, R dtg, int fixed, uint32
                     tn = ts + dt;
                     for (int i = 0; i < S; ++i) y1[i] = y[i];
                 }
-                aqss_rhs<R>(t, y, q, d, act);
+                aqss_rhs<R>(t, y, q, d, act, cell);
                 eps = (R)1e-10;
                 for (int i = 0; i < S; ++i) {  // corrector
                     const R rtaub = (R)(0.5 * (rtaus[i] + dt * d[i] / y[i]));
@@ -254,7 +277,7 @@ int aqss_cell(const oracle_reaction_table *t, R *state, R dtg, int fixed, uint32
             dto = dt / dto;
             for (int i = 0; i < S; ++i) rtaus[i] *= dto;
         }
-        aqss_rhs<R>(t, y, q, d, act);
+        aqss_rhs<R>(t, y, q, d, act, cell);
     }
     for (int i = 0; i < S; ++i) state[i] = y[i];
     return iters;
@@ -269,7 +292,7 @@ int aqss_grid(R *state, int W, int H, const oracle_reaction_table *t, R dtg, int
         R x[ORACLE_MAX_SPECIES];
         for (int s = 0; s < S; ++s) x[s] = state[c + s * A];
         const uint32_t act = t->localized ? (uint32_t)t->act[c] : 0xffffffffu;
-        const int it = aqss_cell<R>(t, x, dtg, fixed, act);
+        const int it = aqss_cell<R>(t, x, dtg, fixed, act, c);
         worst = (it < 0 || worst < 0) ? -1 : std::max(worst, it);
         for (int s = 0; s < S; ++s) state[c + s * A] = x[s];
     }

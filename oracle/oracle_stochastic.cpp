@@ -93,6 +93,43 @@ extern "C" double oracle_logd(double y) {
 
 extern "C" float oracle_logf(float y) { return (float)oracle_logd((double)y); }
 
+extern "C" double oracle_expd(double x);
+
+// The kinetic laws' functions (exp, log, log10 in double; pow by binary powering of an
+// integral exponent up to 64, else exp(y log x) with the limits at x <= 0), on the oracle's
+// own exp/log.  Pinned by: libm exp/log/log10/pow within a few
+// ulp, and bit-equality with the HIP library's host evaluation of the same laws (tests).
+extern "C" double oracle_law_fn_d(int fn, double x) {
+    switch (fn) {
+        case 0: return oracle_expd(x);
+        case 1: return oracle_logd(x);
+        default: return oracle_logd(x) * 0.43429448190325176;
+    }
+}
+
+extern "C" double oracle_law_pow_d(double x, double y) {
+    if (y == 0.0) return 1.0;
+    if (x != x || y != y) return x + y;
+    if (std::floor(y) == y && std::fabs(y) <= 64.0) {  // binary powering of an integral exponent
+        int n = (int)std::fabs(y);
+        double r = 1.0, b = x;
+        while (n) {
+            if (n & 1) r = r * b;
+            b = b * b;
+            n >>= 1;
+        }
+        return (y < 0.0) ? 1.0 / r : r;
+    }
+    if (x == 0.0) return (y > 0.0) ? 0.0 : INFINITY;
+    if (x < 0.0) {
+        if (std::floor(y) != y) return NAN;
+        const double m = oracle_expd(y * oracle_logd(-x));
+        const double half = y * 0.5;
+        return (std::floor(half) == half) ? m : -m;
+    }
+    return oracle_expd(y * oracle_logd(x));
+}
+
 // Host seed schedule: boost::mt19937 rng(seed0); uniform_int<>(0, INT_MAX) and
 // seed_n = draw_n + n (DiffusionModel.cpp:1467-1484).  For a 32-bit engine and
 // range 2^31 Boost 1.54's generate_uniform_int uses bucket size 2, so the draw is
@@ -259,6 +296,11 @@ int threads_or_default(int n) {
 // Pinned by: laws evaluated by hand in the tests (int/double literals, localized reactions
 // with the mask's float chain against the unlocalized int chain, mask 0).
 extern "C" void oracle_hazards(const oracle_reaction_table *t, const int32_t *x, int cell, float *h) {
+    if (t->hazards_fn) {
+        const int W = t->grid_w > 0 ? t->grid_w : 1;
+        t->hazards_fn(x, cell % W, cell / W, t->localized ? (uint32_t)t->act[cell] : 0xffffffffu, h);
+        return;
+    }
     h[0] = 0.f;
     for (int i = 0; i < t->num_reactions; ++i) {
         h[i + 1] = add_law(t, i, h[0], x, cell);
@@ -283,10 +325,14 @@ extern "C" void oracle_ssa(int32_t *state, int W, int H, const oracle_reaction_t
         float h[ORACLE_MAX_REACTIONS + 1];
         float time = 0.f;
         while (time < deltat) {
-            h[0] = 0.f;
-            for (int i = 0; i < R; ++i) {
-                h[i + 1] = add_law(t, i, h[0], x, c);
-                h[0] = h[i + 1];
+            if (t->hazards_fn) {
+                t->hazards_fn(x, c % W, c / W, t->localized ? (uint32_t)t->act[c] : 0xffffffffu, h);
+            } else {
+                h[0] = 0.f;
+                for (int i = 0; i < R; ++i) {
+                    h[i + 1] = add_law(t, i, h[0], x, c);
+                    h[0] = h[i + 1];
+                }
             }
             if (h[0] == 0) break;
             if (ctr == (1u << 24)) break;  // same runaway guard as the HIP kernel
