@@ -115,6 +115,8 @@ SIGNATURES = {
     "gpgmp_model_set_kinetic_law": [_P, _I, C.c_char_p],
     "gpgmp_model_law_hazard": [_P, _I, _pI, _I, _I, _F, _U, _I, _pF],
     "gpgmp_model_law_rhs": [_P, _I, _I, _pF, _I, _I, _U, _pF],
+    "gpgmp_model_eval_drift_diffusivity": [_P, C.c_char_p, C.POINTER(C.c_char_p), _I, _pF, _pI, _pF, _F, _pF, _P],
+    "gpgmp_model_method_host": [_P, C.c_char_p, C.POINTER(C.c_char_p), _I, _pF, _pI, _pF, _F, _pF],
     "gpgmp_solver_create": [_P, _P, _U, C.POINTER(_P)],
     "gpgmp_solver_destroy": [_P],
     "gpgmp_solver_set_state": [_P, _pI],

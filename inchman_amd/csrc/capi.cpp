@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "det_solver.h"
+#include "fields_launch.h"
 #include "law_compile.h"
 #include "host.h"
 
@@ -186,6 +187,73 @@ int gpgmp_model_law_rhs(const gpgmp_model *m, int32_t species, int32_t which, co
         GPGMP_REQUIRE(species >= 0 && species < m->m.table.S && which >= 0 && which <= 2, GPGMP_E_INVALID,
                       "bad species or expression");
         (void)law_rhs_host(m->m, species, which, y, gx, gy, act, out);
+    });
+}
+
+int gpgmp_model_eval_drift_diffusivity(const gpgmp_model *m, const char *method, const char *const *field_names,
+                                       int32_t nfields, const float *fields, const int32_t *state,
+                                       const float *current, float sim_time, float *out, void *stream) {
+    return guarded([&] {
+        GPGMP_REQUIRE(m && method && state && out, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(nfields >= 0 && (nfields == 0 || (field_names && fields)), GPGMP_E_INVALID,
+                      "fields need their names and values");
+        const Model &md = m->m;
+        const int W = md.desc.width, H = md.desc.height, S = md.desc.num_species;
+        const size_t A = (size_t)W * H;
+        LawEnv env = law_env(md, false);
+        for (int k = 0; k < nfields; ++k) env.fields.push_back(field_names[k]);
+        const std::vector<LawProgram> progs = compile_method(method, env);
+        std::vector<uint32_t> code, table;
+        for (const LawProgram &p : progs) {
+            GPGMP_REQUIRE(code.size() + p.code.size() <= (size_t)1 << 20, GPGMP_E_CAPACITY, "method too long");
+            table.push_back((uint32_t)code.size());
+            table.push_back((uint32_t)p.type);
+            code.insert(code.end(), p.code.begin(), p.code.end());
+        }
+        hipStream_t st = (hipStream_t)stream;
+        DevBuf<uint32_t> dcode(code.size()), dtab(table.size());
+        DevBuf<int32_t> dstate((size_t)S * A);
+        DevBuf<float> dfields(std::max<size_t>((size_t)nfields * A, 1)), dcur((size_t)4 * S * A),
+            dout((size_t)4 * S * A);
+        GPGMP_HIP_CHECK(hipMemcpyAsync(dcode.get(), code.data(), code.size() * 4, hipMemcpyHostToDevice, st));
+        GPGMP_HIP_CHECK(hipMemcpyAsync(dtab.get(), table.data(), table.size() * 4, hipMemcpyHostToDevice, st));
+        GPGMP_HIP_CHECK(hipMemcpyAsync(dstate.get(), state, (size_t)S * A * 4, hipMemcpyHostToDevice, st));
+        if (nfields > 0)
+            GPGMP_HIP_CHECK(hipMemcpyAsync(dfields.get(), fields, (size_t)nfields * A * 4, hipMemcpyHostToDevice, st));
+        if (current)
+            GPGMP_HIP_CHECK(hipMemcpyAsync(dcur.get(), current, (size_t)4 * S * A * 4, hipMemcpyHostToDevice, st));
+        else
+            GPGMP_HIP_CHECK(hipMemsetAsync(dcur.get(), 0, (size_t)4 * S * A * 4, st));
+        LawExtra ex{dstate.get(), dfields.get(), dcur.get(), W, H, S, 0.f, 0.f, sim_time};
+        const float pm = header_float(md.desc.physical_length);
+        GPGMP_HIP_CHECK(launch_drift_fields(dcode.get(), dtab.get(), ex, pm, pm, dout.get(), st));
+        GPGMP_HIP_CHECK(hipMemcpyAsync(out, dout.get(), (size_t)4 * S * A * 4, hipMemcpyDeviceToHost, st));
+        GPGMP_HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+int gpgmp_model_method_host(const gpgmp_model *m, const char *method, const char *const *field_names,
+                            int32_t nfields, const float *fields, const int32_t *state, const float *current,
+                            float sim_time, float *out) {
+    return guarded([&] {
+        GPGMP_REQUIRE(m && method && state && current && out, GPGMP_E_INVALID, "null argument");
+        const Model &md = m->m;
+        const int W = md.desc.width, H = md.desc.height, S = md.desc.num_species;
+        const size_t A = (size_t)W * H;
+        LawEnv env = law_env(md, false);
+        for (int k = 0; k < nfields; ++k) env.fields.push_back(field_names[k]);
+        const std::vector<LawProgram> progs = compile_method(method, env);
+        const float pm = header_float(md.desc.physical_length);
+        const int32_t none[1] = {0};
+        for (size_t c = 0; c < A; ++c) {
+            const int gx = (int)(c % W), gy = (int)(c / W);
+            LawExtra ex{state, fields, current, W, H, S, 0.f, 0.f, sim_time};
+            ex.px = (float)((double)(pm / (float)W) * ((double)(float)gx - (double)(float)W / 2.));
+            ex.py = (float)((double)(pm / (float)H) * ((double)(float)gy - (double)(float)H / 2.));
+            for (size_t k = 0; k < progs.size(); ++k)
+                out[k * A + c] = law_to_float(law_run<int32_t, 1>(progs[k].code.data(), none, LawCell{gx, gy, 0u, 0.f}, &ex),
+                                              progs[k].type);
+        }
     });
 }
 

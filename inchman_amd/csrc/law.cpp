@@ -14,13 +14,16 @@ namespace gpgmp {
 namespace {
 
 struct Node {
-    enum Kind { NUM, SPEC, GX, GY, H0, MASK, BIN, NEG, CALL } kind;
+    // CMP: comparison (op '<' '>' 'l' (<=) 'g' (>=) '=' '!'), SEL: a ? b : c, CVT: a to `type`;
+    // STATE/CUR/CDX/CDY/FIELD/PX/PY/TIME: the inputs of drift/diffusivity methods (LawExtra)
+    enum Kind { NUM, SPEC, GX, GY, H0, MASK, BIN, NEG, CALL, CMP, SEL, CVT, STATE, CUR, CDX, CDY, FIELD, PX, PY,
+                TIME } kind;
     LawType type = LT_I32;
     double value = 0;   // NUM
-    int idx = 0;        // SPEC / MASK: species or reaction
+    int idx = 0;        // SPEC / MASK / STATE / CUR / CDX / CDY / FIELD: species, reaction, field
     char op = 0;        // BIN: + - * /
     uint32_t fn = 0;    // CALL: LawFn, or 100 = pow, 101 = abs (int)
-    int a = -1, b = -1; // children
+    int a = -1, b = -1, c = -1; // children
 };
 
 LawType promote(LawType a, LawType b) {
@@ -41,8 +44,99 @@ class Parser {
     }
     std::vector<Node> nodes;
 
+    // A drift/diffusivity method (DiffusionModel::computeDriftDiffusivityMethod, pasted into
+    // computeDiffusionConstants_forSpecies_<id>, Solver.cpp:659-777) run for species `all`:
+    // `Real x = e;` locals, `X->DiffusivityX = e;` (and Y, DriftX, DriftY) assignments,
+    // `x = e;` reassignments, comments.  Returns the final expression of each of the four
+    // fields of every species (cur_out[f * S + s]); -1 where the method does not assign it.
+    void method(int all, std::vector<int> &cur_out) {
+        all_ = all;
+        const int S = (int)env_.species.size();
+        cur_.assign((size_t)4 * S, -1);
+        for (;;) {
+            skip_ws_comments();
+            if (p_ >= s_.size()) break;
+            std::string id = ident_token();
+            if (id.empty()) fail("expected a statement");
+            if (id == "Real" || id == "float" || id == "double" || id == "int") {
+                skip_ws_comments();
+                const std::string name = ident_token();
+                if (name.empty()) fail("expected a variable name");
+                if (!eat('=')) fail("expected '=' in the declaration of " + name);
+                const LawType t = (id == "double") ? LT_F64 : (id == "int") ? LT_I32 : LT_F32;
+                locals_[name] = convert(expr(), t);
+            } else if (peek("->")) {
+                p_ += 2;
+                const int sp = species_ref(id);
+                const int f = field_index(ident_token());
+                if (!eat('=')) fail("expected '='");
+                cur_[(size_t)f * S + sp] = convert(expr(), LT_F32);
+            } else {
+                auto it = locals_.find(id);
+                if (it == locals_.end()) fail("assignment to unknown variable " + id);
+                if (!eat('=')) fail("expected '='");
+                it->second = convert(expr(), nodes[it->second].type);
+            }
+            if (!eat(';')) fail("expected ';'");
+        }
+        cur_out = cur_;
+    }
+
   private:
+    int all_ = -1;
+    std::map<std::string, int> locals_;
+    std::vector<int> cur_;
+
+    void skip_ws_comments() {
+        for (;;) {
+            skip();
+            if (p_ + 1 < s_.size() && s_[p_] == '/' && s_[p_ + 1] == '/') {
+                while (p_ < s_.size() && s_[p_] != '\n') ++p_;
+            } else if (p_ + 1 < s_.size() && s_[p_] == '/' && s_[p_ + 1] == '*') {
+                const size_t e = s_.find("*/", p_ + 2);
+                p_ = (e == std::string::npos) ? s_.size() : e + 2;
+            } else {
+                return;
+            }
+        }
+    }
+    bool peek(const char *t) {
+        skip();
+        return s_.compare(p_, std::strlen(t), t) == 0;
+    }
+    std::string ident_token() {
+        skip();
+        const size_t st = p_;
+        if (p_ < s_.size() && (std::isalpha((unsigned char)s_[p_]) || s_[p_] == '_'))
+            while (p_ < s_.size() && (std::isalnum((unsigned char)s_[p_]) || s_[p_] == '_')) ++p_;
+        return s_.substr(st, p_ - st);
+    }
+    int species_ref(const std::string &id) {
+        if (id == "All" && all_ >= 0) return all_;
+        for (size_t i = 0; i < env_.species.size(); ++i)
+            if (env_.species[i] == id) return (int)i;
+        fail("unknown species " + id);
+    }
+    int field_index(const std::string &f) {
+        static const char *names[] = {"DiffusivityX", "DiffusivityY", "DriftX", "DriftY"};
+        for (int k = 0; k < 4; ++k)
+            if (f == names[k]) return k;
+        if (f == "State") return 4;
+        if (f == "MeanX") fail("MeanX (the moments of computeMoments models) is not supported");
+        fail("unknown species field " + f);
+    }
+    int convert(int n, LawType t) {
+        if (nodes[n].type == t) return n;
+        Node c;
+        c.kind = Node::CVT;
+        c.type = t;
+        c.a = n;
+        return add(c);
+    }
     [[noreturn]] void fail(const std::string &m) {
+        if (all_ >= 0)
+            throw Error(GPGMP_E_INVALID, "drift/diffusivity method, near \"" + s_.substr(p_ > 20 ? p_ - 20 : 0, 40) +
+                                             "\": " + m);
         throw Error(GPGMP_E_INVALID, "kinetic law \"" + s_ + "\": " + m);
     }
     void skip() {
@@ -70,7 +164,66 @@ class Parser {
         n.type = promote(nodes[a].type, nodes[b].type);
         return add(n);
     }
-    int expr() {
+    int expr() {  // conditional-expression
+        const int c = equality();
+        if (eat('?')) {
+            const int a = expr();
+            if (!eat(':')) fail("expected ':'");
+            const int b = expr();
+            Node n;
+            n.kind = Node::SEL;
+            n.a = c;
+            n.b = a;
+            n.c = b;
+            n.type = promote(nodes[a].type, nodes[b].type);
+            return add(n);
+        }
+        return c;
+    }
+    int cmp(char op, int a, int b) {
+        Node n;
+        n.kind = Node::CMP;
+        n.op = op;
+        n.a = a;
+        n.b = b;
+        n.type = LT_I32;
+        return add(n);
+    }
+    int equality() {
+        int n = relational();
+        for (;;) {
+            if (peek("==")) {
+                p_ += 2;
+                n = cmp('=', n, relational());
+            } else if (peek("!=")) {
+                p_ += 2;
+                n = cmp('!', n, relational());
+            } else {
+                return n;
+            }
+        }
+    }
+    int relational() {
+        int n = additive();
+        for (;;) {
+            if (peek("<=")) {
+                p_ += 2;
+                n = cmp('l', n, additive());
+            } else if (peek(">=")) {
+                p_ += 2;
+                n = cmp('g', n, additive());
+            } else if (peek("<")) {
+                ++p_;
+                n = cmp('<', n, additive());
+            } else if (peek(">")) {
+                ++p_;
+                n = cmp('>', n, additive());
+            } else {
+                return n;
+            }
+        }
+    }
+    int additive() {
         int n = term();
         for (;;) {
             if (eat('+')) n = bin('+', n, term());
@@ -145,6 +298,14 @@ class Parser {
             }
         }
         Node n;
+        if (name == "getCentralDifferenceX" || name == "getCentralDifferenceY") {
+            // args were parsed as expressions: the one argument must be a species reference
+            if (args.size() != 1 || nodes[args[0]].kind != Node::STATE) fail(name + " takes a species");
+            n.kind = (name == "getCentralDifferenceX") ? Node::CDX : Node::CDY;
+            n.idx = nodes[args[0]].idx;
+            n.type = LT_F32;
+            return add(n);
+        }
         n.kind = Node::CALL;
         static const char *unary_fns[] = {"exp", "log", "log10", "sqrt", "fabs", "floor", "ceil"};
         if (name == "pow") {
@@ -194,6 +355,50 @@ class Parser {
     }
     int ident(const std::string &id) {
         Node n;
+        if (all_ >= 0) {  // method scope
+            auto l = locals_.find(id);
+            if (l != locals_.end()) return l->second;
+            if (peek("->")) {
+                p_ += 2;
+                const int sp = species_ref(id);
+                const int f = field_index(ident_token());
+                const int S = (int)env_.species.size();
+                if (f == 4) {
+                    n.kind = Node::STATE;
+                    n.idx = sp;
+                    n.type = LT_I32;
+                    return add(n);
+                }
+                if (cur_[(size_t)f * S + sp] >= 0) return cur_[(size_t)f * S + sp];
+                n.kind = Node::CUR;
+                n.idx = sp * 4 + f;
+                n.type = LT_F32;
+                return add(n);
+            }
+            if (id == "All" || [&] {
+                    for (const auto &x : env_.species)
+                        if (x == id) return true;
+                    return false;
+                }()) {
+                // a bare species: the argument of getCentralDifferenceX/Y (a _species_t *)
+                n.kind = Node::STATE;
+                n.idx = species_ref(id);
+                n.type = LT_I32;
+                return add(n);
+            }
+            if (id == "PhysicalX" || id == "PhysicalY" || id == "PhysicalSimTime") {
+                n.kind = (id == "PhysicalX") ? Node::PX : (id == "PhysicalY") ? Node::PY : Node::TIME;
+                n.type = LT_F32;
+                return add(n);
+            }
+            for (size_t k = 0; k < env_.fields.size(); ++k)
+                if (env_.fields[k] == id) {
+                    n.kind = Node::FIELD;
+                    n.idx = (int)k;
+                    n.type = LT_F32;
+                    return add(n);
+                }
+        }
         for (size_t i = 0; i < env_.species.size(); ++i) {
             if (env_.species[i] == id) {
                 n.kind = Node::SPEC;
@@ -240,6 +445,9 @@ bool is_const(const std::vector<Node> &N, int i) {
         case Node::BIN: return is_const(N, n.a) && is_const(N, n.b);
         case Node::NEG: return is_const(N, n.a);
         case Node::CALL: return is_const(N, n.a) && (n.b < 0 || is_const(N, n.b));
+        case Node::CMP: return is_const(N, n.a) && is_const(N, n.b);
+        case Node::SEL: return is_const(N, n.a) && is_const(N, n.b) && is_const(N, n.c);
+        case Node::CVT: return is_const(N, n.a);
         default: return false;
     }
 }
@@ -267,6 +475,41 @@ class Emitter {
             case Node::GY: word(LO_GRIDY, 0); grow(); break;
             case Node::H0: word(LO_H0, 0); grow(); break;
             case Node::MASK: word(LO_MASK, (uint32_t)n.idx); grow(); break;
+            case Node::STATE: word(LO_STATE, (uint32_t)n.idx); grow(); break;
+            case Node::FIELD: word(LO_FIELD, (uint32_t)n.idx); grow(); break;
+            case Node::CUR: word(LO_CUR, (uint32_t)n.idx); grow(); break;
+            case Node::CDX: word(LO_CDX, (uint32_t)n.idx); grow(); break;
+            case Node::CDY: word(LO_CDY, (uint32_t)n.idx); grow(); break;
+            case Node::PX: word(LO_PHYSX, 0); grow(); break;
+            case Node::PY: word(LO_PHYSY, 0); grow(); break;
+            case Node::TIME: word(LO_SIMTIME, 0); grow(); break;
+            case Node::CVT: {
+                const LawType from = N_[n.a].type;
+                emit(n.a);
+                // int -> double and float -> double are exact; double/int -> float round
+                if (n.type == LT_F32 && from == LT_F64) word(LO_D2F, 0);
+                else if (n.type == LT_F32 && from == LT_I32) word(LO_I2F, 0);
+                else if (n.type == LT_I32) throw Error(GPGMP_E_INVALID, "conversion to int is not supported");
+                break;
+            }
+            case Node::CMP: {
+                const LawType t = promote(N_[n.a].type, N_[n.b].type);
+                operand(n.a, t);
+                operand(n.b, t);
+                const uint32_t base = (t == LT_I32) ? LO_LT_I : (t == LT_F32) ? LO_LT_F : LO_LT_D;
+                const uint32_t k = (n.op == '<') ? 0 : (n.op == '>') ? 1 : (n.op == 'l') ? 2 : (n.op == 'g') ? 3
+                                 : (n.op == '=') ? 4 : 5;
+                word(base + k, 0);
+                --depth;
+                break;
+            }
+            case Node::SEL:
+                emit(n.a);
+                operand(n.b, n.type);
+                operand(n.c, n.type);
+                word(LO_SEL, 0);
+                depth -= 2;
+                break;
             case Node::NEG:
                 emit(n.a);
                 word(n.type == LT_I32 ? LO_NEG_I : n.type == LT_F32 ? LO_NEG_F : LO_NEG_D, 0);
@@ -330,6 +573,7 @@ void fold(std::vector<Node> &N, int i) {
     }
     if (n.a >= 0) fold(N, n.a);
     if (n.b >= 0) fold(N, n.b);
+    if (n.c >= 0) fold(N, n.c);
 }
 
 // Factor of the mass-action shape: species X, or X - m with an int literal 0 <= m <= 255.
@@ -365,6 +609,38 @@ LawProgram compile_law(const std::string &text, const LawEnv &env) {
     LawProgram out;
     out.code = std::move(e.code);
     out.type = N[root].type;
+    return out;
+}
+
+std::vector<LawProgram> compile_method(const std::string &text, const LawEnv &env) {
+    const int S = (int)env.species.size();
+    std::vector<LawProgram> out((size_t)4 * S);
+    for (int s = 0; s < S; ++s) {
+        Parser p(text, env);
+        std::vector<int> cur;
+        p.method(s, cur);
+        std::vector<Node> &N = p.nodes;
+        for (int f = 0; f < 4; ++f) {
+            int root = cur[(size_t)f * S + s];
+            if (root < 0) {  // not assigned: the field keeps its current value
+                Node n;
+                n.kind = Node::CUR;
+                n.idx = s * 4 + f;
+                n.type = LT_F32;
+                N.push_back(n);
+                root = (int)N.size() - 1;
+            }
+            fold(N, root);
+            Emitter e(N);
+            e.emit(root);
+            e.code.push_back(LO_END);
+            if (e.max_depth > kLawStack)
+                throw Error(GPGMP_E_INVALID, "drift/diffusivity method needs a deeper stack than " +
+                                                 std::to_string(kLawStack));
+            out[(size_t)f * S + s].code = std::move(e.code);
+            out[(size_t)f * S + s].type = N[root].type;
+        }
+    }
     return out;
 }
 

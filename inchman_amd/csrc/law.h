@@ -40,6 +40,20 @@ enum LawOp : uint32_t {
     LO_POW_F,    // pow in float of the two top values
     LO_POW_D,
     LO_ABS_I,
+    // drift/diffusivity methods (LawExtra inputs)
+    LO_D2F,      // top: (float)(double)
+    LO_LT_I, LO_GT_I, LO_LE_I, LO_GE_I, LO_EQ_I, LO_NE_I,   // comparisons (int result)
+    LO_LT_F, LO_GT_F, LO_LE_F, LO_GE_F, LO_EQ_F, LO_NE_F,
+    LO_LT_D, LO_GT_D, LO_LE_D, LO_GE_D, LO_EQ_D, LO_NE_D,
+    LO_SEL,      // c ? a : b of the three top values (c deepest)
+    LO_STATE,    // push species arg's count in this cell (int)
+    LO_FIELD,    // push field parameter arg's value in this cell (float)
+    LO_PHYSX,    // push PhysicalX / PhysicalY of this cell (float)
+    LO_PHYSY,
+    LO_SIMTIME,  // push PhysicalSimTime (float)
+    LO_CDX,      // push getCentralDifferenceX / Y of species arg (float)
+    LO_CDY,
+    LO_CUR,      // push the current value of field (arg & 3) of species (arg >> 2) (float)
     LO_COUNT
 };
 
@@ -106,11 +120,22 @@ struct LawCell {
     float h0;
 };
 
+// Inputs of a drift/diffusivity method (the generated computeDiffusionConstants kernel's
+// scope, Solver.cpp:659-777): counts [S][A], field parameters [nf][A], the current
+// diffusivity/drift fields [4][S][A] (X, Y diffusivity, X, Y drift), the grid and the clock.
+struct LawExtra {
+    const int32_t *state;
+    const float *fields;
+    const float *cur;
+    int32_t W, H, S;
+    float px, py, sim_time;
+};
+
 // Runs one program.  The opcodes are the same for every lane (the program is per model), so
 // the dispatch is a uniform branch; the stack index is uniform too and selects registers
 // (no private-memory array).  Returns the result as a double (exact for its type).
 template <typename X, int S>
-GPGMP_HD double law_run(const uint32_t *code, const X (&x)[S], const LawCell &cell) {
+GPGMP_HD double law_run(const uint32_t *code, const X (&x)[S], const LawCell &cell, const LawExtra *ex = nullptr) {
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, top = 0;
     int sp = 0;  // values on the stack; top is the top one when sp > 0
     auto below = [&](int i) -> double {  // stack slot i (0-based, below top)
@@ -166,6 +191,40 @@ GPGMP_HD double law_run(const uint32_t *code, const X (&x)[S], const LawCell &ce
                 top = (double)(int32_t)((v < 0) ? 0u - (uint32_t)v : (uint32_t)v);
                 break;
             }
+            case LO_D2F: top = (double)(float)top; break;
+            case LO_STATE: push((double)ex->state[(size_t)arg * ex->W * ex->H + (size_t)cell.gy * ex->W + cell.gx]); break;
+            case LO_FIELD: push((double)ex->fields[(size_t)arg * ex->W * ex->H + (size_t)cell.gy * ex->W + cell.gx]); break;
+            case LO_PHYSX: push((double)ex->px); break;
+            case LO_PHYSY: push((double)ex->py); break;
+            case LO_SIMTIME: push((double)ex->sim_time); break;
+            case LO_CUR: {
+                const size_t A = (size_t)ex->W * ex->H;
+                push((double)ex->cur[((size_t)(arg & 3u) * ex->S + (arg >> 2)) * A + (size_t)cell.gy * ex->W + cell.gx]);
+                break;
+            }
+            case LO_CDX:
+            case LO_CDY: {
+                // getCentralDifferenceX/Y (cl_include/helpers.cl:39-87): the mirrored neighbour
+                // at an edge; the Y form bounds dy by the grid width, as the reference does
+                const int32_t *st = ex->state + (size_t)arg * ex->W * ex->H;
+                const int dx = cell.gx, dy = cell.gy, W = ex->W;
+                int32_t lo, hi;
+                if (op == LO_CDX) {
+                    lo = (dx > 0) ? st[(dx - 1) + dy * W] : st[(dx + 1) + dy * W];
+                    hi = (dx < W - 1) ? st[(dx + 1) + dy * W] : st[(dx - 1) + dy * W];
+                } else {
+                    lo = (dy > 0) ? st[dx + (dy - 1) * W] : st[dx + (dy + 1) * W];
+                    hi = (dy < W - 1) ? st[dx + (dy + 1) * W] : st[dx + (dy - 1) * W];
+                }
+                push((double)(float)(int32_t)((uint32_t)hi - (uint32_t)lo));
+                break;
+            }
+            case LO_SEL: {  // c (sp-3) ? a (sp-2) : b (top)
+                const double c = below(sp - 3), a = below(sp - 2);
+                sp -= 2;
+                top = (c != 0.0) ? a : top;
+                break;
+            }
             default: {  // binary: a (below) op b (top)
                 const double a = below(sp - 2), b = top;
                 double r;
@@ -189,7 +248,14 @@ GPGMP_HD double law_run(const uint32_t *code, const X (&x)[S], const LawCell &ce
                     case LO_MUL_D: r = a * b; break;
                     case LO_DIV_D: r = a / b; break;
                     case LO_POW_F: r = (double)law_pow_f((float)a, (float)b); break;
-                    default: r = law_pow_d(a, b); break;
+                    case LO_POW_D: r = law_pow_d(a, b); break;
+                    default: {  // comparisons: int, float and double operands are exact doubles
+                        const uint32_t k = (op - LO_LT_I) % 6u;
+                        const bool v = (k == 0) ? a < b : (k == 1) ? a > b : (k == 2) ? a <= b : (k == 3) ? a >= b
+                                     : (k == 4) ? a == b : a != b;
+                        r = v ? 1.0 : 0.0;
+                        break;
+                    }
                 }
                 --sp;
                 top = r;

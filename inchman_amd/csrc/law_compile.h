@@ -17,6 +17,7 @@ namespace gpgmp {
 struct LawEnv {
     std::vector<std::string> species;
     std::map<std::string, float> params;  // parameters and header constants, as float values
+    std::vector<std::string> fields;      // field parameters (drift/diffusivity methods)
     bool deterministic = false;
 };
 
@@ -38,6 +39,12 @@ struct MassAction {
     bool commuted = false;  // written X * c * ...: the constant second
 };
 bool mass_action_shape(const std::string &text, const LawEnv &env, MassAction &ma);
+
+// A drift/diffusivity method compiled for every species: programs[f * S + s] gives field f
+// (0 DiffusivityX, 1 DiffusivityY, 2 DriftX, 3 DriftY) of species s -- the value the method
+// leaves in species s's struct when run for s (`All` = s), or its current value when the
+// method does not assign it (Solver.cpp:659-777 saves the function species' fields only).
+std::vector<LawProgram> compile_method(const std::string &text, const LawEnv &env);
 
 // The names a model's laws may use, and reaction i's law text (custom or generated).
 LawEnv law_env(const Model &m, bool deterministic);

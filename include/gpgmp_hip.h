@@ -148,6 +148,26 @@ int gpgmp_model_law_hazard(const gpgmp_model *m, int32_t reaction, const int32_t
  * laws only. */
 int gpgmp_model_law_rhs(const gpgmp_model *m, int32_t species, int32_t which, const float *y, int32_t gx,
                         int32_t gy, uint32_t act, float *out);
+/* The model's drift/diffusivity method (DiffusionModel::computeDriftDiffusivityMethod; the C
+ * statements the reference pastes into computeDiffusionConstants_forSpecies_<id>,
+ * Solver.cpp:659-777, and InhomogeneousSolver::computeDiffusivityDrift runs,
+ * InhomogeneousSolver.cpp:437-470) evaluated on the device for every cell and species:
+ * `Real x = e;` locals, `X->DiffusivityX/DiffusivityY/DriftX/DriftY = e;` assignments (All =
+ * the species being computed), X->State, getCentralDifferenceX/Y(X), PhysicalX/PhysicalY,
+ * PhysicalSimTime, the header constants, the parameters, the field parameters
+ * (field_names[nfields], values fields[nfields][H][W]), comparisons and ?:.  state: counts
+ * [S][H][W]; current: the fields before the call [4][S][H][W] (DiffusivityX, DiffusivityY,
+ * DriftX, DriftY; NULL: zeros); out: the same layout after it -- what
+ * gpgmp_model_desc.diffusivity_x/_y and drift_x/_y take.  MeanX (computeMoments) is not
+ * supported. */
+int gpgmp_model_eval_drift_diffusivity(const gpgmp_model *m, const char *method, const char *const *field_names,
+                                       int32_t nfields, const float *fields, const int32_t *state,
+                                       const float *current, float sim_time, float *out, void *stream);
+/* The same evaluated on the host, for checking the method compiler (current is required).
+ * No counterpart in the reference. */
+int gpgmp_model_method_host(const gpgmp_model *m, const char *method, const char *const *field_names,
+                            int32_t nfields, const float *fields, const int32_t *state, const float *current,
+                            float sim_time, float *out);
 /* Bit i set when reaction i is localized (Reaction::isLocalized, Reaction.h:196). */
 int gpgmp_model_localized(const gpgmp_model *m, uint32_t *mask);
 

@@ -214,3 +214,99 @@ def attach(table, model):
         setattr(table, field, C.cast(getattr(lib, fn), C.c_void_p).value)
     table.grid_w = W
     return table
+
+
+# ---- drift/diffusivity methods -----------------------------------------------------------
+METHOD_PRELUDE = r"""
+struct _species_t {
+    const int State;
+    const int *SpeciesState;
+    const float MeanX;
+    float DiffusivityX, DiffusivityY, DriftX, DriftY;
+};
+static int _gx, _gy, _W;
+// getCentralDifferenceX/Y (cl_include/helpers.cl:39-87): the mirrored neighbour at an edge,
+// the Y form bounded by the grid width as written there
+static inline float getCentralDifferenceX(_species_t *species) {
+    const int *d = species->SpeciesState;
+    const int sl = (_gx > 0) ? d[(_gx - 1) + _gy * _W] : d[(_gx + 1) + _gy * _W];
+    const int sr = (_gx < _W - 1) ? d[(_gx + 1) + _gy * _W] : d[(_gx - 1) + _gy * _W];
+    return sr - sl;
+}
+static inline float getCentralDifferenceY(_species_t *species) {
+    const int *d = species->SpeciesState;
+    const int st = (_gy > 0) ? d[_gx + (_gy - 1) * _W] : d[_gx + (_gy + 1) * _W];
+    const int sb = (_gy < _W - 1) ? d[_gx + (_gy + 1) * _W] : d[_gx + (_gy - 1) * _W];
+    return sb - st;
+}
+"""
+
+
+def method_source(model, method, field_names, header):
+    """The reference's computeDiffusionConstants_forSpecies_<id> (Solver.cpp:659-777) for every
+    species as one C++ function over all cells."""
+    S = len(model.species)
+    ids = [sp.id for sp in model.species]
+    params = dict(getattr(model, "parameters", {}) or {})
+    src = [PRELUDE, METHOD_PRELUDE, "typedef float Real;"]
+    src.append('extern "C" void method_fields(const int *_state, const float *_field, const float *_cur, int W, '
+               'int H, float PhysicalSimTime, float *_out) {')
+    src.append("    const int __area = W * H; _W = W; (void)PhysicalSimTime;")
+    for k, v in header.items():
+        src.append("    const float %s = %s; (void)%s;" % (k, _flit(v), k))
+    for k, v in params.items():
+        src.append("    const Real %s = %s; (void)%s;" % (k, "%g" % np.float32(v), k))
+    src.append("    for (int __sp = 0; __sp < %d; ++__sp)" % S)
+    src.append("    for (int __cell = 0; __cell < __area; ++__cell) {")
+    src.append("        const int GridX = __cell % W, GridY = __cell / W; _gx = GridX; _gy = GridY; (void)GridY;")
+    # getPhysicalX/Y (cl_include/helpers.cl:29-36)
+    src.append("        const Real PhysicalX = (Real)((double)(PhysicalModelWidth / (Real)W) * "
+               "((double)(Real)GridX - (double)(Real)W / 2.)); (void)PhysicalX;")
+    src.append("        const Real PhysicalY = (Real)((double)(PhysicalModelHeight / (Real)H) * "
+               "((double)(Real)GridY - (double)(Real)H / 2.)); (void)PhysicalY;")
+    for s, sid in enumerate(ids):
+        src.append("        _species_t _%s = {_state[%d * __area + __cell], _state + %d * __area, 0.f, _cur[(0 * %d + %d) * __area + __cell], "
+                   "_cur[(1 * %d + %d) * __area + __cell], _cur[(2 * %d + %d) * __area + __cell], _cur[(3 * %d + %d) * __area + __cell]};"
+                   % (sid, s, s, S, s, S, s, S, s, S, s))
+        src.append("        _species_t *%s = &_%s; (void)%s;" % (sid, sid, sid))
+    src.append("        _species_t *__all[%d] = {%s};" % (S, ", ".join("&_" + sid for sid in ids)))
+    src.append("        _species_t *All = __all[__sp]; (void)All;")
+    for k, name in enumerate(field_names):
+        src.append("#define %s (_field[%d * __area + __cell])" % (name, k))
+    src.append("        {")
+    src.append(method)
+    src.append("        }")
+    for name in field_names:
+        src.append("#undef %s" % name)
+    src.append("        _out[(0 * %d + __sp) * __area + __cell] = All->DiffusivityX;" % S)
+    src.append("        _out[(1 * %d + __sp) * __area + __cell] = All->DiffusivityY;" % S)
+    src.append("        _out[(2 * %d + __sp) * __area + __cell] = All->DriftX;" % S)
+    src.append("        _out[(3 * %d + __sp) * __area + __cell] = All->DriftY;" % S)
+    src.append("    }")
+    src.append("}")
+    return "\n".join(src) + "\n"
+
+
+def method_fields(model, method, field_names=(), fields=None, state=None, current=None, sim_time=0.0):
+    """The drift/diffusivity fields [4][S][H][W] the method gives (g++-compiled)."""
+    from . import oracle as o
+
+    L = o.lib()
+    W, H, S = model.width, model.height, len(model.species)
+    header = {
+        "PhysicalCellWidth": L.oracle_header_float(C.c_float(np.float32(model.physical_length) / np.float32(W))),
+        "PhysicalCellHeight": L.oracle_header_float(C.c_float(np.float32(model.physical_length) /
+                                                              np.float32(H))),
+        "PhysicalModelWidth": L.oracle_header_float(C.c_float(model.physical_length)),
+        "PhysicalModelHeight": L.oracle_header_float(C.c_float(model.physical_length)),
+        "StayProbability": o.stay_probability(model.p0),
+    }
+    lib = build(method_source(model, method, list(field_names), header))
+    st = np.ascontiguousarray(state if state is not None else np.zeros((S, H, W)), np.int32)
+    fl = np.ascontiguousarray(fields if fields is not None and len(field_names) else np.zeros(1), np.float32)
+    cur = np.ascontiguousarray(current if current is not None else np.zeros((4, S, H, W)), np.float32)
+    out = np.zeros((4, S, H, W), np.float32)
+    fn = lib.method_fields
+    fn.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_int, C.c_float, C.c_void_p]
+    fn(st.ctypes.data, fl.ctypes.data, cur.ctypes.data, W, H, float(sim_time), out.ctypes.data)
+    return out
