@@ -207,41 +207,65 @@ GPGMP_HD float ind_cell_dt(float Df, float rf, float hf) {
     return (float)(h / r * th * temp);
 }
 
-// sin(pi t) in double: t reduced to [0,2), folded to [0,1/2], Taylor series to x^21
-GPGMP_HD double sinpi_d(double t) {
-    double t2 = t - 2.0 * __builtin_floor(t * 0.5);
-    double sign = 1.0;
-    if (t2 >= 1.0) {
-        sign = -1.0;
-        t2 = t2 - 1.0;
+// log(u) in float for a normal u > 0: u = 2^e m with m in (sqrt(1/2), sqrt(2)], f = m - 1
+// (exact), s = f / (2 + f), log(m) = 2 atanh(s) = 2 (s + s^3/3 + ... + s^9/9) (|s| < 0.172:
+// truncation below 4e-10), Horner in s^2 with fused multiply-adds.  IEEE operations only, so
+// the CPU and the GPU give the same bits.
+GPGMP_HD float log_pf(float u) {
+    uint32_t b;
+    __builtin_memcpy(&b, &u, 4);
+    int e = (int)((b >> 23) & 0xffu) - 127;
+    b = (b & 0x007fffffu) | 0x3f800000u;
+    float m;
+    __builtin_memcpy(&m, &b, 4);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e = e + 1;
     }
-    const double r = (t2 > 0.5) ? (1.0 - t2) : t2;
-    const double x = 3.141592653589793116 * r;
-    const double z = x * x;
-    double p = 1.0 / 51090942171709440000.0;
-    p = -1.0 / 121645100408832000.0 + z * p;
-    p = 1.0 / 355687428096000.0 + z * p;
-    p = -1.0 / 1307674368000.0 + z * p;
-    p = 1.0 / 6227020800.0 + z * p;
-    p = -1.0 / 39916800.0 + z * p;
-    p = 1.0 / 362880.0 + z * p;
-    p = -1.0 / 5040.0 + z * p;
-    p = 1.0 / 120.0 + z * p;
-    p = -1.0 / 6.0 + z * p;
-    return sign * (x + x * (z * p));
+    const float f = m - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    float p = 1.0f / 9.0f;
+    p = __builtin_fmaf(z, p, 1.0f / 7.0f);
+    p = __builtin_fmaf(z, p, 1.0f / 5.0f);
+    p = __builtin_fmaf(z, p, 1.0f / 3.0f);
+    const float two_s = 2.0f * s;
+    return __builtin_fmaf((float)e, 0.693147182f, __builtin_fmaf(two_s, z * p, two_s));
+}
+
+// sin(pi t) in float: t reduced to [0,2), folded to r in [0,1/2], x = pi r, Taylor series to
+// x^13 (truncation below 7e-10) in Horner form with fused multiply-adds.
+GPGMP_HD float sinpi_pf(float t) {
+    float t2 = t - 2.0f * __builtin_floorf(t * 0.5f);
+    float sign = 1.0f;
+    if (t2 >= 1.0f) {
+        sign = -1.0f;
+        t2 = t2 - 1.0f;
+    }
+    const float r = (t2 > 0.5f) ? (1.0f - t2) : t2;
+    const float x = 3.14159274f * r;
+    const float z = x * x;
+    float p = 1.0f / 6227020800.0f;
+    p = __builtin_fmaf(z, p, -1.0f / 39916800.0f);
+    p = __builtin_fmaf(z, p, 1.0f / 362880.0f);
+    p = __builtin_fmaf(z, p, -1.0f / 5040.0f);
+    p = __builtin_fmaf(z, p, 1.0f / 120.0f);
+    p = __builtin_fmaf(z, p, -1.0f / 6.0f);
+    return sign * __builtin_fmaf(x, z * p, x);
 }
 
 // Gaussian pair by the Box-Muller transform of one Threefry call (key {id, seed}, counter
-// {step, 0}): u1 = (w.x + 1/2) 2^-32 and u2 = (w.y + 1/2) 2^-32 in (0, 1),
-// g_x = sqrt(-2 ln u1) cos(2 pi u2), g_y = sqrt(-2 ln u1) sin(2 pi u2), with cos(2 pi u2) =
-// sin(pi (2 u2 + 1/2)); no rejection loop, so the lanes of a wave never wait for each other.
-GPGMP_HD void gauss2(uint32_t id, uint32_t seed, uint32_t step, double *gx, double *gy) {
+// {step, 0}), in float: u1 = (w.x / 2^9 + 1/2) 2^-23 and u2 likewise from w.y, in (0, 1)
+// and exact, g_x = sqrt(-2 ln u1) cos(2 pi u2), g_y = sqrt(-2 ln u1) sin(2 pi u2), with
+// cos(2 pi u2) = sin(pi (2 u2 + 1/2)); no rejection loop, so the lanes of a wave never wait
+// for each other.  The largest |g| is sqrt(48 ln 2) = 5.77.
+GPGMP_HD void gauss2_f(uint32_t id, uint32_t seed, uint32_t step, float *gx, float *gy) {
     const u32x2 w = threefry2x32_20(step, 0u, id, seed);
-    const double u1 = ((double)w.x + 0.5) * 2.3283064365386962890625e-10;
-    const double u2 = ((double)w.y + 0.5) * 2.3283064365386962890625e-10;
-    const double r = __builtin_sqrt(-2.0 * log_d(u1));
-    *gx = r * sinpi_d(2.0 * u2 + 0.5);
-    *gy = r * sinpi_d(2.0 * u2);
+    const float u1 = ((float)(w.x >> 9) + 0.5f) * 0x1p-23f;
+    const float u2 = ((float)(w.y >> 9) + 0.5f) * 0x1p-23f;
+    const float r = __builtin_sqrtf(-2.0f * log_pf(u1));
+    *gx = r * sinpi_pf(2.0f * u2 + 0.5f);
+    *gy = r * sinpi_pf(2.0f * u2);
 }
 
 }  // namespace gpgmp

@@ -27,6 +27,9 @@ static int guarded_p(F &&f) {
     }
 }
 
+// The sorted-list step packs a far particle's old cell into 28 bits of its code word.
+constexpr int kSortedMaxSide = 16384;
+
 static int bits_for(uint64_t ncells) {
     int b = 0;
     while (((uint64_t)1 << b) < ncells) ++b;
@@ -39,11 +42,14 @@ struct gpgmp_particles {
     hipStream_t stream = nullptr;
     // current list (x, y, id, key_out with offsets off) and the buffers of the general step
     DevBuf<float> x, y, x2, y2;
-    DevBuf<uint32_t> id, id2, key, val, key2, val2, key_out, keep, off, tile_sum, hist, digit_total;
-    // step on the sorted list: the next list (xa, ya, ida, keya, off2) and per-cell work
+    DevBuf<uint32_t> id, id2, key, key2, key_out, off, hist, digit_total;
+    // per-cell work of both steps
+    DevBuf<uint32_t> tot, kmask, surv, tile_cells, off_sorted;
+    // step on the sorted list: the next list (xa, ya, ida, off2) and its per-cell work
     DevBuf<float> xa, ya;
-    DevBuf<uint32_t> ida, keya, off2, cnt, pre, tot, kmask, surv, tile_cells, far_cnt, far_in;
-    DevBuf<uint8_t> dir, far_list;
+    DevBuf<uint32_t> ida, off2, cnt, pre, far_cnt, far_in;
+    DevBuf<uint32_t> code;
+    DevBuf<uint8_t> far_list;
     DevBuf<int> dn;  // live count in slot cur, the next count in the other
     int cur = 0;
     int far_slot = 0;
@@ -54,6 +60,7 @@ struct gpgmp_particles {
     int n_ub = 0;
     bool stepped = false;  // keys/offsets exist once a step has sorted the list
     bool sorted_ok = false;  // moves stay within the 3x3 neighbourhood all but never
+    bool keys_stale = false;  // the sorted-list step stores no keys: key_out is rebuilt on demand
     ~gpgmp_particles() {
         (void)hipStreamSynchronize(stream);
         if (n_ev) (void)hipEventDestroy(n_ev);
@@ -62,25 +69,27 @@ struct gpgmp_particles {
     ParticleBuffers buffers() {
         ParticleBuffers b;
         b.x = x.get(); b.y = y.get(); b.x2 = x2.get(); b.y2 = y2.get();
-        b.id = id.get(); b.id2 = id2.get(); b.key = key.get(); b.val = val.get();
-        b.key2 = key2.get(); b.val2 = val2.get(); b.key_out = key_out.get(); b.keep = keep.get();
-        b.off = off.get(); b.tile_sum = tile_sum.get(); b.hist = hist.get(); b.digit_total = digit_total.get();
+        b.id = id.get(); b.id2 = id2.get(); b.key = key.get(); b.key2 = key2.get(); b.key_out = key_out.get();
+        b.off = off.get(); b.hist = hist.get(); b.digit_total = digit_total.get();
+        b.tot = tot.get(); b.kmask = kmask.get(); b.surv = surv.get();
+        b.tile_s = tile_cells.get(); b.off_sorted = off_sorted.get();
         b.d_n = dn.get() + cur; b.d_n2 = dn.get() + (cur ^ 1);
         b.processed = processed.get();
         b.n_ub = n_ub;
+        b.capacity = capacity;
         return b;
     }
     SortedStepBuffers sorted_buffers() {
         SortedStepBuffers b;
-        b.x = x.get(); b.y = y.get(); b.id = id.get(); b.key = key_out.get(); b.off = off.get();
-        b.x2 = x2.get(); b.y2 = y2.get(); b.dir = dir.get();
+        b.x = x.get(); b.y = y.get(); b.id = id.get(); b.off = off.get();
+        b.x2 = x2.get(); b.y2 = y2.get(); b.code = code.get();
         b.cnt = cnt.get(); b.pre = pre.get(); b.tot = tot.get(); b.kmask = kmask.get(); b.surv = surv.get();
         b.tile_sum = tile_cells.get(); b.off_new = off2.get();
         b.far_cnt = far_cnt.get() + far_slot; b.far_cnt_next = far_cnt.get() + (far_slot ^ 1);
         b.far_in = far_in.get(); b.far_list = far_list.get();
         b.n_cur = dn.get() + cur; b.n_next = dn.get() + (cur ^ 1);
         b.processed = processed.get();
-        b.xo = xa.get(); b.yo = ya.get(); b.ido = ida.get(); b.keyo = keya.get();
+        b.xo = xa.get(); b.yo = ya.get(); b.ido = ida.get();
         b.n_ub = n_ub;
         b.capacity = capacity;
         return b;
@@ -164,34 +173,30 @@ int gpgmp_particles_create(const gpgmp_particle_params *p, int32_t capacity, voi
         ps->stream = (hipStream_t)stream;  // NULL: the default stream
         const size_t n = capacity;
         for (DevBuf<float> *b : {&ps->x, &ps->y, &ps->x2, &ps->y2}) b->alloc(n);
-        for (DevBuf<uint32_t> *b : {&ps->id, &ps->id2, &ps->key, &ps->val, &ps->key2, &ps->val2, &ps->key_out,
-                                    &ps->keep})
-            b->alloc(n);
+        for (DevBuf<uint32_t> *b : {&ps->id, &ps->id2, &ps->key, &ps->key2, &ps->key_out}) b->alloc(n);
         const size_t ncells = (size_t)p->side * p->side;
         ps->off.alloc(ncells + 1);
+        ps->off_sorted.alloc(ncells + 1);
+        for (DevBuf<uint32_t> *b : {&ps->tot, &ps->kmask, &ps->surv}) b->alloc(ncells);
+        ps->tile_cells.alloc((ncells + 255) / 256);
         const int nt = std::max(radix_tiles(capacity), 1);
-        ps->tile_sum.alloc(nt);
         ps->hist.alloc((size_t)256 * nt);
         ps->digit_total.alloc(256);
         // the step on the sorted list: moves of at most one cell per axis unless a Gaussian
         // displacement of 6 sigma plus the largest drift moves a whole cell (then the far
         // list, exact but slow when long, would fill; those parameters take the radix step)
         const double sigma = std::sqrt(2.0 * (double)p->D * (double)p->dt);
-        ps->sorted_ok = p->side >= 3 && 6.0 * sigma + std::fabs((double)p->vortex_amp) * (double)p->dt < 1.0;
+        ps->sorted_ok = p->side >= 3 && p->side <= kSortedMaxSide &&
+                        6.0 * sigma + std::fabs((double)p->vortex_amp) * (double)p->dt < 1.0;
         if (ps->sorted_ok) {
             ps->xa.alloc(n);
             ps->ya.alloc(n);
             ps->ida.alloc(n);
-            ps->keya.alloc(n);
-            ps->dir.alloc(n);
+            ps->code.alloc(n);
             ps->far_list.alloc(n * far_entry_bytes());
             ps->off2.alloc(ncells + 1);
             ps->cnt.alloc(9 * ncells);
             ps->pre.alloc(9 * ncells);
-            ps->tot.alloc(ncells);
-            ps->kmask.alloc(ncells);
-            ps->surv.alloc(ncells);
-            ps->tile_cells.alloc((ncells + 255) / 256);
             ps->far_cnt.alloc(2);
             ps->far_in.alloc(ncells);
             GPGMP_HIP_CHECK(hipMemsetAsync(ps->far_in.get(), 0, ncells * sizeof(uint32_t), ps->stream));
@@ -255,6 +260,11 @@ int gpgmp_particles_get(gpgmp_particles *ps, float *x, float *y, uint32_t *id, u
         *n_out = n;
         hipStream_t s = ps->stream;
         const size_t ncells = (size_t)ps->p.side * ps->p.side;
+        if (n && ps->keys_stale && (keys || off)) {
+            GPGMP_HIP_CHECK(launch_keys_from_pos(ps->x.get(), ps->y.get(), ps->dn.get() + ps->cur, n, ps->p.side,
+                                                 ps->key_out.get(), s));
+            ps->keys_stale = false;
+        }
         if (n) {
             if (x) GPGMP_HIP_CHECK(hipMemcpyAsync(x, ps->x.get(), (size_t)n * 4, hipMemcpyDeviceToHost, s));
             if (y) GPGMP_HIP_CHECK(hipMemcpyAsync(y, ps->y.get(), (size_t)n * 4, hipMemcpyDeviceToHost, s));
@@ -273,23 +283,19 @@ int gpgmp_particles_get(gpgmp_particles *ps, float *x, float *y, uint32_t *id, u
 int gpgmp_particles_set_sorted_step(gpgmp_particles *ps, int32_t mode) {
     return guarded_p([&] {
         GPGMP_REQUIRE(ps && mode >= 0 && mode <= 2, GPGMP_E_INVALID, "mode must be 0, 1 or 2");
-        GPGMP_REQUIRE(mode != 2 || ps->p.side >= 3, GPGMP_E_INVALID, "the sorted-list step needs side >= 3");
+        GPGMP_REQUIRE(mode != 2 || (ps->p.side >= 3 && ps->p.side <= kSortedMaxSide), GPGMP_E_INVALID,
+                      "the sorted-list step needs 3 <= side <= 16384");
         GPGMP_HIP_CHECK(hipStreamSynchronize(ps->stream));
         if (mode == 2 && !ps->xa.get()) {
             const size_t n = ps->capacity, ncells = (size_t)ps->p.side * ps->p.side;
             ps->xa.alloc(n);
             ps->ya.alloc(n);
             ps->ida.alloc(n);
-            ps->keya.alloc(n);
-            ps->dir.alloc(n);
+            ps->code.alloc(n);
             ps->far_list.alloc(n * far_entry_bytes());
             ps->off2.alloc(ncells + 1);
             ps->cnt.alloc(9 * ncells);
             ps->pre.alloc(9 * ncells);
-            ps->tot.alloc(ncells);
-            ps->kmask.alloc(ncells);
-            ps->surv.alloc(ncells);
-            ps->tile_cells.alloc((ncells + 255) / 256);
             ps->far_cnt.alloc(2);
             ps->far_in.alloc(ncells);
             GPGMP_HIP_CHECK(hipMemsetAsync(ps->far_in.get(), 0, ncells * sizeof(uint32_t), ps->stream));
@@ -300,7 +306,7 @@ int gpgmp_particles_set_sorted_step(gpgmp_particles *ps, int32_t mode) {
         if (mode == 2) ps->sorted_ok = true;
         if (mode == 1) {
             const double sigma = std::sqrt(2.0 * (double)ps->p.D * (double)ps->p.dt);
-            ps->sorted_ok = ps->xa.get() && ps->p.side >= 3 &&
+            ps->sorted_ok = ps->xa.get() && ps->p.side >= 3 && ps->p.side <= kSortedMaxSide &&
                             6.0 * sigma + std::fabs((double)ps->p.vortex_amp) * (double)ps->p.dt < 1.0;
         }
     });
@@ -335,18 +341,16 @@ int gpgmp_particles_step(gpgmp_particles *ps, int32_t step0, int32_t steps) {
                 std::swap(ps->x, ps->xa);
                 std::swap(ps->y, ps->ya);
                 std::swap(ps->id, ps->ida);
-                std::swap(ps->key_out, ps->keya);
                 std::swap(ps->off, ps->off2);
                 ps->cur ^= 1;
                 ps->far_slot ^= 1;
+                ps->keys_stale = true;
             } else {
                 // a list in any order: radix sort, offsets, annihilation, compaction
                 ParticleBuffers b = ps->buffers();
-                GPGMP_HIP_CHECK(particle_step(a, step0 + k, b, ps->stream));
-                // offsets of the survivors, which the next step on the sorted list starts from
-                GPGMP_HIP_CHECK(launch_lower_bound(ps->key_out.get(), ps->dn.get() + ps->cur, 0,
-                                                   (int)((size_t)ps->p.side * ps->p.side), ps->off.get(),
-                                                   ps->stream));
+                GPGMP_HIP_CHECK(particle_step(a, step0 + k, b, ps->stream));  // survivors' offsets to off
+                ps->cur ^= 1;
+                ps->keys_stale = false;
             }
             ps->stepped = true;
             ps->poll_n();

@@ -213,118 +213,60 @@ __global__ void k_left_right(const uint32_t *__restrict__ off, int ncells, uint3
 struct PartConsts {
     int side;
     float L;
-    double sigma;  // sqrt(2 D dt)
-    double dt;
-    double amp;
+    float inv_L;  // 1/L, exact when the side is a power of two
+    float sigma;  // sqrt(2 D dt), in float
+    float dt;
+    float amp;
     float p_ann;
     uint32_t seed;
 };
 
-__device__ __forceinline__ float wrap_pos(double xd, double L) {
-    xd = xd - L * floor(xd / L);
-    float xf = (float)xd;
-    if (xf >= (float)L) xf = xf - (float)L;
-    return xf;
+// v folded into [0, L): v - L floor(v / L), then one fold each way for the rounding.  POW2
+// (the side a power of two): v / L as the product v (1/L), which is then exact, so the bits
+// are those of the division.
+template <bool POW2>
+__device__ __forceinline__ float wrap_pos(float v, const PartConsts &p) {
+    const float L = p.L;
+    const float q = __builtin_floorf(POW2 ? v * p.inv_L : v / L);
+    v = v - L * q;
+    if (v < 0.0f) v = v + L;
+    if (v >= L) v = v - L;
+    return v;
 }
 
-// Move: x += v_x dt + sigma g_x with v = A(-sin(2 pi y/L), sin(2 pi x/L)), periodic;
-// writes the cell key for the sort.
+// Move, in float: x += v_x dt + sigma g_x with v = A(-sin(2 pi y/L), sin(2 pi x/L)),
+// periodic, g the Box-Muller pair of Threefry key {id, seed}, counter {step, 0}.
+template <bool POW2>
+__device__ __forceinline__ void move_particle(float xo, float yo, uint32_t idv, int step, const PartConsts &p,
+                                              float &xn, float &yn) {
+    float gx, gy;
+    gauss2_f(idv, p.seed, (uint32_t)step, &gx, &gy);
+    const float ty = POW2 ? (2.0f * yo) * p.inv_L : (2.0f * yo) / p.L;
+    const float tx = POW2 ? (2.0f * xo) * p.inv_L : (2.0f * xo) / p.L;
+    const float vx = -p.amp * sinpi_pf(ty);
+    const float vy = p.amp * sinpi_pf(tx);
+    xn = wrap_pos<POW2>(xo + vx * p.dt + p.sigma * gx, p);
+    yn = wrap_pos<POW2>(yo + vy * p.dt + p.sigma * gy, p);
+}
+
+// Move in place and write the cell key for the sort.
+template <bool POW2>
 __global__ void k_move(float *__restrict__ x, float *__restrict__ y, const uint32_t *__restrict__ id,
-                       uint32_t *__restrict__ keys, const int *d_n, int step,
-                       PartConsts p) {
+                       uint32_t *__restrict__ keys, const int *d_n, int step, PartConsts p) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *d_n) return;
-    double gx, gy;
-    gauss2(id[i], p.seed, (uint32_t)step, &gx, &gy);
-    const double xd = x[i], yd = y[i];
-    const double L = p.L;
-    const double vx = -p.amp * sinpi_d(2.0 * yd / L);
-    const double vy = p.amp * sinpi_d(2.0 * xd / L);
-    const float xn = wrap_pos(xd + vx * p.dt + p.sigma * gx, L);
-    const float yn = wrap_pos(yd + vy * p.dt + p.sigma * gy, L);
+    float xn, yn;
+    move_particle<POW2>(x[i], y[i], id[i], step, p, xn, yn);
     x[i] = xn;
     y[i] = yn;
     keys[i] = (uint32_t)((int)yn * p.side + (int)xn);
 }
 
-// A+A->0: the disjoint consecutive pairs (off[c]+2j, off[c]+2j+1) of cell c are removed
-// when u01(threefry({step, j}, {c, ~seed})) < p; keep[i] = 1 for survivors.
-__global__ void k_annihilate(const uint32_t *__restrict__ off, int ncells, uint32_t *__restrict__ keep, int step,
-                             PartConsts p) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    const uint32_t b = off[c], e = off[c + 1];
-    const uint32_t pairs = (e - b) >> 1;
-    for (uint32_t j = 0; j < pairs; ++j) {
-        const u32x2 w = threefry2x32_20((uint32_t)step, j, (uint32_t)c, ~p.seed);
-        const uint32_t k = (u01(w.x) < p.p_ann) ? 0u : 1u;
-        keep[b + 2 * j] = k;
-        keep[b + 2 * j + 1] = k;
-    }
-    if ((e - b) & 1u) keep[e - 1] = 1u;
-}
-
-// Per-tile survivor counts (compaction = a stable 1-bit partition).
-__global__ __launch_bounds__(RS_THREADS) void k_tile_count(const uint32_t *__restrict__ keep, const int *d_n,
-                                                            uint32_t *__restrict__ tile_sum) {
-    __shared__ uint32_t sm[RS_WAVES];
-    const int n = *d_n;
-    const size_t base = (size_t)blockIdx.x * RS_TILE;
-    uint32_t s = 0;
-    for (int j = 0; j < RS_ITEMS; ++j) {
-        const size_t i = base + (size_t)j * RS_THREADS + threadIdx.x;
-        if (i < (size_t)n) s += keep[i];
-    }
-    uint32_t tot;
-    (void)block_exclusive_scan(s, sm, tot);
-    if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
-}
-
-// Exclusive scan of tile sums (one block, any count) and the new particle count.
-__global__ __launch_bounds__(RS_THREADS) void k_scan_tiles(uint32_t *__restrict__ tile_sum, int ntiles,
-                                                            int *__restrict__ d_n_out) {
-    __shared__ uint32_t sm[RS_WAVES];
-    uint32_t carry = 0;
-    for (int c0 = 0; c0 < ntiles; c0 += RS_THREADS) {
-        const int i = c0 + threadIdx.x;
-        const uint32_t v = (i < ntiles) ? tile_sum[i] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_exclusive_scan(v, sm, tot);
-        if (i < ntiles) tile_sum[i] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) *d_n_out = (int)carry;
-}
-
-__global__ __launch_bounds__(RS_THREADS) void k_compact(
-    const uint32_t *__restrict__ keep, const uint32_t *__restrict__ tile_off, const int *d_n,
-    const float *__restrict__ x, const float *__restrict__ y, const uint32_t *__restrict__ id,
-    const uint32_t *__restrict__ key, float *__restrict__ xo, float *__restrict__ yo, uint32_t *__restrict__ ido,
-    uint32_t *__restrict__ keyo) {
-    __shared__ uint32_t sm[RS_WAVES];
-    const int n = *d_n;
-    const size_t base = (size_t)blockIdx.x * RS_TILE;
-    uint32_t run = tile_off[blockIdx.x];
-    for (int j = 0; j < RS_ITEMS; ++j) {
-        const size_t i = base + (size_t)j * RS_THREADS + threadIdx.x;
-        const uint32_t k = (i < (size_t)n) ? keep[i] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_exclusive_scan(k, sm, tot);
-        if (k) {
-            const uint32_t o = run + ex;
-            xo[o] = x[i];
-            yo[o] = y[i];
-            ido[o] = id[i];
-            keyo[o] = key[i];
-        }
-        run += tot;
-    }
-}
-
-// End of a step: count the particles the step processed, then install the new count.
-__global__ void k_finish_step(const int *n_next, int *n, unsigned long long *processed) {
-    *processed += (unsigned long long)*n;
-    *n = *n_next;
+// A+A->0 draws: pairs 2m and 2m+1 of cell c take the two words of the Threefry block
+// {step, m}, key {c, ~seed}; pair j is removed when u01(its word) < p.
+__device__ __forceinline__ bool pair_killed(int step, uint32_t j, uint32_t c, const PartConsts &p) {
+    const u32x2 w = threefry2x32_20((uint32_t)step, j >> 1, c, ~p.seed);
+    return u01((j & 1u) ? w.y : w.x) < p.p_ann;
 }
 
 // ---- particle step on a list already sorted by cell ---------------------------------------
@@ -338,11 +280,17 @@ __global__ void k_finish_step(const int *n_next, int *n, unsigned long long *pro
 // among the particles of c, which is the position std::stable_sort gives it (the list is
 // sorted by old key, so a smaller source key means a smaller index).  Particles that leave
 // the 3x3 neighbourhood go on a far list and are ranked the same way.  No radix pass and no
-// gather: every particle is read and written once, in nearly sequential order.
+// gather: every particle is read and written once per kernel, in nearly sequential order.
+//
+// The move takes tiles of consecutive old cells (64 per wave): a tile's particles are the
+// consecutive range off[c0] .. off[c0 + 64), so the per-cell counts and every particle's rank
+// among the earlier particles of its cell with its move class stay inside the wave (LDS).
+// Each particle then carries one code word to the scatter, which is flat over the particles.
 //
 // Direction codes: d = (dy + 1) * 3 + (dx + 1) for a move by (dx, dy) in {-1, 0, 1}^2 on the
-// torus (side >= 3), kFar otherwise.
+// torus (side >= 3), kFar otherwise; a near particle's old cell is its new cell moved by 8 - d.
 constexpr int kFar = 9;
+constexpr int CT = 256;  // old cells per block of the move (64 per wave)
 
 struct FarEntry {
     uint32_t i;  // index in the old order
@@ -366,118 +314,207 @@ __device__ __forceinline__ uint32_t moved_cell(int cx, int cy, int d, int L) {
     return (uint32_t)(wrap1(cy + (d / 3 - 1), L) * L + wrap1(cx + (d % 3 - 1), L));
 }
 
-// 1. move (as k_move) and classify the move; far moves are listed (order-free atomics: the
-//    list is only ever counted, never walked in order)
-__global__ void k_coh_move(const float *__restrict__ x, const float *__restrict__ y, const uint32_t *__restrict__ id,
-                           float *__restrict__ x2, float *__restrict__ y2, uint8_t *__restrict__ dir, const int *d_n,
-                           int step, PartConsts p, uint32_t *__restrict__ far_cnt, FarEntry *__restrict__ far_list,
-                           uint32_t *__restrict__ far_in) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *d_n) return;
-    double gx, gy;
-    gauss2(id[i], p.seed, (uint32_t)step, &gx, &gy);
-    const float xo = x[i], yo = y[i];
-    const double xd = xo, yd = yo;
-    const double L = p.L;
-    const double vx = -p.amp * sinpi_d(2.0 * yd / L);
-    const double vy = p.amp * sinpi_d(2.0 * xd / L);
-    const float xn = wrap_pos(xd + vx * p.dt + p.sigma * gx, L);
-    const float yn = wrap_pos(yd + vy * p.dt + p.sigma * gy, L);
-    x2[i] = xn;
-    y2[i] = yn;
+// Block -> work item with XCD-aware placement: the hardware dispatches block b to XCD b % 8,
+// so XCD k gets the contiguous items [k per, (k + 1) per) and the gathers from neighbouring
+// rows, like the partial-line stores into one region, stay in one L2.  The grid is 8 * per
+// blocks; a block past the last item returns at once.
+__device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t per) { return (b & 7u) * per + (b >> 3); }
+
+// 1. move the particles of the wave's 64 old cells, 64 consecutive particles per round,
+//    classify each move and rank it: among the particles of one old cell with one move class
+//    the rank is the count carried from the earlier rounds (LDS, per wave) plus the lower
+//    lanes of the round in the same cell run with the same class (a few __ballot calls).  Code
+//    word per particle: d | rank << 4 for a near move, kFar | old cell << 4 for a far one.
+//    The carried counts end as cnt[d][o] (every entry of the wave's cells written).  Far
+//    moves are listed with order-free atomics: the list is only counted, never walked in order.
+template <bool POW2>
+__global__ __launch_bounds__(CT) void k_cs_move(const float *__restrict__ x, const float *__restrict__ y,
+                                                const uint32_t *__restrict__ id, const uint32_t *__restrict__ off,
+                                                const int *d_n, int A, int step, PartConsts p,
+                                                float *__restrict__ x2, float *__restrict__ y2,
+                                                uint32_t *__restrict__ code, uint32_t *__restrict__ cnt,
+                                                uint32_t *__restrict__ far_cnt, FarEntry *__restrict__ far_list,
+                                                uint32_t *__restrict__ far_in) {
+    __shared__ uint32_t carry[CT / 64][9][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cw = blockIdx.x * CT + wave * 64;  // the wave's first cell
+    const int ncell = (A - cw < 64) ? A - cw : 64;
+    if (ncell <= 0) return;  // wave-uniform; no block barrier follows
+    uint32_t(*cr)[64] = carry[wave];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) cr[d][lane] = 0u;
+    const uint32_t n = (uint32_t)*d_n;
+    const uint32_t q0 = min(off[cw], n), q1 = min(off[cw + ncell], n);
+    const uint64_t lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int side = p.side;
-    const int cx = (int)xn, cy = (int)yn;
-    const int dx = torus_step((int)xo, cx, side);  // the old cell is the old position's (the key)
-    const int dy = torus_step((int)yo, cy, side);
-    const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1;
-    dir[i] = (uint8_t)(near ? (dy + 1) * 3 + (dx + 1) : kFar);
-    if (!near) {
-        const uint32_t c = (uint32_t)(cy * side + cx);
-        const uint32_t k = atomicAdd(far_cnt, 1u);
-        far_list[k] = FarEntry{(uint32_t)i, c};
-        atomicAdd(&far_in[c], 1u);
+    for (uint32_t b0 = q0; b0 < q1; b0 += 64) {
+        const uint32_t i = b0 + (uint32_t)lane;
+        const bool valid = i < q1;
+        float xo = 0.f, yo = 0.f;
+        uint32_t idv = 0u;
+        if (valid) {
+            xo = x[i];
+            yo = y[i];
+            idv = id[i];
+        }
+        float xn, yn;
+        move_particle<POW2>(xo, yo, idv, step, p, xn, yn);
+        const int ox = (int)xo, oy = (int)yo, cx = (int)xn, cy = (int)yn;
+        const int dx = torus_step(ox, cx, side);
+        const int dy = torus_step(oy, cy, side);
+        const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1;
+        const uint32_t d = near ? (uint32_t)((dy + 1) * 3 + (dx + 1)) : (uint32_t)kFar;
+        const uint32_t o = (uint32_t)(oy * side + ox);
+        const uint32_t lo = o - (uint32_t)cw;  // the wave's cell i sits in (below 64 for a consistent list)
+        const bool ranked = valid && near && lo < (uint32_t)ncell;
+        // the round's lanes are sorted by cell: a lane's cell run starts at the last head
+        // (a lane whose cell differs from its lower neighbour's) at or below it
+        const uint32_t lo_prev = (uint32_t)__shfl_up((int)lo, 1);
+        const uint64_t heads = __ballot(lane == 0 || lo != lo_prev);
+        const int s0 = 63 - __clzll(heads & (lower | (1ull << lane)));
+        const uint64_t run_lower = lower & ~((1ull << s0) - 1ull);
+        uint64_t same_d = __ballot(ranked);
+#pragma unroll
+        for (int bit = 0; bit < 4; ++bit) {
+            const bool v = (d >> bit) & 1u;
+            const uint64_t m = __ballot(v);
+            same_d &= v ? m : ~m;
+        }
+        uint32_t rank = 0;
+        if (ranked) rank = cr[d][lo] + (uint32_t)__popcll(same_d & run_lower);
+        // every read of the round precedes its increments (one wave: LDS operations in order)
+        if (ranked) atomicAdd(&cr[d][lo], 1u);
+        if (valid) {
+            x2[i] = xn;
+            y2[i] = yn;
+            code[i] = ranked ? (d | (rank << 4)) : ((uint32_t)kFar | (o << 4));
+            if (!near) {
+                const uint32_t c = (uint32_t)(cy * side + cx);
+                const uint32_t k = atomicAdd(far_cnt, 1u);
+                far_list[k] = FarEntry{i, c};
+                atomicAdd(&far_in[c], 1u);
+            }
+        }
+    }
+    if (lane < ncell) {
+#pragma unroll
+        for (int d = 0; d < 9; ++d) cnt[(size_t)d * A + cw + lane] = cr[d][lane];
     }
 }
 
-// 2. per old cell: its particles by direction, cnt[d][o] (every entry written)
-__global__ void k_coh_count(const uint32_t *__restrict__ off, const uint8_t *__restrict__ dir, int A,
-                            uint32_t *__restrict__ cnt) {
-    const int o = blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= A) return;
-    const uint32_t b = off[o], e = off[o + 1];
-    uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0, c6 = 0, c7 = 0, c8 = 0;
-    for (uint32_t i = b; i < e; ++i) {
-        const uint32_t d = dir[i];
-        c0 += d == 0; c1 += d == 1; c2 += d == 2; c3 += d == 3; c4 += d == 4;
-        c5 += d == 5; c6 += d == 6; c7 += d == 7; c8 += d == 8;
+// The A+A->0 draws of the pairs of a wave's 64 cells (lane k: cell cell0 + k with `pairs`
+// pairs), one Threefry block per two pairs, flattened over the wave: lane k takes the wave's
+// blocks k, k + 64, ..., whichever cells they belong to, so a wave runs ceil(blocks / 64)
+// draws instead of its busiest cell's count.  Returns the lane's cell's kills and the kill
+// mask of its first 32 pairs.  sbeg, spairs, skill, smask: 64 words of LDS per wave.  Every
+// thread of the block calls it (it holds block barriers).
+__device__ __forceinline__ void wave_pair_draws(uint32_t pairs, uint32_t cell0, int step, const PartConsts &p,
+                                                uint32_t *sbeg, uint32_t *spairs, uint32_t *skill,
+                                                uint32_t *smask, uint32_t &kills, uint32_t &mask) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t nb = (pairs + 1) >> 1;
+    const uint32_t incl = wave_incl_scan_u32(nb);
+    const uint32_t W = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    sbeg[lane] = incl - nb;
+    spairs[lane] = pairs;
+    skill[lane] = 0u;
+    smask[lane] = 0u;
+    __syncthreads();
+    for (uint32_t it = (uint32_t)lane; it < W; it += 64) {
+        int lo = 0, hi = 63;  // the last lane whose cell's blocks start at or before it
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (sbeg[mid] <= it) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t m = it - sbeg[lo];
+        const uint32_t j0 = 2 * m;
+        const u32x2 w = threefry2x32_20((uint32_t)step, m, cell0 + (uint32_t)lo, ~p.seed);
+        const bool k0 = u01(w.x) < p.p_ann;
+        const bool k1 = (j0 + 1 < spairs[lo]) && u01(w.y) < p.p_ann;
+        if (k0 || k1) {
+            atomicAdd(&skill[lo], (uint32_t)k0 + (uint32_t)k1);
+            if (j0 < 32) atomicOr(&smask[lo], (k0 ? 1u << j0 : 0u) | (k1 ? 1u << (j0 + 1) : 0u));
+        }
     }
-    const size_t sA = (size_t)A;
-    cnt[0 * sA + o] = c0; cnt[1 * sA + o] = c1; cnt[2 * sA + o] = c2;
-    cnt[3 * sA + o] = c3; cnt[4 * sA + o] = c4; cnt[5 * sA + o] = c5;
-    cnt[6 * sA + o] = c6; cnt[7 * sA + o] = c7; cnt[8 * sA + o] = c8;
+    __syncthreads();
+    kills = skill[lane];
+    mask = smask[lane];
 }
 
-__device__ __forceinline__ bool pair_killed(int step, uint32_t j, uint32_t c, const PartConsts &p) {
-    const u32x2 w = threefry2x32_20((uint32_t)step, j, c, ~p.seed);
-    return u01(w.x) < p.p_ann;
+// Place of a particle at rank r among the T particles of new cell c (kill mask m of the
+// first 32 pairs, survivors' offset base): false when its pair is killed, else the index
+// base + r - 2 (killed pairs before it).
+__device__ __forceinline__ bool survivor_place(uint32_t r, uint32_t T, uint32_t m, uint32_t base, uint32_t c,
+                                               int step, const PartConsts &p, uint32_t &pos) {
+    const uint32_t j = r >> 1;
+    const bool partner = (r | 1u) < T;
+    bool killed = false;
+    if (partner) killed = (j < 32) ? ((m >> j) & 1u) != 0 : pair_killed(step, j, c, p);
+    if (killed) return false;
+    uint32_t before = (uint32_t)__popc(m & ((j < 32) ? ((1u << j) - 1u) : 0xffffffffu));
+    for (uint32_t jj = 32; jj < j; ++jj) before += pair_killed(step, jj, c, p) ? 1u : 0u;
+    pos = base + r - 2u * before;
+    return true;
 }
 
-// 3. per new cell: the particles arriving from each source (pre[d][c] = the arrivals from
-//    the sources with a smaller key than direction d's), the total, the A+A->0 pair draws
-//    (first 32 pairs as a mask) and the survivors, summed per 256-cell tile
-__global__ __launch_bounds__(256) void k_coh_cells(const uint32_t *__restrict__ cnt, uint32_t *__restrict__ far_in,
-                                                   int A, int side, int step, PartConsts p, uint32_t *__restrict__ pre,
-                                                   uint32_t *__restrict__ tot, uint32_t *__restrict__ kmask,
-                                                   uint32_t *__restrict__ surv, uint32_t *__restrict__ tile_sum,
-                                                   uint32_t *__restrict__ far_cnt_next) {
+// 2. per new cell: the particles arriving from each source (pre[d][c] = the arrivals from
+//    the sources with a smaller key than direction d's), the total T with the far arrivals,
+//    the A+A->0 draws of its T/2 pairs, the kill mask of the first 32 pairs and the
+//    survivors, summed per 256-cell tile (the draws flattened over the wave, wave_pair_draws).
+__global__ __launch_bounds__(256) void k_cs_cells(const uint32_t *__restrict__ cnt, uint32_t *__restrict__ far_in,
+                                                  int A, int side, int step, PartConsts p, uint32_t *__restrict__ pre,
+                                                  uint32_t *__restrict__ tot, uint32_t *__restrict__ kmask,
+                                                  uint32_t *__restrict__ surv, uint32_t *__restrict__ tile_sum,
+                                                  uint32_t *__restrict__ far_cnt_next, uint32_t ntiles, uint32_t per) {
     __shared__ uint32_t sm[RS_WAVES];
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t sbeg[RS_WAVES][64];   // the wave's first block of each lane's cell
+    __shared__ uint32_t spairs[RS_WAVES][64];
+    __shared__ uint32_t skill[RS_WAVES][64];
+    __shared__ uint32_t smask[RS_WAVES][64];
+    const uint32_t tile = xcd_item(blockIdx.x, per);
+    if (tile >= ntiles) return;  // block-uniform, before any barrier
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = (int)tile * 256 + threadIdx.x;
     if (c == 0) *far_cnt_next = 0u;  // the far list of the next step starts empty
-    uint32_t S = 0;
+    uint32_t T = 0;
     if (c < A) {
         const int cy = c / side, cx = c - cy * side;
-        uint32_t key[9], n[9];
-        uint32_t T = 0;
+        uint32_t key[9], nn[9];
 #pragma unroll
         for (int d = 0; d < 9; ++d) {
             key[d] = moved_cell(cx, cy, 8 - d, side);  // the source whose move d lands on c
-            n[d] = cnt[(size_t)d * A + key[d]];
-            T += n[d];
+            nn[d] = cnt[(size_t)d * A + key[d]];
+            T += nn[d];
         }
 #pragma unroll
         for (int d = 0; d < 9; ++d) {
             uint32_t q = 0;
 #pragma unroll
-            for (int e = 0; e < 9; ++e) q += (key[e] < key[d]) ? n[e] : 0u;
+            for (int e = 0; e < 9; ++e) q += (key[e] < key[d]) ? nn[e] : 0u;
             pre[(size_t)d * A + c] = q;
         }
         const uint32_t fin = far_in[c];
         if (fin) far_in[c] = 0u;  // counted: clear for the next step
         T += fin;
-        uint32_t mask = 0, kills = 0;
-        const uint32_t pairs = T >> 1;
-        for (uint32_t j = 0; j < pairs; ++j) {
-            if (pair_killed(step, j, (uint32_t)c, p)) {
-                ++kills;
-                if (j < 32) mask |= 1u << j;
-            }
-        }
-        S = T - 2 * kills;
+    }
+    uint32_t kills, mask;
+    wave_pair_draws(T >> 1, tile * 256u + (uint32_t)(wave * 64), step, p, sbeg[wave], spairs[wave], skill[wave],
+                    smask[wave], kills, mask);
+    const uint32_t S = T - 2 * kills;
+    if (c < A) {
         tot[c] = T;
         kmask[c] = mask;
         surv[c] = S;
     }
     uint32_t total;
     (void)block_exclusive_scan(S, sm, total);
-    if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+    if (threadIdx.x == 0) tile_sum[tile] = total;
 }
 
-// 4. exclusive scan of the tile sums (one block of 1024 threads, 4 tiles per thread per
-//    round); the new count goes to the other count slot
-__global__ __launch_bounds__(1024) void k_coh_scan(uint32_t *__restrict__ tile_sum, int ntiles,
-                                                    const int *__restrict__ n_cur, int *__restrict__ n_next,
-                                                    unsigned long long *processed) {
-    __shared__ uint32_t wsum[16];
+// Exclusive scan in place of ntiles tile sums by one block of 1024 threads (4 tiles per thread
+// per round); returns the total.
+__device__ __forceinline__ uint32_t scan_tiles_1024(uint32_t *__restrict__ tile_sum, int ntiles, uint32_t *wsum) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t carry = 0;
     for (int c0 = 0; c0 < ntiles; c0 += 4096) {
@@ -507,13 +544,22 @@ __global__ __launch_bounds__(1024) void k_coh_scan(uint32_t *__restrict__ tile_s
         }
         carry += all;
     }
+    return carry;
+}
+
+// 3. exclusive scan of the survivors' tile sums; the new count goes to the other count slot
+__global__ __launch_bounds__(1024) void k_coh_scan(uint32_t *__restrict__ tile_sum, int ntiles,
+                                                    const int *__restrict__ n_cur, int *__restrict__ n_next,
+                                                    unsigned long long *processed) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t total = scan_tiles_1024(tile_sum, ntiles, wsum);
     if (threadIdx.x == 0) {
         *processed += (unsigned long long)*n_cur;
-        *n_next = (int)carry;
+        *n_next = (int)total;
     }
 }
 
-// 5. survivors' CSR offsets off[c] (and off[A] = the total)
+// 4. survivors' CSR offsets off[c] (and off[A] = the total)
 __global__ __launch_bounds__(256) void k_coh_offsets(const uint32_t *__restrict__ surv,
                                                      const uint32_t *__restrict__ tile_off, int A,
                                                      uint32_t *__restrict__ off) {
@@ -526,95 +572,170 @@ __global__ __launch_bounds__(256) void k_coh_offsets(const uint32_t *__restrict_
     if (c == A - 1) off[A] = tile_off[blockIdx.x] + ex + S;
 }
 
-// 6. every particle to its place in the new order; the killed pairs drop out.  A particle's
-//    rank among the earlier particles of its cell with its move class is counted over the
-//    move classes of the block's particles and the 256 before them, staged in LDS (a cell
-//    reaching further back counts the rest from global memory).  A thread takes CS_PER
-//    particles (a block CS_BLOCK * CS_PER consecutive ones) and issues their loads together:
-//    the kernel is bound by memory latency, not bandwidth, so each wave keeps several
-//    particles' loads in flight.
-constexpr int CS_BLOCK = 256, CS_PER = 4, CS_SPAN = CS_BLOCK * CS_PER, CS_BACK = 256;
+// 5. every particle to its place in the new order; the killed pairs drop out.  Flat over the
+//    particles: a thread takes SC_PER of them and issues their loads together (the kernel is
+//    bound by memory latency, so each wave keeps several particles' loads in flight).  A near
+//    particle's rank in its new cell is pre[d][c] plus its rank within (old cell, d) from the
+//    code word; a far one's counts the arrivals from the sources below its old cell.  Both
+//    add the far arrivals with a smaller index.
+constexpr int SC_BLOCK = 256, SC_PER = 4, SC_SPAN = SC_BLOCK * SC_PER;
 
-__global__ __launch_bounds__(CS_BLOCK) void k_coh_scatter(
+__global__ __launch_bounds__(SC_BLOCK) void k_cs_scatter(
     const float *__restrict__ x2, const float *__restrict__ y2, const uint32_t *__restrict__ id,
-    const uint32_t *__restrict__ key, const uint8_t *__restrict__ dir, const uint32_t *__restrict__ off_old,
-    const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ pre, const uint32_t *__restrict__ tot,
-    const uint32_t *__restrict__ kmask, const uint32_t *__restrict__ off_new, const uint32_t *__restrict__ far_cnt,
-    const FarEntry *__restrict__ far_list, const int *d_n, int A, int side, int step, PartConsts p,
-    float *__restrict__ xo, float *__restrict__ yo, uint32_t *__restrict__ ido, uint32_t *__restrict__ keyo,
-    uint32_t cap) {
-    __shared__ uint8_t sdir[CS_BACK + CS_SPAN];
-    const int n = *d_n;
-    const int b0 = blockIdx.x * CS_SPAN;
-    const int w0 = b0 - CS_BACK;  // first index staged
-    for (int k = threadIdx.x; k < CS_BACK + CS_SPAN; k += CS_BLOCK) {
-        const int j = w0 + k;
-        sdir[k] = (j >= 0 && j < n) ? dir[j] : (uint8_t)kFar;
-    }
-    __syncthreads();
+    const uint32_t *__restrict__ code, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ pre,
+    const uint32_t *__restrict__ tot, const uint32_t *__restrict__ kmask, const uint32_t *__restrict__ off_new,
+    const uint32_t *__restrict__ far_cnt, const FarEntry *__restrict__ far_list, const int *d_n, int A, int side,
+    int step, PartConsts p, float *__restrict__ xo, float *__restrict__ yo, uint32_t *__restrict__ ido,
+    uint32_t cap, uint32_t per) {
+    const uint32_t n = (uint32_t)*d_n;
+    const uint32_t b0 = xcd_item(blockIdx.x, per) * SC_SPAN;
+    if (b0 >= n) return;
     const uint32_t nf = *far_cnt;
-    float xv[CS_PER], yv[CS_PER];
-    uint32_t idv[CS_PER], o[CS_PER], d[CS_PER], c[CS_PER];
-    // every load that depends only on the index, then every one that depends only on the
-    // cells, so a wave waits on two rounds of memory latency for its CS_PER particles
+    float xv[SC_PER], yv[SC_PER];
+    uint32_t idv[SC_PER], cd[SC_PER];
 #pragma unroll
-    for (int q = 0; q < CS_PER; ++q) {
-        const int i = b0 + q * CS_BLOCK + (int)threadIdx.x;
+    for (int q = 0; q < SC_PER; ++q) {
+        const uint32_t i = b0 + q * SC_BLOCK + threadIdx.x;
         const bool v = i < n;
         xv[q] = v ? x2[i] : 0.f;
         yv[q] = v ? y2[i] : 0.f;
         idv[q] = v ? id[i] : 0u;
-        o[q] = v ? key[i] : 0u;
-        d[q] = v ? sdir[CS_BACK + q * CS_BLOCK + threadIdx.x] : (uint32_t)kFar;
+        cd[q] = v ? code[i] : (uint32_t)kFar;
     }
-    int s0[CS_PER];
-    uint32_t pr[CS_PER], T[CS_PER], m[CS_PER], base[CS_PER];
+    uint32_t c[SC_PER], pr[SC_PER], T[SC_PER], m[SC_PER], base[SC_PER];
 #pragma unroll
-    for (int q = 0; q < CS_PER; ++q) {
-        const int oy = (int)(o[q] / (uint32_t)side), ox = (int)o[q] - oy * side;
-        c[q] = (d[q] < 9) ? moved_cell(ox, oy, (int)d[q], side) : (uint32_t)((int)yv[q] * side + (int)xv[q]);
-        s0[q] = (int)off_old[o[q]];
-        pr[q] = (d[q] < 9) ? pre[(size_t)d[q] * A + c[q]] : 0u;
-        T[q] = tot[c[q]];
-        m[q] = kmask[c[q]];
-        base[q] = off_new[c[q]];
+    for (int q = 0; q < SC_PER; ++q) {
+        const uint32_t i = b0 + q * SC_BLOCK + threadIdx.x;
+        const bool v = i < n;
+        const uint32_t d = cd[q] & 15u;
+        c[q] = (uint32_t)((int)yv[q] * side + (int)xv[q]);
+        pr[q] = (v && d < 9) ? pre[(size_t)d * A + c[q]] + (cd[q] >> 4) : 0u;
+        T[q] = v ? tot[c[q]] : 0u;
+        m[q] = v ? kmask[c[q]] : 0u;
+        base[q] = v ? off_new[c[q]] : 0u;
     }
 #pragma unroll
-    for (int q = 0; q < CS_PER; ++q) {
-        const int i = b0 + q * CS_BLOCK + (int)threadIdx.x;
+    for (int q = 0; q < SC_PER; ++q) {
+        const uint32_t i = b0 + q * SC_BLOCK + threadIdx.x;
         if (i >= n) continue;
-        uint32_t r;
-        if (d[q] < 9) {
-            uint32_t k = 0;
-            for (int j = (s0[q] > w0 ? s0[q] : w0); j < i; ++j) k += sdir[j - w0] == d[q];
-            for (int j = s0[q]; j < w0; ++j) k += dir[j] == d[q];
-            r = pr[q] + k;
-        } else {
-            const int cy = (int)c[q] / side, cx = (int)c[q] - cy * side;
-            r = 0;
+        uint32_t r = pr[q];
+        if ((cd[q] & 15u) >= 9) {
+            const int cy = (int)yv[q], cx = (int)xv[q];
+            const uint32_t o = cd[q] >> 4;
             for (int e = 0; e < 9; ++e) {
                 const uint32_t s = moved_cell(cx, cy, 8 - e, side);
-                if (s < o[q]) r += cnt[(size_t)e * A + s];
+                if (s < o) r += cnt[(size_t)e * A + s];
             }
         }
         for (uint32_t f = 0; f < nf; ++f) {
             const FarEntry fe = far_list[f];
-            r += (fe.c == c[q] && fe.i < (uint32_t)i) ? 1u : 0u;
+            r += (fe.c == c[q] && fe.i < i) ? 1u : 0u;
         }
-        const uint32_t j = r >> 1;
-        const bool partner = (r | 1u) < T[q];
-        bool killed = false;
-        if (partner) killed = (j < 32) ? ((m[q] >> j) & 1u) != 0 : pair_killed(step, j, c[q], p);
-        if (killed) continue;
-        uint32_t before = (uint32_t)__popc(m[q] & ((j < 32) ? ((1u << j) - 1u) : 0xffffffffu));
-        for (uint32_t jj = 32; jj < j; ++jj) before += pair_killed(step, jj, c[q], p) ? 1u : 0u;
-        const uint32_t pos = base[q] + r - 2u * before;
+        uint32_t pos;
+        if (!survivor_place(r, T[q], m[q], base[q], c[q], step, p, pos)) continue;
+        if (pos >= cap) continue;  // never for a consistent list; keeps every store inside the buffers
+        xo[pos] = xv[q];
+        yo[pos] = yv[q];
+        ido[pos] = idv[q];
+    }
+}
+
+// ---- after the radix sort (a list in any order, moved and sorted by the new cell) ----------
+// off[c] = #(key < c) for c in [0, A] from the sorted keys: particle i writes off[c] = i for the
+// cells after the previous particle's key up to its own (the empty cells before it and its
+// own cell when it starts one); the thread past the last particle writes the cells after it.
+// Every entry is written once; an empty stretch of cells is written by one thread.
+__global__ void k_offsets_sorted(const uint32_t *__restrict__ key, const int *d_n, int A,
+                                 uint32_t *__restrict__ off) {
+    const uint32_t n = (uint32_t)*d_n;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    const uint32_t a = (uint32_t)A;
+    const uint32_t lo = (i == 0) ? 0u : min(key[i - 1], a - 1u) + 1u;
+    const uint32_t hi = (i == n) ? a : min(key[i], a);
+    for (uint32_t c = lo; c <= hi; ++c) off[c] = i;
+}
+
+// per cell: its particles T (from the sorted list's offsets), the A+A->0 draws, the kill mask
+// and the survivors, summed per 256-cell tile
+__global__ __launch_bounds__(256) void k_rs_cells(const uint32_t *__restrict__ off_sorted, int A, int step,
+                                                  PartConsts p, uint32_t *__restrict__ tot,
+                                                  uint32_t *__restrict__ kmask, uint32_t *__restrict__ surv,
+                                                  uint32_t *__restrict__ tile_s, uint32_t ntiles, uint32_t per) {
+    __shared__ uint32_t sm[RS_WAVES];
+    __shared__ uint32_t sbeg[RS_WAVES][64], spairs[RS_WAVES][64], skill[RS_WAVES][64], smask[RS_WAVES][64];
+    const uint32_t tile = xcd_item(blockIdx.x, per);
+    if (tile >= ntiles) return;  // block-uniform, before any barrier
+    const int wave = threadIdx.x >> 6;
+    const int c = (int)tile * 256 + threadIdx.x;
+    const uint32_t T = (c < A) ? off_sorted[c + 1] - off_sorted[c] : 0u;
+    uint32_t kills, mask;
+    wave_pair_draws(T >> 1, tile * 256u + (uint32_t)(wave * 64), step, p, sbeg[wave], spairs[wave], skill[wave],
+                    smask[wave], kills, mask);
+    const uint32_t S = T - 2 * kills;
+    if (c < A) {
+        tot[c] = T;
+        kmask[c] = mask;
+        surv[c] = S;
+    }
+    uint32_t total;
+    (void)block_exclusive_scan(S, sm, total);
+    if (threadIdx.x == 0) tile_s[tile] = total;
+}
+
+// the survivors of the sorted list to their places (stable compaction), flat over the list:
+// a particle's rank in its cell is its index less the cell's first index
+__global__ __launch_bounds__(SC_BLOCK) void k_rs_compact(
+    const uint32_t *__restrict__ key, const float *__restrict__ xs, const float *__restrict__ ys,
+    const uint32_t *__restrict__ ids, const uint32_t *__restrict__ off_sorted, const uint32_t *__restrict__ tot,
+    const uint32_t *__restrict__ kmask, const uint32_t *__restrict__ off_new, const int *d_n, int step, PartConsts p,
+    float *__restrict__ xo, float *__restrict__ yo, uint32_t *__restrict__ ido, uint32_t *__restrict__ keyo,
+    uint32_t cap, uint32_t per) {
+    const uint32_t n = (uint32_t)*d_n;
+    const uint32_t b0 = xcd_item(blockIdx.x, per) * SC_SPAN;
+    if (b0 >= n) return;
+    const uint32_t A = (uint32_t)p.side * (uint32_t)p.side;
+    float xv[SC_PER], yv[SC_PER];
+    uint32_t idv[SC_PER], c[SC_PER];
+#pragma unroll
+    for (int q = 0; q < SC_PER; ++q) {
+        const uint32_t i = b0 + q * SC_BLOCK + threadIdx.x;
+        const bool v = i < n;
+        c[q] = v ? key[i] : 0u;
+        xv[q] = v ? xs[i] : 0.f;
+        yv[q] = v ? ys[i] : 0.f;
+        idv[q] = v ? ids[i] : 0u;
+    }
+    uint32_t s0[SC_PER], T[SC_PER], m[SC_PER], base[SC_PER];
+#pragma unroll
+    for (int q = 0; q < SC_PER; ++q) {
+        const uint32_t i = b0 + q * SC_BLOCK + threadIdx.x;
+        const bool v = i < n && c[q] < A;
+        s0[q] = v ? off_sorted[c[q]] : i;
+        T[q] = v ? tot[c[q]] : 0u;
+        m[q] = v ? kmask[c[q]] : 0u;
+        base[q] = v ? off_new[c[q]] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < SC_PER; ++q) {
+        const uint32_t i = b0 + q * SC_BLOCK + threadIdx.x;
+        if (i >= n || c[q] >= A) continue;
+        uint32_t pos;
+        if (!survivor_place(i - s0[q], T[q], m[q], base[q], c[q], step, p, pos)) continue;
         if (pos >= cap) continue;  // never for a consistent list; keeps every store inside the buffers
         xo[pos] = xv[q];
         yo[pos] = yv[q];
         ido[pos] = idv[q];
         keyo[pos] = c[q];
     }
+}
+
+// Cell keys of a list from its positions (the sorted-list step does not store them).
+__global__ void k_keys_from_pos(const float *__restrict__ x, const float *__restrict__ y, const int *d_n, int side,
+                                uint32_t *__restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *d_n) return;
+    keys[i] = (uint32_t)((int)y[i] * side + (int)x[i]);
 }
 
 }  // namespace gpgmp
@@ -686,7 +807,8 @@ static PartConsts consts(const ParticleStepArgs &a) {
     PartConsts p;
     p.side = a.side;
     p.L = (float)a.side;
-    p.sigma = __builtin_sqrt(2.0 * (double)a.D * (double)a.dt);
+    p.inv_L = 1.0f / (float)a.side;
+    p.sigma = __builtin_sqrtf(2.0f * a.D * a.dt);
     p.dt = a.dt;
     p.amp = a.vortex_amp;
     p.p_ann = a.p_annihilate;
@@ -694,23 +816,33 @@ static PartConsts consts(const ParticleStepArgs &a) {
     return p;
 }
 
+static bool pow2_side(int side) { return side > 0 && (side & (side - 1)) == 0; }
+
 hipError_t particle_step_sorted(const ParticleStepArgs &a, int step, const SortedStepBuffers &b, hipStream_t s) {
     const PartConsts p = consts(a);
     const int A = a.side * a.side;
-    const dim3 blk(256);
-    const dim3 gN((b.n_ub + 255) / 256 > 0 ? (b.n_ub + 255) / 256 : 1), gA((A + 255) / 256);
-    const int ntiles = (A + 255) / 256;
-    hipLaunchKernelGGL(k_coh_move, gN, blk, 0, s, b.x, b.y, b.id, b.x2, b.y2, b.dir, b.n_cur, step, p, b.far_cnt,
-                       (FarEntry *)b.far_list, b.far_in);
-    hipLaunchKernelGGL(k_coh_count, gA, blk, 0, s, b.off, b.dir, A, b.cnt);
-    hipLaunchKernelGGL(k_coh_cells, gA, blk, 0, s, b.cnt, b.far_in, A, a.side, step, p, b.pre, b.tot, b.kmask, b.surv,
-                       b.tile_sum, b.far_cnt_next);
-    hipLaunchKernelGGL(k_coh_scan, dim3(1), dim3(1024), 0, s, b.tile_sum, ntiles, b.n_cur, b.n_next, b.processed);
-    hipLaunchKernelGGL(k_coh_offsets, gA, blk, 0, s, b.surv, b.tile_sum, A, b.off_new);
-    hipLaunchKernelGGL(k_coh_scatter, dim3((b.n_ub + CS_SPAN - 1) / CS_SPAN > 0 ? (b.n_ub + CS_SPAN - 1) / CS_SPAN : 1),
-                       dim3(CS_BLOCK), 0, s, b.x2, b.y2, b.id, b.key, b.dir, b.off, b.cnt, b.pre, b.tot, b.kmask,
-                       b.off_new, b.far_cnt, (const FarEntry *)b.far_list, b.n_cur, A, a.side, step, p, b.xo, b.yo,
-                       b.ido, b.keyo, (uint32_t)b.capacity);
+    const int tiles = (A + CT - 1) / CT;
+    const dim3 blk(CT);
+    hipLaunchKernelGGL((pow2_side(a.side) ? k_cs_move<true> : k_cs_move<false>), dim3(tiles), blk, 0, s, b.x, b.y,
+                       b.id, b.off, b.n_cur, A, step, p, b.x2, b.y2, b.code, b.cnt, b.far_cnt, (FarEntry *)b.far_list,
+                       b.far_in);
+    const uint32_t tper = (uint32_t)(tiles + 7) / 8;
+    hipLaunchKernelGGL(k_cs_cells, dim3(8 * tper), blk, 0, s, b.cnt, b.far_in, A, a.side, step, p, b.pre, b.tot,
+                       b.kmask, b.surv, b.tile_sum, b.far_cnt_next, (uint32_t)tiles, tper);
+    hipLaunchKernelGGL(k_coh_scan, dim3(1), dim3(1024), 0, s, b.tile_sum, tiles, b.n_cur, b.n_next, b.processed);
+    hipLaunchKernelGGL(k_coh_offsets, dim3(tiles), blk, 0, s, b.surv, b.tile_sum, A, b.off_new);
+    const int sblocks = std::max((b.n_ub + SC_SPAN - 1) / SC_SPAN, 1);
+    const uint32_t sper = (uint32_t)(sblocks + 7) / 8;
+    hipLaunchKernelGGL(k_cs_scatter, dim3(8 * sper), dim3(SC_BLOCK), 0, s, b.x2, b.y2, b.id, b.code, b.cnt, b.pre,
+                       b.tot, b.kmask, b.off_new, b.far_cnt, (const FarEntry *)b.far_list, b.n_cur, A, a.side, step, p,
+                       b.xo, b.yo, b.ido, (uint32_t)b.capacity, sper);
+    return hipGetLastError();
+}
+
+hipError_t launch_keys_from_pos(const float *x, const float *y, const int *d_n, int n_ub, int side, uint32_t *keys,
+                                hipStream_t s) {
+    if (n_ub <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_keys_from_pos, dim3((n_ub + 255) / 256), dim3(256), 0, s, x, y, d_n, side, keys);
     return hipGetLastError();
 }
 
@@ -719,10 +851,11 @@ size_t far_entry_bytes() { return sizeof(FarEntry); }
 hipError_t particle_step(const ParticleStepArgs &a, int step, ParticleBuffers &b, hipStream_t s) {
     const PartConsts p = consts(a);
     const int n_ub = b.n_ub;
-    const int ncells = a.side * a.side;
+    const int A = a.side * a.side;
     const dim3 gN((n_ub + 255) / 256 > 0 ? (n_ub + 255) / 256 : 1), blk(256);
     // 1. move (positions in place) and cell keys
-    hipLaunchKernelGGL(k_move, gN, blk, 0, s, b.x, b.y, b.id, b.key, b.d_n, step, p);
+    hipLaunchKernelGGL((pow2_side(a.side) ? k_move<true> : k_move<false>), gN, blk, 0, s, b.x, b.y, b.id, b.key, b.d_n,
+                       step, p);
     // 2. stable sort by cell key carrying the positions and ids; an odd number of passes
     //    (an all-zero digit pass above the key is a stable copy) leaves the result in the
     //    second buffers, from which the compaction writes back into the first ones
@@ -735,18 +868,20 @@ hipError_t particle_step(const ParticleStepArgs &a, int step, ParticleBuffers &b
                                       &in_tmp);
     if (e != hipSuccess) return e;
     if (!in_tmp) return hipErrorInvalidValue;
-    uint32_t *sk = b.key2;
-    // 3. offsets
-    hipLaunchKernelGGL(k_lower_bound, dim3((ncells + 1 + 255) / 256), dim3(256), 0, s, sk, b.d_n, 0, ncells, b.off);
-    // 4. annihilation flags
-    hipLaunchKernelGGL(k_annihilate, dim3((ncells + 255) / 256), dim3(256), 0, s, b.off, ncells, b.keep, step, p);
-    // 5. compaction back into the primary buffers
-    const int nt = std::max(radix_tiles(n_ub), 1);
-    hipLaunchKernelGGL(k_tile_count, dim3(nt), dim3(RS_THREADS), 0, s, b.keep, b.d_n, b.tile_sum);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(RS_THREADS), 0, s, b.tile_sum, nt, b.d_n2);
-    hipLaunchKernelGGL(k_compact, dim3(nt), dim3(RS_THREADS), 0, s, b.keep, b.tile_sum, b.d_n, b.x2, b.y2, b.id2, sk,
-                       b.x, b.y, b.id, b.key_out);
-    hipLaunchKernelGGL(k_finish_step, dim3(1), dim3(1), 0, s, b.d_n2, b.d_n, b.processed);
+    // 3. the sorted list's offsets; per cell: A+A->0 draws and survivors, scanned into the
+    //    survivors' (the next step's) offsets; the new count to d_n2
+    const int tiles = (A + 255) / 256;
+    const uint32_t tper = (uint32_t)(tiles + 7) / 8;
+    hipLaunchKernelGGL(k_offsets_sorted, dim3(n_ub / 256 + 1), blk, 0, s, b.key2, b.d_n, A, b.off_sorted);
+    hipLaunchKernelGGL(k_rs_cells, dim3(8 * tper), blk, 0, s, b.off_sorted, A, step, p, b.tot, b.kmask, b.surv,
+                       b.tile_s, (uint32_t)tiles, tper);
+    hipLaunchKernelGGL(k_coh_scan, dim3(1), dim3(1024), 0, s, b.tile_s, tiles, b.d_n, b.d_n2, b.processed);
+    hipLaunchKernelGGL(k_coh_offsets, dim3(tiles), blk, 0, s, b.surv, b.tile_s, A, b.off);
+    // 4. stable compaction back into the primary buffers
+    const int cblocks = std::max((n_ub + SC_SPAN - 1) / SC_SPAN, 1);
+    const uint32_t cper = (uint32_t)(cblocks + 7) / 8;
+    hipLaunchKernelGGL(k_rs_compact, dim3(8 * cper), dim3(SC_BLOCK), 0, s, b.key2, b.x2, b.y2, b.id2, b.off_sorted,
+                       b.tot, b.kmask, b.off, b.d_n, step, p, b.x, b.y, b.id, b.key_out, (uint32_t)b.capacity, cper);
     return hipGetLastError();
 }
 

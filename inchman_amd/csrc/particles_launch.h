@@ -18,19 +18,23 @@ struct ParticleStepArgs {
 // Device buffers of a particle system (capacity >= every n it holds).
 struct ParticleBuffers {
     float *x, *y, *x2, *y2;
-    uint32_t *id, *id2, *key, *val, *key2, *val2, *key_out, *keep, *off, *tile_sum, *hist, *digit_total;
+    uint32_t *id, *id2, *key, *key2, *key_out, *off, *hist, *digit_total;
+    uint32_t *tot, *kmask, *surv;  // per cell
+    uint32_t *tile_s;              // per 256-cell tile
+    uint32_t *off_sorted;          // offsets of the sorted list before the annihilation
     int *d_n, *d_n2;  // live count (device) and its next value
     unsigned long long *processed;  // particle-steps done so far (device)
     int n_ub;         // host upper bound of the live count (grid sizes)
+    int capacity;     // elements of every per-particle buffer
 };
 
-// Buffers of a step on a list already sorted by cell (the current list x, y, id, key with
-// its offsets off; the new list goes to xo, yo, ido, keyo with offsets off_new).
+// Buffers of a step on a list already sorted by cell (the current list x, y, id with its
+// offsets off; the new list goes to xo, yo, ido with offsets off_new; neither stores keys).
 struct SortedStepBuffers {
     const float *x, *y;
-    const uint32_t *id, *key, *off;
+    const uint32_t *id, *off;
     float *x2, *y2;           // moved positions, old order
-    uint8_t *dir;             // move class per particle
+    uint32_t *code;           // move class and rank (or old cell) per particle
     uint32_t *cnt, *pre;      // [9][A]
     uint32_t *tot, *kmask, *surv, *tile_sum, *off_new;
     uint32_t *far_cnt, *far_cnt_next, *far_in;
@@ -39,7 +43,7 @@ struct SortedStepBuffers {
     int *n_next;
     unsigned long long *processed;
     float *xo, *yo;
-    uint32_t *ido, *keyo;
+    uint32_t *ido;
     int n_ub;
     int capacity;             // elements of every per-particle buffer
 };
@@ -56,6 +60,11 @@ hipError_t radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *ktmp, uint
 hipError_t launch_lower_bound(const uint32_t *keys, const int *d_n, int n_host, int ncells, uint32_t *off,
                               hipStream_t s);
 hipError_t launch_left_right(const uint32_t *off, int ncells, uint32_t *left, uint32_t *right, hipStream_t s);
+// A step on a list in any order: move, radix sort, A+A->0, compaction; the survivors' count
+// goes to d_n2 and their offsets to off.
 hipError_t particle_step(const ParticleStepArgs &a, int step, ParticleBuffers &b, hipStream_t s);
+// keys[i] = floor(y) * side + floor(x) for the live particles (n_ub bounds the grid)
+hipError_t launch_keys_from_pos(const float *x, const float *y, const int *d_n, int n_ub, int side, uint32_t *keys,
+                                hipStream_t s);
 
 }  // namespace gpgmp

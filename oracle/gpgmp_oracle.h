@@ -79,7 +79,7 @@ void oracle_threefry2x32_20(uint32_t c0, uint32_t c1, uint32_t k0, uint32_t k1,
 float oracle_u01(uint32_t w);
 float oracle_logf(float y);
 /* the particle step's Gaussian pair (Box-Muller on Threefry key {id, seed}, counter {step, 0}) */
-void oracle_gauss2(uint32_t id, uint32_t seed, uint32_t step, double *gx, double *gy);
+void oracle_gauss2(uint32_t id, uint32_t seed, uint32_t step, float *gx, float *gy);
 double oracle_logd(double y);
 void oracle_mt19937_seeds(uint32_t seed0, int n, int32_t *out);
 

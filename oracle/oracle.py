@@ -604,6 +604,6 @@ def det_run(model, state, steps, flags=0):
 
 def gauss2(id_, seed, step):
     """The particle step's Gaussian pair for one particle."""
-    gx, gy = C.c_double(), C.c_double()
+    gx, gy = C.c_float(), C.c_float()
     lib().oracle_gauss2(C.c_uint32(id_), C.c_uint32(seed), C.c_uint32(step), C.byref(gx), C.byref(gy))
     return gx.value, gy.value

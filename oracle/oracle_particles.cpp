@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
@@ -57,60 +58,82 @@ extern "C" void oracle_cell_left_right(const uint32_t *k, int n, int ncells, uin
 
 namespace {
 
-// sin(pi t) in double: t reduced to [0,2), folded to r in [0,1/2], Taylor to x^21.
-// Pinned by: libm sin(M_PI*t) within 1e-15 absolute.
-double sinpi_d(double t) {
-    double t2 = t - 2.0 * std::floor(t * 0.5);
-    double sign = 1.0;
-    if (t2 >= 1.0) {
-        sign = -1.0;
-        t2 = t2 - 1.0;
+// log(u) in float for a normal u > 0: u = 2^e m, m in (sqrt(1/2), sqrt(2)], f = m - 1,
+// s = f / (2 + f), log(m) = 2 atanh(s) to s^9, Horner in s^2 with std::fma.
+// Pinned by: libm log within 3e-7 relative over the Box-Muller domain (test_gauss2_is_box_muller).
+float log_pf(float u) {
+    uint32_t b;
+    std::memcpy(&b, &u, 4);
+    int e = (int)((b >> 23) & 0xffu) - 127;
+    b = (b & 0x007fffffu) | 0x3f800000u;
+    float m;
+    std::memcpy(&m, &b, 4);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e = e + 1;
     }
-    double r = (t2 > 0.5) ? (1.0 - t2) : t2;
-    const double x = 3.141592653589793116 * r;
-    const double z = x * x;
-    // 1/(2k+1)! for k = 10..1, Horner in z
-    double p = 1.0 / 51090942171709440000.0;
-    p = -1.0 / 121645100408832000.0 + z * p;
-    p = 1.0 / 355687428096000.0 + z * p;
-    p = -1.0 / 1307674368000.0 + z * p;
-    p = 1.0 / 6227020800.0 + z * p;
-    p = -1.0 / 39916800.0 + z * p;
-    p = 1.0 / 362880.0 + z * p;
-    p = -1.0 / 5040.0 + z * p;
-    p = 1.0 / 120.0 + z * p;
-    p = -1.0 / 6.0 + z * p;
-    return sign * (x + x * (z * p));
+    const float f = m - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    float p = 1.0f / 9.0f;
+    p = std::fma(z, p, 1.0f / 7.0f);
+    p = std::fma(z, p, 1.0f / 5.0f);
+    p = std::fma(z, p, 1.0f / 3.0f);
+    const float two_s = 2.0f * s;
+    return std::fma((float)e, 0.693147182f, std::fma(two_s, z * p, two_s));
+}
+
+// sin(pi t) in float: t reduced to [0,2), folded to r in [0,1/2], x = pi r, Taylor to x^13 in
+// Horner form with std::fma (float overloads, correctly rounded as the GPU's).
+// Pinned by: libm sin(M_PI*t) within 1e-6 (test_gauss2_is_box_muller, both phases).
+float sinpi_pf(float t) {
+    float t2 = t - 2.0f * std::floor(t * 0.5f);
+    float sign = 1.0f;
+    if (t2 >= 1.0f) {
+        sign = -1.0f;
+        t2 = t2 - 1.0f;
+    }
+    const float r = (t2 > 0.5f) ? (1.0f - t2) : t2;
+    const float x = 3.14159274f * r;
+    const float z = x * x;
+    float p = 1.0f / 6227020800.0f;
+    p = std::fma(z, p, -1.0f / 39916800.0f);
+    p = std::fma(z, p, 1.0f / 362880.0f);
+    p = std::fma(z, p, -1.0f / 5040.0f);
+    p = std::fma(z, p, 1.0f / 120.0f);
+    p = std::fma(z, p, -1.0f / 6.0f);
+    return sign * std::fma(x, z * p, x);
 }
 
 // Gaussian pair by the Box-Muller transform on one Threefry call, key {id, seed}, counter
-// {step, 0}: u1, u2 = (w + 1/2) 2^-32, g = sqrt(-2 ln u1) (cos 2 pi u2, sin 2 pi u2) with
-// cos(2 pi u2) = sin(pi (2 u2 + 1/2)).  Pinned by: mean 0, variance 1 and independence of the
-// pair (test_oracle_particles), libm for log and sin.
-void gauss2(uint32_t id, uint32_t seed, uint32_t step, double *gx, double *gy) {
-    const double inv32 = 2.32830643653869628906e-010;
+// {step, 0}, in float: u1, u2 = (w / 2^9 + 1/2) 2^-23, g = sqrt(-2 ln u1) (cos 2 pi u2,
+// sin 2 pi u2) with cos(2 pi u2) = sin(pi (2 u2 + 1/2)).  Pinned by: libm on the same words,
+// mean 0, variance 1 and independence of the pair (test_oracle_particles).
+void gauss2(uint32_t id, uint32_t seed, uint32_t step, float *gx, float *gy) {
     uint32_t w0, w1;
     oracle_threefry2x32_20(step, 0u, id, seed, &w0, &w1);
-    const double u1 = ((double)w0 + 0.5) * inv32;
-    const double u2 = ((double)w1 + 0.5) * inv32;
-    const double r = std::sqrt(-2.0 * oracle_logd(u1));
-    *gx = r * sinpi_d(2.0 * u2 + 0.5);
-    *gy = r * sinpi_d(2.0 * u2);
+    const float u1 = ((float)(w0 >> 9) + 0.5f) * 0x1p-23f;
+    const float u2 = ((float)(w1 >> 9) + 0.5f) * 0x1p-23f;
+    const float r = std::sqrt(-2.0f * log_pf(u1));
+    *gx = r * sinpi_pf(2.0f * u2 + 0.5f);
+    *gy = r * sinpi_pf(2.0f * u2);
 }
 
 }  // namespace
 
-extern "C" void oracle_gauss2(uint32_t id, uint32_t seed, uint32_t step, double *gx, double *gy) {
+extern "C" void oracle_gauss2(uint32_t id, uint32_t seed, uint32_t step, float *gx, float *gy) {
     gauss2(id, seed, step, gx, gy);
 }
 
 namespace {
 
-inline float wrap_pos(double xd, double L) {
-    xd = xd - L * std::floor(xd / L);
-    float xf = (float)xd;
-    if (xf >= (float)L) xf = xf - (float)L;
-    return xf;
+// v folded into [0, L): v - L floor(v / L) in float, then one fold each way for the rounding.
+inline float wrap_pos(float v, float L) {
+    const float q = std::floor(v / L);
+    v = v - L * q;
+    if (v < 0.0f) v = v + L;
+    if (v >= L) v = v - L;
+    return v;
 }
 
 int nthr(int n) {
@@ -127,12 +150,13 @@ int nthr(int n) {
 // One step of the individual-particle workload (no reference counterpart; nearest
 // analogues: individual_diffuseX/Y IndividualSolver.cl:750-1014 for the move and
 // maintainPopulation :1017-1882 for removal):
-//  1. move: x += v_x dt + sqrt(2 D dt) g_x, v = A(-sin(2 pi y/L), sin(2 pi x/L)),
+//  1. move, in float: x += v_x dt + sqrt(2 D dt) g_x, v = A(-sin(2 pi y/L), sin(2 pi x/L)),
 //     periodic on [0,L), key = floor(y)*side + floor(x);
 //  2. stable sort by key (payload: x, y, id);
 //  3. offsets off[c] = #(key < c);
 //  4. A+A->0: in every cell the disjoint consecutive pairs (off[c]+2j, off[c]+2j+1)
-//     are removed when u01(threefry({step, j}, {c, ~seed})) < p;
+//     are removed when u01(w) < p, w = word (j mod 2) of threefry({step, j / 2}, {c, ~seed})
+//     (one Threefry block serves two pairs);
 //  5. order-preserving compaction; off recomputed for the survivors.
 // Returns the new particle count.  Pinned by: per-step invariants (sortedness,
 // stability, offsets vs lower_bound, conservation without annihilation, displacement
@@ -140,20 +164,20 @@ int nthr(int n) {
 extern "C" int oracle_particle_step(const oracle_particle_params *p, int step, int n, float *x,
                                     float *y, uint32_t *id, uint32_t *keys, uint32_t *off, int nt) {
     const int side = p->side;
-    const double L = (double)side;
+    const float L = (float)side;
     const int A = side * side;
-    const double sigma = std::sqrt(2.0 * (double)p->D * (double)p->dt);
-    const double dt = p->dt, amp = p->vortex_amp;
+    const float sigma = std::sqrt(2.0f * p->D * p->dt);
+    const float dt = p->dt, amp = p->vortex_amp;
     std::vector<uint32_t> k(n);
 #pragma omp parallel for schedule(static) num_threads(nthr(nt))
     for (int i = 0; i < n; ++i) {
-        double gx, gy;
+        float gx, gy;
         gauss2(id[i], p->seed, (uint32_t)step, &gx, &gy);
-        const double xd = x[i], yd = y[i];
-        const double vx = -amp * sinpi_d(2.0 * yd / L);
-        const double vy = amp * sinpi_d(2.0 * xd / L);
-        const float xn = wrap_pos(xd + vx * dt + sigma * gx, L);
-        const float yn = wrap_pos(yd + vy * dt + sigma * gy, L);
+        const float xo = x[i], yo = y[i];
+        const float vx = -amp * sinpi_pf((2.0f * yo) / L);
+        const float vy = amp * sinpi_pf((2.0f * xo) / L);
+        const float xn = wrap_pos(xo + vx * dt + sigma * gx, L);
+        const float yn = wrap_pos(yo + vy * dt + sigma * gy, L);
         x[i] = xn;
         y[i] = yn;
         k[i] = (uint32_t)((int)yn * side + (int)xn);
@@ -178,8 +202,8 @@ extern "C" int oracle_particle_step(const oracle_particle_params *p, int step, i
         const uint32_t pairs = (e - b) / 2;
         for (uint32_t j = 0; j < pairs; ++j) {
             uint32_t w0, w1;
-            oracle_threefry2x32_20((uint32_t)step, j, (uint32_t)c, ~p->seed, &w0, &w1);
-            if (oracle_u01(w0) < p->p_annihilate) {
+            oracle_threefry2x32_20((uint32_t)step, j >> 1, (uint32_t)c, ~p->seed, &w0, &w1);
+            if (oracle_u01((j & 1u) ? w1 : w0) < p->p_annihilate) {
                 keep[b + 2 * j] = 0;
                 keep[b + 2 * j + 1] = 0;
             }

@@ -63,8 +63,8 @@ def test_particle_annihilation_rate(oracle):
 
 
 def test_gauss2_is_box_muller(oracle):
-    """The particle Gaussian is the Box-Muller transform of one Threefry call: against libm on
-    the same words (a swapped sine/cosine, a missing factor or a wrong word would differ),
+    """The particle Gaussian is the Box-Muller transform of one Threefry call, in float: against
+    libm on the same words (a swapped sine/cosine, a missing factor or a wrong word would differ),
     and the pair is standard normal and uncorrelated."""
     import math
 
@@ -74,10 +74,11 @@ def test_gauss2_is_box_muller(oracle):
     for k, i in enumerate(ids):
         gx, gy = oracle.gauss2(int(i), 1234567, k % 7)
         w0, w1 = oracle.threefry(k % 7, 0, int(i), 1234567)
-        u1, u2 = (w0 + 0.5) * 2.0**-32, (w1 + 0.5) * 2.0**-32
+        u1, u2 = ((w0 >> 9) + 0.5) * 2.0**-23, ((w1 >> 9) + 0.5) * 2.0**-23
         r = math.sqrt(-2.0 * math.log(u1))
-        assert abs(gx - r * math.cos(2 * math.pi * u2)) <= 1e-12 * max(1.0, abs(gx))
-        assert abs(gy - r * math.sin(2 * math.pi * u2)) <= 1e-12 * max(1.0, abs(gy))
+        # float arithmetic: a few float ulps of r, and the float rounding of the phase
+        assert abs(gx - r * math.cos(2 * math.pi * u2)) <= 2e-6 * max(1.0, r)
+        assert abs(gy - r * math.sin(2 * math.pi * u2)) <= 2e-6 * max(1.0, r)
         g.append((gx, gy))
     g = np.array(g)
     assert np.all(np.abs(g.mean(axis=0)) < 0.1)
