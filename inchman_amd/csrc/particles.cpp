@@ -44,7 +44,8 @@ struct gpgmp_particles {
     DevBuf<float> x, y, x2, y2;
     DevBuf<uint32_t> id, id2, key, key2, key_out, off, hist, digit_total;
     // per-cell work of both steps
-    DevBuf<uint32_t> tot, kmask, surv, tile_cells, off_sorted;
+    DevBuf<uint4> rec;
+    DevBuf<uint32_t> surv, tile_cells, off_sorted;
     // step on the sorted list: the next list (xa, ya, ida, off2) and its per-cell work
     DevBuf<float> xa, ya;
     DevBuf<uint32_t> ida, off2, cnt, pre, far_cnt, far_in;
@@ -71,7 +72,7 @@ struct gpgmp_particles {
         b.x = x.get(); b.y = y.get(); b.x2 = x2.get(); b.y2 = y2.get();
         b.id = id.get(); b.id2 = id2.get(); b.key = key.get(); b.key2 = key2.get(); b.key_out = key_out.get();
         b.off = off.get(); b.hist = hist.get(); b.digit_total = digit_total.get();
-        b.tot = tot.get(); b.kmask = kmask.get(); b.surv = surv.get();
+        b.rec = rec.get(); b.surv = surv.get();
         b.tile_s = tile_cells.get(); b.off_sorted = off_sorted.get();
         b.d_n = dn.get() + cur; b.d_n2 = dn.get() + (cur ^ 1);
         b.processed = processed.get();
@@ -83,7 +84,7 @@ struct gpgmp_particles {
         SortedStepBuffers b;
         b.x = x.get(); b.y = y.get(); b.id = id.get(); b.off = off.get();
         b.x2 = x2.get(); b.y2 = y2.get(); b.code = code.get();
-        b.cnt = cnt.get(); b.pre = pre.get(); b.tot = tot.get(); b.kmask = kmask.get(); b.surv = surv.get();
+        b.cnt = cnt.get(); b.pre = pre.get(); b.rec = rec.get(); b.surv = surv.get();
         b.tile_sum = tile_cells.get(); b.off_new = off2.get();
         b.far_cnt = far_cnt.get() + far_slot; b.far_cnt_next = far_cnt.get() + (far_slot ^ 1);
         b.far_in = far_in.get(); b.far_list = far_list.get();
@@ -177,7 +178,8 @@ int gpgmp_particles_create(const gpgmp_particle_params *p, int32_t capacity, voi
         const size_t ncells = (size_t)p->side * p->side;
         ps->off.alloc(ncells + 1);
         ps->off_sorted.alloc(ncells + 1);
-        for (DevBuf<uint32_t> *b : {&ps->tot, &ps->kmask, &ps->surv}) b->alloc(ncells);
+        ps->rec.alloc(ncells);
+        ps->surv.alloc(ncells);
         ps->tile_cells.alloc((ncells + 255) / 256);
         const int nt = std::max(radix_tiles(capacity), 1);
         ps->hist.alloc((size_t)256 * nt);

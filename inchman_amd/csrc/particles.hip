@@ -6,8 +6,9 @@
 // (IndividualSolver.cl:36-553, IndividualSolver.cpp:673-808).  Here a pass uses 8-bit
 // digits and wave64 ranking: the lanes holding the same digit are found with eight
 // __ballot calls, a lane's rank is the popcount of its lower peers, and the per-wave
-// digit counts are combined in LDS in (round, wave, lane) order, which is the original
-// index order -- so every pass is stable and the result equals std::stable_sort.
+// digit counts are combined in LDS in (wave, round, lane) order over a wave's contiguous
+// part of the tile, which is the original index order -- so every pass is stable and the
+// result equals std::stable_sort.
 #include <hip/hip_runtime.h>
 
 #include "gpgmp_numeric.h"
@@ -91,58 +92,50 @@ struct RadixPayload {
     uint32_t *out[3];
 };
 
-// Stable scatter of one pass, staged in LDS.  A tile of 4096 keys (16 per thread, in registers
-// with their payload) is first counted by digit, then ranked in rounds j of 256 consecutive
-// keys (index tile*4096 + j*256 + thread, so (round, wave, lane) order is index order): the
-// lanes of a wave holding the same digit are found with eight __ballot calls, a key's rank
-// is the popcount of its lower peers plus the counts of the earlier waves and rounds, and it
-// is written to LDS at its digit's place in the tile.  The tile, now sorted by digit, is
-// written out in LDS order: consecutive threads write consecutive places of one digit's run,
-// so the stores coalesce instead of landing one word at a time all over the output.
+// Stable scatter of one pass, staged in LDS.  Wave w of a block takes the contiguous part
+// [w * 64 * RS_ITEMS, (w + 1) * 64 * RS_ITEMS) of the tile, 64 consecutive keys per round, with
+// the keys and their payload in registers, and ranks each key without block barriers: the
+// lanes of the round holding the same digit are found with eight __ballot calls, and the
+// key's rank within the wave is the count of its digit in the wave's earlier rounds (kept in
+// the wave's own LDS counters, bumped by the digit group's last lane) plus its lower peers.
+// Then one pass over the digits turns the per-wave counts into each wave's offset within its
+// digit's run of the tile -- (wave, round, lane) order is index order, so the pass is stable
+// -- and the keys go to LDS at their digit's place.  The tile, now sorted by digit, is written
+// out in LDS order: consecutive threads write consecutive places of one digit's run, so the
+// stores coalesce instead of landing one word at a time all over the output.
 template <int NP>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const uint32_t *__restrict__ kin, uint32_t *__restrict__ kout, RadixPayload pl, const int *d_n, int n_host,
     int shift, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ digit_base, int ntiles) {
     __shared__ uint32_t skey[RS_TILE];
     __shared__ uint32_t spay[NP][RS_TILE];
-    __shared__ uint32_t tstart[256];  // the tile's first place of each digit
-    __shared__ uint32_t gbase[256];   // output place of the tile's first key of each digit
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wc[RS_WAVES][256];
+    __shared__ uint32_t wcnt[RS_WAVES][256];  // per wave: its keys of each digit, then its offset in the run
+    __shared__ uint32_t tstart[256];          // the tile's first place of each digit
+    __shared__ uint32_t gbase[256];           // output place of the tile's first key of each digit
     __shared__ uint32_t sm[RS_WAVES];
     const int n = load_n(d_n, n_host);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const size_t tbase = (size_t)blockIdx.x * RS_TILE;
-    uint32_t key[RS_ITEMS], pay[NP][RS_ITEMS];
-    run[threadIdx.x] = 0;
+    const size_t wbase = tbase + (size_t)wave * (RS_ITEMS * 64);
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) wcnt[w][threadIdx.x] = 0;
     gbase[threadIdx.x] = digit_base[threadIdx.x] + hist[(size_t)threadIdx.x * ntiles + blockIdx.x];
-    __syncthreads();
+    uint32_t key[RS_ITEMS], pay[NP][RS_ITEMS], rank[RS_ITEMS];
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
-        const size_t i = tbase + (size_t)j * RS_THREADS + threadIdx.x;
+        const size_t i = wbase + (size_t)j * 64 + lane;
         const bool valid = i < (size_t)n;
         key[j] = valid ? kin[i] : 0u;
 #pragma unroll
         for (int q = 0; q < NP; ++q) pay[q][j] = valid ? pl.in[q][i] : 0u;
-        if (valid) atomicAdd(&run[(key[j] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    {
-        uint32_t tot;
-        const uint32_t c = run[threadIdx.x];
-        tstart[threadIdx.x] = block_exclusive_scan(c, sm, tot);
-        run[threadIdx.x] = 0;
-    }
-    __syncthreads();
+    uint32_t *cw = wcnt[wave];
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
-        const size_t i = tbase + (size_t)j * RS_THREADS + threadIdx.x;
-        const bool valid = i < (size_t)n;
+        const bool valid = wbase + (size_t)j * 64 + lane < (size_t)n;
         const uint32_t d = (key[j] >> shift) & 255u;
-#pragma unroll
-        for (int w = 0; w < RS_WAVES; ++w) wc[w][threadIdx.x] = 0;
-        __syncthreads();
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -150,29 +143,36 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
             const uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
         }
-        const uint32_t rank = (uint32_t)__popcll(peers & lt);
-        if (valid && rank == 0) wc[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {
-            const uint32_t r = run[threadIdx.x];
-            uint32_t acc = r;
+        const uint32_t before = cw[d];
+        rank[j] = before + (uint32_t)__popcll(peers & lt);
+        // every lane has read the count before the group's last lane bumps it (one wave:
+        // LDS operations in order)
+        if (valid && (peers >> lane) == 1ull) cw[d] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    {
+        uint32_t acc = 0;  // thread t: digit t
 #pragma unroll
-            for (int w = 0; w < RS_WAVES; ++w) {
-                const uint32_t c = wc[w][threadIdx.x];
-                wc[w][threadIdx.x] = acc;
-                acc += c;
-            }
-            run[threadIdx.x] = acc;
+        for (int w = 0; w < RS_WAVES; ++w) {
+            const uint32_t c = wcnt[w][threadIdx.x];
+            wcnt[w][threadIdx.x] = acc;
+            acc += c;
         }
-        __syncthreads();
-        if (valid) {
-            const uint32_t lp = tstart[d] + wc[wave][d] + rank;
+        uint32_t tot;
+        tstart[threadIdx.x] = block_exclusive_scan(acc, sm, tot);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        if (wbase + (size_t)j * 64 + lane < (size_t)n) {
+            const uint32_t d = (key[j] >> shift) & 255u;
+            const uint32_t lp = tstart[d] + cw[d] + rank[j];
             skey[lp] = key[j];
 #pragma unroll
             for (int q = 0; q < NP; ++q) spay[q][lp] = pay[q][j];
         }
-        __syncthreads();  // wc is cleared by the next round
     }
+    __syncthreads();
     const int cnt = (n - (int)tbase < RS_TILE) ? n - (int)tbase : RS_TILE;
     for (int lp = threadIdx.x; lp < cnt; lp += RS_THREADS) {
         const uint32_t k = skey[lp];
@@ -459,13 +459,14 @@ __device__ __forceinline__ bool survivor_place(uint32_t r, uint32_t T, uint32_t 
 }
 
 // 2. per new cell: the particles arriving from each source (pre[d][c] = the arrivals from
-//    the sources with a smaller key than direction d's), the total T with the far arrivals,
+//    the sources with a smaller key than direction d's; pre[4][c], the particles that stay,
+//    goes to the cell record), the total T with the far arrivals,
 //    the A+A->0 draws of its T/2 pairs, the kill mask of the first 32 pairs and the
 //    survivors, summed per 256-cell tile (the draws flattened over the wave, wave_pair_draws).
 __global__ __launch_bounds__(256) void k_cs_cells(const uint32_t *__restrict__ cnt, uint32_t *__restrict__ far_in,
                                                   int A, int side, int step, PartConsts p, uint32_t *__restrict__ pre,
-                                                  uint32_t *__restrict__ tot, uint32_t *__restrict__ kmask,
-                                                  uint32_t *__restrict__ surv, uint32_t *__restrict__ tile_sum,
+                                                  uint4 *__restrict__ rec, uint32_t *__restrict__ surv,
+                                                  uint32_t *__restrict__ tile_sum,
                                                   uint32_t *__restrict__ far_cnt_next, uint32_t ntiles, uint32_t per) {
     __shared__ uint32_t sm[RS_WAVES];
     __shared__ uint32_t sbeg[RS_WAVES][64];   // the wave's first block of each lane's cell
@@ -477,7 +478,7 @@ __global__ __launch_bounds__(256) void k_cs_cells(const uint32_t *__restrict__ c
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = (int)tile * 256 + threadIdx.x;
     if (c == 0) *far_cnt_next = 0u;  // the far list of the next step starts empty
-    uint32_t T = 0;
+    uint32_t T = 0, pre4 = 0;
     if (c < A) {
         const int cy = c / side, cx = c - cy * side;
         uint32_t key[9], nn[9];
@@ -492,7 +493,8 @@ __global__ __launch_bounds__(256) void k_cs_cells(const uint32_t *__restrict__ c
             uint32_t q = 0;
 #pragma unroll
             for (int e = 0; e < 9; ++e) q += (key[e] < key[d]) ? nn[e] : 0u;
-            pre[(size_t)d * A + c] = q;
+            if (d == 4) pre4 = q;  // the particles that stay: kept in the cell record
+            else pre[(size_t)d * A + c] = q;
         }
         const uint32_t fin = far_in[c];
         if (fin) far_in[c] = 0u;  // counted: clear for the next step
@@ -503,8 +505,7 @@ __global__ __launch_bounds__(256) void k_cs_cells(const uint32_t *__restrict__ c
                     smask[wave], kills, mask);
     const uint32_t S = T - 2 * kills;
     if (c < A) {
-        tot[c] = T;
-        kmask[c] = mask;
+        rec[c] = make_uint4(0u, T, mask, pre4);  // .x: the survivors' offset, written after the scan
         surv[c] = S;
     }
     uint32_t total;
@@ -559,32 +560,36 @@ __global__ __launch_bounds__(1024) void k_coh_scan(uint32_t *__restrict__ tile_s
     }
 }
 
-// 4. survivors' CSR offsets off[c] (and off[A] = the total)
+// 4. survivors' CSR offsets off[c] (and off[A] = the total), also into the cell records
+//    {offset, particles, kill mask, particles that stay} that the scatter reads in one load
 __global__ __launch_bounds__(256) void k_coh_offsets(const uint32_t *__restrict__ surv,
                                                      const uint32_t *__restrict__ tile_off, int A,
-                                                     uint32_t *__restrict__ off) {
+                                                     uint32_t *__restrict__ off, uint4 *__restrict__ rec) {
     __shared__ uint32_t sm[RS_WAVES];
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t S = (c < A) ? surv[c] : 0u;
     uint32_t total;
     const uint32_t ex = block_exclusive_scan(S, sm, total);
-    if (c < A) off[c] = tile_off[blockIdx.x] + ex;
-    if (c == A - 1) off[A] = tile_off[blockIdx.x] + ex + S;
+    const uint32_t o = tile_off[blockIdx.x] + ex;
+    if (c < A) {
+        off[c] = o;
+        rec[c].x = o;
+    }
+    if (c == A - 1) off[A] = o + S;
 }
 
 // 5. every particle to its place in the new order; the killed pairs drop out.  Flat over the
 //    particles: a thread takes SC_PER of them and issues their loads together (the kernel is
-//    bound by memory latency, so each wave keeps several particles' loads in flight).  A near
-//    particle's rank in its new cell is pre[d][c] plus its rank within (old cell, d) from the
-//    code word; a far one's counts the arrivals from the sources below its old cell.  Both
+//    bound by memory latency, so each wave keeps several particles' loads in flight), and one
+//    16-byte load of its new cell's record.  A near particle's rank in its new cell is
+//    pre[d][c] (the record's for d = 4) plus its rank within (old cell, d) from the code word; a far one's counts the arrivals from the sources below its old cell.  Both
 //    add the far arrivals with a smaller index.
 constexpr int SC_BLOCK = 256, SC_PER = 4, SC_SPAN = SC_BLOCK * SC_PER;
 
 __global__ __launch_bounds__(SC_BLOCK) void k_cs_scatter(
     const float *__restrict__ x2, const float *__restrict__ y2, const uint32_t *__restrict__ id,
     const uint32_t *__restrict__ code, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ pre,
-    const uint32_t *__restrict__ tot, const uint32_t *__restrict__ kmask, const uint32_t *__restrict__ off_new,
-    const uint32_t *__restrict__ far_cnt, const FarEntry *__restrict__ far_list, const int *d_n, int A, int side,
+    const uint4 *__restrict__ rec, const uint32_t *__restrict__ far_cnt, const FarEntry *__restrict__ far_list, const int *d_n, int A, int side,
     int step, PartConsts p, float *__restrict__ xo, float *__restrict__ yo, uint32_t *__restrict__ ido,
     uint32_t cap, uint32_t per) {
     const uint32_t n = (uint32_t)*d_n;
@@ -609,10 +614,11 @@ __global__ __launch_bounds__(SC_BLOCK) void k_cs_scatter(
         const bool v = i < n;
         const uint32_t d = cd[q] & 15u;
         c[q] = (uint32_t)((int)yv[q] * side + (int)xv[q]);
-        pr[q] = (v && d < 9) ? pre[(size_t)d * A + c[q]] + (cd[q] >> 4) : 0u;
-        T[q] = v ? tot[c[q]] : 0u;
-        m[q] = v ? kmask[c[q]] : 0u;
-        base[q] = v ? off_new[c[q]] : 0u;
+        const uint4 rc = v ? rec[c[q]] : make_uint4(0u, 0u, 0u, 0u);
+        pr[q] = (v && d < 9) ? ((d == 4) ? rc.w : pre[(size_t)d * A + c[q]]) + (cd[q] >> 4) : 0u;
+        T[q] = rc.y;
+        m[q] = rc.z;
+        base[q] = rc.x;
     }
 #pragma unroll
     for (int q = 0; q < SC_PER; ++q) {
@@ -659,8 +665,7 @@ __global__ void k_offsets_sorted(const uint32_t *__restrict__ key, const int *d_
 // per cell: its particles T (from the sorted list's offsets), the A+A->0 draws, the kill mask
 // and the survivors, summed per 256-cell tile
 __global__ __launch_bounds__(256) void k_rs_cells(const uint32_t *__restrict__ off_sorted, int A, int step,
-                                                  PartConsts p, uint32_t *__restrict__ tot,
-                                                  uint32_t *__restrict__ kmask, uint32_t *__restrict__ surv,
+                                                  PartConsts p, uint4 *__restrict__ rec, uint32_t *__restrict__ surv,
                                                   uint32_t *__restrict__ tile_s, uint32_t ntiles, uint32_t per) {
     __shared__ uint32_t sm[RS_WAVES];
     __shared__ uint32_t sbeg[RS_WAVES][64], spairs[RS_WAVES][64], skill[RS_WAVES][64], smask[RS_WAVES][64];
@@ -674,8 +679,7 @@ __global__ __launch_bounds__(256) void k_rs_cells(const uint32_t *__restrict__ o
                     smask[wave], kills, mask);
     const uint32_t S = T - 2 * kills;
     if (c < A) {
-        tot[c] = T;
-        kmask[c] = mask;
+        rec[c] = make_uint4(0u, T, mask, 0u);
         surv[c] = S;
     }
     uint32_t total;
@@ -687,8 +691,8 @@ __global__ __launch_bounds__(256) void k_rs_cells(const uint32_t *__restrict__ o
 // a particle's rank in its cell is its index less the cell's first index
 __global__ __launch_bounds__(SC_BLOCK) void k_rs_compact(
     const uint32_t *__restrict__ key, const float *__restrict__ xs, const float *__restrict__ ys,
-    const uint32_t *__restrict__ ids, const uint32_t *__restrict__ off_sorted, const uint32_t *__restrict__ tot,
-    const uint32_t *__restrict__ kmask, const uint32_t *__restrict__ off_new, const int *d_n, int step, PartConsts p,
+    const uint32_t *__restrict__ ids, const uint32_t *__restrict__ off_sorted, const uint4 *__restrict__ rec,
+    const int *d_n, int step, PartConsts p,
     float *__restrict__ xo, float *__restrict__ yo, uint32_t *__restrict__ ido, uint32_t *__restrict__ keyo,
     uint32_t cap, uint32_t per) {
     const uint32_t n = (uint32_t)*d_n;
@@ -712,9 +716,10 @@ __global__ __launch_bounds__(SC_BLOCK) void k_rs_compact(
         const uint32_t i = b0 + q * SC_BLOCK + threadIdx.x;
         const bool v = i < n && c[q] < A;
         s0[q] = v ? off_sorted[c[q]] : i;
-        T[q] = v ? tot[c[q]] : 0u;
-        m[q] = v ? kmask[c[q]] : 0u;
-        base[q] = v ? off_new[c[q]] : 0u;
+        const uint4 rc = v ? rec[c[q]] : make_uint4(0u, 0u, 0u, 0u);
+        T[q] = rc.y;
+        m[q] = rc.z;
+        base[q] = rc.x;
     }
 #pragma unroll
     for (int q = 0; q < SC_PER; ++q) {
@@ -827,14 +832,14 @@ hipError_t particle_step_sorted(const ParticleStepArgs &a, int step, const Sorte
                        b.id, b.off, b.n_cur, A, step, p, b.x2, b.y2, b.code, b.cnt, b.far_cnt, (FarEntry *)b.far_list,
                        b.far_in);
     const uint32_t tper = (uint32_t)(tiles + 7) / 8;
-    hipLaunchKernelGGL(k_cs_cells, dim3(8 * tper), blk, 0, s, b.cnt, b.far_in, A, a.side, step, p, b.pre, b.tot,
-                       b.kmask, b.surv, b.tile_sum, b.far_cnt_next, (uint32_t)tiles, tper);
+    hipLaunchKernelGGL(k_cs_cells, dim3(8 * tper), blk, 0, s, b.cnt, b.far_in, A, a.side, step, p, b.pre, b.rec,
+                       b.surv, b.tile_sum, b.far_cnt_next, (uint32_t)tiles, tper);
     hipLaunchKernelGGL(k_coh_scan, dim3(1), dim3(1024), 0, s, b.tile_sum, tiles, b.n_cur, b.n_next, b.processed);
-    hipLaunchKernelGGL(k_coh_offsets, dim3(tiles), blk, 0, s, b.surv, b.tile_sum, A, b.off_new);
+    hipLaunchKernelGGL(k_coh_offsets, dim3(tiles), blk, 0, s, b.surv, b.tile_sum, A, b.off_new, b.rec);
     const int sblocks = std::max((b.n_ub + SC_SPAN - 1) / SC_SPAN, 1);
     const uint32_t sper = (uint32_t)(sblocks + 7) / 8;
     hipLaunchKernelGGL(k_cs_scatter, dim3(8 * sper), dim3(SC_BLOCK), 0, s, b.x2, b.y2, b.id, b.code, b.cnt, b.pre,
-                       b.tot, b.kmask, b.off_new, b.far_cnt, (const FarEntry *)b.far_list, b.n_cur, A, a.side, step, p,
+                       b.rec, b.far_cnt, (const FarEntry *)b.far_list, b.n_cur, A, a.side, step, p,
                        b.xo, b.yo, b.ido, (uint32_t)b.capacity, sper);
     return hipGetLastError();
 }
@@ -873,15 +878,15 @@ hipError_t particle_step(const ParticleStepArgs &a, int step, ParticleBuffers &b
     const int tiles = (A + 255) / 256;
     const uint32_t tper = (uint32_t)(tiles + 7) / 8;
     hipLaunchKernelGGL(k_offsets_sorted, dim3(n_ub / 256 + 1), blk, 0, s, b.key2, b.d_n, A, b.off_sorted);
-    hipLaunchKernelGGL(k_rs_cells, dim3(8 * tper), blk, 0, s, b.off_sorted, A, step, p, b.tot, b.kmask, b.surv,
+    hipLaunchKernelGGL(k_rs_cells, dim3(8 * tper), blk, 0, s, b.off_sorted, A, step, p, b.rec, b.surv,
                        b.tile_s, (uint32_t)tiles, tper);
     hipLaunchKernelGGL(k_coh_scan, dim3(1), dim3(1024), 0, s, b.tile_s, tiles, b.d_n, b.d_n2, b.processed);
-    hipLaunchKernelGGL(k_coh_offsets, dim3(tiles), blk, 0, s, b.surv, b.tile_s, A, b.off);
+    hipLaunchKernelGGL(k_coh_offsets, dim3(tiles), blk, 0, s, b.surv, b.tile_s, A, b.off, b.rec);
     // 4. stable compaction back into the primary buffers
     const int cblocks = std::max((n_ub + SC_SPAN - 1) / SC_SPAN, 1);
     const uint32_t cper = (uint32_t)(cblocks + 7) / 8;
     hipLaunchKernelGGL(k_rs_compact, dim3(8 * cper), dim3(SC_BLOCK), 0, s, b.key2, b.x2, b.y2, b.id2, b.off_sorted,
-                       b.tot, b.kmask, b.off, b.d_n, step, p, b.x, b.y, b.id, b.key_out, (uint32_t)b.capacity, cper);
+                       b.rec, b.d_n, step, p, b.x, b.y, b.id, b.key_out, (uint32_t)b.capacity, cper);
     return hipGetLastError();
 }
 

@@ -19,7 +19,8 @@ struct ParticleStepArgs {
 struct ParticleBuffers {
     float *x, *y, *x2, *y2;
     uint32_t *id, *id2, *key, *key2, *key_out, *off, *hist, *digit_total;
-    uint32_t *tot, *kmask, *surv;  // per cell
+    uint4 *rec;                    // per cell: survivors' offset, particles, kill mask, 0
+    uint32_t *surv;                // per cell
     uint32_t *tile_s;              // per 256-cell tile
     uint32_t *off_sorted;          // offsets of the sorted list before the annihilation
     int *d_n, *d_n2;  // live count (device) and its next value
@@ -36,7 +37,8 @@ struct SortedStepBuffers {
     float *x2, *y2;           // moved positions, old order
     uint32_t *code;           // move class and rank (or old cell) per particle
     uint32_t *cnt, *pre;      // [9][A]
-    uint32_t *tot, *kmask, *surv, *tile_sum, *off_new;
+    uint4 *rec;               // per new cell: survivors' offset, particles, kill mask, particles that stay
+    uint32_t *surv, *tile_sum, *off_new;
     uint32_t *far_cnt, *far_cnt_next, *far_in;
     void *far_list;           // far_entry_bytes() per entry, capacity entries
     const int *n_cur;
