@@ -217,10 +217,17 @@ def rdd_extra(reps=3):
             fw.append(e[0].elapsed_time(e[1]))
             bw.append(e[1].elapsed_time(e[2]))
     f, b = min(fw), min(bw)
+    # four fused steps per launch: u, v, D read once with a 4-cell halo ((64+8)x(16+8) per
+    # 64x16 tile) and u, v written once per four steps
+    moved = (12.0 * (72 * 24) / (64 * 16) + 8.0) / 4
     return {"workload": "Schnakenberg RDD 1024x1024, 2000 fp32 Euler steps, checkpoints every 100",
             "forward_ms": f, "backward_ms": b, "adjoint_over_forward": b / f, "grad_per_s": 1e3 / (f + b),
             "forward_cell_steps_per_s": W * H * steps / (f * 1e-3),
-            "forward_algorithmic_GBps": 20.0 * W * H * steps / (f * 1e-3) / 1e9}
+            "forward_algorithmic_GBps": 20.0 * W * H * steps / (f * 1e-3) / 1e9,
+            "forward_algorithmic_note": "20 B per cell-step: u, v, D read and u, v written by an unfused step",
+            "forward_steps_per_launch": 4,
+            "forward_moved_bytes_per_cell_step": moved,
+            "forward_moved_GBps": moved * W * H * steps / (f * 1e-3) / 1e9}
 
 
 def det_diffusion_extra(side=8192, reps=20):
@@ -451,7 +458,17 @@ def particles_extra(n=10_000_000, side=1024, steps=1000):
     ps = ParticleSystem(side, D=1.0, dt=0.01, vortex_amp=0.1, p_annihilate=0.05, seed=17, capacity=n)
     out = {"workload": "10^7 particles, 1024x1024 periodic cells, %d steps" % steps,
            "note": "the first step after set() sorts an unsorted list (radix step); later steps run on the "
-                   "list sorted by cell (counts of the moves to the 3x3 neighbours and one scatter)"}
+                   "list sorted by cell (counts of the moves to the 3x3 neighbours and one scatter); an "
+                   "untimed step of a small list first loads every kernel's code",
+           "algorithmic_bytes": {"sorted_step_per_particle": 52, "sorted_step_per_cell": 160,
+                                 "radix_pass_per_particle": 32, "note": "sorted step: move reads x, y, id "
+                                 "and writes x2, y2, code (24 B), the scatter reads x2, y2, id, code and "
+                                 "writes x, y, id (28 B); per cell: the move counts (36 B), the cell pass "
+                                 "(cnt 36 B in, pre 32 + record 16 + survivors 4 B out), the offsets and "
+                                 "records (16 B), the scatter's record and pre reads (~20 B)"}}
+    ps.set(x[:200000], y[:200000], ids[:200000])
+    ps.step(0, 3)
+    ps.count()
     # (first step timed, k steps from it): from the unsorted list; steps 1..10 from the list the
     # first step sorted (that step untimed); the whole decaying run
     for label, first, k in (("first_10_steps", 0, 10), ("sorted_10_steps", 1, 10), ("full_run", 0, steps)):
@@ -467,6 +484,11 @@ def particles_extra(n=10_000_000, side=1024, steps=1000):
         done = ps.processed() - p0
         out[label] = {"particle_steps_per_s": done / wall, "particle_steps": done, "wall_s": wall,
                       "live_at_end": n_end}
+        if label == "sorted_10_steps":
+            # algorithmic HBM bytes of the sorted steps / wall time, against the 8 TB/s peak
+            b = 52.0 * done + 160.0 * side * side * k
+            out[label]["algorithmic_GBps"] = b / wall / 1e9
+            out[label]["hbm_frac"] = b / wall / 8e12
     return out
 
 

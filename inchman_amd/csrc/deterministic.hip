@@ -443,12 +443,12 @@ __global__ __launch_bounds__(256) void k_det_aqss(float *__restrict__ state, int
 //   u' = u + dt*( cu*(sum_nbr D_n u_n - 4 D u) + k1 - k4 u + k3 u^2 v )
 //   v' = v + dt*( cv*(sum_nbr D_n v_n - 4 D v) + k2 - k3 u^2 v ),  c_s = D0_s/h^2.
 // A 256-thread block owns a 64x16 tile and stages u, v, D of the tile plus an NS-cell
-// halo in LDS; NS = 2 advances two steps per launch (temporal blocking), recomputing
-// the one-cell ring of the intermediate step, so each cell's inputs are read from HBM
-// once per two steps.  The halo outside the grid is the reference's mirror image
-// (g -> -g, g -> 2(W-1)-g), and the neighbour sum is taken as (l + r) + (t + b), which
-// is invariant under both reflections, so a mirrored halo cell evolves bit-identically
-// to the cell it mirrors and NS = 2 equals two NS = 1 launches bit for bit.
+// halo in LDS; NS (1..4) steps per launch (temporal blocking) recompute the shrinking
+// rings of the intermediate steps, so each cell's inputs are read from HBM once per NS
+// steps.  The halo outside the grid is the reference's mirror image (g -> -g,
+// g -> 2(W-1)-g), and the neighbour sum is taken as (l + r) + (t + b), which is invariant
+// under both reflections, so a mirrored halo cell evolves bit-identically to the cell it
+// mirrors and NS steps per launch equal NS one-step launches bit for bit (sides >= NS + 1).
 struct RddK {
     float dt, cu, cv, k1, k2, k3, k4;
 };
@@ -833,13 +833,10 @@ hipError_t launch_rdd_step(const float *u, const float *v, const float *D, float
                            hipEvent_t e1) {
     const RddK k = make_k(c);
     const dim3 g = rdd_grid(W, H), b(256);
-    if (ns == 2) {
-        if (e0) hipExtLaunchKernelGGL(k_rdd_step<2>, g, b, 0, s, e0, e1, 0, u, v, D, umid, vmid, un, vn, W, H, k, c.kdev);
-        else hipLaunchKernelGGL(k_rdd_step<2>, g, b, 0, s, u, v, D, umid, vmid, un, vn, W, H, k, c.kdev);
-    } else {
-        if (e0) hipExtLaunchKernelGGL(k_rdd_step<1>, g, b, 0, s, e0, e1, 0, u, v, D, umid, vmid, un, vn, W, H, k, c.kdev);
-        else hipLaunchKernelGGL(k_rdd_step<1>, g, b, 0, s, u, v, D, umid, vmid, un, vn, W, H, k, c.kdev);
-    }
+    if (ns < 1 || ns > 4 || (ns > 2 && umid)) return hipErrorInvalidValue;  // the mid state only with 2 steps
+    auto kern = (ns == 4) ? k_rdd_step<4> : (ns == 3) ? k_rdd_step<3> : (ns == 2) ? k_rdd_step<2> : k_rdd_step<1>;
+    if (e0) hipExtLaunchKernelGGL(kern, g, b, 0, s, e0, e1, 0, u, v, D, umid, vmid, un, vn, W, H, k, c.kdev);
+    else hipLaunchKernelGGL(kern, g, b, 0, s, u, v, D, umid, vmid, un, vn, W, H, k, c.kdev);
     return hipGetLastError();
 }
 

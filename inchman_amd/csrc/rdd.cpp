@@ -41,7 +41,7 @@ struct gpgmp_rdd {
     int ntiles = 0;
     bool saved = false;
     uint64_t generation = 0;  // saving forward runs so far; the last one is what backward differentiates
-    int ns = 2;  // steps per launch (temporal blocking)
+    int ns = 4;  // forward steps per launch (temporal blocking); the adjoint takes up to 2
     RddConsts c{};
     DevBuf<float> kdev;  // rates set on the device (gpgmp_rdd_set_rates_device)
     ~gpgmp_rdd() {
@@ -129,7 +129,8 @@ int gpgmp_rdd_create(const gpgmp_rdd_params *p, int32_t steps, int32_t ckpt_ever
         r->gk_tiles.alloc((size_t)r->ntiles * 4);
         r->c = rdd_consts(*p);
         r->c.kdev = nullptr;
-        r->ns = (p->width >= 3 && p->height >= 3) ? 2 : 1;
+        const int side = std::min(p->width, p->height);
+        r->ns = side >= 5 ? 4 : (side >= 3 ? 2 : 1);
         *out = r.release();
     });
 }
@@ -167,9 +168,9 @@ int gpgmp_rdd_generation(gpgmp_rdd *r, uint64_t *generation) {
 
 int gpgmp_rdd_set_steps_per_launch(gpgmp_rdd *r, int32_t ns) {
     return guarded_det([&] {
-        GPGMP_REQUIRE(r && (ns == 1 || ns == 2), GPGMP_E_INVALID, "steps per launch must be 1 or 2");
-        GPGMP_REQUIRE(ns == 1 || (r->p.width >= 3 && r->p.height >= 3), GPGMP_E_INVALID,
-                      "two steps per launch need sides >= 3");
+        GPGMP_REQUIRE(r && ns >= 1 && ns <= 4, GPGMP_E_INVALID, "steps per launch must be 1 .. 4");
+        GPGMP_REQUIRE(std::min(r->p.width, r->p.height) >= ns + 1, GPGMP_E_INVALID,
+                      "ns steps per launch need sides >= ns + 1");
         r->ns = ns;
     });
 }
@@ -189,7 +190,7 @@ int gpgmp_rdd_forward(gpgmp_rdd *r, const float *D, float *u, float *v, int32_t 
                 GPGMP_HIP_CHECK(hipMemcpyAsync(c, bu[cur], A * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
                 GPGMP_HIP_CHECK(hipMemcpyAsync(c + A, bv[cur], A * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
             }
-            // two steps per launch unless a checkpoint or the end falls in between
+            // ns steps per launch unless a checkpoint or the end falls in between
             int k = std::min(r->ns, r->steps - n);
             if (save) k = std::min(k, r->ckpt - n % r->ckpt);
             GPGMP_HIP_CHECK(launch_rdd_step(bu[cur], bv[cur], D, nullptr, nullptr, bu[cur ^ 1], bv[cur ^ 1], W, H, r->c, k,
@@ -229,7 +230,7 @@ int gpgmp_rdd_backward(gpgmp_rdd *r, uint64_t generation, const float *D, float 
             // recompute the states n0+1 .. n1-1 of the segment
             int n = n0;
             while (n + 1 < n1) {
-                if (r->ns == 2 && n + 2 < n1) {
+                if (r->ns >= 2 && n + 2 < n1) {
                     GPGMP_HIP_CHECK(launch_rdd_step(su(n), su(n) + A, D, su(n + 1), su(n + 1) + A, su(n + 2),
                                                     su(n + 2) + A, W, H, r->c, 2, r->stream));
                     n += 2;
@@ -242,7 +243,7 @@ int gpgmp_rdd_backward(gpgmp_rdd *r, uint64_t generation, const float *D, float 
             // walk the segment's steps backwards
             n = n1 - 1;
             while (n >= n0) {
-                if (r->ns == 2 && n - 1 >= n0) {
+                if (r->ns >= 2 && n - 1 >= n0) {
                     GPGMP_HIP_CHECK(launch_rdd_adjoint(blu[cur], blv[cur], su(n), su(n) + A, su(n - 1), su(n - 1) + A, D,
                                                        blu[cur ^ 1], blv[cur ^ 1], gD, r->gk_tiles.get(), W, H, r->c, 2,
                                                        r->stream));

@@ -200,14 +200,18 @@ def _rdd_run(prob, steps, ckpt, ns):
 @pytest.mark.parametrize("W,H,steps,ckpt", [(100, 37, 37, 10), (3, 5, 9, 4), (64, 16, 20, 7), (130, 70, 41, 100),
                                             (257, 19, 3, 1)])
 def test_rdd_two_steps_per_launch_bit_identical(gpu, oracle, W, H, steps, ckpt):
-    """Temporal blocking (two Euler steps per launch, 2-cell LDS halo) gives the same bits
-    as one step per launch, forward and adjoint, on ragged tiles, odd step counts and
+    """Temporal blocking (two or four Euler steps per launch, 2- or 4-cell LDS halo) gives the
+    same bits as one step per launch, forward and adjoint, on ragged tiles, odd step counts and
     checkpoint intervals that split the launch pairs; both agree with the FP64 oracle."""
     prob = configs.rdd_problem(W, H, steps=steps)
     one = _rdd_run(prob, steps, ckpt, 1)
     two = _rdd_run(prob, steps, ckpt, 2)
     for a, b in zip(one, two):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    if min(W, H) >= 5:  # four steps per launch (4-cell halo; 3 where a checkpoint splits a launch)
+        four = _rdd_run(prob, steps, ckpt, 4)
+        for a, b in zip(one, four):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     p = _oracle_params(oracle, prob)
     uw, vw = oracle.rdd_forward(p, prob["D"].astype(np.float64), prob["u0"], prob["v0"], steps)
     assert _rel(two[0], uw) <= 1e-5 and _rel(two[1], vw) <= 1e-5
