@@ -500,6 +500,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3000)
     ap.add_argument("--cpu-single-steps", type=int, default=60)
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--only-extra", default=None,
+                    help="run one extra workload (ensemble, rdd, det_diffusion, particles, individuals, aqss, "
+                         "rk4_moments) and print its JSON line; for profiling")
+    ap.add_argument("--particle-steps", type=int, default=1000, help="steps of the particle extra's full run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver runs on a single MI355X "
@@ -511,6 +515,12 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.only_extra:
+        fns = {"ensemble": ensemble_extra, "rdd": rdd_extra, "det_diffusion": det_diffusion_extra,
+               "particles": lambda: particles_extra(steps=args.particle_steps), "individuals": individuals_extra,
+               "aqss": aqss_extra, "rk4_moments": rk4_moments_extra}
+        print(json.dumps({args.only_extra: fns[args.only_extra]()}), flush=True)
+        return
     m = configs.schnakenberg_model(256)
     st = configs.schnakenberg_state(256)
     seed0 = 5
@@ -594,7 +604,8 @@ def main():
         extra["gpu_on_cpu_sample_cell_steps_per_s"] = cells * args.cpu_steps / w2
     if not args.no_extra:
         for key, fn in (("ensemble", ensemble_extra), ("rdd", rdd_extra), ("det_diffusion", det_diffusion_extra),
-                        ("particles", particles_extra), ("individuals", individuals_extra),
+                        ("particles", lambda: particles_extra(steps=args.particle_steps)),
+                        ("individuals", individuals_extra),
                         ("aqss", aqss_extra), ("rk4_moments", rk4_moments_extra)):
             extra[key] = fn()
             progress(key + " done")
