@@ -40,15 +40,16 @@ struct gpgmp_particles {
     gpgmp_particle_params p{};
     int capacity = 0;
     hipStream_t stream = nullptr;
-    // current list (x, y, id, key_out with offsets off) and the buffers of the general step
+    // current list (x, y, id; key_out and the offsets `off` are rebuilt for get) and the buffers
+    // of the general step
     DevBuf<float> x, y, x2, y2;
     DevBuf<uint32_t> id, id2, key, key2, key_out, off, hist, digit_total;
     // per-cell work of both steps
-    DevBuf<uint4> rec;
-    DevBuf<uint32_t> surv, tile_cells, off_sorted;
-    // step on the sorted list: the next list (xa, ya, ida, off2) and its per-cell work
+    DevBuf<uint4> rec;  // per cell: survivors before it in its tile, particles, kill mask, stayers
+    DevBuf<uint32_t> tile_cells, off_sorted;
+    // step on the sorted list: the next list (xa, ya, ida) and its per-cell work
     DevBuf<float> xa, ya;
-    DevBuf<uint32_t> ida, off2, cnt, pre, far_cnt, far_in;
+    DevBuf<uint32_t> ida, cnt, pre, far_cnt, far_in;
     DevBuf<uint32_t> code;
     DevBuf<uint8_t> far_list;
     DevBuf<int> dn;  // live count in slot cur, the next count in the other
@@ -71,8 +72,8 @@ struct gpgmp_particles {
         ParticleBuffers b;
         b.x = x.get(); b.y = y.get(); b.x2 = x2.get(); b.y2 = y2.get();
         b.id = id.get(); b.id2 = id2.get(); b.key = key.get(); b.key2 = key2.get(); b.key_out = key_out.get();
-        b.off = off.get(); b.hist = hist.get(); b.digit_total = digit_total.get();
-        b.rec = rec.get(); b.surv = surv.get();
+        b.hist = hist.get(); b.digit_total = digit_total.get();
+        b.rec = rec.get();
         b.tile_s = tile_cells.get(); b.off_sorted = off_sorted.get();
         b.d_n = dn.get() + cur; b.d_n2 = dn.get() + (cur ^ 1);
         b.processed = processed.get();
@@ -82,10 +83,10 @@ struct gpgmp_particles {
     }
     SortedStepBuffers sorted_buffers() {
         SortedStepBuffers b;
-        b.x = x.get(); b.y = y.get(); b.id = id.get(); b.off = off.get();
+        b.x = x.get(); b.y = y.get(); b.id = id.get();
         b.x2 = x2.get(); b.y2 = y2.get(); b.code = code.get();
-        b.cnt = cnt.get(); b.pre = pre.get(); b.rec = rec.get(); b.surv = surv.get();
-        b.tile_sum = tile_cells.get(); b.off_new = off2.get();
+        b.cnt = cnt.get(); b.pre = pre.get(); b.rec = rec.get();
+        b.tile_sum = tile_cells.get();
         b.far_cnt = far_cnt.get() + far_slot; b.far_cnt_next = far_cnt.get() + (far_slot ^ 1);
         b.far_in = far_in.get(); b.far_list = far_list.get();
         b.n_cur = dn.get() + cur; b.n_next = dn.get() + (cur ^ 1);
@@ -179,7 +180,6 @@ int gpgmp_particles_create(const gpgmp_particle_params *p, int32_t capacity, voi
         ps->off.alloc(ncells + 1);
         ps->off_sorted.alloc(ncells + 1);
         ps->rec.alloc(ncells);
-        ps->surv.alloc(ncells);
         ps->tile_cells.alloc((ncells + 255) / 256);
         const int nt = std::max(radix_tiles(capacity), 1);
         ps->hist.alloc((size_t)256 * nt);
@@ -196,7 +196,6 @@ int gpgmp_particles_create(const gpgmp_particle_params *p, int32_t capacity, voi
             ps->ida.alloc(n);
             ps->code.alloc(n);
             ps->far_list.alloc(n * far_entry_bytes());
-            ps->off2.alloc(ncells + 1);
             ps->cnt.alloc(9 * ncells);
             ps->pre.alloc(9 * ncells);
             ps->far_cnt.alloc(2);
@@ -295,7 +294,6 @@ int gpgmp_particles_set_sorted_step(gpgmp_particles *ps, int32_t mode) {
             ps->ida.alloc(n);
             ps->code.alloc(n);
             ps->far_list.alloc(n * far_entry_bytes());
-            ps->off2.alloc(ncells + 1);
             ps->cnt.alloc(9 * ncells);
             ps->pre.alloc(9 * ncells);
             ps->far_cnt.alloc(2);
@@ -343,7 +341,6 @@ int gpgmp_particles_step(gpgmp_particles *ps, int32_t step0, int32_t steps) {
                 std::swap(ps->x, ps->xa);
                 std::swap(ps->y, ps->ya);
                 std::swap(ps->id, ps->ida);
-                std::swap(ps->off, ps->off2);
                 ps->cur ^= 1;
                 ps->far_slot ^= 1;
                 ps->keys_stale = true;

@@ -320,6 +320,13 @@ __device__ __forceinline__ uint32_t moved_cell(int cx, int cy, int d, int L) {
 // blocks; a block past the last item returns at once.
 __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t per) { return (b & 7u) * per + (b >> 3); }
 
+// First index of cell c in the list a step leaves behind: the scanned offset of its 256-cell
+// tile plus the record's offset within the tile.
+__device__ __forceinline__ uint32_t cell_offset(const uint4 *__restrict__ rec, const uint32_t *__restrict__ tile_off,
+                                                int c) {
+    return tile_off[c >> 8] + rec[c].x;
+}
+
 // 1. move the particles of the wave's 64 old cells, 64 consecutive particles per round,
 //    classify each move and rank it: among the particles of one old cell with one move class
 //    the rank is the count carried from the earlier rounds (LDS, per wave) plus the lower
@@ -329,8 +336,9 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t per) { return 
 //    moves are listed with order-free atomics: the list is only counted, never walked in order.
 template <bool POW2>
 __global__ __launch_bounds__(CT) void k_cs_move(const float *__restrict__ x, const float *__restrict__ y,
-                                                const uint32_t *__restrict__ id, const uint32_t *__restrict__ off,
-                                                const int *d_n, int A, int step, PartConsts p,
+                                                const uint32_t *__restrict__ id, const uint4 *__restrict__ rec,
+                                                const uint32_t *__restrict__ tile_off, const int *d_n, int A,
+                                                int step, PartConsts p,
                                                 float *__restrict__ x2, float *__restrict__ y2,
                                                 uint32_t *__restrict__ code, uint32_t *__restrict__ cnt,
                                                 uint32_t *__restrict__ far_cnt, FarEntry *__restrict__ far_list,
@@ -344,7 +352,8 @@ __global__ __launch_bounds__(CT) void k_cs_move(const float *__restrict__ x, con
 #pragma unroll
     for (int d = 0; d < 9; ++d) cr[d][lane] = 0u;
     const uint32_t n = (uint32_t)*d_n;
-    const uint32_t q0 = min(off[cw], n), q1 = min(off[cw + ncell], n);
+    const uint32_t q0 = min(cell_offset(rec, tile_off, cw), n);
+    const uint32_t q1 = (cw + ncell < A) ? min(cell_offset(rec, tile_off, cw + ncell), n) : n;
     const uint64_t lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int side = p.side;
     for (uint32_t b0 = q0; b0 < q1; b0 += 64) {
@@ -465,8 +474,7 @@ __device__ __forceinline__ bool survivor_place(uint32_t r, uint32_t T, uint32_t 
 //    survivors, summed per 256-cell tile (the draws flattened over the wave, wave_pair_draws).
 __global__ __launch_bounds__(256) void k_cs_cells(const uint32_t *__restrict__ cnt, uint32_t *__restrict__ far_in,
                                                   int A, int side, int step, PartConsts p, uint32_t *__restrict__ pre,
-                                                  uint4 *__restrict__ rec, uint32_t *__restrict__ surv,
-                                                  uint32_t *__restrict__ tile_sum,
+                                                  uint4 *__restrict__ rec, uint32_t *__restrict__ tile_sum,
                                                   uint32_t *__restrict__ far_cnt_next, uint32_t ntiles, uint32_t per) {
     __shared__ uint32_t sm[RS_WAVES];
     __shared__ uint32_t sbeg[RS_WAVES][64];   // the wave's first block of each lane's cell
@@ -504,12 +512,9 @@ __global__ __launch_bounds__(256) void k_cs_cells(const uint32_t *__restrict__ c
     wave_pair_draws(T >> 1, tile * 256u + (uint32_t)(wave * 64), step, p, sbeg[wave], spairs[wave], skill[wave],
                     smask[wave], kills, mask);
     const uint32_t S = T - 2 * kills;
-    if (c < A) {
-        rec[c] = make_uint4(0u, T, mask, pre4);  // .x: the survivors' offset, written after the scan
-        surv[c] = S;
-    }
     uint32_t total;
-    (void)block_exclusive_scan(S, sm, total);
+    const uint32_t ex = block_exclusive_scan(S, sm, total);
+    if (c < A) rec[c] = make_uint4(ex, T, mask, pre4);  // .x: the survivors before c in its tile
     if (threadIdx.x == 0) tile_sum[tile] = total;
 }
 
@@ -560,25 +565,7 @@ __global__ __launch_bounds__(1024) void k_coh_scan(uint32_t *__restrict__ tile_s
     }
 }
 
-// 4. survivors' CSR offsets off[c] (and off[A] = the total), also into the cell records
-//    {offset, particles, kill mask, particles that stay} that the scatter reads in one load
-__global__ __launch_bounds__(256) void k_coh_offsets(const uint32_t *__restrict__ surv,
-                                                     const uint32_t *__restrict__ tile_off, int A,
-                                                     uint32_t *__restrict__ off, uint4 *__restrict__ rec) {
-    __shared__ uint32_t sm[RS_WAVES];
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t S = (c < A) ? surv[c] : 0u;
-    uint32_t total;
-    const uint32_t ex = block_exclusive_scan(S, sm, total);
-    const uint32_t o = tile_off[blockIdx.x] + ex;
-    if (c < A) {
-        off[c] = o;
-        rec[c].x = o;
-    }
-    if (c == A - 1) off[A] = o + S;
-}
-
-// 5. every particle to its place in the new order; the killed pairs drop out.  Flat over the
+// 4. every particle to its place in the new order; the killed pairs drop out.  Flat over the
 //    particles: a thread takes SC_PER of them and issues their loads together (the kernel is
 //    bound by memory latency, so each wave keeps several particles' loads in flight), and one
 //    16-byte load of its new cell's record.  A near particle's rank in its new cell is
@@ -589,9 +576,9 @@ constexpr int SC_BLOCK = 256, SC_PER = 4, SC_SPAN = SC_BLOCK * SC_PER;
 __global__ __launch_bounds__(SC_BLOCK) void k_cs_scatter(
     const float *__restrict__ x2, const float *__restrict__ y2, const uint32_t *__restrict__ id,
     const uint32_t *__restrict__ code, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ pre,
-    const uint4 *__restrict__ rec, const uint32_t *__restrict__ far_cnt, const FarEntry *__restrict__ far_list, const int *d_n, int A, int side,
-    int step, PartConsts p, float *__restrict__ xo, float *__restrict__ yo, uint32_t *__restrict__ ido,
-    uint32_t cap, uint32_t per) {
+    const uint4 *__restrict__ rec, const uint32_t *__restrict__ tile_off, const uint32_t *__restrict__ far_cnt,
+    const FarEntry *__restrict__ far_list, const int *d_n, int A, int side, int step, PartConsts p,
+    float *__restrict__ xo, float *__restrict__ yo, uint32_t *__restrict__ ido, uint32_t cap, uint32_t per) {
     const uint32_t n = (uint32_t)*d_n;
     const uint32_t b0 = xcd_item(blockIdx.x, per) * SC_SPAN;
     if (b0 >= n) return;
@@ -618,7 +605,7 @@ __global__ __launch_bounds__(SC_BLOCK) void k_cs_scatter(
         pr[q] = (v && d < 9) ? ((d == 4) ? rc.w : pre[(size_t)d * A + c[q]]) + (cd[q] >> 4) : 0u;
         T[q] = rc.y;
         m[q] = rc.z;
-        base[q] = rc.x;
+        base[q] = v ? tile_off[c[q] >> 8] + rc.x : 0u;
     }
 #pragma unroll
     for (int q = 0; q < SC_PER; ++q) {
@@ -665,7 +652,7 @@ __global__ void k_offsets_sorted(const uint32_t *__restrict__ key, const int *d_
 // per cell: its particles T (from the sorted list's offsets), the A+A->0 draws, the kill mask
 // and the survivors, summed per 256-cell tile
 __global__ __launch_bounds__(256) void k_rs_cells(const uint32_t *__restrict__ off_sorted, int A, int step,
-                                                  PartConsts p, uint4 *__restrict__ rec, uint32_t *__restrict__ surv,
+                                                  PartConsts p, uint4 *__restrict__ rec,
                                                   uint32_t *__restrict__ tile_s, uint32_t ntiles, uint32_t per) {
     __shared__ uint32_t sm[RS_WAVES];
     __shared__ uint32_t sbeg[RS_WAVES][64], spairs[RS_WAVES][64], skill[RS_WAVES][64], smask[RS_WAVES][64];
@@ -678,12 +665,9 @@ __global__ __launch_bounds__(256) void k_rs_cells(const uint32_t *__restrict__ o
     wave_pair_draws(T >> 1, tile * 256u + (uint32_t)(wave * 64), step, p, sbeg[wave], spairs[wave], skill[wave],
                     smask[wave], kills, mask);
     const uint32_t S = T - 2 * kills;
-    if (c < A) {
-        rec[c] = make_uint4(0u, T, mask, 0u);
-        surv[c] = S;
-    }
     uint32_t total;
-    (void)block_exclusive_scan(S, sm, total);
+    const uint32_t ex = block_exclusive_scan(S, sm, total);
+    if (c < A) rec[c] = make_uint4(ex, T, mask, 0u);  // .x: the survivors before c in its tile
     if (threadIdx.x == 0) tile_s[tile] = total;
 }
 
@@ -692,7 +676,7 @@ __global__ __launch_bounds__(256) void k_rs_cells(const uint32_t *__restrict__ o
 __global__ __launch_bounds__(SC_BLOCK) void k_rs_compact(
     const uint32_t *__restrict__ key, const float *__restrict__ xs, const float *__restrict__ ys,
     const uint32_t *__restrict__ ids, const uint32_t *__restrict__ off_sorted, const uint4 *__restrict__ rec,
-    const int *d_n, int step, PartConsts p,
+    const uint32_t *__restrict__ tile_off, const int *d_n, int step, PartConsts p,
     float *__restrict__ xo, float *__restrict__ yo, uint32_t *__restrict__ ido, uint32_t *__restrict__ keyo,
     uint32_t cap, uint32_t per) {
     const uint32_t n = (uint32_t)*d_n;
@@ -719,7 +703,7 @@ __global__ __launch_bounds__(SC_BLOCK) void k_rs_compact(
         const uint4 rc = v ? rec[c[q]] : make_uint4(0u, 0u, 0u, 0u);
         T[q] = rc.y;
         m[q] = rc.z;
-        base[q] = rc.x;
+        base[q] = v ? tile_off[c[q] >> 8] + rc.x : 0u;
     }
 #pragma unroll
     for (int q = 0; q < SC_PER; ++q) {
@@ -828,19 +812,20 @@ hipError_t particle_step_sorted(const ParticleStepArgs &a, int step, const Sorte
     const int A = a.side * a.side;
     const int tiles = (A + CT - 1) / CT;
     const dim3 blk(CT);
+    // the current list's offsets: b.tile_sum (scanned) plus the records' in-tile offsets, from
+    // the step that made the list; both are overwritten only after the move has read them
     hipLaunchKernelGGL((pow2_side(a.side) ? k_cs_move<true> : k_cs_move<false>), dim3(tiles), blk, 0, s, b.x, b.y,
-                       b.id, b.off, b.n_cur, A, step, p, b.x2, b.y2, b.code, b.cnt, b.far_cnt, (FarEntry *)b.far_list,
-                       b.far_in);
+                       b.id, (const uint4 *)b.rec, (const uint32_t *)b.tile_sum, b.n_cur, A, step, p, b.x2, b.y2,
+                       b.code, b.cnt, b.far_cnt, (FarEntry *)b.far_list, b.far_in);
     const uint32_t tper = (uint32_t)(tiles + 7) / 8;
     hipLaunchKernelGGL(k_cs_cells, dim3(8 * tper), blk, 0, s, b.cnt, b.far_in, A, a.side, step, p, b.pre, b.rec,
-                       b.surv, b.tile_sum, b.far_cnt_next, (uint32_t)tiles, tper);
+                       b.tile_sum, b.far_cnt_next, (uint32_t)tiles, tper);
     hipLaunchKernelGGL(k_coh_scan, dim3(1), dim3(1024), 0, s, b.tile_sum, tiles, b.n_cur, b.n_next, b.processed);
-    hipLaunchKernelGGL(k_coh_offsets, dim3(tiles), blk, 0, s, b.surv, b.tile_sum, A, b.off_new, b.rec);
     const int sblocks = std::max((b.n_ub + SC_SPAN - 1) / SC_SPAN, 1);
     const uint32_t sper = (uint32_t)(sblocks + 7) / 8;
     hipLaunchKernelGGL(k_cs_scatter, dim3(8 * sper), dim3(SC_BLOCK), 0, s, b.x2, b.y2, b.id, b.code, b.cnt, b.pre,
-                       b.rec, b.far_cnt, (const FarEntry *)b.far_list, b.n_cur, A, a.side, step, p,
-                       b.xo, b.yo, b.ido, (uint32_t)b.capacity, sper);
+                       (const uint4 *)b.rec, (const uint32_t *)b.tile_sum, b.far_cnt, (const FarEntry *)b.far_list,
+                       b.n_cur, A, a.side, step, p, b.xo, b.yo, b.ido, (uint32_t)b.capacity, sper);
     return hipGetLastError();
 }
 
@@ -873,20 +858,21 @@ hipError_t particle_step(const ParticleStepArgs &a, int step, ParticleBuffers &b
                                       &in_tmp);
     if (e != hipSuccess) return e;
     if (!in_tmp) return hipErrorInvalidValue;
-    // 3. the sorted list's offsets; per cell: A+A->0 draws and survivors, scanned into the
-    //    survivors' (the next step's) offsets; the new count to d_n2
+    // 3. the sorted list's offsets; per cell: A+A->0 draws and survivors with their in-tile
+    //    offsets (the records), the tile sums scanned: a survivor cell's first index (this
+    //    step's output, the next step's input) is tile offset + record offset; the new count
+    //    to d_n2
     const int tiles = (A + 255) / 256;
     const uint32_t tper = (uint32_t)(tiles + 7) / 8;
     hipLaunchKernelGGL(k_offsets_sorted, dim3(n_ub / 256 + 1), blk, 0, s, b.key2, b.d_n, A, b.off_sorted);
-    hipLaunchKernelGGL(k_rs_cells, dim3(8 * tper), blk, 0, s, b.off_sorted, A, step, p, b.rec, b.surv,
-                       b.tile_s, (uint32_t)tiles, tper);
+    hipLaunchKernelGGL(k_rs_cells, dim3(8 * tper), blk, 0, s, b.off_sorted, A, step, p, b.rec, b.tile_s,
+                       (uint32_t)tiles, tper);
     hipLaunchKernelGGL(k_coh_scan, dim3(1), dim3(1024), 0, s, b.tile_s, tiles, b.d_n, b.d_n2, b.processed);
-    hipLaunchKernelGGL(k_coh_offsets, dim3(tiles), blk, 0, s, b.surv, b.tile_s, A, b.off, b.rec);
     // 4. stable compaction back into the primary buffers
     const int cblocks = std::max((n_ub + SC_SPAN - 1) / SC_SPAN, 1);
     const uint32_t cper = (uint32_t)(cblocks + 7) / 8;
     hipLaunchKernelGGL(k_rs_compact, dim3(8 * cper), dim3(SC_BLOCK), 0, s, b.key2, b.x2, b.y2, b.id2, b.off_sorted,
-                       b.rec, b.d_n, step, p, b.x, b.y, b.id, b.key_out, (uint32_t)b.capacity, cper);
+                       b.rec, b.tile_s, b.d_n, step, p, b.x, b.y, b.id, b.key_out, (uint32_t)b.capacity, cper);
     return hipGetLastError();
 }
 

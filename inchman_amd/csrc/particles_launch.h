@@ -18,10 +18,9 @@ struct ParticleStepArgs {
 // Device buffers of a particle system (capacity >= every n it holds).
 struct ParticleBuffers {
     float *x, *y, *x2, *y2;
-    uint32_t *id, *id2, *key, *key2, *key_out, *off, *hist, *digit_total;
-    uint4 *rec;                    // per cell: survivors' offset, particles, kill mask, 0
-    uint32_t *surv;                // per cell
-    uint32_t *tile_s;              // per 256-cell tile
+    uint32_t *id, *id2, *key, *key2, *key_out, *hist, *digit_total;
+    uint4 *rec;                    // per cell: survivors before it in its tile, particles, kill mask, 0
+    uint32_t *tile_s;              // per 256-cell tile: survivors, then their scanned offsets
     uint32_t *off_sorted;          // offsets of the sorted list before the annihilation
     int *d_n, *d_n2;  // live count (device) and its next value
     unsigned long long *processed;  // particle-steps done so far (device)
@@ -29,16 +28,17 @@ struct ParticleBuffers {
     int capacity;     // elements of every per-particle buffer
 };
 
-// Buffers of a step on a list already sorted by cell (the current list x, y, id with its
-// offsets off; the new list goes to xo, yo, ido with offsets off_new; neither stores keys).
+// Buffers of a step on a list already sorted by cell (the current list x, y, id; its cells'
+// first indices are tile_sum[c / 256] + rec[c].x, left by the step that made it; the new
+// list goes to xo, yo, ido with its offsets in the same form; neither stores keys).
 struct SortedStepBuffers {
     const float *x, *y;
-    const uint32_t *id, *off;
+    const uint32_t *id;
     float *x2, *y2;           // moved positions, old order
     uint32_t *code;           // move class and rank (or old cell) per particle
     uint32_t *cnt, *pre;      // [9][A]
-    uint4 *rec;               // per new cell: survivors' offset, particles, kill mask, particles that stay
-    uint32_t *surv, *tile_sum, *off_new;
+    uint4 *rec;               // per new cell: survivors before it in its tile, particles, kill mask, stayers
+    uint32_t *tile_sum;       // per 256-cell tile: survivors, then their scanned offsets
     uint32_t *far_cnt, *far_cnt_next, *far_in;
     void *far_list;           // far_entry_bytes() per entry, capacity entries
     const int *n_cur;
@@ -63,7 +63,7 @@ hipError_t launch_lower_bound(const uint32_t *keys, const int *d_n, int n_host, 
                               hipStream_t s);
 hipError_t launch_left_right(const uint32_t *off, int ncells, uint32_t *left, uint32_t *right, hipStream_t s);
 // A step on a list in any order: move, radix sort, A+A->0, compaction; the survivors' count
-// goes to d_n2 and their offsets to off.
+// goes to d_n2 and their cells' first indices to tile_s (scanned) + rec[c].x.
 hipError_t particle_step(const ParticleStepArgs &a, int step, ParticleBuffers &b, hipStream_t s);
 // keys[i] = floor(y) * side + floor(x) for the live particles (n_ub bounds the grid)
 hipError_t launch_keys_from_pos(const float *x, const float *y, const int *d_n, int n_ub, int side, uint32_t *keys,
