@@ -597,6 +597,21 @@ def main():
              "kernel_launches_timed": {names[i]: int(n[i]) for i in range(5) if n[i]},
              "kernel_timing": "every launch of 4 consecutive steps out of every %d of the timed region"
                                % TIMING_EVERY}
+    # the state handed over from and back to host memory (the boundary's host-buffer calls,
+    # 2 x 65536 int32 each way), timed on their own after the run and added to its wall time:
+    # the rate a caller that keeps the state on the host would see; not the headline value
+    import torch as _t
+    _t.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.synchronizeStateBufferToDevice(st)
+    solver.synchronize()
+    t1 = time.perf_counter()
+    host_state = solver.synchronizeStateBufferFromDevice()
+    t2 = time.perf_counter()
+    extra["pcie_inclusive"] = {"cell_steps_per_s": cells * args.steps / (wall + (t2 - t0)),
+                               "copy_in_s": t1 - t0, "copy_out_s": t2 - t1,
+                               "note": "the timed run's wall time plus one state copy from and one to host memory"}
+    del host_state
     progress("headline done: %.4g cell-steps/s" % value)
     if args.cpu_steps > 0:
         # same sample as the CPU baseline, for a like-for-like ratio
