@@ -75,13 +75,15 @@ def test_particle_steps(gpu, oracle, side, n, steps, p):
 
 @pytest.mark.parametrize("mode", [0, 2])
 @pytest.mark.parametrize("side,n,D,dt,amp,p", [(64, 40000, 1.0, 0.01, 0.1, 0.05), (3, 500, 0.05, 0.1, 0.3, 0.3),
-                                                (16, 20000, 1.0, 0.2, 2.0, 0.1), (32, 60000, 0.02, 0.01, 0.0, 0.5)])
+                                                (16, 20000, 1.0, 0.2, 2.0, 0.1), (32, 60000, 0.02, 0.01, 0.0, 0.5),
+                                                (128, 300, 1.0, 0.01, 0.1, 0.3), (96, 400, 1.0, 0.2, 2.0, 0.2)])
 def test_sorted_list_step(gpu, oracle, mode, side, n, D, dt, amp, p):
     """The step on the sorted list (counts of the moves to the 3x3 neighbours and one scatter
     to the stable-order place, far moves on the far list) and the radix step give the
     oracle's particles, keys and offsets: ordinary moves, the smallest torus (side 3, every
     neighbour wraps), moves of several cells (sigma 0.63 plus drift: many far moves), crowded
-    cells (pairs beyond the 32 of the kill mask)."""
+    cells (pairs beyond the 32 of the kill mask), sparse lists where most cells and most
+    64-cell groups are empty, with and without far moves."""
     rng = np.random.default_rng(side * 7 + n)
     if n > 50000:  # crowded: every particle in 4 cells
         x = (rng.integers(0, 2, n) + rng.uniform(0, 1, n)).astype(np.float32)
