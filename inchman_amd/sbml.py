@@ -7,7 +7,8 @@ the standard XML parser instead of libSBML:
   overrides (:99-119, :285-304);
 * the <im:sbml> annotation: solver, runs, steps, time, outputInterval, the init and events
   scripts with their time stamps, the drift/diffusivity method with its nonLinear and
-  computeMoments flags, the setField and newIndividuals methods (:122-210);
+  computeMoments flags, the setField and newIndividuals methods (:122-210; the latter as
+  the library's per-property rule, new_individual_rule);
 * the <im:model> annotation: grid width/height and the physical width (:213-220);
 * compartments with x, y, width, height (Compartment::setX0/setY0/setWidth/setHeight,
   Compartment.h:82-100; :307-318), species with their diffusion constant, initial amounts per
@@ -24,6 +25,7 @@ with the namespace of DiffusionModel::callInitScript (DiffusionModel.cpp:728-920
 import copy
 import ctypes as C
 import math
+import re
 import xml.etree.ElementTree as ET
 
 import numpy as np
@@ -471,6 +473,135 @@ def drift_diffusivity_fields(im, state, fields=None, current=None, sim_time=0.0,
          None if cur is None else cur.ctypes.data_as(C.POINTER(C.c_float)), C.c_float(sim_time),
          out.ctypes.data_as(C.POINTER(C.c_float)), stream_ptr)
     return out[0], out[1], out[2], out[3]
+
+
+class _Affine:
+    """a + b * RANDOM for the newIndividuals method's expressions (RANDOM the uniform draw)."""
+
+    def __init__(self, a, b=0.0):
+        self.a, self.b = float(a), float(b)
+
+    @staticmethod
+    def of(v):
+        return v if isinstance(v, _Affine) else _Affine(v)
+
+    def __add__(self, o):
+        o = _Affine.of(o)
+        return _Affine(self.a + o.a, self.b + o.b)
+
+    __radd__ = __add__
+
+    def __sub__(self, o):
+        o = _Affine.of(o)
+        return _Affine(self.a - o.a, self.b - o.b)
+
+    def __rsub__(self, o):
+        return _Affine.of(o) - self
+
+    def __neg__(self):
+        return _Affine(-self.a, -self.b)
+
+    def __pos__(self):
+        return self
+
+    def __mul__(self, o):
+        o = _Affine.of(o)
+        if self.b and o.b:
+            raise NotImplementedError("a newIndividuals expression with RANDOM * RANDOM")
+        return _Affine(self.a * o.a, self.a * o.b + self.b * o.a)
+
+    __rmul__ = __mul__
+
+    def __truediv__(self, o):
+        o = _Affine.of(o)
+        if o.b:
+            raise NotImplementedError("a newIndividuals expression dividing by RANDOM")
+        return _Affine(self.a / o.a, self.b / o.a)
+
+    def __rtruediv__(self, o):
+        if self.b:
+            raise NotImplementedError("a newIndividuals expression dividing by RANDOM")
+        return _Affine(float(o) / self.a)
+
+
+class _CInt(int):
+    """An integer literal: division of two of them truncates toward zero, as in C."""
+
+    def __truediv__(self, o):
+        if isinstance(o, _CInt):
+            q = abs(int(self)) // abs(int(o))
+            return _CInt(q if (self < 0) == (o < 0) else -q)
+        return _Affine.of(float(self)) / o
+
+    def __rtruediv__(self, o):
+        return _Affine.of(o) / float(self)
+
+    def __neg__(self):
+        return _CInt(-int(self))
+
+    def _int_op(self, o, f):
+        if isinstance(o, int) and not isinstance(o, bool):
+            return _CInt(f(int(self), int(o)))
+        return f(_Affine.of(float(self)), o)
+
+    def __add__(self, o):
+        return self._int_op(o, lambda p, q: p + q)
+
+    def __radd__(self, o):
+        return self._int_op(o, lambda p, q: q + p)
+
+    def __sub__(self, o):
+        return self._int_op(o, lambda p, q: p - q)
+
+    def __rsub__(self, o):
+        return self._int_op(o, lambda p, q: q - p)
+
+    def __mul__(self, o):
+        return self._int_op(o, lambda p, q: p * q)
+
+    def __rmul__(self, o):
+        return self._int_op(o, lambda p, q: q * p)
+
+
+_NEW_IND_PROPS = ("DiffusivityX", "DiffusivityY", "DriftX", "DriftY")
+
+
+def new_individual_rule(im):
+    """The newIndividuals method (DiffusionModel::getNewIndividualsMethod, pasted into
+    updateNewParticles, IndividualSolver.cl:1284 and IndividualSolver.cpp:264-305) as the
+    library's rule: ((mode per property), (a), (b)) with mode 0 keep, 1 the constant a,
+    2 a + b * RANDOM, for the properties DiffusivityX, DiffusivityY, DriftX, DriftY; None for
+    an empty method.  Statements `All-><property> = <expr>;` with parameters, numbers (C
+    literals, integer division truncating) and RANDOM entering affinely; anything else
+    raises NotImplementedError (the reference pastes arbitrary C there)."""
+    text = re.sub(r"//[^\n]*", "", im.new_individuals_method or "")
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    modes, a, b = [0] * 4, [0.0] * 4, [0.0] * 4
+    env = {k: float(v) for k, v in im.model.parameters.items()}
+    env["RANDOM"] = _Affine(0.0, 1.0)
+    any_stmt = False
+    for stmt in text.split(";"):
+        stmt = stmt.strip()
+        if not stmt:
+            continue
+        m = re.fullmatch(r"All\s*->\s*(\w+)\s*=\s*(.+)", stmt, re.S)
+        if not m or m.group(1) not in _NEW_IND_PROPS:
+            raise NotImplementedError("newIndividuals statement not supported: %r" % stmt)
+        expr = m.group(2)
+        if re.search(r"[^\w\s.+\-*/()]", expr):
+            raise NotImplementedError("newIndividuals expression not supported: %r" % expr)
+        # C literals: float suffixes dropped; plain integers keep C's integer division
+        lit = r"(?<![\w.])(\d+(?:\.\d*)?[eE][+-]?\d+|\d+\.\d*|\.\d+|\d+)[fF]?(?![\w.])"
+        names = set(re.findall(r"[A-Za-z_]\w*", re.sub(lit, " ", expr)))
+        expr = re.sub(lit, lambda t: t.group(1) if re.search(r"[.eE]", t.group(1)) else "_CInt(%s)" % t.group(1), expr)
+        unknown = names - set(env)
+        if unknown:
+            raise NotImplementedError("newIndividuals expression names %s" % sorted(unknown))
+        v = _Affine.of(eval(expr, {"__builtins__": {}, "_CInt": _CInt}, env))  # noqa: S307 (model text)
+        k = _NEW_IND_PROPS.index(m.group(1))
+        modes[k], a[k], b[k] = (2 if v.b else 1), v.a, v.b
+        any_stmt = True
+    return (tuple(modes), tuple(a), tuple(b)) if any_stmt else None
 
 
 def build(im, run_init_script=True):

@@ -128,3 +128,40 @@ def test_reference_examples_parse():
     assert laws["ABReaction.xml"] == ["k1 * SpeciesA * (SpeciesA - 1)", "k2 * SpeciesA * SpeciesB", "k3", "k4"]
     assert laws["Calcium.xml"][0] == "beta * 0.001 * na * pow(PhysicalCellWidth * omega, 3)"
     assert laws["Oregonator.xml"][1] == "k2 / Omega * X * Y"
+
+
+def test_new_individuals_method_as_rule():
+    """The newIndividuals method text (pasted into updateNewParticles, IndividualSolver.cl:1284)
+    as the library's per-property rule: RANDOM affine terms, constants, parameters, C literals
+    and integer division; other code is refused."""
+    im = sbml.load(os.path.join(GOLDEN, "inchman_homogeneous.xml"))
+    im.model.setParameter("k", 0.25)
+    cases = {
+        "": None,
+        "/* nothing */ // here": None,
+        "All->DiffusivityX = RANDOM;\n All->DriftY = 0.5 + 2*RANDOM; // comment": (
+            (2, 0, 0, 2), (0.0, 0.0, 0.0, 0.5), (1.0, 0.0, 0.0, 2.0)),
+        "All->DriftX = (1+1)/4 + k*RANDOM*3;": ((0, 0, 2, 0), (0.0, 0.0, 0.0, 0.0), (0.0, 0.0, 0.75, 0.0)),
+        "All->DiffusivityY = 3/2; All->DriftX = -7/2 + 2.f;": (
+            (0, 1, 1, 0), (0.0, 1.0, -1.0, 0.0), (0.0, 0.0, 0.0, 0.0)),
+        "All->DriftX = 1.5e-1f - RANDOM/4;": ((0, 0, 2, 0), (0.0, 0.0, 0.15, 0.0), (0.0, 0.0, -0.25, 0.0)),
+    }
+    for text, want in cases.items():
+        im.new_individuals_method = text
+        assert sbml.new_individual_rule(im) == want, text
+    for bad in ["All->Foo = 1;", "All->DriftX = RANDOM*RANDOM;", "All->DriftX = sqrt(2);", "B->DriftX = 1;",
+                "All->DriftX = q;", "All->DriftX = 1/RANDOM;"]:
+        im.new_individuals_method = bad
+        with pytest.raises(NotImplementedError):
+            sbml.new_individual_rule(im)
+
+
+@pytest.mark.skipif(not os.path.isdir(EXAMPLES), reason="reference examples not present")
+def test_reference_new_individuals_methods():
+    """Every example model's newIndividuals method maps to a rule (the reference's own
+    ABReactionIndividualNew draws every property from RANDOM)."""
+    for n in sorted(f for f in os.listdir(EXAMPLES) if f.endswith(".xml")):
+        im = sbml.load(os.path.join(EXAMPLES, n))
+        rule = sbml.new_individual_rule(im)
+        if n == "ABReactionIndividualNew.xml":
+            assert rule == ((2, 2, 2, 2), (0.0,) * 4, (1.0,) * 4)
