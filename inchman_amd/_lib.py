@@ -117,6 +117,7 @@ SIGNATURES = {
     "gpgmp_model_law_rhs": [_P, _I, _I, _pF, _I, _I, _U, _pF],
     "gpgmp_model_eval_drift_diffusivity": [_P, C.c_char_p, C.POINTER(C.c_char_p), _I, _pF, _pI, _pF, _F, _pF, _P],
     "gpgmp_model_method_host": [_P, C.c_char_p, C.POINTER(C.c_char_p), _I, _pF, _pI, _pF, _F, _pF],
+    "gpgmp_model_method_dependencies": [_P, C.c_char_p, C.POINTER(C.c_char_p), _I, C.POINTER(C.c_uint32)],
     "gpgmp_solver_create": [_P, _P, _U, C.POINTER(_P)],
     "gpgmp_solver_destroy": [_P],
     "gpgmp_solver_set_state": [_P, _pI],

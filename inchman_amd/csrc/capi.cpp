@@ -232,6 +232,17 @@ int gpgmp_model_eval_drift_diffusivity(const gpgmp_model *m, const char *method,
     });
 }
 
+int gpgmp_model_method_dependencies(const gpgmp_model *m, const char *method, const char *const *field_names,
+                                    int32_t nfields, uint32_t *deps) {
+    return guarded([&] {
+        GPGMP_REQUIRE(m && method && deps, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(nfields >= 0 && (nfields == 0 || field_names), GPGMP_E_INVALID, "fields need their names");
+        LawEnv env = law_env(m->m, false);
+        for (int k = 0; k < nfields; ++k) env.fields.push_back(field_names[k]);
+        *deps = method_dependencies(compile_method(method, env), (int)env.species.size());
+    });
+}
+
 int gpgmp_model_method_host(const gpgmp_model *m, const char *method, const char *const *field_names,
                             int32_t nfields, const float *fields, const int32_t *state, const float *current,
                             float sim_time, float *out) {

@@ -644,6 +644,25 @@ std::vector<LawProgram> compile_method(const std::string &text, const LawEnv &en
     return out;
 }
 
+uint32_t method_dependencies(const std::vector<LawProgram> &progs, int S) {
+    uint32_t deps = 0;
+    for (size_t i = 0; i < progs.size(); ++i) {
+        const LawProgram &p = progs[i];
+        // a field the method does not assign keeps its value: that program reads only itself
+        const uint32_t f = (uint32_t)(i / (size_t)S), s = (uint32_t)(i % (size_t)S);
+        if (p.code.size() == 2 && p.code[0] == (LO_CUR | ((s * 4 + f) << 8)) && p.code[1] == LO_END) continue;
+        for (size_t pc = 0; pc < p.code.size(); ++pc) {
+            const uint32_t op = p.code[pc] & 0xffu;
+            if (op == LO_END) break;
+            if (op == LO_CONST) pc += 2;  // the double's two words
+            if (op == LO_STATE || op == LO_CDX || op == LO_CDY) deps |= kMethodReadsState;
+            if (op == LO_SIMTIME) deps |= kMethodReadsTime;
+            if (op == LO_CUR) deps |= kMethodReadsFields;
+        }
+    }
+    return deps;
+}
+
 bool mass_action_shape(const std::string &text, const LawEnv &env, MassAction &ma) {
     if (env.deterministic) return false;
     Parser p(text, env);

@@ -45,6 +45,13 @@ bool mass_action_shape(const std::string &text, const LawEnv &env, MassAction &m
 // leaves in species s's struct when run for s (`All` = s), or its current value when the
 // method does not assign it (Solver.cpp:659-777 saves the function species' fields only).
 std::vector<LawProgram> compile_method(const std::string &text, const LawEnv &env);
+// What a compiled method reads besides the cell's position, the constants and the field
+// parameters: the counts (X->State, central differences), the clock, the current fields.
+// A method that reads none of them gives the same fields every time it is evaluated (a field
+// it does not assign keeps its value, which is not a dependency).  progs: compile_method's,
+// for S species.
+constexpr uint32_t kMethodReadsState = 1, kMethodReadsTime = 2, kMethodReadsFields = 4;
+uint32_t method_dependencies(const std::vector<LawProgram> &progs, int S);
 
 // The names a model's laws may use, and reaction i's law text (custom or generated).
 LawEnv law_env(const Model &m, bool deterministic);

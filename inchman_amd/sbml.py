@@ -475,6 +475,38 @@ def drift_diffusivity_fields(im, state, fields=None, current=None, sim_time=0.0,
     return out[0], out[1], out[2], out[3]
 
 
+def method_dependencies(im):
+    """What the drift/diffusivity method reads besides the cell's position, the constants and
+    the field parameters (gpgmp_model_method_dependencies): bit 0 the counts, bit 1
+    PhysicalSimTime, bit 2 the current fields."""
+    from ._lib import call
+
+    tmp = copy.copy(im.model)
+    tmp.solver = STOCHASTIC_HOMOGENEOUS
+    tmp.fields = None
+    cm = tmp.compile()
+    names = list(im.field_parameters)
+    cnames = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+    deps = C.c_uint32()
+    call("gpgmp_model_method_dependencies", cm.handle, im.drift_diffusivity_method.encode(), cnames, len(names),
+         C.byref(deps))
+    return deps.value
+
+
+def check_nonlinear(im):
+    """A non-linear model (fields recomputed after every sweep, InhomogeneousSolver.cpp:304-336,
+    384-417) runs when its method reads none of what a sweep changes -- the counts, the clock,
+    the current fields: re-evaluating it then reproduces the first fields and time steps.
+    Otherwise raises NotImplementedError with what it reads."""
+    if not im.nonlinear_diffusivity:
+        return
+    deps = method_dependencies(im)
+    if deps:
+        what = [n for b, n in ((1, "the counts"), (2, "PhysicalSimTime"), (4, "the current fields")) if deps & b]
+        raise NotImplementedError("non-linear diffusivity whose method reads %s (fields recomputed after every "
+                                  "sweep) is not supported" % " and ".join(what))
+
+
 class _Affine:
     """a + b * RANDOM for the newIndividuals method's expressions (RANDOM the uniform draw)."""
 
@@ -613,8 +645,7 @@ def build(im, run_init_script=True):
     m = im.model
     if m.solver in (STOCHASTIC_INHOMOGENEOUS, STOCHASTIC_INHOMOGENEOUS_FPE, DETERMINISTIC_INHOMOGENEOUS_RK4,
                     DETERMINISTIC_INHOMOGENEOUS_AQSS):
-        if im.nonlinear_diffusivity:
-            raise NotImplementedError("non-linear diffusivity (fields recomputed every sweep) is not supported")
+        check_nonlinear(im)
         S, H, W = m.numSpecies(), m.height, m.width
         cur = np.zeros((4, S, H, W), np.float32)
         for s, sp in enumerate(m.species):  # the species' constant as the starting value

@@ -168,6 +168,14 @@ int gpgmp_model_eval_drift_diffusivity(const gpgmp_model *m, const char *method,
 int gpgmp_model_method_host(const gpgmp_model *m, const char *method, const char *const *field_names,
                             int32_t nfields, const float *fields, const int32_t *state, const float *current,
                             float sim_time, float *out);
+/* What the drift/diffusivity method reads besides the cell's position, the constants and the
+ * field parameters: bit 0 the counts (X->State, getCentralDifferenceX/Y), bit 1
+ * PhysicalSimTime, bit 2 the current fields (a field the method does not assign keeps its
+ * value, which does not count).  With none of them, the re-evaluation a
+ * non-linear model gets after every sweep (InhomogeneousSolver.cpp:304-336,384-417)
+ * reproduces the fields and time steps it starts with.  No counterpart in the reference. */
+int gpgmp_model_method_dependencies(const gpgmp_model *m, const char *method, const char *const *field_names,
+                                    int32_t nfields, uint32_t *deps);
 /* Bit i set when reaction i is localized (Reaction::isLocalized, Reaction.h:196). */
 int gpgmp_model_localized(const gpgmp_model *m, uint32_t *mask);
 

@@ -46,3 +46,29 @@ def test_drift_method_fields_and_run(gpu, oracle):
     want_st, tw = oracle.run(m, st, 60, 3)
     assert t == tw
     assert np.array_equal(s.synchronizeStateBufferFromDevice(), want_st)
+
+
+def test_nonlinear_model_with_fixed_fields_runs(gpu, oracle):
+    """A non-linear model whose method reads only positions, parameters and field parameters
+    (as the reference's OrnsteinUhlenbeck and MultiplicativeNoise examples): its fields after
+    any sweep are the first ones, so it builds and runs as the oracle's loop over them; one that
+    reads the counts is refused."""
+    im = sbml.load(os.path.join(GOLDEN, "inchman_drift.xml"))
+    im.nonlinear_diffusivity = True
+    im.drift_diffusivity_method = ("All->DiffusivityX = 0.8 + 0.2 * gradient;\nAll->DiffusivityY = 0.6;\n"
+                                   "All->DriftX = -(0.01 * PhysicalX);\nAll->DriftY = -(0.005 * PhysicalY);")
+    st, real, fields = sbml.build(im)
+    m = im.model
+    again = np.stack(sbml.drift_diffusivity_fields(im, st * 3 + 1, fields, np.ones((4, 1, 64, 64), np.float32),
+                                                   sim_time=5.0))
+    first = np.stack(sbml.drift_diffusivity_fields(im, st, fields))
+    assert again.tobytes() == first.tobytes()  # other counts, clock and current fields: same fields
+    s = m.createSolver()
+    s.synchronizeStateBufferToDevice(st)
+    t = s.run(40, 9)
+    want, tw = oracle.run(m, st, 40, 9)
+    assert t == tw
+    assert np.array_equal(s.synchronizeStateBufferFromDevice(), want)
+    im.drift_diffusivity_method = "All->DriftX = 0.001 * All->State;"
+    with pytest.raises(NotImplementedError):
+        sbml.build(im)

@@ -165,3 +165,47 @@ def test_reference_new_individuals_methods():
         rule = sbml.new_individual_rule(im)
         if n == "ABReactionIndividualNew.xml":
             assert rule == ((2, 2, 2, 2), (0.0,) * 4, (1.0,) * 4)
+
+
+def test_nonlinear_method_dependencies():
+    """A non-linear model's method is re-evaluated after every sweep; the compiler reports what
+    it reads, so one that reads neither the counts, the clock nor the current fields (its
+    fields cannot change) builds, and the others are refused with the reason."""
+    im = sbml.load(os.path.join(GOLDEN, "inchman_drift.xml"))
+    base = im.drift_diffusivity_method
+    assert sbml.method_dependencies(im) & 1  # the fixture's method reads the counts
+    cases = {
+        "All->DiffusivityX = 1.0; All->DriftX = 0.5 * PhysicalX + gradient;": 0,
+        "All->DriftY = PhysicalSimTime > 2 ? 1 : 0;": 2,
+        "All->DriftX = getCentralDifferenceX(All);": 1,
+        "All->DiffusivityY = 2 * All->DiffusivityY;": 4,
+        "All->DriftX = All->State + PhysicalSimTime;": 3,
+    }
+    for text, want in cases.items():
+        im.drift_diffusivity_method = text
+        assert sbml.method_dependencies(im) == want, text
+    im.nonlinear_diffusivity = True
+    im.drift_diffusivity_method = "All->DiffusivityX = 1.0; All->DiffusivityY = 1.0; All->DriftX = 0.1 * PhysicalX;"
+    sbml.check_nonlinear(im)  # accepted: nothing a sweep changes is read
+    im.drift_diffusivity_method = base
+    with pytest.raises(NotImplementedError, match="the counts"):
+        sbml.check_nonlinear(im)
+
+
+@pytest.mark.skipif(not os.path.isdir(EXAMPLES), reason="reference examples not present")
+def test_reference_nonlinear_examples():
+    """The reference's non-linear examples: those whose method reads only positions and
+    parameters build (their re-evaluated fields equal the first ones), Slit (counts and clock)
+    is refused, Nonlinear (MeanX) fails to compile."""
+    from inchman_amd import GpgmpError
+
+    for n, ok in (("MultiplicativeNoise.xml", True), ("OrnsteinUhlenbeck.xml", True), ("Slit.xml", False)):
+        im = sbml.load(os.path.join(EXAMPLES, n))
+        assert im.nonlinear_diffusivity
+        if ok:
+            assert sbml.method_dependencies(im) == 0, n
+        else:
+            assert sbml.method_dependencies(im) == 3, n
+    im = sbml.load(os.path.join(EXAMPLES, "Nonlinear.xml"))
+    with pytest.raises(GpgmpError):
+        sbml.method_dependencies(im)
