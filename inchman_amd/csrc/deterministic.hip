@@ -489,7 +489,7 @@ __global__ __launch_bounds__(256) void k_rdd_step(const float *__restrict__ u, c
                                                   const float *__restrict__ D, float *__restrict__ umid,
                                                   float *__restrict__ vmid, float *__restrict__ un,
                                                   float *__restrict__ vn, int W, int H, RddK p,
-                                                  const float *__restrict__ kdev) {
+                                                  const float *__restrict__ kdev, size_t mid_stride) {
     if (kdev) rdd_rates(p, kdev);
     constexpr int LX = RT_X + 2 * NS, LY = RT_Y + 2 * NS;
     __shared__ float sd[LY][LX];
@@ -522,9 +522,9 @@ __global__ __launch_bounds__(256) void k_rdd_step(const float *__restrict__ u, c
             sv[b][ly][lx] = vo;
             if (umid && lx >= NS && lx < NS + RT_X && ly >= NS && ly < NS + RT_Y) {
                 const int gx = x0 + lx - NS, gy = y0 + ly - NS;
-                if (gx < W && gy < H) {
-                    umid[(size_t)gy * W + gx] = uo;
-                    vmid[(size_t)gy * W + gx] = vo;
+                if (gx < W && gy < H) {  // the state after intermediate step s
+                    umid[(size_t)(s - 1) * mid_stride + (size_t)gy * W + gx] = uo;
+                    vmid[(size_t)(s - 1) * mid_stride + (size_t)gy * W + gx] = vo;
                 }
             }
         }
@@ -722,6 +722,114 @@ __global__ __launch_bounds__(256) void k_rdd_adjoint(
     tile_gk_add(gk, red, gk_tiles, p.dt);
 }
 
+// NS (3, 4) steps back per launch, generalising the NS = 2 walk: lambda after the latest
+// step is staged with an NS-cell halo; step k back (k = 0: the latest) forms lambda before
+// it on the tile plus a ring of NS - 1 - k cells (ring cells: no dL/dD or dL/dk, zero
+// outside the grid), with the state before that step, st.u[k], st.v[k].  The tile's own
+// cells are computed by their owner threads exactly as a one-step launch would (the same
+// partial sums of dL/dk, added to the tile's accumulator step by step; dL/dD summed in the
+// steps' order), so the result equals NS one-step launches bit for bit.
+struct RddStates {
+    const float *u[4], *v[4];
+};
+
+template <int NS>
+__global__ __launch_bounds__(256) void k_rdd_adjoint_n(const float *__restrict__ lu, const float *__restrict__ lv,
+                                                       RddStates st, const float *__restrict__ D,
+                                                       float *__restrict__ lun, float *__restrict__ lvn,
+                                                       float *__restrict__ gD, double *__restrict__ gk_tiles, int W,
+                                                       int H, RddK p, const float *__restrict__ kdev) {
+    if (kdev) rdd_rates(p, kdev);
+    constexpr int LX = RT_X + 2 * NS, LY = RT_Y + 2 * NS;
+    __shared__ float sl[2][2][LY][LX];  // [level parity][lambda_u, lambda_v]
+    __shared__ double red[4][4];
+    const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
+    for (int idx = threadIdx.x; idx < LX * LY; idx += blockDim.x) {
+        const int ly = idx / LX, lx = idx - ly * LX;
+        const int gx = x0 + lx - NS, gy = y0 + ly - NS;
+        float a = 0.f, b = 0.f;
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const size_t g = (size_t)gy * W + gx;
+            a = lu[g];
+            b = lv[g];
+        }
+        sl[0][0][ly][lx] = a;
+        sl[0][1][ly][lx] = b;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x & (RT_X - 1);
+    const int ty0 = (threadIdx.x >> 6) * 4;
+    const int gx = x0 + tx;
+    float gdv[NS][4];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int c = k & 1, n = c ^ 1;
+        const int r = NS - 1 - k;  // ring of this step's lambda
+        const float *__restrict__ us = st.u[k];
+        const float *__restrict__ vs = st.v[k];
+        float gk[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) gdv[k][j] = 0.f;
+        if (gx < W) {  // the tile's cells, by their owner threads
+            const float wl = tw(gx - 1, W), wr = tw(gx + 1, W);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int ty = ty0 + j;
+                const int gy = y0 + ty;
+                if (gy >= H) break;
+                const size_t g = (size_t)gy * W + gx;
+                const int lx = tx + NS, ly = ty + NS;
+                float lo, mo;
+                adj_cell(sl[c][0][ly][lx], sl[c][1][ly][lx], sl[c][0][ly][lx - 1], sl[c][0][ly][lx + 1],
+                         sl[c][0][ly - 1][lx], sl[c][0][ly + 1][lx], sl[c][1][ly][lx - 1], sl[c][1][ly][lx + 1],
+                         sl[c][1][ly - 1][lx], sl[c][1][ly + 1][lx], wl, wr, tw(gy - 1, H), tw(gy + 1, H), us[g], vs[g],
+                         D[g], p, lo, mo, gdv[k][j], gk);
+                if (k == NS - 1) {
+                    float acc = gD[g];
+#pragma unroll
+                    for (int kk = 0; kk < NS; ++kk) acc = acc + gdv[kk][j];
+                    gD[g] = acc;
+                    lun[g] = lo;
+                    lvn[g] = mo;
+                } else {
+                    sl[n][0][ly][lx] = lo;
+                    sl[n][1][ly][lx] = mo;
+                }
+            }
+        }
+        if (r > 0) {
+            const int RX = RT_X + 2 * r, RY = RT_Y + 2 * r;
+            for (int idx = threadIdx.x; idx < RX * RY; idx += blockDim.x) {
+                const int ry = idx / RX, rx = idx - ry * RX;
+                const int cxl = rx - r, cyl = ry - r;  // relative to the tile's corner
+                if (cxl >= 0 && cxl < RT_X && cyl >= 0 && cyl < RT_Y) {
+                    // a tile cell beyond the grid edge of a partial tile holds no lambda
+                    if (x0 + cxl >= W || y0 + cyl >= H) {
+                        sl[n][0][cyl + NS][cxl + NS] = 0.f;
+                        sl[n][1][cyl + NS][cxl + NS] = 0.f;
+                    }
+                    continue;
+                }
+                const int cx = x0 + cxl, cy = y0 + cyl;
+                const int lx = cxl + NS, ly = cyl + NS;
+                float lo = 0.f, mo = 0.f;
+                if (cx >= 0 && cx < W && cy >= 0 && cy < H) {
+                    const size_t g = (size_t)cy * W + cx;
+                    float gd;
+                    float gtmp[4] = {0.f, 0.f, 0.f, 0.f};
+                    adj_cell(sl[c][0][ly][lx], sl[c][1][ly][lx], sl[c][0][ly][lx - 1], sl[c][0][ly][lx + 1],
+                             sl[c][0][ly - 1][lx], sl[c][0][ly + 1][lx], sl[c][1][ly][lx - 1], sl[c][1][ly][lx + 1],
+                             sl[c][1][ly - 1][lx], sl[c][1][ly + 1][lx], tw(cx - 1, W), tw(cx + 1, W), tw(cy - 1, H),
+                             tw(cy + 1, H), us[g], vs[g], D[g], p, lo, mo, gd, gtmp);
+                }
+                sl[n][0][ly][lx] = lo;
+                sl[n][1][ly][lx] = mo;
+            }
+        }
+        tile_gk_add(gk, red, gk_tiles, p.dt);  // contains the barriers that publish this level
+    }
+}
+
 // dL/dk = sum over tiles (fixed order) -> float[4]
 __global__ void k_rdd_gk_reduce(const double *__restrict__ gk_tiles, int ntiles, float *__restrict__ gk) {
     __shared__ double red[256];
@@ -830,13 +938,14 @@ dim3 rdd_grid(int W, int H) { return dim3((W + RT_X - 1) / RT_X, (H + RT_Y - 1) 
 
 hipError_t launch_rdd_step(const float *u, const float *v, const float *D, float *umid, float *vmid, float *un,
                            float *vn, int W, int H, const RddConsts &c, int ns, hipStream_t s, hipEvent_t e0,
-                           hipEvent_t e1) {
+                           hipEvent_t e1, size_t mid_stride) {
     const RddK k = make_k(c);
     const dim3 g = rdd_grid(W, H), b(256);
-    if (ns < 1 || ns > 4 || (ns > 2 && umid)) return hipErrorInvalidValue;  // the mid state only with 2 steps
+    if (ns < 1 || ns > 4 || (ns > 2 && umid && mid_stride == 0)) return hipErrorInvalidValue;
     auto kern = (ns == 4) ? k_rdd_step<4> : (ns == 3) ? k_rdd_step<3> : (ns == 2) ? k_rdd_step<2> : k_rdd_step<1>;
-    if (e0) hipExtLaunchKernelGGL(kern, g, b, 0, s, e0, e1, 0, u, v, D, umid, vmid, un, vn, W, H, k, c.kdev);
-    else hipLaunchKernelGGL(kern, g, b, 0, s, u, v, D, umid, vmid, un, vn, W, H, k, c.kdev);
+    if (e0)
+        hipExtLaunchKernelGGL(kern, g, b, 0, s, e0, e1, 0, u, v, D, umid, vmid, un, vn, W, H, k, c.kdev, mid_stride);
+    else hipLaunchKernelGGL(kern, g, b, 0, s, u, v, D, umid, vmid, un, vn, W, H, k, c.kdev, mid_stride);
     return hipGetLastError();
 }
 
@@ -851,6 +960,24 @@ hipError_t launch_rdd_adjoint(const float *lu, const float *lv, const float *u1,
     else
         hipLaunchKernelGGL(k_rdd_adjoint<1>, g, b, 0, s, lu, lv, u1, v1, u0, v0, D, lun, lvn, gD, gk_tiles, W, H, k,
                            c.kdev);
+    return hipGetLastError();
+}
+
+hipError_t launch_rdd_adjoint_n(const float *lu, const float *lv, const float *const *us, const float *const *vs,
+                                const float *D, float *lun, float *lvn, float *gD, double *gk_tiles, int W, int H,
+                                const RddConsts &c, int ns, hipStream_t s) {
+    if (ns < 3 || ns > 4) return hipErrorInvalidValue;
+    const RddK k = make_k(c);
+    const dim3 g = rdd_grid(W, H), b(256);
+    RddStates st;
+    for (int j = 0; j < 4; ++j) {
+        st.u[j] = j < ns ? us[j] : nullptr;
+        st.v[j] = j < ns ? vs[j] : nullptr;
+    }
+    if (ns == 4)
+        hipLaunchKernelGGL(k_rdd_adjoint_n<4>, g, b, 0, s, lu, lv, st, D, lun, lvn, gD, gk_tiles, W, H, k, c.kdev);
+    else
+        hipLaunchKernelGGL(k_rdd_adjoint_n<3>, g, b, 0, s, lu, lv, st, D, lun, lvn, gD, gk_tiles, W, H, k, c.kdev);
     return hipGetLastError();
 }
 

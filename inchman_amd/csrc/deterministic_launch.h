@@ -20,15 +20,21 @@ hipError_t launch_det_inhom_sweep(const float *in, float *out, const float *D, c
 hipError_t launch_det_rk4(float *state, int A, const RxTable &T, float dt, hipStream_t s);
 // alpha-QSS reactions over dt per cell; fixed = 1: the predictor's Pade factor in the corrector
 hipError_t launch_det_aqss(float *state, int A, const RxTable &T, float dt, int fixed, hipStream_t s);
-// ns = 1 or 2 steps per launch (2 needs W, H >= 3); umid/vmid (may be NULL) receive
-// the intermediate state of a two-step launch
+// ns = 1 .. 4 steps per launch (sides >= ns + 1); umid/vmid (may be NULL) receive the
+// intermediate states: the state after intermediate step j (1 .. ns - 1) at umid/vmid +
+// (j - 1) * mid_stride (mid_stride may be 0 for ns = 2)
 hipError_t launch_rdd_step(const float *u, const float *v, const float *D, float *umid, float *vmid, float *un,
                            float *vn, int W, int H, const RddConsts &c, int ns, hipStream_t s,
-                           hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                           hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, size_t mid_stride = 0);
 // adjoint of ns steps: u1/v1 = state before the later step, u0/v0 = before the earlier one
 hipError_t launch_rdd_adjoint(const float *lu, const float *lv, const float *u1, const float *v1, const float *u0,
                               const float *v0, const float *D, float *lun, float *lvn, float *gD, double *gk_tiles,
                               int W, int H, const RddConsts &c, int ns, hipStream_t s);
+// adjoint of ns = 3 or 4 steps: us[k]/vs[k] = the state before the k-th step back (k = 0:
+// the latest)
+hipError_t launch_rdd_adjoint_n(const float *lu, const float *lv, const float *const *us, const float *const *vs,
+                                const float *D, float *lun, float *lvn, float *gD, double *gk_tiles, int W, int H,
+                                const RddConsts &c, int ns, hipStream_t s);
 hipError_t launch_rdd_gk_reduce(const double *gk_tiles, int ntiles, float *gk, hipStream_t s);
 hipError_t launch_fill_f32(float *p, size_t n, float v, hipStream_t s);
 hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t s);

@@ -35,17 +35,27 @@ struct gpgmp_rdd {
     hipStream_t stream = nullptr;
     size_t A = 0;
     DevBuf<float> ck;        // nck checkpoints of (u, v)
-    DevBuf<float> seg;       // ckpt recomputed states of one segment
+    DevBuf<float> seg, seg2; // ckpt recomputed states of a segment, two segments (seg2 on first backward)
     DevBuf<float> tmp;       // ping-pong partner of the caller's buffers
     DevBuf<double> gk_tiles; // per-tile dL/dk accumulators
     int ntiles = 0;
     bool saved = false;
     uint64_t generation = 0;  // saving forward runs so far; the last one is what backward differentiates
-    int ns = 4;  // forward steps per launch (temporal blocking); the adjoint takes up to 2
+    int ns = 4;  // forward steps per launch (temporal blocking); the adjoint takes up to 4
     RddConsts c{};
     DevBuf<float> kdev;  // rates set on the device (gpgmp_rdd_set_rates_device)
+    // the backward recomputes the next segment on `side` while the caller's stream walks the
+    // current one (default priority, non-blocking; events order the two)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_start = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_adj[2] = {nullptr, nullptr};
     ~gpgmp_rdd() {
         (void)hipStreamSynchronize(stream);
+        if (side) {
+            (void)hipStreamSynchronize(side);
+            (void)hipStreamDestroy(side);
+        }
+        for (hipEvent_t e : {ev_start, ev_rec[0], ev_rec[1], ev_adj[0], ev_adj[1]})
+            if (e) (void)hipEventDestroy(e);
     }
 };
 
@@ -221,29 +231,52 @@ int gpgmp_rdd_backward(gpgmp_rdd *r, uint64_t generation, const float *D, float 
         float *blu[2] = {lu, r->tmp.get()};
         float *blv[2] = {lv, r->tmp.get() + A};
         int cur = 0;
-        for (int c = r->nck - 1; c >= 0; --c) {
+        if (!r->side) {
+            r->seg2.alloc((size_t)r->ckpt * 2 * A);
+            GPGMP_HIP_CHECK(hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking));
+            for (hipEvent_t *e : {&r->ev_start, &r->ev_rec[0], &r->ev_rec[1], &r->ev_adj[0], &r->ev_adj[1]})
+                GPGMP_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        }
+        float *segs[2] = {r->seg.get(), r->seg2.get()};
+        // state n of segment c (its checkpoint at n0, the recomputed states after it)
+        auto state = [&](int c, int n) -> float * {
+            const int n0 = c * r->ckpt;
+            return (n == n0) ? r->ck.get() + (size_t)c * 2 * A : segs[c & 1] + (size_t)(n - n0) * 2 * A;
+        };
+        // recompute the states n0+1 .. n1-1 of segment c on stream st
+        auto recompute = [&](int c, hipStream_t st) {
             const int n0 = c * r->ckpt, n1 = std::min(r->steps, n0 + r->ckpt);
-            const float *cku = r->ck.get() + (size_t)c * 2 * A;
-            auto su = [&](int n) -> float * {
-                return (n == n0) ? const_cast<float *>(cku) : r->seg.get() + (size_t)(n - n0) * 2 * A;
-            };
-            // recompute the states n0+1 .. n1-1 of the segment
+            auto su = [&](int n) { return state(c, n); };
             int n = n0;
             while (n + 1 < n1) {
-                if (r->ns >= 2 && n + 2 < n1) {
+                if (r->ns >= 4 && n + 4 < n1) {  // states n+1 .. n+3 from the launch's LDS, n+4 its output
+                    GPGMP_HIP_CHECK(launch_rdd_step(su(n), su(n) + A, D, su(n + 1), su(n + 1) + A, su(n + 4),
+                                                    su(n + 4) + A, W, H, r->c, 4, st, nullptr, nullptr, 2 * A));
+                    n += 4;
+                } else if (r->ns >= 2 && n + 2 < n1) {
                     GPGMP_HIP_CHECK(launch_rdd_step(su(n), su(n) + A, D, su(n + 1), su(n + 1) + A, su(n + 2),
-                                                    su(n + 2) + A, W, H, r->c, 2, r->stream));
+                                                    su(n + 2) + A, W, H, r->c, 2, st));
                     n += 2;
                 } else {
                     GPGMP_HIP_CHECK(launch_rdd_step(su(n), su(n) + A, D, nullptr, nullptr, su(n + 1), su(n + 1) + A, W, H,
-                                                    r->c, 1, r->stream));
+                                                    r->c, 1, st));
                     n += 1;
                 }
             }
-            // walk the segment's steps backwards
-            n = n1 - 1;
+        };
+        // walk the steps of segment c backwards on the caller's stream
+        auto walk = [&](int c) {
+            const int n0 = c * r->ckpt, n1 = std::min(r->steps, n0 + r->ckpt);
+            auto su = [&](int n) { return state(c, n); };
+            int n = n1 - 1;
             while (n >= n0) {
-                if (r->ns >= 2 && n - 1 >= n0) {
+                if (r->ns >= 4 && n - 3 >= n0) {
+                    const float *us[4] = {su(n), su(n - 1), su(n - 2), su(n - 3)};
+                    const float *vs[4] = {su(n) + A, su(n - 1) + A, su(n - 2) + A, su(n - 3) + A};
+                    GPGMP_HIP_CHECK(launch_rdd_adjoint_n(blu[cur], blv[cur], us, vs, D, blu[cur ^ 1], blv[cur ^ 1], gD,
+                                                         r->gk_tiles.get(), W, H, r->c, 4, r->stream));
+                    n -= 4;
+                } else if (r->ns >= 2 && n - 1 >= n0) {
                     GPGMP_HIP_CHECK(launch_rdd_adjoint(blu[cur], blv[cur], su(n), su(n) + A, su(n - 1), su(n - 1) + A, D,
                                                        blu[cur ^ 1], blv[cur ^ 1], gD, r->gk_tiles.get(), W, H, r->c, 2,
                                                        r->stream));
@@ -256,6 +289,25 @@ int gpgmp_rdd_backward(gpgmp_rdd *r, uint64_t generation, const float *D, float 
                 }
                 cur ^= 1;
             }
+        };
+        // segment c-1 is recomputed on the side stream while segment c is walked (two segment
+        // buffers; a buffer is recomputed again only after the walk that read it)
+        GPGMP_HIP_CHECK(hipEventRecord(r->ev_start, r->stream));
+        GPGMP_HIP_CHECK(hipStreamWaitEvent(r->side, r->ev_start, 0));
+        const int last = r->nck - 1;
+        if (last >= 0) {
+            recompute(last, r->side);
+            GPGMP_HIP_CHECK(hipEventRecord(r->ev_rec[last & 1], r->side));
+        }
+        for (int c = last; c >= 0; --c) {
+            if (c >= 1) {
+                if (c + 1 <= last) GPGMP_HIP_CHECK(hipStreamWaitEvent(r->side, r->ev_adj[(c + 1) & 1], 0));
+                recompute(c - 1, r->side);
+                GPGMP_HIP_CHECK(hipEventRecord(r->ev_rec[(c - 1) & 1], r->side));
+            }
+            GPGMP_HIP_CHECK(hipStreamWaitEvent(r->stream, r->ev_rec[c & 1], 0));
+            walk(c);
+            GPGMP_HIP_CHECK(hipEventRecord(r->ev_adj[c & 1], r->stream));
         }
         if (cur) {
             GPGMP_HIP_CHECK(hipMemcpyAsync(lu, blu[1], A * sizeof(float), hipMemcpyDeviceToDevice, r->stream));

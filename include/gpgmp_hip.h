@@ -383,9 +383,9 @@ int gpgmp_rdd_set_rates_device(gpgmp_rdd *r, const float *k_dev);
 int gpgmp_rdd_generation(gpgmp_rdd *r, uint64_t *generation);
 /* Euler steps per forward kernel launch, 1 .. 4 (sides >= ns + 1; default 4 when both sides
  * are >= 5, else 2 when >= 3): ns steps per launch keep the intermediate states in LDS
- * (ns-cell halo) and give results bit-identical to one step per launch; the adjoint takes
- * min(ns, 2) steps per launch.  No counterpart in the reference (a launch shape of this
- * build's forward/adjoint). */
+ * (ns-cell halo) and give results bit-identical to one step per launch; the adjoint and its
+ * segment recompute take 4 steps per launch when ns = 4, else min(ns, 2).  No counterpart in
+ * the reference (a launch shape of this build's forward/adjoint). */
 int gpgmp_rdd_set_steps_per_launch(gpgmp_rdd *r, int32_t ns);
 /* Forward: u,v (device, in/out) advanced `steps` fused Euler steps with field D; stores
  * checkpoints when save != 0. */
