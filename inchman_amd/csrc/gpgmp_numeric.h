@@ -233,10 +233,9 @@ GPGMP_HD float log_pf(float u) {
     return __builtin_fmaf((float)e, 0.693147182f, __builtin_fmaf(two_s, z * p, two_s));
 }
 
-// sin(pi t) in float: t reduced to [0,2), folded to r in [0,1/2], x = pi r, Taylor series to
+// sin(pi t2) in float for t2 in [0,2): folded to r in [0,1/2], x = pi r, Taylor series to
 // x^13 (truncation below 7e-10) in Horner form with fused multiply-adds.
-GPGMP_HD float sinpi_pf(float t) {
-    float t2 = t - 2.0f * __builtin_floorf(t * 0.5f);
+GPGMP_HD float sinpi_02(float t2) {
     float sign = 1.0f;
     if (t2 >= 1.0f) {
         sign = -1.0f;
@@ -254,6 +253,13 @@ GPGMP_HD float sinpi_pf(float t) {
     return sign * __builtin_fmaf(x, z * p, x);
 }
 
+// sin(pi t) in float: t reduced to t - 2 floor(t / 2) in [0,2), then sinpi_02.
+GPGMP_HD float sinpi_pf(float t) { return sinpi_02(t - 2.0f * __builtin_floorf(t * 0.5f)); }
+
+// sinpi_pf(t) for 0 <= t < 4, bit for bit, with the reduction as one compare: below 2 the
+// floor is 0, and from 2 on t - 2 is exact (t and 2 within a factor of two).
+GPGMP_HD float sinpi_pf4(float t) { return sinpi_02((t >= 2.0f) ? t - 2.0f : t); }
+
 // Gaussian pair by the Box-Muller transform of one Threefry call (key {id, seed}, counter
 // {step, 0}), in float: u1 = (w.x / 2^9 + 1/2) 2^-23 and u2 likewise from w.y, in (0, 1)
 // and exact, g_x = sqrt(-2 ln u1) cos(2 pi u2), g_y = sqrt(-2 ln u1) sin(2 pi u2), with
@@ -264,8 +270,8 @@ GPGMP_HD void gauss2_f(uint32_t id, uint32_t seed, uint32_t step, float *gx, flo
     const float u1 = ((float)(w.x >> 9) + 0.5f) * 0x1p-23f;
     const float u2 = ((float)(w.y >> 9) + 0.5f) * 0x1p-23f;
     const float r = __builtin_sqrtf(-2.0f * log_pf(u1));
-    *gx = r * sinpi_pf(2.0f * u2 + 0.5f);
-    *gy = r * sinpi_pf(2.0f * u2);
+    *gx = r * sinpi_pf4(2.0f * u2 + 0.5f);  // arguments in (0, 2.5) and (0, 2)
+    *gy = r * sinpi_pf4(2.0f * u2);
 }
 
 }  // namespace gpgmp

@@ -43,7 +43,8 @@ struct gpgmp_particles {
     // current list (x, y, id; key_out and the offsets `off` are rebuilt for get) and the buffers
     // of the general step
     DevBuf<float> x, y, x2, y2;
-    DevBuf<uint32_t> id, id2, key, key2, key_out, off, hist, digit_total;
+    DevBuf<uint32_t> id, key, key2, key_out, off, hist, digit_total;
+    DevBuf<uint32_t> prec;  // the general step's {x, y, id} records: two arrays of 3 words per particle
     // per-cell work of both steps
     DevBuf<uint4> rec;  // per cell: survivors before it in its tile, particles, kill mask, stayers
     DevBuf<uint32_t> tile_cells, off_sorted;
@@ -62,7 +63,7 @@ struct gpgmp_particles {
     int n_ub = 0;
     bool stepped = false;  // keys/offsets exist once a step has sorted the list
     bool sorted_ok = false;  // moves stay within the 3x3 neighbourhood all but never
-    bool keys_stale = false;  // the sorted-list step stores no keys: key_out is rebuilt on demand
+    bool keys_stale = false;  // the steps store no keys: key_out is rebuilt from the positions on demand
     ~gpgmp_particles() {
         (void)hipStreamSynchronize(stream);
         if (n_ev) (void)hipEventDestroy(n_ev);
@@ -70,8 +71,9 @@ struct gpgmp_particles {
     }
     ParticleBuffers buffers() {
         ParticleBuffers b;
-        b.x = x.get(); b.y = y.get(); b.x2 = x2.get(); b.y2 = y2.get();
-        b.id = id.get(); b.id2 = id2.get(); b.key = key.get(); b.key2 = key2.get(); b.key_out = key_out.get();
+        b.x = x.get(); b.y = y.get();
+        b.id = id.get(); b.key = key.get(); b.key2 = key2.get();
+        b.recA = prec.get(); b.recB = prec.get() + (size_t)3 * capacity;
         b.hist = hist.get(); b.digit_total = digit_total.get();
         b.rec = rec.get();
         b.tile_s = tile_cells.get(); b.off_sorted = off_sorted.get();
@@ -175,7 +177,8 @@ int gpgmp_particles_create(const gpgmp_particle_params *p, int32_t capacity, voi
         ps->stream = (hipStream_t)stream;  // NULL: the default stream
         const size_t n = capacity;
         for (DevBuf<float> *b : {&ps->x, &ps->y, &ps->x2, &ps->y2}) b->alloc(n);
-        for (DevBuf<uint32_t> *b : {&ps->id, &ps->id2, &ps->key, &ps->key2, &ps->key_out}) b->alloc(n);
+        for (DevBuf<uint32_t> *b : {&ps->id, &ps->key, &ps->key2, &ps->key_out}) b->alloc(n);
+        ps->prec.alloc(6 * n);
         const size_t ncells = (size_t)p->side * p->side;
         ps->off.alloc(ncells + 1);
         ps->off_sorted.alloc(ncells + 1);
@@ -347,9 +350,9 @@ int gpgmp_particles_step(gpgmp_particles *ps, int32_t step0, int32_t steps) {
             } else {
                 // a list in any order: radix sort, offsets, annihilation, compaction
                 ParticleBuffers b = ps->buffers();
-                GPGMP_HIP_CHECK(particle_step(a, step0 + k, b, ps->stream));  // survivors' offsets to off
+                GPGMP_HIP_CHECK(particle_step(a, step0 + k, b, ps->stream));
                 ps->cur ^= 1;
-                ps->keys_stale = false;
+                ps->keys_stale = true;
             }
             ps->stepped = true;
             ps->poll_n();

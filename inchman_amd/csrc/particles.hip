@@ -23,7 +23,25 @@ constexpr int RS_WAVES = RS_THREADS / 64;
 
 __device__ __forceinline__ int load_n(const int *d_n, int n_host) { return d_n ? *d_n : n_host; }
 
-// Per-tile digit histogram of one pass: hist[d * ntiles + tile].
+// The lanes among `among` whose low NB bits of `key` equal this lane's: one __ballot per bit,
+// the lane's own bit keeping the ballot or its complement, in two 32-bit halves (a bit of 1
+// keeps m, a bit of 0 keeps ~m = m ^ 0xffffffff).
+template <int NB>
+__device__ __forceinline__ uint64_t match_low_bits(uint32_t key, uint64_t among) {
+    uint32_t lo = (uint32_t)among, hi = (uint32_t)(among >> 32);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const uint32_t bit = (key >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        const uint32_t f = bit - 1u;
+        lo &= (uint32_t)m ^ f;
+        hi &= (uint32_t)(m >> 32) ^ f;
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Per-tile digit histogram of one pass: hist[d * ntiles + tile].  A full tile of 16-byte
+// aligned keys is read as 16-byte words, every load of the thread issued before the counts.
 __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t *__restrict__ keys, const int *d_n,
                                                             int n_host, int shift, uint32_t *__restrict__ hist,
                                                             int ntiles) {
@@ -32,24 +50,40 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t *__res
     cnt[threadIdx.x] = 0;
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * RS_TILE;
+    const bool vec = base + RS_TILE <= (size_t)n && (reinterpret_cast<uintptr_t>(keys) & 15u) == 0;
+    if (vec) {
+        const uint4 *k4 = reinterpret_cast<const uint4 *>(keys + base);
+        uint4 v[RS_ITEMS / 4];
+#pragma unroll
+        for (int j = 0; j < RS_ITEMS / 4; ++j) v[j] = k4[j * RS_THREADS + threadIdx.x];
+#pragma unroll
+        for (int j = 0; j < RS_ITEMS / 4; ++j) {
+            atomicAdd(&cnt[(v[j].x >> shift) & 255u], 1u);
+            atomicAdd(&cnt[(v[j].y >> shift) & 255u], 1u);
+            atomicAdd(&cnt[(v[j].z >> shift) & 255u], 1u);
+            atomicAdd(&cnt[(v[j].w >> shift) & 255u], 1u);
+        }
+    } else {
 #pragma unroll 4
-    for (int j = 0; j < RS_ITEMS; ++j) {
-        const size_t i = base + (size_t)j * RS_THREADS + threadIdx.x;
-        if (i < (size_t)n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+        for (int j = 0; j < RS_ITEMS; ++j) {
+            const size_t i = base + (size_t)j * RS_THREADS + threadIdx.x;
+            if (i < (size_t)n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+        }
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Block-wide exclusive scan of one value per thread (256 threads); returns the total.
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *smem /*[RS_WAVES]*/, uint32_t &total) {
+// Block-wide exclusive scan of one value per thread (NW waves); returns the total.
+template <int NW = RS_WAVES>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *smem /*[NW]*/, uint32_t &total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t x = wave_incl_scan_u32(v);
     if (lane == 63) smem[wave] = x;
     __syncthreads();
     uint32_t wbase = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < RS_WAVES; ++w) {
+    for (int w = 0; w < NW; ++w) {
         const uint32_t s = smem[w];
         if (w < wave) wbase += s;
         tot += s;
@@ -59,41 +93,48 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s
     return wbase + x - v;
 }
 
-// Exclusive scan of each digit's row hist[d][0..ntiles) (one block per digit, in
-// chunks of 256), writing the digit's total.
+// Exclusive scan of each digit's row hist[d][0..ntiles) (one block per digit; a thread takes
+// RS_ITEMS consecutive entries of a round of RS_TILE), writing the digit's total.
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scan_rows(uint32_t *__restrict__ hist, int ntiles,
                                                                  uint32_t *__restrict__ digit_total) {
     __shared__ uint32_t sm[RS_WAVES];
     uint32_t *row = hist + (size_t)blockIdx.x * ntiles;
     uint32_t carry = 0;
-    for (int c0 = 0; c0 < ntiles; c0 += RS_THREADS) {
-        const int i = c0 + threadIdx.x;
-        const uint32_t v = (i < ntiles) ? row[i] : 0u;
+    for (int c0 = 0; c0 < ntiles; c0 += RS_TILE) {
+        const int i0 = c0 + RS_ITEMS * (int)threadIdx.x;
+        uint32_t v[RS_ITEMS], t = 0;
+#pragma unroll
+        for (int k = 0; k < RS_ITEMS; ++k) {
+            v[k] = (i0 + k < ntiles) ? row[i0 + k] : 0u;
+            t += v[k];
+        }
         uint32_t tot;
-        const uint32_t ex = block_exclusive_scan(v, sm, tot);
-        if (i < ntiles) row[i] = carry + ex;
+        uint32_t run = carry + block_exclusive_scan(t, sm, tot);
+#pragma unroll
+        for (int k = 0; k < RS_ITEMS; ++k) {
+            if (i0 + k < ntiles) row[i0 + k] = run;
+            run += v[k];
+        }
         carry += tot;
     }
     if (threadIdx.x == 0) digit_total[blockIdx.x] = carry;
 }
 
-// Exclusive scan of the 256 digit totals -> digit bases (one block).
-__global__ __launch_bounds__(RS_THREADS) void k_radix_scan_totals(uint32_t *__restrict__ digit_total) {
-    __shared__ uint32_t sm[RS_WAVES];
-    uint32_t tot;
-    const uint32_t v = digit_total[threadIdx.x];
-    const uint32_t ex = block_exclusive_scan(v, sm, tot);
-    digit_total[threadIdx.x] = ex;
-}
-
-// Payload words carried with the keys through a pass (NP of them).
+// Payload words carried with the keys through a pass: NP separate arrays in[q] / out[q], or
+// (AOS) one array of NP-word records in[0] / out[0], moved with one load and one store each.
 struct RadixPayload {
     const uint32_t *in[3];
     uint32_t *out[3];
 };
 
-// Stable scatter of one pass, staged in LDS.  Wave w of a block takes the contiguous part
-// [w * 64 * RS_ITEMS, (w + 1) * 64 * RS_ITEMS) of the tile, 64 consecutive keys per round, with
+template <int NP>
+struct PayRec {
+    uint32_t w[NP];
+};
+
+// Stable scatter of one pass, staged in LDS, by blocks of SX_THREADS threads (8 waves, SX_ITEMS
+// keys each, so two blocks share a CU's LDS).  Wave w of a block takes the contiguous part
+// [w * 64 * SX_ITEMS, (w + 1) * 64 * SX_ITEMS) of the tile, 64 consecutive keys per round, with
 // the keys and their payload in registers, and ranks each key without block barriers: the
 // lanes of the round holding the same digit are found with eight __ballot calls, and the
 // key's rank within the wave is the count of its digit in the wave's earlier rounds (kept in
@@ -103,46 +144,58 @@ struct RadixPayload {
 // -- and the keys go to LDS at their digit's place.  The tile, now sorted by digit, is written
 // out in LDS order: consecutive threads write consecutive places of one digit's run, so the
 // stores coalesce instead of landing one word at a time all over the output.
-template <int NP>
-__global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
+constexpr int SX_THREADS = 512;
+constexpr int SX_WAVES = SX_THREADS / 64;
+constexpr int SX_ITEMS = RS_TILE / SX_THREADS;  // 8 keys per thread
+static_assert(SX_ITEMS * SX_THREADS == RS_TILE, "a scatter block covers one tile");
+
+template <int NP, bool AOS>
+__global__ __launch_bounds__(SX_THREADS) void k_radix_scatter(
     const uint32_t *__restrict__ kin, uint32_t *__restrict__ kout, RadixPayload pl, const int *d_n, int n_host,
-    int shift, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ digit_base, int ntiles) {
+    int shift, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ digit_total, int ntiles) {
     __shared__ uint32_t skey[RS_TILE];
     __shared__ uint32_t spay[NP][RS_TILE];
-    __shared__ uint32_t wcnt[RS_WAVES][256];  // per wave: its keys of each digit, then its offset in the run
+    __shared__ uint32_t wcnt[SX_WAVES][256];  // per wave: its keys of each digit, then its offset in the run
     __shared__ uint32_t tstart[256];          // the tile's first place of each digit
     __shared__ uint32_t gbase[256];           // output place of the tile's first key of each digit
-    __shared__ uint32_t sm[RS_WAVES];
+    __shared__ uint32_t sm[SX_WAVES];
     const int n = load_n(d_n, n_host);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const size_t tbase = (size_t)blockIdx.x * RS_TILE;
-    const size_t wbase = tbase + (size_t)wave * (RS_ITEMS * 64);
+    const size_t wbase = tbase + (size_t)wave * (SX_ITEMS * 64);
 #pragma unroll
-    for (int w = 0; w < RS_WAVES; ++w) wcnt[w][threadIdx.x] = 0;
-    gbase[threadIdx.x] = digit_base[threadIdx.x] + hist[(size_t)threadIdx.x * ntiles + blockIdx.x];
-    uint32_t key[RS_ITEMS], pay[NP][RS_ITEMS], rank[RS_ITEMS];
+    for (int k = 0; k < SX_WAVES * 256 / SX_THREADS; ++k) (&wcnt[0][0])[k * SX_THREADS + threadIdx.x] = 0;
+    {
+        // the digits' first output places: the exclusive scan of the pass's digit totals
+        uint32_t all;
+        const uint32_t dbase = block_exclusive_scan<SX_WAVES>((threadIdx.x < 256) ? digit_total[threadIdx.x] : 0u,
+                                                              sm, all);
+        if (threadIdx.x < 256) gbase[threadIdx.x] = dbase + hist[(size_t)threadIdx.x * ntiles + blockIdx.x];
+    }
+    uint32_t key[SX_ITEMS], pay[NP][SX_ITEMS], rank[SX_ITEMS];
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; ++j) {
+    for (int j = 0; j < SX_ITEMS; ++j) {
         const size_t i = wbase + (size_t)j * 64 + lane;
         const bool valid = i < (size_t)n;
         key[j] = valid ? kin[i] : 0u;
+        if (AOS) {
+            PayRec<NP> r{};
+            if (valid) r = reinterpret_cast<const PayRec<NP> *>(pl.in[0])[i];
 #pragma unroll
-        for (int q = 0; q < NP; ++q) pay[q][j] = valid ? pl.in[q][i] : 0u;
+            for (int q = 0; q < NP; ++q) pay[q][j] = r.w[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < NP; ++q) pay[q][j] = valid ? pl.in[q][i] : 0u;
+        }
     }
     __syncthreads();
     uint32_t *cw = wcnt[wave];
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; ++j) {
+    for (int j = 0; j < SX_ITEMS; ++j) {
         const bool valid = wbase + (size_t)j * 64 + lane < (size_t)n;
         const uint32_t d = (key[j] >> shift) & 255u;
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
-        }
+        const uint64_t peers = match_low_bits<8>(d, __ballot(valid));
         const uint32_t before = cw[d];
         rank[j] = before + (uint32_t)__popcll(peers & lt);
         // every lane has read the count before the group's last lane bumps it (one wave:
@@ -151,19 +204,22 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     }
     __syncthreads();
     {
-        uint32_t acc = 0;  // thread t: digit t
+        uint32_t acc = 0;  // thread t < 256: digit t
+        if (threadIdx.x < 256) {
 #pragma unroll
-        for (int w = 0; w < RS_WAVES; ++w) {
-            const uint32_t c = wcnt[w][threadIdx.x];
-            wcnt[w][threadIdx.x] = acc;
-            acc += c;
+            for (int w = 0; w < SX_WAVES; ++w) {
+                const uint32_t c = wcnt[w][threadIdx.x];
+                wcnt[w][threadIdx.x] = acc;
+                acc += c;
+            }
         }
         uint32_t tot;
-        tstart[threadIdx.x] = block_exclusive_scan(acc, sm, tot);
+        const uint32_t ex = block_exclusive_scan<SX_WAVES>(acc, sm, tot);
+        if (threadIdx.x < 256) tstart[threadIdx.x] = ex;
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; ++j) {
+    for (int j = 0; j < SX_ITEMS; ++j) {
         if (wbase + (size_t)j * 64 + lane < (size_t)n) {
             const uint32_t d = (key[j] >> shift) & 255u;
             const uint32_t lp = tstart[d] + cw[d] + rank[j];
@@ -174,13 +230,20 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     }
     __syncthreads();
     const int cnt = (n - (int)tbase < RS_TILE) ? n - (int)tbase : RS_TILE;
-    for (int lp = threadIdx.x; lp < cnt; lp += RS_THREADS) {
+    for (int lp = threadIdx.x; lp < cnt; lp += SX_THREADS) {
         const uint32_t k = skey[lp];
         const uint32_t d = (k >> shift) & 255u;
         const uint32_t pos = gbase[d] + ((uint32_t)lp - tstart[d]);
         kout[pos] = k;
+        if (AOS) {
+            PayRec<NP> r;
 #pragma unroll
-        for (int q = 0; q < NP; ++q) pl.out[q][pos] = spay[q][lp];
+            for (int q = 0; q < NP; ++q) r.w[q] = spay[q][lp];
+            reinterpret_cast<PayRec<NP> *>(pl.out[0])[pos] = r;
+        } else {
+#pragma unroll
+            for (int q = 0; q < NP; ++q) pl.out[q][pos] = spay[q][lp];
+        }
     }
 }
 
@@ -243,23 +306,25 @@ __device__ __forceinline__ void move_particle(float xo, float yo, uint32_t idv, 
     gauss2_f(idv, p.seed, (uint32_t)step, &gx, &gy);
     const float ty = POW2 ? (2.0f * yo) * p.inv_L : (2.0f * yo) / p.L;
     const float tx = POW2 ? (2.0f * xo) * p.inv_L : (2.0f * xo) / p.L;
-    const float vx = -p.amp * sinpi_pf(ty);
-    const float vy = p.amp * sinpi_pf(tx);
+    // positions in [0, L): ty, tx in [0, 2]
+    const float vx = -p.amp * sinpi_pf4(ty);
+    const float vy = p.amp * sinpi_pf4(tx);
     xn = wrap_pos<POW2>(xo + vx * p.dt + p.sigma * gx, p);
     yn = wrap_pos<POW2>(yo + vy * p.dt + p.sigma * gy, p);
 }
 
-// Move in place and write the cell key for the sort.
+// Move, writing the cell key and the record {x, y, id} that the sort carries.
 template <bool POW2>
-__global__ void k_move(float *__restrict__ x, float *__restrict__ y, const uint32_t *__restrict__ id,
-                       uint32_t *__restrict__ keys, const int *d_n, int step, PartConsts p) {
+__global__ void k_move(const float *__restrict__ x, const float *__restrict__ y, const uint32_t *__restrict__ id,
+                       uint32_t *__restrict__ keys, PayRec<3> *__restrict__ recs, const int *d_n, int step,
+                       PartConsts p) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *d_n) return;
     float xn, yn;
-    move_particle<POW2>(x[i], y[i], id[i], step, p, xn, yn);
-    x[i] = xn;
-    y[i] = yn;
+    const uint32_t idv = id[i];
+    move_particle<POW2>(x[i], y[i], idv, step, p, xn, yn);
     keys[i] = (uint32_t)((int)yn * p.side + (int)xn);
+    recs[i] = PayRec<3>{{__float_as_uint(xn), __float_as_uint(yn), idv}};
 }
 
 // A+A->0 draws: pairs 2m and 2m+1 of cell c take the two words of the Threefry block
@@ -382,13 +447,7 @@ __global__ __launch_bounds__(CT) void k_cs_move(const float *__restrict__ x, con
         const uint64_t heads = __ballot(lane == 0 || lo != lo_prev);
         const int s0 = 63 - __clzll(heads & (lower | (1ull << lane)));
         const uint64_t run_lower = lower & ~((1ull << s0) - 1ull);
-        uint64_t same_d = __ballot(ranked);
-#pragma unroll
-        for (int bit = 0; bit < 4; ++bit) {
-            const bool v = (d >> bit) & 1u;
-            const uint64_t m = __ballot(v);
-            same_d &= v ? m : ~m;
-        }
+        const uint64_t same_d = match_low_bits<4>(d, __ballot(ranked));
         uint32_t rank = 0;
         if (ranked) rank = cr[d][lo] + (uint32_t)__popcll(same_d & run_lower);
         // every read of the round precedes its increments (one wave: LDS operations in order)
@@ -408,6 +467,55 @@ __global__ __launch_bounds__(CT) void k_cs_move(const float *__restrict__ x, con
     if (lane < ncell) {
 #pragma unroll
         for (int d = 0; d < 9; ++d) cnt[(size_t)d * A + cw + lane] = cr[d][lane];
+    }
+}
+
+// Exclusive scan in place of ntiles tile sums by one block of NT threads (4096 / NT tiles per
+// thread per round); returns the total.
+template <int NT>
+__device__ __forceinline__ uint32_t scan_tiles(uint32_t *__restrict__ tile_sum, int ntiles, uint32_t *wsum) {
+    constexpr int PER = 4096 / NT, NW = NT / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (int c0 = 0; c0 < ntiles; c0 += 4096) {
+        const int i0 = c0 + PER * (int)threadIdx.x;
+        uint32_t v[PER], t = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            v[k] = (i0 + k < ntiles) ? tile_sum[i0 + k] : 0u;
+            t += v[k];
+        }
+        const uint32_t incl = wave_incl_scan_u32(t);
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        uint32_t wbase = 0, all = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t sw = wsum[w];
+            wbase += (w < wave) ? sw : 0u;
+            all += sw;
+        }
+        __syncthreads();
+        uint32_t run = carry + wbase + incl - t;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            if (i0 + k < ntiles) tile_sum[i0 + k] = run;
+            run += v[k];
+        }
+        carry += all;
+    }
+    return carry;
+}
+
+// 3. exclusive scan of the survivors' tile sums; the new count goes to the other count slot
+__global__ __launch_bounds__(1024) void k_coh_scan(uint32_t *__restrict__ tile_sum, int ntiles,
+                                                    const int *__restrict__ n_cur, int *__restrict__ n_next,
+                                                    unsigned long long *processed) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t total = scan_tiles<1024>(tile_sum, ntiles, wsum);
+    if (threadIdx.x == 0) {
+        *processed += (unsigned long long)*n_cur;
+        *n_next = (int)total;
     }
 }
 
@@ -496,14 +604,28 @@ __global__ __launch_bounds__(256) void k_cs_cells(const uint32_t *__restrict__ c
             nn[d] = cnt[(size_t)d * A + key[d]];
             T += nn[d];
         }
+        uint32_t q[9];
+        if (cx >= 1 && cx < side - 1 && cy >= 1 && cy < side - 1) {
+            // no source wraps: key[d] = c + (1 - d / 3) side + (1 - d % 3) rises as d falls, so
+            // the sources below d's are those of d + 1 .. 8
+            uint32_t run = 0;
 #pragma unroll
-        for (int d = 0; d < 9; ++d) {
-            uint32_t q = 0;
+            for (int d = 8; d >= 0; --d) {
+                q[d] = run;
+                run += nn[d];
+            }
+        } else {
 #pragma unroll
-            for (int e = 0; e < 9; ++e) q += (key[e] < key[d]) ? nn[e] : 0u;
-            if (d == 4) pre4 = q;  // the particles that stay: kept in the cell record
-            else pre[(size_t)d * A + c] = q;
+            for (int d = 0; d < 9; ++d) {
+                q[d] = 0;
+#pragma unroll
+                for (int e = 0; e < 9; ++e) q[d] += (key[e] < key[d]) ? nn[e] : 0u;
+            }
         }
+        pre4 = q[4];  // the particles that stay: kept in the cell record
+#pragma unroll
+        for (int d = 0; d < 9; ++d)
+            if (d != 4) pre[(size_t)d * A + c] = q[d];
         const uint32_t fin = far_in[c];
         if (fin) far_in[c] = 0u;  // counted: clear for the next step
         T += fin;
@@ -516,53 +638,6 @@ __global__ __launch_bounds__(256) void k_cs_cells(const uint32_t *__restrict__ c
     const uint32_t ex = block_exclusive_scan(S, sm, total);
     if (c < A) rec[c] = make_uint4(ex, T, mask, pre4);  // .x: the survivors before c in its tile
     if (threadIdx.x == 0) tile_sum[tile] = total;
-}
-
-// Exclusive scan in place of ntiles tile sums by one block of 1024 threads (4 tiles per thread
-// per round); returns the total.
-__device__ __forceinline__ uint32_t scan_tiles_1024(uint32_t *__restrict__ tile_sum, int ntiles, uint32_t *wsum) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t carry = 0;
-    for (int c0 = 0; c0 < ntiles; c0 += 4096) {
-        const int i0 = c0 + 4 * (int)threadIdx.x;
-        uint32_t v[4], t = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            v[k] = (i0 + k < ntiles) ? tile_sum[i0 + k] : 0u;
-            t += v[k];
-        }
-        const uint32_t incl = wave_incl_scan_u32(t);
-        if (lane == 63) wsum[wave] = incl;
-        __syncthreads();
-        uint32_t wbase = 0, all = 0;
-#pragma unroll
-        for (int w = 0; w < 16; ++w) {
-            const uint32_t sw = wsum[w];
-            wbase += (w < wave) ? sw : 0u;
-            all += sw;
-        }
-        __syncthreads();
-        uint32_t run = carry + wbase + incl - t;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (i0 + k < ntiles) tile_sum[i0 + k] = run;
-            run += v[k];
-        }
-        carry += all;
-    }
-    return carry;
-}
-
-// 3. exclusive scan of the survivors' tile sums; the new count goes to the other count slot
-__global__ __launch_bounds__(1024) void k_coh_scan(uint32_t *__restrict__ tile_sum, int ntiles,
-                                                    const int *__restrict__ n_cur, int *__restrict__ n_next,
-                                                    unsigned long long *processed) {
-    __shared__ uint32_t wsum[16];
-    const uint32_t total = scan_tiles_1024(tile_sum, ntiles, wsum);
-    if (threadIdx.x == 0) {
-        *processed += (unsigned long long)*n_cur;
-        *n_next = (int)total;
-    }
 }
 
 // 4. every particle to its place in the new order; the killed pairs drop out.  Flat over the
@@ -637,16 +712,39 @@ __global__ __launch_bounds__(SC_BLOCK) void k_cs_scatter(
 // off[c] = #(key < c) for c in [0, A] from the sorted keys: particle i writes off[c] = i for the
 // cells after the previous particle's key up to its own (the empty cells before it and its
 // own cell when it starts one); the thread past the last particle writes the cells after it.
-// Every entry is written once; an empty stretch of cells is written by one thread.
-__global__ void k_offsets_sorted(const uint32_t *__restrict__ key, const int *d_n, int A,
-                                 uint32_t *__restrict__ off) {
+// Every entry is written once; an empty stretch of cells is written by one thread.  A thread
+// takes 4 consecutive particles (one 16-byte load where the keys are aligned and all live),
+// the key before them from the lower lane.
+__global__ __launch_bounds__(256) void k_offsets_sorted(const uint32_t *__restrict__ key, const int *d_n, int A,
+                                                        uint32_t *__restrict__ off) {
     const uint32_t n = (uint32_t)*d_n;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
+    const uint32_t i0 = 4u * (blockIdx.x * blockDim.x + threadIdx.x);
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t k[4];
+    if (i0 + 4u <= n && (reinterpret_cast<uintptr_t>(key) & 15u) == 0) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(key + i0);
+        k[0] = v.x;
+        k[1] = v.y;
+        k[2] = v.z;
+        k[3] = v.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) k[j] = (i0 + j < n) ? key[i0 + j] : 0u;
+    }
+    // the key of particle i0 - 1: the lower lane's last, or loaded by a wave's first lane
+    uint32_t prev = (uint32_t)__shfl_up((int)k[3], 1);
+    if (lane == 0 && i0 > 0 && i0 - 1 < n) prev = key[i0 - 1];
+    if (i0 > n) return;  // after the shuffle: every lane of the wave took part
     const uint32_t a = (uint32_t)A;
-    const uint32_t lo = (i == 0) ? 0u : min(key[i - 1], a - 1u) + 1u;
-    const uint32_t hi = (i == n) ? a : min(key[i], a);
-    for (uint32_t c = lo; c <= hi; ++c) off[c] = i;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t i = i0 + j;
+        if (i > n) break;
+        const uint32_t lo = (i == 0) ? 0u : min(prev, a - 1u) + 1u;
+        const uint32_t hi = (i == n) ? a : min(k[j], a);
+        for (uint32_t c = lo; c <= hi; ++c) off[c] = i;
+        prev = k[j];
+    }
 }
 
 // per cell: its particles T (from the sorted list's offsets), the A+A->0 draws, the kill mask
@@ -674,11 +772,9 @@ __global__ __launch_bounds__(256) void k_rs_cells(const uint32_t *__restrict__ o
 // the survivors of the sorted list to their places (stable compaction), flat over the list:
 // a particle's rank in its cell is its index less the cell's first index
 __global__ __launch_bounds__(SC_BLOCK) void k_rs_compact(
-    const uint32_t *__restrict__ key, const float *__restrict__ xs, const float *__restrict__ ys,
-    const uint32_t *__restrict__ ids, const uint32_t *__restrict__ off_sorted, const uint4 *__restrict__ rec,
-    const uint32_t *__restrict__ tile_off, const int *d_n, int step, PartConsts p,
-    float *__restrict__ xo, float *__restrict__ yo, uint32_t *__restrict__ ido, uint32_t *__restrict__ keyo,
-    uint32_t cap, uint32_t per) {
+    const uint32_t *__restrict__ key, const PayRec<3> *__restrict__ recs, const uint32_t *__restrict__ off_sorted,
+    const uint4 *__restrict__ rec, const uint32_t *__restrict__ tile_off, const int *d_n, int step, PartConsts p,
+    float *__restrict__ xo, float *__restrict__ yo, uint32_t *__restrict__ ido, uint32_t cap, uint32_t per) {
     const uint32_t n = (uint32_t)*d_n;
     const uint32_t b0 = xcd_item(blockIdx.x, per) * SC_SPAN;
     if (b0 >= n) return;
@@ -690,9 +786,11 @@ __global__ __launch_bounds__(SC_BLOCK) void k_rs_compact(
         const uint32_t i = b0 + q * SC_BLOCK + threadIdx.x;
         const bool v = i < n;
         c[q] = v ? key[i] : 0u;
-        xv[q] = v ? xs[i] : 0.f;
-        yv[q] = v ? ys[i] : 0.f;
-        idv[q] = v ? ids[i] : 0u;
+        PayRec<3> r{};
+        if (v) r = recs[i];
+        xv[q] = __uint_as_float(r.w[0]);
+        yv[q] = __uint_as_float(r.w[1]);
+        idv[q] = r.w[2];
     }
     uint32_t s0[SC_PER], T[SC_PER], m[SC_PER], base[SC_PER];
 #pragma unroll
@@ -715,7 +813,6 @@ __global__ __launch_bounds__(SC_BLOCK) void k_rs_compact(
         xo[pos] = xv[q];
         yo[pos] = yv[q];
         ido[pos] = idv[q];
-        keyo[pos] = c[q];
     }
 }
 
@@ -737,14 +834,16 @@ namespace gpgmp {
 int radix_tiles(int n) { return (n + RS_TILE - 1) / RS_TILE; }
 
 // LSD passes of 8-bit digits over key_bits bits, ping-ponging the keys and np payload arrays
-// between the given and the tmp buffers; *result_in_tmp tells where the result is.
+// (aos: pay[0] and ptmp[0] are arrays of np-word records) between the given and the tmp
+// buffers; *result_in_tmp tells where the result is.
 hipError_t radix_sort_payload(uint32_t *keys, uint32_t *ktmp, uint32_t *const *pay, uint32_t *const *ptmp, int np,
-                              const int *d_n, int n_ub, int key_bits, uint32_t *hist, uint32_t *digit_total,
-                              hipStream_t s, bool *result_in_tmp) {
+                              bool aos, const int *d_n, int n_ub, int key_bits, uint32_t *hist,
+                              uint32_t *digit_total, hipStream_t s, bool *result_in_tmp) {
+    const int narr = aos ? 1 : np;
     const int nt = std::max(radix_tiles(n_ub), 1);
     uint32_t *ki = keys, *ko = ktmp;
     uint32_t *pi[3] = {nullptr, nullptr, nullptr}, *po[3] = {nullptr, nullptr, nullptr};
-    for (int q = 0; q < np; ++q) {
+    for (int q = 0; q < narr; ++q) {
         pi[q] = pay[q];
         po[q] = ptmp[q];
     }
@@ -752,22 +851,24 @@ hipError_t radix_sort_payload(uint32_t *keys, uint32_t *ktmp, uint32_t *const *p
     for (int shift = 0; shift < key_bits; shift += 8) {
         hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(RS_THREADS), 0, s, ki, d_n, n_ub, shift, hist, nt);
         hipLaunchKernelGGL(k_radix_scan_rows, dim3(256), dim3(RS_THREADS), 0, s, hist, nt, digit_total);
-        hipLaunchKernelGGL(k_radix_scan_totals, dim3(1), dim3(RS_THREADS), 0, s, digit_total);
         RadixPayload pl;
         for (int q = 0; q < 3; ++q) {
             pl.in[q] = pi[q];
             pl.out[q] = po[q];
         }
-        if (np == 1)
-            hipLaunchKernelGGL(k_radix_scatter<1>, dim3(nt), dim3(RS_THREADS), 0, s, ki, ko, pl, d_n, n_ub, shift, hist,
-                               digit_total, nt);
-        else if (np == 3)
-            hipLaunchKernelGGL(k_radix_scatter<3>, dim3(nt), dim3(RS_THREADS), 0, s, ki, ko, pl, d_n, n_ub, shift, hist,
-                               digit_total, nt);
+        if (np == 1 && !aos)
+            hipLaunchKernelGGL((k_radix_scatter<1, false>), dim3(nt), dim3(SX_THREADS), 0, s, ki, ko, pl, d_n, n_ub,
+                               shift, hist, digit_total, nt);
+        else if (np == 3 && !aos)
+            hipLaunchKernelGGL((k_radix_scatter<3, false>), dim3(nt), dim3(SX_THREADS), 0, s, ki, ko, pl, d_n, n_ub,
+                               shift, hist, digit_total, nt);
+        else if (np == 3 && aos)
+            hipLaunchKernelGGL((k_radix_scatter<3, true>), dim3(nt), dim3(SX_THREADS), 0, s, ki, ko, pl, d_n, n_ub,
+                               shift, hist, digit_total, nt);
         else
             return hipErrorInvalidValue;
         std::swap(ki, ko);
-        for (int q = 0; q < np; ++q) std::swap(pi[q], po[q]);
+        for (int q = 0; q < narr; ++q) std::swap(pi[q], po[q]);
         ++passes;
     }
     *result_in_tmp = (passes & 1) != 0;
@@ -778,7 +879,8 @@ hipError_t radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *ktmp, uint
                             int n_ub, int key_bits, uint32_t *hist, uint32_t *digit_total, hipStream_t s,
                             bool *result_in_tmp) {
     uint32_t *pay[1] = {vals}, *ptmp[1] = {vtmp};
-    return radix_sort_payload(keys, ktmp, pay, ptmp, 1, d_n, n_ub, key_bits, hist, digit_total, s, result_in_tmp);
+    return radix_sort_payload(keys, ktmp, pay, ptmp, 1, false, d_n, n_ub, key_bits, hist, digit_total, s,
+                              result_in_tmp);
 }
 
 hipError_t launch_lower_bound(const uint32_t *keys, const int *d_n, int n_host, int ncells, uint32_t *off,
@@ -843,19 +945,19 @@ hipError_t particle_step(const ParticleStepArgs &a, int step, ParticleBuffers &b
     const int n_ub = b.n_ub;
     const int A = a.side * a.side;
     const dim3 gN((n_ub + 255) / 256 > 0 ? (n_ub + 255) / 256 : 1), blk(256);
-    // 1. move (positions in place) and cell keys
-    hipLaunchKernelGGL((pow2_side(a.side) ? k_move<true> : k_move<false>), gN, blk, 0, s, b.x, b.y, b.id, b.key, b.d_n,
-                       step, p);
-    // 2. stable sort by cell key carrying the positions and ids; an odd number of passes
-    //    (an all-zero digit pass above the key is a stable copy) leaves the result in the
-    //    second buffers, from which the compaction writes back into the first ones
+    // 1. move: cell keys and the records {x, y, id}
+    hipLaunchKernelGGL((pow2_side(a.side) ? k_move<true> : k_move<false>), gN, blk, 0, s, b.x, b.y, b.id, b.key,
+                       (PayRec<3> *)b.recA, b.d_n, step, p);
+    // 2. stable sort by cell key carrying the records; an odd number of passes (an all-zero
+    //    digit pass above the key is a stable copy) leaves the result in the second buffers,
+    //    from which the compaction writes the list back
     int bits = a.key_bits;
     if ((((bits + 7) / 8) & 1) == 0) bits += 8;
     bool in_tmp = false;
-    uint32_t *pay[3] = {(uint32_t *)b.x, (uint32_t *)b.y, b.id};
-    uint32_t *ptmp[3] = {(uint32_t *)b.x2, (uint32_t *)b.y2, b.id2};
-    hipError_t e = radix_sort_payload(b.key, b.key2, pay, ptmp, 3, b.d_n, n_ub, bits, b.hist, b.digit_total, s,
-                                      &in_tmp);
+    uint32_t *pay[1] = {b.recA};
+    uint32_t *ptmp[1] = {b.recB};
+    hipError_t e = radix_sort_payload(b.key, b.key2, pay, ptmp, 3, true, b.d_n, n_ub, bits, b.hist, b.digit_total,
+                                      s, &in_tmp);
     if (e != hipSuccess) return e;
     if (!in_tmp) return hipErrorInvalidValue;
     // 3. the sorted list's offsets; per cell: A+A->0 draws and survivors with their in-tile
@@ -864,15 +966,15 @@ hipError_t particle_step(const ParticleStepArgs &a, int step, ParticleBuffers &b
     //    to d_n2
     const int tiles = (A + 255) / 256;
     const uint32_t tper = (uint32_t)(tiles + 7) / 8;
-    hipLaunchKernelGGL(k_offsets_sorted, dim3(n_ub / 256 + 1), blk, 0, s, b.key2, b.d_n, A, b.off_sorted);
+    hipLaunchKernelGGL(k_offsets_sorted, dim3(n_ub / 1024 + 1), blk, 0, s, b.key2, b.d_n, A, b.off_sorted);
     hipLaunchKernelGGL(k_rs_cells, dim3(8 * tper), blk, 0, s, b.off_sorted, A, step, p, b.rec, b.tile_s,
                        (uint32_t)tiles, tper);
     hipLaunchKernelGGL(k_coh_scan, dim3(1), dim3(1024), 0, s, b.tile_s, tiles, b.d_n, b.d_n2, b.processed);
     // 4. stable compaction back into the primary buffers
     const int cblocks = std::max((n_ub + SC_SPAN - 1) / SC_SPAN, 1);
     const uint32_t cper = (uint32_t)(cblocks + 7) / 8;
-    hipLaunchKernelGGL(k_rs_compact, dim3(8 * cper), dim3(SC_BLOCK), 0, s, b.key2, b.x2, b.y2, b.id2, b.off_sorted,
-                       b.rec, b.tile_s, b.d_n, step, p, b.x, b.y, b.id, b.key_out, (uint32_t)b.capacity, cper);
+    hipLaunchKernelGGL(k_rs_compact, dim3(8 * cper), dim3(SC_BLOCK), 0, s, b.key2, (const PayRec<3> *)b.recB,
+                       b.off_sorted, b.rec, b.tile_s, b.d_n, step, p, b.x, b.y, b.id, (uint32_t)b.capacity, cper);
     return hipGetLastError();
 }
 

@@ -17,8 +17,9 @@ struct ParticleStepArgs {
 
 // Device buffers of a particle system (capacity >= every n it holds).
 struct ParticleBuffers {
-    float *x, *y, *x2, *y2;
-    uint32_t *id, *id2, *key, *key2, *key_out, *hist, *digit_total;
+    float *x, *y;                  // the list (read by the move, written by the compaction)
+    uint32_t *id, *key, *key2, *hist, *digit_total;
+    uint32_t *recA, *recB;         // {x, y, id} records carried by the sort, 3 words per particle
     uint4 *rec;                    // per cell: survivors before it in its tile, particles, kill mask, 0
     uint32_t *tile_s;              // per 256-cell tile: survivors, then their scanned offsets
     uint32_t *off_sorted;          // offsets of the sorted list before the annihilation
@@ -54,8 +55,8 @@ int radix_tiles(int n);
 size_t far_entry_bytes();
 hipError_t particle_step_sorted(const ParticleStepArgs &a, int step, const SortedStepBuffers &b, hipStream_t s);
 hipError_t radix_sort_payload(uint32_t *keys, uint32_t *ktmp, uint32_t *const *pay, uint32_t *const *ptmp, int np,
-                              const int *d_n, int n_ub, int key_bits, uint32_t *hist, uint32_t *digit_total,
-                              hipStream_t s, bool *result_in_tmp);
+                              bool aos, const int *d_n, int n_ub, int key_bits, uint32_t *hist,
+                              uint32_t *digit_total, hipStream_t s, bool *result_in_tmp);
 hipError_t radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *ktmp, uint32_t *vtmp, const int *d_n,
                             int n_ub, int key_bits, uint32_t *hist, uint32_t *digit_total, hipStream_t s,
                             bool *result_in_tmp);
