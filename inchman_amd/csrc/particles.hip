@@ -313,18 +313,35 @@ __device__ __forceinline__ void move_particle(float xo, float yo, uint32_t idv, 
     yn = wrap_pos<POW2>(yo + vy * p.dt + p.sigma * gy, p);
 }
 
-// Move, writing the cell key and the record {x, y, id} that the sort carries.
+// Move, writing the cell key and the record {x, y, id} that the sort carries, and the sort's
+// first-pass histogram (low 8 bits of the key) of the block's tile: one block per tile.
+constexpr int MV_THREADS = 1024;
+
 template <bool POW2>
-__global__ void k_move(const float *__restrict__ x, const float *__restrict__ y, const uint32_t *__restrict__ id,
-                       uint32_t *__restrict__ keys, PayRec<3> *__restrict__ recs, const int *d_n, int step,
-                       PartConsts p) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *d_n) return;
-    float xn, yn;
-    const uint32_t idv = id[i];
-    move_particle<POW2>(x[i], y[i], idv, step, p, xn, yn);
-    keys[i] = (uint32_t)((int)yn * p.side + (int)xn);
-    recs[i] = PayRec<3>{{__float_as_uint(xn), __float_as_uint(yn), idv}};
+__global__ __launch_bounds__(MV_THREADS) void k_move(const float *__restrict__ x, const float *__restrict__ y,
+                                                     const uint32_t *__restrict__ id, uint32_t *__restrict__ keys,
+                                                     PayRec<3> *__restrict__ recs, const int *d_n, int step,
+                                                     PartConsts p, uint32_t *__restrict__ hist, int ntiles) {
+    __shared__ uint32_t cnt[256];
+    const int n = *d_n;
+    if (threadIdx.x < 256) cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int j = 0; j < RS_TILE / MV_THREADS; ++j) {
+        const int i = base + j * MV_THREADS + (int)threadIdx.x;
+        if (i < n) {
+            float xn, yn;
+            const uint32_t idv = id[i];
+            move_particle<POW2>(x[i], y[i], idv, step, p, xn, yn);
+            const uint32_t k = (uint32_t)((int)yn * p.side + (int)xn);
+            keys[i] = k;
+            recs[i] = PayRec<3>{{__float_as_uint(xn), __float_as_uint(yn), idv}};
+            atomicAdd(&cnt[k & 255u], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 256) hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
 // A+A->0 draws: pairs 2m and 2m+1 of cell c take the two words of the Threefry block
@@ -770,9 +787,10 @@ __global__ __launch_bounds__(256) void k_rs_cells(const uint32_t *__restrict__ o
 }
 
 // the survivors of the sorted list to their places (stable compaction), flat over the list:
-// a particle's rank in its cell is its index less the cell's first index
+// a particle's rank in its cell is its index less the cell's first index; its cell comes from
+// its position as the move computed its key
 __global__ __launch_bounds__(SC_BLOCK) void k_rs_compact(
-    const uint32_t *__restrict__ key, const PayRec<3> *__restrict__ recs, const uint32_t *__restrict__ off_sorted,
+    const PayRec<3> *__restrict__ recs, const uint32_t *__restrict__ off_sorted,
     const uint4 *__restrict__ rec, const uint32_t *__restrict__ tile_off, const int *d_n, int step, PartConsts p,
     float *__restrict__ xo, float *__restrict__ yo, uint32_t *__restrict__ ido, uint32_t cap, uint32_t per) {
     const uint32_t n = (uint32_t)*d_n;
@@ -785,12 +803,12 @@ __global__ __launch_bounds__(SC_BLOCK) void k_rs_compact(
     for (int q = 0; q < SC_PER; ++q) {
         const uint32_t i = b0 + q * SC_BLOCK + threadIdx.x;
         const bool v = i < n;
-        c[q] = v ? key[i] : 0u;
         PayRec<3> r{};
         if (v) r = recs[i];
         xv[q] = __uint_as_float(r.w[0]);
         yv[q] = __uint_as_float(r.w[1]);
         idv[q] = r.w[2];
+        c[q] = v ? (uint32_t)((int)yv[q] * p.side + (int)xv[q]) : 0u;
     }
     uint32_t s0[SC_PER], T[SC_PER], m[SC_PER], base[SC_PER];
 #pragma unroll
@@ -835,10 +853,11 @@ int radix_tiles(int n) { return (n + RS_TILE - 1) / RS_TILE; }
 
 // LSD passes of 8-bit digits over key_bits bits, ping-ponging the keys and np payload arrays
 // (aos: pay[0] and ptmp[0] are arrays of np-word records) between the given and the tmp
-// buffers; *result_in_tmp tells where the result is.
+// buffers; *result_in_tmp tells where the result is.  hist0_ready: the first pass's tile
+// histograms are already in hist.
 hipError_t radix_sort_payload(uint32_t *keys, uint32_t *ktmp, uint32_t *const *pay, uint32_t *const *ptmp, int np,
                               bool aos, const int *d_n, int n_ub, int key_bits, uint32_t *hist,
-                              uint32_t *digit_total, hipStream_t s, bool *result_in_tmp) {
+                              uint32_t *digit_total, hipStream_t s, bool *result_in_tmp, bool hist0_ready) {
     const int narr = aos ? 1 : np;
     const int nt = std::max(radix_tiles(n_ub), 1);
     uint32_t *ki = keys, *ko = ktmp;
@@ -849,7 +868,8 @@ hipError_t radix_sort_payload(uint32_t *keys, uint32_t *ktmp, uint32_t *const *p
     }
     int passes = 0;
     for (int shift = 0; shift < key_bits; shift += 8) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(RS_THREADS), 0, s, ki, d_n, n_ub, shift, hist, nt);
+        if (shift > 0 || !hist0_ready)
+            hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(RS_THREADS), 0, s, ki, d_n, n_ub, shift, hist, nt);
         hipLaunchKernelGGL(k_radix_scan_rows, dim3(256), dim3(RS_THREADS), 0, s, hist, nt, digit_total);
         RadixPayload pl;
         for (int q = 0; q < 3; ++q) {
@@ -880,7 +900,7 @@ hipError_t radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *ktmp, uint
                             bool *result_in_tmp) {
     uint32_t *pay[1] = {vals}, *ptmp[1] = {vtmp};
     return radix_sort_payload(keys, ktmp, pay, ptmp, 1, false, d_n, n_ub, key_bits, hist, digit_total, s,
-                              result_in_tmp);
+                              result_in_tmp, false);
 }
 
 hipError_t launch_lower_bound(const uint32_t *keys, const int *d_n, int n_host, int ncells, uint32_t *off,
@@ -944,10 +964,11 @@ hipError_t particle_step(const ParticleStepArgs &a, int step, ParticleBuffers &b
     const PartConsts p = consts(a);
     const int n_ub = b.n_ub;
     const int A = a.side * a.side;
-    const dim3 gN((n_ub + 255) / 256 > 0 ? (n_ub + 255) / 256 : 1), blk(256);
-    // 1. move: cell keys and the records {x, y, id}
-    hipLaunchKernelGGL((pow2_side(a.side) ? k_move<true> : k_move<false>), gN, blk, 0, s, b.x, b.y, b.id, b.key,
-                       (PayRec<3> *)b.recA, b.d_n, step, p);
+    const dim3 blk(256);
+    // 1. move: cell keys, the records {x, y, id} and the first digit's tile histograms
+    const int nt = std::max(radix_tiles(n_ub), 1);
+    hipLaunchKernelGGL((pow2_side(a.side) ? k_move<true> : k_move<false>), dim3(nt), dim3(MV_THREADS), 0, s, b.x,
+                       b.y, b.id, b.key, (PayRec<3> *)b.recA, b.d_n, step, p, b.hist, nt);
     // 2. stable sort by cell key carrying the records; an odd number of passes (an all-zero
     //    digit pass above the key is a stable copy) leaves the result in the second buffers,
     //    from which the compaction writes the list back
@@ -957,7 +978,7 @@ hipError_t particle_step(const ParticleStepArgs &a, int step, ParticleBuffers &b
     uint32_t *pay[1] = {b.recA};
     uint32_t *ptmp[1] = {b.recB};
     hipError_t e = radix_sort_payload(b.key, b.key2, pay, ptmp, 3, true, b.d_n, n_ub, bits, b.hist, b.digit_total,
-                                      s, &in_tmp);
+                                      s, &in_tmp, true);
     if (e != hipSuccess) return e;
     if (!in_tmp) return hipErrorInvalidValue;
     // 3. the sorted list's offsets; per cell: A+A->0 draws and survivors with their in-tile
@@ -973,7 +994,7 @@ hipError_t particle_step(const ParticleStepArgs &a, int step, ParticleBuffers &b
     // 4. stable compaction back into the primary buffers
     const int cblocks = std::max((n_ub + SC_SPAN - 1) / SC_SPAN, 1);
     const uint32_t cper = (uint32_t)(cblocks + 7) / 8;
-    hipLaunchKernelGGL(k_rs_compact, dim3(8 * cper), dim3(SC_BLOCK), 0, s, b.key2, (const PayRec<3> *)b.recB,
+    hipLaunchKernelGGL(k_rs_compact, dim3(8 * cper), dim3(SC_BLOCK), 0, s, (const PayRec<3> *)b.recB,
                        b.off_sorted, b.rec, b.tile_s, b.d_n, step, p, b.x, b.y, b.id, (uint32_t)b.capacity, cper);
     return hipGetLastError();
 }

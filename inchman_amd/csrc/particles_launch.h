@@ -56,7 +56,7 @@ size_t far_entry_bytes();
 hipError_t particle_step_sorted(const ParticleStepArgs &a, int step, const SortedStepBuffers &b, hipStream_t s);
 hipError_t radix_sort_payload(uint32_t *keys, uint32_t *ktmp, uint32_t *const *pay, uint32_t *const *ptmp, int np,
                               bool aos, const int *d_n, int n_ub, int key_bits, uint32_t *hist,
-                              uint32_t *digit_total, hipStream_t s, bool *result_in_tmp);
+                              uint32_t *digit_total, hipStream_t s, bool *result_in_tmp, bool hist0_ready);
 hipError_t radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *ktmp, uint32_t *vtmp, const int *d_n,
                             int n_ub, int key_bits, uint32_t *hist, uint32_t *digit_total, hipStream_t s,
                             bool *result_in_tmp);
