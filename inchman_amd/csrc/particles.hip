@@ -663,7 +663,7 @@ __global__ __launch_bounds__(256) void k_cs_cells(const uint32_t *__restrict__ c
 //    16-byte load of its new cell's record.  A near particle's rank in its new cell is
 //    pre[d][c] (the record's for d = 4) plus its rank within (old cell, d) from the code word; a far one's counts the arrivals from the sources below its old cell.  Both
 //    add the far arrivals with a smaller index.
-constexpr int SC_BLOCK = 256, SC_PER = 4, SC_SPAN = SC_BLOCK * SC_PER;
+constexpr int SC_BLOCK = 256, SC_PER = 2, SC_SPAN = SC_BLOCK * SC_PER;
 
 __global__ __launch_bounds__(SC_BLOCK) void k_cs_scatter(
     const float *__restrict__ x2, const float *__restrict__ y2, const uint32_t *__restrict__ id,

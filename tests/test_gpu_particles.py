@@ -37,6 +37,28 @@ def test_sort_pairs(gpu, oracle, n, bits, hi):
     assert np.array_equal(_np_u32(tv), vs)
 
 
+@pytest.mark.parametrize("shift", [1, 2, 3])
+def test_sort_pairs_unaligned(gpu, oracle, shift):
+    """Keys that start 4, 8 or 12 bytes past a 16-byte boundary (a slice of a larger buffer):
+    the histogram's 16-byte loads are skipped for them, the order is the same."""
+    n = 3 * 4096 + 77
+    rng = np.random.default_rng(shift)
+    k = rng.integers(0, 1 << 20, n, dtype=np.uint64).astype(np.uint32)
+    v = np.arange(n, dtype=np.uint32)
+    big_k, big_v = _u32(np.zeros(n + 4, np.uint32)), _u32(np.zeros(n + 4, np.uint32))
+    big_k[shift:shift + n] = _u32(k)
+    big_v[shift:shift + n] = _u32(v)
+    tk, tv = big_k[shift:shift + n], big_v[shift:shift + n]
+    assert tk.data_ptr() % 16 == 4 * shift % 16
+    t1, t2 = torch.empty_like(tk), torch.empty_like(tv)
+    call("gpgmp_sort_pairs_device", tk.data_ptr(), tv.data_ptr(), t1.data_ptr(), t2.data_ptr(), n, 20, None)
+    torch.cuda.synchronize()
+    ks, vs = oracle.stable_sort_pairs(k, v)
+    assert np.array_equal(_np_u32(tk), ks)
+    assert np.array_equal(_np_u32(tv), vs)
+    assert np.array_equal(_np_u32(big_k[:shift]), np.zeros(shift, np.uint32))  # nothing written before the slice
+
+
 def test_offsets_left_right(gpu, oracle):
     rng = np.random.default_rng(3)
     ncells = 1 << 14
