@@ -209,3 +209,42 @@ def test_reference_nonlinear_examples():
     im = sbml.load(os.path.join(EXAMPLES, "Nonlinear.xml"))
     with pytest.raises(GpgmpError):
         sbml.method_dependencies(im)
+
+
+@pytest.mark.skipif(not os.path.isdir(EXAMPLES), reason="reference examples not present")
+def test_reference_drift_methods_match_the_c_compiler(oracle):
+    """Every example model's drift/diffusivity method, compiled by the library and evaluated on
+    the host over the model's own grid and initial state, equals the method text compiled by
+    g++ bit for bit; the two that cannot run say why: Nonlinear reads MeanX (the moments of
+    computeMoments models), RandomDrift has 26 species (the kernels hold at most 8)."""
+    from inchman_amd import GpgmpError
+    from oracle import lawgen
+
+    checked = []
+    for n in sorted(f for f in os.listdir(EXAMPLES) if f.endswith(".xml")):
+        im = sbml.load(os.path.join(EXAMPLES, n))
+        if not (im.drift_diffusivity_method or "").strip():
+            continue
+        if n == "Nonlinear.xml":
+            with pytest.raises(GpgmpError, match="MeanX"):
+                sbml.drift_diffusivity_fields(im, *_example_inputs(im), on_host=True)
+            continue
+        if n == "RandomDrift.xml":
+            with pytest.raises(ValueError, match="8 species"):
+                sbml.drift_diffusivity_fields(im, *_example_inputs(im), on_host=True)
+            continue
+        st, fields, cur = _example_inputs(im)
+        m = im.model
+        fp = np.stack([fields[f][: m.width * m.height].reshape(m.height, m.width) for f in im.field_parameters]) \
+            if im.field_parameters else np.zeros((0, m.height, m.width), np.float32)
+        want = lawgen.method_fields(m, im.drift_diffusivity_method, im.field_parameters, fp, st, cur)
+        got = np.stack(sbml.drift_diffusivity_fields(im, st, fields, cur, on_host=True))
+        assert got.tobytes() == want.tobytes(), n
+        checked.append(n)
+    assert len(checked) >= 8 and "Slit.xml" in checked and "ABAnnihilationDrift.xml" in checked
+
+
+def _example_inputs(im):
+    st, real, fields = sbml.initial_state(im)
+    m = im.model
+    return st, fields, np.zeros((4, len(m.species), m.height, m.width), np.float32)
