@@ -851,10 +851,10 @@ namespace gpgmp {
 
 int radix_tiles(int n) { return (n + RS_TILE - 1) / RS_TILE; }
 
-// LSD passes of 8-bit digits over key_bits bits, ping-ponging the keys and np payload arrays
-// (aos: pay[0] and ptmp[0] are arrays of np-word records) between the given and the tmp
-// buffers; *result_in_tmp tells where the result is.  hist0_ready: the first pass's tile
-// histograms are already in hist.
+// LSD passes of 8-bit digits over key_bits bits, ping-ponging the keys and their payload
+// between the given and the tmp buffers: one value array (np = 1), or one array of 3-word
+// records (np = 3, aos); *result_in_tmp tells where the result is.  hist0_ready: the first
+// pass's tile histograms are already in hist.
 hipError_t radix_sort_payload(uint32_t *keys, uint32_t *ktmp, uint32_t *const *pay, uint32_t *const *ptmp, int np,
                               bool aos, const int *d_n, int n_ub, int key_bits, uint32_t *hist,
                               uint32_t *digit_total, hipStream_t s, bool *result_in_tmp, bool hist0_ready) {
@@ -878,9 +878,6 @@ hipError_t radix_sort_payload(uint32_t *keys, uint32_t *ktmp, uint32_t *const *p
         }
         if (np == 1 && !aos)
             hipLaunchKernelGGL((k_radix_scatter<1, false>), dim3(nt), dim3(SX_THREADS), 0, s, ki, ko, pl, d_n, n_ub,
-                               shift, hist, digit_total, nt);
-        else if (np == 3 && !aos)
-            hipLaunchKernelGGL((k_radix_scatter<3, false>), dim3(nt), dim3(SX_THREADS), 0, s, ki, ko, pl, d_n, n_ub,
                                shift, hist, digit_total, nt);
         else if (np == 3 && aos)
             hipLaunchKernelGGL((k_radix_scatter<3, true>), dim3(nt), dim3(SX_THREADS), 0, s, ki, ko, pl, d_n, n_ub,
