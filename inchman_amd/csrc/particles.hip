@@ -313,6 +313,18 @@ __device__ __forceinline__ void move_particle(float xo, float yo, uint32_t idv, 
     yn = wrap_pos<POW2>(yo + vy * p.dt + p.sigma * gy, p);
 }
 
+// Element i of a per-particle array through a 32-bit byte offset (i < 2^28, the capacity
+// limit): the address is the array's scalar base plus a 32-bit vector offset, with no
+// 64-bit address arithmetic per access.
+template <typename T>
+__device__ __forceinline__ T &at(T *p, uint32_t i) {
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + (i << 2));
+}
+template <typename T>
+__device__ __forceinline__ const T &at(const T *p, uint32_t i) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + (i << 2));
+}
+
 // Move, writing the cell key and the record {x, y, id} that the sort carries, and the sort's
 // first-pass histogram (low 8 bits of the key) of the block's tile: one block per tile.
 constexpr int MV_THREADS = 1024;
@@ -332,10 +344,10 @@ __global__ __launch_bounds__(MV_THREADS) void k_move(const float *__restrict__ x
         const int i = base + j * MV_THREADS + (int)threadIdx.x;
         if (i < n) {
             float xn, yn;
-            const uint32_t idv = id[i];
-            move_particle<POW2>(x[i], y[i], idv, step, p, xn, yn);
+            const uint32_t idv = at(id, (uint32_t)i);
+            move_particle<POW2>(at(x, (uint32_t)i), at(y, (uint32_t)i), idv, step, p, xn, yn);
             const uint32_t k = (uint32_t)((int)yn * p.side + (int)xn);
-            keys[i] = k;
+            at(keys, (uint32_t)i) = k;
             recs[i] = PayRec<3>{{__float_as_uint(xn), __float_as_uint(yn), idv}};
             atomicAdd(&cnt[k & 255u], 1u);
         }
@@ -444,9 +456,9 @@ __global__ __launch_bounds__(CT) void k_cs_move(const float *__restrict__ x, con
         float xo = 0.f, yo = 0.f;
         uint32_t idv = 0u;
         if (valid) {
-            xo = x[i];
-            yo = y[i];
-            idv = id[i];
+            xo = at(x, i);
+            yo = at(y, i);
+            idv = at(id, i);
         }
         float xn, yn;
         move_particle<POW2>(xo, yo, idv, step, p, xn, yn);
@@ -470,9 +482,9 @@ __global__ __launch_bounds__(CT) void k_cs_move(const float *__restrict__ x, con
         // every read of the round precedes its increments (one wave: LDS operations in order)
         if (ranked) atomicAdd(&cr[d][lo], 1u);
         if (valid) {
-            x2[i] = xn;
-            y2[i] = yn;
-            code[i] = ranked ? (d | (rank << 4)) : ((uint32_t)kFar | (o << 4));
+            at(x2, i) = xn;
+            at(y2, i) = yn;
+            at(code, i) = ranked ? (d | (rank << 4)) : ((uint32_t)kFar | (o << 4));
             if (!near) {
                 const uint32_t c = (uint32_t)(cy * side + cx);
                 const uint32_t k = atomicAdd(far_cnt, 1u);
@@ -681,10 +693,10 @@ __global__ __launch_bounds__(SC_BLOCK) void k_cs_scatter(
     for (int q = 0; q < SC_PER; ++q) {
         const uint32_t i = b0 + q * SC_BLOCK + threadIdx.x;
         const bool v = i < n;
-        xv[q] = v ? x2[i] : 0.f;
-        yv[q] = v ? y2[i] : 0.f;
-        idv[q] = v ? id[i] : 0u;
-        cd[q] = v ? code[i] : (uint32_t)kFar;
+        xv[q] = v ? at(x2, i) : 0.f;
+        yv[q] = v ? at(y2, i) : 0.f;
+        idv[q] = v ? at(id, i) : 0u;
+        cd[q] = v ? at(code, i) : (uint32_t)kFar;
     }
     uint32_t c[SC_PER], pr[SC_PER], T[SC_PER], m[SC_PER], base[SC_PER];
 #pragma unroll
@@ -719,9 +731,9 @@ __global__ __launch_bounds__(SC_BLOCK) void k_cs_scatter(
         uint32_t pos;
         if (!survivor_place(r, T[q], m[q], base[q], c[q], step, p, pos)) continue;
         if (pos >= cap) continue;  // never for a consistent list; keeps every store inside the buffers
-        xo[pos] = xv[q];
-        yo[pos] = yv[q];
-        ido[pos] = idv[q];
+        at(xo, pos) = xv[q];
+        at(yo, pos) = yv[q];
+        at(ido, pos) = idv[q];
     }
 }
 
@@ -804,7 +816,7 @@ __global__ __launch_bounds__(SC_BLOCK) void k_rs_compact(
         const uint32_t i = b0 + q * SC_BLOCK + threadIdx.x;
         const bool v = i < n;
         PayRec<3> r{};
-        if (v) r = recs[i];
+        if (v) r = *reinterpret_cast<const PayRec<3> *>(reinterpret_cast<const char *>(recs) + i * 12u);
         xv[q] = __uint_as_float(r.w[0]);
         yv[q] = __uint_as_float(r.w[1]);
         idv[q] = r.w[2];
@@ -828,9 +840,9 @@ __global__ __launch_bounds__(SC_BLOCK) void k_rs_compact(
         uint32_t pos;
         if (!survivor_place(i - s0[q], T[q], m[q], base[q], c[q], step, p, pos)) continue;
         if (pos >= cap) continue;  // never for a consistent list; keeps every store inside the buffers
-        xo[pos] = xv[q];
-        yo[pos] = yv[q];
-        ido[pos] = idv[q];
+        at(xo, pos) = xv[q];
+        at(yo, pos) = yv[q];
+        at(ido, pos) = idv[q];
     }
 }
 
