@@ -128,6 +128,8 @@ SIGNATURES = {
     "gpgmp_solver_do_reactions": [_P, _F, _pI],
     "gpgmp_solver_diffuse": [_P, _I, _F, _pI],
     "gpgmp_solver_next_diffusion_time": [_P, _pF],
+    "gpgmp_solver_set_nonlinear_method": [_P, C.c_char_p, C.POINTER(C.c_char_p), _I, _pF],
+    "gpgmp_loop_seeds": [C.c_uint32, _I, _pI],
     "gpgmp_solver_time_steps": [_P, _pF, _pF],
     "gpgmp_solver_run": [_P, _I, _U, _pF],
     "gpgmp_solver_set_output": [_P, _F, _F, _I],

@@ -72,6 +72,14 @@ struct gpgmp_solver {
         gpgmp_new_individual_rule rule;
     };
     std::map<int, Individuals> inds;
+    // non-linear diffusivity (InhomogeneousSolver.cpp:304-336,384-417): the drift/diffusivity
+    // method's programs, its field parameters [nfields][A], and the fields [4][S][A] (cur:
+    // DiffusivityX, DiffusivityY, DriftX, DriftY as the sweeps use them; out: the method's)
+    struct Nonlinear {
+        DevBuf<uint32_t> code, tab;
+        DevBuf<float> params, cur, out;
+    };
+    std::unique_ptr<Nonlinear> nl;
     void record_dump(float t) {
         const size_t n = eng->state_elems();
         if ((int32_t)dump_times.size() < max_dumps)
@@ -465,6 +473,92 @@ int gpgmp_solver_individuals(gpgmp_solver *s, int32_t species, gpgmp_individuals
     });
 }
 
+// After a sweep of a species on one axis of a non-linear model, as InhomogeneousSolver::diffuse
+// does (InhomogeneousSolver.cpp:304-336 after X, :384-417 after Y): the method re-evaluated on
+// the swept state at the current time gives the species' four fields (the other species keep
+// theirs); the swept axis' time step becomes the minimum admissible step over the cells
+// (computeIndividualTimestepX/Y + reduceBlocks, as at set-up), and the sweeps of both axes
+// take their probabilities from the current fields and time steps, which the reference's
+// diffuseX/Y kernels compute at sweep time.
+static void nonlinear_update(gpgmp_solver *s, int sp, int axis, float t) {
+    const gpgmp_model_desc &d = s->m.desc;
+    const int W = d.width, H = d.height, S = d.num_species;
+    const size_t A = (size_t)W * H;
+    hipStream_t st = s->eng->stream();
+    gpgmp_solver::Nonlinear &nl = *s->nl;
+    LawExtra ex{s->eng->state(), nl.params.get(), nl.cur.get(), W, H, S, 0.f, 0.f, t};
+    const float pm = header_float(d.physical_length);
+    GPGMP_HIP_CHECK(launch_drift_fields(nl.code.get(), nl.tab.get(), ex, pm, pm, nl.out.get(), st));
+    std::vector<float> *host[4] = {&s->m.Dx, &s->m.Dy, &s->m.rx, &s->m.ry};
+    for (int f = 0; f < 4; ++f) {
+        const size_t off = ((size_t)f * S + sp) * A;
+        GPGMP_HIP_CHECK(hipMemcpyAsync(nl.cur.get() + off, nl.out.get() + off, A * sizeof(float),
+                                       hipMemcpyDeviceToDevice, st));
+        GPGMP_HIP_CHECK(hipMemcpyAsync(host[f]->data() + (size_t)sp * A, nl.out.get() + off, A * sizeof(float),
+                                       hipMemcpyDeviceToHost, st));
+    }
+    GPGMP_HIP_CHECK(hipStreamSynchronize(st));
+    const float *Dx = s->m.Dx.data() + (size_t)sp * A, *Dy = s->m.Dy.data() + (size_t)sp * A;
+    const float *rx = s->m.rx.data() + (size_t)sp * A, *ry = s->m.ry.data() + (size_t)sp * A;
+    const float *D = axis ? Dy : Dx, *r = axis ? ry : rx;
+    float m = INFINITY;
+    for (size_t c = 0; c < A; ++c) m = std::min(m, inhomogeneous_cell_dt(D[c], r[c], s->m.cell_width));
+    (axis ? s->dty : s->dtx)[sp] = m;
+    s->eng->set_probs(0, sp, 0, Dx, rx, s->dtx[sp], s->m.cell_width);
+    s->eng->set_probs(0, sp, 1, Dy, ry, s->dty[sp], s->m.cell_height);
+}
+
+int gpgmp_loop_seeds(uint32_t seed0, int32_t n, int32_t *out) {
+    return guarded([&] {
+        GPGMP_REQUIRE(n >= 0 && (n == 0 || out), GPGMP_E_INVALID, "bad argument");
+        SeedStream ss(seed0);
+        for (int32_t i = 0; i < n; ++i) out[i] = ss.next();
+    });
+}
+
+int gpgmp_solver_set_nonlinear_method(gpgmp_solver *s, const char *method, const char *const *field_names,
+                                      int32_t nfields, const float *fields) {
+    return guarded([&] {
+        GPGMP_REQUIRE(s && method, GPGMP_E_INVALID, "null argument");
+        GPGMP_REQUIRE(nfields >= 0 && (nfields == 0 || (field_names && fields)), GPGMP_E_INVALID,
+                      "fields need their names and values");
+        GPGMP_REQUIRE(s->eng && s->m.desc.solver_type == GPGMP_STOCHASTIC_INHOMOGENEOUS, GPGMP_E_UNSUPPORTED,
+                      "non-linear diffusivity needs the stochastic inhomogeneous solver");
+        const gpgmp_model_desc &d = s->m.desc;
+        const int S = d.num_species;
+        const size_t A = (size_t)d.width * d.height;
+        LawEnv env = law_env(s->m, false);
+        for (int k = 0; k < nfields; ++k) env.fields.push_back(field_names[k]);
+        const std::vector<LawProgram> progs = compile_method(method, env);
+        std::vector<uint32_t> code, table;
+        for (const LawProgram &p : progs) {
+            GPGMP_REQUIRE(code.size() + p.code.size() <= (size_t)1 << 20, GPGMP_E_CAPACITY, "method too long");
+            table.push_back((uint32_t)code.size());
+            table.push_back((uint32_t)p.type);
+            code.insert(code.end(), p.code.begin(), p.code.end());
+        }
+        std::unique_ptr<gpgmp_solver::Nonlinear> nl(new gpgmp_solver::Nonlinear());
+        hipStream_t st = s->eng->stream();
+        nl->code.alloc(std::max<size_t>(code.size(), 1));
+        nl->tab.alloc(std::max<size_t>(table.size(), 1));
+        nl->params.alloc(std::max<size_t>((size_t)nfields * A, 1));
+        nl->cur.alloc((size_t)4 * S * A);
+        nl->out.alloc((size_t)4 * S * A);
+        GPGMP_HIP_CHECK(hipMemcpyAsync(nl->code.get(), code.data(), code.size() * 4, hipMemcpyHostToDevice, st));
+        GPGMP_HIP_CHECK(hipMemcpyAsync(nl->tab.get(), table.data(), table.size() * 4, hipMemcpyHostToDevice, st));
+        if (nfields > 0)
+            GPGMP_HIP_CHECK(hipMemcpyAsync(nl->params.get(), fields, (size_t)nfields * A * 4, hipMemcpyHostToDevice,
+                                           st));
+        const std::vector<float> *host[4] = {&s->m.Dx, &s->m.Dy, &s->m.rx, &s->m.ry};
+        for (int f = 0; f < 4; ++f)
+            GPGMP_HIP_CHECK(hipMemcpyAsync(nl->cur.get() + (size_t)f * S * A, host[f]->data(), (size_t)S * A * 4,
+                                           hipMemcpyHostToDevice, st));
+        GPGMP_HIP_CHECK(hipStreamSynchronize(st));
+        s->nl = std::move(nl);
+        s->runner.reset();  // the device loop does not re-evaluate the method
+    });
+}
+
 int gpgmp_solver_diffuse(gpgmp_solver *s, int32_t species, float t, int32_t *seed) {
     return guarded([&] {
         GPGMP_REQUIRE(s && seed, GPGMP_E_INVALID, "null argument");
@@ -515,6 +609,7 @@ int gpgmp_solver_diffuse(gpgmp_solver *s, int32_t species, float t, int32_t *see
             g.gather_mask = bit;
             api_launch(s, 2, g, *seed, *seed, s->eng->lost(0), s->eng->lost(1));
             *seed = (int32_t)((uint32_t)*seed + 1u);
+            if (s->nl) nonlinear_update(s, species, 0, t);
             s->nx[species] += s->dtx[species];
         }
         if (s->ny[species] <= t) {
@@ -525,6 +620,7 @@ int gpgmp_solver_diffuse(gpgmp_solver *s, int32_t species, float t, int32_t *see
             g.gather_mask = bit;
             api_launch(s, 1, g, *seed, *seed, s->eng->lost(1), s->eng->lost(0));
             *seed = (int32_t)((uint32_t)*seed + 1u);
+            if (s->nl) nonlinear_update(s, species, 1, t);
             s->ny[species] += s->dty[species];
         }
     });
@@ -564,6 +660,8 @@ int gpgmp_solver_run(gpgmp_solver *s, int32_t steps, uint32_t seed0, float *sim_
         }
         GPGMP_REQUIRE(s->inds.empty(), GPGMP_E_UNSUPPORTED,
                       "models with individual species run through the per-call API (do_reactions / diffuse)");
+        GPGMP_REQUIRE(!s->nl, GPGMP_E_UNSUPPORTED,
+                      "non-linear diffusivity runs through the per-call API (do_reactions / diffuse)");
         s->runner.reset();
         s->runner.reset(new GmpRunner(*s->eng, s->dtx, s->dty, {seed0}));
         s->runner_seed0 = seed0;

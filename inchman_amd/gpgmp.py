@@ -221,12 +221,23 @@ class DiffusionModel:
         return act
 
     def createSolver(self, stream=None, flags=0):
-        """Factory of DiffusionModel::oclSolve (DiffusionModel.cpp:1422-1440)."""
+        """Factory of DiffusionModel::oclSolve (DiffusionModel.cpp:1422-1440).  A model given a
+        non-linear drift/diffusivity method (setNonlinearMethod) gets it attached."""
         cm = self.compile()
         if self.solver in DETERMINISTIC_TYPES:
             return DeterministicSolver(cm, stream, flags)
         cls = HomogeneousSolver if self.solver == STOCHASTIC_HOMOGENEOUS else InhomogeneousSolver
-        return cls(cm, stream)
+        s = cls(cm, stream)
+        nl = getattr(self, "nonlinear_method", None)
+        if nl is not None:
+            s.setNonlinearMethod(*nl)
+        return s
+
+    def setNonlinearMethod(self, method, field_names=(), fields=None):
+        """m_nonlinearDiffusivity: the drift/diffusivity method re-evaluated after every sweep
+        (InhomogeneousSolver.cpp:304-336,384-417) by the solvers createSolver makes from now
+        on; fields: [len(field_names)][H][W] values of the field parameters."""
+        self.nonlinear_method = (method, tuple(field_names), fields)
 
 
 class CompiledModel:
@@ -257,6 +268,13 @@ def _stream_ptr(stream):
     if isinstance(stream, int):
         return stream
     return int(stream.cuda_stream)  # torch.cuda.Stream
+
+
+def loopSeeds(seed0, n):
+    """seed_n of the host loop's first n iterations from seed0 (DiffusionModel.cpp:1467-1482)."""
+    out = np.zeros(max(int(n), 1), np.int32)
+    call("gpgmp_loop_seeds", C.c_uint32(seed0), int(n), _iptr(out))
+    return out[: int(n)]
 
 
 class Solver:
@@ -312,6 +330,40 @@ class Solver:
         s = C.c_int32(seed)
         call("gpgmp_solver_diffuse", self.handle, int(species), C.c_float(current_time), C.byref(s))
         return s.value
+
+    def setNonlinearMethod(self, method, field_names=(), fields=None):
+        """Non-linear diffusivity (gpgmp_solver_set_nonlinear_method): after every sweep that
+        diffuse runs, the method on the swept counts at the call's time gives the species' fields,
+        the swept axis' time step and both axes' probabilities.  The model then runs through
+        runPerCall (run refuses it)."""
+        H, W = self.model.height, self.model.width
+        names = list(field_names)
+        fv = np.zeros((max(len(names), 1), H, W), np.float32)
+        if names:
+            fv[: len(names)] = np.asarray(fields, np.float32).reshape(len(names), H, W)
+        cnames = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+        call("gpgmp_solver_set_nonlinear_method", self.handle, method.encode(), cnames, len(names), _fptr(fv))
+
+    def runPerCall(self, steps, seed0, sim_time=0.0):
+        """DiffusionModel::oclSolve's loop (DiffusionModel.cpp:1479-1603) through doReactions,
+        diffuse and getNextDiffusionTime, one call each per step, for what the device loop does
+        not run (non-linear diffusivity, individual species): from the clock sim_time, seeds
+        seed_n of seed0 (loopSeeds).  Returns the clock."""
+        seeds = loopSeeds(seed0, steps)
+        sim = np.float32(sim_time)
+        react = len(self.model.reactions) > 0
+        for it in range(steps):
+            nxt = self.getNextDiffusionTime()
+            minspec = int(np.argmin(nxt))  # the first minimum
+            mindt = nxt[minspec]
+            seed = int(seeds[it])
+            if react:
+                seed = self.doReactions(np.float32(mindt - sim), seed)
+            for i in range(len(nxt)):
+                if (i == minspec or nxt[i] <= mindt) and nxt[i] > 0:
+                    seed = self.diffuse(i, mindt, seed)
+            sim = mindt
+        return float(sim)
 
     # Solver::getNextDiffusionTime()
     def getNextDiffusionTime(self):

@@ -495,16 +495,21 @@ def method_dependencies(im):
 
 def check_nonlinear(im):
     """A non-linear model (fields recomputed after every sweep, InhomogeneousSolver.cpp:304-336,
-    384-417) runs when its method reads none of what a sweep changes -- the counts, the clock,
-    the current fields: re-evaluating it then reproduces the first fields and time steps.
-    Otherwise raises NotImplementedError with what it reads."""
+    384-417): True when its method reads what a sweep changes -- the counts, the clock, the
+    current fields -- so that the solver must re-evaluate it (the stochastic inhomogeneous
+    solver does, gpgmp_solver_set_nonlinear_method); False when re-evaluating it reproduces
+    the first fields and time steps.  Other solver types with such a method raise
+    NotImplementedError with what it reads."""
     if not im.nonlinear_diffusivity:
-        return
+        return False
     deps = method_dependencies(im)
-    if deps:
-        what = [n for b, n in ((1, "the counts"), (2, "PhysicalSimTime"), (4, "the current fields")) if deps & b]
-        raise NotImplementedError("non-linear diffusivity whose method reads %s (fields recomputed after every "
-                                  "sweep) is not supported" % " and ".join(what))
+    if not deps:
+        return False
+    if im.model.solver == STOCHASTIC_INHOMOGENEOUS:
+        return True
+    what = [n for b, n in ((1, "the counts"), (2, "PhysicalSimTime"), (4, "the current fields")) if deps & b]
+    raise NotImplementedError("non-linear diffusivity whose method reads %s (fields recomputed after every "
+                              "sweep) is supported by the stochastic inhomogeneous solver only" % " and ".join(what))
 
 
 class _Affine:
@@ -645,11 +650,16 @@ def build(im, run_init_script=True):
     m = im.model
     if m.solver in (STOCHASTIC_INHOMOGENEOUS, STOCHASTIC_INHOMOGENEOUS_FPE, DETERMINISTIC_INHOMOGENEOUS_RK4,
                     DETERMINISTIC_INHOMOGENEOUS_AQSS):
-        check_nonlinear(im)
+        recompute = check_nonlinear(im)
         S, H, W = m.numSpecies(), m.height, m.width
         cur = np.zeros((4, S, H, W), np.float32)
         for s, sp in enumerate(m.species):  # the species' constant as the starting value
             cur[0, s] = cur[1, s] = sp.diffusionConstant
         Dx, Dy, rx, ry = drift_diffusivity_fields(im, state, fields, cur)
         m.setDriftDiffusivityFields(Dx, Dy, rx, ry)
+        if recompute:  # the solvers re-evaluate the method after every sweep (runPerCall)
+            names = list(im.field_parameters)
+            fv = np.stack([np.asarray(fields[n], np.float32).reshape(-1)[: H * W].reshape(H, W) for n in names]) \
+                if names else None
+            m.setNonlinearMethod(im.drift_diffusivity_method, names, fv)
     return state, real, fields

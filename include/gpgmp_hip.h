@@ -221,6 +221,20 @@ int gpgmp_solver_diffuse(gpgmp_solver *s, int32_t species, float t, int32_t *see
 /* Solver::getNextDiffusionTime() (Solver.h:68; HomogeneousSolver.h:42,
  * InhomogeneousSolver.cpp:238-252): out[num_species]. */
 int gpgmp_solver_next_diffusion_time(gpgmp_solver *s, float *out);
+/* The seed_n of the host loop's first n iterations from seed0 (DiffusionModel.cpp:1467-1482:
+ * boost::mt19937(seed0), uniform_int<>(0, INT_MAX), plus the iteration number), for a loop
+ * written over the per-call API. */
+int gpgmp_loop_seeds(uint32_t seed0, int32_t n, int32_t *out);
+/* Non-linear diffusivity (DiffusionModel m_nonlinearDiffusivity; InhomogeneousSolver.cpp:304-336,
+ * 384-417, the generated computeDiffusionConstants_forSpecies kernels of Solver.cpp:659-777):
+ * the drift/diffusivity method (the language of gpgmp_model_eval_drift_diffusivity, with its
+ * field parameters, nfields x [H][W] floats) is re-evaluated after every sweep that
+ * gpgmp_solver_diffuse runs, on the swept counts at the call's time t: the species' four fields,
+ * then the swept axis' time step and both axes' sweep probabilities.  Stochastic
+ * inhomogeneous solvers only; the model then runs through the per-call API
+ * (gpgmp_solver_run refuses it). */
+int gpgmp_solver_set_nonlinear_method(gpgmp_solver *s, const char *method, const char *const *field_names,
+                                      int32_t nfields, const float *fields);
 /* Makes `species` an individual (particle-based) species of this stochastic inhomogeneous
  * solver (Solver(..., hasIndividualSpecies), IndividualSolver.cpp:37-130): its counts in the
  * state become the particles' (gpgmp_individuals_bind_counts), gpgmp_solver_do_reactions

@@ -6,7 +6,7 @@ import os
 import numpy as np
 import pytest
 
-from inchman_amd import sbml
+from inchman_amd import GpgmpError, sbml
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -52,7 +52,7 @@ def test_nonlinear_model_with_fixed_fields_runs(gpu, oracle):
     """A non-linear model whose method reads only positions, parameters and field parameters
     (as the reference's OrnsteinUhlenbeck and MultiplicativeNoise examples): its fields after
     any sweep are the first ones, so it builds and runs as the oracle's loop over them; one that
-    reads the counts is refused."""
+    reads the counts gets the re-evaluating solver, which the device loop refuses."""
     im = sbml.load(os.path.join(GOLDEN, "inchman_drift.xml"))
     im.nonlinear_diffusivity = True
     im.drift_diffusivity_method = ("All->DiffusivityX = 0.8 + 0.2 * gradient;\nAll->DiffusivityY = 0.6;\n"
@@ -70,5 +70,86 @@ def test_nonlinear_model_with_fixed_fields_runs(gpu, oracle):
     assert t == tw
     assert np.array_equal(s.synchronizeStateBufferFromDevice(), want)
     im.drift_diffusivity_method = "All->DriftX = 0.001 * All->State;"
-    with pytest.raises(NotImplementedError):
-        sbml.build(im)
+    st, real, fields = sbml.build(im)  # reads the counts: re-evaluated after every sweep
+    s2 = im.model.createSolver()
+    with pytest.raises(GpgmpError):
+        s2.run(5, 9)  # the device loop does not re-evaluate it: the per-call loop does
+
+
+def _inc(seed):
+    """seed + 1 with int32 wrap-around, as the reference's int seed."""
+    return (seed + 1 + 2 ** 31) % 2 ** 32 - 2 ** 31
+
+
+def test_nonlinear_recompute_matches_oracle_loop(gpu, oracle):
+    """Non-linear diffusivity whose method reads the counts (the drift fixture's: a central
+    difference of the counts over a crowding term): DiffusionModel::oclSolve's loop through the
+    solver calls, with the method re-evaluated after every sweep on the swept counts at the
+    current time (InhomogeneousSolver.cpp:304-336,384-417), against the same loop over the
+    oracle with the g++-compiled method: counts, clock, next diffusion times and time steps
+    bit-exact after every iteration; then runPerCall over the same steps."""
+    from oracle import lawgen
+
+    im = sbml.load(os.path.join(GOLDEN, "inchman_drift.xml"))
+    im.nonlinear_diffusivity = True
+    st, real, fields = sbml.build(im)
+    m = im.model
+    S, H, W = m.numSpecies(), m.height, m.width
+    names = list(im.field_parameters)
+    fp = np.stack([np.asarray(fields[n], np.float32).reshape(-1)[: H * W].reshape(H, W) for n in names])
+    sol = m.createSolver()
+    sol.synchronizeStateBufferToDevice(st)
+    # oracle: the build's fields, the time steps as set up, the reference's loop
+    Dx, Dy, rx, ry = [np.array(a, np.float32).reshape(S, H, W) for a in m.fields]
+    h = oracle.header_float(m.physical_length / W)
+    dtx = [np.float32(oracle.inhomogeneous_dt(Dx[i], rx[i], h)) for i in range(S)]
+    dty = [np.float32(oracle.inhomogeneous_dt(Dy[i], ry[i], h)) for i in range(S)]
+    gx, gy = sol.timeSteps()
+    assert list(gx) == dtx and list(gy) == dty
+    nx, ny = list(dtx), list(dty)
+    table = oracle.table_from_model(m)
+    steps, seed0 = 30, 11
+    seeds = oracle.seeds(seed0, steps)
+    ost = st.copy()
+    sim = np.float32(0)
+    changed = False
+    for it in range(steps):
+        seed = int(seeds[it])
+        nxt = sol.getNextDiffusionTime()
+        assert list(nxt) == [np.float32(min(nx[i], ny[i])) for i in range(S)], it
+        minspec = int(np.argmin(nxt))
+        mindt = nxt[minspec]
+        deltat = np.float32(mindt - sim)
+        g_seed = sol.doReactions(deltat, seed)
+        ost = oracle.ssa(ost, table, seed, deltat)
+        seed = _inc(seed)
+        assert g_seed == seed
+        for i in range(S):
+            if not ((i == minspec or nxt[i] <= mindt) and nxt[i] > 0):
+                continue
+            g_seed = sol.diffuse(i, mindt, g_seed)
+            for axis, nn, dd in ((0, nx, dtx), (1, ny, dty)):
+                if not (nn[i] <= mindt):
+                    continue
+                ost = oracle.inhom_sweep(ost, i, axis, seed, dd[i], Dx if axis == 0 else Dy, rx if axis == 0 else ry, h)
+                seed = _inc(seed)
+                new = lawgen.method_fields(m, im.drift_diffusivity_method, names, fp, ost,
+                                           np.stack([Dx, Dy, rx, ry]), sim_time=mindt)
+                before = rx[i].copy()
+                Dx[i], Dy[i], rx[i], ry[i] = new[0][i], new[1][i], new[2][i], new[3][i]
+                changed |= not np.array_equal(before, rx[i])  # the drift follows the counts
+                dd[i] = np.float32(oracle.inhomogeneous_dt(Dx[i] if axis == 0 else Dy[i],
+                                                           rx[i] if axis == 0 else ry[i], h))
+                nn[i] = np.float32(nn[i] + dd[i])
+            assert g_seed == seed
+        sim = mindt
+        np.testing.assert_array_equal(sol.synchronizeStateBufferFromDevice(), ost, err_msg="iteration %d" % it)
+        gx, gy = sol.timeSteps()
+        assert list(gx) == dtx and list(gy) == dty, it
+    assert changed
+    # the same steps by runPerCall on a fresh solver
+    sol2 = m.createSolver()
+    sol2.synchronizeStateBufferToDevice(st)
+    t = sol2.runPerCall(steps, seed0)
+    assert t == sim
+    np.testing.assert_array_equal(sol2.synchronizeStateBufferFromDevice(), ost)

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from inchman_amd import sbml
-from inchman_amd.gpgmp import STOCHASTIC_HOMOGENEOUS, STOCHASTIC_INHOMOGENEOUS
+from inchman_amd.gpgmp import DETERMINISTIC_INHOMOGENEOUS_RK4, STOCHASTIC_HOMOGENEOUS, STOCHASTIC_INHOMOGENEOUS
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 EXAMPLES = "/root/reference/doc/examples"
@@ -169,8 +169,9 @@ def test_reference_new_individuals_methods():
 
 def test_nonlinear_method_dependencies():
     """A non-linear model's method is re-evaluated after every sweep; the compiler reports what
-    it reads, so one that reads neither the counts, the clock nor the current fields (its
-    fields cannot change) builds, and the others are refused with the reason."""
+    it reads: one that reads neither the counts, the clock nor the current fields (its fields
+    cannot change) runs as it is, the others need the solver's re-evaluation, which the
+    stochastic inhomogeneous solver has (other types are refused with the reason)."""
     im = sbml.load(os.path.join(GOLDEN, "inchman_drift.xml"))
     base = im.drift_diffusivity_method
     assert sbml.method_dependencies(im) & 1  # the fixture's method reads the counts
@@ -186,8 +187,10 @@ def test_nonlinear_method_dependencies():
         assert sbml.method_dependencies(im) == want, text
     im.nonlinear_diffusivity = True
     im.drift_diffusivity_method = "All->DiffusivityX = 1.0; All->DiffusivityY = 1.0; All->DriftX = 0.1 * PhysicalX;"
-    sbml.check_nonlinear(im)  # accepted: nothing a sweep changes is read
+    assert sbml.check_nonlinear(im) is False  # nothing a sweep changes is read
     im.drift_diffusivity_method = base
+    assert sbml.check_nonlinear(im) is True  # re-evaluated after every sweep
+    im.model.solver = DETERMINISTIC_INHOMOGENEOUS_RK4
     with pytest.raises(NotImplementedError, match="the counts"):
         sbml.check_nonlinear(im)
 
@@ -195,8 +198,9 @@ def test_nonlinear_method_dependencies():
 @pytest.mark.skipif(not os.path.isdir(EXAMPLES), reason="reference examples not present")
 def test_reference_nonlinear_examples():
     """The reference's non-linear examples: those whose method reads only positions and
-    parameters build (their re-evaluated fields equal the first ones), Slit (counts and clock)
-    is refused, Nonlinear (MeanX) fails to compile."""
+    parameters run as they are (their re-evaluated fields equal the first ones), Slit (counts
+    and clock) has its method re-evaluated after every sweep, Nonlinear (MeanX) fails to
+    compile."""
     from inchman_amd import GpgmpError
 
     for n, ok in (("MultiplicativeNoise.xml", True), ("OrnsteinUhlenbeck.xml", True), ("Slit.xml", False)):
@@ -248,3 +252,19 @@ def _example_inputs(im):
     st, real, fields = sbml.initial_state(im)
     m = im.model
     return st, fields, np.zeros((4, len(m.species), m.height, m.width), np.float32)
+
+
+def test_loop_seeds_are_the_oracles():
+    """The host loop's seed_n (boost::mt19937(seed0), uniform_int<>(0, INT_MAX) plus the
+    iteration, DiffusionModel.cpp:1467-1482) from the library, for loops over the per-call API,
+    equal the oracle's (pinned by mt19937's 10000th output)."""
+    from inchman_amd.gpgmp import loopSeeds
+
+    for seed0 in (0, 1, 99, 2**32 - 1):
+        assert np.array_equal(loopSeeds(seed0, 2000), oracle_seeds(seed0, 2000))
+
+
+def oracle_seeds(seed0, n):
+    from oracle import oracle as o
+
+    return o.seeds(seed0, n)
