@@ -477,9 +477,11 @@ int gpgmp_solver_individuals(gpgmp_solver *s, int32_t species, gpgmp_individuals
 // does (InhomogeneousSolver.cpp:304-336 after X, :384-417 after Y): the method re-evaluated on
 // the swept state at the current time gives the species' four fields (the other species keep
 // theirs); the swept axis' time step becomes the minimum admissible step over the cells
-// (computeIndividualTimestepX/Y + reduceBlocks, as at set-up), and the sweeps of both axes
-// take their probabilities from the current fields and time steps, which the reference's
-// diffuseX/Y kernels compute at sweep time.
+// (computeIndividualTimestepX/Y + reduceBlocks, as at set-up).  The cellular-automaton sweeps
+// of both axes then take their probabilities from the current fields and time steps, which
+// the reference's diffuseX/Y kernels compute at sweep time; the Fokker-Planck sweeps use the
+// rates that only the swept axis' time-step kernel stores (.x/.y for X, .z/.w for Y), times
+// that axis' step, so only the swept axis' probabilities change.
 static void nonlinear_update(gpgmp_solver *s, int sp, int axis, float t) {
     const gpgmp_model_desc &d = s->m.desc;
     const int W = d.width, H = d.height, S = d.num_species;
@@ -501,11 +503,14 @@ static void nonlinear_update(gpgmp_solver *s, int sp, int axis, float t) {
     const float *Dx = s->m.Dx.data() + (size_t)sp * A, *Dy = s->m.Dy.data() + (size_t)sp * A;
     const float *rx = s->m.rx.data() + (size_t)sp * A, *ry = s->m.ry.data() + (size_t)sp * A;
     const float *D = axis ? Dy : Dx, *r = axis ? ry : rx;
+    const bool fpe = s->m.fpe();
     float m = INFINITY;
-    for (size_t c = 0; c < A; ++c) m = std::min(m, inhomogeneous_cell_dt(D[c], r[c], s->m.cell_width));
+    for (size_t c = 0; c < A; ++c)
+        m = std::min(m, fpe ? fpe_cell_dt(D[c], r[c], s->m.cell_width, s->m.probs.stay)
+                            : inhomogeneous_cell_dt(D[c], r[c], s->m.cell_width));
     (axis ? s->dty : s->dtx)[sp] = m;
-    s->eng->set_probs(0, sp, 0, Dx, rx, s->dtx[sp], s->m.cell_width);
-    s->eng->set_probs(0, sp, 1, Dy, ry, s->dty[sp], s->m.cell_height);
+    if (!fpe || axis == 0) s->eng->set_probs(0, sp, 0, Dx, rx, s->dtx[sp], s->m.cell_width);
+    if (!fpe || axis == 1) s->eng->set_probs(0, sp, 1, Dy, ry, s->dty[sp], s->m.cell_height);
 }
 
 int gpgmp_loop_seeds(uint32_t seed0, int32_t n, int32_t *out) {
@@ -522,8 +527,8 @@ int gpgmp_solver_set_nonlinear_method(gpgmp_solver *s, const char *method, const
         GPGMP_REQUIRE(s && method, GPGMP_E_INVALID, "null argument");
         GPGMP_REQUIRE(nfields >= 0 && (nfields == 0 || (field_names && fields)), GPGMP_E_INVALID,
                       "fields need their names and values");
-        GPGMP_REQUIRE(s->eng && s->m.desc.solver_type == GPGMP_STOCHASTIC_INHOMOGENEOUS, GPGMP_E_UNSUPPORTED,
-                      "non-linear diffusivity needs the stochastic inhomogeneous solver");
+        GPGMP_REQUIRE(s->eng && (s->m.desc.solver_type == GPGMP_STOCHASTIC_INHOMOGENEOUS || s->m.fpe()),
+                      GPGMP_E_UNSUPPORTED, "non-linear diffusivity needs a stochastic inhomogeneous solver");
         const gpgmp_model_desc &d = s->m.desc;
         const int S = d.num_species;
         const size_t A = (size_t)d.width * d.height;

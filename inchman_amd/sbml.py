@@ -497,7 +497,7 @@ def check_nonlinear(im):
     """A non-linear model (fields recomputed after every sweep, InhomogeneousSolver.cpp:304-336,
     384-417): True when its method reads what a sweep changes -- the counts, the clock, the
     current fields -- so that the solver must re-evaluate it (the stochastic inhomogeneous
-    solver does, gpgmp_solver_set_nonlinear_method); False when re-evaluating it reproduces
+    solvers do, gpgmp_solver_set_nonlinear_method); False when re-evaluating it reproduces
     the first fields and time steps.  Other solver types with such a method raise
     NotImplementedError with what it reads."""
     if not im.nonlinear_diffusivity:
@@ -505,11 +505,11 @@ def check_nonlinear(im):
     deps = method_dependencies(im)
     if not deps:
         return False
-    if im.model.solver == STOCHASTIC_INHOMOGENEOUS:
+    if im.model.solver in (STOCHASTIC_INHOMOGENEOUS, STOCHASTIC_INHOMOGENEOUS_FPE):
         return True
     what = [n for b, n in ((1, "the counts"), (2, "PhysicalSimTime"), (4, "the current fields")) if deps & b]
     raise NotImplementedError("non-linear diffusivity whose method reads %s (fields recomputed after every "
-                              "sweep) is supported by the stochastic inhomogeneous solver only" % " and ".join(what))
+                              "sweep) is supported by the stochastic inhomogeneous solvers only" % " and ".join(what))
 
 
 class _Affine:

@@ -230,9 +230,10 @@ int gpgmp_loop_seeds(uint32_t seed0, int32_t n, int32_t *out);
  * the drift/diffusivity method (the language of gpgmp_model_eval_drift_diffusivity, with its
  * field parameters, nfields x [H][W] floats) is re-evaluated after every sweep that
  * gpgmp_solver_diffuse runs, on the swept counts at the call's time t: the species' four fields,
- * then the swept axis' time step and both axes' sweep probabilities.  Stochastic
- * inhomogeneous solvers only; the model then runs through the per-call API
- * (gpgmp_solver_run refuses it). */
+ * then the swept axis' time step and the sweep probabilities (both axes', the Fokker-Planck
+ * type's swept axis' only, as its rates are stored per axis).  Stochastic inhomogeneous
+ * solvers (both types); the model then runs through the per-call API (gpgmp_solver_run
+ * refuses it). */
 int gpgmp_solver_set_nonlinear_method(gpgmp_solver *s, const char *method, const char *const *field_names,
                                       int32_t nfields, const float *fields);
 /* Makes `species` an individual (particle-based) species of this stochastic inhomogeneous

@@ -81,17 +81,23 @@ def _inc(seed):
     return (seed + 1 + 2 ** 31) % 2 ** 32 - 2 ** 31
 
 
-def test_nonlinear_recompute_matches_oracle_loop(gpu, oracle):
+@pytest.mark.parametrize("fpe", [False, True])
+def test_nonlinear_recompute_matches_oracle_loop(gpu, oracle, fpe):
     """Non-linear diffusivity whose method reads the counts (the drift fixture's: a central
     difference of the counts over a crowding term): DiffusionModel::oclSolve's loop through the
     solver calls, with the method re-evaluated after every sweep on the swept counts at the
     current time (InhomogeneousSolver.cpp:304-336,384-417), against the same loop over the
     oracle with the g++-compiled method: counts, clock, next diffusion times and time steps
-    bit-exact after every iteration; then runPerCall over the same steps."""
+    bit-exact after every iteration; then runPerCall over the same steps.  The Fokker-Planck
+    type's sweeps use the rates its swept axis' time-step kernel last stored (computed from the
+    fields of that moment) and its own time-step formula."""
+    from inchman_amd.gpgmp import STOCHASTIC_INHOMOGENEOUS_FPE
     from oracle import lawgen
 
     im = sbml.load(os.path.join(GOLDEN, "inchman_drift.xml"))
     im.nonlinear_diffusivity = True
+    if fpe:
+        im.model.solver = STOCHASTIC_INHOMOGENEOUS_FPE
     st, real, fields = sbml.build(im)
     m = im.model
     S, H, W = m.numSpecies(), m.height, m.width
@@ -102,8 +108,16 @@ def test_nonlinear_recompute_matches_oracle_loop(gpu, oracle):
     # oracle: the build's fields, the time steps as set up, the reference's loop
     Dx, Dy, rx, ry = [np.array(a, np.float32).reshape(S, H, W) for a in m.fields]
     h = oracle.header_float(m.physical_length / W)
-    dtx = [np.float32(oracle.inhomogeneous_dt(Dx[i], rx[i], h)) for i in range(S)]
-    dty = [np.float32(oracle.inhomogeneous_dt(Dy[i], ry[i], h)) for i in range(S)]
+    stay = oracle.stay_probability(m.p0)
+
+    def step(D, r):
+        if fpe:
+            return np.float32(min(oracle.fpe_cell_dt(d, v, h, stay) for d, v in zip(D.ravel(), r.ravel())))
+        return np.float32(oracle.inhomogeneous_dt(D, r, h))
+
+    dtx = [step(Dx[i], rx[i]) for i in range(S)]
+    dty = [step(Dy[i], ry[i]) for i in range(S)]
+    rates = [[Dx.copy(), rx.copy()], [Dy.copy(), ry.copy()]]  # the fields the FPE rates were stored from
     gx, gy = sol.timeSteps()
     assert list(gx) == dtx and list(gy) == dty
     nx, ny = list(dtx), list(dty)
@@ -131,15 +145,20 @@ def test_nonlinear_recompute_matches_oracle_loop(gpu, oracle):
             for axis, nn, dd in ((0, nx, dtx), (1, ny, dty)):
                 if not (nn[i] <= mindt):
                     continue
-                ost = oracle.inhom_sweep(ost, i, axis, seed, dd[i], Dx if axis == 0 else Dy, rx if axis == 0 else ry, h)
+                if fpe:
+                    ost = oracle.fpe_sweep(ost, i, axis, seed, dd[i], rates[axis][0], rates[axis][1], h)
+                else:
+                    ost = oracle.inhom_sweep(ost, i, axis, seed, dd[i], Dx if axis == 0 else Dy,
+                                             rx if axis == 0 else ry, h)
                 seed = _inc(seed)
                 new = lawgen.method_fields(m, im.drift_diffusivity_method, names, fp, ost,
                                            np.stack([Dx, Dy, rx, ry]), sim_time=mindt)
                 before = rx[i].copy()
                 Dx[i], Dy[i], rx[i], ry[i] = new[0][i], new[1][i], new[2][i], new[3][i]
                 changed |= not np.array_equal(before, rx[i])  # the drift follows the counts
-                dd[i] = np.float32(oracle.inhomogeneous_dt(Dx[i] if axis == 0 else Dy[i],
-                                                           rx[i] if axis == 0 else ry[i], h))
+                dd[i] = step(Dx[i] if axis == 0 else Dy[i], rx[i] if axis == 0 else ry[i])
+                rates[axis][0][i] = (Dx if axis == 0 else Dy)[i]
+                rates[axis][1][i] = (rx if axis == 0 else ry)[i]
                 nn[i] = np.float32(nn[i] + dd[i])
             assert g_seed == seed
         sim = mindt
