@@ -348,7 +348,8 @@ class Solver:
         """DiffusionModel::oclSolve's loop (DiffusionModel.cpp:1479-1603) through doReactions,
         diffuse and getNextDiffusionTime, one call each per step, for what the device loop does
         not run (non-linear diffusivity, individual species): from the clock sim_time, seeds
-        seed_n of seed0 (loopSeeds).  Returns the clock."""
+        seed_n of seed0 (loopSeeds), diffusion events only (no output or alert events: read the
+        state between calls).  Returns the clock."""
         seeds = loopSeeds(seed0, steps)
         sim = np.float32(sim_time)
         react = len(self.model.reactions) > 0
