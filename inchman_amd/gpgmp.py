@@ -344,27 +344,46 @@ class Solver:
         cnames = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
         call("gpgmp_solver_set_nonlinear_method", self.handle, method.encode(), cnames, len(names), _fptr(fv))
 
-    def runPerCall(self, steps, seed0, sim_time=0.0):
+    def runPerCall(self, steps, seed0, sim_time=0.0, output_interval=float("inf"), max_time=float("inf"),
+                   on_output=None):
         """DiffusionModel::oclSolve's loop (DiffusionModel.cpp:1479-1603) through doReactions,
         diffuse and getNextDiffusionTime, one call each per step, for what the device loop does
         not run (non-linear diffusivity, individual species): from the clock sim_time, seeds
-        seed_n of seed0 (loopSeeds), diffusion events only (no output or alert events: read the
-        state between calls).  Returns the clock."""
+        seed_n of seed0 (loopSeeds).  Output events every output_interval (:1519-1527,
+        1553-1572): the initial state is written first (Dump_0), an output time before the next
+        diffusion makes the iteration reactions only, and once the clock has reached it the
+        iteration after writes -- each write calls on_output(time, counts); the loop stops once
+        the clock reaches max_time (:1479-1480).  Returns (clock, iterations run)."""
         seeds = loopSeeds(seed0, steps)
         sim = np.float32(sim_time)
+        interval, out_last = np.float32(output_interval), np.float32(0)
         react = len(self.model.reactions) > 0
-        for it in range(steps):
+        if on_output is not None and (np.isfinite(interval) or np.isfinite(max_time)):
+            on_output(float(sim), self.synchronizeStateBufferFromDevice())
+        it = 0
+        while it < steps and sim < np.float32(max_time):
             nxt = self.getNextDiffusionTime()
             minspec = int(np.argmin(nxt))  # the first minimum
             mindt = nxt[minspec]
+            out_time = np.float32(out_last + interval)
+            output_event = out_time < mindt
+            if output_event:
+                mindt = out_time
             seed = int(seeds[it])
             if react:
                 seed = self.doReactions(np.float32(mindt - sim), seed)
-            for i in range(len(nxt)):
-                if (i == minspec or nxt[i] <= mindt) and nxt[i] > 0:
-                    seed = self.diffuse(i, mindt, seed)
+            if output_event:
+                if sim >= np.float32(out_last + interval):
+                    if on_output is not None:
+                        on_output(float(mindt), self.synchronizeStateBufferFromDevice())
+                    out_last = np.float32(out_last + interval)
+            else:
+                for i in range(len(nxt)):
+                    if (i == minspec or nxt[i] <= mindt) and nxt[i] > 0:
+                        seed = self.diffuse(i, mindt, seed)
             sim = mindt
-        return float(sim)
+            it += 1
+        return float(sim), it
 
     # Solver::getNextDiffusionTime()
     def getNextDiffusionTime(self):

@@ -169,6 +169,35 @@ def test_nonlinear_recompute_matches_oracle_loop(gpu, oracle, fpe):
     # the same steps by runPerCall on a fresh solver
     sol2 = m.createSolver()
     sol2.synchronizeStateBufferToDevice(st)
-    t = sol2.runPerCall(steps, seed0)
-    assert t == sim
+    t, n = sol2.runPerCall(steps, seed0)
+    assert t == sim and n == steps
     np.testing.assert_array_equal(sol2.synchronizeStateBufferFromDevice(), ost)
+
+
+@pytest.mark.parametrize("case", ["inhomogeneous", "max_time"])
+def test_run_per_call_output_events_match_oracle(gpu, oracle, case):
+    """runPerCall's output events (a reaction step up to each output time, then a zero-length
+    iteration that writes) and its max_time stop, on models the device loop also runs, against
+    the oracle's loop with output events: final counts, clock, iterations, every dump and its
+    time."""
+    from inchman_amd import configs
+
+    if case == "inhomogeneous":
+        m, st = configs.schnakenberg_model(64), configs.schnakenberg_state(64, seed=3)
+        steps, interval, max_time = 120, 0.5, float("inf")
+    else:
+        m, st = configs.plain_gmp_model(32), configs.plain_gmp_state(32)
+        steps, interval, max_time = 2000, 0.3, 3.0
+    s = m.createSolver()
+    s.synchronizeStateBufferToDevice(st)
+    dumps, times = [], []
+    t, n = s.runPerCall(steps, 21, output_interval=interval, max_time=max_time,
+                        on_output=lambda tm, c: (times.append(np.float32(tm)), dumps.append(c)))
+    want, wt, wit, wdumps, wtimes, writes = oracle.run_output(m, st, steps, 21, interval, max_time, max_dumps=64)
+    assert np.float32(t) == wt and n == wit and len(dumps) == writes
+    np.testing.assert_array_equal(s.synchronizeStateBufferFromDevice(), want)
+    k = min(len(dumps), 64)
+    assert np.array_equal(np.array(times[:k], np.float32), wtimes)
+    assert np.array_equal(np.stack(dumps[:k]), wdumps)
+    if case == "max_time":
+        assert n < steps  # the clock reached max_time first
